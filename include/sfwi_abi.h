@@ -1,0 +1,175 @@
+/*
+ * sfwi_abi.h -- C ABI of the MI355X-native SFWI per-shot elastic modeling and
+ * gradient library (libsfwi_amd.so, HIP for gfx950).
+ *
+ * It replaces the device-side C++ interface that the reference's host driver
+ * (sfwi-stage{1,2}/main.cpp) calls: class FD2DGPU_ELASTIC and the per-shot
+ * free functions declared in sfwi-stage1/FD2DGPU.cuh and
+ * sfwi-stage2/FD2DGPU.cuh.  Every entry point below names the reference
+ * interface it stands in for.
+ *
+ * Conventions
+ *   - Plain C types only; all host buffers are caller-owned.
+ *   - Models and gradients use the reference's .bin layout: allnx*allnz
+ *     floats, x-major, z fastest (index ix*allnz + iz).
+ *   - Records are trace-major [ntr][nt] floats (index tr*nt + it).
+ *   - Return value 0 = success, negative = error; sfwi_last_error() gives the
+ *     message.  A context is used by one host thread at a time.
+ *   - One GPU per context (the current HIP device at creation).
+ */
+#ifndef SFWI_ABI_H
+#define SFWI_ABI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SFWI_OK 0
+#define SFWI_EINVAL (-1)   /* bad argument                                   */
+#define SFWI_EHIP (-2)     /* HIP runtime error                              */
+#define SFWI_ENOMEM (-3)   /* device allocation failed or over HBM budget    */
+#define SFWI_ESTATE (-4)   /* call order (model/wavelet not set, ...)        */
+
+/* Numerical modes. */
+#define SFWI_MODE_FAST 0       /* FP32, unsplit interior, FMA stencils          */
+#define SFWI_MODE_EXACT 1      /* reference operation order everywhere (split  */
+                               /* PML fields on every cell, FP64 promotions)   */
+
+/* Forward-wavefield storage used by the reverse-time reconstruction. */
+#define SFWI_STORE_STRIPS_F32 0   /* 8-cell boundary strips of vx, vz, FP32    */
+#define SFWI_STORE_STRIPS_F16 1   /* same strips, FP16 with per-step scale     */
+#define SFWI_STORE_STRIPS_BF16 2  /* same strips, BF16                         */
+
+typedef struct sfwi_ctx sfwi_ctx;
+
+/* Creation parameters: the FD2DGPU_ELASTIC constructor arguments
+ * (sfwi-stage1/FD2DGPU.cuh:152-153: sou, dx, dz, dt, nxpml, nzpml, allnx,
+ * allnz, scale, pml, nt, nop, ntr) plus the constants the reference hard-codes. */
+typedef struct sfwi_params {
+    float dx, dz, dt;      /* grid spacing (m) and time step (s)                   */
+    int nx, nz;            /* shot window; nxpml = nx + 2 pml, nzpml = nz + 2 pml  */
+    int pml;               /* PML width (cells)                                     */
+    int nt;                /* time samples                                          */
+    int nop;               /* stencil half width, must be 4                         */
+    int ntr;               /* maximum traces per shot                               */
+    int allnx, allnz;      /* full model size                                       */
+    float rr;              /* PML reflection coefficient (reference: 0.001)         */
+    float rho_const;       /* > 0: padded density forced to this value (the stage-2
+                              medium kernel forces 2000); 0: use the model density */
+    int adjoint_dir;       /* adjoint propagation direction: +1 stage 1, -1 stage 2 */
+    int max_batch;         /* shots resident on the device at once (>= 1)           */
+    int mode;              /* SFWI_MODE_*                                           */
+    int storage;           /* SFWI_STORE_*                                          */
+    int use_graph;         /* 1: replay the time loop from a captured hipGraph      */
+} sfwi_params;
+
+/* One shot: the fields the reference driver writes into FD2DGPU_ELASTIC
+ * before each IMAGE_SINGLESHOT call (sfwi-stage1/main.cpp:596-625). */
+typedef struct sfwi_shot {
+    int left;              /* first model column of the window                      */
+                           /*   (delta_left + record_left_in_v)                     */
+    int isx, isz;          /* source column / row on the padded grid                */
+    int igz;               /* receiver row on the padded grid                       */
+    int ngx_left;          /* coordinate of the window's left edge                  */
+    int ntr;               /* traces of this shot (<= sfwi_params.ntr)              */
+    const int *gc;         /* receiver coordinates [ntr]                             */
+    const float *obs_z;    /* observed vz traces [ntr][nt] (gradient calls)         */
+    const float *obs_x;    /* observed vx traces [ntr][nt]                          */
+} sfwi_shot;
+
+/* Replaces the FD2DGPU_ELASTIC constructor (sfwi-stage1/FD2DGPU.cu:1366-1655)
+ * minus the source wavelet, which sfwi_set_wavelet uploads. */
+int sfwi_ctx_create(const sfwi_params *p, sfwi_ctx **out);
+/* Replaces ~FD2DGPU_ELASTIC. */
+void sfwi_ctx_destroy(sfwi_ctx *ctx);
+/* Text of the last error on this context (or of a failed create when ctx is NULL). */
+const char *sfwi_last_error(const sfwi_ctx *ctx);
+
+/* Source wavelet (nt floats): the `sou` constructor argument, uploaded in
+ * sfwi-stage1/FD2DGPU.cu:46-47. */
+int sfwi_set_wavelet(sfwi_ctx *ctx, const float *wavelet);
+/* Full models (allnx*allnz each): FD2DGPU_ELASTIC::GPUbufferVPVS
+ * (sfwi-stage1/FD2DGPU.cu:1322-1327).  rho may be NULL when rho_const > 0. */
+int sfwi_set_model(sfwi_ctx *ctx, const float *vp, const float *vs, const float *rho);
+/* Model perturbation for Born modeling: FD2DGPU_ELASTIC::GPUbufferM
+ * (sfwi-stage1/FD2DGPU.cu:1330-1334). */
+int sfwi_set_perturbation(sfwi_ctx *ctx, const float *dvp, const float *dvs);
+
+/* Stage-1 per-shot gradient: IMAGE_SINGLESHOT (sfwi-stage1/FD2DGPU.cu:4321-4695,
+ * declared sfwi-stage1/FD2DGPU.cuh:466-467).  Adds the L2 misfit to *misfit and
+ * accumulates the Vp/Vs gradients and the illumination on the device until
+ * sfwi_flush. */
+int sfwi_shot_gradient(sfwi_ctx *ctx, const sfwi_shot *shot, double *misfit);
+/* Stage-2 per-shot gradient of the Born misfit: IMAGE_SINGLESHOT of stage 2
+ * (sfwi-stage2/FD2DGPU.cu:3978-4570, declared sfwi-stage2/FD2DGPU.cuh:455-456),
+ * residual = born - obs + background, perturbation from sfwi_set_perturbation. */
+int sfwi_shot_born_gradient(sfwi_ctx *ctx, const sfwi_shot *shot, double *misfit);
+/* Many shots at once (the reference farms them over MPI ranks,
+ * sfwi-stage1/main.cpp:417-634); misfits[i] receives shot i's misfit (+=). */
+int sfwi_batch_gradient(sfwi_ctx *ctx, int nshots, const sfwi_shot *shots, double *misfits);
+int sfwi_batch_born_gradient(sfwi_ctx *ctx, int nshots, const sfwi_shot *shots, double *misfits);
+
+/* Forward solve (born_modeling mode 0 + record_background_wave, the inner loop
+ * of CALCULATE_ALPHA_parabola_mb, sfwi-stage1/FD2DGPU.cu:4864-4870):
+ * synthetic vz / vx traces [ntr][nt]. */
+int sfwi_shot_forward(sfwi_ctx *ctx, const sfwi_shot *shot, float *syn_z, float *syn_x);
+/* Born forward solve (born_modeling mode 1 + record_scatter, the inner loop of
+ * CALCULATE_ALPHA, sfwi-stage2/FD2DGPU.cu:4607-4620): scattered traces b_*
+ * and background traces bg_* (either pair may be NULL). */
+int sfwi_shot_born_forward(sfwi_ctx *ctx, const sfwi_shot *shot, float *b_z, float *b_x,
+                           float *bg_z, float *bg_x);
+
+/* Stage-1 line search, CALCULATE_ALPHA_parabola_mb (sfwi-stage1/FD2DGPU.cu:4819-5028,
+ * declared sfwi-stage1/FD2DGPU.cuh:471-472): adds to m[0..2] the misfits of
+ * forward solves at vel, vel - step1*dir and vel - step2*dir. */
+int sfwi_linesearch_parabola(sfwi_ctx *ctx, const sfwi_shot *shot, const float *dir_vp,
+                             const float *dir_vs, float step1, float step2, double m[3]);
+/* Stage-2 exact step, CALCULATE_ALPHA (sfwi-stage2/FD2DGPU.cu:4576-4722,
+ * declared sfwi-stage2/FD2DGPU.cuh:457-458): fenzi += sum L(cg).r,
+ * fenmu += sum L(cg)^2 with r from the current perturbation. */
+int sfwi_linesearch_linear(sfwi_ctx *ctx, const sfwi_shot *shot, const float *cg_vp,
+                           const float *cg_vs, double *fenzi, double *fenmu);
+
+/* Gradient flush, FD2DGPU_ELASTIC::imagebuffer_resettozero
+ * (sfwi-stage1/FD2DGPU.cu:3646-3673): copies the accumulated Vp and Vs
+ * gradients and the illumination (allnx*allnz each) to the host and zeroes
+ * them on the device.  Any pointer may be NULL. */
+int sfwi_flush(sfwi_ctx *ctx, float *grad_vp, float *grad_vs, float *illum);
+
+/* Optimal-transport trace misfit and adjoint source.  Declared but never
+ * defined in the reference (OT, OT_power, OT_alltrace,
+ * sfwi-stage1/FD2DGPU.cuh:21-25); arguments follow those declarations:
+ * observed and synthetic traces [ntr][nt] per component, adjoint source
+ * written to res_*, W2^2 misfit added to *misfit.
+ * variant 0: per trace, linear shift to positivity; 1: per trace, squared
+ * ("power"); 2: all traces of a component as one distribution. */
+int sfwi_ot_misfit(sfwi_ctx *ctx, int variant, int ntr, int nt, float dt, const float *obs_x,
+                   const float *obs_z, const float *syn_x, const float *syn_z, float *res_x,
+                   float *res_z, double *misfit);
+
+/* Reference host-driver helper: Ricker wavelet (sfwi-stage1/main.cpp:251-260). */
+void sfwi_ricker(float *w, int nt, float dt, float f0);
+
+/* Measurement hooks (no reference counterpart).  With sample_every = k > 0
+ * every k-th time step of a solve brackets each stencil kernel launch with
+ * HIP events on the context's stream; sfwi_kernel_time reports the mean
+ * device time per launch of one kernel since the last sfwi_set_timing call.
+ * Kernel ids: SFWI_K_* below. */
+#define SFWI_K_FWD_VEL 0      /* forward velocity half step (whole batch)      */
+#define SFWI_K_FWD_STR 1      /* forward stress half step                      */
+#define SFWI_K_REC_STR 2      /* reconstruction stress step + derivatives      */
+#define SFWI_K_REC_VEL 3      /* reconstruction velocity step                  */
+#define SFWI_K_ADJ_STR 4      /* adjoint stress-like step                      */
+#define SFWI_K_ADJ_VEL 5      /* adjoint velocity step fused with imaging      */
+#define SFWI_K_COUNT 6
+int sfwi_set_timing(sfwi_ctx *ctx, int sample_every);
+int sfwi_kernel_time(sfwi_ctx *ctx, int kernel_id, double *avg_ms, int *launches);
+const char *sfwi_kernel_name(int kernel_id);
+
+/* Library version string. */
+const char *sfwi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SFWI_ABI_H */

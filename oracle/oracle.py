@@ -1,0 +1,198 @@
+"""ctypes binding of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the sfwi_amd package.  The C++ source
+(sfwi_oracle.cpp) restates the reference's per-shot algorithm; see the header
+sfwi_oracle.h for the meaning of every entry point.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+
+
+class OrConfig(ctypes.Structure):
+    """Mirror of `or_config` in sfwi_oracle.h."""
+
+    _fields_ = [
+        ("allnx", ctypes.c_int), ("allnz", ctypes.c_int),
+        ("nx", ctypes.c_int), ("nz", ctypes.c_int),
+        ("pml", ctypes.c_int), ("nt", ctypes.c_int), ("ntr", ctypes.c_int),
+        ("dx", ctypes.c_float), ("dz", ctypes.c_float), ("dt", ctypes.c_float),
+        ("rr", ctypes.c_float),
+        ("left", ctypes.c_int),
+        ("isx", ctypes.c_int), ("isz", ctypes.c_int), ("igz", ctypes.c_int),
+        ("ngx_left", ctypes.c_int),
+        ("adjoint_dir", ctypes.c_int),
+        ("rho_const", ctypes.c_float),
+        ("pml_off", ctypes.c_int),
+        ("nthreads", ctypes.c_int),
+    ]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run `make -C oracle` (or __graft_entry__.build())")
+        L = ctypes.CDLL(path)
+        cfg = ctypes.POINTER(OrConfig)
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.or_coeffs_f32.argtypes = [_f32p]
+        L.or_coeffs_f64.argtypes = [_f64p]
+        L.or_ricker.argtypes = [_f32p, ctypes.c_int, ctypes.c_float, ctypes.c_float]
+        L.or_setup_f32.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p]
+        L.or_setup_f64.argtypes = [cfg, _f32p, _f32p, _f32p, _f64p]
+        L.or_forward_f32.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _i32p, _f32p, _f32p]
+        L.or_forward_f64.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _i32p, _f64p, _f64p]
+        L.or_gradient_f32.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _i32p, _f32p, _f32p,
+                                      _f32p, _f32p, _f32p, dp, _f32p, _f32p]
+        L.or_gradient_f64.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _i32p, _f32p, _f32p,
+                                      _f64p, _f64p, _f64p, dp, _f64p, _f64p]
+        L.or_born_forward_f32.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _i32p,
+                                          _f32p, _f32p, _f32p, _f32p]
+        L.or_born_forward_f64.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _i32p,
+                                          _f64p, _f64p, _f64p, _f64p]
+        L.or_born_gradient_f32.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _i32p,
+                                           _f32p, _f32p, _f32p, _f32p, _f32p, dp]
+        L.or_born_gradient_f64.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _i32p,
+                                           _f32p, _f32p, _f64p, _f64p, _f64p, dp]
+        L.or_parabola_misfits_f32.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _f32p,
+                                              ctypes.c_float, ctypes.c_float, _f32p, _i32p,
+                                              _f32p, _f32p, _f64p]
+        L.or_linear_alpha_f32.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p,
+                                          _f32p, _i32p, _f32p, _f32p, dp, dp]
+        L.or_stencils_f64.argtypes = [ctypes.c_int, ctypes.c_int, _f64p, _f64p, _f64p, _f64p, _f64p]
+        L.or_propagate_f64.argtypes = [cfg, _f32p, _f32p, _f32p, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int, _f64p]
+        _LIB = L
+    return _LIB
+
+
+def config(**kw):
+    """Build an OrConfig; unspecified fields take the reference defaults."""
+    c = OrConfig()
+    c.rr = 0.001
+    c.adjoint_dir = 1
+    c.rho_const = 0.0
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def ricker(nt, dt, f0):
+    w = np.zeros(nt, np.float32)
+    lib().or_ricker(w, nt, dt, f0)
+    return w
+
+
+def coeffs(prec="f64"):
+    if prec == "f32":
+        c = np.zeros(5, np.float32)
+        lib().or_coeffs_f32(c)
+    else:
+        c = np.zeros(5, np.float64)
+        lib().or_coeffs_f64(c)
+    return c
+
+
+def setup(c, vp, vs, rho, prec="f32"):
+    """Padded (vp, vs, rho, lam, mu, dampx, dampz), each (nxpml, nzpml)."""
+    nxp, nzp = c.nx + 2 * c.pml, c.nz + 2 * c.pml
+    dt = np.float32 if prec == "f32" else np.float64
+    out = np.zeros(7 * nxp * nzp, dt)
+    getattr(lib(), "or_setup_" + prec)(ctypes.byref(c), _f32(vp), _f32(vs), _f32(rho), out)
+    return out.reshape(7, nxp, nzp)
+
+
+def forward(c, vp, vs, rho, wav, gc, prec="f32"):
+    dt = np.float32 if prec == "f32" else np.float64
+    sz = np.zeros((c.ntr, c.nt), dt)
+    sx = np.zeros((c.ntr, c.nt), dt)
+    getattr(lib(), "or_forward_" + prec)(ctypes.byref(c), _f32(vp), _f32(vs), _f32(rho), _f32(wav),
+                                         np.ascontiguousarray(gc, np.int32), sz, sx)
+    return sz, sx
+
+
+def gradient(c, vp, vs, rho, wav, gc, obs_z, obs_x, prec="f32"):
+    """Stage-1 per-shot gradient: returns (grad_vp, grad_vs, illum, misfit, res_z, res_x)."""
+    dt = np.float32 if prec == "f32" else np.float64
+    n = c.allnx * c.allnz
+    g1, g2, g3 = np.zeros(n, dt), np.zeros(n, dt), np.zeros(n, dt)
+    rz = np.zeros((c.ntr, c.nt), dt)
+    rx = np.zeros((c.ntr, c.nt), dt)
+    m = ctypes.c_double(0.0)
+    getattr(lib(), "or_gradient_" + prec)(ctypes.byref(c), _f32(vp), _f32(vs), _f32(rho), _f32(wav),
+                                          np.ascontiguousarray(gc, np.int32), _f32(obs_z), _f32(obs_x),
+                                          g1, g2, g3, ctypes.byref(m), rz, rx)
+    sh = (c.allnx, c.allnz)
+    return g1.reshape(sh), g2.reshape(sh), g3.reshape(sh), m.value, rz, rx
+
+
+def born_forward(c, vp, vs, rho, dvp, dvs, wav, gc, prec="f32"):
+    dt = np.float32 if prec == "f32" else np.float64
+    out = [np.zeros((c.ntr, c.nt), dt) for _ in range(4)]
+    getattr(lib(), "or_born_forward_" + prec)(ctypes.byref(c), _f32(vp), _f32(vs), _f32(rho),
+                                              _f32(dvp), _f32(dvs), _f32(wav),
+                                              np.ascontiguousarray(gc, np.int32), *out)
+    return tuple(out)
+
+
+def born_gradient(c, vp, vs, rho, dvp, dvs, wav, gc, obs_z, obs_x, prec="f32"):
+    dt = np.float32 if prec == "f32" else np.float64
+    n = c.allnx * c.allnz
+    g1, g2, g3 = np.zeros(n, dt), np.zeros(n, dt), np.zeros(n, dt)
+    m = ctypes.c_double(0.0)
+    getattr(lib(), "or_born_gradient_" + prec)(ctypes.byref(c), _f32(vp), _f32(vs), _f32(rho),
+                                               _f32(dvp), _f32(dvs), _f32(wav),
+                                               np.ascontiguousarray(gc, np.int32),
+                                               _f32(obs_z), _f32(obs_x), g1, g2, g3, ctypes.byref(m))
+    sh = (c.allnx, c.allnz)
+    return g1.reshape(sh), g2.reshape(sh), g3.reshape(sh), m.value
+
+
+def parabola_misfits(c, vp, vs, rho, dir_vp, dir_vs, step1, step2, wav, gc, obs_z, obs_x):
+    m = np.zeros(3, np.float64)
+    lib().or_parabola_misfits_f32(ctypes.byref(c), _f32(vp), _f32(vs), _f32(rho), _f32(dir_vp),
+                                  _f32(dir_vs), step1, step2, _f32(wav),
+                                  np.ascontiguousarray(gc, np.int32), _f32(obs_z), _f32(obs_x), m)
+    return m
+
+
+def linear_alpha(c, vp, vs, rho, dvp, dvs, cg_vp, cg_vs, wav, gc, obs_z, obs_x):
+    fz, fm = ctypes.c_double(0.0), ctypes.c_double(0.0)
+    lib().or_linear_alpha_f32(ctypes.byref(c), _f32(vp), _f32(vs), _f32(rho), _f32(dvp), _f32(dvs),
+                              _f32(cg_vp), _f32(cg_vs), _f32(wav), np.ascontiguousarray(gc, np.int32),
+                              _f32(obs_z), _f32(obs_x), ctypes.byref(fz), ctypes.byref(fm))
+    return fz.value, fm.value
+
+
+def stencils(f):
+    f = np.ascontiguousarray(f, np.float64)
+    nxp, nzp = f.shape
+    outs = [np.zeros_like(f) for _ in range(4)]
+    lib().or_stencils_f64(nxp, nzp, f, *outs)
+    return outs  # dxp, dxm, dzp, dzm
+
+
+STATE_NAMES = ("vx", "vz", "txx", "tzz", "txz", "vx_x", "vx_z", "vz_x", "vz_z",
+               "txx_x", "txx_z", "tzz_x", "tzz_z", "txz_x", "txz_z")
+
+
+def propagate(c, vp, vs, rho, state, kind, direction, nsteps):
+    """Free propagation of a 15-array state (see STATE_NAMES), FP64."""
+    s = np.ascontiguousarray(state, np.float64).copy()
+    lib().or_propagate_f64(ctypes.byref(c), _f32(vp), _f32(vs), _f32(rho), kind, direction, nsteps, s)
+    return s

@@ -1,0 +1,813 @@
+// CPU oracle for SFWI's per-shot elastic modeling + gradient path.
+//
+// TEST INFRASTRUCTURE ONLY: this file is the checker for the HIP library in
+// sfwi_amd/.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+// leg load it; the product never does.
+//
+// It restates the reference's live per-shot algorithm, cell by cell, in one
+// template instantiated twice:
+//   T = float  : the reference's FP32 operation order.  Where the reference's
+//                expression contains a double literal (1.0/rho, 5.0*vp, ...),
+//                the same promotion to FP64 happens here, because the code is
+//                written with the same operand types and built with
+//                -ffp-contract=off.
+//   T = double : every array and every operation in FP64.
+//
+// Citations are file:line in the reference repository (285639863/SFWI).
+// The layout is the reference's: padded arrays nxpml*nzpml, index
+// ix*nzpml+iz (z fastest); model arrays allnx*allnz, index ix*allnz+iz.
+
+#include "sfwi_oracle.h"
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr int NOP = 4;  // stencil half width and strip half width: sfwi-stage1/main.cpp:1495
+
+// 8th-order staggered coefficients: sfwi-stage1/FD2DGPU.cuh:16 (double
+// fractions stored into a float constant array).
+template <typename T> struct FD;
+template <> struct FD<float> {
+    static constexpr float c[5] = {0.0f, (float)(1225.0 / 1024.0), (float)(-245.0 / 3072.0),
+                                   (float)(49.0 / 5120.0), (float)(-5.0 / 7168.0)};
+};
+template <> struct FD<double> {
+    static constexpr double c[5] = {0.0, 1225.0 / 1024.0, -245.0 / 3072.0, 49.0 / 5120.0,
+                                    -5.0 / 7168.0};
+};
+
+// Difference centred half a cell ahead: taps i-3..i+4 (stride s).
+//   sum_k c_k (f[i+k] - f[i-k+1])        e.g. sfwi-stage1/born.cu:376 (txx along x)
+template <typename T> inline T d_plus(const T *f, long i, long s) {
+    T t = 0;
+    for (int k = 1; k <= NOP; ++k) t += FD<T>::c[k] * (f[i + k * s] - f[i - (k - 1) * s]);
+    return t;
+}
+// Difference centred half a cell behind: taps i-4..i+3.
+//   sum_k c_k (f[i+k-1] - f[i-k])        e.g. sfwi-stage1/born.cu:377 (txz along z)
+template <typename T> inline T d_minus(const T *f, long i, long s) {
+    T t = 0;
+    for (int k = 1; k <= NOP; ++k) t += FD<T>::c[k] * (f[i + (k - 1) * s] - f[i - k * s]);
+    return t;
+}
+// The same stencils applied to m*f evaluated at every tap (the adjoint
+// velocity update keeps the moduli inside the derivative,
+// sfwi-stage1/born.cu:541-548).
+template <typename T> inline T d_plus_w(const T *m, const T *f, long i, long s) {
+    T t = 0;
+    for (int k = 1; k <= NOP; ++k)
+        t += FD<T>::c[k] * ((m[i + k * s]) * f[i + k * s] - (m[i - (k - 1) * s]) * f[i - (k - 1) * s]);
+    return t;
+}
+template <typename T> inline T d_minus_w(const T *m, const T *f, long i, long s) {
+    T t = 0;
+    for (int k = 1; k <= NOP; ++k)
+        t += FD<T>::c[k] * (m[i + (k - 1) * s] * f[i + (k - 1) * s] - m[i - k * s] * f[i - k * s]);
+    return t;
+}
+// (lambda + 2 mu) formed per tap inside the sum, as sfwi-stage1/born.cu:543,546.
+template <typename T> inline T d_plus_p(const T *lam, const T *mu, const T *f, long i, long s) {
+    T t = 0;
+    for (int k = 1; k <= NOP; ++k) {
+        long a = i + k * s, b = i - (k - 1) * s;
+        t += FD<T>::c[k] * ((lam[a] + 2 * mu[a]) * f[a] - (lam[b] + 2 * mu[b]) * f[b]);
+    }
+    return t;
+}
+
+struct Grid {
+    int nx, nz, pml, nxp, nzp, nt, ntr;
+    long n;
+};
+
+Grid make_grid(const or_config *c) {
+    Grid g;
+    g.nx = c->nx;
+    g.nz = c->nz;
+    g.pml = c->pml;
+    g.nxp = c->nx + 2 * c->pml;
+    g.nzp = c->nz + 2 * c->pml;
+    g.nt = c->nt;
+    g.ntr = c->ntr;
+    g.n = (long)g.nxp * g.nzp;
+    return g;
+}
+
+// The five wavefield totals and their ten split parts of one propagator.
+template <typename T> struct Wave {
+    std::vector<T> vx, vz, txx, tzz, txz, vx_x, vx_z, vz_x, vz_z, txx_x, txx_z, tzz_x, tzz_z, txz_x,
+        txz_z;
+    explicit Wave(long n) {
+        for (std::vector<T> *a : all()) a->assign(n, T(0));
+    }
+    std::vector<std::vector<T> *> all() {
+        return {&vx, &vz, &txx, &tzz, &txz, &vx_x, &vx_z, &vz_x, &vz_z,
+                &txx_x, &txx_z, &tzz_x, &tzz_z, &txz_x, &txz_z};
+    }
+};
+
+template <typename T> struct Medium {
+    std::vector<T> vp, vs, rho, lam, mu, dampx, dampz, dvp, dvs;
+};
+
+// Scalars the kernels take by value (float in the reference).
+template <typename T> struct Steps {
+    T dx, dz, dt;
+};
+
+int nthreads_of(const or_config *c) {
+#ifdef _OPENMP
+    return c->nthreads > 0 ? c->nthreads : omp_get_max_threads();
+#else
+    (void)c;
+    return 1;
+#endif
+}
+
+// ---------------------------------------------------------------- set-up ---
+
+// Window of the full model copied into the padded grid, PML filled by
+// clamping to the edge: z first, then x (corners come from the x pass).
+// sfwi-stage1/FD2DGPU.cu:112-128 (copyvelocityHtoD), :233-240 (copypmlVz),
+// :242-249 (copypmlVx), sequence :1348-1364 (bufferVpVsHtoD).
+template <typename T>
+void window_pad(const Grid &g, int allnz, int left, const float *all, std::vector<T> &out) {
+    out.assign(g.n, T(0));
+    for (int ix = 0; ix < g.nx; ++ix)
+        for (int iz = 0; iz < g.nz; ++iz)
+            out[(long)(ix + g.pml) * g.nzp + iz + g.pml] = (T)all[(long)(left + ix) * allnz + iz];
+    for (int ix = 0; ix < g.nxp; ++ix)
+        for (int iz = 0; iz < g.pml; ++iz) {
+            out[(long)ix * g.nzp + iz] = out[(long)ix * g.nzp + g.pml];
+            out[(long)ix * g.nzp + g.nzp - g.pml + iz] = out[(long)ix * g.nzp + g.nzp - g.pml - 1];
+        }
+    for (int ix = 0; ix < g.pml; ++ix)
+        for (int iz = 0; iz < g.nzp; ++iz) {
+            out[(long)ix * g.nzp + iz] = out[(long)g.pml * g.nzp + iz];
+            out[(long)(g.nxp - g.pml + ix) * g.nzp + iz] = out[(long)(g.nxp - g.pml - 1) * g.nzp + iz];
+        }
+}
+
+// PML profile value, sfwi-stage1/FD2DGPU.cu:460-465 (dampX) and :484-489
+// (dampZ):  log10(1/rr) * (5.0*vp/(2.0*pml)) * powf(1.0*d/pml, 4.0).
+// FP32 order: log10 of the float 1/rr rounded to float (CUDA log10f
+// overload), the 5.0/2.0 factor in double, powf's argument rounded to float
+// and x^4 taken as (x^2)^2 in double then rounded to float; product in double,
+// stored as float.
+inline float pml_profile_f32(float lg, float vp, int dist, int pml) {
+    float t = (float)(1.0 * dist / pml);
+    double t2 = (double)t * (double)t;
+    float p4 = (float)(t2 * t2);
+    return (float)((double)lg * (5.0 * (double)vp / (2.0 * pml)) * (double)p4);
+}
+inline double pml_profile_f64(double lg, double vp, int dist, int pml) {
+    double t = 1.0 * dist / pml;
+    return lg * (5.0 * vp / (2.0 * pml)) * (t * t * t * t);
+}
+
+template <typename T>
+void damping(const Grid &g, float rr, bool off, const std::vector<T> &vp, std::vector<T> &dampx,
+             std::vector<T> &dampz) {
+    // Interior stays 0: d_damp and d_dampz are memset once in the constructor
+    // (sfwi-stage1/FD2DGPU.cu:44, FD2DGPU.cu ctor of FD2DGPU_ELASTIC) and the
+    // profile kernels only write PML cells.
+    dampx.assign(g.n, T(0));
+    dampz.assign(g.n, T(0));
+    if (off) return;
+    const bool f32 = sizeof(T) == sizeof(float);
+    const float lg32 = (float)std::log10((double)(1.0f / rr));
+    const double lg64 = std::log10(1.0 / (double)rr);
+    for (int ix = 0; ix < g.nxp; ++ix)
+        for (int iz = 0; iz < g.nzp; ++iz) {
+            long i = (long)ix * g.nzp + iz;
+            int dxl = -1, dzl = -1;
+            if (ix < g.pml) dxl = g.pml - ix;
+            if (ix > g.nxp - g.pml) dxl = ix - (g.nxp - g.pml);   // strict >, Q3
+            if (iz < g.pml) dzl = g.pml - iz;
+            if (iz > g.nzp - g.pml) dzl = iz - (g.nzp - g.pml);
+            if (dxl >= 0)
+                dampx[i] = f32 ? (T)pml_profile_f32(lg32, (float)vp[i], dxl, g.pml)
+                               : (T)pml_profile_f64(lg64, (double)vp[i], dxl, g.pml);
+            if (dzl >= 0)
+                dampz[i] = f32 ? (T)pml_profile_f32(lg32, (float)vp[i], dzl, g.pml)
+                               : (T)pml_profile_f64(lg64, (double)vp[i], dzl, g.pml);
+        }
+}
+
+// lambda = rho (vp^2 - 2 vs^2), mu = rho vs^2: sfwi-stage1/FD2DGPU.cu:1116-1124;
+// stage 2 forces rho := 2000 inside the same kernel (sfwi-stage2/FD2DGPU.cu:1111).
+template <typename T>
+void build_medium(const or_config *c, const Grid &g, const float *vp, const float *vs,
+                  const float *rho, const float *dvp, const float *dvs, Medium<T> &m) {
+    window_pad(g, c->allnz, c->left, vp, m.vp);
+    window_pad(g, c->allnz, c->left, vs, m.vs);
+    window_pad(g, c->allnz, c->left, rho, m.rho);
+    if (c->rho_const > 0)
+        for (long i = 0; i < g.n; ++i) m.rho[i] = (T)c->rho_const;
+    damping(g, c->rr, c->pml_off != 0, m.vp, m.dampx, m.dampz);
+    m.lam.resize(g.n);
+    m.mu.resize(g.n);
+    for (long i = 0; i < g.n; ++i) {
+        m.lam[i] = m.rho[i] * (m.vp[i] * m.vp[i] - 2 * m.vs[i] * m.vs[i]);
+        m.mu[i] = m.rho[i] * m.vs[i] * m.vs[i];
+    }
+    if (dvp) window_pad(g, c->allnz, c->left, dvp, m.dvp);   // buffer_image_extrapolation :1336-1345
+    if (dvs) window_pad(g, c->allnz, c->left, dvs, m.dvs);
+}
+
+// --------------------------------------------------------------- kernels ---
+
+// Velocity update with split PML, sfwi-stage1/born.cu:355-394 (update_vx_vz).
+// The 1.0/rho literal promotes the update expression to FP64 (quirk Q1).
+template <typename T>
+void update_v(const Grid &g, const Steps<T> &st, const Medium<T> &m, Wave<T> &w, int dir, int nth) {
+    const long S = g.nzp;
+    T *vx = w.vx.data(), *vz = w.vz.data(), *vx_x = w.vx_x.data(), *vx_z = w.vx_z.data();
+    T *vz_x = w.vz_x.data(), *vz_z = w.vz_z.data();
+    const T *txx = w.txx.data(), *tzz = w.tzz.data(), *txz = w.txz.data();
+#pragma omp parallel for schedule(static) num_threads(nth)
+    for (int ix = NOP; ix < g.nxp - NOP; ++ix)
+        for (int iz = NOP; iz < g.nzp - NOP; ++iz) {
+            const long i = (long)ix * S + iz;
+            T d1 = 1 - st.dt * m.dampx[i] / 2, d2 = 1 + st.dt * m.dampx[i] / 2;
+            T d3 = 1 - st.dt * m.dampz[i] / 2, d4 = 1 + st.dt * m.dampz[i] / 2;
+            T s_txx_x = d_plus(txx, i, S);
+            T s_txz_z = d_minus(txz, i, 1);
+            T s_tzz_z = d_plus(tzz, i, 1);
+            T s_txz_x = d_minus(txz, i, S);
+            const double b = 1.0 / (double)m.rho[i];
+            vx_x[i] = (T)(((double)(d1 * vx_x[i]) + dir * b * (double)(st.dt / st.dx) * (double)s_txx_x) / (double)d2);
+            vx_z[i] = (T)(((double)(d3 * vx_z[i]) + dir * b * (double)(st.dt / st.dz) * (double)s_txz_z) / (double)d4);
+            vz_x[i] = (T)(((double)(d1 * vz_x[i]) + dir * b * (double)(st.dt / st.dx) * (double)s_txz_x) / (double)d2);
+            vz_z[i] = (T)(((double)(d3 * vz_z[i]) + dir * b * (double)(st.dt / st.dz) * (double)s_tzz_z) / (double)d4);
+            vx[i] = vx_x[i] + vx_z[i];
+            vz[i] = vz_x[i] + vz_z[i];
+        }
+}
+
+// Stress update with split PML, sfwi-stage1/born.cu:399-440
+// (update_txx_tzz_txz); with `grad` it is update_txx_tzz_txz_grad,
+// sfwi-stage1/born.cu:451-499, which also stores the four velocity
+// derivatives of the pre-update velocity.  mu is not averaged to the txz
+// position (quirk Q2).
+template <typename T>
+void update_s(const Grid &g, const Steps<T> &st, const Medium<T> &m, Wave<T> &w, int dir, int nth,
+              T *vx_gx = nullptr, T *vz_gz = nullptr, T *vx_gz = nullptr, T *vz_gx = nullptr) {
+    const long S = g.nzp;
+    const T *vx = w.vx.data(), *vz = w.vz.data();
+    T *txx = w.txx.data(), *tzz = w.tzz.data(), *txz = w.txz.data();
+    T *txx_x = w.txx_x.data(), *txx_z = w.txx_z.data(), *tzz_x = w.tzz_x.data();
+    T *tzz_z = w.tzz_z.data(), *txz_x = w.txz_x.data(), *txz_z = w.txz_z.data();
+    const T D = (T)dir;
+#pragma omp parallel for schedule(static) num_threads(nth)
+    for (int ix = NOP; ix < g.nxp - NOP; ++ix)
+        for (int iz = NOP; iz < g.nzp - NOP; ++iz) {
+            const long i = (long)ix * S + iz;
+            T d1 = 1 - st.dt * m.dampx[i] / 2, d2 = 1 + st.dt * m.dampx[i] / 2;
+            T d3 = 1 - st.dt * m.dampz[i] / 2, d4 = 1 + st.dt * m.dampz[i] / 2;
+            T s_vx_x = d_minus(vx, i, S);
+            T s_vz_z = d_minus(vz, i, 1);
+            T s_vx_z = d_plus(vx, i, 1);
+            T s_vz_x = d_plus(vz, i, S);
+            const T lam = m.lam[i], mu = m.mu[i];
+            txx_x[i] = (d1 * txx_x[i] + D * (lam + 2 * mu) * (st.dt / st.dx) * s_vx_x) / d2;
+            txx_z[i] = (d3 * txx_z[i] + D * lam * (st.dt / st.dz) * s_vz_z) / d4;
+            tzz_x[i] = (d1 * tzz_x[i] + D * lam * (st.dt / st.dx) * s_vx_x) / d2;
+            tzz_z[i] = (d3 * tzz_z[i] + D * (lam + 2 * mu) * (st.dt / st.dz) * s_vz_z) / d4;
+            txz_x[i] = (d1 * txz_x[i] + D * (st.dt / st.dx) * mu * s_vz_x) / d2;
+            txz_z[i] = (d3 * txz_z[i] + D * (st.dt / st.dz) * mu * s_vx_z) / d4;
+            txx[i] = txx_x[i] + txx_z[i];
+            tzz[i] = tzz_x[i] + tzz_z[i];
+            txz[i] = txz_x[i] + txz_z[i];
+            if (vx_gx) {
+                vx_gx[i] = s_vx_x / st.dx;
+                vz_gz[i] = s_vz_z / st.dz;
+                vx_gz[i] = s_vx_z / st.dz;
+                vz_gx[i] = s_vz_x / st.dx;
+            }
+        }
+}
+
+// Adjoint "stress" update, sfwi-stage1/born.cu:565-603
+// (GPUcalculate_elastic_txx_tzz_txz_back): no moduli, txx = txx_x and
+// tzz = tzz_z (quirk Q8).
+template <typename T>
+void adjoint_s(const Grid &g, const Steps<T> &st, const Medium<T> &m, Wave<T> &w, int dir, int nth) {
+    const long S = g.nzp;
+    const T *vx = w.vx.data(), *vz = w.vz.data();
+    T *txx = w.txx.data(), *tzz = w.tzz.data(), *txz = w.txz.data();
+    T *txx_x = w.txx_x.data(), *tzz_z = w.tzz_z.data(), *txz_x = w.txz_x.data(), *txz_z = w.txz_z.data();
+    const T D = (T)dir;
+#pragma omp parallel for schedule(static) num_threads(nth)
+    for (int ix = NOP; ix < g.nxp - NOP; ++ix)
+        for (int iz = NOP; iz < g.nzp - NOP; ++iz) {
+            const long i = (long)ix * S + iz;
+            T d1 = 1 - st.dt * m.dampx[i] / 2, d2 = 1 + st.dt * m.dampx[i] / 2;
+            T d3 = 1 - st.dt * m.dampz[i] / 2, d4 = 1 + st.dt * m.dampz[i] / 2;
+            T s_vx_x = d_minus(vx, i, S);
+            T s_vz_z = d_minus(vz, i, 1);
+            T s_vx_z = d_plus(vx, i, 1);
+            T s_vz_x = d_plus(vz, i, S);
+            txx_x[i] = (d1 * txx_x[i] + D * (st.dt / st.dx) * s_vx_x) / d2;
+            tzz_z[i] = (d3 * tzz_z[i] + D * (st.dt / st.dz) * s_vz_z) / d4;
+            txz_x[i] = (d1 * txz_x[i] + D * (st.dt / st.dx) * s_vz_x) / d2;
+            txz_z[i] = (d3 * txz_z[i] + D * (st.dt / st.dz) * s_vx_z) / d4;
+            txx[i] = txx_x[i];
+            tzz[i] = tzz_z[i];
+            txz[i] = txz_x[i] + txz_z[i];
+        }
+}
+
+// Adjoint velocity update with the moduli inside the derivatives,
+// sfwi-stage1/born.cu:504-562 (GPUcalculate_elastic_vx_vz_back).  FP64
+// promotion through 1.0/rho as in update_v.
+template <typename T>
+void adjoint_v(const Grid &g, const Steps<T> &st, const Medium<T> &m, Wave<T> &w, int dir, int nth) {
+    const long S = g.nzp;
+    T *vx = w.vx.data(), *vz = w.vz.data(), *vx_x = w.vx_x.data(), *vx_z = w.vx_z.data();
+    T *vz_x = w.vz_x.data(), *vz_z = w.vz_z.data();
+    const T *txx = w.txx.data(), *tzz = w.tzz.data(), *txz = w.txz.data();
+    const T *lam = m.lam.data(), *mu = m.mu.data();
+#pragma omp parallel for schedule(static) num_threads(nth)
+    for (int ix = NOP; ix < g.nxp - NOP; ++ix)
+        for (int iz = NOP; iz < g.nzp - NOP; ++iz) {
+            const long i = (long)ix * S + iz;
+            T d1 = 1 - st.dt * m.dampx[i] / 2, d2 = 1 + st.dt * m.dampx[i] / 2;
+            T d3 = 1 - st.dt * m.dampz[i] / 2, d4 = 1 + st.dt * m.dampz[i] / 2;
+            T s_txx_x = d_plus_p(lam, mu, txx, i, S);
+            T s_txz_z = d_minus_w(mu, txz, i, 1);
+            T s_tzz_x = d_plus_w(lam, tzz, i, S);
+            T s_tzz_z = d_plus_p(lam, mu, tzz, i, 1);
+            T s_txz_x = d_minus_w(mu, txz, i, S);
+            T s_txx_z = d_plus_w(lam, txx, i, 1);
+            const double b = 1.0 / (double)m.rho[i];
+            const double ax = (double)(st.dt / st.dx), az = (double)(st.dt / st.dz);
+            vx_x[i] = (T)(((double)(d1 * vx_x[i]) + dir * b * ax * (double)s_txx_x + dir * b * ax * (double)s_tzz_x) / (double)d2);
+            vx_z[i] = (T)(((double)(d3 * vx_z[i]) + dir * b * az * (double)s_txz_z) / (double)d4);
+            vz_x[i] = (T)(((double)(d1 * vz_x[i]) + dir * b * ax * (double)s_txz_x) / (double)d2);
+            vz_z[i] = (T)(((double)(d3 * vz_z[i]) + dir * b * az * (double)s_tzz_z + dir * b * az * (double)s_txx_z) / (double)d4);
+            vz[i] = vz_x[i] + vz_z[i];
+            vx[i] = vx_x[i] + vx_z[i];
+        }
+}
+
+// Born virtual source into the scattered split stresses from the updated
+// background velocity, stage-2 live kernel sfwi-stage2/born.cu:749-801
+// (add_born_source_components; bounds [nop, n-nop), quirk Q9).
+template <typename T>
+void born_source(const Grid &g, const Steps<T> &st, const Medium<T> &m, const Wave<T> &bg, Wave<T> &b,
+                 int dir, int nth) {
+    const long S = g.nzp;
+    const T *vx = bg.vx.data(), *vz = bg.vz.data();
+    const T D = (T)dir;
+#pragma omp parallel for schedule(static) num_threads(nth)
+    for (int ix = NOP; ix < g.nxp - NOP; ++ix)
+        for (int iz = NOP; iz < g.nzp - NOP; ++iz) {
+            const long i = (long)ix * S + iz;
+            T s_vx_x = d_minus(vx, i, S);
+            T s_vz_z = d_minus(vz, i, 1);
+            T s_vx_z = d_plus(vx, i, 1);
+            T s_vz_x = d_plus(vz, i, S);
+            const T rho = m.rho[i], vp = m.vp[i], vs = m.vs[i], dp = m.dvp[i], ds = m.dvs[i];
+            b.txx_x[i] = b.txx_x[i] + D * st.dt * (2 * (rho * vp) * (dp) * (s_vx_x) / st.dx);
+            b.txx_z[i] = b.txx_z[i] + D * st.dt * ((2 * (rho * vp) * (dp) - 4 * (rho * vs) * (ds)) * (s_vz_z) / st.dz);
+            b.tzz_x[i] = b.tzz_x[i] + D * st.dt * ((2 * (rho * vp) * (dp) - 4 * (rho * vs) * (ds)) * (s_vx_x) / st.dx);
+            b.tzz_z[i] = b.tzz_z[i] + D * st.dt * (2 * (rho * vp) * (dp) * (s_vz_z) / st.dz);
+            b.txz_x[i] = b.txz_x[i] + D * st.dt * ((2 * (rho * vs) * (ds)) * ((s_vz_x) / st.dx));
+            b.txz_z[i] = b.txz_z[i] + D * st.dt * ((2 * (rho * vs) * (ds)) * ((s_vx_z) / st.dz));
+        }
+}
+
+// ------------------------------------------------- sources and receivers ---
+
+// Receiver column on the padded grid: sfwi-stage1/FD2DGPU.cu:3162.
+std::vector<int> receiver_columns(const or_config *c, const int *gc) {
+    std::vector<int> igx(c->ntr);
+    for (int tr = 0; tr < c->ntr; ++tr)
+        igx[tr] = c->pml + (int)((float)(gc[tr] - c->ngx_left) / c->dx);
+    return igx;
+}
+
+// Boundary strips of vx ("omega") and vz ("theta"), N = 4 cells either side
+// of the PML interface, corners excluded (quirk Q14):
+// sfwi-stage1/FD2DGPU.cu:3842-3869 (save) and :3873-3899 (restore).
+template <typename T> struct Strips {
+    int nx, nz, pml, nxp, nzp, N;
+    std::vector<T> up, down, left, right;   // per field: [field][it][...]
+    void init(const Grid &g) {
+        nx = g.nx; nz = g.nz; pml = g.pml; nxp = g.nxp; nzp = g.nzp; N = NOP;
+        long a = 2L * (long)g.nt * 2 * N * nx, b = 2L * (long)g.nt * 2 * N * nz;
+        up.assign(a, T(0)); down.assign(a, T(0)); left.assign(b, T(0)); right.assign(b, T(0));
+    }
+    template <bool SAVE> void move(T *fvx, T *fvz, int it, int nt) {
+        T *f[2] = {fvx, fvz};
+        for (int k = 0; k < 2; ++k) {
+            long ou = (long)k * nt * 2 * N * nx + (long)it * nx * 2 * N;
+            long ol = (long)k * nt * 2 * N * nz + (long)it * 2 * N * nz;
+            for (int ix = pml; ix < nxp - pml; ++ix) {
+                for (int iz = pml - N; iz < pml + N; ++iz)
+                    sw<SAVE>(f[k][(long)ix * nzp + iz], up[ou + (long)(ix - pml) * 2 * N + (iz - (pml - N))]);
+                for (int iz = nzp - pml - N; iz < nzp - pml + N; ++iz)
+                    sw<SAVE>(f[k][(long)ix * nzp + iz], down[ou + (long)(ix - pml) * 2 * N + (iz - (nzp - pml - N))]);
+            }
+            for (int ix = pml - N; ix < pml + N; ++ix)
+                for (int iz = pml; iz < nzp - pml; ++iz)
+                    sw<SAVE>(f[k][(long)ix * nzp + iz], left[ol + (long)(ix - (pml - N)) * nz + (iz - pml)]);
+            for (int ix = nxp - pml - N; ix < nxp - pml + N; ++ix)
+                for (int iz = pml; iz < nzp - pml; ++iz)
+                    sw<SAVE>(f[k][(long)ix * nzp + iz], right[ol + (long)(ix - (nxp - pml - N)) * nz + (iz - pml)]);
+        }
+    }
+    template <bool SAVE> static void sw(T &field, T &buf) {
+        if (SAVE) buf = field; else field = buf;
+    }
+};
+
+// Zero-lag imaging of one reverse step, sfwi-stage1/FD2DGPU.cu:3350-3382
+// (doimage_gradient) and :3324-3331 (doimage_illum).
+template <typename T, typename G>
+void image_step(const or_config *c, const Grid &g, const Medium<T> &m, const Wave<T> &fw,
+                const Wave<T> &bk, const std::vector<T> &vx_gx, const std::vector<T> &vz_gz,
+                const std::vector<T> &vx_gz, const std::vector<T> &vz_gx, G *gvp, G *gvs, G *ill,
+                int nth) {
+#pragma omp parallel for schedule(static) num_threads(nth)
+    for (int ix = 0; ix < g.nx; ++ix)
+        for (int iz = 0; iz < g.nz; ++iz) {
+            const long p = (long)(ix + g.pml) * g.nzp + iz + g.pml;
+            const long q = (long)(c->left + ix) * g.nz + iz;
+            T lg = -((vx_gx[p] + vz_gz[p]) * (bk.txx[p] + bk.tzz[p]));
+            T mg = -2 * (vx_gx[p] * bk.txx[p] + vz_gz[p] * bk.tzz[p]) - (vx_gz[p] + vz_gx[p]) * bk.txz[p];
+            T pp = 2 * m.vp[p] * m.rho[p] * lg;
+            T ps = -4 * m.vs[p] * m.rho[p] * lg + 2 * m.vs[p] * m.rho[p] * mg;
+            T one = 1;
+            gvp[q] += one * pp;
+            gvs[q] += one * ps;
+            T il = fw.vx[p] * fw.vx[p] + fw.vz[p] * fw.vz[p];
+            ill[q] += one * il;
+        }
+}
+
+// Misfit sum over the residual, i-major then j, each term formed in the
+// record precision and accumulated into a double (sfwi-stage1/FD2DGPU.cu:4587-4590).
+template <typename T> double l2_sum(const std::vector<T> &rz, const std::vector<T> &rx, int ntr, int nt) {
+    double s = 0;
+    for (int i = 0; i < ntr; ++i)
+        for (int j = 0; j < nt; ++j) {
+            long k = (long)i * nt + j;
+            s += (rz[k] * rz[k] + rx[k] * rx[k]);
+        }
+    return s;
+}
+
+// ---------------------------------------------------------- per-shot runs ---
+
+template <typename T> struct Run {
+    const or_config *c;
+    Grid g;
+    Steps<T> st;
+    Medium<T> m;
+    int nth;
+    std::vector<int> igx;
+    Run(const or_config *cfg, const float *vp, const float *vs, const float *rho, const float *dvp,
+        const float *dvs, const int *gc)
+        : c(cfg), g(make_grid(cfg)) {
+        st.dx = (T)cfg->dx;
+        st.dz = (T)cfg->dz;
+        st.dt = (T)cfg->dt;
+        nth = nthreads_of(cfg);
+        build_medium(cfg, g, vp, vs, rho, dvp, dvs, m);
+        if (gc) igx = receiver_columns(cfg, gc);
+    }
+    long at(int ix, int iz) const { return (long)ix * g.nzp + iz; }
+    // Explosive source on txx and tzz before the velocity update,
+    // sfwi-stage1/FD2DGPU.cu:2104-2114.
+    void add_source(Wave<T> &w, const float *wav, int it) {
+        long i = at(c->isx, c->isz);
+        w.txx[i] += (T)wav[it];
+        w.tzz[i] += (T)wav[it];
+    }
+    // Receiver sampling, sfwi-stage1/FD2DGPU.cu:3158-3166.
+    void record(const Wave<T> &w, int it, std::vector<T> &rz, std::vector<T> &rx) {
+        for (int tr = 0; tr < g.ntr; ++tr) {
+            long i = at(igx[tr], c->igz);
+            rz[(long)tr * g.nt + it] = w.vz[i];
+            rx[(long)tr * g.nt + it] = w.vx[i];
+        }
+    }
+    // Adjoint source: the residual overwrites the adjoint velocity at the
+    // receivers (stack=false), sfwi-stage1/FD2DGPU.cu:3425-3441 (quirk Q6).
+    void inject(Wave<T> &bk, int it, const std::vector<T> &rz, const std::vector<T> &rx) {
+        for (int tr = 0; tr < g.ntr; ++tr) {
+            long i = at(igx[tr], c->igz);
+            bk.vz[i] = rz[(long)tr * g.nt + it];
+            bk.vx[i] = rx[(long)tr * g.nt + it];
+        }
+    }
+
+    // Background forward modeling, born_modeling mode 0
+    // (sfwi-stage1/FD2DGPU.cu:1864-1874) inside the loop :4535-4564.
+    void forward(const float *wav, std::vector<T> &rz, std::vector<T> &rx, Wave<T> *keep = nullptr,
+                 Strips<T> *strips = nullptr) {
+        Wave<T> w(g.n);
+        for (int it = 0; it < g.nt; ++it) {
+            add_source(w, wav, it);
+            update_v(g, st, m, w, +1, nth);
+            update_s(g, st, m, w, +1, nth);
+            record(w, it, rz, rx);
+            if (strips) strips->template move<true>(w.vx.data(), w.vz.data(), it, g.nt);
+        }
+        if (keep) *keep = w;
+    }
+
+    // Stage-2 forward, mode 1 (sfwi-stage2/FD2DGPU.cu:1814-1845) inside
+    // sfwi-stage2/FD2DGPU.cu:4223-4314: background step, scattered velocity,
+    // Born source from the updated background velocity, scattered stress.
+    void born_forward(const float *wav, std::vector<T> &bz, std::vector<T> &bx, std::vector<T> &gz,
+                      std::vector<T> &gx, Wave<T> *keep = nullptr, Strips<T> *strips = nullptr) {
+        Wave<T> w(g.n), b(g.n);
+        for (int it = 0; it < g.nt; ++it) {
+            add_source(w, wav, it);
+            update_v(g, st, m, w, +1, nth);
+            update_s(g, st, m, w, +1, nth);
+            update_v(g, st, m, b, +1, nth);
+            born_source(g, st, m, w, b, +1, nth);
+            update_s(g, st, m, b, +1, nth);
+            record(b, it, bz, bx);   // record_scatter: scatter = b, scatter2 = background
+            record(w, it, gz, gx);
+            if (strips) strips->template move<true>(w.vx.data(), w.vz.data(), it, g.nt);
+        }
+        if (keep) *keep = w;
+    }
+
+    // Reverse loop of IMAGE_SINGLESHOT (sfwi-stage1/FD2DGPU.cu:4600-4640;
+    // stage 2 :4379-4510): restore strips, reconstruct (mode 6: stress with
+    // derivatives, then velocity, direction -1, source not subtracted, Q5),
+    // inject the residual, adjoint step (mode 8) with the stage's direction
+    // (Q7), image.
+    template <typename G>
+    void reverse(Wave<T> &fw, Strips<T> &strips, const std::vector<T> &rz, const std::vector<T> &rx,
+                 G *gvp, G *gvs, G *ill) {
+        Wave<T> bk(g.n);
+        std::vector<T> vx_gx(g.n, T(0)), vz_gz(g.n, T(0)), vx_gz(g.n, T(0)), vz_gx(g.n, T(0));
+        for (int it = g.nt - 1; it >= 0; --it) {
+            strips.template move<false>(fw.vx.data(), fw.vz.data(), it, g.nt);
+            update_s(g, st, m, fw, -1, nth, vx_gx.data(), vz_gz.data(), vx_gz.data(), vz_gx.data());
+            update_v(g, st, m, fw, -1, nth);
+            inject(bk, it, rz, rx);
+            adjoint_s(g, st, m, bk, c->adjoint_dir, nth);
+            adjoint_v(g, st, m, bk, c->adjoint_dir, nth);
+            image_step(c, g, m, fw, bk, vx_gx, vz_gz, vx_gz, vz_gx, gvp, gvs, ill, nth);
+        }
+    }
+};
+
+template <typename T, typename G>
+void gradient_impl(const or_config *c, const float *vp, const float *vs, const float *rho,
+                   const float *wav, const int *gc, const float *obs_z, const float *obs_x, G *gvp,
+                   G *gvs, G *ill, double *misfit, G *res_z, G *res_x) {
+    Run<T> r(c, vp, vs, rho, nullptr, nullptr, gc);
+    const long nr = (long)c->ntr * c->nt;
+    std::vector<T> sz(nr), sx(nr);
+    Strips<T> strips;
+    strips.init(r.g);
+    Wave<T> fw(1);
+    r.forward(wav, sz, sx, &fw, &strips);
+    // r = 1.0*syn - obs  (sfwi-stage1/FD2DGPU.cu:3181-3195 with alpha = 1)
+    for (long k = 0; k < nr; ++k) {
+        sz[k] = (T)1.0f * sz[k] - (T)obs_z[k];
+        sx[k] = (T)1.0f * sx[k] - (T)obs_x[k];
+    }
+    *misfit += l2_sum(sz, sx, c->ntr, c->nt);
+    if (res_z)
+        for (long k = 0; k < nr; ++k) { res_z[k] = (G)sz[k]; res_x[k] = (G)sx[k]; }
+    r.reverse(fw, strips, sz, sx, gvp, gvs, ill);
+}
+
+template <typename T, typename G>
+void born_gradient_impl(const or_config *c, const float *vp, const float *vs, const float *rho,
+                        const float *dvp, const float *dvs, const float *wav, const int *gc,
+                        const float *obs_z, const float *obs_x, G *gvp, G *gvs, G *ill, double *misfit) {
+    Run<T> r(c, vp, vs, rho, dvp, dvs, gc);
+    const long nr = (long)c->ntr * c->nt;
+    std::vector<T> bz(nr), bx(nr), gz(nr), gx(nr);
+    Strips<T> strips;
+    strips.init(r.g);
+    Wave<T> fw(1);
+    r.born_forward(wav, bz, bx, gz, gx, &fw, &strips);
+    // r = (b - obs) then r = 1*r + background: sfwi-stage2/FD2DGPU.cu:3119-3130,
+    // :3144-3158 (the subtract ignores alpha, quirk Q16).
+    for (long k = 0; k < nr; ++k) {
+        bz[k] = bz[k] - (T)obs_z[k];
+        bx[k] = bx[k] - (T)obs_x[k];
+        bz[k] = (T)1.0f * bz[k] + gz[k];
+        bx[k] = (T)1.0f * bx[k] + gx[k];
+    }
+    *misfit += l2_sum(bz, bx, c->ntr, c->nt);
+    r.reverse(fw, strips, bz, bx, gvp, gvs, ill);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- C entries ---
+
+extern "C" {
+
+void or_coeffs_f32(float c[5]) { for (int k = 0; k < 5; ++k) c[k] = FD<float>::c[k]; }
+void or_coeffs_f64(double c[5]) { for (int k = 0; k < 5; ++k) c[k] = FD<double>::c[k]; }
+
+// sfwi-stage1/main.cpp:251-260 (PI = 3.14159265358979, main.cpp:8).
+void or_ricker(float *w, int nt, float dt, float f0) {
+    const double pi = 3.14159265358979;
+    float nk = pi * pi * f0 * f0 * dt * dt;
+    int t0 = (int)std::ceil(1.0 / (f0 * dt));
+    for (int t = 0; t < nt; ++t)
+        w[t] = (1.0 - 2.0 * nk * (t - t0) * (t - t0)) * std::exp(-nk * (t - t0) * (t - t0));
+}
+
+}  // extern "C"
+
+template <typename T>
+static void setup_impl(const or_config *c, const float *vp, const float *vs, const float *rho, T *out) {
+    Grid g = make_grid(c);
+    Medium<T> m;
+    build_medium(c, g, vp, vs, rho, nullptr, nullptr, m);
+    const std::vector<T> *arr[7] = {&m.vp, &m.vs, &m.rho, &m.lam, &m.mu, &m.dampx, &m.dampz};
+    for (int k = 0; k < 7; ++k) std::memcpy(out + (long)k * g.n, arr[k]->data(), g.n * sizeof(T));
+}
+
+extern "C" {
+void or_setup_f32(const or_config *c, const float *vp, const float *vs, const float *rho, float *o) { setup_impl(c, vp, vs, rho, o); }
+void or_setup_f64(const or_config *c, const float *vp, const float *vs, const float *rho, double *o) { setup_impl(c, vp, vs, rho, o); }
+
+}  // extern "C"
+
+template <typename T>
+static void forward_impl(const or_config *c, const float *vp, const float *vs, const float *rho,
+                         const float *wav, const int *gc, T *syn_z, T *syn_x) {
+    Run<T> r(c, vp, vs, rho, nullptr, nullptr, gc);
+    const long nr = (long)c->ntr * c->nt;
+    std::vector<T> sz(nr), sx(nr);
+    r.forward(wav, sz, sx);
+    std::memcpy(syn_z, sz.data(), nr * sizeof(T));
+    std::memcpy(syn_x, sx.data(), nr * sizeof(T));
+}
+
+extern "C" {
+void or_forward_f32(const or_config *c, const float *vp, const float *vs, const float *rho,
+                    const float *wav, const int *gc, float *syn_z, float *syn_x) {
+    forward_impl<float>(c, vp, vs, rho, wav, gc, syn_z, syn_x);
+}
+void or_forward_f64(const or_config *c, const float *vp, const float *vs, const float *rho,
+                    const float *wav, const int *gc, double *syn_z, double *syn_x) {
+    forward_impl<double>(c, vp, vs, rho, wav, gc, syn_z, syn_x);
+}
+
+void or_gradient_f32(const or_config *c, const float *vp, const float *vs, const float *rho,
+                     const float *wav, const int *gc, const float *obs_z, const float *obs_x,
+                     float *gvp, float *gvs, float *ill, double *misfit, float *res_z, float *res_x) {
+    gradient_impl<float, float>(c, vp, vs, rho, wav, gc, obs_z, obs_x, gvp, gvs, ill, misfit, res_z, res_x);
+}
+void or_gradient_f64(const or_config *c, const float *vp, const float *vs, const float *rho,
+                     const float *wav, const int *gc, const float *obs_z, const float *obs_x,
+                     double *gvp, double *gvs, double *ill, double *misfit, double *res_z, double *res_x) {
+    gradient_impl<double, double>(c, vp, vs, rho, wav, gc, obs_z, obs_x, gvp, gvs, ill, misfit, res_z, res_x);
+}
+
+}  // extern "C"
+
+template <typename T>
+static void born_forward_impl(const or_config *c, const float *vp, const float *vs, const float *rho,
+                              const float *dvp, const float *dvs, const float *wav, const int *gc,
+                              T *b_z, T *b_x, T *bg_z, T *bg_x) {
+    Run<T> r(c, vp, vs, rho, dvp, dvs, gc);
+    const long nr = (long)c->ntr * c->nt;
+    std::vector<T> bz(nr), bx(nr), gz(nr), gx(nr);
+    r.born_forward(wav, bz, bx, gz, gx);
+    std::memcpy(b_z, bz.data(), nr * sizeof(T));
+    std::memcpy(b_x, bx.data(), nr * sizeof(T));
+    std::memcpy(bg_z, gz.data(), nr * sizeof(T));
+    std::memcpy(bg_x, gx.data(), nr * sizeof(T));
+}
+
+extern "C" {
+void or_born_forward_f32(const or_config *c, const float *vp, const float *vs, const float *rho,
+                         const float *dvp, const float *dvs, const float *wav, const int *gc,
+                         float *b_z, float *b_x, float *bg_z, float *bg_x) {
+    born_forward_impl<float>(c, vp, vs, rho, dvp, dvs, wav, gc, b_z, b_x, bg_z, bg_x);
+}
+void or_born_forward_f64(const or_config *c, const float *vp, const float *vs, const float *rho,
+                         const float *dvp, const float *dvs, const float *wav, const int *gc,
+                         double *b_z, double *b_x, double *bg_z, double *bg_x) {
+    born_forward_impl<double>(c, vp, vs, rho, dvp, dvs, wav, gc, b_z, b_x, bg_z, bg_x);
+}
+
+void or_born_gradient_f32(const or_config *c, const float *vp, const float *vs, const float *rho,
+                          const float *dvp, const float *dvs, const float *wav, const int *gc,
+                          const float *obs_z, const float *obs_x, float *gvp, float *gvs, float *ill,
+                          double *misfit) {
+    born_gradient_impl<float, float>(c, vp, vs, rho, dvp, dvs, wav, gc, obs_z, obs_x, gvp, gvs, ill, misfit);
+}
+void or_born_gradient_f64(const or_config *c, const float *vp, const float *vs, const float *rho,
+                          const float *dvp, const float *dvs, const float *wav, const int *gc,
+                          const float *obs_z, const float *obs_x, double *gvp, double *gvs, double *ill,
+                          double *misfit) {
+    born_gradient_impl<double, double>(c, vp, vs, rho, dvp, dvs, wav, gc, obs_z, obs_x, gvp, gvs, ill, misfit);
+}
+
+// Stage-1 line search: three forward solves at vel, vel-step1*d and
+// vel-step2*d (test model formed in float on the host),
+// sfwi-stage1/FD2DGPU.cu:4819-5028 (CALCULATE_ALPHA_parabola_mb).
+void or_parabola_misfits_f32(const or_config *c, const float *vp, const float *vs, const float *rho,
+                             const float *dir_vp, const float *dir_vs, float step1, float step2,
+                             const float *wav, const int *gc, const float *obs_z, const float *obs_x,
+                             double m[3]) {
+    const long nm = (long)c->allnx * c->allnz, nr = (long)c->ntr * c->nt;
+    std::vector<float> tp(nm), ts(nm), sz(nr), sx(nr);
+    const float steps[3] = {0.0f, step1, step2};
+    for (int k = 0; k < 3; ++k) {
+        const float *mp = vp, *ms = vs;
+        if (k > 0) {
+            for (long i = 0; i < nm; ++i) {
+                tp[i] = vp[i] - steps[k] * dir_vp[i];
+                ts[i] = vs[i] - steps[k] * dir_vs[i];
+            }
+            mp = tp.data();
+            ms = ts.data();
+        }
+        forward_impl<float>(c, mp, ms, rho, wav, gc, sz.data(), sx.data());
+        std::vector<float> rz(nr), rx(nr);
+        for (long i = 0; i < nr; ++i) {
+            rz[i] = 1.0f * sz[i] - obs_z[i];
+            rx[i] = 1.0f * sx[i] - obs_x[i];
+        }
+        m[k] += l2_sum(rz, rx, c->ntr, c->nt);
+    }
+}
+
+// Stage-2 exact step: sfwi-stage2/FD2DGPU.cu:4576-4722 (CALCULATE_ALPHA).
+void or_linear_alpha_f32(const or_config *c, const float *vp, const float *vs, const float *rho,
+                         const float *dvp, const float *dvs, const float *cg_vp, const float *cg_vs,
+                         const float *wav, const int *gc, const float *obs_z, const float *obs_x,
+                         double *fenzi, double *fenmu) {
+    const long nr = (long)c->ntr * c->nt;
+    std::vector<float> bz(nr), bx(nr), gz(nr), gx(nr), lz(nr), lx(nr);
+    born_forward_impl<float>(c, vp, vs, rho, dvp, dvs, wav, gc, bz.data(), bx.data(), gz.data(), gx.data());
+    for (long k = 0; k < nr; ++k) {
+        bz[k] = bz[k] - obs_z[k];
+        bx[k] = bx[k] - obs_x[k];
+        bz[k] = 1.0f * bz[k] + gz[k];
+        bx[k] = 1.0f * bx[k] + gx[k];
+    }
+    born_forward_impl<float>(c, vp, vs, rho, cg_vp, cg_vs, wav, gc, lz.data(), lx.data(), gz.data(), gx.data());
+    for (int i = 0; i < c->ntr; ++i)
+        for (int j = 0; j < c->nt; ++j) {
+            long k = (long)i * c->nt + j;
+            *fenmu += (lz[k] * lz[k] + lx[k] * lx[k]);
+            *fenzi += (lz[k] * bz[k] + lx[k] * bx[k]);
+        }
+}
+
+void or_stencils_f64(int nxpml, int nzpml, const double *f, double *dxp, double *dxm, double *dzp,
+                     double *dzm) {
+    const long S = nzpml;
+    for (int ix = NOP; ix < nxpml - NOP; ++ix)
+        for (int iz = NOP; iz < nzpml - NOP; ++iz) {
+            long i = (long)ix * S + iz;
+            dxp[i] = d_plus(f, i, S);
+            dxm[i] = d_minus(f, i, S);
+            dzp[i] = d_plus(f, i, 1);
+            dzm[i] = d_minus(f, i, 1);
+        }
+}
+
+void or_propagate_f64(const or_config *c, const float *vp, const float *vs, const float *rho, int kind,
+                      int dir, int nsteps, double *state) {
+    Run<double> r(c, vp, vs, rho, nullptr, nullptr, nullptr);
+    Wave<double> w(r.g.n);
+    auto arrs = w.all();
+    for (size_t k = 0; k < arrs.size(); ++k)
+        std::memcpy(arrs[k]->data(), state + (long)k * r.g.n, r.g.n * sizeof(double));
+    for (int s = 0; s < nsteps; ++s) {
+        if (kind == 0) {
+            update_v(r.g, r.st, r.m, w, dir, r.nth);
+            update_s(r.g, r.st, r.m, w, dir, r.nth);
+        } else if (kind == 1) {
+            update_s(r.g, r.st, r.m, w, dir, r.nth);
+            update_v(r.g, r.st, r.m, w, dir, r.nth);
+        } else {
+            adjoint_s(r.g, r.st, r.m, w, dir, r.nth);
+            adjoint_v(r.g, r.st, r.m, w, dir, r.nth);
+        }
+    }
+    for (size_t k = 0; k < arrs.size(); ++k)
+        std::memcpy(state + (long)k * r.g.n, arrs[k]->data(), r.g.n * sizeof(double));
+}
+
+}  // extern "C"

@@ -1,0 +1,110 @@
+/*
+ * CPU oracle for SFWI's per-shot elastic modeling + gradient path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: tests/, the smoke
+ * check in __graft_entry__.py and bench.py's cpu_baseline leg may load it,
+ * nothing in sfwi_amd/ may.  It restates the reference algorithm on the CPU in
+ * two precisions:
+ *   *_f32 : the reference's FP32 operation order, including the places where a
+ *           double literal promotes an expression to FP64 (built with
+ *           -ffp-contract=off so no FMA is formed that the C source lacks);
+ *   *_f64 : every array and every operation in FP64 (accuracy oracle and timed
+ *           CPU baseline).
+ *
+ * All model/gradient arrays are the reference's .bin layout: x-major,
+ * z fastest, index ix*allnz + iz.  Records are trace-major [ntr][nt].
+ */
+#ifndef SFWI_ORACLE_H
+#define SFWI_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct or_config {
+    int allnx, allnz;     /* full model size (x-major, z fastest)                       */
+    int nx, nz;           /* shot window (nz == allnz in the reference)                 */
+    int pml, nt, ntr;     /* PML width, time samples, traces                             */
+    float dx, dz, dt;     /* grid spacing (m), time step (s)                             */
+    float rr;             /* PML reflection coefficient, 0.001 in the reference           */
+    int left;             /* first model column of the window (delta_left+record_left)   */
+    int isx, isz, igz;    /* source column/row and receiver row on the padded grid       */
+    int ngx_left;         /* coordinate of the window's left edge (receiver placement)   */
+    int adjoint_dir;      /* +1 stage 1, -1 stage 2                                       */
+    float rho_const;      /* > 0: padded rho forced to this value (stage-2 kernel)       */
+    int pml_off;          /* test hook: damping profiles forced to zero                   */
+    int nthreads;         /* OpenMP threads, 0 = runtime default                          */
+} or_config;
+
+/* FD coefficients c1..c4 of the 8th-order staggered stencil (index 0 unused). */
+void or_coeffs_f32(float c[5]);
+void or_coeffs_f64(double c[5]);
+
+/* Ricker wavelet exactly as the host driver builds it. */
+void or_ricker(float *w, int nt, float dt, float f0);
+
+/* Padded medium of one shot window: vp, vs, rho, lambda, mu, dampx, dampz
+ * (each nxpml*nzpml, z fastest).  out7 = 7 consecutive arrays. */
+void or_setup_f32(const or_config *c, const float *vp, const float *vs, const float *rho, float *out7);
+void or_setup_f64(const or_config *c, const float *vp, const float *vs, const float *rho, double *out7);
+
+/* Forward solve: receiver traces of vz and vx, [ntr][nt]. */
+void or_forward_f32(const or_config *c, const float *vp, const float *vs, const float *rho,
+                    const float *wav, const int *gc, float *syn_z, float *syn_x);
+void or_forward_f64(const or_config *c, const float *vp, const float *vs, const float *rho,
+                    const float *wav, const int *gc, double *syn_z, double *syn_x);
+
+/* Stage-1 per-shot gradient.  Accumulates into grad_vp, grad_vs, illum
+ * (allnx*allnz) and misfit; writes the residual syn-obs when res_* non-NULL. */
+void or_gradient_f32(const or_config *c, const float *vp, const float *vs, const float *rho,
+                     const float *wav, const int *gc, const float *obs_z, const float *obs_x,
+                     float *grad_vp, float *grad_vs, float *illum, double *misfit,
+                     float *res_z, float *res_x);
+void or_gradient_f64(const or_config *c, const float *vp, const float *vs, const float *rho,
+                     const float *wav, const int *gc, const float *obs_z, const float *obs_x,
+                     double *grad_vp, double *grad_vs, double *illum, double *misfit,
+                     double *res_z, double *res_x);
+
+/* Stage-2 Born forward: scattered traces b_* and background traces bg_*. */
+void or_born_forward_f32(const or_config *c, const float *vp, const float *vs, const float *rho,
+                         const float *dvp, const float *dvs, const float *wav, const int *gc,
+                         float *b_z, float *b_x, float *bg_z, float *bg_x);
+void or_born_forward_f64(const or_config *c, const float *vp, const float *vs, const float *rho,
+                         const float *dvp, const float *dvs, const float *wav, const int *gc,
+                         double *b_z, double *b_x, double *bg_z, double *bg_x);
+
+/* Stage-2 per-shot gradient (residual = born - obs + background). */
+void or_born_gradient_f32(const or_config *c, const float *vp, const float *vs, const float *rho,
+                          const float *dvp, const float *dvs, const float *wav, const int *gc,
+                          const float *obs_z, const float *obs_x,
+                          float *grad_vp, float *grad_vs, float *illum, double *misfit);
+void or_born_gradient_f64(const or_config *c, const float *vp, const float *vs, const float *rho,
+                          const float *dvp, const float *dvs, const float *wav, const int *gc,
+                          const float *obs_z, const float *obs_x,
+                          double *grad_vp, double *grad_vs, double *illum, double *misfit);
+
+/* Stage-1 line search: misfits at vel, vel - step1*dir, vel - step2*dir. */
+void or_parabola_misfits_f32(const or_config *c, const float *vp, const float *vs, const float *rho,
+                             const float *dir_vp, const float *dir_vs, float step1, float step2,
+                             const float *wav, const int *gc, const float *obs_z, const float *obs_x,
+                             double m[3]);
+/* Stage-2 exact step: fenzi += sum L(cg).r, fenmu += sum L(cg)^2. */
+void or_linear_alpha_f32(const or_config *c, const float *vp, const float *vs, const float *rho,
+                         const float *dvp, const float *dvs, const float *cg_vp, const float *cg_vs,
+                         const float *wav, const int *gc, const float *obs_z, const float *obs_x,
+                         double *fenzi, double *fenmu);
+
+/* Test hooks (FP64): the four stencils of one field, and free propagation of
+ * a given state.  state = 15 arrays of nxpml*nzpml in the order
+ * vx, vz, txx, tzz, txz, vx_x, vx_z, vz_x, vz_z, txx_x, txx_z, tzz_x, tzz_z, txz_x, txz_z.
+ * kind 0: forward steps (v then sigma) with direction dir;
+ * kind 1: reconstruction steps (sigma then v) with direction dir;
+ * kind 2: adjoint steps (adjoint sigma then adjoint v) with direction dir. */
+void or_stencils_f64(int nxpml, int nzpml, const double *f,
+                     double *dxp, double *dxm, double *dzp, double *dzm);
+void or_propagate_f64(const or_config *c, const float *vp, const float *vs, const float *rho,
+                      int kind, int dir, int nsteps, double *state);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

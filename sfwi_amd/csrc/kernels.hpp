@@ -1,0 +1,753 @@
+// Device code of the SFWI per-shot elastic path for gfx950 (MI355X).
+//
+// Data layout (HBM): every padded field is nxpml*nzpml floats, index
+// ix*nzpml + iz (z fastest, the reference's layout), one contiguous block per
+// shot slot.  A batch of shots is one launch: blockIdx.y selects the shot,
+// blockIdx.x a run of 256 consecutive cells of the update domain
+// [4, nxpml-4) x [4, nzpml-4) in z-fastest order, so every load that is not a
+// stencil tap is a fully coalesced 1 KiB wave access and the x taps are the
+// same runs shifted by whole columns.  Workgroups are remapped so that each of
+// the 8 XCDs owns a contiguous eighth of every shot's cells: the medium arrays
+// shared by all shots of a batch then stay in that XCD's L2.
+//
+// Two numerical modes share the kernels:
+//   EXACT = true : every cell keeps the reference's split PML fields and the
+//                  reference's operation order, including the FP64 promotion
+//                  of the velocity updates (1.0/rho) and no FMA contraction
+//                  (the translation unit is built with -ffp-contract=off).
+//   EXACT = false: split fields only where the reference's split state can
+//                  differ from the total (the damped PML, the 4-cell strips
+//                  that the reconstruction overwrites, the source cell for the
+//                  stresses and the receiver row for the adjoint velocity);
+//                  everywhere else a cell carries only the five totals, which
+//                  is the same update without the split.  FP32 with explicit
+//                  FMAs in the stencil sums.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sfwi {
+
+constexpr int NOP = 4;        // stencil half width (reference nop)
+constexpr int NSTRIP = 4;     // strip half width (reference N = nop)
+constexpr int BLK = 256;      // threads per workgroup
+
+// 8th-order staggered coefficients, float as in sfwi-stage1/FD2DGPU.cuh:16.
+__constant__ float kC[5] = {0.0f, (float)(1225.0 / 1024.0), (float)(-245.0 / 3072.0),
+                            (float)(49.0 / 5120.0), (float)(-5.0 / 7168.0)};
+
+// Pointers of one propagator's fields.
+struct Fields {
+    float *vx, *vz, *txx, *tzz, *txz;
+    float *vx_x, *vx_z, *vz_x, *vz_z;
+    float *txx_x, *txx_z, *tzz_x, *tzz_z, *txz_x, *txz_z;
+};
+
+// Padded medium of one shot window.
+struct Medium {
+    const float *vp, *vs, *rho, *lam, *mu, *dampx, *dampz;
+    const float *bu;          // 1/rho (FP32 fast mode)
+    const float *icp, *ics;   // imaging coefficients 2 vp rho, 2 vs rho (fast mode)
+    const float *dvp, *dvs;   // padded perturbation (Born)
+};
+
+// Per-shot device descriptor, read with scalar loads (uniform per block).
+struct DevShot {
+    Medium m;
+    Fields fw, bk, bn;                     // forward/reconstructed, adjoint, Born scattered
+    float *dgx, *dgz, *dgxz, *dgzx;        // velocity derivatives for imaging
+    float *rz, *rx;                        // synthetic (then residual) traces [ntr][nt]
+    float *r2z, *r2x;                      // background traces of a Born solve
+    const float *oz, *ox;                  // observed traces
+    const int *igx;                        // receiver columns [ntr]
+    const int *inj;                        // traces injected by the adjoint [ninj]
+    float *strip;                          // boundary strips, [nt][2 fields][SF]
+    float *gvp, *gvs, *gil;                // per-shot gradient accumulators [nx][nz]
+    double *trace_sum;                     // per-trace partial misfit [ntr]
+    int isx, isz, igz, ntr, ninj, left;
+};
+
+// Geometry shared by every shot of a batch.
+struct Geo {
+    int nxp, nzp, pml, nx, nz, nt, allnz;
+    int nzd, ndom;          // update domain height and cell count
+    int per_xcd;            // blocks per XCD in the remapped grid
+    int sf;                 // strip floats per field and step
+    float inv_nzd;
+    float dt, dx, dz, dtdx, dtdz, idx, idz;
+};
+
+// ------------------------------------------------------------- helpers ---
+
+// Cell of this thread in the update domain, XCD-aware block order.
+__device__ inline bool cell_of(const Geo &g, int &ix, int &iz) {
+    const int b = blockIdx.x;
+    const int L = (b & 7) * g.per_xcd + (b >> 3);
+    const int l = L * BLK + (int)threadIdx.x;
+    if (l >= g.ndom) return false;
+    int x = (int)(((float)l + 0.5f) * g.inv_nzd);
+    int z = l - x * g.nzd;
+    if (z < 0) { --x; z += g.nzd; } else if (z >= g.nzd) { ++x; z -= g.nzd; }
+    ix = x + NOP;
+    iz = z + NOP;
+    return true;
+}
+
+// Cells whose split state must be kept in fast mode: within pml+4 of an edge
+// (damped PML plus the strip band of the reconstruction).
+__device__ inline bool in_band(const Geo &g, int ix, int iz) {
+    const int w = g.pml + NSTRIP;
+    return ix < w || ix >= g.nxp - w || iz < w || iz >= g.nzp - w;
+}
+
+// sum_k c_k (f[i+k s] - f[i-(k-1) s])   taps i-3..i+4
+template <bool EXACT> __device__ inline float d_plus(const float *__restrict__ f, int i, int s) {
+    if (EXACT) {
+        float t = 0.0f;
+        t += kC[1] * (f[i + s] - f[i]);
+        t += kC[2] * (f[i + 2 * s] - f[i - s]);
+        t += kC[3] * (f[i + 3 * s] - f[i - 2 * s]);
+        t += kC[4] * (f[i + 4 * s] - f[i - 3 * s]);
+        return t;
+    } else {
+        float t = kC[1] * (f[i + s] - f[i]);
+        t = __builtin_fmaf(kC[2], f[i + 2 * s] - f[i - s], t);
+        t = __builtin_fmaf(kC[3], f[i + 3 * s] - f[i - 2 * s], t);
+        t = __builtin_fmaf(kC[4], f[i + 4 * s] - f[i - 3 * s], t);
+        return t;
+    }
+}
+// sum_k c_k (f[i+(k-1) s] - f[i-k s])   taps i-4..i+3
+template <bool EXACT> __device__ inline float d_minus(const float *__restrict__ f, int i, int s) {
+    if (EXACT) {
+        float t = 0.0f;
+        t += kC[1] * (f[i] - f[i - s]);
+        t += kC[2] * (f[i + s] - f[i - 2 * s]);
+        t += kC[3] * (f[i + 2 * s] - f[i - 3 * s]);
+        t += kC[4] * (f[i + 3 * s] - f[i - 4 * s]);
+        return t;
+    } else {
+        float t = kC[1] * (f[i] - f[i - s]);
+        t = __builtin_fmaf(kC[2], f[i + s] - f[i - 2 * s], t);
+        t = __builtin_fmaf(kC[3], f[i + 2 * s] - f[i - 3 * s], t);
+        t = __builtin_fmaf(kC[4], f[i + 3 * s] - f[i - 4 * s], t);
+        return t;
+    }
+}
+
+// Damping factors of one cell: d1 = 1 - dt*a/2, d2 = 1 + dt*a/2 (reference order).
+struct Damp {
+    float d1, d2, d3, d4;
+};
+__device__ inline Damp damp_of(const Geo &g, const Medium &m, int i) {
+    const float ax = m.dampx[i], az = m.dampz[i];
+    Damp d;
+    d.d1 = 1.0f - g.dt * ax / 2.0f;
+    d.d2 = 1.0f + g.dt * ax / 2.0f;
+    d.d3 = 1.0f - g.dt * az / 2.0f;
+    d.d4 = 1.0f + g.dt * az / 2.0f;
+    return d;
+}
+
+// --------------------------------------------------- one-cell updates ---
+
+// Velocity update (sfwi-stage1/born.cu:355-394, update_vx_vz).
+template <bool EXACT>
+__device__ inline void vel_cell(const Geo &g, const Medium &m, const Fields &F, int i, bool split,
+                                int dir) {
+    const int S = g.nzp;
+    const float a = d_plus<EXACT>(F.txx, i, S);   // Dx+ txx
+    const float b = d_minus<EXACT>(F.txz, i, 1);  // Dz- txz
+    const float c = d_plus<EXACT>(F.tzz, i, 1);   // Dz+ tzz
+    const float e = d_minus<EXACT>(F.txz, i, S);  // Dx- txz
+    if (EXACT) {
+        const Damp d = damp_of(g, m, i);
+        const double r = 1.0 / (double)m.rho[i];
+        const float vxx = (float)(((double)(d.d1 * F.vx_x[i]) + dir * r * (double)g.dtdx * (double)a) / (double)d.d2);
+        const float vxz = (float)(((double)(d.d3 * F.vx_z[i]) + dir * r * (double)g.dtdz * (double)b) / (double)d.d4);
+        const float vzx = (float)(((double)(d.d1 * F.vz_x[i]) + dir * r * (double)g.dtdx * (double)e) / (double)d.d2);
+        const float vzz = (float)(((double)(d.d3 * F.vz_z[i]) + dir * r * (double)g.dtdz * (double)c) / (double)d.d4);
+        F.vx_x[i] = vxx;
+        F.vx_z[i] = vxz;
+        F.vz_x[i] = vzx;
+        F.vz_z[i] = vzz;
+        F.vx[i] = vxx + vxz;
+        F.vz[i] = vzx + vzz;
+    } else if (split) {
+        const Damp d = damp_of(g, m, i);
+        const float bu = (float)dir * m.bu[i];
+        const float bx = bu * g.dtdx, bz = bu * g.dtdz;
+        const float vxx = __builtin_fmaf(d.d1, F.vx_x[i], bx * a) / d.d2;
+        const float vxz = __builtin_fmaf(d.d3, F.vx_z[i], bz * b) / d.d4;
+        const float vzx = __builtin_fmaf(d.d1, F.vz_x[i], bx * e) / d.d2;
+        const float vzz = __builtin_fmaf(d.d3, F.vz_z[i], bz * c) / d.d4;
+        F.vx_x[i] = vxx;
+        F.vx_z[i] = vxz;
+        F.vz_x[i] = vzx;
+        F.vz_z[i] = vzz;
+        F.vx[i] = vxx + vxz;
+        F.vz[i] = vzx + vzz;
+    } else {
+        const float bu = (float)dir * m.bu[i];
+        F.vx[i] = __builtin_fmaf(bu, __builtin_fmaf(g.dtdx, a, g.dtdz * b), F.vx[i]);
+        F.vz[i] = __builtin_fmaf(bu, __builtin_fmaf(g.dtdx, e, g.dtdz * c), F.vz[i]);
+    }
+}
+
+// Velocity derivatives used by the stress update.
+struct VGrad {
+    float xx, zz, xz, zx;   // Dx- vx, Dz- vz, Dz+ vx, Dx+ vz
+};
+template <bool EXACT> __device__ inline VGrad vgrad(const Geo &g, const Fields &F, int i) {
+    VGrad v;
+    v.xx = d_minus<EXACT>(F.vx, i, g.nzp);
+    v.zz = d_minus<EXACT>(F.vz, i, 1);
+    v.xz = d_plus<EXACT>(F.vx, i, 1);
+    v.zx = d_plus<EXACT>(F.vz, i, g.nzp);
+    return v;
+}
+
+// Born source terms of one cell from the background velocity derivatives
+// (sfwi-stage2/born.cu:787-797, add_born_source_components).
+struct BornSrc {
+    float xx_x, xx_z, zz_x, zz_z, xz_x, xz_z;
+};
+template <bool EXACT>
+__device__ inline BornSrc born_src(const Geo &g, const Medium &m, const VGrad &v, int i, int dir) {
+    const float rho = m.rho[i], vp = m.vp[i], vs = m.vs[i], dp = m.dvp[i], ds = m.dvs[i];
+    const float D = (float)dir;
+    BornSrc s;
+    if (EXACT) {
+        s.xx_x = D * g.dt * (2.0f * (rho * vp) * (dp) * (v.xx) / g.dx);
+        s.xx_z = D * g.dt * ((2.0f * (rho * vp) * (dp) - 4.0f * (rho * vs) * (ds)) * (v.zz) / g.dz);
+        s.zz_x = D * g.dt * ((2.0f * (rho * vp) * (dp) - 4.0f * (rho * vs) * (ds)) * (v.xx) / g.dx);
+        s.zz_z = D * g.dt * (2.0f * (rho * vp) * (dp) * (v.zz) / g.dz);
+        s.xz_x = D * g.dt * ((2.0f * (rho * vs) * (ds)) * ((v.zx) / g.dx));
+        s.xz_z = D * g.dt * ((2.0f * (rho * vs) * (ds)) * ((v.xz) / g.dz));
+    } else {
+        const float P = 2.0f * (rho * vp) * dp;
+        const float Sv = 2.0f * (rho * vs) * ds;
+        const float Q = __builtin_fmaf(-2.0f, Sv, P);
+        const float kx = D * g.dt * g.idx, kz = D * g.dt * g.idz;
+        s.xx_x = kx * P * v.xx;
+        s.xx_z = kz * Q * v.zz;
+        s.zz_x = kx * Q * v.xx;
+        s.zz_z = kz * P * v.zz;
+        s.xz_x = kx * Sv * v.zx;
+        s.xz_z = kz * Sv * v.xz;
+    }
+    return s;
+}
+
+// Stress update (sfwi-stage1/born.cu:399-440, update_txx_tzz_txz); with
+// a Born source the terms are added to the split parts first
+// (sfwi-stage2/FD2DGPU.cu:1839-1840 order: source, then update).
+template <bool EXACT, bool HAS_SRC>
+__device__ inline void str_cell(const Geo &g, const Medium &m, const Fields &F, const VGrad &v, int i,
+                                bool split, int dir, const BornSrc &s) {
+    const float lam = m.lam[i], mu = m.mu[i];
+    if (EXACT) {
+        const Damp d = damp_of(g, m, i);
+        const float D = (float)dir;
+        float xx_x = F.txx_x[i], xx_z = F.txx_z[i], zz_x = F.tzz_x[i], zz_z = F.tzz_z[i];
+        float xz_x = F.txz_x[i], xz_z = F.txz_z[i];
+        if (HAS_SRC) {
+            xx_x = xx_x + s.xx_x; xx_z = xx_z + s.xx_z; zz_x = zz_x + s.zz_x;
+            zz_z = zz_z + s.zz_z; xz_x = xz_x + s.xz_x; xz_z = xz_z + s.xz_z;
+        }
+        xx_x = (d.d1 * xx_x + D * (lam + 2.0f * mu) * g.dtdx * v.xx) / d.d2;
+        xx_z = (d.d3 * xx_z + D * lam * g.dtdz * v.zz) / d.d4;
+        zz_x = (d.d1 * zz_x + D * lam * g.dtdx * v.xx) / d.d2;
+        zz_z = (d.d3 * zz_z + D * (lam + 2.0f * mu) * g.dtdz * v.zz) / d.d4;
+        xz_x = (d.d1 * xz_x + D * g.dtdx * mu * v.zx) / d.d2;
+        xz_z = (d.d3 * xz_z + D * g.dtdz * mu * v.xz) / d.d4;
+        F.txx_x[i] = xx_x; F.txx_z[i] = xx_z; F.tzz_x[i] = zz_x;
+        F.tzz_z[i] = zz_z; F.txz_x[i] = xz_x; F.txz_z[i] = xz_z;
+        F.txx[i] = xx_x + xx_z;
+        F.tzz[i] = zz_x + zz_z;
+        F.txz[i] = xz_x + xz_z;
+    } else {
+        const float D = (float)dir;
+        const float l2m = __builtin_fmaf(2.0f, mu, lam);
+        const float ax = D * g.dtdx * v.xx, az = D * g.dtdz * v.zz;
+        const float gx = D * g.dtdx * v.zx, gz = D * g.dtdz * v.xz;
+        if (split) {
+            const Damp d = damp_of(g, m, i);
+            float xx_x = F.txx_x[i], xx_z = F.txx_z[i], zz_x = F.tzz_x[i], zz_z = F.tzz_z[i];
+            float xz_x = F.txz_x[i], xz_z = F.txz_z[i];
+            if (HAS_SRC) {
+                xx_x += s.xx_x; xx_z += s.xx_z; zz_x += s.zz_x;
+                zz_z += s.zz_z; xz_x += s.xz_x; xz_z += s.xz_z;
+            }
+            xx_x = __builtin_fmaf(d.d1, xx_x, l2m * ax) / d.d2;
+            xx_z = __builtin_fmaf(d.d3, xx_z, lam * az) / d.d4;
+            zz_x = __builtin_fmaf(d.d1, zz_x, lam * ax) / d.d2;
+            zz_z = __builtin_fmaf(d.d3, zz_z, l2m * az) / d.d4;
+            xz_x = __builtin_fmaf(d.d1, xz_x, mu * gx) / d.d2;
+            xz_z = __builtin_fmaf(d.d3, xz_z, mu * gz) / d.d4;
+            F.txx_x[i] = xx_x; F.txx_z[i] = xx_z; F.tzz_x[i] = zz_x;
+            F.tzz_z[i] = zz_z; F.txz_x[i] = xz_x; F.txz_z[i] = xz_z;
+            F.txx[i] = xx_x + xx_z;
+            F.tzz[i] = zz_x + zz_z;
+            F.txz[i] = xz_x + xz_z;
+        } else {
+            float sxx = __builtin_fmaf(l2m, ax, lam * az);
+            float szz = __builtin_fmaf(lam, ax, l2m * az);
+            float sxz = mu * (gx + gz);
+            if (HAS_SRC) {
+                sxx += s.xx_x + s.xx_z;
+                szz += s.zz_x + s.zz_z;
+                sxz += s.xz_x + s.xz_z;
+            }
+            F.txx[i] += sxx;
+            F.tzz[i] += szz;
+            F.txz[i] += sxz;
+        }
+    }
+}
+
+// Adjoint stress-like update (sfwi-stage1/born.cu:565-603):
+// txx = txx_x, tzz = tzz_z, txz = txz_x + txz_z.
+template <bool EXACT>
+__device__ inline void adj_str_cell(const Geo &g, const Medium &m, const Fields &F, int i, bool split,
+                                    int dir) {
+    const VGrad v = vgrad<EXACT>(g, F, i);
+    const float D = (float)dir;
+    if (EXACT || split) {
+        const Damp d = damp_of(g, m, i);
+        float xx, zz, xzx, xzz;
+        if (EXACT) {
+            xx = (d.d1 * F.txx_x[i] + D * g.dtdx * v.xx) / d.d2;
+            zz = (d.d3 * F.tzz_z[i] + D * g.dtdz * v.zz) / d.d4;
+            xzx = (d.d1 * F.txz_x[i] + D * g.dtdx * v.zx) / d.d2;
+            xzz = (d.d3 * F.txz_z[i] + D * g.dtdz * v.xz) / d.d4;
+        } else {
+            xx = __builtin_fmaf(d.d1, F.txx_x[i], D * g.dtdx * v.xx) / d.d2;
+            zz = __builtin_fmaf(d.d3, F.tzz_z[i], D * g.dtdz * v.zz) / d.d4;
+            xzx = __builtin_fmaf(d.d1, F.txz_x[i], D * g.dtdx * v.zx) / d.d2;
+            xzz = __builtin_fmaf(d.d3, F.txz_z[i], D * g.dtdz * v.xz) / d.d4;
+        }
+        F.txx_x[i] = xx;
+        F.tzz_z[i] = zz;
+        F.txz_x[i] = xzx;
+        F.txz_z[i] = xzz;
+        F.txx[i] = xx;
+        F.tzz[i] = zz;
+        F.txz[i] = xzx + xzz;
+    } else {
+        F.txx[i] = __builtin_fmaf(D * g.dtdx, v.xx, F.txx[i]);
+        F.tzz[i] = __builtin_fmaf(D * g.dtdz, v.zz, F.tzz[i]);
+        F.txz[i] = __builtin_fmaf(D, __builtin_fmaf(g.dtdx, v.zx, g.dtdz * v.xz), F.txz[i]);
+    }
+}
+
+// Adjoint velocity update with the moduli inside the derivatives
+// (sfwi-stage1/born.cu:504-562).
+template <bool EXACT>
+__device__ inline void adj_vel_cell(const Geo &g, const Medium &m, const Fields &F, int i, bool split,
+                                    int dir) {
+    const int S = g.nzp;
+    const float *__restrict__ lam = m.lam;
+    const float *__restrict__ mu = m.mu;
+    const float *__restrict__ txx = F.txx;
+    const float *__restrict__ tzz = F.tzz;
+    const float *__restrict__ txz = F.txz;
+    if (EXACT) {
+        float s_txx_x = 0.f, s_txz_z = 0.f, s_tzz_x = 0.f, s_tzz_z = 0.f, s_txz_x = 0.f, s_txx_z = 0.f;
+#pragma unroll
+        for (int k = 1; k <= NOP; ++k) {
+            const int pa = i + k * S, pb = i - (k - 1) * S;          // x taps of Dx+
+            const int qa = i + (k - 1) * S, qb = i - k * S;          // x taps of Dx-
+            const int za = i + k, zb = i - (k - 1);                  // z taps of Dz+
+            const int wa = i + (k - 1), wb = i - k;                  // z taps of Dz-
+            s_txx_x += kC[k] * ((lam[pa] + 2.0f * mu[pa]) * txx[pa] - (lam[pb] + 2.0f * mu[pb]) * txx[pb]);
+            s_txz_z += kC[k] * (mu[wa] * txz[wa] - mu[wb] * txz[wb]);
+            s_tzz_x += kC[k] * ((lam[pa]) * tzz[pa] - (lam[pb]) * tzz[pb]);
+            s_tzz_z += kC[k] * ((lam[za] + 2.0f * mu[za]) * tzz[za] - (lam[zb] + 2.0f * mu[zb]) * tzz[zb]);
+            s_txz_x += kC[k] * (mu[qa] * txz[qa] - mu[qb] * txz[qb]);
+            s_txx_z += kC[k] * ((lam[za]) * txx[za] - (lam[zb]) * txx[zb]);
+        }
+        const Damp d = damp_of(g, m, i);
+        const double r = 1.0 / (double)m.rho[i];
+        const double ax = (double)g.dtdx, az = (double)g.dtdz;
+        const float vxx = (float)(((double)(d.d1 * F.vx_x[i]) + dir * r * ax * (double)s_txx_x + dir * r * ax * (double)s_tzz_x) / (double)d.d2);
+        const float vxz = (float)(((double)(d.d3 * F.vx_z[i]) + dir * r * az * (double)s_txz_z) / (double)d.d4);
+        const float vzx = (float)(((double)(d.d1 * F.vz_x[i]) + dir * r * ax * (double)s_txz_x) / (double)d.d2);
+        const float vzz = (float)(((double)(d.d3 * F.vz_z[i]) + dir * r * az * (double)s_tzz_z + dir * r * az * (double)s_txx_z) / (double)d.d4);
+        F.vx_x[i] = vxx;
+        F.vx_z[i] = vxz;
+        F.vz_x[i] = vzx;
+        F.vz_z[i] = vzz;
+        F.vz[i] = vzx + vzz;
+        F.vx[i] = vxx + vxz;
+    } else {
+        // Fast form: Dx+ of (lam+2mu) txx + lam tzz, Dz+ of (lam+2mu) tzz + lam txx.
+        float sx = 0.f, sz = 0.f, sxz_z = 0.f, sxz_x = 0.f;
+#pragma unroll
+        for (int k = 1; k <= NOP; ++k) {
+            const int pa = i + k * S, pb = i - (k - 1) * S;
+            const int qa = i + (k - 1) * S, qb = i - k * S;
+            const int za = i + k, zb = i - (k - 1);
+            const int wa = i + (k - 1), wb = i - k;
+            const float la = lam[pa], lb = lam[pb];
+            const float fa = __builtin_fmaf(__builtin_fmaf(2.0f, mu[pa], la), txx[pa], la * tzz[pa]);
+            const float fb = __builtin_fmaf(__builtin_fmaf(2.0f, mu[pb], lb), txx[pb], lb * tzz[pb]);
+            sx = __builtin_fmaf(kC[k], fa - fb, sx);
+            const float lza = lam[za], lzb = lam[zb];
+            const float ga = __builtin_fmaf(__builtin_fmaf(2.0f, mu[za], lza), tzz[za], lza * txx[za]);
+            const float gb = __builtin_fmaf(__builtin_fmaf(2.0f, mu[zb], lzb), tzz[zb], lzb * txx[zb]);
+            sz = __builtin_fmaf(kC[k], ga - gb, sz);
+            sxz_z = __builtin_fmaf(kC[k], mu[wa] * txz[wa] - mu[wb] * txz[wb], sxz_z);
+            sxz_x = __builtin_fmaf(kC[k], mu[qa] * txz[qa] - mu[qb] * txz[qb], sxz_x);
+        }
+        const float bu = (float)dir * m.bu[i];
+        if (split) {
+            const Damp d = damp_of(g, m, i);
+            const float bx = bu * g.dtdx, bz = bu * g.dtdz;
+            const float vxx = __builtin_fmaf(d.d1, F.vx_x[i], bx * sx) / d.d2;
+            const float vxz = __builtin_fmaf(d.d3, F.vx_z[i], bz * sxz_z) / d.d4;
+            const float vzx = __builtin_fmaf(d.d1, F.vz_x[i], bx * sxz_x) / d.d2;
+            const float vzz = __builtin_fmaf(d.d3, F.vz_z[i], bz * sz) / d.d4;
+            F.vx_x[i] = vxx;
+            F.vx_z[i] = vxz;
+            F.vz_x[i] = vzx;
+            F.vz_z[i] = vzz;
+            F.vz[i] = vzx + vzz;
+            F.vx[i] = vxx + vxz;
+        } else {
+            F.vx[i] = __builtin_fmaf(bu, __builtin_fmaf(g.dtdx, sx, g.dtdz * sxz_z), F.vx[i]);
+            F.vz[i] = __builtin_fmaf(bu, __builtin_fmaf(g.dtdx, sxz_x, g.dtdz * sz), F.vz[i]);
+        }
+    }
+}
+
+// ------------------------------------------------------------ kernels ---
+
+// Strip slots of a padded cell: up, down, left, right (-1 when outside).
+// Layout per field and step: up [nx][2N] | down [nx][2N] | left [2N][nz] | right [2N][nz]
+// (sfwi-stage1/FD2DGPU.cu:3842-3869 index formulas).
+__device__ inline void strip_slots(const Geo &g, int ix, int iz, int s[4]) {
+    const int N = NSTRIP, p = g.pml;
+    s[0] = s[1] = s[2] = s[3] = -1;
+    if (ix >= p && ix < g.nxp - p) {
+        if (iz >= p - N && iz < p + N) s[0] = (ix - p) * 2 * N + (iz - (p - N));
+        if (iz >= g.nzp - p - N && iz < g.nzp - p + N)
+            s[1] = 2 * N * g.nx + (ix - p) * 2 * N + (iz - (g.nzp - p - N));
+    }
+    if (iz >= p && iz < g.nzp - p) {
+        if (ix >= p - N && ix < p + N) s[2] = 4 * N * g.nx + (ix - (p - N)) * g.nz + (iz - p);
+        if (ix >= g.nxp - p - N && ix < g.nxp - p + N)
+            s[3] = 4 * N * g.nx + 2 * N * g.nz + (ix - (g.nxp - p - N)) * g.nz + (iz - p);
+    }
+}
+
+// Explosive source on txx and tzz of every shot (sfwi-stage1/FD2DGPU.cu:2104-2114).
+__global__ void k_source(Geo g, const DevShot *__restrict__ shots, const float *__restrict__ wav,
+                         int it, int nshots) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nshots) return;
+    const DevShot &d = shots[s];
+    const int i = d.isx * g.nzp + d.isz;
+    d.fw.txx[i] += wav[it];
+    d.fw.tzz[i] += wav[it];
+}
+
+// Velocity half step of every shot.  FWD also stores the boundary strips of
+// the new velocity; BORN also advances the scattered field.
+template <bool EXACT, bool SAVE, bool BORN>
+__global__ void __launch_bounds__(BLK) k_vel(Geo g, const DevShot *__restrict__ shots, int it, int dir) {
+    int ix, iz;
+    if (!cell_of(g, ix, iz)) return;
+    const DevShot &d = shots[blockIdx.y];
+    const int i = ix * g.nzp + iz;
+    const bool split = in_band(g, ix, iz);
+    vel_cell<EXACT>(g, d.m, d.fw, i, split, dir);
+    if (BORN) vel_cell<EXACT>(g, d.m, d.bn, i, split, dir);
+    if (SAVE && split) {
+        int sl[4];
+        strip_slots(g, ix, iz, sl);
+        if (sl[0] >= 0 || sl[1] >= 0 || sl[2] >= 0 || sl[3] >= 0) {
+            const float vx = d.fw.vx[i], vz = d.fw.vz[i];
+            float *base = d.strip + (size_t)it * 2 * g.sf;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (sl[k] >= 0) {
+                    base[sl[k]] = vx;
+                    base[g.sf + sl[k]] = vz;
+                }
+        }
+    }
+}
+
+// Stress half step of every shot.  GRAD stores the four velocity derivatives
+// (update_txx_tzz_txz_grad, sfwi-stage1/born.cu:451-499); BORN also advances
+// the scattered stresses with the Born source of the background velocity.
+template <bool EXACT, bool GRAD, bool BORN>
+__global__ void __launch_bounds__(BLK) k_str(Geo g, const DevShot *__restrict__ shots, int dir) {
+    int ix, iz;
+    if (!cell_of(g, ix, iz)) return;
+    const DevShot &d = shots[blockIdx.y];
+    const int i = ix * g.nzp + iz;
+    const bool split = in_band(g, ix, iz) || (ix == d.isx && iz == d.isz);
+    const VGrad v = vgrad<EXACT>(g, d.fw, i);
+    BornSrc none{};
+    str_cell<EXACT, false>(g, d.m, d.fw, v, i, split, dir, none);
+    if (GRAD) {
+        if (EXACT) {
+            d.dgx[i] = v.xx / g.dx;
+            d.dgz[i] = v.zz / g.dz;
+            d.dgxz[i] = v.xz / g.dz;
+            d.dgzx[i] = v.zx / g.dx;
+        } else {
+            d.dgx[i] = v.xx * g.idx;
+            d.dgz[i] = v.zz * g.idz;
+            d.dgxz[i] = v.xz * g.idz;
+            d.dgzx[i] = v.zx * g.idx;
+        }
+    }
+    if (BORN) {
+        const BornSrc s = born_src<EXACT>(g, d.m, v, i, dir);
+        const VGrad vb = vgrad<EXACT>(g, d.bn, i);
+        str_cell<EXACT, true>(g, d.m, d.bn, vb, i, in_band(g, ix, iz), dir, s);
+    }
+}
+
+// Restore the boundary strips of step `it` into vx, vz
+// (sfwi-stage1/FD2DGPU.cu:3873-3899).
+__global__ void k_strip_load(Geo g, const DevShot *__restrict__ shots, int it) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.sf) return;
+    const DevShot &d = shots[blockIdx.y];
+    const int N = NSTRIP, p = g.pml;
+    int ix, iz;
+    if (t < 2 * N * g.nx) {
+        ix = p + t / (2 * N);
+        iz = p - N + t % (2 * N);
+    } else if (t < 4 * N * g.nx) {
+        const int u = t - 2 * N * g.nx;
+        ix = p + u / (2 * N);
+        iz = g.nzp - p - N + u % (2 * N);
+    } else if (t < 4 * N * g.nx + 2 * N * g.nz) {
+        const int u = t - 4 * N * g.nx;
+        ix = p - N + u / g.nz;
+        iz = p + u % g.nz;
+    } else {
+        const int u = t - 4 * N * g.nx - 2 * N * g.nz;
+        ix = g.nxp - p - N + u / g.nz;
+        iz = p + u % g.nz;
+    }
+    const int i = ix * g.nzp + iz;
+    const float *base = d.strip + (size_t)it * 2 * g.sf;
+    d.fw.vx[i] = base[t];
+    d.fw.vz[i] = base[g.sf + t];
+}
+
+// Receiver sampling (sfwi-stage1/FD2DGPU.cu:3158-3166): vz, vx of the
+// forward field into rz, rx; with BORN the scattered field goes to rz, rx and
+// the background to r2z, r2x (record_scatter, sfwi-stage2/FD2DGPU.cu:3511-3519).
+template <bool BORN>
+__global__ void k_record(Geo g, const DevShot *__restrict__ shots, int it) {
+    const int tr = blockIdx.x * blockDim.x + threadIdx.x;
+    const DevShot &d = shots[blockIdx.y];
+    if (tr >= d.ntr) return;
+    const int i = d.igx[tr] * g.nzp + d.igz;
+    const size_t o = (size_t)tr * g.nt + it;
+    if (BORN) {
+        d.rz[o] = d.bn.vz[i];
+        d.rx[o] = d.bn.vx[i];
+        d.r2z[o] = d.fw.vz[i];
+        d.r2x[o] = d.fw.vx[i];
+    } else {
+        d.rz[o] = d.fw.vz[i];
+        d.rx[o] = d.fw.vx[i];
+    }
+}
+
+// Residual in place of the synthetic traces: stage 1 r = 1*syn - obs
+// (sfwi-stage1/FD2DGPU.cu:3181-3195); stage 2 r = (b - obs) then 1*r + bg
+// (sfwi-stage2/FD2DGPU.cu:3119-3158).
+template <bool BORN>
+__global__ void k_residual(Geo g, const DevShot *__restrict__ shots) {
+    const DevShot &d = shots[blockIdx.y];
+    const size_t n = (size_t)d.ntr * g.nt;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+        if (BORN) {
+            float z = d.rz[k] - d.oz[k], x = d.rx[k] - d.ox[k];
+            d.rz[k] = 1.0f * z + d.r2z[k];
+            d.rx[k] = 1.0f * x + d.r2x[k];
+        } else {
+            d.rz[k] = 1.0f * d.rz[k] - d.oz[k];
+            d.rx[k] = 1.0f * d.rx[k] - d.ox[k];
+        }
+    }
+}
+
+// Per-trace misfit partial sums: each term formed in FP32 and accumulated in
+// FP64 along the trace (sfwi-stage1/FD2DGPU.cu:4587-4590).
+__global__ void k_trace_l2(Geo g, const DevShot *__restrict__ shots) {
+    const int tr = blockIdx.x * blockDim.x + threadIdx.x;
+    const DevShot &d = shots[blockIdx.y];
+    if (tr >= d.ntr) return;
+    const float *z = d.rz + (size_t)tr * g.nt, *x = d.rx + (size_t)tr * g.nt;
+    double s = 0.0;
+    for (int j = 0; j < g.nt; ++j) s += (double)(z[j] * z[j] + x[j] * x[j]);
+    d.trace_sum[tr] = s;
+}
+// Shot misfit = ordered sum of the trace sums.
+__global__ void k_shot_l2(const DevShot *__restrict__ shots, double *out, int nshots) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nshots) return;
+    const DevShot &d = shots[s];
+    double m = 0.0;
+    for (int tr = 0; tr < d.ntr; ++tr) m += d.trace_sum[tr];
+    out[s] = m;
+}
+
+// Adjoint source: the residual overwrites the adjoint velocity at the
+// receivers (sfwi-stage1/FD2DGPU.cu:3425-3441, stack = false).  `inj` lists
+// the traces to inject, one per receiver cell (the last trace of coincident
+// receivers, the sequential order's winner), so threads never collide.
+__global__ void k_inject(Geo g, const DevShot *__restrict__ shots, int it) {
+    const DevShot &d = shots[blockIdx.y];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d.ninj) return;
+    const int tr = d.inj[k];
+    const int i = d.igx[tr] * g.nzp + d.igz;
+    d.bk.vz[i] = d.rz[(size_t)tr * g.nt + it];
+    d.bk.vx[i] = d.rx[(size_t)tr * g.nt + it];
+}
+
+template <bool EXACT>
+__global__ void __launch_bounds__(BLK) k_adj_str(Geo g, const DevShot *__restrict__ shots, int dir) {
+    int ix, iz;
+    if (!cell_of(g, ix, iz)) return;
+    const DevShot &d = shots[blockIdx.y];
+    const int i = ix * g.nzp + iz;
+    adj_str_cell<EXACT>(g, d.m, d.bk, i, in_band(g, ix, iz), dir);
+}
+
+// Adjoint velocity step fused with the zero-lag imaging of this reverse step
+// (sfwi-stage1/FD2DGPU.cu:3350-3382 doimage_gradient, :3324-3331 doimage_illum).
+template <bool EXACT>
+__global__ void __launch_bounds__(BLK) k_adj_vel_image(Geo g, const DevShot *__restrict__ shots, int dir) {
+    int ix, iz;
+    if (!cell_of(g, ix, iz)) return;
+    const DevShot &d = shots[blockIdx.y];
+    const int i = ix * g.nzp + iz;
+    const bool split = in_band(g, ix, iz) || iz == d.igz;
+    adj_vel_cell<EXACT>(g, d.m, d.bk, i, split, dir);
+    const int jx = ix - g.pml, jz = iz - g.pml;
+    if (jx < 0 || jx >= g.nx || jz < 0 || jz >= g.nz) return;
+    const int q = jx * g.nz + jz;
+    const float gx = d.dgx[i], gz = d.dgz[i], gxz = d.dgxz[i], gzx = d.dgzx[i];
+    const float sxx = d.bk.txx[i], szz = d.bk.tzz[i], sxz = d.bk.txz[i];
+    const float vx = d.fw.vx[i], vz = d.fw.vz[i];
+    if (EXACT) {
+        const float lg = -((gx + gz) * (sxx + szz));
+        const float mg = -2.0f * (gx * sxx + gz * szz) - (gxz + gzx) * sxz;
+        const float vp = d.m.vp[i], vs = d.m.vs[i], rho = d.m.rho[i];
+        const float pp = 2.0f * vp * rho * lg;
+        const float ps = -4.0f * vs * rho * lg + 2.0f * vs * rho * mg;
+        d.gvp[q] += 1.0f * pp;
+        d.gvs[q] += 1.0f * ps;
+        d.gil[q] += 1.0f * (vx * vx + vz * vz);
+    } else {
+        const float lg = -((gx + gz) * (sxx + szz));
+        const float mg = __builtin_fmaf(-(gxz + gzx), sxz, -2.0f * __builtin_fmaf(gx, sxx, gz * szz));
+        d.gvp[q] = __builtin_fmaf(d.m.icp[i], lg, d.gvp[q]);
+        d.gvs[q] = __builtin_fmaf(d.m.ics[i], __builtin_fmaf(-2.0f, lg, mg), d.gvs[q]);
+        d.gil[q] = __builtin_fmaf(vx, vx, __builtin_fmaf(vz, vz, d.gil[q]));
+    }
+}
+
+// Per-shot accumulators added into the full-model images, shots in batch
+// order so the result does not depend on scheduling.
+__global__ void k_merge(Geo g, const DevShot *__restrict__ shots, int nshots, float *gvp, float *gvs,
+                        float *gil) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.nx * g.nz) return;
+    const int jx = q / g.nz, jz = q - jx * g.nz;
+    for (int s = 0; s < nshots; ++s) {
+        const DevShot &d = shots[s];
+        const size_t o = (size_t)(d.left + jx) * g.allnz + jz;
+        gvp[o] += d.gvp[q];
+        gvs[o] += d.gvs[q];
+        gil[o] += d.gil[q];
+    }
+}
+
+// ------------------------------------------------------------- set-up ---
+
+// Window of a full-model array into the padded grid with edge clamping; one
+// pass equals the reference's copy, z-pad, x-pad sequence
+// (sfwi-stage1/FD2DGPU.cu:112-128, 233-249).
+__global__ void k_window(Geo g, const float *__restrict__ all, int left, float *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.nxp * g.nzp) return;
+    const int ix = i / g.nzp, iz = i - ix * g.nzp;
+    const int sx = min(max(ix, g.pml), g.nxp - g.pml - 1) - g.pml;
+    const int sz = min(max(iz, g.pml), g.nzp - g.pml - 1) - g.pml;
+    out[i] = all[(size_t)(left + sx) * g.allnz + sz];
+}
+
+// PML profile (sfwi-stage1/FD2DGPU.cu:443-493) and Lame parameters
+// (sfwi-stage1/FD2DGPU.cu:1116-1124), plus the fast-mode coefficients.
+// `lg` is log10(1/rr) rounded to float.
+__global__ void k_medium(Geo g, float lg, float rho_const, float *vp, float *vs, float *rho, float *lam,
+                         float *mu, float *dampx, float *dampz, float *bu, float *icp, float *ics) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.nxp * g.nzp) return;
+    const int ix = i / g.nzp, iz = i - ix * g.nzp;
+    if (rho_const > 0.0f) rho[i] = rho_const;
+    const float p = vp[i], s = vs[i], r = rho[i];
+    int dxl = -1, dzl = -1;
+    if (ix < g.pml) dxl = g.pml - ix;
+    if (ix > g.nxp - g.pml) dxl = ix - (g.nxp - g.pml);
+    if (iz < g.pml) dzl = g.pml - iz;
+    if (iz > g.nzp - g.pml) dzl = iz - (g.nzp - g.pml);
+    float ax = 0.0f, az = 0.0f;
+    if (dxl >= 0) {
+        const float t = (float)(1.0 * dxl / g.pml);
+        const double t2 = (double)t * (double)t;
+        ax = (float)((double)lg * (5.0 * (double)p / (2.0 * g.pml)) * (double)(float)(t2 * t2));
+    }
+    if (dzl >= 0) {
+        const float t = (float)(1.0 * dzl / g.pml);
+        const double t2 = (double)t * (double)t;
+        az = (float)((double)lg * (5.0 * (double)p / (2.0 * g.pml)) * (double)(float)(t2 * t2));
+    }
+    dampx[i] = ax;
+    dampz[i] = az;
+    lam[i] = r * (p * p - 2.0f * s * s);
+    mu[i] = r * s * s;
+    bu[i] = 1.0f / r;
+    icp[i] = 2.0f * p * r;
+    ics[i] = 2.0f * s * r;
+}
+
+// Test model of the stage-1 line search: v - step*d (host order,
+// sfwi-stage1/FD2DGPU.cu:4902-4907).
+__global__ void k_axpy_model(int n, const float *__restrict__ v, const float *__restrict__ d, float step,
+                             float *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = v[i] - step * d[i];
+}
+
+// Stage-2 step sums: fenzi += sum L.r, fenmu += sum L^2, per trace in FP64
+// along the trace, then in trace order (sfwi-stage2/FD2DGPU.cu:4707-4712).
+__global__ void k_trace_dot(Geo g, int ntr, const float *lz, const float *lx, const float *rz,
+                            const float *rx, double *num, double *den) {
+    const int tr = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tr >= ntr) return;
+    double a = 0.0, b = 0.0;
+    const size_t o = (size_t)tr * g.nt;
+    for (int j = 0; j < g.nt; ++j) {
+        const float z = lz[o + j], x = lx[o + j];
+        b += (double)(z * z + x * x);
+        a += (double)(z * rz[o + j] + x * rx[o + j]);
+    }
+    num[tr] = a;
+    den[tr] = b;
+}
+
+}  // namespace sfwi

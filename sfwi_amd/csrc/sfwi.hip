@@ -1,0 +1,701 @@
+// Host side of libsfwi_amd: context, device memory, batch orchestration and
+// the C ABI declared in include/sfwi_abi.h.
+//
+// One context = one GPU, one HIP stream.  A solve runs a batch of up to
+// max_batch shots: each shot owns a slot (its wavefields, records, strips and
+// gradient accumulators in HBM) and every stencil launch advances all shots of
+// the batch by one half step.  Padded medium windows are built on the device
+// once per window position and shared by every shot that uses it.
+#include "sfwi_abi.h"
+#include "kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+using namespace sfwi;
+
+namespace {
+
+std::string g_create_error;
+
+struct Slot {
+    char *block = nullptr;      // fw(15) + bk(15) + derivatives(4) fields
+    char *born = nullptr;       // bn(15) fields
+    char *rec = nullptr;        // rz rx r2z r2x oz ox, ints, accumulators, trace sums
+    float *strip = nullptr;     // [nt][2][sf]
+    Fields fw{}, bk{}, bn{};
+    float *dg[4]{};
+    float *rz, *rx, *r2z, *r2x, *oz, *ox;
+    int *igx, *inj;
+    float *gvp, *gvs, *gil;
+    double *trace_sum;
+    std::vector<int> h_igx, h_inj;   // host copies kept alive for async uploads
+};
+
+struct Window {
+    float *base = nullptr;
+    Medium m{};
+    bool has_pert = false;
+};
+
+enum Kind { K_FWD, K_FWD_MISFIT, K_GRAD, K_BORN_FWD, K_BORN_GRAD };
+
+}  // namespace
+
+struct sfwi_ctx {
+    sfwi_params p{};
+    Geo g{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    size_t ncell = 0, nmodel = 0, nrec = 0, nint = 0;
+    float *wav = nullptr;
+    float *mvp = nullptr, *mvs = nullptr, *mrho = nullptr, *mdvp = nullptr, *mdvs = nullptr;
+    float *tmp_model = nullptr;   // 4 full-model scratch arrays (line-search models)
+    float *Gvp = nullptr, *Gvs = nullptr, *Gil = nullptr;
+    float *res_keep = nullptr;    // 2 record arrays: residual kept across two Born solves
+    std::vector<Slot> slots;
+    DevShot *d_shots = nullptr;
+    std::vector<DevShot> h_shots;
+    std::map<int, Window> windows;
+    double *d_out = nullptr;      // per-shot misfits [max_batch] + 2 scalars
+    double *d_num = nullptr, *d_den = nullptr;
+    bool have_model = false, have_wav = false, have_pert = false;
+    // timing
+    int timing_every = 0;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> ev_kid;
+    size_t ev_used = 0;
+    double kt_ms[SFWI_K_COUNT] = {};
+    int kt_n[SFWI_K_COUNT] = {};
+};
+
+namespace {
+
+int fail(sfwi_ctx *c, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf; else g_create_error = buf;
+    return code;
+}
+
+#define HIPCK(c, expr)                                                                      \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) return fail(c, SFWI_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+int dalloc(sfwi_ctx *c, void **p, size_t bytes) {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess && bytes + (64u << 20) > fr)
+        return fail(c, SFWI_ENOMEM, "device allocation of %zu MiB exceeds free HBM (%zu MiB free)",
+                    bytes >> 20, fr >> 20);
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return fail(c, SFWI_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    return 0;
+}
+
+// ------------------------------------------------------------- timing ---
+
+int tbeg(sfwi_ctx *c, int kid, int it) {
+    if (c->timing_every <= 0 || (it % c->timing_every) != 0) return -1;
+    if (c->ev_used + 2 > c->ev.size()) {
+        for (int k = 0; k < 64; ++k) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return -1;
+            c->ev.push_back(e);
+            c->ev_kid.push_back(0);
+        }
+    }
+    const int idx = (int)c->ev_used;
+    c->ev_used += 2;
+    c->ev_kid[idx] = kid;
+    hipEventRecord(c->ev[idx], c->stream);
+    return idx;
+}
+void tend(sfwi_ctx *c, int idx) {
+    if (idx >= 0) hipEventRecord(c->ev[idx + 1], c->stream);
+}
+void tcollect(sfwi_ctx *c) {
+    for (size_t k = 0; k + 1 < c->ev_used; k += 2) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]) == hipSuccess) {
+            c->kt_ms[c->ev_kid[k]] += ms;
+            c->kt_n[c->ev_kid[k]] += 1;
+        }
+    }
+    c->ev_used = 0;
+}
+
+// ------------------------------------------------------------ windows ---
+
+int build_window(sfwi_ctx *c, Window &w, int left, const float *vp, const float *vs, const float *rho,
+                 const float *dvp, const float *dvs) {
+    const size_t n = c->ncell;
+    if (!w.base) {
+        int r = dalloc(c, (void **)&w.base, 12 * n * sizeof(float));
+        if (r) return r;
+    }
+    float *a[12];
+    for (int k = 0; k < 12; ++k) a[k] = w.base + k * n;
+    const dim3 blk(256), grd((unsigned)((n + 255) / 256));
+    hipLaunchKernelGGL(k_window, grd, blk, 0, c->stream, c->g, vp, left, a[0]);
+    hipLaunchKernelGGL(k_window, grd, blk, 0, c->stream, c->g, vs, left, a[1]);
+    hipLaunchKernelGGL(k_window, grd, blk, 0, c->stream, c->g, rho, left, a[2]);
+    const float lg = (float)std::log10((double)(1.0f / c->p.rr));
+    hipLaunchKernelGGL(k_medium, grd, blk, 0, c->stream, c->g, lg, c->p.rho_const, a[0], a[1], a[2], a[3],
+                       a[4], a[5], a[6], a[7], a[8], a[9]);
+    w.has_pert = dvp && dvs;
+    if (w.has_pert) {
+        hipLaunchKernelGGL(k_window, grd, blk, 0, c->stream, c->g, dvp, left, a[10]);
+        hipLaunchKernelGGL(k_window, grd, blk, 0, c->stream, c->g, dvs, left, a[11]);
+    }
+    HIPCK(c, hipGetLastError());
+    w.m.vp = a[0]; w.m.vs = a[1]; w.m.rho = a[2]; w.m.lam = a[3]; w.m.mu = a[4];
+    w.m.dampx = a[5]; w.m.dampz = a[6]; w.m.bu = a[7]; w.m.icp = a[8]; w.m.ics = a[9];
+    w.m.dvp = a[10]; w.m.dvs = a[11];
+    return 0;
+}
+
+void drop_windows(sfwi_ctx *c) {
+    for (auto &kv : c->windows) hipFree(kv.second.base);
+    c->windows.clear();
+}
+
+int model_window(sfwi_ctx *c, int left, bool born, const Medium **out) {
+    auto it = c->windows.find(left);
+    if (it != c->windows.end() && (!born || it->second.has_pert)) {
+        *out = &it->second.m;
+        return 0;
+    }
+    if (c->windows.size() >= 256 && it == c->windows.end()) drop_windows(c);
+    Window &w = c->windows[left];
+    int r = build_window(c, w, left, c->mvp, c->mvs, c->mrho, c->have_pert ? c->mdvp : nullptr,
+                         c->have_pert ? c->mdvs : nullptr);
+    if (r) return r;
+    *out = &w.m;
+    return 0;
+}
+
+// -------------------------------------------------------------- slots ---
+
+int ensure_slots(sfwi_ctx *c, int n, bool born, bool strips) {
+    const size_t cell = c->ncell * sizeof(float);
+    while ((int)c->slots.size() < n) c->slots.emplace_back();
+    for (int s = 0; s < n; ++s) {
+        Slot &S = c->slots[s];
+        if (!S.block) {
+            int r = dalloc(c, (void **)&S.block, 34 * cell);
+            if (r) return r;
+            float *f = (float *)S.block;
+            float **fw[15] = {&S.fw.vx, &S.fw.vz, &S.fw.txx, &S.fw.tzz, &S.fw.txz, &S.fw.vx_x, &S.fw.vx_z,
+                              &S.fw.vz_x, &S.fw.vz_z, &S.fw.txx_x, &S.fw.txx_z, &S.fw.tzz_x, &S.fw.tzz_z,
+                              &S.fw.txz_x, &S.fw.txz_z};
+            float **bk[15] = {&S.bk.vx, &S.bk.vz, &S.bk.txx, &S.bk.tzz, &S.bk.txz, &S.bk.vx_x, &S.bk.vx_z,
+                              &S.bk.vz_x, &S.bk.vz_z, &S.bk.txx_x, &S.bk.txx_z, &S.bk.tzz_x, &S.bk.tzz_z,
+                              &S.bk.txz_x, &S.bk.txz_z};
+            for (int k = 0; k < 15; ++k) *fw[k] = f + k * c->ncell;
+            for (int k = 0; k < 15; ++k) *bk[k] = f + (15 + k) * c->ncell;
+            for (int k = 0; k < 4; ++k) S.dg[k] = f + (30 + k) * c->ncell;
+            // records, receiver tables, accumulators, per-trace sums
+            const size_t rbytes = 6 * c->nrec * sizeof(float) + 2 * (size_t)c->p.ntr * sizeof(int) +
+                                  3 * c->nint * sizeof(float) + (size_t)c->p.ntr * sizeof(double) + 64;
+            r = dalloc(c, (void **)&S.rec, rbytes);
+            if (r) return r;
+            char *q = S.rec;
+            S.trace_sum = (double *)q; q += (size_t)c->p.ntr * sizeof(double);
+            float *rr = (float *)q;
+            S.rz = rr; S.rx = rr + c->nrec; S.r2z = rr + 2 * c->nrec; S.r2x = rr + 3 * c->nrec;
+            S.oz = rr + 4 * c->nrec; S.ox = rr + 5 * c->nrec;
+            float *acc = rr + 6 * c->nrec;
+            S.gvp = acc; S.gvs = acc + c->nint; S.gil = acc + 2 * c->nint;
+            S.igx = (int *)(acc + 3 * c->nint);
+            S.inj = S.igx + c->p.ntr;
+        }
+        if (born && !S.born) {
+            int r = dalloc(c, (void **)&S.born, 15 * cell);
+            if (r) return r;
+            float *f = (float *)S.born;
+            float **bn[15] = {&S.bn.vx, &S.bn.vz, &S.bn.txx, &S.bn.tzz, &S.bn.txz, &S.bn.vx_x, &S.bn.vx_z,
+                              &S.bn.vz_x, &S.bn.vz_z, &S.bn.txx_x, &S.bn.txx_z, &S.bn.tzz_x, &S.bn.tzz_z,
+                              &S.bn.txz_x, &S.bn.txz_z};
+            for (int k = 0; k < 15; ++k) *bn[k] = f + k * c->ncell;
+        }
+        if (strips && !S.strip) {
+            int r = dalloc(c, (void **)&S.strip, (size_t)c->p.nt * 2 * c->g.sf * sizeof(float));
+            if (r) return r;
+        }
+    }
+    return 0;
+}
+
+int check_shot(sfwi_ctx *c, const sfwi_shot &s, int k) {
+    const Geo &g = c->g;
+    if (s.ntr < 1 || s.ntr > c->p.ntr) return fail(c, SFWI_EINVAL, "shot %d: ntr=%d outside [1,%d]", k, s.ntr, c->p.ntr);
+    if (!s.gc) return fail(c, SFWI_EINVAL, "shot %d: gc is NULL", k);
+    if (s.left < 0 || s.left + g.nx > c->p.allnx)
+        return fail(c, SFWI_EINVAL, "shot %d: window [%d,%d) outside the model (allnx=%d)", k, s.left,
+                    s.left + g.nx, c->p.allnx);
+    if (s.isx < 0 || s.isx >= g.nxp || s.isz < 0 || s.isz >= g.nzp)
+        return fail(c, SFWI_EINVAL, "shot %d: source (%d,%d) outside the padded grid", k, s.isx, s.isz);
+    if (s.igz < 0 || s.igz >= g.nzp) return fail(c, SFWI_EINVAL, "shot %d: receiver row %d outside", k, s.igz);
+    return 0;
+}
+
+// Receiver columns (sfwi-stage1/FD2DGPU.cu:3162) and the de-duplicated list
+// of injected traces (last trace per cell, the sequential order's winner).
+int receiver_tables(sfwi_ctx *c, const sfwi_shot &s, int k, Slot &S) {
+    S.h_igx.resize(s.ntr);
+    for (int tr = 0; tr < s.ntr; ++tr) {
+        const int x = c->g.pml + (int)((float)(s.gc[tr] - s.ngx_left) / c->p.dx);
+        if (x < 0 || x >= c->g.nxp)
+            return fail(c, SFWI_EINVAL, "shot %d trace %d: receiver column %d outside the padded grid", k, tr, x);
+        S.h_igx[tr] = x;
+    }
+    std::vector<int> last(c->g.nxp, -1);
+    for (int tr = 0; tr < s.ntr; ++tr) last[S.h_igx[tr]] = tr;
+    S.h_inj.clear();
+    for (int tr = 0; tr < s.ntr; ++tr)
+        if (last[S.h_igx[tr]] == tr) S.h_inj.push_back(tr);
+    return 0;
+}
+
+// Prepare n shots in slots 0..n-1: medium windows, tables, records, zeroed state.
+int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium *override_m) {
+    const bool born = kind == K_BORN_FWD || kind == K_BORN_GRAD;
+    const bool grad = kind == K_GRAD || kind == K_BORN_GRAD;
+    const bool obs = grad || kind == K_FWD_MISFIT;
+    if (!c->have_model) return fail(c, SFWI_ESTATE, "model not set (sfwi_set_model)");
+    if (!c->have_wav) return fail(c, SFWI_ESTATE, "wavelet not set (sfwi_set_wavelet)");
+    if (born && !override_m && !c->have_pert) return fail(c, SFWI_ESTATE, "perturbation not set (sfwi_set_perturbation)");
+    if (n < 1 || n > c->p.max_batch) return fail(c, SFWI_EINVAL, "batch of %d shots outside [1,%d]", n, c->p.max_batch);
+    int r = ensure_slots(c, n, born, grad);
+    if (r) return r;
+    c->h_shots.resize(n);
+    for (int k = 0; k < n; ++k) {
+        const sfwi_shot &s = shots[k];
+        if ((r = check_shot(c, s, k))) return r;
+        if (obs && (!s.obs_z || !s.obs_x)) return fail(c, SFWI_EINVAL, "shot %d: observed traces missing", k);
+        Slot &S = c->slots[k];
+        if ((r = receiver_tables(c, s, k, S))) return r;
+        const Medium *m = override_m;
+        if (!m && (r = model_window(c, s.left, born, &m))) return r;
+        DevShot d{};
+        d.m = *m;
+        d.fw = S.fw; d.bk = S.bk; d.bn = S.bn;
+        d.dgx = S.dg[0]; d.dgz = S.dg[1]; d.dgxz = S.dg[2]; d.dgzx = S.dg[3];
+        d.rz = S.rz; d.rx = S.rx; d.r2z = S.r2z; d.r2x = S.r2x; d.oz = S.oz; d.ox = S.ox;
+        d.igx = S.igx; d.inj = S.inj; d.strip = S.strip;
+        d.gvp = S.gvp; d.gvs = S.gvs; d.gil = S.gil; d.trace_sum = S.trace_sum;
+        d.isx = s.isx; d.isz = s.isz; d.igz = s.igz; d.ntr = s.ntr; d.ninj = (int)S.h_inj.size();
+        d.left = s.left;
+        c->h_shots[k] = d;
+        const size_t cell = c->ncell * sizeof(float);
+        HIPCK(c, hipMemsetAsync(S.block, 0, (grad ? 34 : 15) * cell, c->stream));
+        if (born) HIPCK(c, hipMemsetAsync(S.born, 0, 15 * cell, c->stream));
+        HIPCK(c, hipMemcpyAsync(S.igx, S.h_igx.data(), s.ntr * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        HIPCK(c, hipMemcpyAsync(S.inj, S.h_inj.data(), S.h_inj.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        if (obs) {
+            const size_t rb = (size_t)s.ntr * c->p.nt * sizeof(float);
+            HIPCK(c, hipMemcpyAsync(S.oz, s.obs_z, rb, hipMemcpyHostToDevice, c->stream));
+            HIPCK(c, hipMemcpyAsync(S.ox, s.obs_x, rb, hipMemcpyHostToDevice, c->stream));
+        }
+        if (grad) HIPCK(c, hipMemsetAsync(S.gvp, 0, 3 * c->nint * sizeof(float), c->stream));
+    }
+    HIPCK(c, hipMemcpyAsync(c->d_shots, c->h_shots.data(), n * sizeof(DevShot), hipMemcpyHostToDevice, c->stream));
+    return 0;
+}
+
+// ------------------------------------------------------------- solves ---
+
+template <bool E>
+int run_batch(sfwi_ctx *c, Kind kind, int n) {
+    const Geo &g = c->g;
+    const int nt = g.nt;
+    const bool born = kind == K_BORN_FWD || kind == K_BORN_GRAD;
+    const bool grad = kind == K_GRAD || kind == K_BORN_GRAD;
+    const dim3 blk(BLK);
+    const dim3 cells((unsigned)(8 * g.per_xcd), (unsigned)n);
+    const dim3 src1((unsigned)((n + 63) / 64));
+    const dim3 trg((unsigned)((c->p.ntr + 63) / 64), (unsigned)n);
+    hipStream_t st = c->stream;
+    const DevShot *D = c->d_shots;
+
+    for (int it = 0; it < nt; ++it) {
+        hipLaunchKernelGGL(k_source, src1, dim3(64), 0, st, g, D, c->wav, it, n);
+        int t = tbeg(c, SFWI_K_FWD_VEL, it);
+        if (born) {
+            if (grad) hipLaunchKernelGGL((k_vel<E, true, true>), cells, blk, 0, st, g, D, it, 1);
+            else hipLaunchKernelGGL((k_vel<E, false, true>), cells, blk, 0, st, g, D, it, 1);
+        } else {
+            if (grad) hipLaunchKernelGGL((k_vel<E, true, false>), cells, blk, 0, st, g, D, it, 1);
+            else hipLaunchKernelGGL((k_vel<E, false, false>), cells, blk, 0, st, g, D, it, 1);
+        }
+        tend(c, t);
+        t = tbeg(c, SFWI_K_FWD_STR, it);
+        if (born) hipLaunchKernelGGL((k_str<E, false, true>), cells, blk, 0, st, g, D, 1);
+        else hipLaunchKernelGGL((k_str<E, false, false>), cells, blk, 0, st, g, D, 1);
+        tend(c, t);
+        if (born) hipLaunchKernelGGL(k_record<true>, trg, dim3(64), 0, st, g, D, it);
+        else hipLaunchKernelGGL(k_record<false>, trg, dim3(64), 0, st, g, D, it);
+    }
+    HIPCK(c, hipGetLastError());
+    if (kind == K_FWD || kind == K_BORN_FWD) return 0;
+
+    const dim3 resg(256, (unsigned)n);
+    if (born) hipLaunchKernelGGL(k_residual<true>, resg, blk, 0, st, g, D);
+    else hipLaunchKernelGGL(k_residual<false>, resg, blk, 0, st, g, D);
+    hipLaunchKernelGGL(k_trace_l2, trg, dim3(64), 0, st, g, D);
+    hipLaunchKernelGGL(k_shot_l2, src1, dim3(64), 0, st, D, c->d_out, n);
+    HIPCK(c, hipGetLastError());
+    if (!grad) return 0;
+
+    const int adir = c->p.adjoint_dir;
+    const dim3 stripg((unsigned)((g.sf + 255) / 256), (unsigned)n);
+    for (int it = nt - 1; it >= 0; --it) {
+        hipLaunchKernelGGL(k_strip_load, stripg, blk, 0, st, g, D, it);
+        int t = tbeg(c, SFWI_K_REC_STR, it);
+        hipLaunchKernelGGL((k_str<E, true, false>), cells, blk, 0, st, g, D, -1);
+        tend(c, t);
+        t = tbeg(c, SFWI_K_REC_VEL, it);
+        hipLaunchKernelGGL((k_vel<E, false, false>), cells, blk, 0, st, g, D, it, -1);
+        tend(c, t);
+        hipLaunchKernelGGL(k_inject, trg, dim3(64), 0, st, g, D, it);
+        t = tbeg(c, SFWI_K_ADJ_STR, it);
+        hipLaunchKernelGGL(k_adj_str<E>, cells, blk, 0, st, g, D, adir);
+        tend(c, t);
+        t = tbeg(c, SFWI_K_ADJ_VEL, it);
+        hipLaunchKernelGGL(k_adj_vel_image<E>, cells, blk, 0, st, g, D, adir);
+        tend(c, t);
+    }
+    const dim3 mg((unsigned)((c->nint + 255) / 256));
+    hipLaunchKernelGGL(k_merge, mg, blk, 0, st, g, D, n, c->Gvp, c->Gvs, c->Gil);
+    HIPCK(c, hipGetLastError());
+    return 0;
+}
+
+int run(sfwi_ctx *c, Kind kind, int n) {
+    return c->p.mode == SFWI_MODE_EXACT ? run_batch<true>(c, kind, n) : run_batch<false>(c, kind, n);
+}
+
+int finish(sfwi_ctx *c) {
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    tcollect(c);
+    return 0;
+}
+
+// Batched solve with misfit output; shots processed in groups of max_batch.
+int solve_misfits(sfwi_ctx *c, Kind kind, int nshots, const sfwi_shot *shots, double *misfits) {
+    if (!shots || nshots < 0) return fail(c, SFWI_EINVAL, "bad shot list");
+    for (int b0 = 0; b0 < nshots; b0 += c->p.max_batch) {
+        const int n = std::min(c->p.max_batch, nshots - b0);
+        int r = prepare(c, kind, n, shots + b0, nullptr);
+        if (r) return r;
+        if ((r = run(c, kind, n))) return r;
+        std::vector<double> m(n);
+        HIPCK(c, hipMemcpyAsync(m.data(), c->d_out, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        if ((r = finish(c))) return r;
+        if (misfits)
+            for (int k = 0; k < n; ++k) misfits[b0 + k] += m[k];
+    }
+    return 0;
+}
+
+int copy_records(sfwi_ctx *c, const float *src, float *dst, int ntr) {
+    if (dst)
+        HIPCK(c, hipMemcpyAsync(dst, src, (size_t)ntr * c->p.nt * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    return 0;
+}
+
+}  // namespace
+
+// ================================================================ C ABI ===
+
+extern "C" {
+
+const char *sfwi_version(void) { return "sfwi_amd 0.1 (gfx950)"; }
+
+const char *sfwi_last_error(const sfwi_ctx *ctx) {
+    return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+int sfwi_ctx_create(const sfwi_params *p, sfwi_ctx **out) {
+    if (!p || !out) return fail(nullptr, SFWI_EINVAL, "NULL argument");
+    *out = nullptr;
+    if (p->nop != 4) return fail(nullptr, SFWI_EINVAL, "nop=%d: only the 8th-order stencil (nop=4) exists", p->nop);
+    if (p->pml < 8) return fail(nullptr, SFWI_EINVAL, "pml=%d: must be >= 8 so the strips lie inside the grid", p->pml);
+    if (p->nx < 8 || p->nz < 8 || p->nt < 1 || p->ntr < 1 || p->max_batch < 1)
+        return fail(nullptr, SFWI_EINVAL, "bad sizes nx=%d nz=%d nt=%d ntr=%d max_batch=%d", p->nx, p->nz, p->nt,
+                    p->ntr, p->max_batch);
+    if (p->nz != p->allnz) return fail(nullptr, SFWI_EINVAL, "nz (%d) must equal allnz (%d)", p->nz, p->allnz);
+    if (p->nx > p->allnx) return fail(nullptr, SFWI_EINVAL, "nx (%d) exceeds allnx (%d)", p->nx, p->allnx);
+    if (!(p->dx > 0 && p->dz > 0 && p->dt > 0 && p->rr > 0))
+        return fail(nullptr, SFWI_EINVAL, "dx, dz, dt and rr must be positive");
+    if (p->adjoint_dir != 1 && p->adjoint_dir != -1) return fail(nullptr, SFWI_EINVAL, "adjoint_dir must be +1 or -1");
+    if (p->mode != SFWI_MODE_FAST && p->mode != SFWI_MODE_EXACT) return fail(nullptr, SFWI_EINVAL, "unknown mode %d", p->mode);
+    if (p->storage != SFWI_STORE_STRIPS_F32) return fail(nullptr, SFWI_EINVAL, "storage %d not available", p->storage);
+    if (p->use_graph) return fail(nullptr, SFWI_EINVAL, "use_graph not available in this build");
+    const long nxp = p->nx + 2L * p->pml, nzp = p->nz + 2L * p->pml;
+    if (nxp * nzp >= (1L << 31) / 16) return fail(nullptr, SFWI_EINVAL, "grid too large for 32-bit cell indices");
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(nullptr, SFWI_EHIP, "no HIP device");
+
+    sfwi_ctx *c = new sfwi_ctx();
+    c->p = *p;
+    c->device = dev;
+    Geo &g = c->g;
+    g.nxp = (int)nxp; g.nzp = (int)nzp; g.pml = p->pml; g.nx = p->nx; g.nz = p->nz; g.nt = p->nt;
+    g.allnz = p->allnz;
+    g.nzd = g.nzp - 2 * NOP;
+    g.ndom = (g.nxp - 2 * NOP) * g.nzd;
+    const int nb = (g.ndom + BLK - 1) / BLK;
+    g.per_xcd = (nb + 7) / 8;
+    g.sf = 4 * NSTRIP * (p->nx + p->nz);
+    g.inv_nzd = 1.0f / (float)g.nzd;
+    g.dt = p->dt; g.dx = p->dx; g.dz = p->dz;
+    g.dtdx = p->dt / p->dx; g.dtdz = p->dt / p->dz;
+    g.idx = 1.0f / p->dx; g.idz = 1.0f / p->dz;
+    c->ncell = (size_t)nxp * nzp;
+    c->nmodel = (size_t)p->allnx * p->allnz;
+    c->nrec = (size_t)p->ntr * p->nt;
+    c->nint = (size_t)p->nx * p->nz;
+
+    auto bail = [&](int code) { g_create_error = c->err; sfwi_ctx_destroy(c); return code; };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(c, SFWI_EHIP, "hipStreamCreate failed"));
+    int r;
+    if ((r = dalloc(c, (void **)&c->wav, p->nt * sizeof(float)))) return bail(r);
+    if ((r = dalloc(c, (void **)&c->mvp, 5 * c->nmodel * sizeof(float)))) return bail(r);
+    c->mvs = c->mvp + c->nmodel; c->mrho = c->mvp + 2 * c->nmodel;
+    c->mdvp = c->mvp + 3 * c->nmodel; c->mdvs = c->mvp + 4 * c->nmodel;
+    if ((r = dalloc(c, (void **)&c->tmp_model, 4 * c->nmodel * sizeof(float)))) return bail(r);
+    if ((r = dalloc(c, (void **)&c->Gvp, 3 * c->nmodel * sizeof(float)))) return bail(r);
+    c->Gvs = c->Gvp + c->nmodel; c->Gil = c->Gvp + 2 * c->nmodel;
+    if ((r = dalloc(c, (void **)&c->res_keep, 2 * c->nrec * sizeof(float)))) return bail(r);
+    if ((r = dalloc(c, (void **)&c->d_shots, p->max_batch * sizeof(DevShot)))) return bail(r);
+    if ((r = dalloc(c, (void **)&c->d_out, (p->max_batch + 4) * sizeof(double)))) return bail(r);
+    if ((r = dalloc(c, (void **)&c->d_num, 2 * (size_t)p->ntr * sizeof(double)))) return bail(r);
+    c->d_den = c->d_num + p->ntr;
+    if (hipMemsetAsync(c->Gvp, 0, 3 * c->nmodel * sizeof(float), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->mvp, 0, 5 * c->nmodel * sizeof(float), c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return bail(fail(c, SFWI_EHIP, "initial clears failed"));
+    *out = c;
+    return 0;
+}
+
+void sfwi_ctx_destroy(sfwi_ctx *c) {
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (Slot &S : c->slots) {
+        hipFree(S.block);
+        hipFree(S.born);
+        hipFree(S.rec);
+        hipFree(S.strip);
+    }
+    drop_windows(c);
+    for (hipEvent_t e : c->ev) hipEventDestroy(e);
+    hipFree(c->wav); hipFree(c->mvp); hipFree(c->tmp_model); hipFree(c->Gvp); hipFree(c->res_keep);
+    hipFree(c->d_shots); hipFree(c->d_out); hipFree(c->d_num);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int sfwi_set_wavelet(sfwi_ctx *c, const float *w) {
+    if (!c || !w) return fail(c, SFWI_EINVAL, "NULL argument");
+    HIPCK(c, hipMemcpyAsync(c->wav, w, c->p.nt * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    c->have_wav = true;
+    return 0;
+}
+
+int sfwi_set_model(sfwi_ctx *c, const float *vp, const float *vs, const float *rho) {
+    if (!c || !vp || !vs) return fail(c, SFWI_EINVAL, "NULL model");
+    if (!rho && !(c->p.rho_const > 0)) return fail(c, SFWI_EINVAL, "rho is NULL and rho_const is not set");
+    const size_t b = c->nmodel * sizeof(float);
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    drop_windows(c);
+    HIPCK(c, hipMemcpyAsync(c->mvp, vp, b, hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipMemcpyAsync(c->mvs, vs, b, hipMemcpyHostToDevice, c->stream));
+    if (rho) HIPCK(c, hipMemcpyAsync(c->mrho, rho, b, hipMemcpyHostToDevice, c->stream));
+    else HIPCK(c, hipMemsetAsync(c->mrho, 0, b, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    c->have_model = true;
+    return 0;
+}
+
+int sfwi_set_perturbation(sfwi_ctx *c, const float *dvp, const float *dvs) {
+    if (!c || !dvp || !dvs) return fail(c, SFWI_EINVAL, "NULL perturbation");
+    const size_t b = c->nmodel * sizeof(float);
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    drop_windows(c);
+    HIPCK(c, hipMemcpyAsync(c->mdvp, dvp, b, hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipMemcpyAsync(c->mdvs, dvs, b, hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    c->have_pert = true;
+    return 0;
+}
+
+int sfwi_shot_gradient(sfwi_ctx *c, const sfwi_shot *shot, double *misfit) {
+    if (!c) return SFWI_EINVAL;
+    return solve_misfits(c, K_GRAD, 1, shot, misfit);
+}
+int sfwi_shot_born_gradient(sfwi_ctx *c, const sfwi_shot *shot, double *misfit) {
+    if (!c) return SFWI_EINVAL;
+    return solve_misfits(c, K_BORN_GRAD, 1, shot, misfit);
+}
+int sfwi_batch_gradient(sfwi_ctx *c, int nshots, const sfwi_shot *shots, double *misfits) {
+    if (!c) return SFWI_EINVAL;
+    return solve_misfits(c, K_GRAD, nshots, shots, misfits);
+}
+int sfwi_batch_born_gradient(sfwi_ctx *c, int nshots, const sfwi_shot *shots, double *misfits) {
+    if (!c) return SFWI_EINVAL;
+    return solve_misfits(c, K_BORN_GRAD, nshots, shots, misfits);
+}
+
+int sfwi_shot_forward(sfwi_ctx *c, const sfwi_shot *shot, float *syn_z, float *syn_x) {
+    if (!c || !shot) return fail(c, SFWI_EINVAL, "NULL argument");
+    int r = prepare(c, K_FWD, 1, shot, nullptr);
+    if (r || (r = run(c, K_FWD, 1))) return r;
+    if ((r = copy_records(c, c->slots[0].rz, syn_z, shot->ntr)) || (r = copy_records(c, c->slots[0].rx, syn_x, shot->ntr)))
+        return r;
+    return finish(c);
+}
+
+int sfwi_shot_born_forward(sfwi_ctx *c, const sfwi_shot *shot, float *b_z, float *b_x, float *bg_z, float *bg_x) {
+    if (!c || !shot) return fail(c, SFWI_EINVAL, "NULL argument");
+    int r = prepare(c, K_BORN_FWD, 1, shot, nullptr);
+    if (r || (r = run(c, K_BORN_FWD, 1))) return r;
+    const Slot &S = c->slots[0];
+    if ((r = copy_records(c, S.rz, b_z, shot->ntr)) || (r = copy_records(c, S.rx, b_x, shot->ntr)) ||
+        (r = copy_records(c, S.r2z, bg_z, shot->ntr)) || (r = copy_records(c, S.r2x, bg_x, shot->ntr)))
+        return r;
+    return finish(c);
+}
+
+int sfwi_linesearch_parabola(sfwi_ctx *c, const sfwi_shot *shot, const float *dir_vp, const float *dir_vs,
+                             float step1, float step2, double m[3]) {
+    if (!c || !shot || !dir_vp || !dir_vs || !m) return fail(c, SFWI_EINVAL, "NULL argument");
+    if (!c->have_model) return fail(c, SFWI_ESTATE, "model not set");
+    const size_t b = c->nmodel * sizeof(float);
+    float *tvp = c->tmp_model, *tvs = tvp + c->nmodel, *dvp = tvp + 2 * c->nmodel, *dvs = tvp + 3 * c->nmodel;
+    HIPCK(c, hipMemcpyAsync(dvp, dir_vp, b, hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipMemcpyAsync(dvs, dir_vs, b, hipMemcpyHostToDevice, c->stream));
+    const float steps[3] = {0.0f, step1, step2};
+    Window w;
+    for (int k = 0; k < 3; ++k) {
+        const float *mp = c->mvp, *ms = c->mvs;
+        if (k > 0) {
+            const dim3 grd((unsigned)((c->nmodel + 255) / 256));
+            hipLaunchKernelGGL(k_axpy_model, grd, dim3(256), 0, c->stream, (int)c->nmodel, c->mvp, dvp, steps[k], tvp);
+            hipLaunchKernelGGL(k_axpy_model, grd, dim3(256), 0, c->stream, (int)c->nmodel, c->mvs, dvs, steps[k], tvs);
+            mp = tvp;
+            ms = tvs;
+        }
+        int r = build_window(c, w, shot->left, mp, ms, c->mrho, nullptr, nullptr);
+        if (r) { hipFree(w.base); return r; }
+        if ((r = prepare(c, K_FWD_MISFIT, 1, shot, &w.m)) || (r = run(c, K_FWD_MISFIT, 1))) { hipFree(w.base); return r; }
+        double v = 0;
+        if (hipMemcpyAsync(&v, c->d_out, sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            (r = finish(c))) { hipFree(w.base); return r ? r : fail(c, SFWI_EHIP, "copy failed"); }
+        m[k] += v;
+    }
+    hipFree(w.base);
+    return 0;
+}
+
+int sfwi_linesearch_linear(sfwi_ctx *c, const sfwi_shot *shot, const float *cg_vp, const float *cg_vs,
+                           double *fenzi, double *fenmu) {
+    if (!c || !shot || !cg_vp || !cg_vs || !fenzi || !fenmu) return fail(c, SFWI_EINVAL, "NULL argument");
+    if (!shot->obs_z || !shot->obs_x) return fail(c, SFWI_EINVAL, "observed traces missing");
+    // Solve 1: residual r = born(dm) - obs + background.
+    int r = prepare(c, K_BORN_FWD, 1, shot, nullptr);
+    if (r || (r = run(c, K_BORN_FWD, 1))) return r;
+    Slot &S = c->slots[0];
+    const size_t rb = (size_t)shot->ntr * c->p.nt * sizeof(float);
+    HIPCK(c, hipMemcpyAsync(S.oz, shot->obs_z, rb, hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipMemcpyAsync(S.ox, shot->obs_x, rb, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_residual<true>, dim3(256, 1), dim3(BLK), 0, c->stream, c->g, c->d_shots);
+    HIPCK(c, hipMemcpyAsync(c->res_keep, S.rz, rb, hipMemcpyDeviceToDevice, c->stream));
+    HIPCK(c, hipMemcpyAsync(c->res_keep + c->nrec, S.rx, rb, hipMemcpyDeviceToDevice, c->stream));
+    // Solve 2: L(cg) = scattered traces with the perturbation cg.
+    const size_t b = c->nmodel * sizeof(float);
+    float *cvp = c->tmp_model + 2 * c->nmodel, *cvs = c->tmp_model + 3 * c->nmodel;
+    HIPCK(c, hipMemcpyAsync(cvp, cg_vp, b, hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipMemcpyAsync(cvs, cg_vs, b, hipMemcpyHostToDevice, c->stream));
+    Window w;
+    if ((r = build_window(c, w, shot->left, c->mvp, c->mvs, c->mrho, cvp, cvs))) { hipFree(w.base); return r; }
+    if ((r = prepare(c, K_BORN_FWD, 1, shot, &w.m)) || (r = run(c, K_BORN_FWD, 1))) { hipFree(w.base); return r; }
+    hipLaunchKernelGGL(k_trace_dot, dim3((shot->ntr + 63) / 64), dim3(64), 0, c->stream, c->g, shot->ntr, S.rz, S.rx,
+                       c->res_keep, c->res_keep + c->nrec, c->d_num, c->d_den);
+    std::vector<double> num(shot->ntr), den(shot->ntr);
+    if (hipMemcpyAsync(num.data(), c->d_num, shot->ntr * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipMemcpyAsync(den.data(), c->d_den, shot->ntr * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess) {
+        hipFree(w.base);
+        return fail(c, SFWI_EHIP, "copy failed");
+    }
+    r = finish(c);
+    hipFree(w.base);
+    if (r) return r;
+    // The per-trace FP64 partial sums are added in trace order.
+    for (int tr = 0; tr < shot->ntr; ++tr) { *fenzi += num[tr]; *fenmu += den[tr]; }
+    return 0;
+}
+
+int sfwi_flush(sfwi_ctx *c, float *gvp, float *gvs, float *gil) {
+    if (!c) return SFWI_EINVAL;
+    const size_t b = c->nmodel * sizeof(float);
+    if (gvp) HIPCK(c, hipMemcpyAsync(gvp, c->Gvp, b, hipMemcpyDeviceToHost, c->stream));
+    if (gvs) HIPCK(c, hipMemcpyAsync(gvs, c->Gvs, b, hipMemcpyDeviceToHost, c->stream));
+    if (gil) HIPCK(c, hipMemcpyAsync(gil, c->Gil, b, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipMemsetAsync(c->Gvp, 0, 3 * b, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int sfwi_set_timing(sfwi_ctx *c, int every) {
+    if (!c) return SFWI_EINVAL;
+    c->timing_every = every > 0 ? every : 0;
+    for (int k = 0; k < SFWI_K_COUNT; ++k) { c->kt_ms[k] = 0; c->kt_n[k] = 0; }
+    return 0;
+}
+
+int sfwi_kernel_time(sfwi_ctx *c, int kid, double *avg_ms, int *launches) {
+    if (!c || kid < 0 || kid >= SFWI_K_COUNT) return fail(c, SFWI_EINVAL, "bad kernel id %d", kid);
+    if (avg_ms) *avg_ms = c->kt_n[kid] ? c->kt_ms[kid] / c->kt_n[kid] : 0.0;
+    if (launches) *launches = c->kt_n[kid];
+    return 0;
+}
+
+const char *sfwi_kernel_name(int kid) {
+    static const char *names[SFWI_K_COUNT] = {"k_vel(forward)", "k_str(forward)", "k_str(reconstruct)",
+                                              "k_vel(reconstruct)", "k_adj_str", "k_adj_vel_image"};
+    return (kid >= 0 && kid < SFWI_K_COUNT) ? names[kid] : "?";
+}
+
+// Host-driver helper, same arithmetic as sfwi-stage1/main.cpp:251-260:
+// Nk in double stored as float; the exponent's argument is float (Nk is a
+// float and t-t0 an int), so the float overload of exp is taken.
+void sfwi_ricker(float *w, int nt, float dt, float f0) {
+    const double pi = 3.14159265358979;
+    const float nk = pi * pi * f0 * f0 * dt * dt;
+    const int t0 = (int)std::ceil(1.0 / (f0 * dt));
+    for (int t = 0; t < nt; ++t)
+        w[t] = (1.0 - 2.0 * nk * (t - t0) * (t - t0)) * std::exp(-nk * (t - t0) * (t - t0));
+}
+
+int sfwi_ot_misfit(sfwi_ctx *c, int, int, int, float, const float *, const float *, const float *, const float *,
+                   float *, float *, double *) {
+    return fail(c, SFWI_ESTATE, "optimal-transport misfit not built yet");
+}
+
+}  // extern "C"

@@ -1,0 +1,235 @@
+"""Host-side mirror of the reference's per-shot interface over the C ABI.
+
+``Context`` is the pythonic wrapper of one ``sfwi_ctx``.  ``FD2DGPU_ELASTIC``
+and the free functions ``IMAGE_SINGLESHOT``, ``CALCULATE_ALPHA_parabola_mb``
+and ``CALCULATE_ALPHA`` keep the names, argument meaning and call order of the
+reference (sfwi-stage1/FD2DGPU.cuh:146-472, sfwi-stage2/FD2DGPU.cuh:455-458),
+so a driver written against the reference reads the same.  Every call goes to
+the HIP library; errors raise ``SfwiError``.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import SfwiError, SfwiParams, SfwiShot
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class Shot:
+    """One shot's geometry and observed traces (host arrays kept alive here)."""
+
+    def __init__(self, left, isx, isz, igz, ngx_left, gc, obs_z=None, obs_x=None):
+        self.gc = np.ascontiguousarray(gc, dtype=np.int32)
+        self.obs_z = None if obs_z is None else _f32(obs_z)
+        self.obs_x = None if obs_x is None else _f32(obs_x)
+        self.c = SfwiShot()
+        self.c.left, self.c.isx, self.c.isz, self.c.igz = int(left), int(isx), int(isz), int(igz)
+        self.c.ngx_left, self.c.ntr = int(ngx_left), len(self.gc)
+        self.c.gc = self.gc.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        fp = ctypes.POINTER(ctypes.c_float)
+        self.c.obs_z = self.obs_z.ctypes.data_as(fp) if self.obs_z is not None else fp()
+        self.c.obs_x = self.obs_x.ctypes.data_as(fp) if self.obs_x is not None else fp()
+
+    @property
+    def ntr(self):
+        return len(self.gc)
+
+
+class Context:
+    """One device context (``sfwi_ctx``)."""
+
+    def __init__(self, *, dx, dz, dt, nx, nz, pml, nt, ntr, allnx, allnz, rr=0.001, rho_const=0.0,
+                 adjoint_dir=1, max_batch=1, mode="fast", storage="f32", use_graph=False):
+        self.L = _lib.load()
+        p = SfwiParams()
+        p.dx, p.dz, p.dt = dx, dz, dt
+        p.nx, p.nz, p.pml, p.nt, p.nop, p.ntr = nx, nz, pml, nt, 4, ntr
+        p.allnx, p.allnz, p.rr, p.rho_const = allnx, allnz, rr, rho_const
+        p.adjoint_dir, p.max_batch = adjoint_dir, max_batch
+        p.mode = {"fast": _lib.MODE_FAST, "exact": _lib.MODE_EXACT}[mode]
+        p.storage = {"f32": _lib.STORE_STRIPS_F32, "f16": _lib.STORE_STRIPS_F16,
+                     "bf16": _lib.STORE_STRIPS_BF16}[storage]
+        p.use_graph = int(bool(use_graph))
+        self.p = p
+        h = ctypes.c_void_p()
+        rc = self.L.sfwi_ctx_create(ctypes.byref(p), ctypes.byref(h))
+        if rc != 0:
+            raise SfwiError(rc, self.L.sfwi_last_error(None).decode())
+        self.h = h
+        self.nt, self.allnx, self.allnz = nt, allnx, allnz
+
+    def _ck(self, rc):
+        if rc != 0:
+            raise SfwiError(rc, self.L.sfwi_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.sfwi_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---------------------------------------------------------- inputs ---
+    def set_wavelet(self, w):
+        w = _f32(w)
+        assert w.size == self.nt
+        self._ck(self.L.sfwi_set_wavelet(self.h, w))
+
+    def set_model(self, vp, vs, rho=None):
+        self._vp, self._vs = _f32(vp), _f32(vs)
+        self._rho = None if rho is None else _f32(rho)
+        self._ck(self.L.sfwi_set_model(self.h, _lib.ptr(self._vp), _lib.ptr(self._vs), _lib.ptr(self._rho)))
+
+    def set_perturbation(self, dvp, dvs):
+        self._ck(self.L.sfwi_set_perturbation(self.h, _f32(dvp), _f32(dvs)))
+
+    # ---------------------------------------------------------- solves ---
+    def forward(self, shot):
+        z = np.zeros((shot.ntr, self.nt), np.float32)
+        x = np.zeros((shot.ntr, self.nt), np.float32)
+        self._ck(self.L.sfwi_shot_forward(self.h, ctypes.byref(shot.c), _lib.ptr(z), _lib.ptr(x)))
+        return z, x
+
+    def born_forward(self, shot):
+        out = [np.zeros((shot.ntr, self.nt), np.float32) for _ in range(4)]
+        self._ck(self.L.sfwi_shot_born_forward(self.h, ctypes.byref(shot.c), *[_lib.ptr(a) for a in out]))
+        return tuple(out)
+
+    def gradient(self, shot):
+        m = ctypes.c_double(0.0)
+        self._ck(self.L.sfwi_shot_gradient(self.h, ctypes.byref(shot.c), ctypes.byref(m)))
+        return m.value
+
+    def born_gradient(self, shot):
+        m = ctypes.c_double(0.0)
+        self._ck(self.L.sfwi_shot_born_gradient(self.h, ctypes.byref(shot.c), ctypes.byref(m)))
+        return m.value
+
+    def batch_gradient(self, shots, born=False):
+        arr = (SfwiShot * len(shots))(*[s.c for s in shots])
+        m = np.zeros(len(shots), np.float64)
+        fn = self.L.sfwi_batch_born_gradient if born else self.L.sfwi_batch_gradient
+        self._ck(fn(self.h, len(shots), arr, m.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return m
+
+    def linesearch_parabola(self, shot, dir_vp, dir_vs, step1, step2):
+        m = np.zeros(3, np.float64)
+        self._ck(self.L.sfwi_linesearch_parabola(self.h, ctypes.byref(shot.c), _f32(dir_vp), _f32(dir_vs),
+                                                 step1, step2,
+                                                 m.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return m
+
+    def linesearch_linear(self, shot, cg_vp, cg_vs):
+        a, b = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        self._ck(self.L.sfwi_linesearch_linear(self.h, ctypes.byref(shot.c), _f32(cg_vp), _f32(cg_vs),
+                                               ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def flush(self):
+        sh = (self.allnx, self.allnz)
+        g = [np.zeros(sh, np.float32) for _ in range(3)]
+        self._ck(self.L.sfwi_flush(self.h, *[_lib.ptr(a) for a in g]))
+        return tuple(g)
+
+    # ---------------------------------------------------------- timing ---
+    def set_timing(self, every):
+        self._ck(self.L.sfwi_set_timing(self.h, int(every)))
+
+    def kernel_time(self, kid):
+        ms, n = ctypes.c_double(0.0), ctypes.c_int(0)
+        self._ck(self.L.sfwi_kernel_time(self.h, kid, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def ot_misfit(self, obs_x, obs_z, syn_x, syn_z, dt, variant=0):
+        obs_x, obs_z, syn_x, syn_z = map(_f32, (obs_x, obs_z, syn_x, syn_z))
+        ntr, nt = obs_x.shape
+        rx, rz = np.zeros_like(obs_x), np.zeros_like(obs_z)
+        m = ctypes.c_double(0.0)
+        self._ck(self.L.sfwi_ot_misfit(self.h, variant, ntr, nt, dt, obs_x, obs_z, syn_x, syn_z, rx, rz,
+                                       ctypes.byref(m)))
+        return m.value, rx, rz
+
+
+# --------------------------------------------------------------------------
+# Reference-shaped interface
+# --------------------------------------------------------------------------
+
+class FD2DGPU_ELASTIC:
+    """Mirror of class FD2DGPU_ELASTIC (sfwi-stage1/FD2DGPU.cuh:146-459).
+
+    The constructor takes the reference's arguments (sou, dx, dz, dt, nxpml,
+    nzpml, allnx, allnz, scale, pml, nt, nop, ntr); the caller then sets the
+    public per-shot fields (isx, isz, igz, ngx_left, delta_left,
+    record_left_in_v) exactly as sfwi-stage1/main.cpp:596-625 does.
+    """
+
+    def __init__(self, sou, dx, dz, dt, nxpml, nzpml, allnx, allnz, scale, pml, nt, nop, ntr, **kw):
+        if nop != 4:
+            raise ValueError("nop must be 4")
+        self.dx, self.dz, self.dt, self.pml, self.nt = dx, dz, dt, pml, nt
+        self.nxpml, self.nzpml, self.allnx, self.allnz = nxpml, nzpml, allnx, allnz
+        self.nx, self.nz = nxpml - 2 * pml, nzpml - 2 * pml
+        self.ntr = ntr
+        self.ctx = Context(dx=dx, dz=dz, dt=dt, nx=self.nx, nz=self.nz, pml=pml, nt=nt, ntr=ntr,
+                           allnx=allnx, allnz=allnz, **kw)
+        self.ctx.set_wavelet(sou)
+        self.isx = self.isz = self.igz = 0
+        self.ngx_left = self.delta_left = self.record_left_in_v = 0
+        self.cmin = 0
+        self._rec = None
+        self._left = 0
+
+    def GPUbufferVPVS(self, vp_all, vs_all, rho_all=None):
+        self.ctx.set_model(vp_all, vs_all, rho_all)
+
+    def GPUbufferM(self, pp, ps):
+        self.ctx.set_perturbation(pp, ps)
+
+    def record_copytoGPU(self, record, record_x, gc):
+        self._rec = (_f32(record), _f32(record_x), np.ascontiguousarray(gc, np.int32))
+
+    def bufferVpVsHtoD(self, ngx_min):
+        self._left = int(ngx_min)
+
+    def _shot(self):
+        rz, rx, gc = self._rec
+        return Shot(self._left, self.isx, self.isz, self.igz, self.ngx_left, gc[: self.ntr],
+                    rz.reshape(-1)[: self.ntr * self.nt].reshape(self.ntr, self.nt),
+                    rx.reshape(-1)[: self.ntr * self.nt].reshape(self.ntr, self.nt))
+
+    def imagebuffer_resettozero(self):
+        """Returns (pp_gradient, ps_gradient, illumination) and zeroes them."""
+        return self.ctx.flush()
+
+
+def IMAGE_SINGLESHOT(fd, misfit=0.0, born=False):
+    """Per-shot gradient (sfwi-stage1/FD2DGPU.cu:4321-4695; stage 2 with born=True,
+    sfwi-stage2/FD2DGPU.cu:3978-4570).  Returns misfit + this shot's misfit."""
+    s = fd._shot()
+    return misfit + (fd.ctx.born_gradient(s) if born else fd.ctx.gradient(s))
+
+
+def CALCULATE_ALPHA_parabola_mb(fd, cpu_pp_grad_old, cpu_ps_grad_old, step_1, step_2, m=(0.0, 0.0, 0.0)):
+    """Three forward solves of the stage-1 line search (sfwi-stage1/FD2DGPU.cu:4819-5028)."""
+    r = fd.ctx.linesearch_parabola(fd._shot(), cpu_pp_grad_old, cpu_ps_grad_old, step_1, step_2)
+    return tuple(float(a + b) for a, b in zip(m, r))
+
+
+def CALCULATE_ALPHA(fd, cpu_pp_grad_old, cpu_ps_grad_old, fenzi=0.0, fenmu=0.0):
+    """Two Born solves of the stage-2 exact step (sfwi-stage2/FD2DGPU.cu:4576-4722)."""
+    a, b = fd.ctx.linesearch_linear(fd._shot(), cpu_pp_grad_old, cpu_ps_grad_old)
+    return fenzi + a, fenmu + b

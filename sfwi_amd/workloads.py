@@ -1,0 +1,217 @@
+"""Seeded synthetic survey definitions (models, geometry, wavelets).
+
+No field data ships with the reference (its SU records are Git-LFS pointers),
+so every workload here is synthetic and seeded.  Models are float32 arrays in
+the reference's .bin layout: shape (allnx, allnz), x-major, z fastest.
+
+Geometry follows the reference driver's conventions
+(sfwi-stage1/main.cpp:596-625): a shot window starts at the survey's leftmost
+coordinate, ``isx = pml + (sx - ngx_left)/dx``, ``isz = pml + sy``,
+``igz = pml + gy`` and receiver i sits at coordinate ``gc[i]``.
+"""
+from dataclasses import dataclass, field
+from typing import List
+
+import numpy as np
+
+
+@dataclass
+class Shot:
+    sx: int                 # source coordinate (m, integer as in the SU header)
+    gc: np.ndarray          # receiver coordinates (m), int32
+    ngx_left: int           # leftmost coordinate of the shot's aperture
+    left: int               # first model column of the window (delta_left + record_left_in_v)
+
+
+@dataclass
+class Survey:
+    name: str
+    vp: np.ndarray          # true model (allnx, allnz)
+    vs: np.ndarray
+    rho: np.ndarray
+    vp0: np.ndarray         # starting model
+    vs0: np.ndarray
+    rho0: np.ndarray
+    dx: float
+    dz: float
+    dt: float
+    nt: int
+    pml: int
+    f0: float
+    sy: int
+    gy: int
+    nx: int                 # window width (all shots share it here)
+    shots: List[Shot] = field(default_factory=list)
+
+    @property
+    def allnx(self):
+        return self.vp.shape[0]
+
+    @property
+    def allnz(self):
+        return self.vp.shape[1]
+
+    @property
+    def ntr(self):
+        return len(self.shots[0].gc)
+
+    def cells(self):
+        return (self.nx + 2 * self.pml) * (self.allnz + 2 * self.pml)
+
+
+def gardner(vp):
+    """Gardner's relation rho = 310 vp^0.25 (kg/m^3, vp in m/s)."""
+    return (310.0 * np.power(vp, 0.25)).astype(np.float32)
+
+
+def gaussian_smooth(a, sigma):
+    """Separable Gaussian smoothing with edge replication (numpy only)."""
+    if sigma <= 0:
+        return a.copy()
+    r = int(3 * sigma)
+    k = np.exp(-0.5 * (np.arange(-r, r + 1) / sigma) ** 2)
+    k /= k.sum()
+    out = a.astype(np.float64)
+    for ax in (0, 1):
+        pad = [(0, 0), (0, 0)]
+        pad[ax] = (r, r)
+        p = np.pad(out, pad, mode="edge")
+        out = np.apply_along_axis(lambda v: np.convolve(v, k, mode="valid"), ax, p)
+    return out.astype(np.float32)
+
+
+def ricker(nt, dt, f0):
+    """Ricker wavelet exactly as the host driver builds it (sfwi-stage1/main.cpp:251-260)."""
+    pi = 3.14159265358979
+    f, d = float(np.float32(f0)), float(np.float32(dt))
+    nk = float(np.float32(pi * pi * f * f * d * d))          # double product stored as float
+    t0 = int(np.ceil(1.0 / float(np.float32(f0) * np.float32(dt))))   # float product
+    t = np.arange(nt, dtype=np.float64) - t0
+    return ((1.0 - 2.0 * nk * t * t) * np.exp(-nk * t * t)).astype(np.float32)
+
+
+def _full_aperture_shots(nx, dx, nshots):
+    gc = (np.arange(nx) * dx).astype(np.int32)
+    out = []
+    for k in range(nshots):
+        ix = int((k + 0.5) * nx / nshots)
+        out.append(Shot(sx=int(ix * dx), gc=gc.copy(), ngx_left=0, left=0))
+    return out
+
+
+def _layered(nx, nz, tops, vps, rng=None):
+    vp = np.zeros((nx, nz), np.float32)
+    z = np.arange(nz)
+    for top, v in zip(tops, vps):
+        vp[:, z >= top] = v
+    return vp
+
+
+def plain_small(seed=1001, nshots=16, nt=2000):
+    """376x201, dx=dz=10 m, 4 flat layers 1500-4000 m/s, Vs=Vp/1.73 (0 in a
+    10-cell water top), Gardner rho, +10% Gaussian Vp anomaly (sigma 15
+    cells); starting model = truth smoothed by a 20-cell Gaussian; dt 1 ms,
+    20-cell PML, 10 Hz, shots at 2-cell depth, receivers at every surface cell."""
+    rng = np.random.default_rng(seed)
+    nx, nz = 376, 201
+    vp = _layered(nx, nz, [0, 10, 80, 140], [1500.0, 2300.0, 3100.0, 4000.0])
+    cx, cz = rng.uniform(120, 256), rng.uniform(90, 150)
+    X, Z = np.meshgrid(np.arange(nx), np.arange(nz), indexing="ij")
+    g = np.exp(-((X - cx) ** 2 + (Z - cz) ** 2) / (2 * 15.0 ** 2))
+    vp = (vp * (1.0 + 0.10 * g * (Z >= 10))).astype(np.float32)
+    vs = (vp / 1.73).astype(np.float32)
+    vs[:, :10] = 0.0
+    rho = gardner(vp)
+    vp0 = gaussian_smooth(vp, 20.0)
+    vs0 = gaussian_smooth(vs, 20.0)
+    vs0[:, :10] = 0.0
+    rho0 = gardner(vp0)
+    sv = Survey("plain_small_376x201", vp, vs, rho, vp0, vs0, rho0, 10.0, 10.0, 0.001, nt, 20,
+                10.0, 2, 0, nx)
+    sv.shots = _full_aperture_shots(nx, 10.0, nshots)
+    return sv
+
+
+def shot_batch(seed=1002, nshots=64, nt=3000):
+    """401x301, dx=dz=10 m, 6 seeded layers with a 15-degree dipping
+    interface, Vs=Vp/1.73, Gardner rho; dt 1 ms, 30-cell PML, 12 Hz, 401
+    two-component receivers per shot."""
+    rng = np.random.default_rng(seed)
+    nx, nz = 401, 301
+    tops = np.sort(rng.uniform(20, 280, size=5))
+    vps = np.sort(rng.uniform(1600, 4000, size=6))
+    X, Z = np.meshgrid(np.arange(nx), np.arange(nz), indexing="ij")
+    vp = np.full((nx, nz), vps[0], np.float32)
+    dip = np.tan(np.deg2rad(15.0))
+    for k, top in enumerate(tops):
+        zt = top + (dip * (X - nx / 2) if k == 2 else 0.0)
+        vp = np.where(Z >= zt, vps[k + 1], vp).astype(np.float32)
+    vs = (vp / 1.73).astype(np.float32)
+    rho = gardner(vp)
+    vp0 = gaussian_smooth(vp, 10.0)
+    vs0 = gaussian_smooth(vs, 10.0)
+    rho0 = gardner(vp0)
+    sv = Survey("shot_batch_401x301", vp, vs, rho, vp0, vs0, rho0, 10.0, 10.0, 0.001, nt, 30,
+                12.0, 2, 0, nx)
+    sv.shots = _full_aperture_shots(nx, 10.0, nshots)
+    return sv
+
+
+def mixed_precision(seed=1003, nshots=96, nt=6000):
+    """1200x600, dx=dz=5 m, 8 seeded random layers plus 3 Gaussian inclusions
+    of +-8 %; dt 0.5 ms, 40-cell PML, 15 Hz."""
+    rng = np.random.default_rng(seed)
+    nx, nz = 1200, 600
+    tops = np.sort(rng.uniform(30, 560, size=7))
+    vps = np.sort(rng.uniform(1600, 4000, size=8))
+    vp = _layered(nx, nz, [0] + list(tops), vps)
+    X, Z = np.meshgrid(np.arange(nx), np.arange(nz), indexing="ij")
+    for s in (+1, -1, +1):
+        cx, cz, sg = rng.uniform(200, 1000), rng.uniform(150, 500), rng.uniform(20, 40)
+        vp = vp * (1.0 + s * 0.08 * np.exp(-((X - cx) ** 2 + (Z - cz) ** 2) / (2 * sg ** 2)))
+    vp = vp.astype(np.float32)
+    vs = (vp / 1.73).astype(np.float32)
+    rho = gardner(vp)
+    vp0 = gaussian_smooth(vp, 20.0)
+    vs0 = gaussian_smooth(vs, 20.0)
+    rho0 = gardner(vp0)
+    sv = Survey("mixed_precision_1200x600", vp, vs, rho, vp0, vs0, rho0, 5.0, 5.0, 0.0005, nt, 40,
+                15.0, 2, 0, nx)
+    sv.shots = _full_aperture_shots(nx, 5.0, nshots)
+    return sv
+
+
+def tiny(seed=7, nx=48, nz=40, pml=8, nt=200, nshots=2, f0=25.0, elastic=True):
+    """Small seeded layered model for fast parity tests (dx=dz=10 m, dt 1 ms)."""
+    rng = np.random.default_rng(seed)
+    tops = np.sort(rng.uniform(5, nz - 5, size=2))
+    vp = _layered(nx, nz, [0] + list(tops), [1800.0, 2500.0, 3200.0])
+    X, Z = np.meshgrid(np.arange(nx), np.arange(nz), indexing="ij")
+    cx, cz = rng.uniform(nx * 0.3, nx * 0.7), rng.uniform(nz * 0.4, nz * 0.7)
+    vp = (vp * (1.0 + 0.08 * np.exp(-((X - cx) ** 2 + (Z - cz) ** 2) / (2 * 4.0 ** 2)))).astype(np.float32)
+    vs = (vp / 1.73).astype(np.float32) if elastic else np.zeros_like(vp)
+    rho = gardner(vp)
+    vp0 = gaussian_smooth(vp, 3.0)
+    vs0 = gaussian_smooth(vs, 3.0)
+    rho0 = gardner(vp0)
+    sv = Survey("tiny_%dx%d" % (nx, nz), vp, vs, rho, vp0, vs0, rho0, 10.0, 10.0, 0.001, nt, pml,
+                f0, 2, 0, nx)
+    sv.shots = _full_aperture_shots(nx, 10.0, nshots)
+    return sv
+
+
+def shot_geometry(sv: Survey, shot: Shot):
+    """Padded-grid indices of one shot, as the reference driver sets them
+    (sfwi-stage1/main.cpp:619-625)."""
+    isx = sv.pml + int((shot.sx - shot.ngx_left) / np.float32(sv.dx))
+    isz = sv.pml + sv.sy
+    igz = sv.pml + sv.gy
+    return isx, isz, igz
+
+
+BY_NAME = {
+    "plain_small": plain_small,
+    "shot_batch": shot_batch,
+    "mixed_precision": mixed_precision,
+    "tiny": tiny,
+}

@@ -1,0 +1,50 @@
+"""C-ABI checks that need no GPU: the library loads, exports every function
+declared in include/sfwi_abi.h, rejects bad parameters before touching the
+device, and its host-side Ricker helper matches the oracle bit for bit."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from sfwi_amd import _lib
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.load()
+    names = _lib.declared_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_version_string():
+    assert b"gfx950" in _lib.load().sfwi_version()
+
+
+@pytest.mark.parametrize("nt,dt,f0", [(2000, 0.001, 10.0), (3000, 0.001, 12.0), (6000, 0.0005, 15.0),
+                                      (300, 0.001, 25.0), (1500, 0.001, 20.0)])
+def test_ricker_matches_reference_arithmetic(nt, dt, f0):
+    assert np.array_equal(_lib.ricker(nt, dt, f0), O.ricker(nt, dt, f0))
+
+
+def _params(**kw):
+    p = _lib.SfwiParams()
+    p.dx, p.dz, p.dt = 10.0, 10.0, 0.001
+    p.nx, p.nz, p.pml, p.nt, p.nop, p.ntr = 48, 40, 8, 100, 4, 48
+    p.allnx, p.allnz, p.rr, p.rho_const = 48, 40, 0.001, 0.0
+    p.adjoint_dir, p.max_batch, p.mode, p.storage, p.use_graph = 1, 1, 0, 0, 0
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+@pytest.mark.parametrize("bad", [dict(nop=6), dict(pml=4), dict(nz=30), dict(adjoint_dir=0),
+                                 dict(dt=0.0), dict(max_batch=0), dict(mode=7), dict(nx=100)])
+def test_create_rejects_bad_parameters(bad):
+    L = _lib.load()
+    h = ctypes.c_void_p()
+    rc = L.sfwi_ctx_create(ctypes.byref(_params(**bad)), ctypes.byref(h))
+    assert rc == _lib.SFWI_EINVAL
+    assert not h.value
+    assert L.sfwi_last_error(None)
