@@ -113,6 +113,9 @@ int sfwi_batch_born_gradient(sfwi_ctx *ctx, int nshots, const sfwi_shot *shots, 
  * of CALCULATE_ALPHA_parabola_mb, sfwi-stage1/FD2DGPU.cu:4864-4870):
  * synthetic vz / vx traces [ntr][nt]. */
 int sfwi_shot_forward(sfwi_ctx *ctx, const sfwi_shot *shot, float *syn_z, float *syn_x);
+/* Forward solves of many shots at once; traces of shot i at
+ * syn_z + i*ntr_max*nt (and syn_x likewise), ntr_max = sfwi_params.ntr. */
+int sfwi_batch_forward(sfwi_ctx *ctx, int nshots, const sfwi_shot *shots, float *syn_z, float *syn_x);
 /* Born forward solve (born_modeling mode 1 + record_scatter, the inner loop of
  * CALCULATE_ALPHA, sfwi-stage2/FD2DGPU.cu:4607-4620): scattered traces b_*
  * and background traces bg_* (either pair may be NULL). */

@@ -82,6 +82,7 @@ def load():
     L.sfwi_batch_gradient.argtypes = [vp, ctypes.c_int, P(SfwiShot), dp]
     L.sfwi_batch_born_gradient.argtypes = [vp, ctypes.c_int, P(SfwiShot), dp]
     L.sfwi_shot_forward.argtypes = [vp, P(SfwiShot), vp, vp]
+    L.sfwi_batch_forward.argtypes = [vp, ctypes.c_int, P(SfwiShot), vp, vp]
     L.sfwi_shot_born_forward.argtypes = [vp, P(SfwiShot), vp, vp, vp, vp]
     L.sfwi_linesearch_parabola.argtypes = [vp, P(SfwiShot), _f32p, _f32p, ctypes.c_float,
                                            ctypes.c_float, dp]

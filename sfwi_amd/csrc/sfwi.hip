@@ -571,6 +571,23 @@ int sfwi_shot_forward(sfwi_ctx *c, const sfwi_shot *shot, float *syn_z, float *s
     return finish(c);
 }
 
+int sfwi_batch_forward(sfwi_ctx *c, int nshots, const sfwi_shot *shots, float *syn_z, float *syn_x) {
+    if (!c || !shots || nshots < 0) return fail(c, SFWI_EINVAL, "bad shot list");
+    for (int b0 = 0; b0 < nshots; b0 += c->p.max_batch) {
+        const int n = std::min(c->p.max_batch, nshots - b0);
+        int r = prepare(c, K_FWD, n, shots + b0, nullptr);
+        if (r || (r = run(c, K_FWD, n))) return r;
+        for (int k = 0; k < n; ++k) {
+            const size_t o = (size_t)(b0 + k) * c->nrec;
+            if ((r = copy_records(c, c->slots[k].rz, syn_z ? syn_z + o : nullptr, shots[b0 + k].ntr)) ||
+                (r = copy_records(c, c->slots[k].rx, syn_x ? syn_x + o : nullptr, shots[b0 + k].ntr)))
+                return r;
+        }
+        if ((r = finish(c))) return r;
+    }
+    return 0;
+}
+
 int sfwi_shot_born_forward(sfwi_ctx *c, const sfwi_shot *shot, float *b_z, float *b_x, float *bg_z, float *bg_x) {
     if (!c || !shot) return fail(c, SFWI_EINVAL, "NULL argument");
     int r = prepare(c, K_BORN_FWD, 1, shot, nullptr);

@@ -104,6 +104,14 @@ class Context:
         self._ck(self.L.sfwi_shot_forward(self.h, ctypes.byref(shot.c), _lib.ptr(z), _lib.ptr(x)))
         return z, x
 
+    def batch_forward(self, shots):
+        """Traces of every shot: arrays (nshots, ntr_max, nt)."""
+        arr = (SfwiShot * len(shots))(*[s.c for s in shots])
+        z = np.zeros((len(shots), self.p.ntr, self.nt), np.float32)
+        x = np.zeros_like(z)
+        self._ck(self.L.sfwi_batch_forward(self.h, len(shots), arr, _lib.ptr(z), _lib.ptr(x)))
+        return z, x
+
     def born_forward(self, shot):
         out = [np.zeros((shot.ntr, self.nt), np.float32) for _ in range(4)]
         self._ck(self.L.sfwi_shot_born_forward(self.h, ctypes.byref(shot.c), *[_lib.ptr(a) for a in out]))
