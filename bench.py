@@ -59,6 +59,9 @@ def main():
     ap.add_argument("--nt", type=int, default=3000)
     ap.add_argument("--mode", default="fast", choices=["fast", "exact"])
     ap.add_argument("--timing-every", type=int, default=20)
+    ap.add_argument("--march-cols", type=int, default=0,
+                    help="FAST kernels: columns per wave segment (0: library default, <0: per-cell kernels)")
+    ap.add_argument("--march-depth", type=int, default=0, help="FAST march kernels: prefetch depth")
     ap.add_argument("--cpu-shots", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -84,7 +87,8 @@ def main():
         sv.name, len(sv.shots), sv.nx, sv.allnz, sv.pml, n_cells, sv.nt), file=sys.stderr, flush=True)
 
     ctx = Context(dx=sv.dx, dz=sv.dz, dt=sv.dt, nx=sv.nx, nz=sv.allnz, pml=sv.pml, nt=sv.nt, ntr=sv.ntr,
-                  allnx=sv.allnx, allnz=sv.allnz, max_batch=len(sv.shots), mode=args.mode)
+                  allnx=sv.allnx, allnz=sv.allnz, max_batch=len(sv.shots), mode=args.mode,
+                  march_cols=args.march_cols, march_depth=args.march_depth)
     ctx.set_wavelet(w)
     ctx.set_model(sv.vp, sv.vs, sv.rho)
     t0 = time.perf_counter()

@@ -61,6 +61,9 @@ typedef struct sfwi_params {
     int mode;              /* SFWI_MODE_*                                           */
     int storage;           /* SFWI_STORE_*                                          */
     int use_graph;         /* 1: replay the time loop from a captured hipGraph      */
+    int march_cols;        /* FAST kernels: columns per wave segment of the column-  */
+                           /* march kernels (0 = 8); < 0 selects per-cell kernels */
+    int march_depth;       /* columns of operands prefetched ahead (1, 2, 4; 0 = 1) */
 } sfwi_params;
 
 /* One shot: the fields the reference driver writes into FD2DGPU_ELASTIC

@@ -28,6 +28,7 @@ class SfwiParams(ctypes.Structure):
         ("rr", ctypes.c_float), ("rho_const", ctypes.c_float),
         ("adjoint_dir", ctypes.c_int), ("max_batch", ctypes.c_int),
         ("mode", ctypes.c_int), ("storage", ctypes.c_int), ("use_graph", ctypes.c_int),
+        ("march_cols", ctypes.c_int), ("march_depth", ctypes.c_int),
     ]
 
 

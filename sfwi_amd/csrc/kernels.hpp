@@ -72,6 +72,9 @@ struct Geo {
     int nxp, nzp, pml, nx, nz, nt, allnz;
     int nzd, ndom;          // update domain height and cell count
     int per_xcd;            // blocks per XCD in the remapped grid
+    int cm_xs;              // column-march segment width (FAST kernels)
+    int cm_nbands, cm_waves;// row bands per column strip, waves per shot
+    int cm_per_xcd;         // column-march workgroups per XCD
     int sf;                 // strip floats per field and step
     float inv_nzd;
     float dt, dx, dz, dtdx, dtdz, idx, idz;
@@ -151,6 +154,9 @@ __device__ inline Damp damp_of(const Geo &g, const Medium &m, int i) {
 
 // --------------------------------------------------- one-cell updates ---
 
+__device__ inline void vel_update_fast(const Geo &g, const Medium &m, const Fields &F, int i, bool split,
+                                       int dir, float a, float b, float c, float e);
+
 // Velocity update (sfwi-stage1/born.cu:355-394, update_vx_vz).
 template <bool EXACT>
 __device__ inline void vel_cell(const Geo &g, const Medium &m, const Fields &F, int i, bool split,
@@ -173,24 +179,8 @@ __device__ inline void vel_cell(const Geo &g, const Medium &m, const Fields &F, 
         F.vz_z[i] = vzz;
         F.vx[i] = vxx + vxz;
         F.vz[i] = vzx + vzz;
-    } else if (split) {
-        const Damp d = damp_of(g, m, i);
-        const float bu = (float)dir * m.bu[i];
-        const float bx = bu * g.dtdx, bz = bu * g.dtdz;
-        const float vxx = __builtin_fmaf(d.d1, F.vx_x[i], bx * a) / d.d2;
-        const float vxz = __builtin_fmaf(d.d3, F.vx_z[i], bz * b) / d.d4;
-        const float vzx = __builtin_fmaf(d.d1, F.vz_x[i], bx * e) / d.d2;
-        const float vzz = __builtin_fmaf(d.d3, F.vz_z[i], bz * c) / d.d4;
-        F.vx_x[i] = vxx;
-        F.vx_z[i] = vxz;
-        F.vz_x[i] = vzx;
-        F.vz_z[i] = vzz;
-        F.vx[i] = vxx + vxz;
-        F.vz[i] = vzx + vzz;
     } else {
-        const float bu = (float)dir * m.bu[i];
-        F.vx[i] = __builtin_fmaf(bu, __builtin_fmaf(g.dtdx, a, g.dtdz * b), F.vx[i]);
-        F.vz[i] = __builtin_fmaf(bu, __builtin_fmaf(g.dtdx, e, g.dtdz * c), F.vz[i]);
+        vel_update_fast(g, m, F, i, split, dir, a, b, c, e);
     }
 }
 
@@ -239,6 +229,216 @@ __device__ inline BornSrc born_src(const Geo &g, const Medium &m, const VGrad &v
     return s;
 }
 
+// ------------------------------------------------- FAST-mode updates ---
+// The update of one cell given its stencil values, split into the loads of
+// the cell's own state (a marching kernel issues them one column ahead), the
+// arithmetic and the stores.  Split cells follow the reference's split-PML
+// form; unsplit cells advance the totals directly.
+
+__device__ inline Damp damp_v(const Geo &g, float ax, float az) {
+    Damp d;
+    d.d1 = 1.0f - g.dt * ax / 2.0f;
+    d.d2 = 1.0f + g.dt * ax / 2.0f;
+    d.d3 = 1.0f - g.dt * az / 2.0f;
+    d.d4 = 1.0f + g.dt * az / 2.0f;
+    return d;
+}
+
+// Own state of a velocity update (forward, reconstruction or adjoint).
+struct VelCell {
+    float vx, vz, vx_x, vx_z, vz_x, vz_z, bu, ax, az;
+};
+__device__ inline VelCell vel_load(const Medium &m, const Fields &F, int i, bool split) {
+    VelCell s;
+    s.bu = m.bu[i];
+    if (split) {
+        s.vx_x = F.vx_x[i]; s.vx_z = F.vx_z[i]; s.vz_x = F.vz_x[i]; s.vz_z = F.vz_z[i];
+        s.ax = m.dampx[i]; s.az = m.dampz[i];
+        s.vx = 0.f; s.vz = 0.f;
+    } else {
+        s.vx = F.vx[i]; s.vz = F.vz[i];
+        s.vx_x = s.vx_z = s.vz_x = s.vz_z = s.ax = s.az = 0.f;
+    }
+    return s;
+}
+// a, b, c, e drive vx_x, vx_z, vz_z, vz_x: forward a = Dx+ txx, b = Dz- txz,
+// c = Dz+ tzz, e = Dx- txz; adjoint a = Dx+ p1, b = Dz- p2, c = Dz+ p3, e = Dx- p2.
+__device__ inline void vel_math(const Geo &g, VelCell &s, bool split, int dir, float a, float b, float c,
+                                float e) {
+    const float bu = (float)dir * s.bu;
+    if (split) {
+        const Damp d = damp_v(g, s.ax, s.az);
+        const float bx = bu * g.dtdx, bz = bu * g.dtdz;
+        s.vx_x = __builtin_fmaf(d.d1, s.vx_x, bx * a) / d.d2;
+        s.vx_z = __builtin_fmaf(d.d3, s.vx_z, bz * b) / d.d4;
+        s.vz_x = __builtin_fmaf(d.d1, s.vz_x, bx * e) / d.d2;
+        s.vz_z = __builtin_fmaf(d.d3, s.vz_z, bz * c) / d.d4;
+        s.vx = s.vx_x + s.vx_z;
+        s.vz = s.vz_x + s.vz_z;
+    } else {
+        s.vx = __builtin_fmaf(bu, __builtin_fmaf(g.dtdx, a, g.dtdz * b), s.vx);
+        s.vz = __builtin_fmaf(bu, __builtin_fmaf(g.dtdx, e, g.dtdz * c), s.vz);
+    }
+}
+__device__ inline void vel_store(const Fields &F, int i, const VelCell &s, bool split) {
+    if (split) {
+        F.vx_x[i] = s.vx_x; F.vx_z[i] = s.vx_z; F.vz_x[i] = s.vz_x; F.vz_z[i] = s.vz_z;
+    }
+    F.vx[i] = s.vx;
+    F.vz[i] = s.vz;
+}
+__device__ inline void vel_update_fast(const Geo &g, const Medium &m, const Fields &F, int i, bool split,
+                                       int dir, float a, float b, float c, float e) {
+    VelCell s = vel_load(m, F, i, split);
+    vel_math(g, s, split, dir, a, b, c, e);
+    vel_store(F, i, s, split);
+}
+
+// Own state of a stress update.
+struct StrCell {
+    float txx, tzz, txz, xx_x, xx_z, zz_x, zz_z, xz_x, xz_z, lam, mu, ax, az;
+};
+__device__ inline StrCell str_load(const Medium &m, const Fields &F, int i, bool split) {
+    StrCell s;
+    s.lam = m.lam[i];
+    s.mu = m.mu[i];
+    if (split) {
+        s.xx_x = F.txx_x[i]; s.xx_z = F.txx_z[i]; s.zz_x = F.tzz_x[i];
+        s.zz_z = F.tzz_z[i]; s.xz_x = F.txz_x[i]; s.xz_z = F.txz_z[i];
+        s.ax = m.dampx[i]; s.az = m.dampz[i];
+        s.txx = s.tzz = s.txz = 0.f;
+    } else {
+        s.txx = F.txx[i]; s.tzz = F.tzz[i]; s.txz = F.txz[i];
+        s.xx_x = s.xx_z = s.zz_x = s.zz_z = s.xz_x = s.xz_z = s.ax = s.az = 0.f;
+    }
+    return s;
+}
+template <bool HAS_SRC>
+__device__ inline void str_math(const Geo &g, StrCell &s, const VGrad &v, bool split, int dir,
+                                const BornSrc &src) {
+    const float lam = s.lam, mu = s.mu;
+    const float D = (float)dir;
+    const float l2m = __builtin_fmaf(2.0f, mu, lam);
+    const float ax = D * g.dtdx * v.xx, az = D * g.dtdz * v.zz;
+    const float gx = D * g.dtdx * v.zx, gz = D * g.dtdz * v.xz;
+    if (split) {
+        const Damp d = damp_v(g, s.ax, s.az);
+        if (HAS_SRC) {
+            s.xx_x += src.xx_x; s.xx_z += src.xx_z; s.zz_x += src.zz_x;
+            s.zz_z += src.zz_z; s.xz_x += src.xz_x; s.xz_z += src.xz_z;
+        }
+        s.xx_x = __builtin_fmaf(d.d1, s.xx_x, l2m * ax) / d.d2;
+        s.xx_z = __builtin_fmaf(d.d3, s.xx_z, lam * az) / d.d4;
+        s.zz_x = __builtin_fmaf(d.d1, s.zz_x, lam * ax) / d.d2;
+        s.zz_z = __builtin_fmaf(d.d3, s.zz_z, l2m * az) / d.d4;
+        s.xz_x = __builtin_fmaf(d.d1, s.xz_x, mu * gx) / d.d2;
+        s.xz_z = __builtin_fmaf(d.d3, s.xz_z, mu * gz) / d.d4;
+        s.txx = s.xx_x + s.xx_z;
+        s.tzz = s.zz_x + s.zz_z;
+        s.txz = s.xz_x + s.xz_z;
+    } else {
+        float sxx = __builtin_fmaf(l2m, ax, lam * az);
+        float szz = __builtin_fmaf(lam, ax, l2m * az);
+        float sxz = mu * (gx + gz);
+        if (HAS_SRC) {
+            sxx += src.xx_x + src.xx_z;
+            szz += src.zz_x + src.zz_z;
+            sxz += src.xz_x + src.xz_z;
+        }
+        s.txx += sxx;
+        s.tzz += szz;
+        s.txz += sxz;
+    }
+}
+__device__ inline void str_store(const Fields &F, int i, const StrCell &s, bool split) {
+    if (split) {
+        F.txx_x[i] = s.xx_x; F.txx_z[i] = s.xx_z; F.tzz_x[i] = s.zz_x;
+        F.tzz_z[i] = s.zz_z; F.txz_x[i] = s.xz_x; F.txz_z[i] = s.xz_z;
+    }
+    F.txx[i] = s.txx;
+    F.tzz[i] = s.tzz;
+    F.txz[i] = s.txz;
+}
+template <bool HAS_SRC>
+__device__ inline void str_update_fast(const Geo &g, const Medium &m, const Fields &F, const VGrad &v, int i,
+                                       bool split, int dir, const BornSrc &s) {
+    StrCell c = str_load(m, F, i, split);
+    str_math<HAS_SRC>(g, c, v, split, dir, s);
+    str_store(F, i, c, split);
+}
+
+// Own state of the adjoint stress-like update (txx = txx_x, tzz = tzz_z).
+struct AdjStrCell {
+    float txx, tzz, txz, xx_x, zz_z, xz_x, xz_z, ax, az;
+};
+__device__ inline AdjStrCell adj_str_load(const Medium &m, const Fields &F, int i, bool split) {
+    AdjStrCell s;
+    if (split) {
+        s.xx_x = F.txx_x[i]; s.zz_z = F.tzz_z[i]; s.xz_x = F.txz_x[i]; s.xz_z = F.txz_z[i];
+        s.ax = m.dampx[i]; s.az = m.dampz[i];
+        s.txx = s.tzz = s.txz = 0.f;
+    } else {
+        s.txx = F.txx[i]; s.tzz = F.tzz[i]; s.txz = F.txz[i];
+        s.xx_x = s.zz_z = s.xz_x = s.xz_z = s.ax = s.az = 0.f;
+    }
+    return s;
+}
+__device__ inline void adj_str_math(const Geo &g, AdjStrCell &s, const VGrad &v, bool split, int dir) {
+    const float D = (float)dir;
+    if (split) {
+        const Damp d = damp_v(g, s.ax, s.az);
+        s.xx_x = __builtin_fmaf(d.d1, s.xx_x, D * g.dtdx * v.xx) / d.d2;
+        s.zz_z = __builtin_fmaf(d.d3, s.zz_z, D * g.dtdz * v.zz) / d.d4;
+        s.xz_x = __builtin_fmaf(d.d1, s.xz_x, D * g.dtdx * v.zx) / d.d2;
+        s.xz_z = __builtin_fmaf(d.d3, s.xz_z, D * g.dtdz * v.xz) / d.d4;
+        s.txx = s.xx_x;
+        s.tzz = s.zz_z;
+        s.txz = s.xz_x + s.xz_z;
+    } else {
+        s.txx = __builtin_fmaf(D * g.dtdx, v.xx, s.txx);
+        s.tzz = __builtin_fmaf(D * g.dtdz, v.zz, s.tzz);
+        s.txz = __builtin_fmaf(D, __builtin_fmaf(g.dtdx, v.zx, g.dtdz * v.xz), s.txz);
+    }
+}
+__device__ inline void adj_str_store(const Fields &F, int i, const AdjStrCell &s, bool split) {
+    if (split) {
+        F.txx_x[i] = s.xx_x; F.tzz_z[i] = s.zz_z; F.txz_x[i] = s.xz_x; F.txz_z[i] = s.xz_z;
+    }
+    F.txx[i] = s.txx;
+    F.tzz[i] = s.tzz;
+    F.txz[i] = s.txz;
+}
+__device__ inline void adj_str_update_fast(const Geo &g, const Medium &m, const Fields &F, const VGrad &v,
+                                           int i, bool split, int dir) {
+    AdjStrCell s = adj_str_load(m, F, i, split);
+    adj_str_math(g, s, v, split, dir);
+    adj_str_store(F, i, s, split);
+}
+
+// sx = Dx+[(lam+2mu) txx + lam tzz], sz = Dz+[(lam+2mu) tzz + lam txx],
+// sxz_z = Dz-[mu txz], sxz_x = Dx-[mu txz]: the velocity update's form.
+__device__ inline void adj_vel_update_fast(const Geo &g, const Medium &m, const Fields &F, int i, bool split,
+                                           int dir, float sx, float sz, float sxz_z, float sxz_x) {
+    vel_update_fast(g, m, F, i, split, dir, sx, sxz_z, sz, sxz_x);
+}
+
+// Products whose derivatives the adjoint velocity update takes.
+__device__ inline float adj_p1(float lam, float mu, float txx, float tzz) {
+    return __builtin_fmaf(__builtin_fmaf(2.0f, mu, lam), txx, lam * tzz);
+}
+
+// Zero-lag imaging of one interior cell (q = its index in the per-shot image).
+__device__ inline void image_fast(const DevShot &d, int i, int q) {
+    const float gx = d.dgx[i], gz = d.dgz[i], gxz = d.dgxz[i], gzx = d.dgzx[i];
+    const float sxx = d.bk.txx[i], szz = d.bk.tzz[i], sxz = d.bk.txz[i];
+    const float vx = d.fw.vx[i], vz = d.fw.vz[i];
+    const float lg = -((gx + gz) * (sxx + szz));
+    const float mg = __builtin_fmaf(-(gxz + gzx), sxz, -2.0f * __builtin_fmaf(gx, sxx, gz * szz));
+    d.gvp[q] = __builtin_fmaf(d.m.icp[i], lg, d.gvp[q]);
+    d.gvs[q] = __builtin_fmaf(d.m.ics[i], __builtin_fmaf(-2.0f, lg, mg), d.gvs[q]);
+    d.gil[q] = __builtin_fmaf(vx, vx, __builtin_fmaf(vz, vz, d.gil[q]));
+}
+
 // Stress update (sfwi-stage1/born.cu:399-440, update_txx_tzz_txz); with
 // a Born source the terms are added to the split parts first
 // (sfwi-stage2/FD2DGPU.cu:1839-1840 order: source, then update).
@@ -267,42 +467,7 @@ __device__ inline void str_cell(const Geo &g, const Medium &m, const Fields &F, 
         F.tzz[i] = zz_x + zz_z;
         F.txz[i] = xz_x + xz_z;
     } else {
-        const float D = (float)dir;
-        const float l2m = __builtin_fmaf(2.0f, mu, lam);
-        const float ax = D * g.dtdx * v.xx, az = D * g.dtdz * v.zz;
-        const float gx = D * g.dtdx * v.zx, gz = D * g.dtdz * v.xz;
-        if (split) {
-            const Damp d = damp_of(g, m, i);
-            float xx_x = F.txx_x[i], xx_z = F.txx_z[i], zz_x = F.tzz_x[i], zz_z = F.tzz_z[i];
-            float xz_x = F.txz_x[i], xz_z = F.txz_z[i];
-            if (HAS_SRC) {
-                xx_x += s.xx_x; xx_z += s.xx_z; zz_x += s.zz_x;
-                zz_z += s.zz_z; xz_x += s.xz_x; xz_z += s.xz_z;
-            }
-            xx_x = __builtin_fmaf(d.d1, xx_x, l2m * ax) / d.d2;
-            xx_z = __builtin_fmaf(d.d3, xx_z, lam * az) / d.d4;
-            zz_x = __builtin_fmaf(d.d1, zz_x, lam * ax) / d.d2;
-            zz_z = __builtin_fmaf(d.d3, zz_z, l2m * az) / d.d4;
-            xz_x = __builtin_fmaf(d.d1, xz_x, mu * gx) / d.d2;
-            xz_z = __builtin_fmaf(d.d3, xz_z, mu * gz) / d.d4;
-            F.txx_x[i] = xx_x; F.txx_z[i] = xx_z; F.tzz_x[i] = zz_x;
-            F.tzz_z[i] = zz_z; F.txz_x[i] = xz_x; F.txz_z[i] = xz_z;
-            F.txx[i] = xx_x + xx_z;
-            F.tzz[i] = zz_x + zz_z;
-            F.txz[i] = xz_x + xz_z;
-        } else {
-            float sxx = __builtin_fmaf(l2m, ax, lam * az);
-            float szz = __builtin_fmaf(lam, ax, l2m * az);
-            float sxz = mu * (gx + gz);
-            if (HAS_SRC) {
-                sxx += s.xx_x + s.xx_z;
-                szz += s.zz_x + s.zz_z;
-                sxz += s.xz_x + s.xz_z;
-            }
-            F.txx[i] += sxx;
-            F.tzz[i] += szz;
-            F.txz[i] += sxz;
-        }
+        str_update_fast<HAS_SRC>(g, m, F, v, i, split, dir, s);
     }
 }
 
@@ -312,21 +477,13 @@ template <bool EXACT>
 __device__ inline void adj_str_cell(const Geo &g, const Medium &m, const Fields &F, int i, bool split,
                                     int dir) {
     const VGrad v = vgrad<EXACT>(g, F, i);
-    const float D = (float)dir;
-    if (EXACT || split) {
+    if (EXACT) {
+        const float D = (float)dir;
         const Damp d = damp_of(g, m, i);
-        float xx, zz, xzx, xzz;
-        if (EXACT) {
-            xx = (d.d1 * F.txx_x[i] + D * g.dtdx * v.xx) / d.d2;
-            zz = (d.d3 * F.tzz_z[i] + D * g.dtdz * v.zz) / d.d4;
-            xzx = (d.d1 * F.txz_x[i] + D * g.dtdx * v.zx) / d.d2;
-            xzz = (d.d3 * F.txz_z[i] + D * g.dtdz * v.xz) / d.d4;
-        } else {
-            xx = __builtin_fmaf(d.d1, F.txx_x[i], D * g.dtdx * v.xx) / d.d2;
-            zz = __builtin_fmaf(d.d3, F.tzz_z[i], D * g.dtdz * v.zz) / d.d4;
-            xzx = __builtin_fmaf(d.d1, F.txz_x[i], D * g.dtdx * v.zx) / d.d2;
-            xzz = __builtin_fmaf(d.d3, F.txz_z[i], D * g.dtdz * v.xz) / d.d4;
-        }
+        const float xx = (d.d1 * F.txx_x[i] + D * g.dtdx * v.xx) / d.d2;
+        const float zz = (d.d3 * F.tzz_z[i] + D * g.dtdz * v.zz) / d.d4;
+        const float xzx = (d.d1 * F.txz_x[i] + D * g.dtdx * v.zx) / d.d2;
+        const float xzz = (d.d3 * F.txz_z[i] + D * g.dtdz * v.xz) / d.d4;
         F.txx_x[i] = xx;
         F.tzz_z[i] = zz;
         F.txz_x[i] = xzx;
@@ -335,9 +492,7 @@ __device__ inline void adj_str_cell(const Geo &g, const Medium &m, const Fields 
         F.tzz[i] = zz;
         F.txz[i] = xzx + xzz;
     } else {
-        F.txx[i] = __builtin_fmaf(D * g.dtdx, v.xx, F.txx[i]);
-        F.tzz[i] = __builtin_fmaf(D * g.dtdz, v.zz, F.tzz[i]);
-        F.txz[i] = __builtin_fmaf(D, __builtin_fmaf(g.dtdx, v.zx, g.dtdz * v.xz), F.txz[i]);
+        adj_str_update_fast(g, m, F, v, i, split, dir);
     }
 }
 
@@ -390,34 +545,17 @@ __device__ inline void adj_vel_cell(const Geo &g, const Medium &m, const Fields 
             const int za = i + k, zb = i - (k - 1);
             const int wa = i + (k - 1), wb = i - k;
             const float la = lam[pa], lb = lam[pb];
-            const float fa = __builtin_fmaf(__builtin_fmaf(2.0f, mu[pa], la), txx[pa], la * tzz[pa]);
-            const float fb = __builtin_fmaf(__builtin_fmaf(2.0f, mu[pb], lb), txx[pb], lb * tzz[pb]);
+            const float fa = adj_p1(la, mu[pa], txx[pa], tzz[pa]);
+            const float fb = adj_p1(lb, mu[pb], txx[pb], tzz[pb]);
             sx = __builtin_fmaf(kC[k], fa - fb, sx);
             const float lza = lam[za], lzb = lam[zb];
-            const float ga = __builtin_fmaf(__builtin_fmaf(2.0f, mu[za], lza), tzz[za], lza * txx[za]);
-            const float gb = __builtin_fmaf(__builtin_fmaf(2.0f, mu[zb], lzb), tzz[zb], lzb * txx[zb]);
+            const float ga = adj_p1(lza, mu[za], tzz[za], txx[za]);
+            const float gb = adj_p1(lzb, mu[zb], tzz[zb], txx[zb]);
             sz = __builtin_fmaf(kC[k], ga - gb, sz);
             sxz_z = __builtin_fmaf(kC[k], mu[wa] * txz[wa] - mu[wb] * txz[wb], sxz_z);
             sxz_x = __builtin_fmaf(kC[k], mu[qa] * txz[qa] - mu[qb] * txz[qb], sxz_x);
         }
-        const float bu = (float)dir * m.bu[i];
-        if (split) {
-            const Damp d = damp_of(g, m, i);
-            const float bx = bu * g.dtdx, bz = bu * g.dtdz;
-            const float vxx = __builtin_fmaf(d.d1, F.vx_x[i], bx * sx) / d.d2;
-            const float vxz = __builtin_fmaf(d.d3, F.vx_z[i], bz * sxz_z) / d.d4;
-            const float vzx = __builtin_fmaf(d.d1, F.vz_x[i], bx * sxz_x) / d.d2;
-            const float vzz = __builtin_fmaf(d.d3, F.vz_z[i], bz * sz) / d.d4;
-            F.vx_x[i] = vxx;
-            F.vx_z[i] = vxz;
-            F.vz_x[i] = vzx;
-            F.vz_z[i] = vzz;
-            F.vz[i] = vzx + vzz;
-            F.vx[i] = vxx + vxz;
-        } else {
-            F.vx[i] = __builtin_fmaf(bu, __builtin_fmaf(g.dtdx, sx, g.dtdz * sxz_z), F.vx[i]);
-            F.vz[i] = __builtin_fmaf(bu, __builtin_fmaf(g.dtdx, sxz_x, g.dtdz * sz), F.vz[i]);
-        }
+        adj_vel_update_fast(g, m, F, i, split, dir, sx, sz, sxz_z, sxz_x);
     }
 }
 
@@ -652,11 +790,7 @@ __global__ void __launch_bounds__(BLK) k_adj_vel_image(Geo g, const DevShot *__r
         d.gvs[q] += 1.0f * ps;
         d.gil[q] += 1.0f * (vx * vx + vz * vz);
     } else {
-        const float lg = -((gx + gz) * (sxx + szz));
-        const float mg = __builtin_fmaf(-(gxz + gzx), sxz, -2.0f * __builtin_fmaf(gx, sxx, gz * szz));
-        d.gvp[q] = __builtin_fmaf(d.m.icp[i], lg, d.gvp[q]);
-        d.gvs[q] = __builtin_fmaf(d.m.ics[i], __builtin_fmaf(-2.0f, lg, mg), d.gvs[q]);
-        d.gil[q] = __builtin_fmaf(vx, vx, __builtin_fmaf(vz, vz, d.gil[q]));
+        image_fast(d, i, q);
     }
 }
 

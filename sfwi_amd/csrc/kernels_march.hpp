@@ -1,0 +1,371 @@
+// Column-march stencil kernels of the FAST mode.
+//
+// Each wave64 owns a band of 64 consecutive rows (z) of one shot -- 56 output
+// rows plus a 4-row halo on each side -- and marches along x over a segment
+// of columns.  Per field with x taps it keeps an eight-column sliding window
+// in VGPRs and loads one new column per step; the z taps are formed from the
+// lane's own value with DPP whole-wave shifts (v_mov_dpp wave_shl/wave_shr,
+// plain VALU moves, no LDS).  The march is software-pipelined: every operand
+// of column c+1 (new window columns and the cell's own state) is loaded while
+// column c is computed, so memory latency overlaps the arithmetic.  Each
+// output cell then costs about one coalesced wave load per operand, against
+// sixteen per stencil field in the per-cell kernels whose limit is the
+// texture-address unit.  Waves are independent (no LDS, no barriers);
+// workgroups are spread so that each XCD owns a contiguous range of column
+// segments, keeping the medium shared by a batch's shots in that XCD's L2.
+//
+// The segment width is a runtime choice (Geo::cm_xs); the per-cell update
+// math is the FAST-mode code of kernels.hpp (load / math / store), so every
+// FAST kernel family produces identical results.
+#pragma once
+#include "kernels.hpp"
+
+namespace sfwi {
+
+constexpr int CM_ROWS = 56;   // output rows per wave (64 lanes minus 2 x 4 halo)
+
+struct Seg {
+    int x0, x1, zb;   // columns [x0, x1), first output row zb
+};
+
+__device__ inline bool seg_of(const Geo &g, Seg &s) {
+    const int b = blockIdx.x;
+    const int L = (b & 7) * g.cm_per_xcd + (b >> 3);
+    const int W = L * (BLK / 64) + (int)(threadIdx.x >> 6);
+    if (W >= g.cm_waves) return false;
+    const int seg = W / g.cm_nbands;
+    const int band = W - seg * g.cm_nbands;
+    s.x0 = NOP + seg * g.cm_xs;
+    s.x1 = min(s.x0 + g.cm_xs, g.nxp - NOP);
+    s.zb = NOP + band * CM_ROWS;
+    return true;
+}
+
+// Value of lane l+1 / l-1 (whole-wave DPP shifts; the end lanes get 0 and
+// belong to the halo, whose results are discarded).
+__device__ inline float nb_up(float v) { return __builtin_amdgcn_update_dpp(0.0f, v, 0x130, 0xf, 0xf, false); }
+__device__ inline float nb_dn(float v) { return __builtin_amdgcn_update_dpp(0.0f, v, 0x138, 0xf, 0xf, false); }
+
+//   z_plus : sum_k c_k (f[z+k]   - f[z-k+1])
+//   z_minus: sum_k c_k (f[z+k-1] - f[z-k])
+__device__ inline float z_plus(float f) {
+    const float p1 = nb_up(f), p2 = nb_up(p1), p3 = nb_up(p2), p4 = nb_up(p3);
+    const float m1 = nb_dn(f), m2 = nb_dn(m1), m3 = nb_dn(m2);
+    float t = kC[1] * (p1 - f);
+    t = __builtin_fmaf(kC[2], p2 - m1, t);
+    t = __builtin_fmaf(kC[3], p3 - m2, t);
+    t = __builtin_fmaf(kC[4], p4 - m3, t);
+    return t;
+}
+__device__ inline float z_minus(float f) {
+    const float p1 = nb_up(f), p2 = nb_up(p1), p3 = nb_up(p2);
+    const float m1 = nb_dn(f), m2 = nb_dn(m1), m3 = nb_dn(m2), m4 = nb_dn(m3);
+    float t = kC[1] * (f - m1);
+    t = __builtin_fmaf(kC[2], p1 - m2, t);
+    t = __builtin_fmaf(kC[3], p2 - m3, t);
+    t = __builtin_fmaf(kC[4], p3 - m4, t);
+    return t;
+}
+
+// x stencils from an 8-column window; both plus (w[j] = f(c-3+j)) and minus
+// (w[j] = f(c-4+j)) take the form sum_k c_k (w[3+k] - w[4-k]).
+__device__ inline float x_win(const float *w) {
+    float t = kC[1] * (w[4] - w[3]);
+    t = __builtin_fmaf(kC[2], w[5] - w[2], t);
+    t = __builtin_fmaf(kC[3], w[6] - w[1], t);
+    t = __builtin_fmaf(kC[4], w[7] - w[0], t);
+    return t;
+}
+__device__ inline void shift(float *w) {
+#pragma unroll
+    for (int j = 0; j < 7; ++j) w[j] = w[j + 1];
+}
+
+struct Lane {
+    int z;
+    bool zin, act;
+};
+__device__ inline Lane lane_of(const Geo &g, const Seg &s) {
+    const int lane = threadIdx.x & 63;
+    Lane l;
+    l.z = s.zb - 4 + lane;
+    l.zin = l.z < g.nzp;
+    l.act = lane >= 4 && lane < 60 && l.z < g.nzp - NOP;
+    return l;
+}
+__device__ inline float ld(const float *__restrict__ f, const Geo &g, const Lane &l, int x) {
+    return l.zin ? f[x * g.nzp + l.z] : 0.0f;
+}
+
+// Velocity half step (SAVE: store strips of the new velocity; BORN: also the
+// scattered field).  PD columns of operands are in flight ahead of the one
+// being computed (a ring indexed at compile time: the march is unrolled by PD).
+template <bool SAVE, bool BORN, int PD>
+__global__ void __launch_bounds__(BLK) km_vel(Geo g, const DevShot *__restrict__ shots, int it, int dir) {
+    Seg s;
+    if (!seg_of(g, s)) return;
+    const DevShot &d = shots[blockIdx.y];
+    const Lane l = lane_of(g, s);
+    constexpr int NF = BORN ? 2 : 1;
+    float wxx[NF][8], wxz[NF][8];
+    float qxx[NF][PD], qxz[NF][PD], qzz[NF][PD];
+    VelCell qst[NF][PD];
+    auto fetch = [&](int f, int j, int c) {   // operands of column c into ring slot j
+        const Fields &F = f ? d.bn : d.fw;
+        qxx[f][j] = ld(F.txx, g, l, c + 4);
+        qxz[f][j] = ld(F.txz, g, l, c + 3);
+        qzz[f][j] = ld(F.tzz, g, l, c);
+        qst[f][j] = l.act ? vel_load(d.m, F, c * g.nzp + l.z, in_band(g, c, l.z)) : VelCell{};
+    };
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        const Fields &F = f ? d.bn : d.fw;
+#pragma unroll
+        for (int j = 0; j < 7; ++j) {
+            wxx[f][j] = ld(F.txx, g, l, s.x0 - 3 + j);
+            wxz[f][j] = ld(F.txz, g, l, s.x0 - 4 + j);
+        }
+#pragma unroll
+        for (int j = 0; j < PD; ++j)
+            if (s.x0 + j < s.x1) fetch(f, j, s.x0 + j);
+    }
+    for (int c0 = s.x0; c0 < s.x1; c0 += PD) {
+#pragma unroll
+        for (int j = 0; j < PD; ++j) {
+            const int c = c0 + j;
+            if (c >= s.x1) break;
+            const bool split = in_band(g, c, l.z);
+            const int i = c * g.nzp + l.z;
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                const Fields &F = f ? d.bn : d.fw;
+                wxx[f][7] = qxx[f][j];
+                wxz[f][7] = qxz[f][j];
+                const float tzz = qzz[f][j];
+                VelCell st = qst[f][j];
+                if (c + PD < s.x1) fetch(f, j, c + PD);
+                const float a = x_win(wxx[f]);        // Dx+ txx
+                const float e = x_win(wxz[f]);        // Dx- txz
+                const float cz = z_plus(tzz);         // Dz+ tzz
+                const float bz = z_minus(wxz[f][4]);  // Dz- txz (txz at column c)
+                shift(wxx[f]);
+                shift(wxz[f]);
+                if (l.act) {
+                    vel_math(g, st, split, dir, a, bz, cz, e);
+                    vel_store(F, i, st, split);
+                    if (SAVE && f == 0 && split) {
+                        int sl[4];
+                        strip_slots(g, c, l.z, sl);
+                        float *base = d.strip + (size_t)it * 2 * g.sf;
+#pragma unroll
+                        for (int k = 0; k < 4; ++k)
+                            if (sl[k] >= 0) {
+                                base[sl[k]] = st.vx;
+                                base[g.sf + sl[k]] = st.vz;
+                            }
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Stress half step (GRAD: store the velocity derivatives; BORN: also the
+// scattered stresses with the Born source of the background velocity).
+template <bool GRAD, bool BORN, int PD>
+__global__ void __launch_bounds__(BLK) km_str(Geo g, const DevShot *__restrict__ shots, int dir) {
+    Seg s;
+    if (!seg_of(g, s)) return;
+    const DevShot &d = shots[blockIdx.y];
+    const Lane l = lane_of(g, s);
+    constexpr int NF = BORN ? 2 : 1;
+    float wvx[NF][8], wvz[NF][8];   // vx: columns c-4.., vz: columns c-3..
+    float qvx[NF][PD], qvz[NF][PD];
+    StrCell qst[NF][PD];
+    auto split_of = [&](int f, int c) {
+        const bool band = in_band(g, c, l.z);
+        return f ? band : (band || (c == d.isx && l.z == d.isz));
+    };
+    auto fetch = [&](int f, int j, int c) {
+        const Fields &F = f ? d.bn : d.fw;
+        qvx[f][j] = ld(F.vx, g, l, c + 3);
+        qvz[f][j] = ld(F.vz, g, l, c + 4);
+        qst[f][j] = l.act ? str_load(d.m, F, c * g.nzp + l.z, split_of(f, c)) : StrCell{};
+    };
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        const Fields &F = f ? d.bn : d.fw;
+#pragma unroll
+        for (int j = 0; j < 7; ++j) {
+            wvx[f][j] = ld(F.vx, g, l, s.x0 - 4 + j);
+            wvz[f][j] = ld(F.vz, g, l, s.x0 - 3 + j);
+        }
+#pragma unroll
+        for (int j = 0; j < PD; ++j)
+            if (s.x0 + j < s.x1) fetch(f, j, s.x0 + j);
+    }
+    const BornSrc none{};
+    for (int c0 = s.x0; c0 < s.x1; c0 += PD) {
+#pragma unroll
+        for (int j = 0; j < PD; ++j) {
+            const int c = c0 + j;
+            if (c >= s.x1) break;
+            const int i = c * g.nzp + l.z;
+            VGrad v[NF];
+            StrCell st[NF];
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                wvx[f][7] = qvx[f][j];
+                wvz[f][7] = qvz[f][j];
+                st[f] = qst[f][j];
+                if (c + PD < s.x1) fetch(f, j, c + PD);
+                v[f].xx = x_win(wvx[f]);            // Dx- vx
+                v[f].zx = x_win(wvz[f]);            // Dx+ vz
+                v[f].zz = z_minus(wvz[f][3]);       // Dz- vz (vz at column c)
+                v[f].xz = z_plus(wvx[f][4]);        // Dz+ vx (vx at column c)
+                shift(wvx[f]);
+                shift(wvz[f]);
+            }
+            if (l.act) {
+                const bool sp = split_of(0, c);
+                str_math<false>(g, st[0], v[0], sp, dir, none);
+                str_store(d.fw, i, st[0], sp);
+                if (GRAD) {
+                    d.dgx[i] = v[0].xx * g.idx;
+                    d.dgz[i] = v[0].zz * g.idz;
+                    d.dgxz[i] = v[0].xz * g.idz;
+                    d.dgzx[i] = v[0].zx * g.idx;
+                }
+                if (BORN) {
+                    const bool band = split_of(1, c);
+                    const BornSrc src = born_src<false>(g, d.m, v[0], i, dir);
+                    str_math<true>(g, st[NF - 1], v[NF - 1], band, dir, src);
+                    str_store(d.bn, i, st[NF - 1], band);
+                }
+            }
+        }
+    }
+}
+
+// Adjoint stress-like step.
+template <int PD>
+__global__ void __launch_bounds__(BLK) km_adj_str(Geo g, const DevShot *__restrict__ shots, int dir) {
+    Seg s;
+    if (!seg_of(g, s)) return;
+    const DevShot &d = shots[blockIdx.y];
+    const Lane l = lane_of(g, s);
+    const Fields &F = d.bk;
+    float wvx[8], wvz[8], qvx[PD], qvz[PD];
+    AdjStrCell qst[PD];
+    auto fetch = [&](int j, int c) {
+        qvx[j] = ld(F.vx, g, l, c + 3);
+        qvz[j] = ld(F.vz, g, l, c + 4);
+        qst[j] = l.act ? adj_str_load(d.m, F, c * g.nzp + l.z, in_band(g, c, l.z)) : AdjStrCell{};
+    };
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        wvx[j] = ld(F.vx, g, l, s.x0 - 4 + j);
+        wvz[j] = ld(F.vz, g, l, s.x0 - 3 + j);
+    }
+#pragma unroll
+    for (int j = 0; j < PD; ++j)
+        if (s.x0 + j < s.x1) fetch(j, s.x0 + j);
+    for (int c0 = s.x0; c0 < s.x1; c0 += PD) {
+#pragma unroll
+        for (int j = 0; j < PD; ++j) {
+            const int c = c0 + j;
+            if (c >= s.x1) break;
+            const bool split = in_band(g, c, l.z);
+            const int i = c * g.nzp + l.z;
+            wvx[7] = qvx[j];
+            wvz[7] = qvz[j];
+            AdjStrCell st = qst[j];
+            if (c + PD < s.x1) fetch(j, c + PD);
+            VGrad v;
+            v.xx = x_win(wvx);
+            v.zx = x_win(wvz);
+            v.zz = z_minus(wvz[3]);
+            v.xz = z_plus(wvx[4]);
+            shift(wvx);
+            shift(wvz);
+            if (l.act) {
+                adj_str_math(g, st, v, split, dir);
+                adj_str_store(F, i, st, split);
+            }
+        }
+    }
+}
+
+// Adjoint velocity step with the moduli inside the derivatives, fused with
+// the zero-lag imaging.
+template <int PD>
+__global__ void __launch_bounds__(BLK) km_adj_vel_image(Geo g, const DevShot *__restrict__ shots, int dir) {
+    Seg s;
+    if (!seg_of(g, s)) return;
+    const DevShot &d = shots[blockIdx.y];
+    const Lane l = lane_of(g, s);
+    const Fields &F = d.bk;
+    const Medium &m = d.m;
+    // p1 = (lam+2mu) txx + lam tzz on columns c-3..c+4; p2 = mu txz on c-4..c+3;
+    // p3 = (lam+2mu) tzz + lam txx at column c.
+    auto P1 = [&](int x) {
+        if (!l.zin) return 0.0f;
+        const int i = x * g.nzp + l.z;
+        return adj_p1(m.lam[i], m.mu[i], F.txx[i], F.tzz[i]);
+    };
+    auto P2 = [&](int x) {
+        if (!l.zin) return 0.0f;
+        const int i = x * g.nzp + l.z;
+        return m.mu[i] * F.txz[i];
+    };
+    auto P3 = [&](int x) {
+        if (!l.zin) return 0.0f;
+        const int i = x * g.nzp + l.z;
+        return adj_p1(m.lam[i], m.mu[i], F.tzz[i], F.txx[i]);
+    };
+    auto split_of = [&](int c) { return in_band(g, c, l.z) || l.z == d.igz; };
+    float w1[8], w2[8], q1[PD], q2[PD], q3[PD];
+    VelCell qst[PD];
+    auto fetch = [&](int j, int c) {
+        q1[j] = P1(c + 4);
+        q2[j] = P2(c + 3);
+        q3[j] = P3(c);
+        qst[j] = l.act ? vel_load(m, F, c * g.nzp + l.z, split_of(c)) : VelCell{};
+    };
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        w1[j] = P1(s.x0 - 3 + j);
+        w2[j] = P2(s.x0 - 4 + j);
+    }
+#pragma unroll
+    for (int j = 0; j < PD; ++j)
+        if (s.x0 + j < s.x1) fetch(j, s.x0 + j);
+    for (int c0 = s.x0; c0 < s.x1; c0 += PD) {
+#pragma unroll
+        for (int j = 0; j < PD; ++j) {
+            const int c = c0 + j;
+            if (c >= s.x1) break;
+            const bool split = split_of(c);
+            const int i = c * g.nzp + l.z;
+            w1[7] = q1[j];
+            w2[7] = q2[j];
+            const float p3 = q3[j];
+            VelCell st = qst[j];
+            if (c + PD < s.x1) fetch(j, c + PD);
+            const float sx = x_win(w1);          // Dx+ p1
+            const float sxz_x = x_win(w2);       // Dx- p2
+            const float sz = z_plus(p3);         // Dz+ p3
+            const float sxz_z = z_minus(w2[4]);  // Dz- p2 (p2 at column c)
+            shift(w1);
+            shift(w2);
+            if (l.act) {
+                vel_math(g, st, split, dir, sx, sxz_z, sz, sxz_x);
+                vel_store(F, i, st, split);
+                const int jx = c - g.pml, jz = l.z - g.pml;
+                if (jx >= 0 && jx < g.nx && jz >= 0 && jz < g.nz) image_fast(d, i, jx * g.nz + jz);
+            }
+        }
+    }
+}
+
+}  // namespace sfwi

@@ -8,6 +8,7 @@
 // once per window position and shared by every shot that uses it.
 #include "sfwi_abi.h"
 #include "kernels.hpp"
+#include "kernels_march.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -317,7 +318,7 @@ int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium 
 
 // ------------------------------------------------------------- solves ---
 
-template <bool E>
+template <bool E, int PD>
 int run_batch(sfwi_ctx *c, Kind kind, int n) {
     const Geo &g = c->g;
     const int nt = g.nt;
@@ -330,20 +331,37 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
     hipStream_t st = c->stream;
     const DevShot *D = c->d_shots;
 
+    const dim3 tiles((unsigned)(8 * g.cm_per_xcd), (unsigned)n);
+    const bool percell = E || c->p.march_cols < 0;   // per-cell kernels on request (march_cols < 0)
     for (int it = 0; it < nt; ++it) {
         hipLaunchKernelGGL(k_source, src1, dim3(64), 0, st, g, D, c->wav, it, n);
         int t = tbeg(c, SFWI_K_FWD_VEL, it);
-        if (born) {
-            if (grad) hipLaunchKernelGGL((k_vel<E, true, true>), cells, blk, 0, st, g, D, it, 1);
-            else hipLaunchKernelGGL((k_vel<E, false, true>), cells, blk, 0, st, g, D, it, 1);
+        if (percell) {
+            if (born) {
+                if (grad) hipLaunchKernelGGL((k_vel<E, true, true>), cells, blk, 0, st, g, D, it, 1);
+                else hipLaunchKernelGGL((k_vel<E, false, true>), cells, blk, 0, st, g, D, it, 1);
+            } else {
+                if (grad) hipLaunchKernelGGL((k_vel<E, true, false>), cells, blk, 0, st, g, D, it, 1);
+                else hipLaunchKernelGGL((k_vel<E, false, false>), cells, blk, 0, st, g, D, it, 1);
+            }
         } else {
-            if (grad) hipLaunchKernelGGL((k_vel<E, true, false>), cells, blk, 0, st, g, D, it, 1);
-            else hipLaunchKernelGGL((k_vel<E, false, false>), cells, blk, 0, st, g, D, it, 1);
+            if (born) {
+                if (grad) hipLaunchKernelGGL((km_vel<true, true, PD>), tiles, blk, 0, st, g, D, it, 1);
+                else hipLaunchKernelGGL((km_vel<false, true, PD>), tiles, blk, 0, st, g, D, it, 1);
+            } else {
+                if (grad) hipLaunchKernelGGL((km_vel<true, false, PD>), tiles, blk, 0, st, g, D, it, 1);
+                else hipLaunchKernelGGL((km_vel<false, false, PD>), tiles, blk, 0, st, g, D, it, 1);
+            }
         }
         tend(c, t);
         t = tbeg(c, SFWI_K_FWD_STR, it);
-        if (born) hipLaunchKernelGGL((k_str<E, false, true>), cells, blk, 0, st, g, D, 1);
-        else hipLaunchKernelGGL((k_str<E, false, false>), cells, blk, 0, st, g, D, 1);
+        if (percell) {
+            if (born) hipLaunchKernelGGL((k_str<E, false, true>), cells, blk, 0, st, g, D, 1);
+            else hipLaunchKernelGGL((k_str<E, false, false>), cells, blk, 0, st, g, D, 1);
+        } else {
+            if (born) hipLaunchKernelGGL((km_str<false, true, PD>), tiles, blk, 0, st, g, D, 1);
+            else hipLaunchKernelGGL((km_str<false, false, PD>), tiles, blk, 0, st, g, D, 1);
+        }
         tend(c, t);
         if (born) hipLaunchKernelGGL(k_record<true>, trg, dim3(64), 0, st, g, D, it);
         else hipLaunchKernelGGL(k_record<false>, trg, dim3(64), 0, st, g, D, it);
@@ -364,17 +382,21 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
     for (int it = nt - 1; it >= 0; --it) {
         hipLaunchKernelGGL(k_strip_load, stripg, blk, 0, st, g, D, it);
         int t = tbeg(c, SFWI_K_REC_STR, it);
-        hipLaunchKernelGGL((k_str<E, true, false>), cells, blk, 0, st, g, D, -1);
+        if (percell) hipLaunchKernelGGL((k_str<E, true, false>), cells, blk, 0, st, g, D, -1);
+        else hipLaunchKernelGGL((km_str<true, false, PD>), tiles, blk, 0, st, g, D, -1);
         tend(c, t);
         t = tbeg(c, SFWI_K_REC_VEL, it);
-        hipLaunchKernelGGL((k_vel<E, false, false>), cells, blk, 0, st, g, D, it, -1);
+        if (percell) hipLaunchKernelGGL((k_vel<E, false, false>), cells, blk, 0, st, g, D, it, -1);
+        else hipLaunchKernelGGL((km_vel<false, false, PD>), tiles, blk, 0, st, g, D, it, -1);
         tend(c, t);
         hipLaunchKernelGGL(k_inject, trg, dim3(64), 0, st, g, D, it);
         t = tbeg(c, SFWI_K_ADJ_STR, it);
-        hipLaunchKernelGGL(k_adj_str<E>, cells, blk, 0, st, g, D, adir);
+        if (percell) hipLaunchKernelGGL(k_adj_str<E>, cells, blk, 0, st, g, D, adir);
+        else hipLaunchKernelGGL(km_adj_str<PD>, tiles, blk, 0, st, g, D, adir);
         tend(c, t);
         t = tbeg(c, SFWI_K_ADJ_VEL, it);
-        hipLaunchKernelGGL(k_adj_vel_image<E>, cells, blk, 0, st, g, D, adir);
+        if (percell) hipLaunchKernelGGL(k_adj_vel_image<E>, cells, blk, 0, st, g, D, adir);
+        else hipLaunchKernelGGL(km_adj_vel_image<PD>, tiles, blk, 0, st, g, D, adir);
         tend(c, t);
     }
     const dim3 mg((unsigned)((c->nint + 255) / 256));
@@ -384,7 +406,13 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
 }
 
 int run(sfwi_ctx *c, Kind kind, int n) {
-    return c->p.mode == SFWI_MODE_EXACT ? run_batch<true>(c, kind, n) : run_batch<false>(c, kind, n);
+    if (c->p.mode == SFWI_MODE_EXACT) return run_batch<true, 1>(c, kind, n);
+    switch (c->p.march_depth) {
+    case 1: return run_batch<false, 1>(c, kind, n);
+    case 2: return run_batch<false, 2>(c, kind, n);
+    case 4: return run_batch<false, 4>(c, kind, n);
+    default: return run_batch<false, 1>(c, kind, n);
+    }
 }
 
 int finish(sfwi_ctx *c) {
@@ -459,6 +487,10 @@ int sfwi_ctx_create(const sfwi_params *p, sfwi_ctx **out) {
     g.ndom = (g.nxp - 2 * NOP) * g.nzd;
     const int nb = (g.ndom + BLK - 1) / BLK;
     g.per_xcd = (nb + 7) / 8;
+    g.cm_xs = p->march_cols > 0 ? p->march_cols : 8;
+    g.cm_nbands = (g.nzd + CM_ROWS - 1) / CM_ROWS;
+    g.cm_waves = ((g.nxp - 2 * NOP + g.cm_xs - 1) / g.cm_xs) * g.cm_nbands;
+    g.cm_per_xcd = ((g.cm_waves + BLK / 64 - 1) / (BLK / 64) + 7) / 8;
     g.sf = 4 * NSTRIP * (p->nx + p->nz);
     g.inv_nzd = 1.0f / (float)g.nzd;
     g.dt = p->dt; g.dx = p->dx; g.dz = p->dz;

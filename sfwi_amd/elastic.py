@@ -43,7 +43,7 @@ class Context:
     """One device context (``sfwi_ctx``)."""
 
     def __init__(self, *, dx, dz, dt, nx, nz, pml, nt, ntr, allnx, allnz, rr=0.001, rho_const=0.0,
-                 adjoint_dir=1, max_batch=1, mode="fast", storage="f32", use_graph=False):
+                 adjoint_dir=1, max_batch=1, mode="fast", storage="f32", use_graph=False, march_cols=0, march_depth=0):
         self.L = _lib.load()
         p = SfwiParams()
         p.dx, p.dz, p.dt = dx, dz, dt
@@ -54,6 +54,8 @@ class Context:
         p.storage = {"f32": _lib.STORE_STRIPS_F32, "f16": _lib.STORE_STRIPS_F16,
                      "bf16": _lib.STORE_STRIPS_BF16}[storage]
         p.use_graph = int(bool(use_graph))
+        p.march_cols = int(march_cols)
+        p.march_depth = int(march_depth)
         self.p = p
         h = ctypes.c_void_p()
         rc = self.L.sfwi_ctx_create(ctypes.byref(p), ctypes.byref(h))
