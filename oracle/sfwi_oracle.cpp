@@ -801,8 +801,16 @@ void or_propagate_f64(const or_config *c, const float *vp, const float *vs, cons
         } else if (kind == 1) {
             update_s(r.g, r.st, r.m, w, dir, r.nth);
             update_v(r.g, r.st, r.m, w, dir, r.nth);
-        } else {
+        } else if (kind == 2) {
             adjoint_s(r.g, r.st, r.m, w, dir, r.nth);
+            adjoint_v(r.g, r.st, r.m, w, dir, r.nth);
+        } else if (kind == 3) {
+            update_v(r.g, r.st, r.m, w, dir, r.nth);
+        } else if (kind == 4) {
+            update_s(r.g, r.st, r.m, w, dir, r.nth);
+        } else if (kind == 5) {
+            adjoint_s(r.g, r.st, r.m, w, dir, r.nth);
+        } else {
             adjoint_v(r.g, r.st, r.m, w, dir, r.nth);
         }
     }

@@ -98,7 +98,9 @@ void or_linear_alpha_f32(const or_config *c, const float *vp, const float *vs, c
  * vx, vz, txx, tzz, txz, vx_x, vx_z, vz_x, vz_z, txx_x, txx_z, tzz_x, tzz_z, txz_x, txz_z.
  * kind 0: forward steps (v then sigma) with direction dir;
  * kind 1: reconstruction steps (sigma then v) with direction dir;
- * kind 2: adjoint steps (adjoint sigma then adjoint v) with direction dir. */
+ * kind 2: adjoint steps (adjoint sigma then adjoint v) with direction dir;
+ * kinds 3, 4, 5, 6: one half step only -- forward v, forward sigma,
+ * adjoint sigma, adjoint v. */
 void or_stencils_f64(int nxpml, int nzpml, const double *f,
                      double *dxp, double *dxm, double *dzp, double *dzm);
 void or_propagate_f64(const or_config *c, const float *vp, const float *vs, const float *rho,
