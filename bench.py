@@ -15,8 +15,10 @@ in HBM.  value = shot-gradients per second over the timed steps.
 Also reported: Gcell-updates/s (3 propagations per gradient), p50/p99 of the
 step latency, the gradient error of two shots against the FP64 oracle, a
 roofline entry for the dominant kernel (HIP-event timings taken inside the
-timed steps on the library's stream) and a CPU baseline (the FP64 oracle on
-the host cores, bounded 2-shot sample of the same workload).
+timed steps on the library's stream), the plain one-shot path on the same GPU
+(one shot per call, per-cell kernels; FP32 and reference operation order) and
+a CPU baseline (the FP64 oracle on the host cores, bounded 2-shot sample of
+the same workload).
 """
 import argparse
 import json
@@ -32,6 +34,19 @@ if ROOT not in sys.path:
 
 METRIC = "shot-gradients/s & Gcell-updates/s at ≤1e-3 grad err vs FP64; p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
+
+# HBM bytes per launch of the default (column-march, 8 columns, depth 1) FAST
+# kernels on this workload's 461 x 361 padded grid, per shot in the launch:
+# FETCH_SIZE x 2 + WRITE_SIZE x 1 from rocprofv3 --pmc passes over a 64-shot
+# batch, the factors from a plain 1 GiB copy (profiles/shot_batch_march_hbm_traffic.md).
+MEASURED_HBM_BYTES_PER_SHOT = {
+    "k_vel(forward)": 481.7e6 / 64,
+    "k_str(forward)": 582.2e6 / 64,
+    "k_str(reconstruct)": 795.5e6 / 64,
+    "k_vel(reconstruct)": 480.1e6 / 64,
+    "k_adj_str": 516.8e6 / 64,
+    "k_adj_vel_image": 1146.7e6 / 64,
+}
 
 
 def alg_bytes(kernel, n_int, n_pml):
@@ -63,6 +78,8 @@ def main():
                     help="FAST kernels: columns per wave segment (0: library default, <0: per-cell kernels)")
     ap.add_argument("--march-depth", type=int, default=0, help="FAST march kernels: prefetch depth")
     ap.add_argument("--cpu-shots", type=int, default=2)
+    ap.add_argument("--plain-shots", type=int, default=4,
+                    help="shots timed through the plain one-shot path (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
 
@@ -140,9 +157,34 @@ def main():
     roofline = None
     if dom:
         d = kernels[dom]
+        measured = (args.mode == "fast" and args.march_cols in (0, 8) and args.march_depth in (0, 1))
+        traffic = round(MEASURED_HBM_BYTES_PER_SHOT[dom] * nsh) if measured else None
         roofline = {"kernel": dom, "bound": "hbm", "achieved": d["GBps"], "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(d["GBps"] / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(d["GBps"] / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "alg_bytes_per_launch": d["alg_bytes"], "avg_launch_ms": d["avg_ms"]}
+
+    # the plain path on the same GPU: one shot per call, per-cell kernels
+    plain = None
+    if args.plain_shots > 0:
+        plain = {}
+        for label, mode in (("fp32_percell", "fast"), ("reference_order", "exact")):
+            pc = Context(dx=sv.dx, dz=sv.dz, dt=sv.dt, nx=sv.nx, nz=sv.allnz, pml=sv.pml, nt=sv.nt,
+                         ntr=sv.ntr, allnx=sv.allnx, allnz=sv.allnz, max_batch=1, mode=mode, march_cols=-1)
+            pc.set_wavelet(w)
+            pc.set_model(sv.vp0, sv.vs0, sv.rho0)
+            pc.gradient(shots[0])                      # warm-up
+            k = min(args.plain_shots, nsh)
+            t0 = time.perf_counter()
+            for sh in shots[:k]:
+                pc.gradient(sh)
+            tp = time.perf_counter() - t0
+            pc.flush()
+            pc.close()
+            plain[label] = {"value": round(k / tp, 4), "shots": k}
+            print("bench: plain one-shot path (%s) %.3f s per shot" % (label, tp / k), file=sys.stderr,
+                  flush=True)
+        plain["unit"] = "shot-gradients/s"
+        plain["speedup_of_batched"] = round(value / plain["fp32_percell"]["value"], 2)
 
     # accuracy of two shots against the FP64 oracle (and the CPU baseline)
     acc = None
@@ -191,6 +233,7 @@ def main():
         "grad_err_vs_fp64": acc,
         "misfit": misfit,
         "roofline": roofline,
+        "plain_one_shot_path": plain,
         "cpu_baseline": cpu,
         "kernels": kernels,
     }
