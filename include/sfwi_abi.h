@@ -35,6 +35,15 @@ extern "C" {
 #define SFWI_MODE_EXACT 1      /* reference operation order everywhere (split  */
                                /* PML fields on every cell, FP64 promotions)   */
 
+/* Misfit of the gradient and line-search solves.  L2 is the reference's
+ * (sfwi-stage1/FD2DGPU.cu:4583-4590); the OT choices put the optimal-transport
+ * misfit of sfwi_ot_misfit where the reference has its commented-out OT calls
+ * (sfwi-stage1/FD2DGPU.cu:4887-4896, misfit_ot in sfwi-stage1/main.cpp:357,540). */
+#define SFWI_MISFIT_L2 0
+#define SFWI_MISFIT_OT_TRACE 1    /* sfwi_ot_misfit variant 0, per trace        */
+#define SFWI_MISFIT_OT_POWER 2    /* variant 1, per trace, squared signal        */
+#define SFWI_MISFIT_OT_GATHER 3   /* variant 2, whole gather of a component      */
+
 /* Forward-wavefield storage used by the reverse-time reconstruction. */
 #define SFWI_STORE_STRIPS_F32 0   /* 8-cell boundary strips of vx, vz, FP32    */
 #define SFWI_STORE_STRIPS_F16 1   /* same strips, FP16 with per-step scale     */
@@ -64,6 +73,7 @@ typedef struct sfwi_params {
     int march_cols;        /* FAST kernels: columns per wave segment of the column-  */
                            /* march kernels (0 = 8); < 0 selects per-cell kernels */
     int march_depth;       /* columns of operands prefetched ahead (1, 2, 4; 0 = 1) */
+    int misfit;            /* SFWI_MISFIT_*                                          */
 } sfwi_params;
 
 /* One shot: the fields the reference driver writes into FD2DGPU_ELASTIC
@@ -146,9 +156,16 @@ int sfwi_flush(sfwi_ctx *ctx, float *grad_vp, float *grad_vs, float *illum);
  * defined in the reference (OT, OT_power, OT_alltrace,
  * sfwi-stage1/FD2DGPU.cuh:21-25); arguments follow those declarations:
  * observed and synthetic traces [ntr][nt] per component, adjoint source
- * written to res_*, W2^2 misfit added to *misfit.
- * variant 0: per trace, linear shift to positivity; 1: per trace, squared
- * ("power"); 2: all traces of a component as one distribution. */
+ * dW/dsyn written to res_*, W2^2 misfit (time units, summed over traces and
+ * both components) added to *misfit.
+ * variant 0: per trace, signal shifted to positivity by
+ *            c = 1.1 max(max|syn|, max|obs|) (c held fixed in the adjoint);
+ *         1: per trace, squared signal ("power");
+ *         2: all traces of a component as one distribution (flattened
+ *            trace after trace, shift as in variant 0 over the gather).
+ * Each signal is a piecewise-constant density on unit sample cells; the
+ * cumulative sums and quantile inversion run in FP64.  A trace with zero mass
+ * on either side contributes 0 and gets a zero adjoint source. */
 int sfwi_ot_misfit(sfwi_ctx *ctx, int variant, int ntr, int nt, float dt, const float *obs_x,
                    const float *obs_z, const float *syn_x, const float *syn_z, float *res_x,
                    float *res_z, double *misfit);

@@ -57,6 +57,8 @@ def lib():
                                       _f32p, _f32p, _f32p, dp, _f32p, _f32p]
         L.or_gradient_f64.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _i32p, _f32p, _f32p,
                                       _f64p, _f64p, _f64p, dp, _f64p, _f64p]
+        L.or_gradient_res_f32.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _i32p, _f32p, _f32p,
+                                          _f32p, _f32p, _f32p]
         L.or_born_forward_f32.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _i32p,
                                           _f32p, _f32p, _f32p, _f32p]
         L.or_born_forward_f64.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _i32p,
@@ -139,6 +141,16 @@ def gradient(c, vp, vs, rho, wav, gc, obs_z, obs_x, prec="f32"):
                                           g1, g2, g3, ctypes.byref(m), rz, rx)
     sh = (c.allnx, c.allnz)
     return g1.reshape(sh), g2.reshape(sh), g3.reshape(sh), m.value, rz, rx
+
+
+def gradient_res(c, vp, vs, rho, wav, gc, res_z, res_x):
+    """Gradient (FP32 order) of a misfit whose adjoint source is res_z / res_x."""
+    n = c.allnx * c.allnz
+    g1, g2, g3 = np.zeros(n, np.float32), np.zeros(n, np.float32), np.zeros(n, np.float32)
+    lib().or_gradient_res_f32(ctypes.byref(c), _f32(vp), _f32(vs), _f32(rho), _f32(wav),
+                              np.ascontiguousarray(gc, np.int32), _f32(res_z), _f32(res_x), g1, g2, g3)
+    sh = (c.allnx, c.allnz)
+    return g1.reshape(sh), g2.reshape(sh), g3.reshape(sh)
 
 
 def born_forward(c, vp, vs, rho, dvp, dvs, wav, gc, prec="f32"):

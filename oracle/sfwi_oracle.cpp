@@ -590,6 +590,26 @@ void gradient_impl(const or_config *c, const float *vp, const float *vs, const f
     r.reverse(fw, strips, sz, sx, gvp, gvs, ill);
 }
 
+// Forward (strips) then the reverse loop with a caller-supplied adjoint
+// source in place of the L2 residual (used with the OT misfit).
+template <typename T>
+void gradient_res_impl(const or_config *c, const float *vp, const float *vs, const float *rho,
+                       const float *wav, const int *gc, const float *res_z, const float *res_x, T *gvp,
+                       T *gvs, T *ill) {
+    Run<T> r(c, vp, vs, rho, nullptr, nullptr, gc);
+    const long nr = (long)c->ntr * c->nt;
+    std::vector<T> sz(nr), sx(nr);
+    Strips<T> strips;
+    strips.init(r.g);
+    Wave<T> fw(1);
+    r.forward(wav, sz, sx, &fw, &strips);
+    for (long k = 0; k < nr; ++k) {
+        sz[k] = (T)res_z[k];
+        sx[k] = (T)res_x[k];
+    }
+    r.reverse(fw, strips, sz, sx, gvp, gvs, ill);
+}
+
 template <typename T, typename G>
 void born_gradient_impl(const or_config *c, const float *vp, const float *vs, const float *rho,
                         const float *dvp, const float *dvs, const float *wav, const int *gc,
@@ -678,6 +698,11 @@ void or_gradient_f64(const or_config *c, const float *vp, const float *vs, const
                      const float *wav, const int *gc, const float *obs_z, const float *obs_x,
                      double *gvp, double *gvs, double *ill, double *misfit, double *res_z, double *res_x) {
     gradient_impl<double, double>(c, vp, vs, rho, wav, gc, obs_z, obs_x, gvp, gvs, ill, misfit, res_z, res_x);
+}
+void or_gradient_res_f32(const or_config *c, const float *vp, const float *vs, const float *rho,
+                         const float *wav, const int *gc, const float *res_z, const float *res_x,
+                         float *gvp, float *gvs, float *ill) {
+    gradient_res_impl<float>(c, vp, vs, rho, wav, gc, res_z, res_x, gvp, gvs, ill);
 }
 
 }  // extern "C"

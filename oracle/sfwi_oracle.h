@@ -64,6 +64,13 @@ void or_gradient_f64(const or_config *c, const float *vp, const float *vs, const
                      double *grad_vp, double *grad_vs, double *illum, double *misfit,
                      double *res_z, double *res_x);
 
+/* Reverse loop (reconstruction, adjoint, imaging) driven by a caller-supplied
+ * adjoint source res_* [ntr][nt] instead of the L2 residual: the gradient of
+ * a misfit whose adjoint source is res_* (OT misfit).  Stage set by c. */
+void or_gradient_res_f32(const or_config *c, const float *vp, const float *vs, const float *rho,
+                         const float *wav, const int *gc, const float *res_z, const float *res_x,
+                         float *grad_vp, float *grad_vs, float *illum);
+
 /* Stage-2 Born forward: scattered traces b_* and background traces bg_*. */
 void or_born_forward_f32(const or_config *c, const float *vp, const float *vs, const float *rho,
                          const float *dvp, const float *dvs, const float *wav, const int *gc,

@@ -16,6 +16,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "sfwi_abi.h")
 SFWI_OK, SFWI_EINVAL, SFWI_EHIP, SFWI_ENOMEM, SFWI_ESTATE = 0, -1, -2, -3, -4
 MODE_FAST, MODE_EXACT = 0, 1
 STORE_STRIPS_F32, STORE_STRIPS_F16, STORE_STRIPS_BF16 = 0, 1, 2
+MISFIT_L2, MISFIT_OT_TRACE, MISFIT_OT_POWER, MISFIT_OT_GATHER = 0, 1, 2, 3
 K_FWD_VEL, K_FWD_STR, K_REC_STR, K_REC_VEL, K_ADJ_STR, K_ADJ_VEL, K_COUNT = range(7)
 
 
@@ -28,7 +29,7 @@ class SfwiParams(ctypes.Structure):
         ("rr", ctypes.c_float), ("rho_const", ctypes.c_float),
         ("adjoint_dir", ctypes.c_int), ("max_batch", ctypes.c_int),
         ("mode", ctypes.c_int), ("storage", ctypes.c_int), ("use_graph", ctypes.c_int),
-        ("march_cols", ctypes.c_int), ("march_depth", ctypes.c_int),
+        ("march_cols", ctypes.c_int), ("march_depth", ctypes.c_int), ("misfit", ctypes.c_int),
     ]
 
 

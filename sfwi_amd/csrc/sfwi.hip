@@ -9,6 +9,7 @@
 #include "sfwi_abi.h"
 #include "kernels.hpp"
 #include "kernels_march.hpp"
+#include "ot.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -68,6 +69,14 @@ struct sfwi_ctx {
     double *d_out = nullptr;      // per-shot misfits [max_batch] + 2 scalars
     double *d_num = nullptr, *d_den = nullptr;
     bool have_model = false, have_wav = false, have_pert = false;
+    // optimal-transport misfit: job table, FP64 workspace, per-job misfits, I/O staging
+    OtJob *ot_jobs = nullptr;
+    double *ot_ws = nullptr, *ot_mis = nullptr;
+    int *ot_first = nullptr;
+    float *ot_io = nullptr;
+    size_t ot_jobs_cap = 0, ot_ws_cap = 0, ot_mis_cap = 0, ot_first_cap = 0, ot_io_cap = 0;
+    std::vector<OtJob> ot_h;          // host copies kept alive for the async uploads
+    std::vector<int> ot_first_h;
     // timing
     int timing_every = 0;
     std::vector<hipEvent_t> ev;
@@ -318,6 +327,88 @@ int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium 
 
 // ------------------------------------------------------------- solves ---
 
+// ----------------------------------------------------- optimal transport ---
+
+constexpr int OT_MAX_BLOCKS = 1024;   // resident OT workgroups (4 per CU); each owns a workspace
+
+// Grow-only device buffer; waits for the stream before freeing the old one.
+int grow(sfwi_ctx *c, void **p, size_t *cap, size_t need) {
+    if (need <= *cap) return 0;
+    if (*p) {
+        HIPCK(c, hipStreamSynchronize(c->stream));
+        HIPCK(c, hipFree(*p));
+        *p = nullptr;
+        *cap = 0;
+    }
+    int r = dalloc(c, p, need);
+    if (r) return r;
+    *cap = need;
+    return 0;
+}
+
+// Enqueue k_ot over the jobs in c->ot_h (longest job n_max samples).
+int ot_launch(sfwi_ctx *c, size_t n_max, double dt2) {
+    const size_t nj = c->ot_h.size();
+    const int blocks = (int)std::min<size_t>(nj, OT_MAX_BLOCKS);
+    const size_t stride = 4 * (n_max + 1);
+    int r;
+    if ((r = grow(c, (void **)&c->ot_jobs, &c->ot_jobs_cap, nj * sizeof(OtJob)))) return r;
+    if ((r = grow(c, (void **)&c->ot_ws, &c->ot_ws_cap, blocks * stride * sizeof(double)))) return r;
+    HIPCK(c, hipMemcpyAsync(c->ot_jobs, c->ot_h.data(), nj * sizeof(OtJob), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_ot, dim3(blocks), dim3(OT_THREADS), 0, c->stream, c->ot_jobs, (int)nj, c->ot_ws, stride,
+                       dt2);
+    HIPCK(c, hipGetLastError());
+    return 0;
+}
+
+// OT misfit of a prepared batch in place of the L2 residual: the synthetic
+// traces in rz / rx (stage 2: born + background) become the adjoint source,
+// per-shot misfits go to d_out.
+int ot_batch(sfwi_ctx *c, int n, bool born) {
+    const Geo &g = c->g;
+    const int var = c->p.misfit;
+    const bool gather = var == SFWI_MISFIT_OT_GATHER;
+    if (born) {
+        const dim3 grid(256, (unsigned)n);
+        hipLaunchKernelGGL(k_born_total, grid, dim3(BLK), 0, c->stream, g, c->d_shots);
+    }
+    c->ot_h.clear();
+    c->ot_first_h.assign(1, 0);
+    size_t nj = 0;
+    for (int s = 0; s < n; ++s) nj += gather ? 2 : 2 * (size_t)c->h_shots[s].ntr;
+    int r;
+    if ((r = grow(c, (void **)&c->ot_mis, &c->ot_mis_cap, nj * sizeof(double)))) return r;
+    if ((r = grow(c, (void **)&c->ot_first, &c->ot_first_cap, (n + 1) * sizeof(int)))) return r;
+    size_t n_max = 0;
+    for (int s = 0; s < n; ++s) {
+        const DevShot &d = c->h_shots[s];
+        float *syn[2] = {d.rx, d.rz};
+        const float *obs[2] = {d.ox, d.oz};
+        for (int comp = 0; comp < 2; ++comp) {
+            if (gather) {
+                const size_t len = (size_t)d.ntr * g.nt;
+                c->ot_h.push_back({syn[comp], obs[comp], syn[comp], c->ot_mis + c->ot_h.size(), (int)len, 0});
+                n_max = std::max(n_max, len);
+            } else {
+                for (int tr = 0; tr < d.ntr; ++tr) {
+                    const size_t o = (size_t)tr * g.nt;
+                    c->ot_h.push_back({syn[comp] + o, obs[comp] + o, syn[comp] + o, c->ot_mis + c->ot_h.size(), g.nt,
+                                       var == SFWI_MISFIT_OT_POWER});
+                }
+                n_max = std::max(n_max, (size_t)g.nt);
+            }
+        }
+        c->ot_first_h.push_back((int)c->ot_h.size());
+    }
+    HIPCK(c, hipMemcpyAsync(c->ot_first, c->ot_first_h.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice,
+                            c->stream));
+    if ((r = ot_launch(c, n_max, (double)c->p.dt * (double)c->p.dt))) return r;
+    hipLaunchKernelGGL(k_ot_shot_sum, dim3((n + 63) / 64), dim3(64), 0, c->stream, c->ot_mis, c->ot_first,
+                       c->d_out, n);
+    HIPCK(c, hipGetLastError());
+    return 0;
+}
+
 template <bool E, int PD>
 int run_batch(sfwi_ctx *c, Kind kind, int n) {
     const Geo &g = c->g;
@@ -369,12 +460,17 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
     HIPCK(c, hipGetLastError());
     if (kind == K_FWD || kind == K_BORN_FWD) return 0;
 
-    const dim3 resg(256, (unsigned)n);
-    if (born) hipLaunchKernelGGL(k_residual<true>, resg, blk, 0, st, g, D);
-    else hipLaunchKernelGGL(k_residual<false>, resg, blk, 0, st, g, D);
-    hipLaunchKernelGGL(k_trace_l2, trg, dim3(64), 0, st, g, D);
-    hipLaunchKernelGGL(k_shot_l2, src1, dim3(64), 0, st, D, c->d_out, n);
-    HIPCK(c, hipGetLastError());
+    if (c->p.misfit == SFWI_MISFIT_L2) {
+        const dim3 resg(256, (unsigned)n);
+        if (born) hipLaunchKernelGGL(k_residual<true>, resg, blk, 0, st, g, D);
+        else hipLaunchKernelGGL(k_residual<false>, resg, blk, 0, st, g, D);
+        hipLaunchKernelGGL(k_trace_l2, trg, dim3(64), 0, st, g, D);
+        hipLaunchKernelGGL(k_shot_l2, src1, dim3(64), 0, st, D, c->d_out, n);
+        HIPCK(c, hipGetLastError());
+    } else {
+        int r = ot_batch(c, n, born);
+        if (r) return r;
+    }
     if (!grad) return 0;
 
     const int adir = c->p.adjoint_dir;
@@ -471,6 +567,8 @@ int sfwi_ctx_create(const sfwi_params *p, sfwi_ctx **out) {
     if (p->adjoint_dir != 1 && p->adjoint_dir != -1) return fail(nullptr, SFWI_EINVAL, "adjoint_dir must be +1 or -1");
     if (p->mode != SFWI_MODE_FAST && p->mode != SFWI_MODE_EXACT) return fail(nullptr, SFWI_EINVAL, "unknown mode %d", p->mode);
     if (p->storage != SFWI_STORE_STRIPS_F32) return fail(nullptr, SFWI_EINVAL, "storage %d not available", p->storage);
+    if (p->misfit < SFWI_MISFIT_L2 || p->misfit > SFWI_MISFIT_OT_GATHER)
+        return fail(nullptr, SFWI_EINVAL, "unknown misfit %d", p->misfit);
     if (p->use_graph) return fail(nullptr, SFWI_EINVAL, "use_graph not available in this build");
     const long nxp = p->nx + 2L * p->pml, nzp = p->nz + 2L * p->pml;
     if (nxp * nzp >= (1L << 31) / 16) return fail(nullptr, SFWI_EINVAL, "grid too large for 32-bit cell indices");
@@ -538,6 +636,7 @@ void sfwi_ctx_destroy(sfwi_ctx *c) {
     for (hipEvent_t e : c->ev) hipEventDestroy(e);
     hipFree(c->wav); hipFree(c->mvp); hipFree(c->tmp_model); hipFree(c->Gvp); hipFree(c->res_keep);
     hipFree(c->d_shots); hipFree(c->d_out); hipFree(c->d_num);
+    hipFree(c->ot_jobs); hipFree(c->ot_ws); hipFree(c->ot_mis); hipFree(c->ot_first); hipFree(c->ot_io);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -742,9 +841,49 @@ void sfwi_ricker(float *w, int nt, float dt, float f0) {
         w[t] = (1.0 - 2.0 * nk * (t - t0) * (t - t0)) * std::exp(-nk * (t - t0) * (t - t0));
 }
 
-int sfwi_ot_misfit(sfwi_ctx *c, int, int, int, float, const float *, const float *, const float *, const float *,
-                   float *, float *, double *) {
-    return fail(c, SFWI_ESTATE, "optimal-transport misfit not built yet");
+int sfwi_ot_misfit(sfwi_ctx *c, int variant, int ntr, int nt, float dt, const float *obs_x, const float *obs_z,
+                   const float *syn_x, const float *syn_z, float *res_x, float *res_z, double *misfit) {
+    if (!c) return fail(c, SFWI_EINVAL, "NULL context");
+    if (!obs_x || !obs_z || !syn_x || !syn_z || !res_x || !res_z || !misfit)
+        return fail(c, SFWI_EINVAL, "NULL argument");
+    if (variant < 0 || variant > 2) return fail(c, SFWI_EINVAL, "unknown OT variant %d", variant);
+    if (ntr < 1 || nt < 1) return fail(c, SFWI_EINVAL, "ntr=%d nt=%d must be positive", ntr, nt);
+    if (!(dt > 0)) return fail(c, SFWI_EINVAL, "dt must be positive");
+    const size_t m = (size_t)ntr * nt;
+    if (m >= (1u << 30)) return fail(c, SFWI_EINVAL, "gather of %zu samples too large", m);
+    int r;
+    if ((r = grow(c, (void **)&c->ot_io, &c->ot_io_cap, 4 * m * sizeof(float)))) return r;
+    float *ox = c->ot_io, *oz = ox + m, *rx = oz + m, *rz = rx + m;
+    const size_t nj = variant == 2 ? 2 : 2 * (size_t)ntr;
+    if ((r = grow(c, (void **)&c->ot_mis, &c->ot_mis_cap, nj * sizeof(double)))) return r;
+    HIPCK(c, hipMemcpyAsync(ox, obs_x, m * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipMemcpyAsync(oz, obs_z, m * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipMemcpyAsync(rx, syn_x, m * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipMemcpyAsync(rz, syn_z, m * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    c->ot_h.clear();
+    float *syn[2] = {rx, rz};
+    const float *obs[2] = {ox, oz};
+    for (int comp = 0; comp < 2; ++comp) {
+        if (variant == 2) {
+            c->ot_h.push_back({syn[comp], obs[comp], syn[comp], c->ot_mis + c->ot_h.size(), (int)m, 0});
+        } else {
+            for (int tr = 0; tr < ntr; ++tr) {
+                const size_t o = (size_t)tr * nt;
+                c->ot_h.push_back({syn[comp] + o, obs[comp] + o, syn[comp] + o, c->ot_mis + c->ot_h.size(), nt,
+                                   variant == 1});
+            }
+        }
+    }
+    if ((r = ot_launch(c, variant == 2 ? m : (size_t)nt, (double)dt * (double)dt))) return r;
+    std::vector<double> mis(nj);
+    HIPCK(c, hipMemcpyAsync(mis.data(), c->ot_mis, nj * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipMemcpyAsync(res_x, rx, m * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipMemcpyAsync(res_z, rz, m * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    double total = 0.0;
+    for (double v : mis) total += v;
+    *misfit += total;
+    return 0;
 }
 
 }  // extern "C"
