@@ -10,7 +10,9 @@ smoothed starting model.
 
 One step = one batched gradient of all shots (forward with strip storage,
 misfit, reverse reconstruction + adjoint + zero-lag imaging), inputs resident
-in HBM.  value = shot-gradients per second over the timed steps.
+in HBM (the observed traces stay resident from the warm-up call).  value =
+shot-gradients per second over the timed steps; pcie_inclusive_value is one
+more step that uploads the observed traces from host memory.
 
 Also reported: Gcell-updates/s (3 propagations per gradient), p50/p99 of the
 step latency, the gradient error of two shots against the FP64 oracle, a
@@ -121,17 +123,28 @@ def main():
         ctx.flush()
         print("bench: warmup %d %.3f s" % (k, time.perf_counter() - t0), file=sys.stderr, flush=True)
 
+    # timed steps: observed traces already resident in HBM (uploaded by the
+    # warm-up call); the shots pass NULL records and reuse their batch slot's
+    resident = [Shot(s.left, *g, s.ngx_left, s.gc) for s, g in zip(sv.shots, geo)]
+    if args.warmup == 0:
+        ctx.batch_gradient(shots)
+        ctx.flush()
     ctx.set_timing(args.timing_every)
     times = []
     misfit = 0.0
     for k in range(args.steps):
         t0 = time.perf_counter()
-        m = ctx.batch_gradient(shots)          # returns after the stream is drained
+        m = ctx.batch_gradient(resident)       # returns after the stream is drained
         times.append(time.perf_counter() - t0)
         misfit = float(m.sum())
         print("bench: step %d %.3f s" % (k, times[-1]), file=sys.stderr, flush=True)
     kstats = {kid: ctx.kernel_time(kid) for kid in range(_lib.K_COUNT)}
     ctx.set_timing(0)
+    ctx.flush()
+    # one step that uploads the observed traces from host memory (PCIe-inclusive rate)
+    t0 = time.perf_counter()
+    ctx.batch_gradient(shots)
+    t_pcie = time.perf_counter() - t0
     ctx.flush()
 
     total = sum(times)
@@ -228,6 +241,7 @@ def main():
                                "401 receivers, batched gradient" % (sv.nt, nsh),
                    "mode": args.mode, "storage": "boundary strips FP32"},
         "gcell_updates_per_s": round(gcells, 2),
+        "pcie_inclusive_value": round(len(shots) / t_pcie, 4),
         "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 2),
                        "p99": round(1e3 * float(np.percentile(lat, 99)), 2), "unit": "per batch step"},
         "grad_err_vs_fp64": acc,

@@ -88,6 +88,10 @@ typedef struct sfwi_shot {
     const int *gc;         /* receiver coordinates [ntr]                             */
     const float *obs_z;    /* observed vz traces [ntr][nt] (gradient calls)         */
     const float *obs_x;    /* observed vx traces [ntr][nt]                          */
+                           /* Both NULL: reuse the observed traces left in HBM by   */
+                           /* the previous call for the same batch position (shot   */
+                           /* i of a batch of <= max_batch shots sits in slot i);   */
+                           /* the trace count must match.                          */
 } sfwi_shot;
 
 /* Replaces the FD2DGPU_ELASTIC constructor (sfwi-stage1/FD2DGPU.cu:1366-1655)

@@ -37,6 +37,7 @@ struct Slot {
     int *igx, *inj;
     float *gvp, *gvs, *gil;
     double *trace_sum;
+    int obs_ntr = 0;                 // traces of the observed records resident in oz/ox (0: none)
     std::vector<int> h_igx, h_inj;   // host copies kept alive for async uploads
 };
 
@@ -294,7 +295,9 @@ int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium 
     for (int k = 0; k < n; ++k) {
         const sfwi_shot &s = shots[k];
         if ((r = check_shot(c, s, k))) return r;
-        if (obs && (!s.obs_z || !s.obs_x)) return fail(c, SFWI_EINVAL, "shot %d: observed traces missing", k);
+        if (obs && (!s.obs_z || !s.obs_x) && (s.obs_z || s.obs_x || c->slots[k].obs_ntr != s.ntr))
+            return fail(c, SFWI_EINVAL, "shot %d: observed traces missing (and no resident records of %d traces "
+                        "in batch slot %d)", k, s.ntr, k);
         Slot &S = c->slots[k];
         if ((r = receiver_tables(c, s, k, S))) return r;
         const Medium *m = override_m;
@@ -314,10 +317,11 @@ int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium 
         if (born) HIPCK(c, hipMemsetAsync(S.born, 0, 15 * cell, c->stream));
         HIPCK(c, hipMemcpyAsync(S.igx, S.h_igx.data(), s.ntr * sizeof(int), hipMemcpyHostToDevice, c->stream));
         HIPCK(c, hipMemcpyAsync(S.inj, S.h_inj.data(), S.h_inj.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
-        if (obs) {
+        if (obs && s.obs_z) {          // NULL: the records resident in this slot are reused
             const size_t rb = (size_t)s.ntr * c->p.nt * sizeof(float);
             HIPCK(c, hipMemcpyAsync(S.oz, s.obs_z, rb, hipMemcpyHostToDevice, c->stream));
             HIPCK(c, hipMemcpyAsync(S.ox, s.obs_x, rb, hipMemcpyHostToDevice, c->stream));
+            S.obs_ntr = s.ntr;
         }
         if (grad) HIPCK(c, hipMemsetAsync(S.gvp, 0, 3 * c->nint * sizeof(float), c->stream));
     }
