@@ -79,6 +79,8 @@ def main():
     ap.add_argument("--march-cols", type=int, default=0,
                     help="FAST kernels: columns per wave segment (0: library default, <0: per-cell kernels)")
     ap.add_argument("--march-depth", type=int, default=0, help="FAST march kernels: prefetch depth")
+    ap.add_argument("--march-order", type=int, default=0,
+                    help="FAST march kernels: block order (0 default, 1 shot after shot, 2 shots interleaved)")
     ap.add_argument("--cpu-shots", type=int, default=2)
     ap.add_argument("--plain-shots", type=int, default=4,
                     help="shots timed through the plain one-shot path (0: skip)")
@@ -107,7 +109,7 @@ def main():
 
     ctx = Context(dx=sv.dx, dz=sv.dz, dt=sv.dt, nx=sv.nx, nz=sv.allnz, pml=sv.pml, nt=sv.nt, ntr=sv.ntr,
                   allnx=sv.allnx, allnz=sv.allnz, max_batch=len(sv.shots), mode=args.mode,
-                  march_cols=args.march_cols, march_depth=args.march_depth)
+                  march_cols=args.march_cols, march_depth=args.march_depth, march_order=args.march_order)
     ctx.set_wavelet(w)
     ctx.set_model(sv.vp, sv.vs, sv.rho)
     t0 = time.perf_counter()

@@ -74,6 +74,8 @@ typedef struct sfwi_params {
                            /* march kernels (0 = 8); < 0 selects per-cell kernels */
     int march_depth;       /* columns of operands prefetched ahead (1, 2, 4; 0 = 1) */
     int misfit;            /* SFWI_MISFIT_*                                          */
+    int march_order;       /* column-march block order: 0 default (= 1); 1 one shot  */
+                           /* after another; 2 the shots of a tile back to back     */
 } sfwi_params;
 
 /* One shot: the fields the reference driver writes into FD2DGPU_ELASTIC

@@ -30,6 +30,7 @@ class SfwiParams(ctypes.Structure):
         ("adjoint_dir", ctypes.c_int), ("max_batch", ctypes.c_int),
         ("mode", ctypes.c_int), ("storage", ctypes.c_int), ("use_graph", ctypes.c_int),
         ("march_cols", ctypes.c_int), ("march_depth", ctypes.c_int), ("misfit", ctypes.c_int),
+        ("march_order", ctypes.c_int),
     ]
 
 

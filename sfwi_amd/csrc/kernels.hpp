@@ -75,6 +75,8 @@ struct Geo {
     int cm_xs;              // column-march segment width (FAST kernels)
     int cm_nbands, cm_waves;// row bands per column strip, waves per shot
     int cm_per_xcd;         // column-march workgroups per XCD
+    int cm_order;           // 0: grid.y = shot; 1: 1-D grid, shots of a tile adjacent
+    int cm_nshots;          // shots of the launch (cm_order 1)
     int sf;                 // strip floats per field and step
     float inv_nzd;
     float dt, dx, dz, dtdx, dtdz, idx, idz;

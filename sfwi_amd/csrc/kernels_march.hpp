@@ -26,12 +26,27 @@ constexpr int CM_ROWS = 56;   // output rows per wave (64 lanes minus 2 x 4 halo
 
 struct Seg {
     int x0, x1, zb;   // columns [x0, x1), first output row zb
+    int shot;
 };
 
+// Block -> (shot, wave tile).  Block b runs on XCD b & 7; each XCD owns the
+// contiguous range L = (b & 7) * per_xcd + (b >> 3) of the launch's work.
+// Order 0: grid.y is the shot, L the workgroup tile of that shot.  Order 1:
+// one 1-D grid over (tile, shot) pairs with the shot fastest, so the shots of
+// one tile run back to back on one XCD and read the medium they share from
+// that XCD's L2.
 __device__ inline bool seg_of(const Geo &g, Seg &s) {
     const int b = blockIdx.x;
     const int L = (b & 7) * g.cm_per_xcd + (b >> 3);
-    const int W = L * (BLK / 64) + (int)(threadIdx.x >> 6);
+    int tile = L;
+    s.shot = blockIdx.y;
+    if (g.cm_order) {
+        tile = L / g.cm_nshots;
+        s.shot = L - tile * g.cm_nshots;
+    }
+    // the wave index is uniform across the wave: say so, so segment bounds,
+    // loop trip counts and prefetch guards are scalar (no exec-mask branches)
+    const int W = tile * (BLK / 64) + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     if (W >= g.cm_waves) return false;
     const int seg = W / g.cm_nbands;
     const int band = W - seg * g.cm_nbands;
@@ -81,20 +96,67 @@ __device__ inline void shift(float *w) {
     for (int j = 0; j < 7; ++j) w[j] = w[j + 1];
 }
 
+// Window loads come from a valid address (the row clamped into the grid) and
+// the zero of an out-of-grid lane is a select after the load, not a branch
+// around it: a value written into a register that a load in flight also
+// targets makes the wave wait for that load.
 struct Lane {
-    int z;
+    int z, zc;          // row, row clamped into the grid (load address)
     bool zin, act;
 };
 __device__ inline Lane lane_of(const Geo &g, const Seg &s) {
     const int lane = threadIdx.x & 63;
     Lane l;
     l.z = s.zb - 4 + lane;
+    l.zc = min(l.z, g.nzp - 1);
     l.zin = l.z < g.nzp;
     l.act = lane >= 4 && lane < 60 && l.z < g.nzp - NOP;
     return l;
 }
 __device__ inline float ld(const float *__restrict__ f, const Geo &g, const Lane &l, int x) {
-    return l.zin ? f[x * g.nzp + l.z] : 0.0f;
+    const float v = f[x * g.nzp + l.zc];
+    return l.zin ? v : 0.0f;
+}
+// True on every lane when some lane of the wave holds p (wave-uniform value).
+__device__ inline bool any_lane(bool p) { return __ballot(p) != 0; }
+
+// Own-cell loader of the adjoint velocity march, into a ring slot that
+// persists across columns: the totals always, the split parts and the damping
+// only when some lane of the wave is split (a uniform branch with no else, so
+// the slot keeps its registers and no copy waits for the loads).  The update
+// math reads only the variant its lane uses.
+__device__ inline void vel_load_m(const Medium &m, const Fields &F, int i, bool any, VelCell &s) {
+    s.bu = m.bu[i];
+    s.vx = F.vx[i];
+    s.vz = F.vz[i];
+    if (any) {
+        s.vx_x = F.vx_x[i]; s.vx_z = F.vx_z[i]; s.vz_x = F.vz_x[i]; s.vz_z = F.vz_z[i];
+        s.ax = m.dampx[i]; s.az = m.dampz[i];
+    }
+}
+
+// Operands of the zero-lag imaging of one cell (loaded a column ahead, only
+// when some lane of the wave images).  sxx and szz are the adjoint stresses
+// the velocity update loads anyway; the caller fills them from its ring.
+struct ImgCell {
+    float gx, gz, gxz, gzx, sxx, szz, sxz, vx, vz, cp, cs, ip, is, il;
+};
+__device__ inline void img_load(const DevShot &d, int i, int q, bool any, ImgCell &c) {
+    if (any) {
+        c.gx = d.dgx[i]; c.gz = d.dgz[i]; c.gxz = d.dgxz[i]; c.gzx = d.dgzx[i];
+        c.sxz = d.bk.txz[i];
+        c.vx = d.fw.vx[i]; c.vz = d.fw.vz[i];
+        c.cp = d.m.icp[i]; c.cs = d.m.ics[i];
+        c.ip = d.gvp[q]; c.is = d.gvs[q]; c.il = d.gil[q];
+    }
+}
+// The arithmetic of image_fast on prefetched operands.
+__device__ inline void img_store(const DevShot &d, int q, const ImgCell &c) {
+    const float lg = -((c.gx + c.gz) * (c.sxx + c.szz));
+    const float mg = __builtin_fmaf(-(c.gxz + c.gzx), c.sxz, -2.0f * __builtin_fmaf(c.gx, c.sxx, c.gz * c.szz));
+    d.gvp[q] = __builtin_fmaf(c.cp, lg, c.ip);
+    d.gvs[q] = __builtin_fmaf(c.cs, __builtin_fmaf(-2.0f, lg, mg), c.is);
+    d.gil[q] = __builtin_fmaf(c.vx, c.vx, __builtin_fmaf(c.vz, c.vz, c.il));
 }
 
 // Velocity half step (SAVE: store strips of the new velocity; BORN: also the
@@ -104,12 +166,12 @@ template <bool SAVE, bool BORN, int PD>
 __global__ void __launch_bounds__(BLK) km_vel(Geo g, const DevShot *__restrict__ shots, int it, int dir) {
     Seg s;
     if (!seg_of(g, s)) return;
-    const DevShot &d = shots[blockIdx.y];
+    const DevShot d = shots[s.shot];   // by value: the field pointers stay in registers (stores cannot alias them)
     const Lane l = lane_of(g, s);
     constexpr int NF = BORN ? 2 : 1;
     float wxx[NF][8], wxz[NF][8];
     float qxx[NF][PD], qxz[NF][PD], qzz[NF][PD];
-    VelCell qst[NF][PD];
+    VelCell qst[NF][PD] = {};
     auto fetch = [&](int f, int j, int c) {   // operands of column c into ring slot j
         const Fields &F = f ? d.bn : d.fw;
         qxx[f][j] = ld(F.txx, g, l, c + 4);
@@ -176,12 +238,12 @@ template <bool GRAD, bool BORN, int PD>
 __global__ void __launch_bounds__(BLK) km_str(Geo g, const DevShot *__restrict__ shots, int dir) {
     Seg s;
     if (!seg_of(g, s)) return;
-    const DevShot &d = shots[blockIdx.y];
+    const DevShot d = shots[s.shot];   // by value: the field pointers stay in registers (stores cannot alias them)
     const Lane l = lane_of(g, s);
     constexpr int NF = BORN ? 2 : 1;
     float wvx[NF][8], wvz[NF][8];   // vx: columns c-4.., vz: columns c-3..
     float qvx[NF][PD], qvz[NF][PD];
-    StrCell qst[NF][PD];
+    StrCell qst[NF][PD] = {};
     auto split_of = [&](int f, int c) {
         const bool band = in_band(g, c, l.z);
         return f ? band : (band || (c == d.isx && l.z == d.isz));
@@ -252,11 +314,11 @@ template <int PD>
 __global__ void __launch_bounds__(BLK) km_adj_str(Geo g, const DevShot *__restrict__ shots, int dir) {
     Seg s;
     if (!seg_of(g, s)) return;
-    const DevShot &d = shots[blockIdx.y];
+    const DevShot d = shots[s.shot];   // by value: the field pointers stay in registers (stores cannot alias them)
     const Lane l = lane_of(g, s);
     const Fields &F = d.bk;
     float wvx[8], wvz[8], qvx[PD], qvz[PD];
-    AdjStrCell qst[PD];
+    AdjStrCell qst[PD] = {};
     auto fetch = [&](int j, int c) {
         qvx[j] = ld(F.vx, g, l, c + 3);
         qvz[j] = ld(F.vz, g, l, c + 4);
@@ -302,35 +364,62 @@ template <int PD>
 __global__ void __launch_bounds__(BLK) km_adj_vel_image(Geo g, const DevShot *__restrict__ shots, int dir) {
     Seg s;
     if (!seg_of(g, s)) return;
-    const DevShot &d = shots[blockIdx.y];
+    const DevShot d = shots[s.shot];   // by value: the field pointers stay in registers (stores cannot alias them)
     const Lane l = lane_of(g, s);
     const Fields &F = d.bk;
     const Medium &m = d.m;
     // p1 = (lam+2mu) txx + lam tzz on columns c-3..c+4; p2 = mu txz on c-4..c+3;
     // p3 = (lam+2mu) tzz + lam txx at column c.
-    auto P1 = [&](int x) {
-        if (!l.zin) return 0.0f;
-        const int i = x * g.nzp + l.z;
-        return adj_p1(m.lam[i], m.mu[i], F.txx[i], F.tzz[i]);
+    // the raw operands travel in the prefetch ring; products are formed when
+    // the column is consumed (a product at load time would wait for the load)
+    struct PCell {
+        float lam, mu, a, b;
     };
-    auto P2 = [&](int x) {
-        if (!l.zin) return 0.0f;
-        const int i = x * g.nzp + l.z;
-        return m.mu[i] * F.txz[i];
+    auto PL = [&](int x, const float *a, const float *b) {   // adj_p1(lam, mu, a[x], b[x])
+        const int i = x * g.nzp + l.zc;
+        PCell p;
+        p.lam = m.lam[i];
+        p.mu = m.mu[i];
+        p.a = a[i];
+        p.b = b[i];
+        return p;
     };
-    auto P3 = [&](int x) {
-        if (!l.zin) return 0.0f;
-        const int i = x * g.nzp + l.z;
-        return adj_p1(m.lam[i], m.mu[i], F.tzz[i], F.txx[i]);
+    auto PM = [&](int x) {                                   // mu txz
+        const int i = x * g.nzp + l.zc;
+        PCell p;
+        p.lam = 0.0f;
+        p.mu = m.mu[i];
+        p.a = F.txz[i];
+        p.b = 0.0f;
+        return p;
+    };
+    auto Pv = [&](const PCell &p) {
+        const float v = adj_p1(p.lam, p.mu, p.a, p.b);
+        return l.zin ? v : 0.0f;
+    };
+    auto PMv = [&](const PCell &p) {
+        const float v = p.mu * p.a;
+        return l.zin ? v : 0.0f;
+    };
+    auto P1 = [&](int x) { return Pv(PL(x, F.txx, F.tzz)); };
+    auto P2 = [&](int x) { return PMv(PM(x)); };
+    // image index of the lane's cell in column c, or -1 outside the interior
+    auto img_q = [&](int c) {
+        const int jx = c - g.pml, jz = l.z - g.pml;
+        return (l.act && jx >= 0 && jx < g.nx && jz >= 0 && jz < g.nz) ? jx * g.nz + jz : -1;
     };
     auto split_of = [&](int c) { return in_band(g, c, l.z) || l.z == d.igz; };
-    float w1[8], w2[8], q1[PD], q2[PD], q3[PD];
-    VelCell qst[PD];
+    float w1[8], w2[8];
+    PCell q1[PD], q2[PD], q3[PD];
+    VelCell qst[PD] = {};
+    ImgCell qim[PD] = {};
     auto fetch = [&](int j, int c) {
-        q1[j] = P1(c + 4);
-        q2[j] = P2(c + 3);
-        q3[j] = P3(c);
-        qst[j] = l.act ? vel_load(m, F, c * g.nzp + l.z, split_of(c)) : VelCell{};
+        q1[j] = PL(c + 4, F.txx, F.tzz);
+        q2[j] = PM(c + 3);
+        q3[j] = PL(c, F.tzz, F.txx);
+        vel_load_m(m, F, c * g.nzp + l.zc, any_lane(l.act && split_of(c)), qst[j]);
+        const int q = img_q(c);
+        img_load(d, c * g.nzp + l.zc, q < 0 ? 0 : q, any_lane(q >= 0), qim[j]);
     };
 #pragma unroll
     for (int j = 0; j < 7; ++j) {
@@ -347,10 +436,13 @@ __global__ void __launch_bounds__(BLK) km_adj_vel_image(Geo g, const DevShot *__
             if (c >= s.x1) break;
             const bool split = split_of(c);
             const int i = c * g.nzp + l.z;
-            w1[7] = q1[j];
-            w2[7] = q2[j];
-            const float p3 = q3[j];
+            w1[7] = Pv(q1[j]);
+            w2[7] = PMv(q2[j]);
+            const float p3 = Pv(q3[j]);
             VelCell st = qst[j];
+            ImgCell im = qim[j];
+            im.sxx = q3[j].b;                    // txx, tzz at column c
+            im.szz = q3[j].a;
             if (c + PD < s.x1) fetch(j, c + PD);
             const float sx = x_win(w1);          // Dx+ p1
             const float sxz_x = x_win(w2);       // Dx- p2
@@ -361,8 +453,8 @@ __global__ void __launch_bounds__(BLK) km_adj_vel_image(Geo g, const DevShot *__
             if (l.act) {
                 vel_math(g, st, split, dir, sx, sxz_z, sz, sxz_x);
                 vel_store(F, i, st, split);
-                const int jx = c - g.pml, jz = l.z - g.pml;
-                if (jx >= 0 && jx < g.nx && jz >= 0 && jz < g.nz) image_fast(d, i, jx * g.nz + jz);
+                const int q = img_q(c);
+                if (q >= 0) img_store(d, q, im);
             }
         }
     }

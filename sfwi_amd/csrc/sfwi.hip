@@ -426,7 +426,16 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
     hipStream_t st = c->stream;
     const DevShot *D = c->d_shots;
 
-    const dim3 tiles((unsigned)(8 * g.cm_per_xcd), (unsigned)n);
+    // column-march launches: block order per sfwi_params.march_order
+    Geo gm = g;
+    dim3 tiles((unsigned)(8 * g.cm_per_xcd), (unsigned)n);
+    if (c->p.march_order == 2) {
+        const long ntile = (g.cm_waves + BLK / 64 - 1) / (BLK / 64);
+        gm.cm_order = 1;
+        gm.cm_nshots = n;
+        gm.cm_per_xcd = (int)((ntile * n + 7) / 8);
+        tiles = dim3((unsigned)(8 * gm.cm_per_xcd), 1);
+    }
     const bool percell = E || c->p.march_cols < 0;   // per-cell kernels on request (march_cols < 0)
     for (int it = 0; it < nt; ++it) {
         hipLaunchKernelGGL(k_source, src1, dim3(64), 0, st, g, D, c->wav, it, n);
@@ -441,11 +450,11 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
             }
         } else {
             if (born) {
-                if (grad) hipLaunchKernelGGL((km_vel<true, true, PD>), tiles, blk, 0, st, g, D, it, 1);
-                else hipLaunchKernelGGL((km_vel<false, true, PD>), tiles, blk, 0, st, g, D, it, 1);
+                if (grad) hipLaunchKernelGGL((km_vel<true, true, PD>), tiles, blk, 0, st, gm, D, it, 1);
+                else hipLaunchKernelGGL((km_vel<false, true, PD>), tiles, blk, 0, st, gm, D, it, 1);
             } else {
-                if (grad) hipLaunchKernelGGL((km_vel<true, false, PD>), tiles, blk, 0, st, g, D, it, 1);
-                else hipLaunchKernelGGL((km_vel<false, false, PD>), tiles, blk, 0, st, g, D, it, 1);
+                if (grad) hipLaunchKernelGGL((km_vel<true, false, PD>), tiles, blk, 0, st, gm, D, it, 1);
+                else hipLaunchKernelGGL((km_vel<false, false, PD>), tiles, blk, 0, st, gm, D, it, 1);
             }
         }
         tend(c, t);
@@ -454,8 +463,8 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
             if (born) hipLaunchKernelGGL((k_str<E, false, true>), cells, blk, 0, st, g, D, 1);
             else hipLaunchKernelGGL((k_str<E, false, false>), cells, blk, 0, st, g, D, 1);
         } else {
-            if (born) hipLaunchKernelGGL((km_str<false, true, PD>), tiles, blk, 0, st, g, D, 1);
-            else hipLaunchKernelGGL((km_str<false, false, PD>), tiles, blk, 0, st, g, D, 1);
+            if (born) hipLaunchKernelGGL((km_str<false, true, PD>), tiles, blk, 0, st, gm, D, 1);
+            else hipLaunchKernelGGL((km_str<false, false, PD>), tiles, blk, 0, st, gm, D, 1);
         }
         tend(c, t);
         if (born) hipLaunchKernelGGL(k_record<true>, trg, dim3(64), 0, st, g, D, it);
@@ -483,20 +492,20 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
         hipLaunchKernelGGL(k_strip_load, stripg, blk, 0, st, g, D, it);
         int t = tbeg(c, SFWI_K_REC_STR, it);
         if (percell) hipLaunchKernelGGL((k_str<E, true, false>), cells, blk, 0, st, g, D, -1);
-        else hipLaunchKernelGGL((km_str<true, false, PD>), tiles, blk, 0, st, g, D, -1);
+        else hipLaunchKernelGGL((km_str<true, false, PD>), tiles, blk, 0, st, gm, D, -1);
         tend(c, t);
         t = tbeg(c, SFWI_K_REC_VEL, it);
         if (percell) hipLaunchKernelGGL((k_vel<E, false, false>), cells, blk, 0, st, g, D, it, -1);
-        else hipLaunchKernelGGL((km_vel<false, false, PD>), tiles, blk, 0, st, g, D, it, -1);
+        else hipLaunchKernelGGL((km_vel<false, false, PD>), tiles, blk, 0, st, gm, D, it, -1);
         tend(c, t);
         hipLaunchKernelGGL(k_inject, trg, dim3(64), 0, st, g, D, it);
         t = tbeg(c, SFWI_K_ADJ_STR, it);
         if (percell) hipLaunchKernelGGL(k_adj_str<E>, cells, blk, 0, st, g, D, adir);
-        else hipLaunchKernelGGL(km_adj_str<PD>, tiles, blk, 0, st, g, D, adir);
+        else hipLaunchKernelGGL(km_adj_str<PD>, tiles, blk, 0, st, gm, D, adir);
         tend(c, t);
         t = tbeg(c, SFWI_K_ADJ_VEL, it);
         if (percell) hipLaunchKernelGGL(k_adj_vel_image<E>, cells, blk, 0, st, g, D, adir);
-        else hipLaunchKernelGGL(km_adj_vel_image<PD>, tiles, blk, 0, st, g, D, adir);
+        else hipLaunchKernelGGL(km_adj_vel_image<PD>, tiles, blk, 0, st, gm, D, adir);
         tend(c, t);
     }
     const dim3 mg((unsigned)((c->nint + 255) / 256));
@@ -571,6 +580,7 @@ int sfwi_ctx_create(const sfwi_params *p, sfwi_ctx **out) {
     if (p->adjoint_dir != 1 && p->adjoint_dir != -1) return fail(nullptr, SFWI_EINVAL, "adjoint_dir must be +1 or -1");
     if (p->mode != SFWI_MODE_FAST && p->mode != SFWI_MODE_EXACT) return fail(nullptr, SFWI_EINVAL, "unknown mode %d", p->mode);
     if (p->storage != SFWI_STORE_STRIPS_F32) return fail(nullptr, SFWI_EINVAL, "storage %d not available", p->storage);
+    if (p->march_order < 0 || p->march_order > 2) return fail(nullptr, SFWI_EINVAL, "unknown march_order %d", p->march_order);
     if (p->misfit < SFWI_MISFIT_L2 || p->misfit > SFWI_MISFIT_OT_GATHER)
         return fail(nullptr, SFWI_EINVAL, "unknown misfit %d", p->misfit);
     if (p->use_graph) return fail(nullptr, SFWI_EINVAL, "use_graph not available in this build");

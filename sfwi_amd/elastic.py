@@ -44,7 +44,7 @@ class Context:
 
     def __init__(self, *, dx, dz, dt, nx, nz, pml, nt, ntr, allnx, allnz, rr=0.001, rho_const=0.0,
                  adjoint_dir=1, max_batch=1, mode="fast", storage="f32", use_graph=False, march_cols=0, march_depth=0,
-                 misfit="l2"):
+                 misfit="l2", march_order=0):
         self.L = _lib.load()
         p = SfwiParams()
         p.dx, p.dz, p.dt = dx, dz, dt
@@ -59,6 +59,7 @@ class Context:
         p.march_depth = int(march_depth)
         p.misfit = {"l2": _lib.MISFIT_L2, "ot_trace": _lib.MISFIT_OT_TRACE, "ot_power": _lib.MISFIT_OT_POWER,
                     "ot_gather": _lib.MISFIT_OT_GATHER}[misfit]
+        p.march_order = int(march_order)
         self.p = p
         h = ctypes.c_void_p()
         rc = self.L.sfwi_ctx_create(ctypes.byref(p), ctypes.byref(h))

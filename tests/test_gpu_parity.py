@@ -287,11 +287,12 @@ def test_single_trace_and_errors():
         assert np.array_equal(gz2, oz)
 
 
-@pytest.mark.parametrize("cols,depth", [(8, 1), (16, 2), (37, 4)])
-def test_march_kernels_match_per_cell(cols, depth):
+@pytest.mark.parametrize("cols,depth,order", [(8, 1, 2), (8, 1, 1), (16, 2, 2), (37, 4, 1)])
+def test_march_kernels_match_per_cell(cols, depth, order):
     """The column-march FAST kernels use the per-cell FAST update math in the
     same operation order, so gradients, misfits and Born traces agree bitwise
-    for any segment width (37: segments that do not divide the grid)."""
+    for any segment width (37: segments that do not divide the grid) and
+    either block order."""
     sv, dvp, dvs = _born_case()
     w = W.ricker(sv.nt, sv.dt, sv.f0)
     shots = []
@@ -300,7 +301,7 @@ def test_march_kernels_match_per_cell(cols, depth):
         shots.append(lib_shot(sv, sh, oz, ox))
     out = {}
     for mc in (-1, cols):
-        with context(sv, "fast", max_batch=2, march_cols=mc, march_depth=depth) as ctx:
+        with context(sv, "fast", max_batch=2, march_cols=mc, march_depth=depth, march_order=order) as ctx:
             ctx.set_wavelet(w)
             ctx.set_model(sv.vp0, sv.vs0, sv.rho0)
             m = ctx.batch_gradient(shots)
