@@ -72,8 +72,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--shots", type=int, default=64)
-    ap.add_argument("--nt", type=int, default=3000)
+    ap.add_argument("--workload", default="shot_batch", choices=["shot_batch", "mixed_precision", "plain_small"],
+                    help="shot_batch: 401x301, nt 3000, 64 shots (default); mixed_precision: 1200x600, "
+                         "dx 5 m, nt 6000, 96 shots; plain_small: 376x201, nt 2000, 16 shots")
+    ap.add_argument("--shots", type=int, default=0, help="shots (0: the workload's)")
+    ap.add_argument("--nt", type=int, default=0, help="time samples (0: the workload's)")
+    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"],
+                    help="boundary-strip storage of the reconstruction")
     ap.add_argument("--mode", default="fast", choices=["fast", "exact"])
     ap.add_argument("--timing-every", type=int, default=20)
     ap.add_argument("--march-cols", type=int, default=0,
@@ -81,6 +86,14 @@ def main():
     ap.add_argument("--march-depth", type=int, default=0, help="FAST march kernels: prefetch depth")
     ap.add_argument("--march-order", type=int, default=0,
                     help="FAST march kernels: block order (0 default, 1 shot after shot, 2 shots interleaved)")
+    ap.add_argument("--serving", action="store_true",
+                    help="run the serving-loop benchmark (Poisson request stream) instead")
+    ap.add_argument("--uneven", action="store_true",
+                    help="run the uneven-batch benchmark (120 shots over four grids, autotuned, BF16 strips)")
+    ap.add_argument("--requests", type=int, default=24)
+    ap.add_argument("--rate", type=float, default=1.0, help="serving: requests per second")
+    ap.add_argument("--scale", type=float, default=1.0, help="serving: request size factor")
+    ap.add_argument("--workers", type=int, default=4, help="serving: concurrent solves")
     ap.add_argument("--cpu-shots", type=int, default=2)
     ap.add_argument("--plain-shots", type=int, default=4,
                     help="shots timed through the plain one-shot path (0: skip)")
@@ -98,7 +111,17 @@ def main():
     from sfwi_amd import _lib
     from sfwi_amd.elastic import Context, Shot
 
-    sv = W.shot_batch(nshots=args.shots, nt=args.nt)
+    if args.serving:
+        return serving(args)
+    if args.uneven:
+        return uneven(args)
+
+    kw = {}
+    if args.shots:
+        kw["nshots"] = args.shots
+    if args.nt:
+        kw["nt"] = args.nt
+    sv = W.BY_NAME[args.workload](**kw)
     w = W.ricker(sv.nt, sv.dt, sv.f0)
     geo = [W.shot_geometry(sv, s) for s in sv.shots]
     n_int = sv.nx * sv.allnz
@@ -108,7 +131,7 @@ def main():
         sv.name, len(sv.shots), sv.nx, sv.allnz, sv.pml, n_cells, sv.nt), file=sys.stderr, flush=True)
 
     ctx = Context(dx=sv.dx, dz=sv.dz, dt=sv.dt, nx=sv.nx, nz=sv.allnz, pml=sv.pml, nt=sv.nt, ntr=sv.ntr,
-                  allnx=sv.allnx, allnz=sv.allnz, max_batch=len(sv.shots), mode=args.mode,
+                  allnx=sv.allnx, allnz=sv.allnz, max_batch=len(sv.shots), mode=args.mode, storage=args.storage,
                   march_cols=args.march_cols, march_depth=args.march_depth, march_order=args.march_order)
     ctx.set_wavelet(w)
     ctx.set_model(sv.vp, sv.vs, sv.rho)
@@ -172,7 +195,8 @@ def main():
     roofline = None
     if dom:
         d = kernels[dom]
-        measured = (args.mode == "fast" and args.march_cols in (0, 8) and args.march_depth in (0, 1))
+        measured = (args.workload == "shot_batch" and args.mode == "fast" and args.march_cols in (0, 8)
+                    and args.march_depth in (0, 1) and args.march_order in (0, 1))
         traffic = round(MEASURED_HBM_BYTES_PER_SHOT[dom] * nsh) if measured else None
         roofline = {"kernel": dom, "bound": "hbm", "achieved": d["GBps"], "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(d["GBps"] / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -239,9 +263,10 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * total / args.steps, 2),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded model, observed traces from a forward solve of the true model)",
-        "config": {"workload": "shot_batch: 401x301 dx=dz=10m, pml 30, dt 1ms, nt %d, %d shots, 12 Hz, "
-                               "401 receivers, batched gradient" % (sv.nt, nsh),
-                   "mode": args.mode, "storage": "boundary strips FP32"},
+        "config": {"workload": "%s: %dx%d dx=dz=%gm, pml %d, dt %gms, nt %d, %d shots, %g Hz, "
+                               "%d receivers, batched gradient" % (sv.name, sv.nx, sv.allnz, sv.dx, sv.pml,
+                                                                    sv.dt * 1e3, sv.nt, nsh, sv.f0, sv.ntr),
+                   "mode": args.mode, "storage": "boundary strips %s" % args.storage.upper()},
         "gcell_updates_per_s": round(gcells, 2),
         "pcie_inclusive_value": round(len(shots) / t_pcie, 4),
         "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 2),
@@ -255,6 +280,163 @@ def main():
     }
     print(json.dumps(line), flush=True)
     ctx.close()
+
+
+def _request(kind, sv):
+    """Arguments of one serving request (zero observed traces for gradients)."""
+    from sfwi_amd import workloads as W
+    from sfwi_amd.elastic import Shot
+    born = kind.startswith("born")
+    shots = []
+    for s in sv.shots:
+        isx, isz, igz = W.shot_geometry(sv, s)
+        z = np.zeros((len(s.gc), sv.nt), np.float32) if "gradient" in kind else None
+        shots.append(Shot(s.left, isx, isz, igz, s.ngx_left, s.gc, z, z))
+    params = dict(dx=sv.dx, dz=sv.dz, dt=sv.dt, nx=sv.nx, nz=sv.allnz, pml=sv.pml, nt=sv.nt, ntr=sv.ntr,
+                  allnx=sv.allnx, allnz=sv.allnz, max_batch=len(shots),
+                  rho_const=2000.0 if born else 0.0, adjoint_dir=-1 if born else 1)
+    return dict(kind=kind, shots=shots, wavelet=W.ricker(sv.nt, sv.dt, sv.f0), vp=sv.vp0, vs=sv.vs0,
+                rho=sv.rho0, dvp=(sv.vp - sv.vp0) if born else None, dvs=(sv.vs - sv.vs0) if born else None,
+                params=params)
+
+
+def _submit(srv, r):
+    return srv.submit(r["kind"], r["shots"], r["wavelet"], r["vp"], r["vs"], r["rho"], r["dvp"], r["dvs"],
+                      **r["params"])
+
+
+def _run_alone(r):
+    """One request through the plain one-shot path: per-cell kernels, one shot per call."""
+    from sfwi_amd.elastic import Context
+    params = dict(r["params"], max_batch=1, march_cols=-1)
+    with Context(**params) as ctx:
+        ctx.set_wavelet(r["wavelet"])
+        ctx.set_model(r["vp"], r["vs"], r["rho"])
+        if r["dvp"] is not None:
+            ctx.set_perturbation(r["dvp"], r["dvs"])
+        for s in r["shots"]:
+            if r["kind"] == "forward":
+                ctx.forward(s)
+            elif r["kind"] == "born_forward":
+                ctx.born_forward(s)
+            elif r["kind"] == "gradient":
+                ctx.gradient(s)
+            else:
+                ctx.born_gradient(s)
+        ctx.flush()
+
+
+def uneven(args):
+    """Uneven batch: 120 shots over four models of different grid size and
+    record length, one batched gradient request per model, all submitted to
+    the server at once (they run concurrently, each context autotuned for its
+    grid and batch), FP32 compute with BF16 boundary strips.  Observed traces
+    come from forward requests through the true models (untimed)."""
+    from sfwi_amd import workloads as W
+    from sfwi_amd.elastic import Server, Shot
+    svs = W.uneven_batch(shots_per_model=max(1, args.shots // 4) if args.shots else 30)
+    storage = args.storage if args.storage != "f32" else "bf16"
+    reqs = []
+    with Server(workers=4) as srv:
+        fw = []
+        for sv in svs:
+            r = _request("forward", sv)
+            r["vp"], r["vs"], r["rho"] = sv.vp, sv.vs, sv.rho
+            fw.append(_submit(srv, r))
+        obs = [srv.wait(t) for t in fw]
+        for sv, o in zip(svs, obs):
+            r = _request("gradient", sv)
+            r["shots"] = [Shot(sh.c.left, sh.c.isx, sh.c.isz, sh.c.igz, sh.c.ngx_left, sh.gc,
+                               o["syn_z"][k][:sh.ntr], o["syn_x"][k][:sh.ntr]) for k, sh in enumerate(r["shots"])]
+            r["params"] = dict(r["params"], storage=storage)
+            reqs.append(r)
+        t0 = time.perf_counter()
+        tks = [srv.submit(r["kind"], r["shots"], r["wavelet"], r["vp"], r["vs"], r["rho"], autotune=True,
+                          **r["params"]) for r in reqs]
+        outs = [srv.wait(t) for t in tks]
+        makespan = time.perf_counter() - t0
+    nsh = sum(len(r["shots"]) for r in reqs)
+    cells = sum(sv.cells() * sv.nt * len(sv.shots) * 3 for sv in svs)
+    line = {
+        "metric": "shot-gradients/s & Gcell-updates/s (uneven batch)", "value": round(nsh / makespan, 4),
+        "unit": "shot-gradients/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32",
+        "data": "synthetic (seeded models, observed traces from forward solves of the true models)",
+        "config": {"workload": "uneven batch: %d shots over %s" % (nsh, ", ".join(
+            "%dx%d nt %d %g Hz" % (sv.nx, sv.allnz, sv.nt, sv.f0) for sv in svs)),
+            "storage": "boundary strips %s" % storage.upper(), "autotuned": True},
+        "gcell_updates_per_s": round(cells / makespan / 1e9, 2),
+        "makespan_s": round(makespan, 3),
+        "per_model_run_s": [round(o["run_ms"] / 1e3, 3) for o in outs],
+        "misfits": [float(o["misfits"].sum()) for o in outs],
+    }
+    print(json.dumps(line), flush=True)
+
+
+def serving(args):
+    """Serving loop: a seeded Poisson stream of forward, Born and gradient
+    requests of random shape (nx 200-1500, nz 150-800, nt 1500-6000, 1-8
+    shots, 10-20 Hz) submitted in real time to the library's server, which
+    admits them against the HBM budget and overlaps them on the GPU.  Reports
+    p50/p99 of request latency (submit to done) and throughput, beside the same
+    requests run one at a time through the plain one-shot path."""
+    import threading
+    from sfwi_amd import workloads as W
+    from sfwi_amd.elastic import Server
+    stream = W.serving_requests(args.requests, rate=args.rate, scale=args.scale)
+    reqs = [(t, _request(k, sv), sv) for t, k, sv in stream]
+    cells = sum(sv.cells() * sv.nt * len(sv.shots) * {"forward": 1, "born_forward": 2, "gradient": 3,
+                                                        "born_gradient": 4}[r["kind"]] for _, r, sv in reqs)
+    shots = sum(len(r["shots"]) for _, r, _ in reqs)
+    print("serving: %d requests, %d shots, %.1f Gcell-updates, arrivals over %.1f s" % (
+        len(reqs), shots, cells / 1e9, reqs[-1][0]), file=sys.stderr, flush=True)
+    lat = [None] * len(reqs)
+    done_at = [None] * len(reqs)
+    with Server(workers=args.workers) as srv:
+        # warm-up: one small request so code objects are loaded before the clock starts
+        wt = _submit(srv, _request("forward", W.tiny(nt=50)))
+        srv.wait(wt)
+        t0 = time.perf_counter()
+        threads = []
+
+        def waiter(i, tk, t_sub):
+            srv.wait(tk)
+            done_at[i] = time.perf_counter()
+            lat[i] = done_at[i] - t_sub
+
+        for i, (t_arr, r, _) in enumerate(reqs):
+            now = time.perf_counter() - t0
+            if t_arr > now:
+                time.sleep(t_arr - now)
+            t_sub = time.perf_counter()
+            tk = _submit(srv, r)
+            th = threading.Thread(target=waiter, args=(i, tk, t_sub))
+            th.start()
+            threads.append(th)
+        for th in threads:
+            th.join()
+        makespan = max(done_at) - t0
+    print("serving: makespan %.2f s" % makespan, file=sys.stderr, flush=True)
+    t1 = time.perf_counter()
+    for _, r, _ in reqs:
+        _run_alone(r)
+    t_alone = time.perf_counter() - t1
+    print("serving: one at a time %.2f s" % t_alone, file=sys.stderr, flush=True)
+    lms = sorted(1e3 * x for x in lat)
+    line = {
+        "metric": "serving p50/p99 request latency and throughput", "value": round(shots / makespan, 4),
+        "unit": "shot-solves/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32",
+        "data": "synthetic (seeded layered models, zero observed traces)",
+        "config": {"workload": "serving: %d Poisson requests at %.2f/s, nx 200-1500, nz 150-800, nt 1500-6000 "
+                               "(x%.2f), 1-8 shots, forward/Born/gradient mix" % (len(reqs), args.rate, args.scale),
+                   "workers": args.workers},
+        "latency_ms": {"p50": round(float(np.percentile(lms, 50)), 1),
+                       "p99": round(float(np.percentile(lms, 99)), 1), "unit": "submit to done"},
+        "gcell_updates_per_s": round(cells / makespan / 1e9, 2),
+        "makespan_s": round(makespan, 3),
+        "one_at_a_time": {"value": round(shots / t_alone, 4), "unit": "shot-solves/s", "seconds": round(t_alone, 3),
+                          "path": "plain one-shot path (per-cell kernels, one shot per call), requests back to back"},
+    }
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

@@ -44,10 +44,14 @@ extern "C" {
 #define SFWI_MISFIT_OT_POWER 2    /* variant 1, per trace, squared signal        */
 #define SFWI_MISFIT_OT_GATHER 3   /* variant 2, whole gather of a component      */
 
-/* Forward-wavefield storage used by the reverse-time reconstruction. */
-#define SFWI_STORE_STRIPS_F32 0   /* 8-cell boundary strips of vx, vz, FP32    */
-#define SFWI_STORE_STRIPS_F16 1   /* same strips, FP16 with per-step scale     */
-#define SFWI_STORE_STRIPS_BF16 2  /* same strips, BF16                         */
+/* Forward-wavefield storage used by the reverse-time reconstruction: the
+ * 8-cell boundary strips of vx and vz of every step (the reference's
+ * record_edge_wavefield, sfwi-stage1/FD2DGPU.cu:3842-3869).  The 16-bit forms
+ * keep one FP32 max-abs scale per step and field and store v / scale rounded
+ * to nearest even; they halve the strip memory (the largest per-shot buffer). */
+#define SFWI_STORE_STRIPS_F32 0   /* FP32                                        */
+#define SFWI_STORE_STRIPS_F16 1   /* FP16 with per-step max-abs scale            */
+#define SFWI_STORE_STRIPS_BF16 2  /* BF16 with per-step max-abs scale            */
 
 typedef struct sfwi_ctx sfwi_ctx;
 
@@ -69,7 +73,9 @@ typedef struct sfwi_params {
     int max_batch;         /* shots resident on the device at once (>= 1)           */
     int mode;              /* SFWI_MODE_*                                           */
     int storage;           /* SFWI_STORE_*                                          */
-    int use_graph;         /* 1: replay the time loop from a captured hipGraph      */
+    int use_graph;         /* 1: capture each solve (all time steps) of a batch size */
+                           /* once into a hipGraph and replay it (L2 misfit; solves */
+                           /* with sfwi_set_timing on run directly)                 */
     int march_cols;        /* FAST kernels: columns per wave segment of the column-  */
                            /* march kernels (0 = 8); < 0 selects per-cell kernels */
     int march_depth;       /* columns of operands prefetched ahead (1, 2, 4; 0 = 1) */
@@ -176,6 +182,52 @@ int sfwi_ot_misfit(sfwi_ctx *ctx, int variant, int ntr, int nt, float dt, const 
                    const float *obs_z, const float *syn_x, const float *syn_z, float *res_x,
                    float *res_z, double *misfit);
 
+/* ---------------------------------------------------------------------------
+ * Serving loop (no reference counterpart: the reference runs one shot per MPI
+ * rank at a time).  Requests of any grid, record length and kind enter a FIFO
+ * queue; a pool of worker threads, each with its own context and stream,
+ * takes the first queued request whose device memory fits the free part of
+ * the HBM budget, so work that arrives while other solves run is admitted as
+ * workers and memory free up and overlaps with them on the GPU.  Contexts are
+ * cached by (params, kind) with their wavefield slots; idle ones are evicted
+ * least recently used first when memory is needed.  Each request's result is
+ * the one the same solve gives alone.  All buffers of a request are
+ * caller-owned and must stay valid until sfwi_server_wait returns for it.
+ * Errors: negative codes; the message is sfwi_last_error(NULL).
+ * ------------------------------------------------------------------------- */
+#define SFWI_REQ_FORWARD 0          /* sfwi_batch_forward                       */
+#define SFWI_REQ_BORN_FORWARD 1     /* sfwi_shot_born_forward per shot          */
+#define SFWI_REQ_GRADIENT 2         /* sfwi_batch_gradient + sfwi_flush         */
+#define SFWI_REQ_BORN_GRADIENT 3    /* sfwi_batch_born_gradient + sfwi_flush    */
+
+typedef struct sfwi_request {
+    int kind;                       /* SFWI_REQ_*                                   */
+    sfwi_params params;             /* solve shape; max_batch >= nshots             */
+    const float *wavelet;           /* nt                                           */
+    const float *vp, *vs, *rho;     /* allnx*allnz (rho may be NULL with rho_const) */
+    const float *dvp, *dvs;         /* Born kinds                                   */
+    int nshots;
+    const sfwi_shot *shots;
+    float *syn_z, *syn_x;           /* forward kinds: traces of shot i at i*ntr*nt  */
+                                    /* (Born forward: scattered traces)            */
+    double *misfits;                /* gradient kinds: [nshots] (may be NULL)       */
+    float *grad_vp, *grad_vs, *illum; /* gradient kinds: allnx*allnz, summed over   */
+                                    /* the request's shots (any may be NULL)       */
+    int autotune;                   /* 1: run sfwi_autotune (100 steps) on a newly  */
+                                    /* created context before its first solve      */
+} sfwi_request;
+
+typedef struct sfwi_server sfwi_server;
+/* Device memory one context of this shape needs for `nshots` shots of `kind`. */
+size_t sfwi_solve_bytes(const sfwi_params *p, int kind, int nshots);
+/* hbm_budget 0: 15/16 of the free HBM; workers in [1, 16]. */
+int sfwi_server_create(size_t hbm_budget, int workers, sfwi_server **out);
+int sfwi_server_submit(sfwi_server *s, const sfwi_request *req, long long *ticket);
+/* Blocks until the request is done; queue_ms = submit -> start, run_ms = start -> end. */
+int sfwi_server_wait(sfwi_server *s, long long ticket, double *queue_ms, double *run_ms);
+/* Call after every submitted request has been waited for. */
+void sfwi_server_destroy(sfwi_server *s);
+
 /* Reference host-driver helper: Ricker wavelet (sfwi-stage1/main.cpp:251-260). */
 void sfwi_ricker(float *w, int nt, float dt, float f0);
 
@@ -194,6 +246,17 @@ void sfwi_ricker(float *w, int nt, float dt, float f0);
 int sfwi_set_timing(sfwi_ctx *ctx, int sample_every);
 int sfwi_kernel_time(sfwi_ctx *ctx, int kernel_id, double *avg_ms, int *launches);
 const char *sfwi_kernel_name(int kernel_id);
+
+/* Kernel-family autotuner of the FAST mode (no reference counterpart): for a
+ * batch of nshots shots of this context's grid, times a gradient solve of
+ * `steps` time steps for each candidate -- per-cell kernels; column march with
+ * 8 or 16 columns per segment, prefetch depth 1 or 2, shot-major or
+ * shot-interleaved block order -- and keeps the fastest in the context.
+ * Needs the model and wavelet set.  Accumulated gradients are preserved; the
+ * observed records resident in the batch slots are invalidated.  Outputs may
+ * be NULL; best_ms is the winner's time for the `steps`-step solve. */
+int sfwi_autotune(sfwi_ctx *ctx, int nshots, int steps, int *march_cols, int *march_depth, int *march_order,
+                  double *best_ms);
 
 /* Library version string. */
 const char *sfwi_version(void);

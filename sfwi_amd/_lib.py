@@ -17,6 +17,7 @@ SFWI_OK, SFWI_EINVAL, SFWI_EHIP, SFWI_ENOMEM, SFWI_ESTATE = 0, -1, -2, -3, -4
 MODE_FAST, MODE_EXACT = 0, 1
 STORE_STRIPS_F32, STORE_STRIPS_F16, STORE_STRIPS_BF16 = 0, 1, 2
 MISFIT_L2, MISFIT_OT_TRACE, MISFIT_OT_POWER, MISFIT_OT_GATHER = 0, 1, 2, 3
+REQ_FORWARD, REQ_BORN_FORWARD, REQ_GRADIENT, REQ_BORN_GRADIENT = 0, 1, 2, 3
 K_FWD_VEL, K_FWD_STR, K_REC_STR, K_REC_VEL, K_ADJ_STR, K_ADJ_VEL, K_COUNT = range(7)
 
 
@@ -40,6 +41,18 @@ class SfwiShot(ctypes.Structure):
         ("igz", ctypes.c_int), ("ngx_left", ctypes.c_int), ("ntr", ctypes.c_int),
         ("gc", ctypes.POINTER(ctypes.c_int)),
         ("obs_z", ctypes.POINTER(ctypes.c_float)), ("obs_x", ctypes.POINTER(ctypes.c_float)),
+    ]
+
+
+class SfwiRequest(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int), ("params", SfwiParams),
+        ("wavelet", ctypes.c_void_p), ("vp", ctypes.c_void_p), ("vs", ctypes.c_void_p),
+        ("rho", ctypes.c_void_p), ("dvp", ctypes.c_void_p), ("dvs", ctypes.c_void_p),
+        ("nshots", ctypes.c_int), ("shots", ctypes.POINTER(SfwiShot)),
+        ("syn_z", ctypes.c_void_p), ("syn_x", ctypes.c_void_p), ("misfits", ctypes.c_void_p),
+        ("grad_vp", ctypes.c_void_p), ("grad_vs", ctypes.c_void_p), ("illum", ctypes.c_void_p),
+        ("autotune", ctypes.c_int),
     ]
 
 
@@ -99,6 +112,15 @@ def load():
     L.sfwi_kernel_time.argtypes = [vp, ctypes.c_int, dp, P(ctypes.c_int)]
     L.sfwi_kernel_name.restype = ctypes.c_char_p
     L.sfwi_kernel_name.argtypes = [ctypes.c_int]
+    L.sfwi_autotune.argtypes = [vp, ctypes.c_int, ctypes.c_int, P(ctypes.c_int), P(ctypes.c_int),
+                                P(ctypes.c_int), dp]
+    L.sfwi_solve_bytes.restype = ctypes.c_size_t
+    L.sfwi_solve_bytes.argtypes = [P(SfwiParams), ctypes.c_int, ctypes.c_int]
+    L.sfwi_server_create.argtypes = [ctypes.c_size_t, ctypes.c_int, P(vp)]
+    L.sfwi_server_submit.argtypes = [vp, P(SfwiRequest), P(ctypes.c_longlong)]
+    L.sfwi_server_wait.argtypes = [vp, ctypes.c_longlong, dp, dp]
+    L.sfwi_server_destroy.argtypes = [vp]
+    L.sfwi_server_destroy.restype = None
     _LIB = L
     return L
 

@@ -61,7 +61,10 @@ struct DevShot {
     const float *oz, *ox;                  // observed traces
     const int *igx;                        // receiver columns [ntr]
     const int *inj;                        // traces injected by the adjoint [ninj]
-    float *strip;                          // boundary strips, [nt][2 fields][SF]
+    float *strip;                          // boundary strips, [nt][2 fields][SF] (FP32 storage)
+    float *stage;                          // one step's strips [2][SF] before packing (16-bit storage)
+    unsigned short *strip_h;               // packed strips [nt][2][SF] (FP16 / BF16 storage)
+    float *strip_scale;                    // max-abs scale per step and field [nt][2]
     float *gvp, *gvs, *gil;                // per-shot gradient accumulators [nx][nz]
     double *trace_sum;                     // per-trace partial misfit [ntr]
     int isx, isz, igz, ntr, ninj, left;
@@ -78,6 +81,7 @@ struct Geo {
     int cm_order;           // 0: grid.y = shot; 1: 1-D grid, shots of a tile adjacent
     int cm_nshots;          // shots of the launch (cm_order 1)
     int sf;                 // strip floats per field and step
+    int store;              // strip storage: 0 FP32, 1 FP16, 2 BF16 (per-step max-abs scale)
     float inv_nzd;
     float dt, dx, dz, dtdx, dtdz, idx, idz;
 };
@@ -563,6 +567,18 @@ __device__ inline void adj_vel_cell(const Geo &g, const Medium &m, const Fields 
 
 // ------------------------------------------------------------ kernels ---
 
+// 16-bit encodings of a value in [-1, 1]: FP16 (store 1) or BF16 (store 2),
+// both rounded to nearest even.
+__device__ inline unsigned short pack16(int store, float x) {
+    if (store == 1) return __builtin_bit_cast(unsigned short, (_Float16)x);
+    const unsigned u = __builtin_bit_cast(unsigned, x);
+    return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ inline float unpack16(int store, unsigned short h) {
+    if (store == 1) return (float)__builtin_bit_cast(_Float16, h);
+    return __builtin_bit_cast(float, (unsigned)h << 16);
+}
+
 // Strip slots of a padded cell: up, down, left, right (-1 when outside).
 // Layout per field and step: up [nx][2N] | down [nx][2N] | left [2N][nz] | right [2N][nz]
 // (sfwi-stage1/FD2DGPU.cu:3842-3869 index formulas).
@@ -608,7 +624,7 @@ __global__ void __launch_bounds__(BLK) k_vel(Geo g, const DevShot *__restrict__ 
         strip_slots(g, ix, iz, sl);
         if (sl[0] >= 0 || sl[1] >= 0 || sl[2] >= 0 || sl[3] >= 0) {
             const float vx = d.fw.vx[i], vz = d.fw.vz[i];
-            float *base = d.strip + (size_t)it * 2 * g.sf;
+            float *base = g.store ? d.stage : d.strip + (size_t)it * 2 * g.sf;
 #pragma unroll
             for (int k = 0; k < 4; ++k)
                 if (sl[k] >= 0) {
@@ -677,9 +693,39 @@ __global__ void k_strip_load(Geo g, const DevShot *__restrict__ shots, int it) {
         iz = p + u % g.nz;
     }
     const int i = ix * g.nzp + iz;
-    const float *base = d.strip + (size_t)it * 2 * g.sf;
-    d.fw.vx[i] = base[t];
-    d.fw.vz[i] = base[g.sf + t];
+    if (g.store) {
+        const unsigned short *h = d.strip_h + (size_t)it * 2 * g.sf;
+        const float *sc = d.strip_scale + 2 * (size_t)it;
+        d.fw.vx[i] = unpack16(g.store, h[t]) * sc[0];
+        d.fw.vz[i] = unpack16(g.store, h[g.sf + t]) * sc[1];
+    } else {
+        const float *base = d.strip + (size_t)it * 2 * g.sf;
+        d.fw.vx[i] = base[t];
+        d.fw.vz[i] = base[g.sf + t];
+    }
+}
+
+// 16-bit strip storage of step `it`: per field, scale = max |v| over the
+// step's strip, stored value = v / scale rounded to nearest even in FP16
+// (store 1) or BF16 (store 2).  One workgroup per (field, shot).
+__global__ void __launch_bounds__(256) k_strip_pack(Geo g, const DevShot *__restrict__ shots, int it) {
+    __shared__ float red[256];
+    const DevShot &d = shots[blockIdx.y];
+    const int f = blockIdx.x;
+    const float *v = d.stage + (size_t)f * g.sf;
+    float m = 0.0f;
+    for (int k = threadIdx.x; k < g.sf; k += 256) m = fmaxf(m, fabsf(v[k]));
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    const float scale = red[0];
+    const float inv = scale > 0.0f ? 1.0f / scale : 0.0f;
+    unsigned short *h = d.strip_h + ((size_t)it * 2 + f) * g.sf;
+    for (int k = threadIdx.x; k < g.sf; k += 256) h[k] = pack16(g.store, v[k] * inv);
+    if (threadIdx.x == 0) d.strip_scale[2 * (size_t)it + f] = scale;
 }
 
 // Receiver sampling (sfwi-stage1/FD2DGPU.cu:3158-3166): vz, vx of the

@@ -218,7 +218,7 @@ __global__ void __launch_bounds__(BLK) km_vel(Geo g, const DevShot *__restrict__
                     if (SAVE && f == 0 && split) {
                         int sl[4];
                         strip_slots(g, c, l.z, sl);
-                        float *base = d.strip + (size_t)it * 2 * g.sf;
+                        float *base = g.store ? d.stage : d.strip + (size_t)it * 2 * g.sf;
 #pragma unroll
                         for (int k = 0; k < 4; ++k)
                             if (sl[k] >= 0) {

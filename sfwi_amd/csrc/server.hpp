@@ -1,0 +1,280 @@
+// Serving loop over the per-shot solves (host code, included by sfwi.hip).
+//
+// Requests (forward, Born forward, gradient, Born gradient solves of any grid
+// and record length, each with 1..max_batch shots) enter a FIFO queue.  A pool
+// of worker threads, each driving its own context and stream, takes the
+// first queued request whose device memory fits the free part of the HBM
+// budget, so requests that arrive while others run are admitted as workers
+// and memory free up, and their kernels overlap on the GPU.  Contexts are
+// cached by shape (every sfwi_params field) and request kind and stay
+// resident with their wavefield slots; an idle cached context is evicted
+// (least recently used first) when a new request needs its memory.  A request's result does not depend on what else runs:
+// each solve uses its own context and the batched kernels are deterministic.
+#pragma once
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <list>
+#include <mutex>
+#include <thread>
+
+namespace {
+
+size_t solve_bytes(const sfwi_params &p, int kind, int nshots) {
+    const size_t f = sizeof(float);
+    const size_t ncell = (size_t)(p.nx + 2 * p.pml) * (p.nz + 2 * p.pml);
+    const size_t nmodel = (size_t)p.allnx * p.allnz, nrec = (size_t)p.ntr * p.nt, nint = (size_t)p.nx * p.nz;
+    const int n = std::min(std::max(nshots, 1), std::max(p.max_batch, 1));
+    const bool born = kind == SFWI_REQ_BORN_FORWARD || kind == SFWI_REQ_BORN_GRADIENT;
+    const bool grad = kind == SFWI_REQ_GRADIENT || kind == SFWI_REQ_BORN_GRADIENT;
+    const size_t sf = 16 * (size_t)(p.nx + p.nz);
+    size_t fixed = p.nt * f + 12 * nmodel * f + 2 * nrec * f + p.max_batch * 1024 + 2 * p.ntr * 8 + 4096;
+    size_t slot = 34 * ncell * f + 6 * nrec * f + 3 * nint * f + 16 * (size_t)p.ntr + 64;
+    if (born) slot += 15 * ncell * f;
+    if (grad) slot += p.storage ? (size_t)p.nt * 2 * sf * 2 + (size_t)p.nt * 8 + 2 * sf * f + 256
+                                : (size_t)p.nt * 2 * sf * f;
+    const size_t window = 12 * ncell * f;
+    size_t ot = 0;
+    if (p.misfit != SFWI_MISFIT_L2 && grad) {
+        const size_t jobs = p.misfit == SFWI_MISFIT_OT_GATHER ? 2 * (size_t)n : 2 * (size_t)n * p.ntr;
+        const size_t len = p.misfit == SFWI_MISFIT_OT_GATHER ? nrec : (size_t)p.nt;
+        ot = std::min<size_t>(jobs, 1024) * 4 * (len + 1) * 8 + jobs * 48;
+    }
+    return fixed + n * (slot + window) + ot;
+}
+
+bool same_params(const sfwi_params &a, const sfwi_params &b) { return std::memcmp(&a, &b, sizeof a) == 0; }
+
+double ms_since(std::chrono::steady_clock::time_point t0, std::chrono::steady_clock::time_point t1) {
+    return std::chrono::duration<double, std::milli>(t1 - t0).count();
+}
+
+}  // namespace
+
+struct sfwi_server {
+    struct Job {
+        long long ticket;
+        sfwi_request req;
+        std::vector<sfwi_shot> shots;
+        size_t bytes;
+        std::chrono::steady_clock::time_point t_submit, t_start, t_end;
+        int status = 1;   // 1 queued, 2 running, 0 done, < 0 error code
+        std::string err;
+    };
+    struct Cached {
+        sfwi_params p;
+        int kind;
+        sfwi_ctx *ctx;
+        size_t bytes;
+        bool busy;
+        long long last_use;
+    };
+    size_t budget = 0, in_use = 0;   // in_use: running solves + cached idle contexts
+    int device = 0;
+    bool stop = false;
+    long long next_ticket = 1, use_clock = 0;
+    std::mutex m;
+    std::condition_variable cv_work, cv_done;
+    std::deque<long long> queue;
+    std::map<long long, Job> jobs;
+    std::list<Cached> cache;
+    std::vector<std::thread> workers;
+    std::string err;
+
+    // Context of shape p for a job of `bytes` (lock held).  Reuses an idle
+    // cached context, else evicts idle ones until the new one fits.
+    bool acquire(const sfwi_params &p, int kind, size_t bytes, std::list<Cached>::iterator *out) {
+        for (auto it = cache.begin(); it != cache.end(); ++it)
+            if (!it->busy && it->kind == kind && same_params(it->p, p)) {
+                it->busy = true;
+                it->last_use = ++use_clock;
+                *out = it;
+                return true;
+            }
+        while (in_use + bytes > budget) {
+            auto lru = cache.end();
+            for (auto it = cache.begin(); it != cache.end(); ++it)
+                if (!it->busy && (lru == cache.end() || it->last_use < lru->last_use)) lru = it;
+            if (lru == cache.end()) return false;
+            sfwi_ctx_destroy(lru->ctx);
+            in_use -= lru->bytes;
+            cache.erase(lru);
+        }
+        cache.push_back({p, kind, nullptr, bytes, true, ++use_clock});
+        in_use += bytes;
+        *out = std::prev(cache.end());
+        return true;
+    }
+
+    int run_job(Job &J, sfwi_ctx *c, bool fresh) {
+        const sfwi_request &q = J.req;
+        int r = sfwi_set_wavelet(c, q.wavelet);
+        if (!r) r = sfwi_set_model(c, q.vp, q.vs, q.rho);
+        const bool born = q.kind == SFWI_REQ_BORN_FORWARD || q.kind == SFWI_REQ_BORN_GRADIENT;
+        if (!r && born) r = sfwi_set_perturbation(c, q.dvp, q.dvs);
+        if (!r && fresh && q.autotune && q.params.mode == SFWI_MODE_FAST)
+            r = sfwi_autotune(c, q.nshots, 100, nullptr, nullptr, nullptr, nullptr);
+        if (r) return r;
+        const int n = q.nshots;
+        const size_t rec = (size_t)q.params.ntr * q.params.nt;
+        switch (q.kind) {
+        case SFWI_REQ_FORWARD:
+            return sfwi_batch_forward(c, n, J.shots.data(), q.syn_z, q.syn_x);
+        case SFWI_REQ_BORN_FORWARD:
+            for (int k = 0; k < n && !r; ++k)
+                r = sfwi_shot_born_forward(c, &J.shots[k], q.syn_z ? q.syn_z + k * rec : nullptr,
+                                           q.syn_x ? q.syn_x + k * rec : nullptr, nullptr, nullptr);
+            return r;
+        case SFWI_REQ_GRADIENT:
+        case SFWI_REQ_BORN_GRADIENT: {
+            std::vector<double> mis(n, 0.0);
+            r = q.kind == SFWI_REQ_GRADIENT ? sfwi_batch_gradient(c, n, J.shots.data(), mis.data())
+                                            : sfwi_batch_born_gradient(c, n, J.shots.data(), mis.data());
+            if (!r) r = sfwi_flush(c, q.grad_vp, q.grad_vs, q.illum);
+            if (!r && q.misfits)
+                for (int k = 0; k < n; ++k) q.misfits[k] = mis[k];
+            return r;
+        }
+        }
+        return SFWI_EINVAL;
+    }
+
+    void worker() {
+        hipSetDevice(device);
+        std::unique_lock<std::mutex> lk(m);
+        for (;;) {
+            // first queued job whose memory fits (reusing or evicting idle contexts)
+            Job *J = nullptr;
+            std::list<Cached>::iterator slot;
+            for (;;) {
+                if (stop) return;
+                for (auto it = queue.begin(); it != queue.end(); ++it) {
+                    Job &cand = jobs[*it];
+                    if (acquire(cand.req.params, cand.req.kind, cand.bytes, &slot)) {
+                        J = &cand;
+                        queue.erase(it);
+                        break;
+                    }
+                }
+                if (J) break;
+                cv_work.wait(lk);
+            }
+            J->status = 2;
+            J->t_start = std::chrono::steady_clock::now();
+            sfwi_ctx *c = slot->ctx;
+            lk.unlock();
+            int r = 0;
+            std::string msg;
+            const bool fresh = !c;
+            if (!c) {
+                r = sfwi_ctx_create(&J->req.params, &c);
+                if (r) msg = sfwi_last_error(nullptr);
+            }
+            if (!r) {
+                r = run_job(*J, c, fresh);
+                if (r) msg = sfwi_last_error(c);
+            }
+            lk.lock();
+            if (c && !slot->ctx) slot->ctx = c;
+            slot->busy = false;
+            if (!slot->ctx) {   // creation failed: release its reservation
+                in_use -= slot->bytes;
+                cache.erase(slot);
+            }
+            J->t_end = std::chrono::steady_clock::now();
+            J->status = r ? r : 0;
+            J->err = msg;
+            cv_done.notify_all();
+            cv_work.notify_all();
+        }
+    }
+};
+
+extern "C" {
+
+size_t sfwi_solve_bytes(const sfwi_params *p, int kind, int nshots) {
+    return p ? solve_bytes(*p, kind, nshots) : 0;
+}
+
+int sfwi_server_create(size_t hbm_budget, int nworkers, sfwi_server **out) {
+    if (!out) return fail(nullptr, SFWI_EINVAL, "NULL argument");
+    *out = nullptr;
+    if (nworkers < 1 || nworkers > 16) return fail(nullptr, SFWI_EINVAL, "workers=%d outside [1,16]", nworkers);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(nullptr, SFWI_EHIP, "no HIP device");
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return fail(nullptr, SFWI_EHIP, "hipMemGetInfo failed");
+    if (hbm_budget == 0) hbm_budget = fr - fr / 16;
+    if (hbm_budget > fr) return fail(nullptr, SFWI_ENOMEM, "budget %zu MiB exceeds free HBM (%zu MiB)",
+                                     hbm_budget >> 20, fr >> 20);
+    sfwi_server *s = new sfwi_server();
+    s->budget = hbm_budget;
+    s->device = dev;
+    for (int k = 0; k < nworkers; ++k) s->workers.emplace_back([s] { s->worker(); });
+    *out = s;
+    return 0;
+}
+
+int sfwi_server_submit(sfwi_server *s, const sfwi_request *q, long long *ticket) {
+    if (!s || !q || !ticket) return fail(nullptr, SFWI_EINVAL, "NULL argument");
+    if (q->kind < SFWI_REQ_FORWARD || q->kind > SFWI_REQ_BORN_GRADIENT)
+        return fail(nullptr, SFWI_EINVAL, "unknown request kind %d", q->kind);
+    if (q->nshots < 1 || !q->shots || !q->wavelet || !q->vp || !q->vs)
+        return fail(nullptr, SFWI_EINVAL, "request needs shots, wavelet and model");
+    if (q->nshots > q->params.max_batch)
+        return fail(nullptr, SFWI_EINVAL, "request of %d shots exceeds params.max_batch=%d", q->nshots,
+                    q->params.max_batch);
+    const bool born = q->kind == SFWI_REQ_BORN_FORWARD || q->kind == SFWI_REQ_BORN_GRADIENT;
+    if (born && (!q->dvp || !q->dvs)) return fail(nullptr, SFWI_EINVAL, "Born request without perturbation");
+    // a cached context keeps the slots of up to max_batch shots: account for all of them
+    const size_t bytes = solve_bytes(q->params, q->kind, q->params.max_batch);
+    std::lock_guard<std::mutex> lk(s->m);
+    if (bytes > s->budget)
+        return fail(nullptr, SFWI_ENOMEM, "request needs %zu MiB, more than the server budget (%zu MiB)",
+                    bytes >> 20, s->budget >> 20);
+    const long long t = s->next_ticket++;
+    sfwi_server::Job &J = s->jobs[t];
+    J.ticket = t;
+    J.req = *q;
+    J.shots.assign(q->shots, q->shots + q->nshots);
+    J.bytes = bytes;
+    J.t_submit = std::chrono::steady_clock::now();
+    s->queue.push_back(t);
+    s->cv_work.notify_one();
+    *ticket = t;
+    return 0;
+}
+
+int sfwi_server_wait(sfwi_server *s, long long ticket, double *queue_ms, double *run_ms) {
+    if (!s) return fail(nullptr, SFWI_EINVAL, "NULL server");
+    std::unique_lock<std::mutex> lk(s->m);
+    auto it = s->jobs.find(ticket);
+    if (it == s->jobs.end()) return fail(nullptr, SFWI_EINVAL, "unknown ticket %lld", ticket);
+    s->cv_done.wait(lk, [&] { return it->second.status <= 0; });
+    sfwi_server::Job &J = it->second;
+    if (queue_ms) *queue_ms = ms_since(J.t_submit, J.t_start);
+    if (run_ms) *run_ms = ms_since(J.t_start, J.t_end);
+    const int r = J.status;
+    if (r) {
+        s->err = J.err;
+        fail(nullptr, r, "request %lld: %s", ticket, J.err.c_str());
+    }
+    s->jobs.erase(it);
+    return r;
+}
+
+void sfwi_server_destroy(sfwi_server *s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->m);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    for (auto &t : s->workers) t.join();
+    for (auto &cc : s->cache)
+        if (cc.ctx) sfwi_ctx_destroy(cc.ctx);
+    delete s;
+}
+
+}  // extern "C"

@@ -30,7 +30,7 @@ struct Slot {
     char *block = nullptr;      // fw(15) + bk(15) + derivatives(4) fields
     char *born = nullptr;       // bn(15) fields
     char *rec = nullptr;        // rz rx r2z r2x oz ox, ints, accumulators, trace sums
-    float *strip = nullptr;     // [nt][2][sf]
+    float *strip = nullptr;     // FP32 [nt][2][sf], or 16-bit [nt][2][sf] + scales [nt][2] + stage [2][sf]
     Fields fw{}, bk{}, bn{};
     float *dg[4]{};
     float *rz, *rx, *r2z, *r2x, *oz, *ox;
@@ -76,6 +76,7 @@ struct sfwi_ctx {
     int *ot_first = nullptr;
     float *ot_io = nullptr;
     size_t ot_jobs_cap = 0, ot_ws_cap = 0, ot_mis_cap = 0, ot_first_cap = 0, ot_io_cap = 0;
+    std::map<std::pair<int, int>, hipGraphExec_t> graphs;   // captured solves by (kind, batch size)
     std::vector<OtJob> ot_h;          // host copies kept alive for the async uploads
     std::vector<int> ot_first_h;
     // timing
@@ -242,7 +243,10 @@ int ensure_slots(sfwi_ctx *c, int n, bool born, bool strips) {
             for (int k = 0; k < 15; ++k) *bn[k] = f + k * c->ncell;
         }
         if (strips && !S.strip) {
-            int r = dalloc(c, (void **)&S.strip, (size_t)c->p.nt * 2 * c->g.sf * sizeof(float));
+            const size_t nv = (size_t)c->p.nt * 2 * c->g.sf;
+            const size_t bytes = c->g.store ? nv * 2 + (size_t)c->p.nt * 2 * sizeof(float) + 2 * (size_t)c->g.sf * sizeof(float) + 256
+                                            : nv * sizeof(float);
+            int r = dalloc(c, (void **)&S.strip, bytes);
             if (r) return r;
         }
     }
@@ -308,6 +312,14 @@ int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium 
         d.dgx = S.dg[0]; d.dgz = S.dg[1]; d.dgxz = S.dg[2]; d.dgzx = S.dg[3];
         d.rz = S.rz; d.rx = S.rx; d.r2z = S.r2z; d.r2x = S.r2x; d.oz = S.oz; d.ox = S.ox;
         d.igx = S.igx; d.inj = S.inj; d.strip = S.strip;
+        if (c->g.store && S.strip) {
+            const size_t nv = (size_t)c->p.nt * 2 * c->g.sf;
+            char *q = (char *)S.strip;
+            d.strip_scale = (float *)q;
+            d.stage = d.strip_scale + (size_t)c->p.nt * 2;
+            d.strip_h = (unsigned short *)(d.stage + 2 * (size_t)c->g.sf);
+            (void)nv;
+        }
         d.gvp = S.gvp; d.gvs = S.gvs; d.gil = S.gil; d.trace_sum = S.trace_sum;
         d.isx = s.isx; d.isz = s.isz; d.igz = s.igz; d.ntr = s.ntr; d.ninj = (int)S.h_inj.size();
         d.left = s.left;
@@ -330,6 +342,19 @@ int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium 
 }
 
 // ------------------------------------------------------------- solves ---
+
+// Column-march geometry of a segment width (0: 8 columns).
+void set_march_geo(Geo &g, int cols) {
+    g.cm_xs = cols > 0 ? cols : 8;
+    g.cm_nbands = (g.nzd + CM_ROWS - 1) / CM_ROWS;
+    g.cm_waves = ((g.nxp - 2 * NOP + g.cm_xs - 1) / g.cm_xs) * g.cm_nbands;
+    g.cm_per_xcd = ((g.cm_waves + BLK / 64 - 1) / (BLK / 64) + 7) / 8;
+}
+
+void drop_graphs(sfwi_ctx *c) {
+    for (auto &kv : c->graphs) hipGraphExecDestroy(kv.second);
+    c->graphs.clear();
+}
 
 // ----------------------------------------------------- optimal transport ---
 
@@ -458,6 +483,7 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
             }
         }
         tend(c, t);
+        if (grad && g.store) hipLaunchKernelGGL(k_strip_pack, dim3(2, (unsigned)n), dim3(256), 0, st, g, D, it);
         t = tbeg(c, SFWI_K_FWD_STR, it);
         if (percell) {
             if (born) hipLaunchKernelGGL((k_str<E, false, true>), cells, blk, 0, st, g, D, 1);
@@ -514,7 +540,7 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
     return 0;
 }
 
-int run(sfwi_ctx *c, Kind kind, int n) {
+int run_direct(sfwi_ctx *c, Kind kind, int n) {
     if (c->p.mode == SFWI_MODE_EXACT) return run_batch<true, 1>(c, kind, n);
     switch (c->p.march_depth) {
     case 1: return run_batch<false, 1>(c, kind, n);
@@ -522,6 +548,41 @@ int run(sfwi_ctx *c, Kind kind, int n) {
     case 4: return run_batch<false, 4>(c, kind, n);
     default: return run_batch<false, 1>(c, kind, n);
     }
+}
+
+// With use_graph the whole solve of a (kind, batch size) pair -- every time
+// step of the forward and reverse loops -- is captured once from the
+// context's stream into a hipGraph and replayed afterwards.  Every kernel
+// argument is a per-context constant (the geometry, the device shot table,
+// the wavelet and output buffers) or the step index; the per-shot state the
+// kernels read (slot and medium pointers, geometry) lives in the device shot
+// table that prepare() uploads before each replay, so one graph serves any
+// shots of that batch size.  The capture holds kernels only (no memset or
+// host copy).  Solves that need host work between kernels (the OT misfit)
+// and timed solves run directly.
+int run(sfwi_ctx *c, Kind kind, int n) {
+    const bool host_work = c->p.misfit != SFWI_MISFIT_L2 && kind != K_FWD && kind != K_BORN_FWD;
+    if (!c->p.use_graph || host_work || c->timing_every > 0) return run_direct(c, kind, n);
+    const std::pair<int, int> key((int)kind, n);
+    auto it = c->graphs.find(key);
+    if (it == c->graphs.end()) {
+        HIPCK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int r = run_direct(c, kind, n);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        if (r) {
+            if (graph) hipGraphDestroy(graph);
+            return r;
+        }
+        if (e != hipSuccess) return fail(c, SFWI_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+        hipGraphExec_t exec = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (ei != hipSuccess) return fail(c, SFWI_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+        it = c->graphs.emplace(key, exec).first;
+    }
+    HIPCK(c, hipGraphLaunch(it->second, c->stream));
+    return 0;
 }
 
 int finish(sfwi_ctx *c) {
@@ -579,11 +640,11 @@ int sfwi_ctx_create(const sfwi_params *p, sfwi_ctx **out) {
         return fail(nullptr, SFWI_EINVAL, "dx, dz, dt and rr must be positive");
     if (p->adjoint_dir != 1 && p->adjoint_dir != -1) return fail(nullptr, SFWI_EINVAL, "adjoint_dir must be +1 or -1");
     if (p->mode != SFWI_MODE_FAST && p->mode != SFWI_MODE_EXACT) return fail(nullptr, SFWI_EINVAL, "unknown mode %d", p->mode);
-    if (p->storage != SFWI_STORE_STRIPS_F32) return fail(nullptr, SFWI_EINVAL, "storage %d not available", p->storage);
+    if (p->storage < SFWI_STORE_STRIPS_F32 || p->storage > SFWI_STORE_STRIPS_BF16)
+        return fail(nullptr, SFWI_EINVAL, "unknown strip storage %d", p->storage);
     if (p->march_order < 0 || p->march_order > 2) return fail(nullptr, SFWI_EINVAL, "unknown march_order %d", p->march_order);
     if (p->misfit < SFWI_MISFIT_L2 || p->misfit > SFWI_MISFIT_OT_GATHER)
         return fail(nullptr, SFWI_EINVAL, "unknown misfit %d", p->misfit);
-    if (p->use_graph) return fail(nullptr, SFWI_EINVAL, "use_graph not available in this build");
     const long nxp = p->nx + 2L * p->pml, nzp = p->nz + 2L * p->pml;
     if (nxp * nzp >= (1L << 31) / 16) return fail(nullptr, SFWI_EINVAL, "grid too large for 32-bit cell indices");
     int dev = 0;
@@ -599,11 +660,9 @@ int sfwi_ctx_create(const sfwi_params *p, sfwi_ctx **out) {
     g.ndom = (g.nxp - 2 * NOP) * g.nzd;
     const int nb = (g.ndom + BLK - 1) / BLK;
     g.per_xcd = (nb + 7) / 8;
-    g.cm_xs = p->march_cols > 0 ? p->march_cols : 8;
-    g.cm_nbands = (g.nzd + CM_ROWS - 1) / CM_ROWS;
-    g.cm_waves = ((g.nxp - 2 * NOP + g.cm_xs - 1) / g.cm_xs) * g.cm_nbands;
-    g.cm_per_xcd = ((g.cm_waves + BLK / 64 - 1) / (BLK / 64) + 7) / 8;
+    set_march_geo(g, p->march_cols);
     g.sf = 4 * NSTRIP * (p->nx + p->nz);
+    g.store = p->storage;
     g.inv_nzd = 1.0f / (float)g.nzd;
     g.dt = p->dt; g.dx = p->dx; g.dz = p->dz;
     g.dtdx = p->dt / p->dx; g.dtdz = p->dt / p->dz;
@@ -640,6 +699,7 @@ int sfwi_ctx_create(const sfwi_params *p, sfwi_ctx **out) {
 void sfwi_ctx_destroy(sfwi_ctx *c) {
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto &kv : c->graphs) hipGraphExecDestroy(kv.second);
     for (Slot &S : c->slots) {
         hipFree(S.block);
         hipFree(S.born);
@@ -831,6 +891,87 @@ int sfwi_set_timing(sfwi_ctx *c, int every) {
     return 0;
 }
 
+// Kernel-family autotuner of the FAST mode: for a batch of nshots shots of
+// this context's grid, times a gradient solve of `steps` time steps (zero
+// observed traces, one warm-up run per candidate) for each candidate and keeps
+// the fastest in the context.  Candidates: per-cell kernels; column march with
+// 8 or 16 columns per segment, prefetch depth 1 or 2, shot-major or
+// shot-interleaved block order.  Accumulated gradients are preserved; the
+// observed records resident in the batch slots are invalidated.
+int sfwi_autotune(sfwi_ctx *c, int nshots, int steps, int *march_cols, int *march_depth, int *march_order,
+                  double *best_ms) {
+    if (!c) return SFWI_EINVAL;
+    if (c->p.mode != SFWI_MODE_FAST) return fail(c, SFWI_ESTATE, "the autotuner chooses among FAST kernels");
+    if (!c->have_model || !c->have_wav) return fail(c, SFWI_ESTATE, "set the model and the wavelet first");
+    const int n = std::min(std::max(nshots, 1), c->p.max_batch);
+    const int ns = std::min(std::max(steps, 2), c->p.nt);
+    const int ntr = std::min(c->p.ntr, c->p.nx);
+    std::vector<int> gc(ntr);
+    for (int k = 0; k < ntr; ++k) gc[k] = (int)((float)k * c->p.dx);
+    std::vector<float> zero((size_t)ntr * c->p.nt, 0.0f);
+    std::vector<sfwi_shot> shots(n);
+    for (int k = 0; k < n; ++k) {
+        sfwi_shot &s = shots[k];
+        s.left = 0;
+        s.isx = c->g.pml + (int)((k + 0.5) * c->p.nx / n);
+        s.isz = c->g.pml + 1;
+        s.igz = c->g.pml;
+        s.ngx_left = 0;
+        s.ntr = ntr;
+        s.gc = gc.data();
+        s.obs_z = s.obs_x = zero.data();
+    }
+    const size_t acc = 3 * c->nmodel * sizeof(float);
+    HIPCK(c, hipMemcpyAsync(c->tmp_model, c->Gvp, acc, hipMemcpyDeviceToDevice, c->stream));
+    struct Cand { int cols, depth, order; };
+    const Cand cands[] = {{-1, 1, 1}, {8, 1, 1}, {8, 2, 1}, {16, 1, 1}, {8, 1, 2}};
+    const sfwi_params keep = c->p;
+    const int nt_keep = c->g.nt;
+    c->g.nt = ns;                              // a shorter record inside the same buffers
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int r = 0;
+    double best = 1e300;
+    Cand win = cands[1];
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) r = fail(c, SFWI_EHIP, "hipEventCreate");
+    for (const Cand &cd : cands) {
+        if (r) break;
+        c->p.march_cols = cd.cols;
+        c->p.march_depth = cd.depth;
+        c->p.march_order = cd.order;
+        set_march_geo(c->g, cd.cols);
+        float ms = 0.0f;
+        for (int rep = 0; rep < 2 && !r; ++rep) {
+            if ((r = prepare(c, K_GRAD, n, shots.data(), nullptr))) break;
+            if (rep && hipEventRecord(e0, c->stream) != hipSuccess) { r = fail(c, SFWI_EHIP, "hipEventRecord"); break; }
+            if ((r = run_direct(c, K_GRAD, n))) break;
+            if (rep && hipEventRecord(e1, c->stream) != hipSuccess) { r = fail(c, SFWI_EHIP, "hipEventRecord"); break; }
+            if (hipStreamSynchronize(c->stream) != hipSuccess) { r = fail(c, SFWI_EHIP, "autotune solve failed"); break; }
+        }
+        if (!r && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms < best) {
+            best = ms;
+            win = cd;
+        }
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    c->g.nt = nt_keep;
+    c->p = keep;
+    c->p.march_cols = win.cols;
+    c->p.march_depth = win.depth;
+    c->p.march_order = win.order;
+    set_march_geo(c->g, win.cols);
+    drop_graphs(c);                            // captured with another kernel family
+    for (Slot &S : c->slots) S.obs_ntr = 0;
+    HIPCK(c, hipMemcpyAsync(c->Gvp, c->tmp_model, acc, hipMemcpyDeviceToDevice, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    if (r) return r;
+    if (march_cols) *march_cols = win.cols;
+    if (march_depth) *march_depth = win.depth;
+    if (march_order) *march_order = win.order;
+    if (best_ms) *best_ms = best;
+    return 0;
+}
+
 int sfwi_kernel_time(sfwi_ctx *c, int kid, double *avg_ms, int *launches) {
     if (!c || kid < 0 || kid >= SFWI_K_COUNT) return fail(c, SFWI_EINVAL, "bad kernel id %d", kid);
     if (avg_ms) *avg_ms = c->kt_n[kid] ? c->kt_ms[kid] / c->kt_n[kid] : 0.0;
@@ -901,3 +1042,5 @@ int sfwi_ot_misfit(sfwi_ctx *c, int variant, int ntr, int nt, float dt, const fl
 }
 
 }  // extern "C"
+
+#include "server.hpp"

@@ -39,34 +39,39 @@ class Shot:
         return len(self.gc)
 
 
+def make_params(*, dx, dz, dt, nx, nz, pml, nt, ntr, allnx, allnz, rr=0.001, rho_const=0.0, adjoint_dir=1,
+                max_batch=1, mode="fast", storage="f32", use_graph=False, march_cols=0, march_depth=0,
+                misfit="l2", march_order=0):
+    """``sfwi_params`` of a context or request (keyword names as in the C struct)."""
+    p = SfwiParams()
+    p.dx, p.dz, p.dt = dx, dz, dt
+    p.nx, p.nz, p.pml, p.nt, p.nop, p.ntr = nx, nz, pml, nt, 4, ntr
+    p.allnx, p.allnz, p.rr, p.rho_const = allnx, allnz, rr, rho_const
+    p.adjoint_dir, p.max_batch = adjoint_dir, max_batch
+    p.mode = {"fast": _lib.MODE_FAST, "exact": _lib.MODE_EXACT}[mode]
+    p.storage = {"f32": _lib.STORE_STRIPS_F32, "f16": _lib.STORE_STRIPS_F16, "bf16": _lib.STORE_STRIPS_BF16}[storage]
+    p.use_graph = int(bool(use_graph))
+    p.march_cols = int(march_cols)
+    p.march_depth = int(march_depth)
+    p.misfit = {"l2": _lib.MISFIT_L2, "ot_trace": _lib.MISFIT_OT_TRACE, "ot_power": _lib.MISFIT_OT_POWER,
+                "ot_gather": _lib.MISFIT_OT_GATHER}[misfit]
+    p.march_order = int(march_order)
+    return p
+
+
 class Context:
     """One device context (``sfwi_ctx``)."""
 
-    def __init__(self, *, dx, dz, dt, nx, nz, pml, nt, ntr, allnx, allnz, rr=0.001, rho_const=0.0,
-                 adjoint_dir=1, max_batch=1, mode="fast", storage="f32", use_graph=False, march_cols=0, march_depth=0,
-                 misfit="l2", march_order=0):
+    def __init__(self, **kw):
         self.L = _lib.load()
-        p = SfwiParams()
-        p.dx, p.dz, p.dt = dx, dz, dt
-        p.nx, p.nz, p.pml, p.nt, p.nop, p.ntr = nx, nz, pml, nt, 4, ntr
-        p.allnx, p.allnz, p.rr, p.rho_const = allnx, allnz, rr, rho_const
-        p.adjoint_dir, p.max_batch = adjoint_dir, max_batch
-        p.mode = {"fast": _lib.MODE_FAST, "exact": _lib.MODE_EXACT}[mode]
-        p.storage = {"f32": _lib.STORE_STRIPS_F32, "f16": _lib.STORE_STRIPS_F16,
-                     "bf16": _lib.STORE_STRIPS_BF16}[storage]
-        p.use_graph = int(bool(use_graph))
-        p.march_cols = int(march_cols)
-        p.march_depth = int(march_depth)
-        p.misfit = {"l2": _lib.MISFIT_L2, "ot_trace": _lib.MISFIT_OT_TRACE, "ot_power": _lib.MISFIT_OT_POWER,
-                    "ot_gather": _lib.MISFIT_OT_GATHER}[misfit]
-        p.march_order = int(march_order)
+        p = make_params(**kw)
         self.p = p
         h = ctypes.c_void_p()
         rc = self.L.sfwi_ctx_create(ctypes.byref(p), ctypes.byref(h))
         if rc != 0:
             raise SfwiError(rc, self.L.sfwi_last_error(None).decode())
         self.h = h
-        self.nt, self.allnx, self.allnz = nt, allnx, allnz
+        self.nt, self.allnx, self.allnz = p.nt, p.allnx, p.allnz
 
     def _ck(self, rc):
         if rc != 0:
@@ -168,6 +173,15 @@ class Context:
         self._ck(self.L.sfwi_kernel_time(self.h, kid, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
 
+    def autotune(self, nshots, steps=100):
+        """Pick the fastest FAST kernel family for batches of nshots; returns
+        (march_cols, march_depth, march_order, best_ms)."""
+        a, b, o = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+        ms = ctypes.c_double(0.0)
+        self._ck(self.L.sfwi_autotune(self.h, int(nshots), int(steps), ctypes.byref(a), ctypes.byref(b),
+                                      ctypes.byref(o), ctypes.byref(ms)))
+        return a.value, b.value, o.value, ms.value
+
     def ot_misfit(self, obs_x, obs_z, syn_x, syn_z, dt, variant=0):
         obs_x, obs_z, syn_x, syn_z = map(_f32, (obs_x, obs_z, syn_x, syn_z))
         ntr, nt = obs_x.shape
@@ -176,6 +190,92 @@ class Context:
         self._ck(self.L.sfwi_ot_misfit(self.h, variant, ntr, nt, dt, obs_x, obs_z, syn_x, syn_z, rx, rz,
                                        ctypes.byref(m)))
         return m.value, rx, rz
+
+
+class Server:
+    """Serving loop (``sfwi_server_*``): submit forward / Born / gradient
+    requests of any shape; they run on a pool of workers with their own
+    contexts, admitted against an HBM budget, and overlap on the GPU.  The
+    arrays of a request are kept alive here until ``wait`` returns."""
+
+    KINDS = {"forward": _lib.REQ_FORWARD, "born_forward": _lib.REQ_BORN_FORWARD,
+             "gradient": _lib.REQ_GRADIENT, "born_gradient": _lib.REQ_BORN_GRADIENT}
+
+    def __init__(self, budget_bytes=0, workers=4):
+        self.L = _lib.load()
+        h = ctypes.c_void_p()
+        rc = self.L.sfwi_server_create(int(budget_bytes), int(workers), ctypes.byref(h))
+        if rc != 0:
+            raise SfwiError(rc, self.L.sfwi_last_error(None).decode())
+        self.h = h
+        self._live = {}
+
+    @staticmethod
+    def solve_bytes(kind, nshots, **params):
+        return _lib.load().sfwi_solve_bytes(ctypes.byref(make_params(**params)), Server.KINDS[kind], nshots)
+
+    def submit(self, kind, shots, wavelet, vp, vs, rho=None, dvp=None, dvs=None, autotune=False, **params):
+        params.setdefault("max_batch", len(shots))
+        p = make_params(**params)
+        q = _lib.SfwiRequest()
+        q.kind = self.KINDS[kind]
+        q.params = p
+        q.autotune = int(bool(autotune))
+        keep = [_f32(wavelet), _f32(vp), _f32(vs), None if rho is None else _f32(rho),
+                None if dvp is None else _f32(dvp), None if dvs is None else _f32(dvs)]
+        q.wavelet, q.vp, q.vs, q.rho, q.dvp, q.dvs = [_lib.ptr(a) for a in keep]
+        arr = (SfwiShot * len(shots))(*[s.c for s in shots])
+        q.nshots, q.shots = len(shots), arr
+        out = {}
+        if kind in ("forward", "born_forward"):
+            out["syn_z"] = np.zeros((len(shots), p.ntr, p.nt), np.float32)
+            out["syn_x"] = np.zeros_like(out["syn_z"])
+            q.syn_z, q.syn_x = _lib.ptr(out["syn_z"]), _lib.ptr(out["syn_x"])
+        else:
+            n = p.allnx * p.allnz
+            out["misfits"] = np.zeros(len(shots), np.float64)
+            out["grad_vp"], out["grad_vs"], out["illum"] = (np.zeros((p.allnx, p.allnz), np.float32) for _ in range(3))
+            q.misfits = _lib.ptr(out["misfits"])
+            q.grad_vp, q.grad_vs, q.illum = (_lib.ptr(out[k]) for k in ("grad_vp", "grad_vs", "illum"))
+            del n
+        t = ctypes.c_longlong(0)
+        rc = self.L.sfwi_server_submit(self.h, ctypes.byref(q), ctypes.byref(t))
+        if rc != 0:
+            raise SfwiError(rc, self.L.sfwi_last_error(None).decode())
+        self._live[t.value] = (q, keep, arr, list(shots), out)
+        return t.value
+
+    def wait(self, ticket):
+        qm, rm = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        rc = self.L.sfwi_server_wait(self.h, ticket, ctypes.byref(qm), ctypes.byref(rm))
+        rec = self._live.pop(ticket)
+        if rc != 0:
+            raise SfwiError(rc, self.L.sfwi_last_error(None).decode())
+        out = rec[4]
+        out["queue_ms"], out["run_ms"] = qm.value, rm.value
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            for t in list(self._live):
+                try:
+                    self.wait(t)
+                except SfwiError:
+                    pass
+            self.L.sfwi_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # --------------------------------------------------------------------------

@@ -209,9 +209,71 @@ def shot_geometry(sv: Survey, shot: Shot):
     return isx, isz, igz
 
 
+def mid_801x401(seed=1004, nshots=30, nt=4000):
+    """801x401, dx=dz=10 m, seeded 5-layer model with a Gaussian +8% Vp
+    inclusion, dt 1 ms, 30-cell PML, 12 Hz (the third grid of the uneven batch)."""
+    rng = np.random.default_rng(seed)
+    nx, nz = 801, 401
+    tops = np.concatenate(([0], np.sort(rng.choice(np.arange(20, nz - 20), 4, replace=False))))
+    vp = _layered(nx, nz, tops, np.sort(rng.uniform(1700.0, 4300.0, 5)))
+    X, Z = np.meshgrid(np.arange(nx), np.arange(nz), indexing="ij")
+    g = np.exp(-((X - rng.uniform(200, 600)) ** 2 + (Z - rng.uniform(120, 300)) ** 2) / (2 * 25.0 ** 2))
+    vp = (vp * (1.0 + 0.08 * g)).astype(np.float32)
+    vs = (vp / 1.73).astype(np.float32)
+    rho = gardner(vp)
+    vp0, vs0 = gaussian_smooth(vp, 15.0), gaussian_smooth(vs, 15.0)
+    sv = Survey("mid_801x401", vp, vs, rho, vp0, vs0, gardner(vp0), 10.0, 10.0, 0.001, nt, 30, 12.0, 2, 0, nx)
+    sv.shots = _full_aperture_shots(nx, 10.0, nshots)
+    return sv
+
+
+def uneven_batch(shots_per_model=30):
+    """Uneven batch: 120 shots over four models of different size and record
+    length -- 376x201 (nt 2000, 10 Hz), 401x301 (nt 3000, 12 Hz), 801x401
+    (nt 4000, 12 Hz), 1200x600 (dx 5 m, nt 6000, 15 Hz)."""
+    n = shots_per_model
+    return [plain_small(nshots=n, nt=2000), shot_batch(nshots=n, nt=3000), mid_801x401(nshots=n, nt=4000),
+            mixed_precision(nshots=n, nt=6000)]
+
+
+def serving_requests(n=24, seed=1005, rate=1.0, scale=1.0):
+    """Serving-loop request stream: Poisson arrivals at `rate` per second of
+    forward, Born and gradient solves; per request nx uniform in 200-1500, nz
+    in 150-800, nt in 1500-6000 (all multiplied by `scale`, for quick runs), a
+    seeded layered Vp/Vs/rho model, 1-8 shots and a 10-20 Hz Ricker wavelet;
+    dx = dz = 10 m, dt = 1 ms, 20-cell PML.  Gradient requests invert zero
+    observed traces (the misfit is the synthetic energy), Born requests
+    perturb Vp/Vs by a Gaussian anomaly.  Returns [(arrival_s, kind, survey)]."""
+    rng = np.random.default_rng(seed)
+    kinds = ["forward", "born_forward", "gradient", "born_gradient"]
+    t = 0.0
+    out = []
+    for k in range(n):
+        t += rng.exponential(1.0 / rate)
+        nx = max(32, int(rng.integers(200, 1501) * scale))
+        nz = max(32, int(rng.integers(150, 801) * scale))
+        nt = max(64, int(rng.integers(1500, 6001) * scale))
+        nsh = int(rng.integers(1, 9))
+        f0 = float(rng.uniform(10.0, 20.0))
+        nl = int(rng.integers(3, 7))
+        tops = np.sort(rng.choice(np.arange(1, nz - 1), nl - 1, replace=False))
+        vps = np.sort(rng.uniform(1800.0, 4200.0, nl))
+        vp = _layered(nx, nz, np.concatenate(([0], tops)), vps)
+        vs = (vp / 1.75).astype(np.float32)
+        rho = gardner(vp)
+        X, Z = np.meshgrid(np.arange(nx), np.arange(nz), indexing="ij")
+        g = np.exp(-((X - nx / 2) ** 2 + (Z - nz / 2) ** 2) / (2 * (0.1 * min(nx, nz)) ** 2)).astype(np.float32)
+        sv = Survey("request_%d_%dx%d" % (k, nx, nz), (vp * (1 + 0.05 * g)).astype(np.float32), vs, rho,
+                    vp, vs, rho, 10.0, 10.0, 0.001, nt, 20, f0, 2, 0, nx)
+        sv.shots = _full_aperture_shots(nx, 10.0, nsh)
+        out.append((t, kinds[int(rng.integers(0, 4))], sv))
+    return out
+
+
 BY_NAME = {
     "plain_small": plain_small,
     "shot_batch": shot_batch,
     "mixed_precision": mixed_precision,
+    "mid_801x401": mid_801x401,
     "tiny": tiny,
 }
