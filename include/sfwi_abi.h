@@ -82,6 +82,7 @@ typedef struct sfwi_params {
     int misfit;            /* SFWI_MISFIT_*                                          */
     int march_order;       /* column-march block order: 0 default (= 1); 1 one shot  */
                            /* after another; 2 the shots of a tile back to back     */
+    int grad_rho;          /* 1: gradient solves also image the density (sfwi_flush_rho) */
 } sfwi_params;
 
 /* One shot: the fields the reference driver writes into FD2DGPU_ELASTIC
@@ -163,6 +164,15 @@ int sfwi_linesearch_linear(sfwi_ctx *ctx, const sfwi_shot *shot, const float *cg
  * gradients and the illumination (allnx*allnz each) to the host and zeroes
  * them on the device.  Any pointer may be NULL. */
 int sfwi_flush(sfwi_ctx *ctx, float *grad_vp, float *grad_vs, float *illum);
+
+/* Density gradient at fixed Vp, Vs (no reference counterpart: the reference
+ * images Vp and Vs only, sfwi-stage1/FD2DGPU.cu:3350-3382).  With
+ * sfwi_params.grad_rho = 1 every gradient solve also accumulates, per reverse
+ * step and interior cell, -(vx_adj dvx + vz_adj dvz)/dt + (vp^2 - 2 vs^2) lg
+ * + vs^2 mg, where dv = v(it+1) - v(it) is the reconstruction's velocity step
+ * and lg, mg are the lambda and mu correlations of the Vp / Vs imaging, in
+ * the same units.  Copies the accumulated image (allnx*allnz) and zeroes it. */
+int sfwi_flush_rho(sfwi_ctx *ctx, float *grad_rho);
 
 /* Optimal-transport trace misfit and adjoint source.  Declared but never
  * defined in the reference (OT, OT_power, OT_alltrace,

@@ -57,6 +57,10 @@ def lib():
                                       _f32p, _f32p, _f32p, dp, _f32p, _f32p]
         L.or_gradient_f64.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _i32p, _f32p, _f32p,
                                       _f64p, _f64p, _f64p, dp, _f64p, _f64p]
+        L.or_gradient_rho_f32.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _i32p, _f32p, _f32p,
+                                          _f32p, _f32p, _f32p, _f32p, dp]
+        L.or_gradient_rho_f64.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _i32p, _f32p, _f32p,
+                                          _f64p, _f64p, _f64p, _f64p, dp]
         L.or_gradient_res_f32.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _i32p, _f32p, _f32p,
                                           _f32p, _f32p, _f32p]
         L.or_born_forward_f32.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _i32p,
@@ -141,6 +145,19 @@ def gradient(c, vp, vs, rho, wav, gc, obs_z, obs_x, prec="f32"):
                                           g1, g2, g3, ctypes.byref(m), rz, rx)
     sh = (c.allnx, c.allnz)
     return g1.reshape(sh), g2.reshape(sh), g3.reshape(sh), m.value, rz, rx
+
+
+def gradient_rho(c, vp, vs, rho, wav, gc, obs_z, obs_x, prec="f32"):
+    """Stage-1 gradient with the density image: (grad_vp, grad_vs, illum, grad_rho, misfit)."""
+    dt = np.float32 if prec == "f32" else np.float64
+    n = c.allnx * c.allnz
+    g = [np.zeros(n, dt) for _ in range(4)]
+    m = ctypes.c_double(0.0)
+    getattr(lib(), "or_gradient_rho_" + prec)(ctypes.byref(c), _f32(vp), _f32(vs), _f32(rho), _f32(wav),
+                                              np.ascontiguousarray(gc, np.int32), _f32(obs_z), _f32(obs_x),
+                                              *g, ctypes.byref(m))
+    sh = (c.allnx, c.allnz)
+    return g[0].reshape(sh), g[1].reshape(sh), g[2].reshape(sh), g[3].reshape(sh), m.value
 
 
 def gradient_res(c, vp, vs, rho, wav, gc, res_z, res_x):

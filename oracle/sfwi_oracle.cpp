@@ -432,11 +432,19 @@ template <typename T> struct Strips {
 
 // Zero-lag imaging of one reverse step, sfwi-stage1/FD2DGPU.cu:3350-3382
 // (doimage_gradient) and :3324-3331 (doimage_illum).
+// With grho (no reference counterpart: the reference images Vp and Vs only)
+// also the density gradient at fixed Vp, Vs in the same units:
+//   -(vx_adj dvx + vz_adj dvz) / dt + (vp^2 - 2 vs^2) lg + vs^2 mg,
+// dv = v(it+1) - v(it) of the reconstruction step, the first term the
+// d/drho of the velocity update v += dt/rho D sigma through the adjoint
+// mapping (adjoint v = a / rho for the exact multiplier a), the others the
+// chain rule through lambda = rho (vp^2 - 2 vs^2), mu = rho vs^2.
 template <typename T, typename G>
 void image_step(const or_config *c, const Grid &g, const Medium<T> &m, const Wave<T> &fw,
                 const Wave<T> &bk, const std::vector<T> &vx_gx, const std::vector<T> &vz_gz,
                 const std::vector<T> &vx_gz, const std::vector<T> &vz_gx, G *gvp, G *gvs, G *ill,
-                int nth) {
+                int nth, G *grho = nullptr, const std::vector<T> *ovx = nullptr,
+                const std::vector<T> *ovz = nullptr, T dt = T(0)) {
 #pragma omp parallel for schedule(static) num_threads(nth)
     for (int ix = 0; ix < g.nx; ++ix)
         for (int iz = 0; iz < g.nz; ++iz) {
@@ -451,6 +459,12 @@ void image_step(const or_config *c, const Grid &g, const Medium<T> &m, const Wav
             gvs[q] += one * ps;
             T il = fw.vx[p] * fw.vx[p] + fw.vz[p] * fw.vz[p];
             ill[q] += one * il;
+            if (grho) {
+                T dvx = (*ovx)[p] - fw.vx[p], dvz = (*ovz)[p] - fw.vz[p];
+                T rd = -(bk.vx[p] * dvx + bk.vz[p] * dvz) / dt;
+                T rg = rd + (m.vp[p] * m.vp[p] - 2 * m.vs[p] * m.vs[p]) * lg + m.vs[p] * m.vs[p] * mg;
+                grho[q] += one * rg;
+            }
         }
 }
 
@@ -553,17 +567,22 @@ template <typename T> struct Run {
     // (Q7), image.
     template <typename G>
     void reverse(Wave<T> &fw, Strips<T> &strips, const std::vector<T> &rz, const std::vector<T> &rx,
-                 G *gvp, G *gvs, G *ill) {
+                 G *gvp, G *gvs, G *ill, G *grho = nullptr) {
         Wave<T> bk(g.n);
         std::vector<T> vx_gx(g.n, T(0)), vz_gz(g.n, T(0)), vx_gz(g.n, T(0)), vz_gx(g.n, T(0));
+        std::vector<T> ovx, ovz;
         for (int it = g.nt - 1; it >= 0; --it) {
             strips.template move<false>(fw.vx.data(), fw.vz.data(), it, g.nt);
             update_s(g, st, m, fw, -1, nth, vx_gx.data(), vz_gz.data(), vx_gz.data(), vz_gx.data());
+            if (grho) {
+                ovx = fw.vx;
+                ovz = fw.vz;
+            }
             update_v(g, st, m, fw, -1, nth);
             inject(bk, it, rz, rx);
             adjoint_s(g, st, m, bk, c->adjoint_dir, nth);
             adjoint_v(g, st, m, bk, c->adjoint_dir, nth);
-            image_step(c, g, m, fw, bk, vx_gx, vz_gz, vx_gz, vz_gx, gvp, gvs, ill, nth);
+            image_step(c, g, m, fw, bk, vx_gx, vz_gz, vx_gz, vz_gx, gvp, gvs, ill, nth, grho, &ovx, &ovz, st.dt);
         }
     }
 };
@@ -571,7 +590,7 @@ template <typename T> struct Run {
 template <typename T, typename G>
 void gradient_impl(const or_config *c, const float *vp, const float *vs, const float *rho,
                    const float *wav, const int *gc, const float *obs_z, const float *obs_x, G *gvp,
-                   G *gvs, G *ill, double *misfit, G *res_z, G *res_x) {
+                   G *gvs, G *ill, double *misfit, G *res_z, G *res_x, G *grho = nullptr) {
     Run<T> r(c, vp, vs, rho, nullptr, nullptr, gc);
     const long nr = (long)c->ntr * c->nt;
     std::vector<T> sz(nr), sx(nr);
@@ -587,7 +606,7 @@ void gradient_impl(const or_config *c, const float *vp, const float *vs, const f
     *misfit += l2_sum(sz, sx, c->ntr, c->nt);
     if (res_z)
         for (long k = 0; k < nr; ++k) { res_z[k] = (G)sz[k]; res_x[k] = (G)sx[k]; }
-    r.reverse(fw, strips, sz, sx, gvp, gvs, ill);
+    r.reverse(fw, strips, sz, sx, gvp, gvs, ill, grho);
 }
 
 // Forward (strips) then the reverse loop with a caller-supplied adjoint
@@ -698,6 +717,18 @@ void or_gradient_f64(const or_config *c, const float *vp, const float *vs, const
                      const float *wav, const int *gc, const float *obs_z, const float *obs_x,
                      double *gvp, double *gvs, double *ill, double *misfit, double *res_z, double *res_x) {
     gradient_impl<double, double>(c, vp, vs, rho, wav, gc, obs_z, obs_x, gvp, gvs, ill, misfit, res_z, res_x);
+}
+void or_gradient_rho_f32(const or_config *c, const float *vp, const float *vs, const float *rho,
+                         const float *wav, const int *gc, const float *obs_z, const float *obs_x,
+                         float *gvp, float *gvs, float *ill, float *grho, double *misfit) {
+    gradient_impl<float, float>(c, vp, vs, rho, wav, gc, obs_z, obs_x, gvp, gvs, ill, misfit, nullptr, nullptr,
+                                grho);
+}
+void or_gradient_rho_f64(const or_config *c, const float *vp, const float *vs, const float *rho,
+                         const float *wav, const int *gc, const float *obs_z, const float *obs_x,
+                         double *gvp, double *gvs, double *ill, double *grho, double *misfit) {
+    gradient_impl<double, double>(c, vp, vs, rho, wav, gc, obs_z, obs_x, gvp, gvs, ill, misfit, nullptr,
+                                  nullptr, grho);
 }
 void or_gradient_res_f32(const or_config *c, const float *vp, const float *vs, const float *rho,
                          const float *wav, const int *gc, const float *res_z, const float *res_x,

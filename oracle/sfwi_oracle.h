@@ -64,6 +64,17 @@ void or_gradient_f64(const or_config *c, const float *vp, const float *vs, const
                      double *grad_vp, double *grad_vs, double *illum, double *misfit,
                      double *res_z, double *res_x);
 
+/* Stage-1 gradient that also images the density (no reference counterpart):
+ * grad_rho += -(vx_adj dvx + vz_adj dvz)/dt + (vp^2 - 2 vs^2) lg + vs^2 mg per
+ * reverse step, dv the reconstructed velocity step, lg / mg the lambda / mu
+ * correlations of the Vp / Vs imaging. */
+void or_gradient_rho_f32(const or_config *c, const float *vp, const float *vs, const float *rho,
+                         const float *wav, const int *gc, const float *obs_z, const float *obs_x,
+                         float *grad_vp, float *grad_vs, float *illum, float *grad_rho, double *misfit);
+void or_gradient_rho_f64(const or_config *c, const float *vp, const float *vs, const float *rho,
+                         const float *wav, const int *gc, const float *obs_z, const float *obs_x,
+                         double *grad_vp, double *grad_vs, double *illum, double *grad_rho, double *misfit);
+
 /* Reverse loop (reconstruction, adjoint, imaging) driven by a caller-supplied
  * adjoint source res_* [ntr][nt] instead of the L2 residual: the gradient of
  * a misfit whose adjoint source is res_* (OT misfit).  Stage set by c. */

@@ -31,7 +31,7 @@ class SfwiParams(ctypes.Structure):
         ("adjoint_dir", ctypes.c_int), ("max_batch", ctypes.c_int),
         ("mode", ctypes.c_int), ("storage", ctypes.c_int), ("use_graph", ctypes.c_int),
         ("march_cols", ctypes.c_int), ("march_depth", ctypes.c_int), ("misfit", ctypes.c_int),
-        ("march_order", ctypes.c_int),
+        ("march_order", ctypes.c_int), ("grad_rho", ctypes.c_int),
     ]
 
 
@@ -104,6 +104,7 @@ def load():
                                            ctypes.c_float, dp]
     L.sfwi_linesearch_linear.argtypes = [vp, P(SfwiShot), _f32p, _f32p, dp, dp]
     L.sfwi_flush.argtypes = [vp, vp, vp, vp]
+    L.sfwi_flush_rho.argtypes = [vp, vp]
     L.sfwi_ot_misfit.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                                  _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, dp]
     L.sfwi_ricker.argtypes = [_f32p, ctypes.c_int, ctypes.c_float, ctypes.c_float]

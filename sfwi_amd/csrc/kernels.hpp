@@ -66,6 +66,8 @@ struct DevShot {
     unsigned short *strip_h;               // packed strips [nt][2][SF] (FP16 / BF16 storage)
     float *strip_scale;                    // max-abs scale per step and field [nt][2]
     float *gvp, *gvs, *gil;                // per-shot gradient accumulators [nx][nz]
+    float *grho;                           // per-shot density gradient [nx][nz] (grad_rho)
+    float *vold_x, *vold_z;                // v(it+1) before the reconstruction velocity step (grad_rho)
     double *trace_sum;                     // per-trace partial misfit [ntr]
     int isx, isz, igz, ntr, ninj, left;
 };
@@ -82,6 +84,7 @@ struct Geo {
     int cm_nshots;          // shots of the launch (cm_order 1)
     int sf;                 // strip floats per field and step
     int store;              // strip storage: 0 FP32, 1 FP16, 2 BF16 (per-step max-abs scale)
+    int grad_rho;           // 1: also image the density
     float inv_nzd;
     float dt, dx, dz, dtdx, dtdz, idx, idz;
 };
@@ -445,6 +448,16 @@ __device__ inline void image_fast(const DevShot &d, int i, int q) {
     d.gil[q] = __builtin_fmaf(vx, vx, __builtin_fmaf(vz, vz, d.gil[q]));
 }
 
+// Density term of one interior cell (FAST; both FAST kernel families call
+// it with the same operands):  -(vx_adj dvx + vz_adj dvz) / dt
+// + (vp^2 - 2 vs^2) lg + vs^2 mg, dv = v(it+1) - v(it) of the reconstruction.
+__device__ inline float rho_term_fast(const Geo &g, float lg, float mg, float bvx, float bvz, float dvx,
+                                      float dvz, float vp, float vs) {
+    const float rd = -__builtin_fmaf(bvx, dvx, bvz * dvz) / g.dt;
+    const float vs2 = vs * vs;
+    return __builtin_fmaf(__builtin_fmaf(vp, vp, -2.0f * vs2), lg, __builtin_fmaf(vs2, mg, rd));
+}
+
 // Stress update (sfwi-stage1/born.cu:399-440, update_txx_tzz_txz); with
 // a Born source the terms are added to the split parts first
 // (sfwi-stage2/FD2DGPU.cu:1839-1840 order: source, then update).
@@ -617,6 +630,10 @@ __global__ void __launch_bounds__(BLK) k_vel(Geo g, const DevShot *__restrict__ 
     const DevShot &d = shots[blockIdx.y];
     const int i = ix * g.nzp + iz;
     const bool split = in_band(g, ix, iz);
+    if (g.grad_rho && dir < 0) {   // reconstruction: keep v(it+1) for the density image
+        d.vold_x[i] = d.fw.vx[i];
+        d.vold_z[i] = d.fw.vz[i];
+    }
     vel_cell<EXACT>(g, d.m, d.fw, i, split, dir);
     if (BORN) vel_cell<EXACT>(g, d.m, d.bn, i, split, dir);
     if (SAVE && split) {
@@ -837,15 +854,27 @@ __global__ void __launch_bounds__(BLK) k_adj_vel_image(Geo g, const DevShot *__r
         d.gvp[q] += 1.0f * pp;
         d.gvs[q] += 1.0f * ps;
         d.gil[q] += 1.0f * (vx * vx + vz * vz);
+        if (g.grad_rho) {
+            const float dvx = d.vold_x[i] - vx, dvz = d.vold_z[i] - vz;
+            const float rd = -(d.bk.vx[i] * dvx + d.bk.vz[i] * dvz) / g.dt;
+            const float rg = rd + (vp * vp - 2.0f * vs * vs) * lg + vs * vs * mg;
+            d.grho[q] += 1.0f * rg;
+        }
     } else {
         image_fast(d, i, q);
+        if (g.grad_rho) {
+            const float lg = -((gx + gz) * (sxx + szz));
+            const float mg = __builtin_fmaf(-(gxz + gzx), sxz, -2.0f * __builtin_fmaf(gx, sxx, gz * szz));
+            d.grho[q] += rho_term_fast(g, lg, mg, d.bk.vx[i], d.bk.vz[i], d.vold_x[i] - vx, d.vold_z[i] - vz,
+                                       d.m.vp[i], d.m.vs[i]);
+        }
     }
 }
 
 // Per-shot accumulators added into the full-model images, shots in batch
 // order so the result does not depend on scheduling.
 __global__ void k_merge(Geo g, const DevShot *__restrict__ shots, int nshots, float *gvp, float *gvs,
-                        float *gil) {
+                        float *gil, float *grho) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= g.nx * g.nz) return;
     const int jx = q / g.nz, jz = q - jx * g.nz;
@@ -855,6 +884,7 @@ __global__ void k_merge(Geo g, const DevShot *__restrict__ shots, int nshots, fl
         gvp[o] += d.gvp[q];
         gvs[o] += d.gvs[q];
         gil[o] += d.gil[q];
+        if (g.grad_rho) grho[o] += d.grho[q];
     }
 }
 

@@ -140,6 +140,7 @@ __device__ inline void vel_load_m(const Medium &m, const Fields &F, int i, bool 
 // the velocity update loads anyway; the caller fills them from its ring.
 struct ImgCell {
     float gx, gz, gxz, gzx, sxx, szz, sxz, vx, vz, cp, cs, ip, is, il;
+    float ox, oz, vp, vs, ir;   // density image (grad_rho): v(it+1), vp, vs, accumulator
 };
 __device__ inline void img_load(const DevShot &d, int i, int q, bool any, ImgCell &c) {
     if (any) {
@@ -148,6 +149,13 @@ __device__ inline void img_load(const DevShot &d, int i, int q, bool any, ImgCel
         c.vx = d.fw.vx[i]; c.vz = d.fw.vz[i];
         c.cp = d.m.icp[i]; c.cs = d.m.ics[i];
         c.ip = d.gvp[q]; c.is = d.gvs[q]; c.il = d.gil[q];
+    }
+}
+__device__ inline void img_load_rho(const DevShot &d, int i, int q, bool any, ImgCell &c) {
+    if (any) {
+        c.ox = d.vold_x[i]; c.oz = d.vold_z[i];
+        c.vp = d.m.vp[i]; c.vs = d.m.vs[i];
+        c.ir = d.grho[q];
     }
 }
 // The arithmetic of image_fast on prefetched operands.
@@ -213,6 +221,10 @@ __global__ void __launch_bounds__(BLK) km_vel(Geo g, const DevShot *__restrict__
                 shift(wxx[f]);
                 shift(wxz[f]);
                 if (l.act) {
+                    if (g.grad_rho && dir < 0 && f == 0) {   // reconstruction: keep v(it+1)
+                        d.vold_x[i] = split ? F.vx[i] : st.vx;
+                        d.vold_z[i] = split ? F.vz[i] : st.vz;
+                    }
                     vel_math(g, st, split, dir, a, bz, cz, e);
                     vel_store(F, i, st, split);
                     if (SAVE && f == 0 && split) {
@@ -420,6 +432,7 @@ __global__ void __launch_bounds__(BLK) km_adj_vel_image(Geo g, const DevShot *__
         vel_load_m(m, F, c * g.nzp + l.zc, any_lane(l.act && split_of(c)), qst[j]);
         const int q = img_q(c);
         img_load(d, c * g.nzp + l.zc, q < 0 ? 0 : q, any_lane(q >= 0), qim[j]);
+        if (g.grad_rho) img_load_rho(d, c * g.nzp + l.zc, q < 0 ? 0 : q, any_lane(q >= 0), qim[j]);
     };
 #pragma unroll
     for (int j = 0; j < 7; ++j) {
@@ -454,7 +467,16 @@ __global__ void __launch_bounds__(BLK) km_adj_vel_image(Geo g, const DevShot *__
                 vel_math(g, st, split, dir, sx, sxz_z, sz, sxz_x);
                 vel_store(F, i, st, split);
                 const int q = img_q(c);
-                if (q >= 0) img_store(d, q, im);
+                if (q >= 0) {
+                    img_store(d, q, im);
+                    if (g.grad_rho) {
+                        const float lg = -((im.gx + im.gz) * (im.sxx + im.szz));
+                        const float mg = __builtin_fmaf(-(im.gxz + im.gzx), im.sxz,
+                                                        -2.0f * __builtin_fmaf(im.gx, im.sxx, im.gz * im.szz));
+                        d.grho[q] = im.ir + rho_term_fast(g, lg, mg, st.vx, st.vz, im.ox - im.vx, im.oz - im.vz,
+                                                          im.vp, im.vs);
+                    }
+                }
             }
         }
     }

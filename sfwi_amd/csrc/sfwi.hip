@@ -35,7 +35,8 @@ struct Slot {
     float *dg[4]{};
     float *rz, *rx, *r2z, *r2x, *oz, *ox;
     int *igx, *inj;
-    float *gvp, *gvs, *gil;
+    float *gvp, *gvs, *gil, *grho;
+    float *vold[2]{};
     double *trace_sum;
     int obs_ntr = 0;                 // traces of the observed records resident in oz/ox (0: none)
     std::vector<int> h_igx, h_inj;   // host copies kept alive for async uploads
@@ -61,7 +62,7 @@ struct sfwi_ctx {
     float *wav = nullptr;
     float *mvp = nullptr, *mvs = nullptr, *mrho = nullptr, *mdvp = nullptr, *mdvs = nullptr;
     float *tmp_model = nullptr;   // 4 full-model scratch arrays (line-search models)
-    float *Gvp = nullptr, *Gvs = nullptr, *Gil = nullptr;
+    float *Gvp = nullptr, *Gvs = nullptr, *Gil = nullptr, *Grho = nullptr;
     float *res_keep = nullptr;    // 2 record arrays: residual kept across two Born solves
     std::vector<Slot> slots;
     DevShot *d_shots = nullptr;
@@ -206,7 +207,7 @@ int ensure_slots(sfwi_ctx *c, int n, bool born, bool strips) {
     for (int s = 0; s < n; ++s) {
         Slot &S = c->slots[s];
         if (!S.block) {
-            int r = dalloc(c, (void **)&S.block, 34 * cell);
+            int r = dalloc(c, (void **)&S.block, (34 + 2 * c->g.grad_rho) * cell);
             if (r) return r;
             float *f = (float *)S.block;
             float **fw[15] = {&S.fw.vx, &S.fw.vz, &S.fw.txx, &S.fw.tzz, &S.fw.txz, &S.fw.vx_x, &S.fw.vx_z,
@@ -218,9 +219,11 @@ int ensure_slots(sfwi_ctx *c, int n, bool born, bool strips) {
             for (int k = 0; k < 15; ++k) *fw[k] = f + k * c->ncell;
             for (int k = 0; k < 15; ++k) *bk[k] = f + (15 + k) * c->ncell;
             for (int k = 0; k < 4; ++k) S.dg[k] = f + (30 + k) * c->ncell;
+            if (c->g.grad_rho)
+                for (int k = 0; k < 2; ++k) S.vold[k] = f + (34 + k) * c->ncell;
             // records, receiver tables, accumulators, per-trace sums
             const size_t rbytes = 6 * c->nrec * sizeof(float) + 2 * (size_t)c->p.ntr * sizeof(int) +
-                                  3 * c->nint * sizeof(float) + (size_t)c->p.ntr * sizeof(double) + 64;
+                                  4 * c->nint * sizeof(float) + (size_t)c->p.ntr * sizeof(double) + 64;
             r = dalloc(c, (void **)&S.rec, rbytes);
             if (r) return r;
             char *q = S.rec;
@@ -229,8 +232,8 @@ int ensure_slots(sfwi_ctx *c, int n, bool born, bool strips) {
             S.rz = rr; S.rx = rr + c->nrec; S.r2z = rr + 2 * c->nrec; S.r2x = rr + 3 * c->nrec;
             S.oz = rr + 4 * c->nrec; S.ox = rr + 5 * c->nrec;
             float *acc = rr + 6 * c->nrec;
-            S.gvp = acc; S.gvs = acc + c->nint; S.gil = acc + 2 * c->nint;
-            S.igx = (int *)(acc + 3 * c->nint);
+            S.gvp = acc; S.gvs = acc + c->nint; S.gil = acc + 2 * c->nint; S.grho = acc + 3 * c->nint;
+            S.igx = (int *)(acc + 4 * c->nint);
             S.inj = S.igx + c->p.ntr;
         }
         if (born && !S.born) {
@@ -320,7 +323,8 @@ int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium 
             d.strip_h = (unsigned short *)(d.stage + 2 * (size_t)c->g.sf);
             (void)nv;
         }
-        d.gvp = S.gvp; d.gvs = S.gvs; d.gil = S.gil; d.trace_sum = S.trace_sum;
+        d.gvp = S.gvp; d.gvs = S.gvs; d.gil = S.gil; d.grho = S.grho; d.trace_sum = S.trace_sum;
+        d.vold_x = S.vold[0]; d.vold_z = S.vold[1];
         d.isx = s.isx; d.isz = s.isz; d.igz = s.igz; d.ntr = s.ntr; d.ninj = (int)S.h_inj.size();
         d.left = s.left;
         c->h_shots[k] = d;
@@ -335,7 +339,7 @@ int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium 
             HIPCK(c, hipMemcpyAsync(S.ox, s.obs_x, rb, hipMemcpyHostToDevice, c->stream));
             S.obs_ntr = s.ntr;
         }
-        if (grad) HIPCK(c, hipMemsetAsync(S.gvp, 0, 3 * c->nint * sizeof(float), c->stream));
+        if (grad) HIPCK(c, hipMemsetAsync(S.gvp, 0, 4 * c->nint * sizeof(float), c->stream));
     }
     HIPCK(c, hipMemcpyAsync(c->d_shots, c->h_shots.data(), n * sizeof(DevShot), hipMemcpyHostToDevice, c->stream));
     return 0;
@@ -535,7 +539,7 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
         tend(c, t);
     }
     const dim3 mg((unsigned)((c->nint + 255) / 256));
-    hipLaunchKernelGGL(k_merge, mg, blk, 0, st, g, D, n, c->Gvp, c->Gvs, c->Gil);
+    hipLaunchKernelGGL(k_merge, mg, blk, 0, st, g, D, n, c->Gvp, c->Gvs, c->Gil, c->Grho);
     HIPCK(c, hipGetLastError());
     return 0;
 }
@@ -663,6 +667,7 @@ int sfwi_ctx_create(const sfwi_params *p, sfwi_ctx **out) {
     set_march_geo(g, p->march_cols);
     g.sf = 4 * NSTRIP * (p->nx + p->nz);
     g.store = p->storage;
+    g.grad_rho = p->grad_rho ? 1 : 0;
     g.inv_nzd = 1.0f / (float)g.nzd;
     g.dt = p->dt; g.dx = p->dx; g.dz = p->dz;
     g.dtdx = p->dt / p->dx; g.dtdz = p->dt / p->dz;
@@ -681,14 +686,14 @@ int sfwi_ctx_create(const sfwi_params *p, sfwi_ctx **out) {
     c->mvs = c->mvp + c->nmodel; c->mrho = c->mvp + 2 * c->nmodel;
     c->mdvp = c->mvp + 3 * c->nmodel; c->mdvs = c->mvp + 4 * c->nmodel;
     if ((r = dalloc(c, (void **)&c->tmp_model, 4 * c->nmodel * sizeof(float)))) return bail(r);
-    if ((r = dalloc(c, (void **)&c->Gvp, 3 * c->nmodel * sizeof(float)))) return bail(r);
-    c->Gvs = c->Gvp + c->nmodel; c->Gil = c->Gvp + 2 * c->nmodel;
+    if ((r = dalloc(c, (void **)&c->Gvp, 4 * c->nmodel * sizeof(float)))) return bail(r);
+    c->Gvs = c->Gvp + c->nmodel; c->Gil = c->Gvp + 2 * c->nmodel; c->Grho = c->Gvp + 3 * c->nmodel;
     if ((r = dalloc(c, (void **)&c->res_keep, 2 * c->nrec * sizeof(float)))) return bail(r);
     if ((r = dalloc(c, (void **)&c->d_shots, p->max_batch * sizeof(DevShot)))) return bail(r);
     if ((r = dalloc(c, (void **)&c->d_out, (p->max_batch + 4) * sizeof(double)))) return bail(r);
     if ((r = dalloc(c, (void **)&c->d_num, 2 * (size_t)p->ntr * sizeof(double)))) return bail(r);
     c->d_den = c->d_num + p->ntr;
-    if (hipMemsetAsync(c->Gvp, 0, 3 * c->nmodel * sizeof(float), c->stream) != hipSuccess ||
+    if (hipMemsetAsync(c->Gvp, 0, 4 * c->nmodel * sizeof(float), c->stream) != hipSuccess ||
         hipMemsetAsync(c->mvp, 0, 5 * c->nmodel * sizeof(float), c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess)
         return bail(fail(c, SFWI_EHIP, "initial clears failed"));
@@ -873,6 +878,16 @@ int sfwi_linesearch_linear(sfwi_ctx *c, const sfwi_shot *shot, const float *cg_v
     return 0;
 }
 
+int sfwi_flush_rho(sfwi_ctx *c, float *grho) {
+    if (!c) return SFWI_EINVAL;
+    if (!c->p.grad_rho) return fail(c, SFWI_ESTATE, "the context was created without grad_rho");
+    const size_t b = c->nmodel * sizeof(float);
+    if (grho) HIPCK(c, hipMemcpyAsync(grho, c->Grho, b, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipMemsetAsync(c->Grho, 0, b, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 int sfwi_flush(sfwi_ctx *c, float *gvp, float *gvs, float *gil) {
     if (!c) return SFWI_EINVAL;
     const size_t b = c->nmodel * sizeof(float);
@@ -921,7 +936,7 @@ int sfwi_autotune(sfwi_ctx *c, int nshots, int steps, int *march_cols, int *marc
         s.gc = gc.data();
         s.obs_z = s.obs_x = zero.data();
     }
-    const size_t acc = 3 * c->nmodel * sizeof(float);
+    const size_t acc = 4 * c->nmodel * sizeof(float);
     HIPCK(c, hipMemcpyAsync(c->tmp_model, c->Gvp, acc, hipMemcpyDeviceToDevice, c->stream));
     struct Cand { int cols, depth, order; };
     const Cand cands[] = {{-1, 1, 1}, {8, 1, 1}, {8, 2, 1}, {16, 1, 1}, {8, 1, 2}};

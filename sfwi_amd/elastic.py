@@ -41,7 +41,7 @@ class Shot:
 
 def make_params(*, dx, dz, dt, nx, nz, pml, nt, ntr, allnx, allnz, rr=0.001, rho_const=0.0, adjoint_dir=1,
                 max_batch=1, mode="fast", storage="f32", use_graph=False, march_cols=0, march_depth=0,
-                misfit="l2", march_order=0):
+                misfit="l2", march_order=0, grad_rho=False):
     """``sfwi_params`` of a context or request (keyword names as in the C struct)."""
     p = SfwiParams()
     p.dx, p.dz, p.dt = dx, dz, dt
@@ -56,6 +56,7 @@ def make_params(*, dx, dz, dt, nx, nz, pml, nt, ntr, allnx, allnz, rr=0.001, rho
     p.misfit = {"l2": _lib.MISFIT_L2, "ot_trace": _lib.MISFIT_OT_TRACE, "ot_power": _lib.MISFIT_OT_POWER,
                 "ot_gather": _lib.MISFIT_OT_GATHER}[misfit]
     p.march_order = int(march_order)
+    p.grad_rho = int(bool(grad_rho))
     return p
 
 
@@ -165,6 +166,12 @@ class Context:
         return tuple(g)
 
     # ---------------------------------------------------------- timing ---
+    def flush_rho(self):
+        """Accumulated density gradient (allnx x allnz); zeroes it on the device."""
+        g = np.zeros((self.allnx, self.allnz), np.float32)
+        self._ck(self.L.sfwi_flush_rho(self.h, _lib.ptr(g)))
+        return g
+
     def set_timing(self, every):
         self._ck(self.L.sfwi_set_timing(self.h, int(every)))
 
