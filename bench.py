@@ -415,25 +415,40 @@ def serving(args):
         for th in threads:
             th.join()
         makespan = max(done_at) - t0
-    print("serving: makespan %.2f s" % makespan, file=sys.stderr, flush=True)
-    t1 = time.perf_counter()
+        print("serving: paced makespan %.2f s" % makespan, file=sys.stderr, flush=True)
+        # saturated: the same requests all queued at once (throughput)
+        t0 = time.perf_counter()
+        tks = [_submit(srv, r) for _, r, _ in reqs]
+        for tk in tks:
+            srv.wait(tk)
+        t_sat = time.perf_counter() - t0
+    print("serving: saturated makespan %.2f s" % t_sat, file=sys.stderr, flush=True)
+    alone = []
     for _, r, _ in reqs:
+        t1 = time.perf_counter()
         _run_alone(r)
-    t_alone = time.perf_counter() - t1
+        alone.append(time.perf_counter() - t1)
+    t_alone = sum(alone)
     print("serving: one at a time %.2f s" % t_alone, file=sys.stderr, flush=True)
     lms = sorted(1e3 * x for x in lat)
+    ams = sorted(1e3 * x for x in alone)
     line = {
-        "metric": "serving p50/p99 request latency and throughput", "value": round(shots / makespan, 4),
+        "metric": "serving p50/p99 request latency and throughput", "value": round(shots / t_sat, 4),
         "unit": "shot-solves/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32",
         "data": "synthetic (seeded layered models, zero observed traces)",
         "config": {"workload": "serving: %d Poisson requests at %.2f/s, nx 200-1500, nz 150-800, nt 1500-6000 "
                                "(x%.2f), 1-8 shots, forward/Born/gradient mix" % (len(reqs), args.rate, args.scale),
                    "workers": args.workers},
+        "value_note": "saturated: all requests queued at once",
         "latency_ms": {"p50": round(float(np.percentile(lms, 50)), 1),
-                       "p99": round(float(np.percentile(lms, 99)), 1), "unit": "submit to done"},
-        "gcell_updates_per_s": round(cells / makespan / 1e9, 2),
-        "makespan_s": round(makespan, 3),
+                       "p99": round(float(np.percentile(lms, 99)), 1),
+                       "unit": "submit to done, paced Poisson arrivals"},
+        "gcell_updates_per_s": round(cells / t_sat / 1e9, 2),
+        "paced_makespan_s": round(makespan, 3),
+        "saturated_makespan_s": round(t_sat, 3),
         "one_at_a_time": {"value": round(shots / t_alone, 4), "unit": "shot-solves/s", "seconds": round(t_alone, 3),
+                          "request_ms": {"p50": round(float(np.percentile(ams, 50)), 1),
+                                         "p99": round(float(np.percentile(ams, 99)), 1)},
                           "path": "plain one-shot path (per-cell kernels, one shot per call), requests back to back"},
     }
     print(json.dumps(line), flush=True)

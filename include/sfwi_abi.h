@@ -77,7 +77,9 @@ typedef struct sfwi_params {
                            /* once into a hipGraph and replay it (L2 misfit; solves */
                            /* with sfwi_set_timing on run directly)                 */
     int march_cols;        /* FAST kernels: columns per wave segment of the column-  */
-                           /* march kernels (0 = 8); < 0 selects per-cell kernels */
+                           /* march kernels; < 0 selects per-cell kernels; 0 picks */
+                           /* per-cell kernels for batches of fewer than 7168 wave */
+                           /* tiles (too few to fill the GPU) and 8 columns above  */
     int march_depth;       /* columns of operands prefetched ahead (1, 2, 4; 0 = 1) */
     int misfit;            /* SFWI_MISFIT_*                                          */
     int march_order;       /* column-march block order: 0 default (= 1); 1 one shot  */

@@ -465,7 +465,10 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
         gm.cm_per_xcd = (int)((ntile * n + 7) / 8);
         tiles = dim3((unsigned)(8 * gm.cm_per_xcd), 1);
     }
-    const bool percell = E || c->p.march_cols < 0;   // per-cell kernels on request (march_cols < 0)
+    // per-cell kernels on request (march_cols < 0), or automatically (march_cols 0)
+    // when the batch has too few wave tiles to fill the GPU: measured crossover
+    // near 7 march waves per SIMD (1024 SIMDs)
+    const bool percell = E || c->p.march_cols < 0 || (c->p.march_cols == 0 && (long)n * g.cm_waves < 7168);
     for (int it = 0; it < nt; ++it) {
         hipLaunchKernelGGL(k_source, src1, dim3(64), 0, st, g, D, c->wav, it, n);
         int t = tbeg(c, SFWI_K_FWD_VEL, it);

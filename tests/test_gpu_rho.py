@@ -50,7 +50,7 @@ def test_rho_gradient_exact_bitwise():
 def test_rho_gradient_fast_families_and_fp64():
     sv, sh, w, c, oz, ox = _case()
     ref64 = O.gradient_rho(c, sv.vp0, sv.vs0, sv.rho0, w, sh.gc, oz, ox, "f64")
-    _, _, r_march = _run(sv, sh, w, oz, ox, grad_rho=True)
+    _, _, r_march = _run(sv, sh, w, oz, ox, grad_rho=True, march_cols=8)
     _, _, r_cell = _run(sv, sh, w, oz, ox, grad_rho=True, march_cols=-1)
     assert np.array_equal(r_march, r_cell)
     assert rel_l2(r_march, ref64[3]) <= 1e-3

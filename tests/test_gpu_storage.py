@@ -31,7 +31,7 @@ def test_strip_storage_gradient_error(storage, percell):
     r64 = [sum(r[k] for r in ref) for k in range(3)]
     out = {}
     for st in ("f32", storage):
-        with context(sv, max_batch=2, storage=st, march_cols=-1 if percell else 0) as ctx:
+        with context(sv, max_batch=2, storage=st, march_cols=-1 if percell else 8) as ctx:
             ctx.set_wavelet(w)
             ctx.set_model(sv.vp0, sv.vs0, sv.rho0)
             m = ctx.batch_gradient(shots)
