@@ -13,6 +13,14 @@
  *
  * All model/gradient arrays are the reference's .bin layout: x-major,
  * z fastest, index ix*allnz + iz.  Records are trace-major [ntr][nt].
+ *
+ * Pinned independently (tests/test_oracle_pins.py): coefficients, stencils,
+ * window / medium / damping, Ricker, the propagators (energy, reversibility),
+ * the adjoint half steps (dot-product), Born modeling (linearization), the
+ * misfit and line-search sums.  Parity unpinned (restatement of the cited
+ * lines only): the orchestration of the faithful gradient (strip restore,
+ * injection, imaging formula) and the density gradient, which has no
+ * reference counterpart.
  */
 #ifndef SFWI_ORACLE_H
 #define SFWI_ORACLE_H
