@@ -42,12 +42,12 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE x 2 + WRITE_SIZE x 1 from rocprofv3 --pmc passes over a 64-shot
 # batch, the factors from a plain 1 GiB copy (profiles/shot_batch_march_hbm_traffic.md).
 MEASURED_HBM_BYTES_PER_SHOT = {
-    "k_vel(forward)": 481.7e6 / 64,
-    "k_str(forward)": 582.2e6 / 64,
-    "k_str(reconstruct)": 795.5e6 / 64,
-    "k_vel(reconstruct)": 480.1e6 / 64,
-    "k_adj_str": 516.8e6 / 64,
-    "k_adj_vel_image": 1146.7e6 / 64,
+    "k_vel(forward)": 481.5e6 / 64,
+    "k_str(forward)": 584.7e6 / 64,
+    "k_str(reconstruct)": 799.5e6 / 64,
+    "k_vel(reconstruct)": 469.5e6 / 64,
+    "k_adj_str": 513.4e6 / 64,
+    "k_adj_vel_image": 1159.9e6 / 64,
 }
 
 
