@@ -37,17 +37,17 @@ if ROOT not in sys.path:
 METRIC = "shot-gradients/s & Gcell-updates/s at ≤1e-3 grad err vs FP64; p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes per launch of the default (column-march, 8 columns, depth 1) FAST
+# HBM bytes per launch of the default (column-march, 8 columns, depth 4, whole shots per XCD) FAST
 # kernels on this workload's 461 x 361 padded grid, per shot in the launch:
 # FETCH_SIZE x 2 + WRITE_SIZE x 1 from rocprofv3 --pmc passes over a 64-shot
 # batch, the factors from a plain 1 GiB copy (profiles/shot_batch_march_hbm_traffic.md).
 MEASURED_HBM_BYTES_PER_SHOT = {
-    "k_vel(forward)": 481.5e6 / 64,
-    "k_str(forward)": 584.7e6 / 64,
-    "k_str(reconstruct)": 799.5e6 / 64,
-    "k_vel(reconstruct)": 469.5e6 / 64,
-    "k_adj_str": 513.4e6 / 64,
-    "k_adj_vel_image": 1159.9e6 / 64,
+    "k_vel(forward)": 486.7e6 / 64,
+    "k_str(forward)": 636.6e6 / 64,
+    "k_str(reconstruct)": 845.4e6 / 64,
+    "k_vel(reconstruct)": 480.2e6 / 64,
+    "k_adj_str": 468.9e6 / 64,
+    "k_adj_vel_image": 1148.4e6 / 64,
 }
 
 
@@ -85,7 +85,8 @@ def main():
                     help="FAST kernels: columns per wave segment (0: library default, <0: per-cell kernels)")
     ap.add_argument("--march-depth", type=int, default=0, help="FAST march kernels: prefetch depth")
     ap.add_argument("--march-order", type=int, default=0,
-                    help="FAST march kernels: block order (0 default, 1 shot after shot, 2 shots interleaved)")
+                    help="FAST march kernels: block order (0 default, 1 shot after shot, 2 shots interleaved, "
+                         "3 whole shots per XCD)")
     ap.add_argument("--serving", action="store_true",
                     help="run the serving-loop benchmark (Poisson request stream) instead")
     ap.add_argument("--uneven", action="store_true",
@@ -196,7 +197,7 @@ def main():
     if dom:
         d = kernels[dom]
         measured = (args.workload == "shot_batch" and args.mode == "fast" and args.march_cols in (0, 8)
-                    and args.march_depth in (0, 1) and args.march_order in (0, 1))
+                    and args.march_depth in (0, 4) and args.march_order in (0, 3))
         traffic = round(MEASURED_HBM_BYTES_PER_SHOT[dom] * nsh) if measured else None
         roofline = {"kernel": dom, "bound": "hbm", "achieved": d["GBps"], "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(d["GBps"] / HBM_PEAK_GBS, 4), "traffic": traffic,

@@ -80,10 +80,11 @@ typedef struct sfwi_params {
                            /* march kernels; < 0 selects per-cell kernels; 0 picks */
                            /* per-cell kernels for batches of fewer than 7168 wave */
                            /* tiles (too few to fill the GPU) and 8 columns above  */
-    int march_depth;       /* columns of operands prefetched ahead (1, 2, 4; 0 = 1) */
+    int march_depth;       /* columns of operands prefetched ahead (1, 2, 4; 0 = 4) */
     int misfit;            /* SFWI_MISFIT_*                                          */
-    int march_order;       /* column-march block order: 0 default (= 1); 1 one shot  */
-                           /* after another; 2 the shots of a tile back to back     */
+    int march_order;       /* column-march block order: 0 default (= 3); 1 one shot  */
+                           /* after another, split over the XCDs; 2 the shots of a  */
+                           /* tile back to back; 3 whole shots per XCD              */
     int grad_rho;          /* 1: gradient solves also image the density (sfwi_flush_rho) */
 } sfwi_params;
 

@@ -25,6 +25,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 namespace sfwi {
 
@@ -36,39 +37,62 @@ constexpr int BLK = 256;      // threads per workgroup
 __constant__ float kC[5] = {0.0f, (float)(1225.0 / 1024.0), (float)(-245.0 / 3072.0),
                             (float)(49.0 / 5120.0), (float)(-5.0 / 7168.0)};
 
+// Device pointer into global memory (HBM).  The field pointers reach the
+// kernels inside descriptors read from memory, where the compiler cannot tell
+// their address space and would emit flat instructions (which also count
+// against the scalar-memory wait counter, so every scalar-load wait would
+// drain the vector loads in flight).  Indexing through this wrapper emits
+// global loads and stores with the uniform base in SGPRs and a 32-bit
+// unsigned byte offset: indices are non-negative and below 2^30.
+template <class T> struct gptr {
+    T *p;
+    gptr() = default;
+    __host__ __device__ gptr(T *q) : p(q) {}
+    template <class U, class = typename std::enable_if<std::is_convertible<U *, T *>::value>::type>
+    __host__ __device__ gptr(const gptr<U> &o) : p(o.p) {}
+    __host__ __device__ operator T *() const { return p; }
+    __host__ __device__ gptr operator+(size_t n) const { return gptr(p + n); }
+    using G = T __attribute__((address_space(1)));
+    using GB = typename std::conditional<std::is_const<T>::value, const char, char>::type
+        __attribute__((address_space(1)));
+    __device__ G &operator[](unsigned i) const { return *(G *)((GB *)p + i * (unsigned)sizeof(T)); }
+};
+using fptr = gptr<float>;
+using cfptr = gptr<const float>;
+
 // Pointers of one propagator's fields.
 struct Fields {
-    float *vx, *vz, *txx, *tzz, *txz;
-    float *vx_x, *vx_z, *vz_x, *vz_z;
-    float *txx_x, *txx_z, *tzz_x, *tzz_z, *txz_x, *txz_z;
+    fptr vx, vz, txx, tzz, txz;
+    fptr vx_x, vx_z, vz_x, vz_z;
+    fptr txx_x, txx_z, tzz_x, tzz_z, txz_x, txz_z;
 };
 
 // Padded medium of one shot window.
 struct Medium {
-    const float *vp, *vs, *rho, *lam, *mu, *dampx, *dampz;
-    const float *bu;          // 1/rho (FP32 fast mode)
-    const float *icp, *ics;   // imaging coefficients 2 vp rho, 2 vs rho (fast mode)
-    const float *dvp, *dvs;   // padded perturbation (Born)
+    cfptr vp, vs, rho, lam, mu, dampx, dampz;
+    cfptr bu;          // 1/rho (FP32 fast mode)
+    cfptr icp, ics;    // imaging coefficients 2 vp rho, 2 vs rho (fast mode)
+    cfptr dvp, dvs;    // padded perturbation (Born)
 };
 
 // Per-shot device descriptor, read with scalar loads (uniform per block).
 struct DevShot {
     Medium m;
     Fields fw, bk, bn;                     // forward/reconstructed, adjoint, Born scattered
-    float *dgx, *dgz, *dgxz, *dgzx;        // velocity derivatives for imaging
-    float *rz, *rx;                        // synthetic (then residual) traces [ntr][nt]
-    float *r2z, *r2x;                      // background traces of a Born solve
-    const float *oz, *ox;                  // observed traces
-    const int *igx;                        // receiver columns [ntr]
-    const int *inj;                        // traces injected by the adjoint [ninj]
-    float *strip;                          // boundary strips, [nt][2 fields][SF] (FP32 storage)
-    float *stage;                          // one step's strips [2][SF] before packing (16-bit storage)
-    unsigned short *strip_h;               // packed strips [nt][2][SF] (FP16 / BF16 storage)
-    float *strip_scale;                    // max-abs scale per step and field [nt][2]
-    float *gvp, *gvs, *gil;                // per-shot gradient accumulators [nx][nz]
-    float *grho;                           // per-shot density gradient [nx][nz] (grad_rho)
-    float *vold_x, *vold_z;                // v(it+1) before the reconstruction velocity step (grad_rho)
-    double *trace_sum;                     // per-trace partial misfit [ntr]
+    fptr dgx, dgz, dgxz, dgzx;             // velocity derivatives for imaging
+    fptr rz, rx;                           // synthetic (then residual) traces [ntr][nt]
+    fptr r2z, r2x;                         // background traces of a Born solve
+    cfptr oz, ox;                          // observed traces
+    gptr<const int> igx;                   // receiver columns [ntr]
+    gptr<const int> inj;                   // traces injected by the adjoint [ninj]
+    fptr strip;                            // boundary strips, [nt][2 fields][SF] (FP32 storage)
+    fptr stage;                            // one step's strips [2][SF] before packing (16-bit storage)
+    gptr<unsigned short> strip_h;          // packed strips [nt][2][SF] (FP16 / BF16 storage)
+    fptr strip_scale;                      // max-abs scale per step and field [nt][2]
+    fptr gvp, gvs, gil;                    // per-shot gradient accumulators [nx][nz]
+    fptr grho;                             // per-shot density gradient [nx][nz] (grad_rho)
+    fptr vold_x, vold_z;                   // v(it+1) before the reconstruction velocity step (grad_rho)
+    gptr<double> trace_sum;                // per-trace partial misfit [ntr]
     int isx, isz, igz, ntr, ninj, left;
 };
 
@@ -80,8 +104,9 @@ struct Geo {
     int cm_xs;              // column-march segment width (FAST kernels)
     int cm_nbands, cm_waves;// row bands per column strip, waves per shot
     int cm_per_xcd;         // column-march workgroups per XCD
-    int cm_order;           // 0: grid.y = shot; 1: 1-D grid, shots of a tile adjacent
-    int cm_nshots;          // shots of the launch (cm_order 1)
+    int cm_order;           // 0: grid.y = shot; 1: 1-D grid, shots of a tile adjacent; 2: 1-D, whole shots per XCD
+    int cm_nshots;          // shots of the launch (cm_order 1, 2)
+    int cm_wg;              // column-march workgroups per shot
     int sf;                 // strip floats per field and step
     int store;              // strip storage: 0 FP32, 1 FP16, 2 BF16 (per-step max-abs scale)
     int grad_rho;           // 1: also image the density
@@ -113,7 +138,7 @@ __device__ inline bool in_band(const Geo &g, int ix, int iz) {
 }
 
 // sum_k c_k (f[i+k s] - f[i-(k-1) s])   taps i-3..i+4
-template <bool EXACT> __device__ inline float d_plus(const float *__restrict__ f, int i, int s) {
+template <bool EXACT, class P> __device__ inline float d_plus(const P &f, int i, int s) {
     if (EXACT) {
         float t = 0.0f;
         t += kC[1] * (f[i + s] - f[i]);
@@ -130,7 +155,7 @@ template <bool EXACT> __device__ inline float d_plus(const float *__restrict__ f
     }
 }
 // sum_k c_k (f[i+(k-1) s] - f[i-k s])   taps i-4..i+3
-template <bool EXACT> __device__ inline float d_minus(const float *__restrict__ f, int i, int s) {
+template <bool EXACT, class P> __device__ inline float d_minus(const P &f, int i, int s) {
     if (EXACT) {
         float t = 0.0f;
         t += kC[1] * (f[i] - f[i - s]);
@@ -521,11 +546,11 @@ template <bool EXACT>
 __device__ inline void adj_vel_cell(const Geo &g, const Medium &m, const Fields &F, int i, bool split,
                                     int dir) {
     const int S = g.nzp;
-    const float *__restrict__ lam = m.lam;
-    const float *__restrict__ mu = m.mu;
-    const float *__restrict__ txx = F.txx;
-    const float *__restrict__ tzz = F.tzz;
-    const float *__restrict__ txz = F.txz;
+    const cfptr lam = m.lam;
+    const cfptr mu = m.mu;
+    const cfptr txx = F.txx;
+    const cfptr tzz = F.tzz;
+    const cfptr txz = F.txz;
     if (EXACT) {
         float s_txx_x = 0.f, s_txz_z = 0.f, s_tzz_x = 0.f, s_tzz_z = 0.f, s_txz_x = 0.f, s_txx_z = 0.f;
 #pragma unroll
@@ -641,7 +666,7 @@ __global__ void __launch_bounds__(BLK) k_vel(Geo g, const DevShot *__restrict__ 
         strip_slots(g, ix, iz, sl);
         if (sl[0] >= 0 || sl[1] >= 0 || sl[2] >= 0 || sl[3] >= 0) {
             const float vx = d.fw.vx[i], vz = d.fw.vz[i];
-            float *base = g.store ? d.stage : d.strip + (size_t)it * 2 * g.sf;
+            const fptr base = g.store ? d.stage : d.strip + (size_t)it * 2 * g.sf;
 #pragma unroll
             for (int k = 0; k < 4; ++k)
                 if (sl[k] >= 0) {
@@ -711,12 +736,12 @@ __global__ void k_strip_load(Geo g, const DevShot *__restrict__ shots, int it) {
     }
     const int i = ix * g.nzp + iz;
     if (g.store) {
-        const unsigned short *h = d.strip_h + (size_t)it * 2 * g.sf;
-        const float *sc = d.strip_scale + 2 * (size_t)it;
+        const gptr<unsigned short> h = d.strip_h + (size_t)it * 2 * g.sf;
+        const fptr sc = d.strip_scale + 2 * (size_t)it;
         d.fw.vx[i] = unpack16(g.store, h[t]) * sc[0];
         d.fw.vz[i] = unpack16(g.store, h[g.sf + t]) * sc[1];
     } else {
-        const float *base = d.strip + (size_t)it * 2 * g.sf;
+        const fptr base = d.strip + (size_t)it * 2 * g.sf;
         d.fw.vx[i] = base[t];
         d.fw.vz[i] = base[g.sf + t];
     }
@@ -729,7 +754,7 @@ __global__ void __launch_bounds__(256) k_strip_pack(Geo g, const DevShot *__rest
     __shared__ float red[256];
     const DevShot &d = shots[blockIdx.y];
     const int f = blockIdx.x;
-    const float *v = d.stage + (size_t)f * g.sf;
+    const fptr v = d.stage + (size_t)f * g.sf;
     float m = 0.0f;
     for (int k = threadIdx.x; k < g.sf; k += 256) m = fmaxf(m, fabsf(v[k]));
     red[threadIdx.x] = m;
@@ -740,7 +765,7 @@ __global__ void __launch_bounds__(256) k_strip_pack(Geo g, const DevShot *__rest
     }
     const float scale = red[0];
     const float inv = scale > 0.0f ? 1.0f / scale : 0.0f;
-    unsigned short *h = d.strip_h + ((size_t)it * 2 + f) * g.sf;
+    const gptr<unsigned short> h = d.strip_h + ((size_t)it * 2 + f) * g.sf;
     for (int k = threadIdx.x; k < g.sf; k += 256) h[k] = pack16(g.store, v[k] * inv);
     if (threadIdx.x == 0) d.strip_scale[2 * (size_t)it + f] = scale;
 }
@@ -791,7 +816,7 @@ __global__ void k_trace_l2(Geo g, const DevShot *__restrict__ shots) {
     const int tr = blockIdx.x * blockDim.x + threadIdx.x;
     const DevShot &d = shots[blockIdx.y];
     if (tr >= d.ntr) return;
-    const float *z = d.rz + (size_t)tr * g.nt, *x = d.rx + (size_t)tr * g.nt;
+    const fptr z = d.rz + (size_t)tr * g.nt, x = d.rx + (size_t)tr * g.nt;
     double s = 0.0;
     for (int j = 0; j < g.nt; ++j) s += (double)(z[j] * z[j] + x[j] * x[j]);
     d.trace_sum[tr] = s;

@@ -34,15 +34,22 @@ struct Seg {
 // Order 0: grid.y is the shot, L the workgroup tile of that shot.  Order 1:
 // one 1-D grid over (tile, shot) pairs with the shot fastest, so the shots of
 // one tile run back to back on one XCD and read the medium they share from
-// that XCD's L2.
+// that XCD's L2.  Order 2: one 1-D grid over (shot, tile) pairs with the tile
+// fastest, so each XCD owns whole shots: the split PML band, which costs
+// about twice the interior per cell, is then spread evenly over the XCDs
+// instead of landing on the two that own the grid's left and right edges.
 __device__ inline bool seg_of(const Geo &g, Seg &s) {
     const int b = blockIdx.x;
     const int L = (b & 7) * g.cm_per_xcd + (b >> 3);
     int tile = L;
     s.shot = blockIdx.y;
-    if (g.cm_order) {
+    if (g.cm_order == 1) {
         tile = L / g.cm_nshots;
         s.shot = L - tile * g.cm_nshots;
+    } else if (g.cm_order == 2) {
+        s.shot = L / g.cm_wg;
+        tile = L - s.shot * g.cm_wg;
+        if (s.shot >= g.cm_nshots) return false;
     }
     // the wave index is uniform across the wave: say so, so segment bounds,
     // loop trip counts and prefetch guards are scalar (no exec-mask branches)
@@ -96,10 +103,11 @@ __device__ inline void shift(float *w) {
     for (int j = 0; j < 7; ++j) w[j] = w[j + 1];
 }
 
-// Window loads come from a valid address (the row clamped into the grid) and
-// the zero of an out-of-grid lane is a select after the load, not a branch
-// around it: a value written into a register that a load in flight also
-// targets makes the wave wait for that load.
+// Every load comes from a valid address (the row clamped into the grid), and
+// the zero of an out-of-grid lane is a select applied when the value is
+// consumed a column later, not when it is loaded: an instruction that reads a
+// load's result, or writes a register that a load in flight targets, makes
+// the wave wait for that load, which would serialise the prefetch.
 struct Lane {
     int z, zc;          // row, row clamped into the grid (load address)
     bool zin, act;
@@ -113,24 +121,42 @@ __device__ inline Lane lane_of(const Geo &g, const Seg &s) {
     l.act = lane >= 4 && lane < 60 && l.z < g.nzp - NOP;
     return l;
 }
-__device__ inline float ld(const float *__restrict__ f, const Geo &g, const Lane &l, int x) {
-    const float v = f[x * g.nzp + l.zc];
-    return l.zin ? v : 0.0f;
+template <class P> __device__ inline float ld(const P &f, const Geo &g, const Lane &l, int x) {
+    return f[x * g.nzp + l.zc];
 }
+__device__ inline float zs(const Lane &l, float v) { return l.zin ? v : 0.0f; }
 // True on every lane when some lane of the wave holds p (wave-uniform value).
 __device__ inline bool any_lane(bool p) { return __ballot(p) != 0; }
 
-// Own-cell loader of the adjoint velocity march, into a ring slot that
-// persists across columns: the totals always, the split parts and the damping
-// only when some lane of the wave is split (a uniform branch with no else, so
-// the slot keeps its registers and no copy waits for the loads).  The update
-// math reads only the variant its lane uses.
-__device__ inline void vel_load_m(const Medium &m, const Fields &F, int i, bool any, VelCell &s) {
-    s.bu = m.bu[i];
-    s.vx = F.vx[i];
-    s.vz = F.vz[i];
-    if (any) {
+// Own-cell loaders of the marches, into ring slots that persist across
+// columns: an active lane loads the totals when it is unsplit and the split
+// parts and damping when it is split.  Both are lane-masked branches with no
+// else, so a slot keeps its registers (nothing writes a register that a load
+// in flight targets, which would make the wave wait for it), a wave that
+// straddles the band edge fetches only the cache lines each variant needs, and
+// a variant no lane uses is skipped.  The update math reads only the variant
+// its lane uses.
+__device__ inline void vel_load_w(const Medium &m, const Fields &F, int i, bool act, bool sp, VelCell &s) {
+    if (act) s.bu = m.bu[i];
+    if (act && !sp) { s.vx = F.vx[i]; s.vz = F.vz[i]; }
+    if (act && sp) {
         s.vx_x = F.vx_x[i]; s.vx_z = F.vx_z[i]; s.vz_x = F.vz_x[i]; s.vz_z = F.vz_z[i];
+        s.ax = m.dampx[i]; s.az = m.dampz[i];
+    }
+}
+__device__ inline void str_load_w(const Medium &m, const Fields &F, int i, bool act, bool sp, StrCell &s) {
+    if (act) { s.lam = m.lam[i]; s.mu = m.mu[i]; }
+    if (act && !sp) { s.txx = F.txx[i]; s.tzz = F.tzz[i]; s.txz = F.txz[i]; }
+    if (act && sp) {
+        s.xx_x = F.txx_x[i]; s.xx_z = F.txx_z[i]; s.zz_x = F.tzz_x[i];
+        s.zz_z = F.tzz_z[i]; s.xz_x = F.txz_x[i]; s.xz_z = F.txz_z[i];
+        s.ax = m.dampx[i]; s.az = m.dampz[i];
+    }
+}
+__device__ inline void adj_str_load_w(const Medium &m, const Fields &F, int i, bool act, bool sp, AdjStrCell &s) {
+    if (act && !sp) { s.txx = F.txx[i]; s.tzz = F.tzz[i]; s.txz = F.txz[i]; }
+    if (act && sp) {
+        s.xx_x = F.txx_x[i]; s.zz_z = F.tzz_z[i]; s.xz_x = F.txz_x[i]; s.xz_z = F.txz_z[i];
         s.ax = m.dampx[i]; s.az = m.dampz[i];
     }
 }
@@ -185,15 +211,16 @@ __global__ void __launch_bounds__(BLK) km_vel(Geo g, const DevShot *__restrict__
         qxx[f][j] = ld(F.txx, g, l, c + 4);
         qxz[f][j] = ld(F.txz, g, l, c + 3);
         qzz[f][j] = ld(F.tzz, g, l, c);
-        qst[f][j] = l.act ? vel_load(d.m, F, c * g.nzp + l.z, in_band(g, c, l.z)) : VelCell{};
+        const bool sp = in_band(g, c, l.z);
+        vel_load_w(d.m, F, c * g.nzp + l.zc, l.act, sp, qst[f][j]);
     };
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
         const Fields &F = f ? d.bn : d.fw;
 #pragma unroll
         for (int j = 0; j < 7; ++j) {
-            wxx[f][j] = ld(F.txx, g, l, s.x0 - 3 + j);
-            wxz[f][j] = ld(F.txz, g, l, s.x0 - 4 + j);
+            wxx[f][j] = zs(l, ld(F.txx, g, l, s.x0 - 3 + j));
+            wxz[f][j] = zs(l, ld(F.txz, g, l, s.x0 - 4 + j));
         }
 #pragma unroll
         for (int j = 0; j < PD; ++j)
@@ -209,9 +236,9 @@ __global__ void __launch_bounds__(BLK) km_vel(Geo g, const DevShot *__restrict__
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
                 const Fields &F = f ? d.bn : d.fw;
-                wxx[f][7] = qxx[f][j];
-                wxz[f][7] = qxz[f][j];
-                const float tzz = qzz[f][j];
+                wxx[f][7] = zs(l, qxx[f][j]);
+                wxz[f][7] = zs(l, qxz[f][j]);
+                const float tzz = zs(l, qzz[f][j]);
                 VelCell st = qst[f][j];
                 if (c + PD < s.x1) fetch(f, j, c + PD);
                 const float a = x_win(wxx[f]);        // Dx+ txx
@@ -230,7 +257,7 @@ __global__ void __launch_bounds__(BLK) km_vel(Geo g, const DevShot *__restrict__
                     if (SAVE && f == 0 && split) {
                         int sl[4];
                         strip_slots(g, c, l.z, sl);
-                        float *base = g.store ? d.stage : d.strip + (size_t)it * 2 * g.sf;
+                        const fptr base = g.store ? d.stage : d.strip + (size_t)it * 2 * g.sf;
 #pragma unroll
                         for (int k = 0; k < 4; ++k)
                             if (sl[k] >= 0) {
@@ -264,15 +291,16 @@ __global__ void __launch_bounds__(BLK) km_str(Geo g, const DevShot *__restrict__
         const Fields &F = f ? d.bn : d.fw;
         qvx[f][j] = ld(F.vx, g, l, c + 3);
         qvz[f][j] = ld(F.vz, g, l, c + 4);
-        qst[f][j] = l.act ? str_load(d.m, F, c * g.nzp + l.z, split_of(f, c)) : StrCell{};
+        const bool sp = split_of(f, c);
+        str_load_w(d.m, F, c * g.nzp + l.zc, l.act, sp, qst[f][j]);
     };
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
         const Fields &F = f ? d.bn : d.fw;
 #pragma unroll
         for (int j = 0; j < 7; ++j) {
-            wvx[f][j] = ld(F.vx, g, l, s.x0 - 4 + j);
-            wvz[f][j] = ld(F.vz, g, l, s.x0 - 3 + j);
+            wvx[f][j] = zs(l, ld(F.vx, g, l, s.x0 - 4 + j));
+            wvz[f][j] = zs(l, ld(F.vz, g, l, s.x0 - 3 + j));
         }
 #pragma unroll
         for (int j = 0; j < PD; ++j)
@@ -289,8 +317,8 @@ __global__ void __launch_bounds__(BLK) km_str(Geo g, const DevShot *__restrict__
             StrCell st[NF];
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
-                wvx[f][7] = qvx[f][j];
-                wvz[f][7] = qvz[f][j];
+                wvx[f][7] = zs(l, qvx[f][j]);
+                wvz[f][7] = zs(l, qvz[f][j]);
                 st[f] = qst[f][j];
                 if (c + PD < s.x1) fetch(f, j, c + PD);
                 v[f].xx = x_win(wvx[f]);            // Dx- vx
@@ -334,12 +362,13 @@ __global__ void __launch_bounds__(BLK) km_adj_str(Geo g, const DevShot *__restri
     auto fetch = [&](int j, int c) {
         qvx[j] = ld(F.vx, g, l, c + 3);
         qvz[j] = ld(F.vz, g, l, c + 4);
-        qst[j] = l.act ? adj_str_load(d.m, F, c * g.nzp + l.z, in_band(g, c, l.z)) : AdjStrCell{};
+        const bool sp = in_band(g, c, l.z);
+        adj_str_load_w(d.m, F, c * g.nzp + l.zc, l.act, sp, qst[j]);
     };
 #pragma unroll
     for (int j = 0; j < 7; ++j) {
-        wvx[j] = ld(F.vx, g, l, s.x0 - 4 + j);
-        wvz[j] = ld(F.vz, g, l, s.x0 - 3 + j);
+        wvx[j] = zs(l, ld(F.vx, g, l, s.x0 - 4 + j));
+        wvz[j] = zs(l, ld(F.vz, g, l, s.x0 - 3 + j));
     }
 #pragma unroll
     for (int j = 0; j < PD; ++j)
@@ -351,8 +380,8 @@ __global__ void __launch_bounds__(BLK) km_adj_str(Geo g, const DevShot *__restri
             if (c >= s.x1) break;
             const bool split = in_band(g, c, l.z);
             const int i = c * g.nzp + l.z;
-            wvx[7] = qvx[j];
-            wvz[7] = qvz[j];
+            wvx[7] = zs(l, qvx[j]);
+            wvz[7] = zs(l, qvz[j]);
             AdjStrCell st = qst[j];
             if (c + PD < s.x1) fetch(j, c + PD);
             VGrad v;
@@ -387,7 +416,7 @@ __global__ void __launch_bounds__(BLK) km_adj_vel_image(Geo g, const DevShot *__
     struct PCell {
         float lam, mu, a, b;
     };
-    auto PL = [&](int x, const float *a, const float *b) {   // adj_p1(lam, mu, a[x], b[x])
+    auto PL = [&](int x, const fptr &a, const fptr &b) {   // adj_p1(lam, mu, a[x], b[x])
         const int i = x * g.nzp + l.zc;
         PCell p;
         p.lam = m.lam[i];
@@ -429,7 +458,10 @@ __global__ void __launch_bounds__(BLK) km_adj_vel_image(Geo g, const DevShot *__
         q1[j] = PL(c + 4, F.txx, F.tzz);
         q2[j] = PM(c + 3);
         q3[j] = PL(c, F.tzz, F.txx);
-        vel_load_m(m, F, c * g.nzp + l.zc, any_lane(l.act && split_of(c)), qst[j]);
+        {
+            const bool sp = split_of(c);
+            vel_load_w(m, F, c * g.nzp + l.zc, l.act, sp, qst[j]);
+        }
         const int q = img_q(c);
         img_load(d, c * g.nzp + l.zc, q < 0 ? 0 : q, any_lane(q >= 0), qim[j]);
         if (g.grad_rho) img_load_rho(d, c * g.nzp + l.zc, q < 0 ? 0 : q, any_lane(q >= 0), qim[j]);

@@ -210,10 +210,10 @@ int ensure_slots(sfwi_ctx *c, int n, bool born, bool strips) {
             int r = dalloc(c, (void **)&S.block, (34 + 2 * c->g.grad_rho) * cell);
             if (r) return r;
             float *f = (float *)S.block;
-            float **fw[15] = {&S.fw.vx, &S.fw.vz, &S.fw.txx, &S.fw.tzz, &S.fw.txz, &S.fw.vx_x, &S.fw.vx_z,
+            fptr *fw[15] = {&S.fw.vx, &S.fw.vz, &S.fw.txx, &S.fw.tzz, &S.fw.txz, &S.fw.vx_x, &S.fw.vx_z,
                               &S.fw.vz_x, &S.fw.vz_z, &S.fw.txx_x, &S.fw.txx_z, &S.fw.tzz_x, &S.fw.tzz_z,
                               &S.fw.txz_x, &S.fw.txz_z};
-            float **bk[15] = {&S.bk.vx, &S.bk.vz, &S.bk.txx, &S.bk.tzz, &S.bk.txz, &S.bk.vx_x, &S.bk.vx_z,
+            fptr *bk[15] = {&S.bk.vx, &S.bk.vz, &S.bk.txx, &S.bk.tzz, &S.bk.txz, &S.bk.vx_x, &S.bk.vx_z,
                               &S.bk.vz_x, &S.bk.vz_z, &S.bk.txx_x, &S.bk.txx_z, &S.bk.tzz_x, &S.bk.tzz_z,
                               &S.bk.txz_x, &S.bk.txz_z};
             for (int k = 0; k < 15; ++k) *fw[k] = f + k * c->ncell;
@@ -240,7 +240,7 @@ int ensure_slots(sfwi_ctx *c, int n, bool born, bool strips) {
             int r = dalloc(c, (void **)&S.born, 15 * cell);
             if (r) return r;
             float *f = (float *)S.born;
-            float **bn[15] = {&S.bn.vx, &S.bn.vz, &S.bn.txx, &S.bn.tzz, &S.bn.txz, &S.bn.vx_x, &S.bn.vx_z,
+            fptr *bn[15] = {&S.bn.vx, &S.bn.vz, &S.bn.txx, &S.bn.tzz, &S.bn.txz, &S.bn.vx_x, &S.bn.vx_z,
                               &S.bn.vz_x, &S.bn.vz_z, &S.bn.txx_x, &S.bn.txx_z, &S.bn.tzz_x, &S.bn.tzz_z,
                               &S.bn.txz_x, &S.bn.txz_z};
             for (int k = 0; k < 15; ++k) *bn[k] = f + k * c->ncell;
@@ -320,7 +320,7 @@ int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium 
             char *q = (char *)S.strip;
             d.strip_scale = (float *)q;
             d.stage = d.strip_scale + (size_t)c->p.nt * 2;
-            d.strip_h = (unsigned short *)(d.stage + 2 * (size_t)c->g.sf);
+            d.strip_h = (unsigned short *)(float *)(d.stage + 2 * (size_t)c->g.sf);
             (void)nv;
         }
         d.gvp = S.gvp; d.gvs = S.gvs; d.gil = S.gil; d.grho = S.grho; d.trace_sum = S.trace_sum;
@@ -352,7 +352,8 @@ void set_march_geo(Geo &g, int cols) {
     g.cm_xs = cols > 0 ? cols : 8;
     g.cm_nbands = (g.nzd + CM_ROWS - 1) / CM_ROWS;
     g.cm_waves = ((g.nxp - 2 * NOP + g.cm_xs - 1) / g.cm_xs) * g.cm_nbands;
-    g.cm_per_xcd = ((g.cm_waves + BLK / 64 - 1) / (BLK / 64) + 7) / 8;
+    g.cm_wg = (g.cm_waves + BLK / 64 - 1) / (BLK / 64);
+    g.cm_per_xcd = (g.cm_wg + 7) / 8;
 }
 
 void drop_graphs(sfwi_ctx *c) {
@@ -458,11 +459,11 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
     // column-march launches: block order per sfwi_params.march_order
     Geo gm = g;
     dim3 tiles((unsigned)(8 * g.cm_per_xcd), (unsigned)n);
-    if (c->p.march_order == 2) {
-        const long ntile = (g.cm_waves + BLK / 64 - 1) / (BLK / 64);
-        gm.cm_order = 1;
+    const int order = c->p.march_order ? c->p.march_order : 3;
+    if (order >= 2) {
+        gm.cm_order = order - 1;
         gm.cm_nshots = n;
-        gm.cm_per_xcd = (int)((ntile * n + 7) / 8);
+        gm.cm_per_xcd = (int)(((long)g.cm_wg * n + 7) / 8);
         tiles = dim3((unsigned)(8 * gm.cm_per_xcd), 1);
     }
     // per-cell kernels on request (march_cols < 0), or automatically (march_cols 0)
@@ -553,7 +554,7 @@ int run_direct(sfwi_ctx *c, Kind kind, int n) {
     case 1: return run_batch<false, 1>(c, kind, n);
     case 2: return run_batch<false, 2>(c, kind, n);
     case 4: return run_batch<false, 4>(c, kind, n);
-    default: return run_batch<false, 1>(c, kind, n);
+    default: return run_batch<false, 4>(c, kind, n);
     }
 }
 
@@ -649,7 +650,7 @@ int sfwi_ctx_create(const sfwi_params *p, sfwi_ctx **out) {
     if (p->mode != SFWI_MODE_FAST && p->mode != SFWI_MODE_EXACT) return fail(nullptr, SFWI_EINVAL, "unknown mode %d", p->mode);
     if (p->storage < SFWI_STORE_STRIPS_F32 || p->storage > SFWI_STORE_STRIPS_BF16)
         return fail(nullptr, SFWI_EINVAL, "unknown strip storage %d", p->storage);
-    if (p->march_order < 0 || p->march_order > 2) return fail(nullptr, SFWI_EINVAL, "unknown march_order %d", p->march_order);
+    if (p->march_order < 0 || p->march_order > 3) return fail(nullptr, SFWI_EINVAL, "unknown march_order %d", p->march_order);
     if (p->misfit < SFWI_MISFIT_L2 || p->misfit > SFWI_MISFIT_OT_GATHER)
         return fail(nullptr, SFWI_EINVAL, "unknown misfit %d", p->misfit);
     const long nxp = p->nx + 2L * p->pml, nzp = p->nz + 2L * p->pml;
@@ -942,7 +943,7 @@ int sfwi_autotune(sfwi_ctx *c, int nshots, int steps, int *march_cols, int *marc
     const size_t acc = 4 * c->nmodel * sizeof(float);
     HIPCK(c, hipMemcpyAsync(c->tmp_model, c->Gvp, acc, hipMemcpyDeviceToDevice, c->stream));
     struct Cand { int cols, depth, order; };
-    const Cand cands[] = {{-1, 1, 1}, {8, 1, 1}, {8, 2, 1}, {16, 1, 1}, {8, 1, 2}};
+    const Cand cands[] = {{-1, 1, 1}, {8, 4, 3}, {8, 2, 3}, {8, 1, 3}, {16, 2, 3}, {8, 4, 1}, {8, 2, 2}};
     const sfwi_params keep = c->p;
     const int nt_keep = c->g.nt;
     c->g.nt = ns;                              // a shorter record inside the same buffers

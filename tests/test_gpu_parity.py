@@ -287,12 +287,13 @@ def test_single_trace_and_errors():
         assert np.array_equal(gz2, oz)
 
 
-@pytest.mark.parametrize("cols,depth,order", [(8, 1, 2), (8, 1, 1), (16, 2, 2), (37, 4, 1)])
+@pytest.mark.parametrize("cols,depth,order", [(8, 1, 2), (8, 1, 1), (16, 2, 2), (37, 4, 1), (8, 0, 0), (8, 4, 3),
+                                               (16, 1, 3), (37, 2, 3)])
 def test_march_kernels_match_per_cell(cols, depth, order):
     """The column-march FAST kernels use the per-cell FAST update math in the
     same operation order, so gradients, misfits and Born traces agree bitwise
-    for any segment width (37: segments that do not divide the grid) and
-    either block order."""
+    for any segment width (37: segments that do not divide the grid), any
+    prefetch depth (0: the default) and every block order (0: the default)."""
     sv, dvp, dvs = _born_case()
     w = W.ricker(sv.nt, sv.dt, sv.f0)
     shots = []
