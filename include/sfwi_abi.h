@@ -244,6 +244,39 @@ void sfwi_server_destroy(sfwi_server *s);
 /* Reference host-driver helper: Ricker wavelet (sfwi-stage1/main.cpp:251-260). */
 void sfwi_ricker(float *w, int nt, float dt, float f0);
 
+/* ---- Host steps of the inversion loops (CPU code, no device use) -----------
+ * Model-size arrays are allnx*allnz FP32, index x*allnz + z, the reference's
+ * arithmetic types (FP32 arrays, FP64 sums and step lengths).  The caller
+ * keeps the conjugate-gradient state between iterations: the previous
+ * direction (pp_cg_old, ps_cg_old, zero before iteration 0) and beta_fenmu. */
+
+/* Stage-1 search direction (sfwi-stage1/main.cpp:695-750, 809-839,
+ * 1059-1062): g / illumination, zero above cut_layer (the reference uses 26),
+ * beta = sum g (g / illum) / beta_fenmu (0 at iter 0), cg = pc + beta cg_old,
+ * then dir = cg normalized by the joint max-abs of both components.  Updates
+ * pp_cg_old, ps_cg_old (to cg) and *beta_fenmu; *beta (may be NULL) gets beta. */
+int sfwi_s1_direction(int allnx, int allnz, int iter, int cut_layer, const float *pp_grad,
+                      const float *ps_grad, const float *illum, float *pp_cg_old, float *ps_cg_old,
+                      double *beta_fenmu, float *dir_pp, float *dir_ps, double *beta);
+/* Stage-1 step length from the three line-search misfits at 0, step1, step2
+ * (cal_step2, sfwi-stage1/main.cpp:65-72), clamped to [-100, 300] (1229-1238). */
+double sfwi_s1_alpha(double m0, double m1, double m2, float step1, float step2);
+/* Stage-1 model update (sfwi-stage1/main.cpp:1241-1257): vel -= alpha dir, then
+ * every cell with z < cut_layer takes the value of the model's first cell. */
+int sfwi_s1_update(int allnx, int allnz, int cut_layer, double alpha, const float *dir_pp,
+                   const float *dir_ps, float *velp, float *vels);
+/* Stage-2 direction (sfwi-stage2/main.cpp:645-703): rows z >= cut_layer (the
+ * reference uses 26) take g / illum, rows above are zeroed in pp_grad, ps_grad
+ * themselves (both are modified, as in the reference); beta = sum (g^2 / illum)
+ * / beta_fenmu (0 at iter 0); pp_cg, ps_cg = pc + beta cg_old. */
+int sfwi_s2_direction(int allnx, int allnz, int iter, int cut_layer, float *pp_grad, float *ps_grad,
+                      const float *illum, float *pp_cg_old, float *ps_cg_old, double *beta_fenmu,
+                      float *pp_cg, float *ps_cg, double *beta);
+/* Stage-2 update (sfwi-stage2/main.cpp:861-873): alpha = fenzi / fenmu from
+ * sfwi_linesearch_linear summed over shots, image -= alpha cg; *alpha may be NULL. */
+int sfwi_s2_update(int allnx, int allnz, double fenzi, double fenmu, const float *pp_cg,
+                   const float *ps_cg, float *image_pp, float *image_ps, double *alpha);
+
 /* Measurement hooks (no reference counterpart).  With sample_every = k > 0
  * every k-th time step of a solve brackets each stencil kernel launch with
  * HIP events on the context's stream; sfwi_kernel_time reports the mean

@@ -115,6 +115,13 @@ def load():
     L.sfwi_kernel_name.argtypes = [ctypes.c_int]
     L.sfwi_autotune.argtypes = [vp, ctypes.c_int, ctypes.c_int, P(ctypes.c_int), P(ctypes.c_int),
                                 P(ctypes.c_int), dp]
+    i = ctypes.c_int
+    L.sfwi_s1_direction.argtypes = [i, i, i, i, _f32p, _f32p, _f32p, _f32p, _f32p, dp, _f32p, _f32p, dp]
+    L.sfwi_s1_alpha.restype = ctypes.c_double
+    L.sfwi_s1_alpha.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_float, ctypes.c_float]
+    L.sfwi_s1_update.argtypes = [i, i, i, ctypes.c_double, _f32p, _f32p, _f32p, _f32p]
+    L.sfwi_s2_direction.argtypes = [i, i, i, i, _f32p, _f32p, _f32p, _f32p, _f32p, dp, _f32p, _f32p, dp]
+    L.sfwi_s2_update.argtypes = [i, i, ctypes.c_double, ctypes.c_double, _f32p, _f32p, _f32p, _f32p, dp]
     L.sfwi_solve_bytes.restype = ctypes.c_size_t
     L.sfwi_solve_bytes.argtypes = [P(SfwiParams), ctypes.c_int, ctypes.c_int]
     L.sfwi_server_create.argtypes = [ctypes.c_size_t, ctypes.c_int, P(vp)]

@@ -1063,3 +1063,4 @@ int sfwi_ot_misfit(sfwi_ctx *c, int variant, int ntr, int nt, float dt, const fl
 }  // extern "C"
 
 #include "server.hpp"
+#include "inversion.hpp"

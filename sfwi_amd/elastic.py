@@ -146,15 +146,18 @@ class Context:
         self._ck(fn(self.h, len(shots), arr, m.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return m
 
-    def linesearch_parabola(self, shot, dir_vp, dir_vs, step1, step2):
-        m = np.zeros(3, np.float64)
+    def linesearch_parabola(self, shot, dir_vp, dir_vs, step1, step2, m=None):
+        """Misfits at steps 0, step1, step2 added to m (zeros when None), as the
+        reference accumulates them across shots; returns m."""
+        m = np.zeros(3, np.float64) if m is None else np.array(m, np.float64)
         self._ck(self.L.sfwi_linesearch_parabola(self.h, ctypes.byref(shot.c), _f32(dir_vp), _f32(dir_vs),
                                                  step1, step2,
                                                  m.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return m
 
-    def linesearch_linear(self, shot, cg_vp, cg_vs):
-        a, b = ctypes.c_double(0.0), ctypes.c_double(0.0)
+    def linesearch_linear(self, shot, cg_vp, cg_vs, fenzi=0.0, fenmu=0.0):
+        """This shot's terms added to (fenzi, fenmu); returns the sums."""
+        a, b = ctypes.c_double(fenzi), ctypes.c_double(fenmu)
         self._ck(self.L.sfwi_linesearch_linear(self.h, ctypes.byref(shot.c), _f32(cg_vp), _f32(cg_vs),
                                                ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value

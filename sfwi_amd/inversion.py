@@ -1,0 +1,138 @@
+"""The reference drivers' inversion loops over the per-shot solves.
+
+``Stage1`` is the stage-1 iteration of sfwi-stage1/main.cpp (the
+configuration the reference runs: update_mp, no Gauss-Newton inner loop,
+conjugate gradient): batched gradient of every shot, illumination
+preconditioning with the cut layer, CG direction normalized by its joint
+max-abs, parabolic line search at two trial steps, clamped step, model update
+and top-row reset.  ``Stage2`` is the stage-2 iteration of
+sfwi-stage2/main.cpp: Born gradient, preconditioned CG direction, exact
+linear step, image update.  The host steps run in the library
+(sfwi_s1_* / sfwi_s2_*, CPU code with the reference's arithmetic types); the
+solves run on the GPU through a ``Context``.  Shots are summed in list order,
+where the reference sums over MPI ranks.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+
+def _f32c(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class _CG:
+    """Conjugate-gradient state kept between iterations (pp_cg_old,
+    ps_cg_old, beta_fenmu; sfwi-stage1/main.cpp:376-377)."""
+
+    def __init__(self, allnx, allnz, cut_layer):
+        self.allnx, self.allnz, self.cut_layer = allnx, allnz, cut_layer
+        self.pp_cg_old = np.zeros((allnx, allnz), np.float32)
+        self.ps_cg_old = np.zeros((allnx, allnz), np.float32)
+        self.beta_fenmu = 0.0
+        self.iter = 0
+        self.max_misfit = None
+        self.L = _lib.load()
+
+    def _ck(self, rc):
+        if rc != 0:
+            raise _lib.SfwiError(rc, self.L.sfwi_last_error(None).decode())
+
+    def _rms(self, misfit):
+        if self.max_misfit is None:
+            self.max_misfit = misfit
+        return misfit / self.max_misfit
+
+
+class Stage1(_CG):
+    """Stage-1 loop state and steps (sfwi-stage1/main.cpp:366-416 locals:
+    cut_layer 26, step1 30, step2 150)."""
+
+    def __init__(self, allnx, allnz, cut_layer=26, step1=30.0, step2=150.0):
+        super().__init__(allnx, allnz, cut_layer)
+        self.step1, self.step2 = float(step1), float(step2)
+
+    def direction(self, pp_grad, ps_grad, illum):
+        """Search direction from the summed gradients; advances the CG state.
+        Returns (dir_pp, dir_ps, beta)."""
+        d1 = np.zeros((self.allnx, self.allnz), np.float32)
+        d2 = np.zeros_like(d1)
+        fm = ctypes.c_double(self.beta_fenmu)
+        beta = ctypes.c_double(0.0)
+        self._ck(self.L.sfwi_s1_direction(self.allnx, self.allnz, self.iter, self.cut_layer, _f32c(pp_grad),
+                                          _f32c(ps_grad), _f32c(illum), self.pp_cg_old, self.ps_cg_old,
+                                          ctypes.byref(fm), d1, d2, ctypes.byref(beta)))
+        self.beta_fenmu = fm.value
+        return d1, d2, beta.value
+
+    def alpha(self, m0, m1, m2):
+        return self.L.sfwi_s1_alpha(m0, m1, m2, self.step1, self.step2)
+
+    def update(self, velp, vels, dir_pp, dir_ps, alpha):
+        """vel -= alpha dir and the top-row reset, in place on float32 C arrays."""
+        self._ck(self.L.sfwi_s1_update(self.allnx, self.allnz, self.cut_layer, alpha, _f32c(dir_pp),
+                                       _f32c(dir_ps), velp, vels))
+
+    def iterate(self, ctx, shots, velp, vels, rho=None):
+        """One iteration on the context's survey; velp, vels (float32,
+        allnx x allnz) are updated in place.  Returns a record of the
+        iteration's scalars."""
+        ctx.set_model(velp, vels, rho)
+        misfit = sum(float(v) for v in ctx.batch_gradient(shots))
+        gvp, gvs, gil = ctx.flush()
+        d1, d2, beta = self.direction(gvp, gvs, gil)
+        m = np.zeros(3)
+        for s in shots:
+            m = ctx.linesearch_parabola(s, d1, d2, self.step1, self.step2, m)
+        alpha = self.alpha(*m)
+        self.update(velp, vels, d1, d2, alpha)
+        rec = dict(iter=self.iter, misfit=misfit, rms=self._rms(misfit), beta=beta, alpha=alpha,
+                   linesearch=tuple(m))
+        self.iter += 1
+        return rec
+
+
+class Stage2(_CG):
+    """Stage-2 loop state and steps (sfwi-stage2/main.cpp; rows above 26 cut)."""
+
+    def __init__(self, allnx, allnz, cut_layer=26):
+        super().__init__(allnx, allnz, cut_layer)
+
+    def direction(self, pp_grad, ps_grad, illum):
+        """Returns (pp_cg, ps_cg, beta, pp_grad', ps_grad') with the gradients
+        as the reference leaves them (rows above the cut zeroed)."""
+        g1, g2 = np.array(pp_grad, np.float32), np.array(ps_grad, np.float32)
+        c1 = np.zeros((self.allnx, self.allnz), np.float32)
+        c2 = np.zeros_like(c1)
+        fm = ctypes.c_double(self.beta_fenmu)
+        beta = ctypes.c_double(0.0)
+        self._ck(self.L.sfwi_s2_direction(self.allnx, self.allnz, self.iter, self.cut_layer, g1, g2, _f32c(illum),
+                                          self.pp_cg_old, self.ps_cg_old, ctypes.byref(fm), c1, c2,
+                                          ctypes.byref(beta)))
+        self.beta_fenmu = fm.value
+        return c1, c2, beta.value, g1, g2
+
+    def update(self, image_pp, image_ps, pp_cg, ps_cg, fenzi, fenmu):
+        a = ctypes.c_double(0.0)
+        self._ck(self.L.sfwi_s2_update(self.allnx, self.allnz, fenzi, fenmu, _f32c(pp_cg), _f32c(ps_cg),
+                                       image_pp, image_ps, ctypes.byref(a)))
+        return a.value
+
+    def iterate(self, ctx, shots, vp, vs, rho, image_pp, image_ps):
+        """One iteration: the images (float32, allnx x allnz) are the Born
+        perturbation and are updated in place."""
+        ctx.set_model(vp, vs, rho)
+        ctx.set_perturbation(image_pp, image_ps)
+        misfit = sum(float(v) for v in ctx.batch_gradient(shots, born=True))
+        gvp, gvs, gil = ctx.flush()
+        c1, c2, beta, _, _ = self.direction(gvp, gvs, gil)
+        fenzi = fenmu = 0.0
+        for s in shots:
+            fenzi, fenmu = ctx.linesearch_linear(s, c1, c2, fenzi, fenmu)
+        alpha = self.update(image_pp, image_ps, c1, c2, fenzi, fenmu)
+        rec = dict(iter=self.iter, misfit=misfit, rms=self._rms(misfit), beta=beta, alpha=alpha,
+                   fenzi=fenzi, fenmu=fenmu)
+        self.iter += 1
+        return rec

@@ -91,3 +91,43 @@ def test_driver_matches_python_path(tmp_path):
     for name, ref in (("pp_grad", gvp), ("ps_grad", gvs), ("illum", gil)):
         got = np.fromfile(tmp_path / ("o_%s.bin" % name), np.float32).reshape(ref.shape)
         assert np.array_equal(got, ref), name
+
+
+def test_driver_rejects_bad_iteration_arguments(tmp_path):
+    sv = W.tiny(nt=40)
+    recs = [(np.zeros((sv.ntr, sv.nt), np.float32),) * 2 for _ in sv.shots]
+    path = tmp_path / "s.bin"
+    write_survey(path, sv, recs)
+    r = _run(str(path), str(tmp_path / "o_"), "30", "150", "0", "4")
+    assert r.returncode == 2 and "niter" in r.stderr
+
+
+@pytest.mark.gpu
+def test_driver_stage1_iterations_match_python_path(tmp_path):
+    """Two complete stage-1 iterations of the C++ driver (the reference's
+    master-rank sequence over the adapter) equal sfwi_amd.inversion.Stage1 over
+    the Python binding, bit for bit: models, misfits, beta and alpha."""
+    from sfwi_amd.inversion import Stage1
+    sv = W.tiny(nt=220, nshots=3, seed=12)
+    w = W.ricker(sv.nt, sv.dt, sv.f0)
+    with context(sv, max_batch=len(sv.shots)) as ctx:
+        ctx.set_wavelet(w)
+        ctx.set_model(sv.vp, sv.vs, sv.rho)
+        oz, ox = ctx.batch_forward([lib_shot(sv, s) for s in sv.shots])
+        recs = [(oz[k][:len(s.gc)], ox[k][:len(s.gc)]) for k, s in enumerate(sv.shots)]
+        shots = [lib_shot(sv, s, *r) for s, r in zip(sv.shots, recs)]
+        st = Stage1(sv.allnx, sv.allnz, cut_layer=4, step1=30.0, step2=150.0)
+        vp, vs = sv.vp0.copy(), sv.vs0.copy()
+        its = [st.iterate(ctx, shots, vp, vs, sv.rho0) for _ in range(2)]
+    path = tmp_path / "s.bin"
+    write_survey(path, sv, recs)
+    r = _run(str(path), str(tmp_path / "o_"), "30", "150", "2", "4")
+    assert r.returncode == 0, r.stderr
+    lines = [l.split() for l in r.stdout.strip().splitlines()]
+    assert len(lines) == 2
+    for l, it in zip(lines, its):
+        assert int(l[1]) == it["iter"]
+        assert float(l[3]) == it["misfit"] and float(l[5]) == it["beta"] and float(l[7]) == it["alpha"]
+    for name, ref in (("vel_p", vp), ("vel_s", vs)):
+        got = np.fromfile(tmp_path / ("o_%s.bin" % name), np.float32).reshape(ref.shape)
+        assert np.array_equal(got, ref), name
