@@ -244,6 +244,55 @@ void sfwi_server_destroy(sfwi_server *s);
 /* Reference host-driver helper: Ricker wavelet (sfwi-stage1/main.cpp:251-260). */
 void sfwi_ricker(float *w, int nt, float dt, float f0);
 
+/* ---- Survey input of the reference drivers (CPU code, no device use) -------- */
+
+#define SFWI_PATH_MAX 1024
+
+/* The 10-line positional parameter file of both drivers (sfwi-stage1/
+ * main.cpp:1346-1355; the shipped para_stage1 and para_stage2 both follow it,
+ * while sfwi-stage2/main.cpp:946-954 reads one path line fewer). */
+typedef struct sfwi_para {
+    float dx, dz;
+    int pml, allnx, allnz, minshot, maxshot;
+    char vp[SFWI_PATH_MAX], vs[SFWI_PATH_MAX], rho[SFWI_PATH_MAX];
+    char rec_z[SFWI_PATH_MAX], rec_x[SFWI_PATH_MAX], outdir[SFWI_PATH_MAX];
+    float f0;
+    int maxiter, iter_round1;
+    int sy, gy;                        /* source / receiver depth below the PML, cells */
+    int light, rbc, cpu_mem, iointerval;
+} sfwi_para;
+
+/* Parse a parameter file; blank lines are skipped, a line with too few
+ * numbers or a path of SFWI_PATH_MAX characters or more is an error. */
+int sfwi_para_read(const char *path, sfwi_para *out);
+/* An x-major FP32 model (allnx columns of allnz floats, sfwi-stage1/
+ * main.cpp:1468-1473); a file of another size is an error. */
+int sfwi_model_read(const char *path, int allnx, int allnz, float *out);
+/* Shot index of an SU file, index_shot with the per-shot trace count of
+ * sfwi-stage2/main.cpp:121-216 (the stage-1 copy never counts past 1 trace):
+ * consecutive traces of equal sx form a shot; row k of table (9 ints) is
+ * {k+1, ntr, sx, 0, min(sx,gx), 0, max(sx,gx), 0, first trace}, coordinates
+ * divided by coordinate_scale.  nt and dt (s) come from the first header.
+ * *nshots gets the shot count; table may be NULL with max_shots = 0 to count. */
+int sfwi_su_index(const char *path, int coordinate_scale, int max_shots, int *nt, float *dt, int *nshots,
+                  int *table);
+/* Traces pos .. pos+ntr-1 of an SU file into data [ntr][nt] and gc = gx /
+ * coordinate_scale (read_shot_gather_su, sfwi-stage1/main.cpp:204-225). */
+int sfwi_su_read_gather(const char *path, long long pos, int ntr, int nt, int coordinate_scale, float *data,
+                        int *gc);
+/* Write (or append) one shot gather as SU traces: tracl / tracr / tracf =
+ * 1..ntr, fldr = shot, sx, gx[i], offset = gx - sx, scalco = 1, ns = nt,
+ * dt in microseconds (no reference counterpart: synthetic records). */
+int sfwi_su_write(const char *path, int append, int shot, int ntr, int nt, float dt, int sx, const int *gx,
+                  const float *data);
+/* A shot's geometry from its index row, as the worker sets it
+ * (sfwi-stage1/main.cpp:580-625): left = (xmin - all_left) / dx, isx = pml +
+ * (sx - xmin) / dx, isz = pml + sy, igz = pml + gy, ngx_left = xmin, ntr;
+ * *nx_shot (may be NULL) gets the shot's aperture (xmax - xmin) / dx + 1.
+ * gc, obs_z and obs_x are left to the caller. */
+int sfwi_shot_from_table(const int row[9], float dx, int pml, int sy, int gy, int all_left, sfwi_shot *out,
+                         int *nx_shot);
+
 /* ---- Host steps of the inversion loops (CPU code, no device use) -----------
  * Model-size arrays are allnx*allnz FP32, index x*allnz + z, the reference's
  * arithmetic types (FP32 arrays, FP64 sums and step lengths).  The caller

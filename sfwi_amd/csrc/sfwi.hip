@@ -1064,3 +1064,4 @@ int sfwi_ot_misfit(sfwi_ctx *c, int variant, int ntr, int nt, float dt, const fl
 
 #include "server.hpp"
 #include "inversion.hpp"
+#include "suio.hpp"

@@ -79,19 +79,40 @@ class Stage1(_CG):
         """One iteration on the context's survey; velp, vels (float32,
         allnx x allnz) are updated in place.  Returns a record of the
         iteration's scalars."""
-        ctx.set_model(velp, vels, rho)
-        misfit = sum(float(v) for v in ctx.batch_gradient(shots))
-        gvp, gvs, gil = ctx.flush()
-        d1, d2, beta = self.direction(gvp, gvs, gil)
+        return self.iterate_groups([(ctx, shots)], velp, vels, rho)
+
+    def iterate_groups(self, groups, velp, vels, rho=None):
+        """One iteration over several (context, shots) groups, e.g. one
+        context per shot aperture width; gradients are summed over the groups
+        in list order, misfits over the shots in order."""
+        misfit = 0.0
+        G = None
+        for ctx, shots in groups:
+            ctx.set_model(velp, vels, rho)
+            for v in _batched(ctx, shots, lambda b: ctx.batch_gradient(b)):
+                misfit += float(v)
+            g = ctx.flush()
+            G = g if G is None else tuple((a + b).astype(np.float32) for a, b in zip(G, g))
+        d1, d2, beta = self.direction(*G)
         m = np.zeros(3)
-        for s in shots:
-            m = ctx.linesearch_parabola(s, d1, d2, self.step1, self.step2, m)
+        for ctx, shots in groups:
+            for s in shots:
+                m = ctx.linesearch_parabola(s, d1, d2, self.step1, self.step2, m)
         alpha = self.alpha(*m)
         self.update(velp, vels, d1, d2, alpha)
         rec = dict(iter=self.iter, misfit=misfit, rms=self._rms(misfit), beta=beta, alpha=alpha,
                    linesearch=tuple(m))
         self.iter += 1
         return rec
+
+
+def _batched(ctx, shots, fn):
+    """fn over consecutive batches of at most max_batch shots, results chained."""
+    out = []
+    nb = max(int(ctx.p.max_batch), 1)
+    for i in range(0, len(shots), nb):
+        out.extend(fn(shots[i:i + nb]))
+    return out
 
 
 class Stage2(_CG):
@@ -125,7 +146,9 @@ class Stage2(_CG):
         perturbation and are updated in place."""
         ctx.set_model(vp, vs, rho)
         ctx.set_perturbation(image_pp, image_ps)
-        misfit = sum(float(v) for v in ctx.batch_gradient(shots, born=True))
+        misfit = 0.0
+        for v in _batched(ctx, shots, lambda b: ctx.batch_gradient(b, born=True)):
+            misfit += float(v)
         gvp, gvs, gil = ctx.flush()
         c1, c2, beta, _, _ = self.direction(gvp, gvs, gil)
         fenzi = fenmu = 0.0

@@ -1,0 +1,111 @@
+"""A stage-1 inversion read from the reference's input formats: a parameter
+file, x-major model files and SU records of shots with three different
+aperture widths (so three contexts).  The C++ program sfwi_stage1 and the
+Python loop over sfwi_amd.survey_io.SuSurvey give the same models, misfits,
+beta and alpha bit for bit, and both equal the same iterations run on the
+in-memory shots the records were modelled from (the SU round trip and the
+geometry derived from the index are lossless).
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from sfwi_amd import _lib
+from sfwi_amd import survey_io as S
+from sfwi_amd import workloads as W
+from sfwi_amd.elastic import Context, Shot
+from sfwi_amd.inversion import Stage1
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PROG = os.path.join(ROOT, "sfwi_amd", "sfwi_stage1")
+DX, PML, NT, DT, F0, SY, GY, CUT = 10.0, 8, 220, 0.001, 25.0, 2, 0, 4
+# the windows [0, 30), [25, 60), [20, 45) cover the model: a cell outside every window has zero
+# illumination, and the preconditioning divides by it, as in the reference
+SPREADS = [(150, np.arange(0, 291, 10)), (400, np.arange(250, 591, 10)), (300, np.arange(200, 441, 10))]
+
+
+def _model():
+    sv = W.tiny(nx=60, nz=40, seed=17, nt=NT)
+    return sv
+
+
+def _shot(sx, gx, all_left=0):
+    xmin, xmax = min(sx, gx.min()), max(sx, gx.max())
+    left = int((xmin - all_left) / DX)
+    width = int((xmax - xmin) / DX) + 1
+    return Shot(left, PML + int((sx - xmin) / DX), PML + SY, PML + GY, xmin, gx), width
+
+
+def _ctx(sv, width, nshots):
+    return Context(dx=DX, dz=DX, dt=DT, nx=width, nz=sv.allnz, pml=PML, nt=NT, ntr=35, allnx=sv.allnx,
+                   allnz=sv.allnz, max_batch=nshots)
+
+
+def test_stage1_from_su_files(tmp_path):
+    sv = _model()
+    w = _lib.ricker(NT, np.float32(DT), F0)
+    rho = np.full_like(sv.vp, 2000.0)
+    # observed records from the true model, written as SU
+    recs = []
+    for k, (sx, gx) in enumerate(SPREADS):
+        sh, width = _shot(sx, gx)
+        with _ctx(sv, width, 1) as ctx:
+            ctx.set_wavelet(w)
+            ctx.set_model(sv.vp, sv.vs, rho)
+            oz, ox = ctx.forward(sh)
+        S.write_su(str(tmp_path / "z.su"), k + 1, sx, gx, oz, DT, append=k > 0)
+        S.write_su(str(tmp_path / "x.su"), k + 1, sx, gx, ox, DT, append=k > 0)
+        recs.append((oz, ox))
+    sv.vp0.tofile(tmp_path / "vp.bin")
+    sv.vs0.tofile(tmp_path / "vs.bin")
+    sv.vs0.tofile(tmp_path / "rho.bin")
+    para = tmp_path / "para"
+    para.write_text("%g %g %d %d %d 1 3\n%s\n%s\n%s\n%s\n%s\n%s\n%g 2 0\n%d %d\n1 0 0 1000\n" % (
+        DX, DX, PML, sv.allnx, sv.allnz, tmp_path / "vp.bin", tmp_path / "vs.bin", tmp_path / "rho.bin",
+        tmp_path / "z.su", tmp_path / "x.su", tmp_path / "out", F0, SY, GY))
+
+    # Python over the SU survey, one context per aperture width
+    su = S.SuSurvey(para)
+    assert su.nt == NT and len(su.groups()) == 3
+    ctxs = {wd: _ctx(sv, wd, len(ix)) for wd, ix in su.groups().items()}
+    for c in ctxs.values():
+        c.set_wavelet(w)
+    groups = [(ctxs[wd], [su.shots[k] for k in ix]) for wd, ix in su.groups().items()]
+    st = Stage1(sv.allnx, sv.allnz, cut_layer=CUT)
+    vp, vs = su.vp.copy(), su.vs.copy()
+    its = [st.iterate_groups(groups, vp, vs, su.rho) for _ in range(2)]
+    for c in ctxs.values():
+        c.close()
+    assert its[1]["misfit"] < its[0]["misfit"] and np.all(np.isfinite(vp))
+
+    # the same iterations on the in-memory shots
+    st2 = Stage1(sv.allnx, sv.allnz, cut_layer=CUT)
+    vp2, vs2 = sv.vp0.copy(), sv.vs0.copy()
+    mem = []
+    for (sx, gx), (oz, ox) in zip(SPREADS, recs):
+        sh, width = _shot(sx, gx)
+        c = _ctx(sv, width, 1)
+        c.set_wavelet(w)
+        mem.append((c, [Shot(sh.c.left, sh.c.isx, sh.c.isz, sh.c.igz, sh.c.ngx_left, gx, oz, ox)]))
+    its2 = [st2.iterate_groups(mem, vp2, vs2, rho) for _ in range(2)]
+    for c, _ in mem:
+        c.close()
+    assert np.array_equal(vp, vp2) and np.array_equal(vs, vs2)
+    assert [i["alpha"] for i in its] == [i["alpha"] for i in its2]
+
+    # the C++ program on the files
+    r = subprocess.run([PROG, str(para), "2", "64", str(CUT)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    lines = [l.split() for l in r.stdout.strip().splitlines()]
+    assert len(lines) == 2
+    for l, it in zip(lines, its):
+        assert float(l[3]) == it["misfit"] and float(l[5]) == it["beta"] and float(l[7]) == it["alpha"]
+    got = np.fromfile(tmp_path / "out" / ("vel_p_1_%d_%d.dat" % (sv.allnx, sv.allnz)), np.float32)
+    assert np.array_equal(got.reshape(vp.shape), vp)
+    got = np.fromfile(tmp_path / "out" / ("vel_s_1_%d_%d.dat" % (sv.allnx, sv.allnz)), np.float32)
+    assert np.array_equal(got.reshape(vs.shape), vs)
+    assert (tmp_path / "out" / "misfit.txt").read_text().count("misfit") == 2
