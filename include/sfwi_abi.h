@@ -262,8 +262,9 @@ typedef struct sfwi_para {
     int light, rbc, cpu_mem, iointerval;
 } sfwi_para;
 
-/* Parse a parameter file; blank lines are skipped, a line with too few
- * numbers or a path of SFWI_PATH_MAX characters or more is an error. */
+/* Parse a parameter file as a stream of 22 whitespace-separated fields (the
+ * drivers' fscanf); fewer fields, a non-numeric number or a path of
+ * SFWI_PATH_MAX characters or more is an error. */
 int sfwi_para_read(const char *path, sfwi_para *out);
 /* An x-major FP32 model (allnx columns of allnz floats, sfwi-stage1/
  * main.cpp:1468-1473); a file of another size is an error. */

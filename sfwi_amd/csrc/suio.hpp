@@ -2,10 +2,11 @@
 // the 10-line positional parameter file, x-major FP32 model files and SU
 // (240-byte SEG-Y trace header + FP32 samples, no reel headers) shot records.
 //
-//   parameter file (sfwi-stage1/main.cpp:1346-1355; both stages use this
-//   10-line layout: sfwi-stage2/main.cpp:946-954 reads one path line fewer
-//   than its shipped para_stage2 holds, so the shared layout is the one
-//   both shipped files follow):
+//   parameter file (sfwi-stage1/main.cpp:1346-1355, read as a stream of
+//   whitespace-separated fields like its fscanf calls; both stages use this
+//   10-line layout: sfwi-stage2/main.cpp:946-954 reads one path fewer than its
+//   shipped para_stage2 holds, so the shared layout is the one both shipped
+//   files follow):
 //     dx dz pml allnx allnz minshot maxshot
 //     vp path / vs path / rho path / record-z path / record-x path / output dir
 //     f0 maxiter iter_round1
@@ -38,7 +39,7 @@ template <typename T> void su_put(unsigned char *h, int off, T v) { std::memcpy(
 
 int su_copy_path(char *dst, const std::string &src, const char *what) {
     if (src.size() >= SFWI_PATH_MAX)
-        return fail(nullptr, SFWI_EINVAL, "parameter file: %s path longer than %d characters", what, SFWI_PATH_MAX - 1);
+        return fail(nullptr, SFWI_EINVAL, "parameter file: %s longer than %d characters", what, SFWI_PATH_MAX - 1);
     std::memcpy(dst, src.c_str(), src.size() + 1);
     return 0;
 }
@@ -51,43 +52,35 @@ int sfwi_para_read(const char *path, sfwi_para *out) {
     if (!path || !out) return fail(nullptr, SFWI_EINVAL, "sfwi_para_read: NULL argument");
     std::ifstream f(path);
     if (!f) return fail(nullptr, SFWI_EINVAL, "cannot open parameter file %s", path);
-    std::vector<std::string> lines;
-    for (std::string l; std::getline(f, l);) {
-        if (l.find_first_not_of(" \t\r") == std::string::npos) continue;   // blank lines carry nothing
-        lines.push_back(l);
+    // a stream of whitespace-separated fields, as the drivers' fscanf reads it
+    std::vector<std::string> tok;
+    for (std::string t; f >> t;) tok.push_back(t);
+    static const char *names[22] = {"dx", "dz", "pml", "allnx", "allnz", "minshot", "maxshot", "vp path", "vs path",
+                                    "rho path", "record-z path", "record-x path", "output path", "f0", "maxiter",
+                                    "iter_round1", "sy", "gy", "light", "rbc", "cpu_mem", "iointerval"};
+    if (tok.size() < 22)
+        return fail(nullptr, SFWI_EINVAL, "parameter file %s: %zu fields, the layout has 22 (missing: %s)", path,
+                    tok.size(), names[tok.size()]);
+    double v[22];
+    for (int k = 0; k < 22; ++k) {
+        if (k >= 7 && k < 13) continue;
+        char *end = nullptr;
+        v[k] = std::strtod(tok[k].c_str(), &end);
+        if (!end || *end)
+            return fail(nullptr, SFWI_EINVAL, "parameter file %s: %s is '%s', not a number", path, names[k],
+                        tok[k].c_str());
     }
-    if (lines.size() < 10)
-        return fail(nullptr, SFWI_EINVAL, "parameter file %s: %zu non-blank lines, the layout has 10", path,
-                    lines.size());
     sfwi_para p;
     std::memset(&p, 0, sizeof(p));
-    auto nums = [&](int li, int want, double *v) {
-        std::istringstream s(lines[li]);
-        for (int k = 0; k < want; ++k)
-            if (!(s >> v[k]))
-                return fail(nullptr, SFWI_EINVAL, "parameter file %s line %d: expected %d numbers", path, li + 1,
-                            want);
-        return 0;
-    };
-    double a[7];
-    int r;
-    if ((r = nums(0, 7, a))) return r;
-    p.dx = (float)a[0]; p.dz = (float)a[1]; p.pml = (int)a[2]; p.allnx = (int)a[3]; p.allnz = (int)a[4];
-    p.minshot = (int)a[5]; p.maxshot = (int)a[6];
+    p.dx = (float)v[0]; p.dz = (float)v[1]; p.pml = (int)v[2]; p.allnx = (int)v[3]; p.allnz = (int)v[4];
+    p.minshot = (int)v[5]; p.maxshot = (int)v[6];
     char *paths[6] = {p.vp, p.vs, p.rho, p.rec_z, p.rec_x, p.outdir};
-    const char *names[6] = {"vp", "vs", "rho", "record-z", "record-x", "output"};
-    for (int k = 0; k < 6; ++k) {
-        std::istringstream s(lines[1 + k]);
-        std::string tok;
-        s >> tok;
-        if ((r = su_copy_path(paths[k], tok, names[k]))) return r;
-    }
-    if ((r = nums(7, 3, a))) return r;
-    p.f0 = (float)a[0]; p.maxiter = (int)a[1]; p.iter_round1 = (int)a[2];
-    if ((r = nums(8, 2, a))) return r;
-    p.sy = (int)a[0]; p.gy = (int)a[1];
-    if ((r = nums(9, 4, a))) return r;
-    p.light = (int)a[0]; p.rbc = (int)a[1]; p.cpu_mem = (int)a[2]; p.iointerval = (int)a[3];
+    int r;
+    for (int k = 0; k < 6; ++k)
+        if ((r = su_copy_path(paths[k], tok[7 + k], names[7 + k]))) return r;
+    p.f0 = (float)v[13]; p.maxiter = (int)v[14]; p.iter_round1 = (int)v[15];
+    p.sy = (int)v[16]; p.gy = (int)v[17];
+    p.light = (int)v[18]; p.rbc = (int)v[19]; p.cpu_mem = (int)v[20]; p.iointerval = (int)v[21];
     if (!(p.dx > 0) || !(p.dz > 0) || p.pml < 0 || p.allnx < 1 || p.allnz < 1)
         return fail(nullptr, SFWI_EINVAL, "parameter file %s: bad geometry dx=%g dz=%g pml=%d allnx=%d allnz=%d", path,
                     p.dx, p.dz, p.pml, p.allnx, p.allnz);

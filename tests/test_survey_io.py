@@ -34,14 +34,16 @@ def test_reference_parameter_files():
 def test_parameter_file_errors(tmp_path):
     lines = open(os.path.join(GOLDEN, "para_stage1_example1")).read().splitlines()
     f = tmp_path / "p"
-    f.write_text("\n".join(lines[:3] + lines[4:]) + "\n")          # one path line missing: 9 lines
-    with pytest.raises(SfwiError, match="10"):
+    f.write_text("\n".join(lines[:3] + lines[4:]) + "\n")          # one path line missing (stage-2 style)
+    with pytest.raises(SfwiError, match="21 fields.*iointerval"):
         S.read_para(f)
-    f.write_text("\n".join(["10 10 50 376"] + lines[1:]) + "\n")   # short first line
-    with pytest.raises(SfwiError, match="line 1"):
+    f.write_text("\n".join(["10 10 50 376 x 1 38"] + lines[1:]) + "\n")
+    with pytest.raises(SfwiError, match="allnz is 'x'"):
         S.read_para(f)
-    f.write_text("\n".join(lines[:1] + [""] + lines[1:]) + "\n")    # blank lines are skipped
+    f.write_text("\n".join(lines[:1] + [""] + lines[1:]) + "\n")    # blank lines and line breaks are whitespace
     assert S.read_para(f).allnx == 376
+    f.write_text(" ".join(lines) + "\n")
+    assert S.read_para(f).as_dict() == S.read_para(os.path.join(GOLDEN, "para_stage1_example1")).as_dict()
 
 
 def test_model_file(tmp_path):
