@@ -17,10 +17,11 @@
  * Pinned independently (tests/test_oracle_pins.py): coefficients, stencils,
  * window / medium / damping, Ricker, the propagators (energy, reversibility),
  * the adjoint half steps (dot-product), Born modeling (linearization), the
- * misfit and line-search sums.  Parity unpinned (restatement of the cited
- * lines only): the orchestration of the faithful gradient (strip restore,
- * injection, imaging formula) and the density gradient, which has no
- * reference counterpart.
+ * misfit and line-search sums, the overwrite injection with the last of
+ * coincident traces winning.  Parity unpinned (restatement of the cited
+ * lines only): the rest of the orchestration of the faithful gradient (strip
+ * restore, the reconstruction without source subtraction, the imaging
+ * formula) and the density gradient, which has no reference counterpart.
  */
 #ifndef SFWI_ORACLE_H
 #define SFWI_ORACLE_H

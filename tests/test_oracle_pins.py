@@ -19,6 +19,8 @@ operands) would break:
     (central differences converge at second order to the Born traces).
   * Composite solves: misfit, line-search misfits and the linear step sums
     recomputed from independent forward solves.
+  * Adjoint injection: the residual overwrites the adjoint velocity and the
+    last of two coincident traces wins (neither additive nor first-wins).
 """
 import numpy as np
 import pytest
@@ -256,3 +258,30 @@ def test_linear_step_sums_compose_from_born_solves():
     d = lambda a, b: float((a.astype(np.float64) * b.astype(np.float64)).sum())
     assert fz == pytest.approx(d(lz, rz) + d(lx, rx), rel=1e-6)
     assert fm == pytest.approx(d(lz, lz) + d(lx, lx), rel=1e-6)
+
+
+def test_adjoint_injection_overwrites_and_the_last_coincident_trace_wins():
+    """The residual overwrites the adjoint velocity at each receiver cell
+    (sfwi-stage1/FD2DGPU.cu:3425-3441): two traces on one cell leave only the
+    later one's residual, which an additive injection (r1 + r2) or a
+    first-wins order would not reproduce."""
+    sv = W.tiny(nt=160, seed=23)
+    base = sv.shots[0]
+    g0, g1 = int(base.gc[10]), int(base.gc[30])
+    r = np.random.default_rng(4)
+    res = [W.gaussian_smooth(r.standard_normal((1, sv.nt)).astype(np.float32), 3.0)[0] for _ in range(3)]
+    w = W.ricker(sv.nt, sv.dt, sv.f0)
+
+    def grad(gc, rows):
+        sh = W.Shot(sx=base.sx, gc=np.array(gc, np.int32), ngx_left=base.ngx_left, left=base.left)
+        c = oracle_cfg(sv, sh)
+        rz = np.ascontiguousarray(np.stack(rows), np.float32)
+        return O.gradient_res(c, sv.vp0, sv.vs0, sv.rho0, w, sh.gc, rz, rz * np.float32(0.5))
+
+    dup = grad([g0, g1, g1], [res[0], res[1], res[2]])
+    last = grad([g0, g1], [res[0], res[2]])
+    first = grad([g0, g1], [res[0], res[1]])
+    summed = grad([g0, g1], [res[0], res[1] + res[2]])
+    for a, b in zip(dup, last):
+        assert np.array_equal(a, b)
+    assert rel_l2(dup[0], first[0]) > 1e-2 and rel_l2(dup[0], summed[0]) > 1e-2
