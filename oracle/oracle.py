@@ -63,6 +63,8 @@ def lib():
                                           _f64p, _f64p, _f64p, _f64p, dp]
         L.or_gradient_res_f32.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _i32p, _f32p, _f32p,
                                           _f32p, _f32p, _f32p]
+        L.or_gradient_consistent_f64.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _i32p, _f32p, _f32p,
+                                                 _f64p, _f64p, _f64p, dp]
         L.or_born_forward_f32.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _i32p,
                                           _f32p, _f32p, _f32p, _f32p]
         L.or_born_forward_f64.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _i32p,
@@ -158,6 +160,18 @@ def gradient_rho(c, vp, vs, rho, wav, gc, obs_z, obs_x, prec="f32"):
                                               *g, ctypes.byref(m))
     sh = (c.allnx, c.allnz)
     return g[0].reshape(sh), g[1].reshape(sh), g[2].reshape(sh), g[3].reshape(sh), m.value
+
+
+def gradient_consistent(c, vp, vs, rho, wav, gc, obs_z, obs_x):
+    """Exact discrete gradient of sum r^2 (FP64, use with pml_off=1): (grad_vp, grad_vs, grad_rho, misfit)."""
+    n = c.allnx * c.allnz
+    g = [np.zeros(n, np.float64) for _ in range(3)]
+    m = ctypes.c_double(0.0)
+    lib().or_gradient_consistent_f64(ctypes.byref(c), _f32(vp), _f32(vs), _f32(rho), _f32(wav),
+                                     np.ascontiguousarray(gc, np.int32), _f32(obs_z), _f32(obs_x), *g,
+                                     ctypes.byref(m))
+    sh = (c.allnx, c.allnz)
+    return g[0].reshape(sh), g[1].reshape(sh), g[2].reshape(sh), m.value
 
 
 def gradient_res(c, vp, vs, rho, wav, gc, res_z, res_x):

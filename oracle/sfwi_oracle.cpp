@@ -698,6 +698,86 @@ static void forward_impl(const or_config *c, const float *vp, const float *vs, c
     std::memcpy(syn_x, sx.data(), nr * sizeof(T));
 }
 
+// Consistent discrete gradient (test device, no reference counterpart): the
+// exact gradient of the L2 misfit sum_(traces, steps) r^2 with respect to the
+// window's vp, vs and rho for the same forward scheme, built to pin the imaging
+// formula of image_step by finite differences.  It differs from the faithful
+// gradient (gradient_impl) only in the orchestration: the forward velocities
+// and derivatives are kept instead of reconstructed, the residual is added
+// (2 r / rho, the transpose of receiver sampling under a = rho v_adj) instead
+// of overwriting the adjoint velocity, and each step images between the
+// adjoint velocity-like and stress-like half steps (the adjoint of the stress
+// update pairs with the stresses it produced); the result is scaled by dt.
+// With the PML off the adjoint half steps are exact transposes (dot-product
+// pins), so these images must match central differences of the misfit.
+template <typename T>
+static void gradient_consistent_impl(const or_config *c, const float *vp, const float *vs, const float *rho,
+                                     const float *wav, const int *gc, const float *obs_z, const float *obs_x,
+                                     T *gvp, T *gvs, T *grho, double *misfit) {
+    Run<T> r(c, vp, vs, rho, nullptr, nullptr, gc);
+    const Grid &g = r.g;
+    const long n = g.n, nr = (long)c->ntr * c->nt;
+    std::vector<T> sz(nr), sx(nr);
+    std::vector<std::vector<T>> keep(c->nt);   // per step: vx, vz before and after the velocity update, 4 derivatives
+    Wave<T> w(n);
+    for (int it = 0; it < c->nt; ++it) {
+        r.add_source(w, wav, it);
+        std::vector<T> &k = keep[it];
+        k.resize(8 * n);
+        std::copy(w.vx.begin(), w.vx.end(), k.begin());
+        std::copy(w.vz.begin(), w.vz.end(), k.begin() + n);
+        update_v(g, r.st, r.m, w, +1, r.nth);
+        std::copy(w.vx.begin(), w.vx.end(), k.begin() + 2 * n);
+        std::copy(w.vz.begin(), w.vz.end(), k.begin() + 3 * n);
+        update_s(g, r.st, r.m, w, +1, r.nth, k.data() + 4 * n, k.data() + 5 * n, k.data() + 6 * n, k.data() + 7 * n);
+        r.record(w, it, sz, sx);
+    }
+    for (long q = 0; q < nr; ++q) {
+        sz[q] = sz[q] - (T)obs_z[q];
+        sx[q] = sx[q] - (T)obs_x[q];
+    }
+    *misfit += l2_sum(sz, sx, c->ntr, c->nt);
+    Wave<T> bk(n), fw(n);
+    std::vector<T> ovx(n), ovz(n), d[4], ill((size_t)c->allnx * c->allnz);
+    for (int f = 0; f < 4; ++f) d[f].resize(n);
+    const size_t nm = (size_t)c->allnx * c->allnz;
+    std::vector<T> ivp(nm, T(0)), ivs(nm, T(0)), irho(nm, T(0));
+    for (int it = c->nt - 1; it >= 0; --it) {
+        for (int tr = 0; tr < g.ntr; ++tr) {   // the split parts keep summing to the totals
+            const long i = r.at(r.igx[tr], c->igz);
+            const T az = 2 * sz[(long)tr * c->nt + it] / r.m.rho[i];
+            const T ax = 2 * sx[(long)tr * c->nt + it] / r.m.rho[i];
+            bk.vz[i] += az;
+            bk.vz_x[i] += az;
+            bk.vx[i] += ax;
+            bk.vx_x[i] += ax;
+        }
+        adjoint_v(g, r.st, r.m, bk, +1, r.nth);
+        const std::vector<T> &k = keep[it];
+        std::copy(k.begin(), k.begin() + n, fw.vx.begin());
+        std::copy(k.begin() + n, k.begin() + 2 * n, fw.vz.begin());
+        std::copy(k.begin() + 2 * n, k.begin() + 3 * n, ovx.begin());
+        std::copy(k.begin() + 3 * n, k.begin() + 4 * n, ovz.begin());
+        for (int f = 0; f < 4; ++f) std::copy(k.begin() + (4 + f) * n, k.begin() + (5 + f) * n, d[f].begin());
+        image_step(c, g, r.m, fw, bk, d[0], d[1], d[2], d[3], ivp.data(), ivs.data(), ill.data(), r.nth,
+                   irho.data(), &ovx, &ovz, r.st.dt);
+        adjoint_s(g, r.st, r.m, bk, +1, r.nth);
+    }
+    for (size_t q = 0; q < nm; ++q) {
+        gvp[q] = r.st.dt * ivp[q];
+        gvs[q] = r.st.dt * ivs[q];
+        grho[q] = r.st.dt * irho[q];
+    }
+}
+
+extern "C" {
+void or_gradient_consistent_f64(const or_config *c, const float *vp, const float *vs, const float *rho,
+                                const float *wav, const int *gc, const float *obs_z, const float *obs_x,
+                                double *gvp, double *gvs, double *grho, double *misfit) {
+    gradient_consistent_impl<double>(c, vp, vs, rho, wav, gc, obs_z, obs_x, gvp, gvs, grho, misfit);
+}
+}  // extern "C"
+
 extern "C" {
 void or_forward_f32(const or_config *c, const float *vp, const float *vs, const float *rho,
                     const float *wav, const int *gc, float *syn_z, float *syn_x) {

@@ -18,10 +18,12 @@
  * window / medium / damping, Ricker, the propagators (energy, reversibility),
  * the adjoint half steps (dot-product), Born modeling (linearization), the
  * misfit and line-search sums, the overwrite injection with the last of
- * coincident traces winning.  Parity unpinned (restatement of the cited
+ * coincident traces winning, and the imaging step (Vp, Vs and density terms)
+ * through or_gradient_consistent_f64, whose images must equal central
+ * differences of the misfit.  Parity unpinned (restatement of the cited
  * lines only): the rest of the orchestration of the faithful gradient (strip
- * restore, the reconstruction without source subtraction, the imaging
- * formula) and the density gradient, which has no reference counterpart.
+ * restore with corners skipped, the reconstruction without source
+ * subtraction, imaging after the adjoint stress-like half step).
  */
 #ifndef SFWI_ORACLE_H
 #define SFWI_ORACLE_H
@@ -87,6 +89,14 @@ void or_gradient_rho_f64(const or_config *c, const float *vp, const float *vs, c
 /* Reverse loop (reconstruction, adjoint, imaging) driven by a caller-supplied
  * adjoint source res_* [ntr][nt] instead of the L2 residual: the gradient of
  * a misfit whose adjoint source is res_* (OT misfit).  Stage set by c. */
+/* Consistent discrete gradient of the L2 misfit (FP64; a test device with no
+ * reference counterpart): kept forward fields, additive 2 r / rho injection,
+ * imaging between the adjoint half steps, images scaled by dt.  With
+ * pml_off = 1 it is the exact gradient of the same scheme, so central
+ * differences of the misfit pin the imaging formula (Vp, Vs and density). */
+void or_gradient_consistent_f64(const or_config *c, const float *vp, const float *vs, const float *rho,
+                                const float *wav, const int *gc, const float *obs_z, const float *obs_x,
+                                double *gvp, double *gvs, double *grho, double *misfit);
 void or_gradient_res_f32(const or_config *c, const float *vp, const float *vs, const float *rho,
                          const float *wav, const int *gc, const float *res_z, const float *res_x,
                          float *grad_vp, float *grad_vs, float *illum);
