@@ -63,6 +63,10 @@ def alg_bytes(kernel, n_int, n_pml):
         "k_adj_str": (32, 72),             # R 2 v + 3 s, W 3 s (+4 splits RW, 2 damp)
         "k_adj_vel_image": (96, 80),       # R 3 s, lam mu rho, 2 v, W 2 v; imaging: R 4 derivs,
                                            # 2 coeffs, 2 recon v, RW 3 images (interior only)
+        # column-march reverse step without the density image
+        "k_str(reconstruct+image)": (76, 96),   # forward stress + R 3 adjoint s, 2 coeffs, RW 2 images
+        "k_vel(reconstruct+illum)": (40, 72),   # + RW illumination (interior only)
+        "k_adj_vel": (40, 80),                  # R 3 s, lam mu rho, 2 v, W 2 v (+4 splits RW, 2 damp)
     }[kernel]
     return per[0] * n_int + per[1] * n_pml
 
@@ -198,7 +202,8 @@ def main():
         d = kernels[dom]
         measured = (args.workload == "shot_batch" and args.mode == "fast" and args.march_cols in (0, 8)
                     and args.march_depth in (0, 4) and args.march_order in (0, 3))
-        traffic = round(MEASURED_HBM_BYTES_PER_SHOT[dom] * nsh) if measured else None
+        traffic = round(MEASURED_HBM_BYTES_PER_SHOT[dom] * nsh) if measured and dom in MEASURED_HBM_BYTES_PER_SHOT \
+            else None
         roofline = {"kernel": dom, "bound": "hbm", "achieved": d["GBps"], "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(d["GBps"] / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "alg_bytes_per_launch": d["alg_bytes"], "avg_launch_ms": d["avg_ms"]}

@@ -338,7 +338,12 @@ int sfwi_s2_update(int allnx, int allnz, double fenzi, double fenmu, const float
 #define SFWI_K_REC_VEL 3      /* reconstruction velocity step                  */
 #define SFWI_K_ADJ_STR 4      /* adjoint stress-like step                      */
 #define SFWI_K_ADJ_VEL 5      /* adjoint velocity step fused with imaging      */
-#define SFWI_K_COUNT 6
+/* The column-march kernels without the density image image elsewhere in the
+ * reverse step (2, 3 and 5 then do not run): */
+#define SFWI_K_REC_STR_IMG 6  /* reconstruction stress step + Vp / Vs imaging  */
+#define SFWI_K_REC_VEL_ILL 7  /* reconstruction velocity step + illumination   */
+#define SFWI_K_ADJ_VEL_ONLY 8 /* adjoint velocity step                         */
+#define SFWI_K_COUNT 9
 int sfwi_set_timing(sfwi_ctx *ctx, int sample_every);
 int sfwi_kernel_time(sfwi_ctx *ctx, int kernel_id, double *avg_ms, int *launches);
 const char *sfwi_kernel_name(int kernel_id);

@@ -23,6 +23,12 @@
 namespace sfwi {
 
 constexpr int CM_ROWS = 56;   // output rows per wave (64 lanes minus 2 x 4 halo)
+// Prefetch depths of the imaging kernel (each at most the march depth): its
+// operand ring, and the ring of its imaging operands.
+constexpr int ADJV_DEPTH = 4;
+constexpr int IMG_DEPTH = 1;
+constexpr int adjv_depth(int pd) { return pd < ADJV_DEPTH ? pd : ADJV_DEPTH; }
+constexpr int img_depth(int pd) { return adjv_depth(pd) < IMG_DEPTH ? adjv_depth(pd) : IMG_DEPTH; }
 
 struct Seg {
     int x0, x1, zb;   // columns [x0, x1), first output row zb
@@ -161,6 +167,25 @@ __device__ inline void adj_str_load_w(const Medium &m, const Fields &F, int i, b
     }
 }
 
+// Image index of the lane's cell in column c (its offset in the per-shot
+// image of the interior), or -1 outside the interior or on a halo lane.
+__device__ inline int img_index(const Geo &g, const Lane &l, int c) {
+    const int jx = c - g.pml, jz = l.z - g.pml;
+    return (l.act && jx >= 0 && jx < g.nx && jz >= 0 && jz < g.nz) ? jx * g.nz + jz : -1;
+}
+
+// Vp / Vs images of one cell (the arithmetic of image_fast): velocity
+// derivatives gx = Dx- vx / dx, gz = Dz- vz / dz, gxz = Dz+ vx / dz,
+// gzx = Dx+ vz / dx, adjoint stresses, the coefficients 2 vp rho and 2 vs rho,
+// and the images so far; returns the updated images in op, os.
+__device__ inline void image_vpvs(float gx, float gz, float gxz, float gzx, float sxx, float szz, float sxz,
+                                  float cp, float cs, float ip, float is, float &op, float &os) {
+    const float lg = -((gx + gz) * (sxx + szz));
+    const float mg = __builtin_fmaf(-(gxz + gzx), sxz, -2.0f * __builtin_fmaf(gx, sxx, gz * szz));
+    op = __builtin_fmaf(cp, lg, ip);
+    os = __builtin_fmaf(cs, __builtin_fmaf(-2.0f, lg, mg), is);
+}
+
 // Operands of the zero-lag imaging of one cell (loaded a column ahead, only
 // when some lane of the wave images).  sxx and szz are the adjoint stresses
 // the velocity update loads anyway; the caller fills them from its ring.
@@ -186,17 +211,19 @@ __device__ inline void img_load_rho(const DevShot &d, int i, int q, bool any, Im
 }
 // The arithmetic of image_fast on prefetched operands.
 __device__ inline void img_store(const DevShot &d, int q, const ImgCell &c) {
-    const float lg = -((c.gx + c.gz) * (c.sxx + c.szz));
-    const float mg = __builtin_fmaf(-(c.gxz + c.gzx), c.sxz, -2.0f * __builtin_fmaf(c.gx, c.sxx, c.gz * c.szz));
-    d.gvp[q] = __builtin_fmaf(c.cp, lg, c.ip);
-    d.gvs[q] = __builtin_fmaf(c.cs, __builtin_fmaf(-2.0f, lg, mg), c.is);
+    float op, os;
+    image_vpvs(c.gx, c.gz, c.gxz, c.gzx, c.sxx, c.szz, c.sxz, c.cp, c.cs, c.ip, c.is, op, os);
+    d.gvp[q] = op;
+    d.gvs[q] = os;
     d.gil[q] = __builtin_fmaf(c.vx, c.vx, __builtin_fmaf(c.vz, c.vz, c.il));
 }
 
 // Velocity half step (SAVE: store strips of the new velocity; BORN: also the
-// scattered field).  PD columns of operands are in flight ahead of the one
-// being computed (a ring indexed at compile time: the march is unrolled by PD).
-template <bool SAVE, bool BORN, int PD>
+// scattered field; ILL: reconstruction, also accumulate the illumination
+// vx^2 + vz^2 of the new velocity).  PD columns of operands are in flight
+// ahead of the one being computed (a ring indexed at compile time: the march
+// is unrolled by PD).
+template <bool SAVE, bool BORN, int PD, bool ILL = false>
 __global__ void __launch_bounds__(BLK) km_vel(Geo g, const DevShot *__restrict__ shots, int it, int dir) {
     Seg s;
     if (!seg_of(g, s)) return;
@@ -214,6 +241,12 @@ __global__ void __launch_bounds__(BLK) km_vel(Geo g, const DevShot *__restrict__
         const bool sp = in_band(g, c, l.z);
         vel_load_w(d.m, F, c * g.nzp + l.zc, l.act, sp, qst[f][j]);
     };
+    constexpr int IPD = img_depth(PD);
+    float qil[IPD] = {};
+    auto ifetch = [&](int j, int c) {   // illumination so far, IPD columns ahead
+        const int q = img_index(g, l, c);
+        if (any_lane(q >= 0)) qil[j] = d.gil[q < 0 ? 0 : q];
+    };
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
         const Fields &F = f ? d.bn : d.fw;
@@ -226,6 +259,11 @@ __global__ void __launch_bounds__(BLK) km_vel(Geo g, const DevShot *__restrict__
         for (int j = 0; j < PD; ++j)
             if (s.x0 + j < s.x1) fetch(f, j, s.x0 + j);
     }
+    if (ILL) {
+#pragma unroll
+        for (int j = 0; j < IPD; ++j)
+            if (s.x0 + j < s.x1) ifetch(j, s.x0 + j);
+    }
     for (int c0 = s.x0; c0 < s.x1; c0 += PD) {
 #pragma unroll
         for (int j = 0; j < PD; ++j) {
@@ -233,6 +271,11 @@ __global__ void __launch_bounds__(BLK) km_vel(Geo g, const DevShot *__restrict__
             if (c >= s.x1) break;
             const bool split = in_band(g, c, l.z);
             const int i = c * g.nzp + l.z;
+            float il = 0.0f;
+            if (ILL) {
+                il = qil[j % IPD];
+                if (c + IPD < s.x1) ifetch(j % IPD, c + IPD);
+            }
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
                 const Fields &F = f ? d.bn : d.fw;
@@ -254,6 +297,10 @@ __global__ void __launch_bounds__(BLK) km_vel(Geo g, const DevShot *__restrict__
                     }
                     vel_math(g, st, split, dir, a, bz, cz, e);
                     vel_store(F, i, st, split);
+                    if (ILL && f == 0) {
+                        const int q = img_index(g, l, c);
+                        if (q >= 0) d.gil[q] = __builtin_fmaf(st.vx, st.vx, __builtin_fmaf(st.vz, st.vz, il));
+                    }
                     if (SAVE && f == 0 && split) {
                         int sl[4];
                         strip_slots(g, c, l.z, sl);
@@ -272,8 +319,11 @@ __global__ void __launch_bounds__(BLK) km_vel(Geo g, const DevShot *__restrict__
 }
 
 // Stress half step (GRAD: store the velocity derivatives; BORN: also the
-// scattered stresses with the Born source of the background velocity).
-template <bool GRAD, bool BORN, int PD>
+// scattered stresses with the Born source of the background velocity; IMG:
+// reconstruction, also form the Vp / Vs images of the step from the velocity
+// derivatives, which stay in registers, and the adjoint stresses that the
+// adjoint stress-like step of the same time step has produced).
+template <bool GRAD, bool BORN, int PD, bool IMG = false>
 __global__ void __launch_bounds__(BLK) km_str(Geo g, const DevShot *__restrict__ shots, int dir) {
     Seg s;
     if (!seg_of(g, s)) return;
@@ -294,6 +344,24 @@ __global__ void __launch_bounds__(BLK) km_str(Geo g, const DevShot *__restrict__
         const bool sp = split_of(f, c);
         str_load_w(d.m, F, c * g.nzp + l.zc, l.act, sp, qst[f][j]);
     };
+    struct SImg {
+        float sxx, szz, sxz, cp, cs, ip, is;
+    };
+    constexpr int IPD = img_depth(PD);
+    SImg qim[IPD] = {};
+    auto ifetch = [&](int j, int c) {   // imaging operands, IPD columns ahead
+        const int q = img_index(g, l, c);
+        if (any_lane(q >= 0)) {
+            const int i = c * g.nzp + l.zc, r = q < 0 ? 0 : q;
+            qim[j].sxx = d.bk.txx[i];
+            qim[j].szz = d.bk.tzz[i];
+            qim[j].sxz = d.bk.txz[i];
+            qim[j].cp = d.m.icp[i];
+            qim[j].cs = d.m.ics[i];
+            qim[j].ip = d.gvp[r];
+            qim[j].is = d.gvs[r];
+        }
+    };
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
         const Fields &F = f ? d.bn : d.fw;
@@ -306,6 +374,11 @@ __global__ void __launch_bounds__(BLK) km_str(Geo g, const DevShot *__restrict__
         for (int j = 0; j < PD; ++j)
             if (s.x0 + j < s.x1) fetch(f, j, s.x0 + j);
     }
+    if (IMG) {
+#pragma unroll
+        for (int j = 0; j < IPD; ++j)
+            if (s.x0 + j < s.x1) ifetch(j, s.x0 + j);
+    }
     const BornSrc none{};
     for (int c0 = s.x0; c0 < s.x1; c0 += PD) {
 #pragma unroll
@@ -313,6 +386,11 @@ __global__ void __launch_bounds__(BLK) km_str(Geo g, const DevShot *__restrict__
             const int c = c0 + j;
             if (c >= s.x1) break;
             const int i = c * g.nzp + l.z;
+            SImg im{};
+            if (IMG) {
+                im = qim[j % IPD];
+                if (c + IPD < s.x1) ifetch(j % IPD, c + IPD);
+            }
             VGrad v[NF];
             StrCell st[NF];
 #pragma unroll
@@ -337,6 +415,16 @@ __global__ void __launch_bounds__(BLK) km_str(Geo g, const DevShot *__restrict__
                     d.dgz[i] = v[0].zz * g.idz;
                     d.dgxz[i] = v[0].xz * g.idz;
                     d.dgzx[i] = v[0].zx * g.idx;
+                }
+                if (IMG) {
+                    const int q = img_index(g, l, c);
+                    if (q >= 0) {
+                        float op, os;
+                        image_vpvs(v[0].xx * g.idx, v[0].zz * g.idz, v[0].xz * g.idz, v[0].zx * g.idx, im.sxx,
+                                   im.szz, im.sxz, im.cp, im.cs, im.ip, im.is, op, os);
+                        d.gvp[q] = op;
+                        d.gvs[q] = os;
+                    }
                 }
                 if (BORN) {
                     const bool band = split_of(1, c);
@@ -399,10 +487,13 @@ __global__ void __launch_bounds__(BLK) km_adj_str(Geo g, const DevShot *__restri
     }
 }
 
-// Adjoint velocity step with the moduli inside the derivatives, fused with
-// the zero-lag imaging.
-template <int PD>
+// Adjoint velocity step with the moduli inside the derivatives; IMAGE: fused
+// with the zero-lag imaging from the stored velocity derivatives (RHO: also
+// the density image).  The imaging operands have a prefetch ring of their
+// own, IPD columns deep (IPD divides PD).
+template <int PD, bool IMAGE, bool RHO, int IPD>
 __global__ void __launch_bounds__(BLK) km_adj_vel_image(Geo g, const DevShot *__restrict__ shots, int dir) {
+    static_assert(IPD >= 1 && PD % IPD == 0, "the imaging ring depth divides the operand ring depth");
     Seg s;
     if (!seg_of(g, s)) return;
     const DevShot d = shots[s.shot];   // by value: the field pointers stay in registers (stores cannot alias them)
@@ -444,27 +535,22 @@ __global__ void __launch_bounds__(BLK) km_adj_vel_image(Geo g, const DevShot *__
     };
     auto P1 = [&](int x) { return Pv(PL(x, F.txx, F.tzz)); };
     auto P2 = [&](int x) { return PMv(PM(x)); };
-    // image index of the lane's cell in column c, or -1 outside the interior
-    auto img_q = [&](int c) {
-        const int jx = c - g.pml, jz = l.z - g.pml;
-        return (l.act && jx >= 0 && jx < g.nx && jz >= 0 && jz < g.nz) ? jx * g.nz + jz : -1;
-    };
     auto split_of = [&](int c) { return in_band(g, c, l.z) || l.z == d.igz; };
     float w1[8], w2[8];
     PCell q1[PD], q2[PD], q3[PD];
     VelCell qst[PD] = {};
-    ImgCell qim[PD] = {};
+    ImgCell qim[IPD] = {};
     auto fetch = [&](int j, int c) {
         q1[j] = PL(c + 4, F.txx, F.tzz);
         q2[j] = PM(c + 3);
         q3[j] = PL(c, F.tzz, F.txx);
-        {
-            const bool sp = split_of(c);
-            vel_load_w(m, F, c * g.nzp + l.zc, l.act, sp, qst[j]);
-        }
-        const int q = img_q(c);
+        const bool sp = split_of(c);
+        vel_load_w(m, F, c * g.nzp + l.zc, l.act, sp, qst[j]);
+    };
+    auto ifetch = [&](int j, int c) {
+        const int q = img_index(g, l, c);
         img_load(d, c * g.nzp + l.zc, q < 0 ? 0 : q, any_lane(q >= 0), qim[j]);
-        if (g.grad_rho) img_load_rho(d, c * g.nzp + l.zc, q < 0 ? 0 : q, any_lane(q >= 0), qim[j]);
+        if (RHO) img_load_rho(d, c * g.nzp + l.zc, q < 0 ? 0 : q, any_lane(q >= 0), qim[j]);
     };
 #pragma unroll
     for (int j = 0; j < 7; ++j) {
@@ -474,6 +560,11 @@ __global__ void __launch_bounds__(BLK) km_adj_vel_image(Geo g, const DevShot *__
 #pragma unroll
     for (int j = 0; j < PD; ++j)
         if (s.x0 + j < s.x1) fetch(j, s.x0 + j);
+    if (IMAGE) {
+#pragma unroll
+        for (int j = 0; j < IPD; ++j)
+            if (s.x0 + j < s.x1) ifetch(j, s.x0 + j);
+    }
     for (int c0 = s.x0; c0 < s.x1; c0 += PD) {
 #pragma unroll
         for (int j = 0; j < PD; ++j) {
@@ -485,10 +576,14 @@ __global__ void __launch_bounds__(BLK) km_adj_vel_image(Geo g, const DevShot *__
             w2[7] = PMv(q2[j]);
             const float p3 = Pv(q3[j]);
             VelCell st = qst[j];
-            ImgCell im = qim[j];
-            im.sxx = q3[j].b;                    // txx, tzz at column c
-            im.szz = q3[j].a;
+            ImgCell im{};
+            if (IMAGE) {
+                im = qim[j % IPD];
+                im.sxx = q3[j].b;                // txx, tzz at column c
+                im.szz = q3[j].a;
+            }
             if (c + PD < s.x1) fetch(j, c + PD);
+            if (IMAGE && c + IPD < s.x1) ifetch(j % IPD, c + IPD);
             const float sx = x_win(w1);          // Dx+ p1
             const float sxz_x = x_win(w2);       // Dx- p2
             const float sz = z_plus(p3);         // Dz+ p3
@@ -498,10 +593,10 @@ __global__ void __launch_bounds__(BLK) km_adj_vel_image(Geo g, const DevShot *__
             if (l.act) {
                 vel_math(g, st, split, dir, sx, sxz_z, sz, sxz_x);
                 vel_store(F, i, st, split);
-                const int q = img_q(c);
+                const int q = IMAGE ? img_index(g, l, c) : -1;
                 if (q >= 0) {
                     img_store(d, q, im);
-                    if (g.grad_rho) {
+                    if (RHO) {
                         const float lg = -((im.gx + im.gz) * (im.sxx + im.szz));
                         const float mg = __builtin_fmaf(-(im.gxz + im.gzx), im.sxz,
                                                         -2.0f * __builtin_fmaf(im.gx, im.sxx, im.gz * im.szz));

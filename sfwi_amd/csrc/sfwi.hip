@@ -522,7 +522,31 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
 
     const int adir = c->p.adjoint_dir;
     const dim3 stripg((unsigned)((g.sf + 255) / 256), (unsigned)n);
-    for (int it = nt - 1; it >= 0; --it) {
+    // Column-march reverse step without the density image: the adjoint
+    // stress-like step runs first, so the reconstruction's stress step can
+    // image from the velocity derivatives in its registers and the reverse
+    // velocity step can add the illumination; the adjoint velocity step is then
+    // imaging-free.  The reconstruction and the adjoint fields are independent
+    // within a step, so the order changes no value, and the images are the
+    // same arithmetic on the same operands (identical results).
+    const bool march_img = !percell && !g.grad_rho;
+    for (int it = nt - 1; it >= 0 && march_img; --it) {
+        hipLaunchKernelGGL(k_strip_load, stripg, blk, 0, st, g, D, it);
+        hipLaunchKernelGGL(k_inject, trg, dim3(64), 0, st, g, D, it);
+        int t = tbeg(c, SFWI_K_ADJ_STR, it);
+        hipLaunchKernelGGL(km_adj_str<PD>, tiles, blk, 0, st, gm, D, adir);
+        tend(c, t);
+        t = tbeg(c, SFWI_K_REC_STR_IMG, it);
+        hipLaunchKernelGGL((km_str<false, false, PD, true>), tiles, blk, 0, st, gm, D, -1);
+        tend(c, t);
+        t = tbeg(c, SFWI_K_REC_VEL_ILL, it);
+        hipLaunchKernelGGL((km_vel<false, false, PD, true>), tiles, blk, 0, st, gm, D, it, -1);
+        tend(c, t);
+        t = tbeg(c, SFWI_K_ADJ_VEL_ONLY, it);
+        hipLaunchKernelGGL((km_adj_vel_image<adjv_depth(PD), false, false, 1>), tiles, blk, 0, st, gm, D, adir);
+        tend(c, t);
+    }
+    for (int it = nt - 1; it >= 0 && !march_img; --it) {
         hipLaunchKernelGGL(k_strip_load, stripg, blk, 0, st, g, D, it);
         int t = tbeg(c, SFWI_K_REC_STR, it);
         if (percell) hipLaunchKernelGGL((k_str<E, true, false>), cells, blk, 0, st, g, D, -1);
@@ -539,7 +563,12 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
         tend(c, t);
         t = tbeg(c, SFWI_K_ADJ_VEL, it);
         if (percell) hipLaunchKernelGGL(k_adj_vel_image<E>, cells, blk, 0, st, g, D, adir);
-        else hipLaunchKernelGGL(km_adj_vel_image<PD>, tiles, blk, 0, st, gm, D, adir);
+        else if (g.grad_rho)
+            hipLaunchKernelGGL((km_adj_vel_image<adjv_depth(PD), true, true, img_depth(PD)>), tiles, blk, 0, st, gm, D,
+                               adir);
+        else
+            hipLaunchKernelGGL((km_adj_vel_image<adjv_depth(PD), true, false, img_depth(PD)>), tiles, blk, 0, st, gm,
+                               D, adir);
         tend(c, t);
     }
     const dim3 mg((unsigned)((c->nint + 255) / 256));
@@ -999,8 +1028,9 @@ int sfwi_kernel_time(sfwi_ctx *c, int kid, double *avg_ms, int *launches) {
 }
 
 const char *sfwi_kernel_name(int kid) {
-    static const char *names[SFWI_K_COUNT] = {"k_vel(forward)", "k_str(forward)", "k_str(reconstruct)",
-                                              "k_vel(reconstruct)", "k_adj_str", "k_adj_vel_image"};
+    static const char *names[SFWI_K_COUNT] = {
+        "k_vel(forward)", "k_str(forward)", "k_str(reconstruct)", "k_vel(reconstruct)", "k_adj_str",
+        "k_adj_vel_image", "k_str(reconstruct+image)", "k_vel(reconstruct+illum)", "k_adj_vel"};
     return (kid >= 0 && kid < SFWI_K_COUNT) ? names[kid] : "?";
 }
 
