@@ -502,48 +502,35 @@ __global__ void __launch_bounds__(BLK) km_adj_vel_image(Geo g, const DevShot *__
     const Medium &m = d.m;
     // p1 = (lam+2mu) txx + lam tzz on columns c-3..c+4; p2 = mu txz on c-4..c+3;
     // p3 = (lam+2mu) tzz + lam txx at column c.
-    // the raw operands travel in the prefetch ring; products are formed when
-    // the column is consumed (a product at load time would wait for the load)
-    struct PCell {
-        float lam, mu, a, b;
+    // Each column's lam, mu, txx, tzz are loaded once, four columns ahead for
+    // p1, and kept: one column later their mu forms p2 (with txz, the only
+    // other stencil load), four columns later they form p3.  The raw operands
+    // travel in the prefetch ring; products are formed when the column is
+    // consumed (a product at load time would wait for the load).
+    struct Raw {
+        float lam, mu, xx, zz;
     };
-    auto PL = [&](int x, const fptr &a, const fptr &b) {   // adj_p1(lam, mu, a[x], b[x])
+    auto RL = [&](int x) {
         const int i = x * g.nzp + l.zc;
-        PCell p;
-        p.lam = m.lam[i];
-        p.mu = m.mu[i];
-        p.a = a[i];
-        p.b = b[i];
-        return p;
+        Raw r;
+        r.lam = m.lam[i];
+        r.mu = m.mu[i];
+        r.xx = F.txx[i];
+        r.zz = F.tzz[i];
+        return r;
     };
-    auto PM = [&](int x) {                                   // mu txz
-        const int i = x * g.nzp + l.zc;
-        PCell p;
-        p.lam = 0.0f;
-        p.mu = m.mu[i];
-        p.a = F.txz[i];
-        p.b = 0.0f;
-        return p;
-    };
-    auto Pv = [&](const PCell &p) {
-        const float v = adj_p1(p.lam, p.mu, p.a, p.b);
-        return l.zin ? v : 0.0f;
-    };
-    auto PMv = [&](const PCell &p) {
-        const float v = p.mu * p.a;
-        return l.zin ? v : 0.0f;
-    };
-    auto P1 = [&](int x) { return Pv(PL(x, F.txx, F.tzz)); };
-    auto P2 = [&](int x) { return PMv(PM(x)); };
+    auto P1v = [&](const Raw &r) { return l.zin ? adj_p1(r.lam, r.mu, r.xx, r.zz) : 0.0f; };
+    auto P3v = [&](const Raw &r) { return l.zin ? adj_p1(r.lam, r.mu, r.zz, r.xx) : 0.0f; };
+    auto P2v = [&](float mu, float xz) { return l.zin ? mu * xz : 0.0f; };
     auto split_of = [&](int c) { return in_band(g, c, l.z) || l.z == d.igz; };
     float w1[8], w2[8];
-    PCell q1[PD], q2[PD], q3[PD];
+    Raw qr[PD], hist[4];      // hist[k]: the operands of column c + k
+    float qxz[PD];
     VelCell qst[PD] = {};
     ImgCell qim[IPD] = {};
     auto fetch = [&](int j, int c) {
-        q1[j] = PL(c + 4, F.txx, F.tzz);
-        q2[j] = PM(c + 3);
-        q3[j] = PL(c, F.tzz, F.txx);
+        qr[j] = RL(c + 4);
+        qxz[j] = F.txz[(c + 3) * g.nzp + l.zc];
         const bool sp = split_of(c);
         vel_load_w(m, F, c * g.nzp + l.zc, l.act, sp, qst[j]);
     };
@@ -554,8 +541,11 @@ __global__ void __launch_bounds__(BLK) km_adj_vel_image(Geo g, const DevShot *__
     };
 #pragma unroll
     for (int j = 0; j < 7; ++j) {
-        w1[j] = P1(s.x0 - 3 + j);
-        w2[j] = P2(s.x0 - 4 + j);
+        const Raw r = RL(s.x0 - 3 + j);
+        w1[j] = P1v(r);
+        if (j >= 3) hist[j - 3] = r;
+        const int i = (s.x0 - 4 + j) * g.nzp + l.zc;
+        w2[j] = P2v(m.mu[i], F.txz[i]);
     }
 #pragma unroll
     for (int j = 0; j < PD; ++j)
@@ -572,15 +562,20 @@ __global__ void __launch_bounds__(BLK) km_adj_vel_image(Geo g, const DevShot *__
             if (c >= s.x1) break;
             const bool split = split_of(c);
             const int i = c * g.nzp + l.z;
-            w1[7] = Pv(q1[j]);
-            w2[7] = PMv(q2[j]);
-            const float p3 = Pv(q3[j]);
+            const Raw r4 = qr[j];                // column c + 4
+            const Raw r0 = hist[0];              // column c
+            w1[7] = P1v(r4);
+            w2[7] = P2v(hist[3].mu, qxz[j]);   // mu, txz at column c + 3
+            const float p3 = P3v(r0);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) hist[k] = hist[k + 1];
+            hist[3] = r4;
             VelCell st = qst[j];
             ImgCell im{};
             if (IMAGE) {
                 im = qim[j % IPD];
-                im.sxx = q3[j].b;                // txx, tzz at column c
-                im.szz = q3[j].a;
+                im.sxx = r0.xx;                  // txx, tzz at column c
+                im.szz = r0.zz;
             }
             if (c + PD < s.x1) fetch(j, c + PD);
             if (IMAGE && c + IPD < s.x1) ifetch(j % IPD, c + IPD);
