@@ -51,24 +51,50 @@ MEASURED_HBM_BYTES_PER_SHOT = {
 }
 
 
-def alg_bytes(kernel, n_int, n_pml):
+NOP, NSTRIP = 4, 4   # stencil half-width; strip half-width (split band = PML + NSTRIP)
+
+
+def cell_classes(nx, nz, pml):
+    """Cells of one shot that a stencil launch touches: (unsplit, split, imaged).
+    The update domain is the padded grid minus its outer NOP-cell ring (never
+    written); split cells are those of it within pml + NSTRIP of an edge; the
+    imaged cells are the nx x nz interior (some of them split)."""
+    nxp, nzp = nx + 2 * pml, nz + 2 * pml
+    upd = (nxp - 2 * NOP) * (nzp - 2 * NOP)
+    w = pml + NSTRIP
+    un = max(nxp - 2 * w, 0) * max(nzp - 2 * w, 0)
+    return un, upd - un, nx * nz
+
+
+def alg_bytes(kernel, nx, nz, pml):
     """Algorithmic bytes of one launch over one shot: every array the kernel
-    needs read once and every array it produces written once, FP32
-    (interior cells: totals only; PML cells: plus split parts and damping)."""
-    per = {  # (interior B/cell, PML B/cell)
-        "k_vel(forward)": (32, 72),        # R txx tzz txz vx vz rho, W vx vz (+4 splits RW, 2 damp)
-        "k_str(forward)": (40, 96),        # R vx vz txx tzz txz lam mu, W 3 stresses (+6 splits RW, 2 damp)
-        "k_str(reconstruct)": (56, 112),   # as forward + W 4 velocity derivatives
-        "k_vel(reconstruct)": (32, 72),
-        "k_adj_str": (32, 72),             # R 2 v + 3 s, W 3 s (+4 splits RW, 2 damp)
-        "k_adj_vel_image": (96, 80),       # R 3 s, lam mu rho, 2 v, W 2 v; imaging: R 4 derivs,
-                                           # 2 coeffs, 2 recon v, RW 3 images (interior only)
+    needs read once and every array it produces written once, FP32.  An
+    unsplit cell reads its own totals; a split cell reads and writes its split
+    parts and the damping, writes the totals, and does not read its own
+    totals; stencil inputs are totals everywhere; imaging terms count on the
+    imaged cells."""
+    per = {  # (unsplit B/cell, split B/cell, imaged B/cell)
+        # R txx tzz txz, 1/rho, vx vz | split: R 4 parts + 2 damping, W 4 parts; W vx vz
+        "k_vel(forward)": (32, 64, 0),
+        # R vx vz, lam mu, txx tzz txz | split: R 6 parts + 2 damping, W 6 parts; W 3 stresses
+        "k_str(forward)": (40, 84, 0),
+        "k_str(reconstruct)": (56, 100, 0),      # as forward + W 4 velocity derivatives
+        "k_vel(reconstruct)": (32, 64, 0),
+        # R vx vz, txx tzz txz | split: R 4 parts + 2 damping, W 4 parts; W 3 stresses
+        "k_adj_str": (32, 60, 0),
+        # R txx tzz txz, lam mu 1/rho, vx vz | split: R 4 parts + 2 damping, W 4 parts; W vx vz;
+        # imaging: R 4 derivatives, 2 coefficients, reconstructed vx vz, RW 3 images
+        "k_adj_vel_image": (40, 72, 56),
         # column-march reverse step without the density image
-        "k_str(reconstruct+image)": (76, 96),   # forward stress + R 3 adjoint s, 2 coeffs, RW 2 images
-        "k_vel(reconstruct+illum)": (40, 72),   # + RW illumination (interior only)
-        "k_adj_vel": (40, 80),                  # R 3 s, lam mu rho, 2 v, W 2 v (+4 splits RW, 2 damp)
+        "k_str(reconstruct+image)": (40, 84, 36),   # imaging: R 3 adjoint stresses, 2 coefficients, RW 2 images
+        "k_vel(reconstruct+illum)": (32, 64, 8),    # imaging: RW illumination
+        "k_adj_vel": (40, 72, 0),
     }[kernel]
-    return per[0] * n_int + per[1] * n_pml
+    un, sp, im = cell_classes(nx, nz, pml)
+    b = per[0] * un + per[1] * sp + per[2] * im
+    if kernel == "k_vel(forward)":
+        b += 2 * 4 * NSTRIP * (nx + nz) * 4        # W the boundary strips of vx, vz
+    return b
 
 
 def main():
@@ -129,9 +155,7 @@ def main():
     sv = W.BY_NAME[args.workload](**kw)
     w = W.ricker(sv.nt, sv.dt, sv.f0)
     geo = [W.shot_geometry(sv, s) for s in sv.shots]
-    n_int = sv.nx * sv.allnz
     n_cells = sv.cells()
-    n_pml = n_cells - n_int
     print("bench: %s, %d shots, grid %dx%d pml %d (%d cells), nt %d" % (
         sv.name, len(sv.shots), sv.nx, sv.allnz, sv.pml, n_cells, sv.nt), file=sys.stderr, flush=True)
 
@@ -189,7 +213,7 @@ def main():
         if n == 0:
             continue
         name = _lib.load().sfwi_kernel_name(kid).decode()
-        b = alg_bytes(name, n_int, n_pml) * nsh
+        b = alg_bytes(name, sv.nx, sv.allnz, sv.pml) * nsh
         kernels[name] = {"avg_ms": round(ms, 5), "samples": n, "alg_bytes": b,
                          "GBps": round(b / (ms * 1e-3) / 1e9, 1),
                          "share": 0.0}
