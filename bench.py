@@ -83,10 +83,10 @@ def alg_bytes(kernel, nx, nz, pml):
         # R vx vz, txx tzz txz | split: R 4 parts + 2 damping, W 4 parts; W 3 stresses
         "k_adj_str": (32, 60, 0),
         # R txx tzz txz, lam mu 1/rho, vx vz | split: R 4 parts + 2 damping, W 4 parts; W vx vz;
-        # imaging: R 4 derivatives, 2 coefficients, reconstructed vx vz, RW 3 images
-        "k_adj_vel_image": (40, 72, 56),
+        # imaging: R 4 derivatives, reconstructed vx vz, RW 3 images
+        "k_adj_vel_image": (40, 72, 48),
         # column-march reverse step without the density image
-        "k_str(reconstruct+image)": (40, 84, 36),   # imaging: R 3 adjoint stresses, 2 coefficients, RW 2 images
+        "k_str(reconstruct+image)": (40, 84, 28),   # imaging: R 3 adjoint stresses, RW 2 images
         "k_vel(reconstruct+illum)": (32, 64, 8),    # imaging: RW illumination
         "k_adj_vel": (40, 72, 0),
     }[kernel]

@@ -461,15 +461,20 @@ __device__ inline float adj_p1(float lam, float mu, float txx, float tzz) {
     return __builtin_fmaf(__builtin_fmaf(2.0f, mu, lam), txx, lam * tzz);
 }
 
-// Zero-lag imaging of one interior cell (q = its index in the per-shot image).
+// Zero-lag imaging of one interior cell (q = its index in the per-shot image),
+// FAST mode: the per-shot accumulators sum the time-independent factors'
+// cofactors, lg = -(Dvx + Dvz)(sxx + szz) for Vp and -2 lg + mg for Vs;
+// k_merge multiplies them by 2 vp rho and 2 vs rho once per shot (the
+// reference multiplies every step, sfwi-stage1/FD2DGPU.cu:3350-3382: the same
+// sum up to rounding).
 __device__ inline void image_fast(const DevShot &d, int i, int q) {
     const float gx = d.dgx[i], gz = d.dgz[i], gxz = d.dgxz[i], gzx = d.dgzx[i];
     const float sxx = d.bk.txx[i], szz = d.bk.tzz[i], sxz = d.bk.txz[i];
     const float vx = d.fw.vx[i], vz = d.fw.vz[i];
     const float lg = -((gx + gz) * (sxx + szz));
     const float mg = __builtin_fmaf(-(gxz + gzx), sxz, -2.0f * __builtin_fmaf(gx, sxx, gz * szz));
-    d.gvp[q] = __builtin_fmaf(d.m.icp[i], lg, d.gvp[q]);
-    d.gvs[q] = __builtin_fmaf(d.m.ics[i], __builtin_fmaf(-2.0f, lg, mg), d.gvs[q]);
+    d.gvp[q] = d.gvp[q] + lg;
+    d.gvs[q] = d.gvs[q] + __builtin_fmaf(-2.0f, lg, mg);
     d.gil[q] = __builtin_fmaf(vx, vx, __builtin_fmaf(vz, vz, d.gil[q]));
 }
 
@@ -897,17 +902,26 @@ __global__ void __launch_bounds__(BLK) k_adj_vel_image(Geo g, const DevShot *__r
 }
 
 // Per-shot accumulators added into the full-model images, shots in batch
-// order so the result does not depend on scheduling.
+// order so the result does not depend on scheduling.  SCALE (FAST mode): the
+// Vp and Vs accumulators hold cofactors, multiplied here by the imaging
+// coefficients 2 vp rho and 2 vs rho of the shot's window.
+template <bool SCALE>
 __global__ void k_merge(Geo g, const DevShot *__restrict__ shots, int nshots, float *gvp, float *gvs,
                         float *gil, float *grho) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= g.nx * g.nz) return;
     const int jx = q / g.nz, jz = q - jx * g.nz;
+    const int i = (jx + g.pml) * g.nzp + jz + g.pml;
     for (int s = 0; s < nshots; ++s) {
         const DevShot &d = shots[s];
         const size_t o = (size_t)(d.left + jx) * g.allnz + jz;
-        gvp[o] += d.gvp[q];
-        gvs[o] += d.gvs[q];
+        if (SCALE) {   // a rounded product, then the sum: a batch equals its shots flushed one at a time
+            gvp[o] += d.m.icp[i] * d.gvp[q];
+            gvs[o] += d.m.ics[i] * d.gvs[q];
+        } else {
+            gvp[o] += d.gvp[q];
+            gvs[o] += d.gvs[q];
+        }
         gil[o] += d.gil[q];
         if (g.grad_rho) grho[o] += d.grho[q];
     }

@@ -174,23 +174,23 @@ __device__ inline int img_index(const Geo &g, const Lane &l, int c) {
     return (l.act && jx >= 0 && jx < g.nx && jz >= 0 && jz < g.nz) ? jx * g.nz + jz : -1;
 }
 
-// Vp / Vs images of one cell (the arithmetic of image_fast): velocity
-// derivatives gx = Dx- vx / dx, gz = Dz- vz / dz, gxz = Dz+ vx / dz,
-// gzx = Dx+ vz / dx, adjoint stresses, the coefficients 2 vp rho and 2 vs rho,
-// and the images so far; returns the updated images in op, os.
+// Vp / Vs image cofactors of one cell (the arithmetic of image_fast; k_merge
+// applies the coefficients 2 vp rho, 2 vs rho): velocity derivatives
+// gx = Dx- vx / dx, gz = Dz- vz / dz, gxz = Dz+ vx / dz, gzx = Dx+ vz / dx,
+// adjoint stresses and the sums so far; returns the updated sums in op, os.
 __device__ inline void image_vpvs(float gx, float gz, float gxz, float gzx, float sxx, float szz, float sxz,
-                                  float cp, float cs, float ip, float is, float &op, float &os) {
+                                  float ip, float is, float &op, float &os) {
     const float lg = -((gx + gz) * (sxx + szz));
     const float mg = __builtin_fmaf(-(gxz + gzx), sxz, -2.0f * __builtin_fmaf(gx, sxx, gz * szz));
-    op = __builtin_fmaf(cp, lg, ip);
-    os = __builtin_fmaf(cs, __builtin_fmaf(-2.0f, lg, mg), is);
+    op = ip + lg;
+    os = is + __builtin_fmaf(-2.0f, lg, mg);
 }
 
 // Operands of the zero-lag imaging of one cell (loaded a column ahead, only
 // when some lane of the wave images).  sxx and szz are the adjoint stresses
 // the velocity update loads anyway; the caller fills them from its ring.
 struct ImgCell {
-    float gx, gz, gxz, gzx, sxx, szz, sxz, vx, vz, cp, cs, ip, is, il;
+    float gx, gz, gxz, gzx, sxx, szz, sxz, vx, vz, ip, is, il;
     float ox, oz, vp, vs, ir;   // density image (grad_rho): v(it+1), vp, vs, accumulator
 };
 __device__ inline void img_load(const DevShot &d, int i, int q, bool any, ImgCell &c) {
@@ -198,7 +198,6 @@ __device__ inline void img_load(const DevShot &d, int i, int q, bool any, ImgCel
         c.gx = d.dgx[i]; c.gz = d.dgz[i]; c.gxz = d.dgxz[i]; c.gzx = d.dgzx[i];
         c.sxz = d.bk.txz[i];
         c.vx = d.fw.vx[i]; c.vz = d.fw.vz[i];
-        c.cp = d.m.icp[i]; c.cs = d.m.ics[i];
         c.ip = d.gvp[q]; c.is = d.gvs[q]; c.il = d.gil[q];
     }
 }
@@ -212,7 +211,7 @@ __device__ inline void img_load_rho(const DevShot &d, int i, int q, bool any, Im
 // The arithmetic of image_fast on prefetched operands.
 __device__ inline void img_store(const DevShot &d, int q, const ImgCell &c) {
     float op, os;
-    image_vpvs(c.gx, c.gz, c.gxz, c.gzx, c.sxx, c.szz, c.sxz, c.cp, c.cs, c.ip, c.is, op, os);
+    image_vpvs(c.gx, c.gz, c.gxz, c.gzx, c.sxx, c.szz, c.sxz, c.ip, c.is, op, os);
     d.gvp[q] = op;
     d.gvs[q] = os;
     d.gil[q] = __builtin_fmaf(c.vx, c.vx, __builtin_fmaf(c.vz, c.vz, c.il));
@@ -345,7 +344,7 @@ __global__ void __launch_bounds__(BLK) km_str(Geo g, const DevShot *__restrict__
         str_load_w(d.m, F, c * g.nzp + l.zc, l.act, sp, qst[f][j]);
     };
     struct SImg {
-        float sxx, szz, sxz, cp, cs, ip, is;
+        float sxx, szz, sxz, ip, is;
     };
     constexpr int IPD = img_depth(PD);
     SImg qim[IPD] = {};
@@ -356,8 +355,6 @@ __global__ void __launch_bounds__(BLK) km_str(Geo g, const DevShot *__restrict__
             qim[j].sxx = d.bk.txx[i];
             qim[j].szz = d.bk.tzz[i];
             qim[j].sxz = d.bk.txz[i];
-            qim[j].cp = d.m.icp[i];
-            qim[j].cs = d.m.ics[i];
             qim[j].ip = d.gvp[r];
             qim[j].is = d.gvs[r];
         }
@@ -421,7 +418,7 @@ __global__ void __launch_bounds__(BLK) km_str(Geo g, const DevShot *__restrict__
                     if (q >= 0) {
                         float op, os;
                         image_vpvs(v[0].xx * g.idx, v[0].zz * g.idz, v[0].xz * g.idz, v[0].zx * g.idx, im.sxx,
-                                   im.szz, im.sxz, im.cp, im.cs, im.ip, im.is, op, os);
+                                   im.szz, im.sxz, im.ip, im.is, op, os);
                         d.gvp[q] = op;
                         d.gvs[q] = os;
                     }

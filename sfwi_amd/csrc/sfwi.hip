@@ -572,7 +572,7 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
         tend(c, t);
     }
     const dim3 mg((unsigned)((c->nint + 255) / 256));
-    hipLaunchKernelGGL(k_merge, mg, blk, 0, st, g, D, n, c->Gvp, c->Gvs, c->Gil, c->Grho);
+    hipLaunchKernelGGL(k_merge<!E>, mg, blk, 0, st, g, D, n, c->Gvp, c->Gvs, c->Gil, c->Grho);
     HIPCK(c, hipGetLastError());
     return 0;
 }
