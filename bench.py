@@ -44,10 +44,10 @@ HBM_PEAK_GBS = 8000.0
 MEASURED_HBM_BYTES_PER_SHOT = {
     "k_vel(forward)": 486.7e6 / 64,
     "k_str(forward)": 636.6e6 / 64,
-    "k_str(reconstruct)": 845.4e6 / 64,
-    "k_vel(reconstruct)": 480.2e6 / 64,
-    "k_adj_str": 468.9e6 / 64,
-    "k_adj_vel_image": 1148.4e6 / 64,
+    "k_adj_str": 468.4e6 / 64,
+    "k_str(reconstruct+image)": 866.2e6 / 64,
+    "k_vel(reconstruct+illum)": 543.3e6 / 64,
+    "k_adj_vel": 627.9e6 / 64,
 }
 
 
