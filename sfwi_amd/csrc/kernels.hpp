@@ -901,30 +901,41 @@ __global__ void __launch_bounds__(BLK) k_adj_vel_image(Geo g, const DevShot *__r
     }
 }
 
-// Per-shot accumulators added into the full-model images, shots in batch
-// order so the result does not depend on scheduling.  SCALE (FAST mode): the
-// Vp and Vs accumulators hold cofactors, multiplied here by the imaging
-// coefficients 2 vp rho and 2 vs rho of the shot's window.
+// Per-shot accumulators added into the full-model images.  One thread per
+// model cell (allnx columns of nz rows) adds the shots whose window covers
+// its column, in batch order: windows at different offsets overlap, so a
+// thread per window cell would race with the thread of another shot that
+// maps to the same model cell, and the batch order makes the result equal to
+// the shots merged one at a time, bitwise.  SCALE (FAST mode): the Vp and Vs
+// accumulators hold cofactors, multiplied here by the imaging coefficients
+// 2 vp rho and 2 vs rho of the shot's window (a rounded product, then the sum).
 template <bool SCALE>
-__global__ void k_merge(Geo g, const DevShot *__restrict__ shots, int nshots, float *gvp, float *gvs,
+__global__ void k_merge(Geo g, const DevShot *__restrict__ shots, int nshots, int allnx, float *gvp, float *gvs,
                         float *gil, float *grho) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= g.nx * g.nz) return;
-    const int jx = q / g.nz, jz = q - jx * g.nz;
-    const int i = (jx + g.pml) * g.nzp + jz + g.pml;
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= allnx * g.nz) return;
+    const int ox = o / g.nz, jz = o - ox * g.nz;
+    float a = gvp[o], b = gvs[o], c = gil[o], r = g.grad_rho ? grho[o] : 0.0f;
     for (int s = 0; s < nshots; ++s) {
         const DevShot &d = shots[s];
-        const size_t o = (size_t)(d.left + jx) * g.allnz + jz;
-        if (SCALE) {   // a rounded product, then the sum: a batch equals its shots flushed one at a time
-            gvp[o] += d.m.icp[i] * d.gvp[q];
-            gvs[o] += d.m.ics[i] * d.gvs[q];
+        const int jx = ox - d.left;
+        if (jx < 0 || jx >= g.nx) continue;
+        const int q = jx * g.nz + jz;
+        if (SCALE) {
+            const int i = (jx + g.pml) * g.nzp + jz + g.pml;
+            a += d.m.icp[i] * d.gvp[q];
+            b += d.m.ics[i] * d.gvs[q];
         } else {
-            gvp[o] += d.gvp[q];
-            gvs[o] += d.gvs[q];
+            a += d.gvp[q];
+            b += d.gvs[q];
         }
-        gil[o] += d.gil[q];
-        if (g.grad_rho) grho[o] += d.grho[q];
+        c += d.gil[q];
+        if (g.grad_rho) r += d.grho[q];
     }
+    gvp[o] = a;
+    gvs[o] = b;
+    gil[o] = c;
+    if (g.grad_rho) grho[o] = r;
 }
 
 // ------------------------------------------------------------- set-up ---

@@ -571,8 +571,8 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
                                D, adir);
         tend(c, t);
     }
-    const dim3 mg((unsigned)((c->nint + 255) / 256));
-    hipLaunchKernelGGL(k_merge<!E>, mg, blk, 0, st, g, D, n, c->Gvp, c->Gvs, c->Gil, c->Grho);
+    const dim3 mg((unsigned)((c->nmodel + 255) / 256));
+    hipLaunchKernelGGL(k_merge<!E>, mg, blk, 0, st, g, D, n, c->p.allnx, c->Gvp, c->Gvs, c->Gil, c->Grho);
     HIPCK(c, hipGetLastError());
     return 0;
 }
