@@ -23,12 +23,10 @@
 namespace sfwi {
 
 constexpr int CM_ROWS = 56;   // output rows per wave (64 lanes minus 2 x 4 halo)
-// Prefetch depths of the imaging kernel (each at most the march depth): its
-// operand ring, and the ring of its imaging operands.
-constexpr int ADJV_DEPTH = 4;
+// Prefetch depth of the imaging operands (at most the march depth): one
+// column ahead measured the same as deeper rings, with fewer registers.
 constexpr int IMG_DEPTH = 1;
-constexpr int adjv_depth(int pd) { return pd < ADJV_DEPTH ? pd : ADJV_DEPTH; }
-constexpr int img_depth(int pd) { return adjv_depth(pd) < IMG_DEPTH ? adjv_depth(pd) : IMG_DEPTH; }
+constexpr int img_depth(int pd) { return pd < IMG_DEPTH ? pd : IMG_DEPTH; }
 
 struct Seg {
     int x0, x1, zb;   // columns [x0, x1), first output row zb

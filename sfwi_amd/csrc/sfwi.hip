@@ -543,7 +543,7 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
         hipLaunchKernelGGL((km_vel<false, false, PD, true>), tiles, blk, 0, st, gm, D, it, -1);
         tend(c, t);
         t = tbeg(c, SFWI_K_ADJ_VEL_ONLY, it);
-        hipLaunchKernelGGL((km_adj_vel_image<adjv_depth(PD), false, false, 1>), tiles, blk, 0, st, gm, D, adir);
+        hipLaunchKernelGGL((km_adj_vel_image<PD, false, false, 1>), tiles, blk, 0, st, gm, D, adir);
         tend(c, t);
     }
     for (int it = nt - 1; it >= 0 && !march_img; --it) {
@@ -564,11 +564,9 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
         t = tbeg(c, SFWI_K_ADJ_VEL, it);
         if (percell) hipLaunchKernelGGL(k_adj_vel_image<E>, cells, blk, 0, st, g, D, adir);
         else if (g.grad_rho)
-            hipLaunchKernelGGL((km_adj_vel_image<adjv_depth(PD), true, true, img_depth(PD)>), tiles, blk, 0, st, gm, D,
-                               adir);
+            hipLaunchKernelGGL((km_adj_vel_image<PD, true, true, img_depth(PD)>), tiles, blk, 0, st, gm, D, adir);
         else
-            hipLaunchKernelGGL((km_adj_vel_image<adjv_depth(PD), true, false, img_depth(PD)>), tiles, blk, 0, st, gm,
-                               D, adir);
+            hipLaunchKernelGGL((km_adj_vel_image<PD, true, false, img_depth(PD)>), tiles, blk, 0, st, gm, D, adir);
         tend(c, t);
     }
     const dim3 mg((unsigned)((c->nmodel + 255) / 256));
