@@ -109,3 +109,57 @@ def test_stage1_from_su_files(tmp_path):
     got = np.fromfile(tmp_path / "out" / ("vel_s_1_%d_%d.dat" % (sv.allnx, sv.allnz)), np.float32)
     assert np.array_equal(got.reshape(vs.shape), vs)
     assert (tmp_path / "out" / "misfit.txt").read_text().count("misfit") == 2
+
+
+def test_stage1_overlapping_windows_in_batches(tmp_path):
+    """Four shots of one aperture width whose windows overlap at offsets 0, 10,
+    20 and 30 of a 60-column model: the stage-1 program solving them in
+    batches of 3 + 1, the Python loop solving them as one batch of 4, and the
+    Python loop solving one shot per call give the same models, misfits and
+    steps bit for bit (the per-shot images are merged in shot order whatever
+    the batching)."""
+    sv = _model()
+    w = _lib.ricker(NT, np.float32(DT), F0)
+    rho = np.full_like(sv.vp, 2000.0)
+    spreads = [(sx, np.arange(sx - 140, sx + 151, 10)) for sx in (140, 240, 340, 440)]
+    shots = []
+    for k, (sx, gx) in enumerate(spreads):
+        sh, width = _shot(sx, gx)
+        assert width == 30
+        with _ctx(sv, width, 1) as ctx:
+            ctx.set_wavelet(w)
+            ctx.set_model(sv.vp, sv.vs, rho)
+            oz, ox = ctx.forward(sh)
+        S.write_su(str(tmp_path / "z.su"), k + 1, sx, gx, oz, DT, append=k > 0)
+        S.write_su(str(tmp_path / "x.su"), k + 1, sx, gx, ox, DT, append=k > 0)
+        shots.append(Shot(sh.c.left, sh.c.isx, sh.c.isz, sh.c.igz, sh.c.ngx_left, gx, oz, ox))
+    sv.vp0.tofile(tmp_path / "vp.bin")
+    sv.vs0.tofile(tmp_path / "vs.bin")
+    sv.vs0.tofile(tmp_path / "rho.bin")
+    para = tmp_path / "para"
+    para.write_text("%g %g %d %d %d 1 4\n%s\n%s\n%s\n%s\n%s\n%s\n%g 2 0\n%d %d\n1 0 0 1000\n" % (
+        DX, DX, PML, sv.allnx, sv.allnz, tmp_path / "vp.bin", tmp_path / "vs.bin", tmp_path / "rho.bin",
+        tmp_path / "z.su", tmp_path / "x.su", tmp_path / "out", F0, SY, GY))
+
+    results = []
+    for max_batch in (4, 1):
+        st = Stage1(sv.allnx, sv.allnz, cut_layer=CUT)
+        vp, vs = sv.vp0.copy(), sv.vs0.copy()
+        with _ctx(sv, 30, max_batch) as c:
+            c.set_wavelet(w)
+            its = [st.iterate_groups([(c, shots)], vp, vs, rho) for _ in range(2)]
+        results.append((vp, vs, its))
+    (vp, vs, its), (vp1, vs1, its1) = results
+    assert np.array_equal(vp, vp1) and np.array_equal(vs, vs1)
+    assert [(i["misfit"], i["beta"], i["alpha"]) for i in its] == [(i["misfit"], i["beta"], i["alpha"]) for i in its1]
+    assert its[1]["misfit"] < its[0]["misfit"]
+
+    r = subprocess.run([PROG, str(para), "2", "3", str(CUT)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    lines = [l.split() for l in r.stdout.strip().splitlines()]
+    for l, it in zip(lines, its):
+        assert float(l[3]) == it["misfit"] and float(l[5]) == it["beta"] and float(l[7]) == it["alpha"]
+    got = np.fromfile(tmp_path / "out" / ("vel_p_1_%d_%d.dat" % (sv.allnx, sv.allnz)), np.float32)
+    assert np.array_equal(got.reshape(vp.shape), vp)
+    got = np.fromfile(tmp_path / "out" / ("vel_s_1_%d_%d.dat" % (sv.allnx, sv.allnz)), np.float32)
+    assert np.array_equal(got.reshape(vs.shape), vs)
