@@ -86,6 +86,15 @@ typedef struct sfwi_params {
                            /* after another, split over the XCDs; 2 the shots of a  */
                            /* tile back to back; 3 whole shots per XCD              */
     int grad_rho;          /* 1: gradient solves also image the density (sfwi_flush_rho) */
+    int cpu_mem;           /* 1: the boundary strips of a gradient solve live in pinned */
+                           /* host memory (the reference's cpu_mem, sfwi-stage1/      */
+                           /* FD2DGPU.cu:4393-4421, 4545-4560, 4602-4620): the device */
+                           /* keeps two chunks of strip_chunk steps per shot, and     */
+                           /* completed chunks move over PCIe on a second stream      */
+                           /* while the kernels run; results equal those of HBM      */
+                           /* strips bit for bit.  Solves with it run without graphs. */
+    int strip_chunk;       /* steps per host chunk (the reference's ntt = iointerval; */
+                           /* 0 = 100); nt <= 2 strip_chunk keeps every step in HBM   */
 } sfwi_params;
 
 /* One shot: the fields the reference driver writes into FD2DGPU_ELASTIC

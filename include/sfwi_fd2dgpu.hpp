@@ -16,8 +16,10 @@
 //     which sets the adjoint direction (+1 / -1) and the stage-2 density
 //     (2000 kg/m^3) that the two reference builds hard-code;
 //   * errors throw sfwi_ref::Error (the reference does not check errors);
-//   * rbc = true (random boundary) and cpu_mem = true (host strip offload) are
-//     rejected: boundary strips always stay in HBM here;
+//   * rbc = true (random boundary) is rejected: boundary strips are used;
+//     cpu_mem = true keeps the strips in pinned host memory in chunks of ntt
+//     steps (sfwi_params.cpu_mem); the first gradient call after a flush may
+//     switch the setting (the context is rebuilt), a later one may not;
 //   * the OT misfit outputs (misfit_ot, misfit_step*_ot) receive the optimal-
 //     transport misfit when the context was created with an OT misfit, else
 //     they are left untouched (their reference computations are commented out).
@@ -71,6 +73,8 @@ public:
         p.storage = SFWI_STORE_STRIPS_F32;
         p.misfit = misfit;
         check(sfwi_ctx_create(&p, &ctx_), nullptr);
+        p_ = p;
+        sou_.assign(sou, sou + nt1);
         ntr_max_ = ntr1;
         misfit_ = misfit;
         try {
@@ -92,7 +96,37 @@ public:
     }
     void GPUbufferVPVS(float *vp_all, float *vs_all) { GPUbufferVPVS(vp_all, vs_all, nullptr); }
     // Model perturbation (sfwi-stage1/FD2DGPU.cuh:237).
-    void GPUbufferM(float *pp, float *ps) { check(sfwi_set_perturbation(ctx_, pp, ps), ctx_); }
+    void GPUbufferM(float *pp, float *ps) {
+        dvp_ = pp; dvs_ = ps;
+        check(sfwi_set_perturbation(ctx_, pp, ps), ctx_);
+    }
+    // Strip storage of the next gradient call (the per-call cpu_mem and ntt
+    // arguments of the reference): a change rebuilds the context with the
+    // model, perturbation and wavelet it holds, only while no gradient has
+    // been accumulated since the last flush.
+    void use_host_strips(bool cpu_mem, int ntt) {
+        if (cpu_mem && ntt < 1) throw Error(SFWI_EINVAL, "sfwi: cpu_mem needs ntt >= 1 (steps per chunk)");
+        const int want = cpu_mem ? 1 : 0, chunk = cpu_mem ? ntt : 0;
+        if (want == p_.cpu_mem && chunk == p_.strip_chunk) return;
+        if (dirty_) throw Error(SFWI_ESTATE, "sfwi: cpu_mem / ntt changed with gradients accumulated");
+        sfwi_params q = p_;
+        q.cpu_mem = want;
+        q.strip_chunk = chunk;
+        sfwi_ctx *c = nullptr;
+        check(sfwi_ctx_create(&q, &c), nullptr);
+        try {
+            check(sfwi_set_wavelet(c, sou_.data()), c);
+            if (vp_) check(sfwi_set_model(c, vp_, vs_, rho_), c);
+            if (dvp_) check(sfwi_set_perturbation(c, dvp_, dvs_), c);
+        } catch (...) {
+            sfwi_ctx_destroy(c);
+            throw;
+        }
+        sfwi_ctx_destroy(ctx_);
+        ctx_ = c;
+        p_ = q;
+    }
+    void mark_accumulated() { dirty_ = true; }
     // Observed records of the next shot (sfwi-stage1/FD2DGPU.cuh:387): kept by
     // pointer, read by the next per-shot call.
     void record_copytoGPU(float *record, float *record_x, int *gc) {
@@ -105,6 +139,7 @@ public:
     void imagebuffer_resettozero(float *cpu_pp_grad, float *cpu_ps_grad, float *illumination, float *b_pp,
                                  float *b_ps, float *b_pp2, float *b_ps2) {
         check(sfwi_flush(ctx_, cpu_pp_grad, cpu_ps_grad, illumination), ctx_);
+        dirty_ = false;
         const size_t n = (size_t)allnx * allnz;
         for (float *b : {b_pp, b_ps, b_pp2, b_ps2})
             if (b)
@@ -133,51 +168,58 @@ public:
 
 private:
     sfwi_ctx *ctx_ = nullptr;
+    sfwi_params p_{};
+    std::vector<float> sou_;
+    bool dirty_ = false;
     int ntr_max_ = 0, misfit_ = SFWI_MISFIT_L2, left_ = 0;
     const float *rec_z_ = nullptr, *rec_x_ = nullptr;
     const int *gc_ = nullptr;
     const float *vp_ = nullptr, *vs_ = nullptr, *rho_ = nullptr;
+    const float *dvp_ = nullptr, *dvs_ = nullptr;
 };
 
-inline void reject_unsupported(bool rbc, bool cpu_mem) {
+inline void reject_unsupported(bool rbc) {
     if (rbc) throw Error(SFWI_EINVAL, "sfwi: random boundary (rbc) is not available; strips are used");
-    if (cpu_mem) throw Error(SFWI_EINVAL, "sfwi: host strip offload (cpu_mem) is not available");
 }
 
 // Stage-1 per-shot gradient (sfwi-stage1/FD2DGPU.cuh:466-467).
 inline void IMAGE_SINGLESHOT(FD2DGPU_ELASTIC &fd, const int /*sxnum*/, const int /*nDim*/, const int /*myid*/,
-                             float *record, float *record_x, int *gc, bool rbc, bool cpu_mem, int /*ntt*/,
+                             float *record, float *record_x, int *gc, bool rbc, bool cpu_mem, int ntt,
                              int /*iter*/, int /*iter_round1*/, int /*maxiter*/, char * /*snapdir*/,
                              float * /*image_pp*/, float * /*image_ps*/, float * /*cpu_pp_grad_old*/,
                              float * /*cpu_ps_grad_old*/, double &misfit, float *misfit_ot) {
-    reject_unsupported(rbc, cpu_mem);
+    reject_unsupported(rbc);
+    fd.use_host_strips(cpu_mem, ntt);
     const sfwi_shot s = fd.shot(record, record_x, gc);
     double m = 0.0;
     check(sfwi_shot_gradient(fd.ctx(), &s, &m), fd.ctx());
+    fd.mark_accumulated();
     if (fd.misfit_kind() != SFWI_MISFIT_L2 && misfit_ot) *misfit_ot += (float)m;
     misfit += m;
 }
 
 // Stage-2 per-shot gradient of the Born misfit (sfwi-stage2/FD2DGPU.cuh:455-456).
 inline void IMAGE_SINGLESHOT(FD2DGPU_ELASTIC &fd, const int /*sxnum*/, const int /*nDim*/, const int /*myid*/,
-                             float *record, float *record_x, int *gc, bool rbc, bool cpu_mem, int /*ntt*/,
+                             float *record, float *record_x, int *gc, bool rbc, bool cpu_mem, int ntt,
                              int /*iter*/, char * /*snapdir*/, float * /*image_pp*/, float * /*image_ps*/,
                              float * /*cpu_pp_grad_old*/, float * /*cpu_ps_grad_old*/, double &misfit3) {
-    reject_unsupported(rbc, cpu_mem);
+    reject_unsupported(rbc);
+    fd.use_host_strips(cpu_mem, ntt);
     const sfwi_shot s = fd.shot(record, record_x, gc);
     check(sfwi_shot_born_gradient(fd.ctx(), &s, &misfit3), fd.ctx());
+    fd.mark_accumulated();
 }
 
 // Stage-1 parabolic line search (sfwi-stage1/FD2DGPU.cuh:471-472): misfits of
 // forward solves at vel, vel - step_1*grad_old and vel - step_2*grad_old.
 inline void CALCULATE_ALPHA_parabola_mb(FD2DGPU_ELASTIC &fd, const int /*sxnum*/, const int /*nDim*/,
                                         const int /*myid*/, float *record, float *record_x, int *gc, bool rbc,
-                                        bool cpu_mem, int /*ntt*/, int /*iter*/, float * /*image_pp*/,
+                                        bool /*cpu_mem*/, int /*ntt*/, int /*iter*/, float * /*image_pp*/,
                                         float * /*image_ps*/, float *cpu_pp_grad_old, float *cpu_ps_grad_old,
                                         double &misfit_step0, double &misfit_step1, double &misfit_step2,
                                         float step_1, float step_2, float *misfit_step0_ot,
                                         float *misfit_step1_ot, float *misfit_step2_ot) {
-    reject_unsupported(rbc, cpu_mem);
+    reject_unsupported(rbc);   // forward solves only: no strips, cpu_mem has no effect
     const sfwi_shot s = fd.shot(record, record_x, gc);
     double m[3] = {0.0, 0.0, 0.0};
     check(sfwi_linesearch_parabola(fd.ctx(), &s, cpu_pp_grad_old, cpu_ps_grad_old, step_1, step_2, m), fd.ctx());
@@ -194,10 +236,10 @@ inline void CALCULATE_ALPHA_parabola_mb(FD2DGPU_ELASTIC &fd, const int /*sxnum*/
 // Stage-2 exact step (sfwi-stage2/FD2DGPU.cuh:457-458): fenzi += sum L(cg).r,
 // fenmu += sum L(cg)^2, cg = (cpu_pp_grad_old, cpu_ps_grad_old).
 inline void CALCULATE_ALPHA(FD2DGPU_ELASTIC &fd, const int /*sxnum*/, const int /*nDim*/, const int /*myid*/,
-                            float *record, float *record_x, int *gc, char * /*snapdir*/, bool rbc, bool cpu_mem,
+                            float *record, float *record_x, int *gc, char * /*snapdir*/, bool rbc, bool /*cpu_mem*/,
                             int /*ntt*/, int /*iter*/, float * /*image_pp*/, float * /*image_ps*/,
                             float *cpu_pp_grad_old, float *cpu_ps_grad_old, double &fenzi, double &fenmu) {
-    reject_unsupported(rbc, cpu_mem);
+    reject_unsupported(rbc);   // forward solves only: no strips, cpu_mem has no effect
     const sfwi_shot s = fd.shot(record, record_x, gc);
     check(sfwi_linesearch_linear(fd.ctx(), &s, cpu_pp_grad_old, cpu_ps_grad_old, &fenzi, &fenmu), fd.ctx());
 }

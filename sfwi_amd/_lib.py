@@ -33,6 +33,7 @@ class SfwiParams(ctypes.Structure):
         ("mode", ctypes.c_int), ("storage", ctypes.c_int), ("use_graph", ctypes.c_int),
         ("march_cols", ctypes.c_int), ("march_depth", ctypes.c_int), ("misfit", ctypes.c_int),
         ("march_order", ctypes.c_int), ("grad_rho", ctypes.c_int),
+        ("cpu_mem", ctypes.c_int), ("strip_chunk", ctypes.c_int),
     ]
 
 

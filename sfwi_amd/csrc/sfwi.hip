@@ -30,7 +30,8 @@ struct Slot {
     char *block = nullptr;      // fw(15) + bk(15) + derivatives(4) fields
     char *born = nullptr;       // bn(15) fields
     char *rec = nullptr;        // rz rx r2z r2x oz ox, ints, accumulators, trace sums
-    float *strip = nullptr;     // FP32 [nt][2][sf], or 16-bit [nt][2][sf] + scales [nt][2] + stage [2][sf]
+    float *strip = nullptr;     // FP32 [ring][2][sf], or 16-bit: scales [ring][2], stage [2][sf], [ring][2][sf]
+    char *hstrip = nullptr;     // host strips (cpu_mem, pinned): FP32 [nt][2][sf], or 16-bit [nt][2][sf] + scales [nt][2]
     Fields fw{}, bk{}, bn{};
     float *dg[4]{};
     float *rz, *rx, *r2z, *r2x, *oz, *ox;
@@ -78,6 +79,12 @@ struct sfwi_ctx {
     float *ot_io = nullptr;
     size_t ot_jobs_cap = 0, ot_ws_cap = 0, ot_mis_cap = 0, ot_first_cap = 0, ot_io_cap = 0;
     std::map<std::pair<int, int>, hipGraphExec_t> graphs;   // captured solves by (kind, batch size)
+    // boundary strips in host memory (cpu_mem): the device keeps `ring` steps
+    // per slot (nt, or two chunks of `chunk` steps); completed chunks are
+    // copied on `cstream`, ordered against the kernels by the events
+    int ring = 0, chunk = 0;
+    hipStream_t cstream = nullptr;
+    hipEvent_t ev_ring[2]{}, ev_copy[2]{};
     std::vector<OtJob> ot_h;          // host copies kept alive for the async uploads
     std::vector<int> ot_first_h;
     // timing
@@ -246,11 +253,19 @@ int ensure_slots(sfwi_ctx *c, int n, bool born, bool strips) {
             for (int k = 0; k < 15; ++k) *bn[k] = f + k * c->ncell;
         }
         if (strips && !S.strip) {
-            const size_t nv = (size_t)c->p.nt * 2 * c->g.sf;
-            const size_t bytes = c->g.store ? nv * 2 + (size_t)c->p.nt * 2 * sizeof(float) + 2 * (size_t)c->g.sf * sizeof(float) + 256
+            const size_t nv = (size_t)c->ring * 2 * c->g.sf;
+            const size_t bytes = c->g.store ? nv * 2 + (size_t)c->ring * 2 * sizeof(float) + 2 * (size_t)c->g.sf * sizeof(float) + 256
                                             : nv * sizeof(float);
             int r = dalloc(c, (void **)&S.strip, bytes);
             if (r) return r;
+            if (c->ring < c->p.nt) {
+                const size_t hv = (size_t)c->p.nt * 2 * c->g.sf;
+                const size_t hb = c->g.store ? hv * 2 + (size_t)c->p.nt * 2 * sizeof(float) : hv * sizeof(float);
+                if (hipHostMalloc((void **)&S.hstrip, hb, hipHostMallocDefault) != hipSuccess) {
+                    S.hstrip = nullptr;
+                    return fail(c, SFWI_ENOMEM, "pinned host allocation of %zu MiB for the strips failed", hb >> 20);
+                }
+            }
         }
     }
     return 0;
@@ -319,7 +334,7 @@ int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium 
             const size_t nv = (size_t)c->p.nt * 2 * c->g.sf;
             char *q = (char *)S.strip;
             d.strip_scale = (float *)q;
-            d.stage = d.strip_scale + (size_t)c->p.nt * 2;
+            d.stage = d.strip_scale + (size_t)c->ring * 2;
             d.strip_h = (unsigned short *)(float *)(d.stage + 2 * (size_t)c->g.sf);
             (void)nv;
         }
@@ -443,6 +458,34 @@ int ot_batch(sfwi_ctx *c, int n, bool born) {
     return 0;
 }
 
+// Copy steps [k chunk, min((k+1) chunk, nt)) of every slot's boundary strips
+// between ring half k % 2 on the device and the slot's pinned host buffer, on
+// the copy stream (cpu_mem).
+int strip_chunk_copy(sfwi_ctx *c, int n, int k, bool to_host) {
+    const size_t per = 2 * (size_t)c->g.sf;   // values of one step
+    const size_t t0 = (size_t)k * c->chunk;
+    const size_t steps = std::min<size_t>(c->chunk, (size_t)c->p.nt - t0);
+    const size_t r0 = (size_t)(k & 1) * c->chunk;
+    const hipMemcpyKind kd = to_host ? hipMemcpyDeviceToHost : hipMemcpyHostToDevice;
+    auto cp = [&](void *dev, void *host, size_t bytes) {
+        return to_host ? hipMemcpyAsync(host, dev, bytes, kd, c->cstream) : hipMemcpyAsync(dev, host, bytes, kd, c->cstream);
+    };
+    for (int s = 0; s < n; ++s) {
+        Slot &S = c->slots[s];
+        if (!c->g.store) {
+            HIPCK(c, cp(S.strip + r0 * per, (float *)S.hstrip + t0 * per, steps * per * sizeof(float)));
+        } else {
+            float *dscale = S.strip;
+            unsigned short *dh = (unsigned short *)(dscale + (size_t)c->ring * 2 + per);
+            unsigned short *hh = (unsigned short *)S.hstrip;
+            float *hscale = (float *)(hh + (size_t)c->p.nt * per);
+            HIPCK(c, cp(dh + r0 * per, hh + t0 * per, steps * per * sizeof(unsigned short)));
+            HIPCK(c, cp(dscale + r0 * 2, hscale + t0 * 2, steps * 2 * sizeof(float)));
+        }
+    }
+    return 0;
+}
+
 template <bool E, int PD>
 int run_batch(sfwi_ctx *c, Kind kind, int n) {
     const Geo &g = c->g;
@@ -470,28 +513,38 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
     // when the batch has too few wave tiles to fill the GPU: measured crossover
     // near 7 march waves per SIMD (1024 SIMDs)
     const bool percell = E || c->p.march_cols < 0 || (c->p.march_cols == 0 && (long)n * g.cm_waves < 7168);
+    // Strips in host memory (cpu_mem): step it lives at ring position it % ring
+    // (two chunks).  Forward: a chunk is copied to the host once written (the
+    // last two stay on the device), and a ring half is rewritten only after its
+    // previous chunk has left.  Reverse: the last two chunks are resident;
+    // once a chunk is consumed its half takes the chunk two below it, copied
+    // while the chunk in between is computed.
+    const bool hs = grad && c->ring < nt;
+    const int ch = c->chunk, klast = (nt - 1) / ch;
     for (int it = 0; it < nt; ++it) {
+        const int sp = hs ? it % c->ring : it;
+        if (hs && it % ch == 0 && it / ch >= 2) HIPCK(c, hipStreamWaitEvent(st, c->ev_copy[(it / ch) & 1], 0));
         hipLaunchKernelGGL(k_source, src1, dim3(64), 0, st, g, D, c->wav, it, n);
         int t = tbeg(c, SFWI_K_FWD_VEL, it);
         if (percell) {
             if (born) {
-                if (grad) hipLaunchKernelGGL((k_vel<E, true, true>), cells, blk, 0, st, g, D, it, 1);
-                else hipLaunchKernelGGL((k_vel<E, false, true>), cells, blk, 0, st, g, D, it, 1);
+                if (grad) hipLaunchKernelGGL((k_vel<E, true, true>), cells, blk, 0, st, g, D, sp, 1);
+                else hipLaunchKernelGGL((k_vel<E, false, true>), cells, blk, 0, st, g, D, sp, 1);
             } else {
-                if (grad) hipLaunchKernelGGL((k_vel<E, true, false>), cells, blk, 0, st, g, D, it, 1);
-                else hipLaunchKernelGGL((k_vel<E, false, false>), cells, blk, 0, st, g, D, it, 1);
+                if (grad) hipLaunchKernelGGL((k_vel<E, true, false>), cells, blk, 0, st, g, D, sp, 1);
+                else hipLaunchKernelGGL((k_vel<E, false, false>), cells, blk, 0, st, g, D, sp, 1);
             }
         } else {
             if (born) {
-                if (grad) hipLaunchKernelGGL((km_vel<true, true, PD>), tiles, blk, 0, st, gm, D, it, 1);
-                else hipLaunchKernelGGL((km_vel<false, true, PD>), tiles, blk, 0, st, gm, D, it, 1);
+                if (grad) hipLaunchKernelGGL((km_vel<true, true, PD>), tiles, blk, 0, st, gm, D, sp, 1);
+                else hipLaunchKernelGGL((km_vel<false, true, PD>), tiles, blk, 0, st, gm, D, sp, 1);
             } else {
-                if (grad) hipLaunchKernelGGL((km_vel<true, false, PD>), tiles, blk, 0, st, gm, D, it, 1);
-                else hipLaunchKernelGGL((km_vel<false, false, PD>), tiles, blk, 0, st, gm, D, it, 1);
+                if (grad) hipLaunchKernelGGL((km_vel<true, false, PD>), tiles, blk, 0, st, gm, D, sp, 1);
+                else hipLaunchKernelGGL((km_vel<false, false, PD>), tiles, blk, 0, st, gm, D, sp, 1);
             }
         }
         tend(c, t);
-        if (grad && g.store) hipLaunchKernelGGL(k_strip_pack, dim3(2, (unsigned)n), dim3(256), 0, st, g, D, it);
+        if (grad && g.store) hipLaunchKernelGGL(k_strip_pack, dim3(2, (unsigned)n), dim3(256), 0, st, g, D, sp);
         t = tbeg(c, SFWI_K_FWD_STR, it);
         if (percell) {
             if (born) hipLaunchKernelGGL((k_str<E, false, true>), cells, blk, 0, st, g, D, 1);
@@ -503,6 +556,13 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
         tend(c, t);
         if (born) hipLaunchKernelGGL(k_record<true>, trg, dim3(64), 0, st, g, D, it);
         else hipLaunchKernelGGL(k_record<false>, trg, dim3(64), 0, st, g, D, it);
+        if (hs && it % ch == ch - 1 && it / ch <= klast - 2) {   // chunk written: to the host
+            const int k = it / ch;
+            HIPCK(c, hipEventRecord(c->ev_ring[k & 1], st));
+            HIPCK(c, hipStreamWaitEvent(c->cstream, c->ev_ring[k & 1], 0));
+            if (int r = strip_chunk_copy(c, n, k, true)) return r;
+            HIPCK(c, hipEventRecord(c->ev_copy[k & 1], c->cstream));
+        }
     }
     HIPCK(c, hipGetLastError());
     if (kind == K_FWD || kind == K_BORN_FWD) return 0;
@@ -522,6 +582,22 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
 
     const int adir = c->p.adjoint_dir;
     const dim3 stripg((unsigned)((g.sf + 255) / 256), (unsigned)n);
+    auto rev_begin = [&](int it) -> int {   // the step's strips restored (after their chunk has arrived)
+        if (hs && it % ch == ch - 1 && it / ch <= klast - 2)
+            HIPCK(c, hipStreamWaitEvent(st, c->ev_copy[(it / ch) & 1], 0));
+        hipLaunchKernelGGL(k_strip_load, stripg, blk, 0, st, g, D, hs ? it % c->ring : it);
+        return 0;
+    };
+    auto rev_end = [&](int it) -> int {     // chunk consumed: its ring half takes the chunk two below
+        if (hs && it % ch == 0 && it / ch >= 2) {
+            const int k = it / ch;
+            HIPCK(c, hipEventRecord(c->ev_ring[k & 1], st));
+            HIPCK(c, hipStreamWaitEvent(c->cstream, c->ev_ring[k & 1], 0));
+            if (int r = strip_chunk_copy(c, n, k - 2, false)) return r;
+            HIPCK(c, hipEventRecord(c->ev_copy[k & 1], c->cstream));
+        }
+        return 0;
+    };
     // Column-march reverse step without the density image: the adjoint
     // stress-like step runs first, so the reconstruction's stress step can
     // image from the velocity derivatives in its registers and the reverse
@@ -531,7 +607,7 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
     // same arithmetic on the same operands (identical results).
     const bool march_img = !percell && !g.grad_rho;
     for (int it = nt - 1; it >= 0 && march_img; --it) {
-        hipLaunchKernelGGL(k_strip_load, stripg, blk, 0, st, g, D, it);
+        if (int r = rev_begin(it)) return r;
         hipLaunchKernelGGL(k_inject, trg, dim3(64), 0, st, g, D, it);
         int t = tbeg(c, SFWI_K_ADJ_STR, it);
         hipLaunchKernelGGL(km_adj_str<PD>, tiles, blk, 0, st, gm, D, adir);
@@ -545,9 +621,10 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
         t = tbeg(c, SFWI_K_ADJ_VEL_ONLY, it);
         hipLaunchKernelGGL((km_adj_vel_image<PD, false, false, 1>), tiles, blk, 0, st, gm, D, adir);
         tend(c, t);
+        if (int r = rev_end(it)) return r;
     }
     for (int it = nt - 1; it >= 0 && !march_img; --it) {
-        hipLaunchKernelGGL(k_strip_load, stripg, blk, 0, st, g, D, it);
+        if (int r = rev_begin(it)) return r;
         int t = tbeg(c, SFWI_K_REC_STR, it);
         if (percell) hipLaunchKernelGGL((k_str<E, true, false>), cells, blk, 0, st, g, D, -1);
         else hipLaunchKernelGGL((km_str<true, false, PD>), tiles, blk, 0, st, gm, D, -1);
@@ -568,6 +645,7 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
         else
             hipLaunchKernelGGL((km_adj_vel_image<PD, true, false, img_depth(PD)>), tiles, blk, 0, st, gm, D, adir);
         tend(c, t);
+        if (int r = rev_end(it)) return r;
     }
     const dim3 mg((unsigned)((c->nmodel + 255) / 256));
     hipLaunchKernelGGL(k_merge<!E>, mg, blk, 0, st, g, D, n, c->p.allnx, c->Gvp, c->Gvs, c->Gil, c->Grho);
@@ -593,11 +671,12 @@ int run_direct(sfwi_ctx *c, Kind kind, int n) {
 // kernels read (slot and medium pointers, geometry) lives in the device shot
 // table that prepare() uploads before each replay, so one graph serves any
 // shots of that batch size.  The capture holds kernels only (no memset or
-// host copy).  Solves that need host work between kernels (the OT misfit)
-// and timed solves run directly.
+// host copy).  Solves that need host work between kernels (the OT misfit),
+// solves whose strips live in host memory (copies on a second stream) and
+// timed solves run directly.
 int run(sfwi_ctx *c, Kind kind, int n) {
     const bool host_work = c->p.misfit != SFWI_MISFIT_L2 && kind != K_FWD && kind != K_BORN_FWD;
-    if (!c->p.use_graph || host_work || c->timing_every > 0) return run_direct(c, kind, n);
+    if (!c->p.use_graph || host_work || c->timing_every > 0 || c->ring < c->p.nt) return run_direct(c, kind, n);
     const std::pair<int, int> key((int)kind, n);
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
@@ -680,6 +759,8 @@ int sfwi_ctx_create(const sfwi_params *p, sfwi_ctx **out) {
     if (p->march_order < 0 || p->march_order > 3) return fail(nullptr, SFWI_EINVAL, "unknown march_order %d", p->march_order);
     if (p->misfit < SFWI_MISFIT_L2 || p->misfit > SFWI_MISFIT_OT_GATHER)
         return fail(nullptr, SFWI_EINVAL, "unknown misfit %d", p->misfit);
+    if (p->cpu_mem != 0 && p->cpu_mem != 1) return fail(nullptr, SFWI_EINVAL, "cpu_mem must be 0 or 1");
+    if (p->strip_chunk < 0) return fail(nullptr, SFWI_EINVAL, "strip_chunk must be >= 0");
     const long nxp = p->nx + 2L * p->pml, nzp = p->nz + 2L * p->pml;
     if (nxp * nzp >= (1L << 31) / 16) return fail(nullptr, SFWI_EINVAL, "grid too large for 32-bit cell indices");
     int dev = 0;
@@ -708,9 +789,20 @@ int sfwi_ctx_create(const sfwi_params *p, sfwi_ctx **out) {
     c->nrec = (size_t)p->ntr * p->nt;
     c->nint = (size_t)p->nx * p->nz;
 
+    c->chunk = p->strip_chunk > 0 ? p->strip_chunk : 100;
+    c->ring = (p->cpu_mem && 2L * c->chunk < p->nt) ? 2 * c->chunk : p->nt;
+
     auto bail = [&](int code) { g_create_error = c->err; sfwi_ctx_destroy(c); return code; };
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(c, SFWI_EHIP, "hipStreamCreate failed"));
+    if (c->ring < p->nt) {
+        if (hipStreamCreateWithFlags(&c->cstream, hipStreamNonBlocking) != hipSuccess)
+            return bail(fail(c, SFWI_EHIP, "hipStreamCreate failed (strip copies)"));
+        for (int k = 0; k < 2; ++k)
+            if (hipEventCreateWithFlags(&c->ev_ring[k], hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&c->ev_copy[k], hipEventDisableTiming) != hipSuccess)
+                return bail(fail(c, SFWI_EHIP, "hipEventCreate failed (strip copies)"));
+    }
     int r;
     if ((r = dalloc(c, (void **)&c->wav, p->nt * sizeof(float)))) return bail(r);
     if ((r = dalloc(c, (void **)&c->mvp, 5 * c->nmodel * sizeof(float)))) return bail(r);
@@ -735,13 +827,20 @@ int sfwi_ctx_create(const sfwi_params *p, sfwi_ctx **out) {
 void sfwi_ctx_destroy(sfwi_ctx *c) {
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->cstream) hipStreamSynchronize(c->cstream);
     for (auto &kv : c->graphs) hipGraphExecDestroy(kv.second);
     for (Slot &S : c->slots) {
         hipFree(S.block);
         hipFree(S.born);
         hipFree(S.rec);
         hipFree(S.strip);
+        if (S.hstrip) hipHostFree(S.hstrip);
     }
+    for (int k = 0; k < 2; ++k) {
+        if (c->ev_ring[k]) hipEventDestroy(c->ev_ring[k]);
+        if (c->ev_copy[k]) hipEventDestroy(c->ev_copy[k]);
+    }
+    if (c->cstream) hipStreamDestroy(c->cstream);
     drop_windows(c);
     for (hipEvent_t e : c->ev) hipEventDestroy(e);
     hipFree(c->wav); hipFree(c->mvp); hipFree(c->tmp_model); hipFree(c->Gvp); hipFree(c->res_keep);

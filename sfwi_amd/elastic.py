@@ -41,7 +41,7 @@ class Shot:
 
 def make_params(*, dx, dz, dt, nx, nz, pml, nt, ntr, allnx, allnz, rr=0.001, rho_const=0.0, adjoint_dir=1,
                 max_batch=1, mode="fast", storage="f32", use_graph=False, march_cols=0, march_depth=0,
-                misfit="l2", march_order=0, grad_rho=False):
+                misfit="l2", march_order=0, grad_rho=False, cpu_mem=False, strip_chunk=0):
     """``sfwi_params`` of a context or request (keyword names as in the C struct)."""
     p = SfwiParams()
     p.dx, p.dz, p.dt = dx, dz, dt
@@ -57,6 +57,8 @@ def make_params(*, dx, dz, dt, nx, nz, pml, nt, ntr, allnx, allnz, rr=0.001, rho
                 "ot_gather": _lib.MISFIT_OT_GATHER}[misfit]
     p.march_order = int(march_order)
     p.grad_rho = int(bool(grad_rho))
+    p.cpu_mem = int(bool(cpu_mem))
+    p.strip_chunk = int(strip_chunk)
     return p
 
 

@@ -9,7 +9,11 @@
 // clamped parabolic step, model update with the top-row reset) and writes the
 // final model.
 //
-//   fd2dgpu_driver <survey.bin> <out_prefix> [step1 step2 [niter cut_layer]]
+//   fd2dgpu_driver <survey.bin> <out_prefix> [step1 step2 [niter cut_layer [ntt]]]
+//
+// ntt > 0 passes cpu_mem = true with chunks of ntt steps to the per-shot
+// gradient calls, as the reference driver passes its cpu_mem and iointerval
+// (boundary strips in host memory).
 //
 // survey.bin (little endian): int32 header
 //   nx nz allnx allnz pml nt ntr_max nshots mode misfit
@@ -61,13 +65,14 @@ void write_bin(const std::string &path, const std::vector<float> &a) {
 }  // namespace
 
 int main(int argc, char **argv) {
-    if (argc != 3 && argc != 5 && argc != 7) {
-        std::fprintf(stderr, "usage: %s survey.bin out_prefix [step1 step2 [niter cut_layer]]\n", argv[0]);
+    if (argc != 3 && argc != 5 && argc != 7 && argc != 8) {
+        std::fprintf(stderr, "usage: %s survey.bin out_prefix [step1 step2 [niter cut_layer [ntt]]]\n", argv[0]);
         return 2;
     }
-    const int niter = argc == 7 ? std::atoi(argv[5]) : 0, cut_layer = argc == 7 ? std::atoi(argv[6]) : 0;
-    if (argc == 7 && (niter < 1 || cut_layer < 0)) {
-        std::fprintf(stderr, "input error: niter must be >= 1 and cut_layer >= 0\n");
+    const int niter = argc >= 7 ? std::atoi(argv[5]) : 0, cut_layer = argc >= 7 ? std::atoi(argv[6]) : 0;
+    const int ntt = argc == 8 ? std::atoi(argv[7]) : 0;
+    if (argc >= 7 && (niter < 1 || cut_layer < 0 || ntt < 0)) {
+        std::fprintf(stderr, "input error: niter must be >= 1, cut_layer >= 0 and ntt >= 0\n");
         return 2;
     }
     int nx, nz, allnx, allnz, pml, nt, ntr_max, nshots, mode, misfit_kind;
@@ -130,7 +135,7 @@ int main(int argc, char **argv) {
                 image2delastic.cmin = (float)s.left;
                 image2delastic.ngx_left = s.ngx_left;
                 sfwi_ref::IMAGE_SINGLESHOT(image2delastic, k, nxpml * nzpml, 0, s.rec_z.data(), s.rec_x.data(),
-                                           s.gc.data(), false, false, 0, 0, 0, 1, nullptr, nullptr, nullptr,
+                                           s.gc.data(), false, ntt > 0, ntt, 0, 0, 1, nullptr, nullptr, nullptr,
                                            nullptr, nullptr, misfit, &misfit_ot);
             }
             image2delastic.imagebuffer_resettozero(pp_grad.data(), ps_grad.data(), illum.data(), b1.data(),

@@ -12,7 +12,9 @@
 // reference does (sfwi-stage1/main.cpp:1476-1479).  Where the reference sizes
 // its one context by shot 0's aperture and runs other apertures on it
 // (main.cpp:610-617), every aperture width here gets a context of its own;
-// shots of a width are solved in batches of max_batch (default 64).
+// shots of a width are solved in batches of max_batch (default 64).  With
+// cpu_mem = 1 in the file the boundary strips live in pinned host memory in
+// chunks of iointerval steps (the reference's ntt), as the reference does.
 // Per iteration: one line "iter <k> misfit <m> beta <b> alpha <a>" on stdout,
 // and, as the reference writes them, <outdir>/misfit.txt lines and
 // <outdir>/vel_p_<k>_<allnx>_<allnz>.dat, vel_s_<k>_<allnx>_<allnz>.dat.
@@ -74,7 +76,8 @@ int main(int argc, char **argv) {
         const int cut_layer = argc >= 5 ? std::atoi(argv[4]) : 26;   // sfwi-stage1/main.cpp:410
         if (niter < 0 || max_batch < 1 || cut_layer < 0)
             throw Fail(2, "input error: niter must be >= 0, max_batch >= 1, cut_layer >= 0");
-        if (p.rbc || p.cpu_mem) throw Fail(2, "input error: rbc and cpu_mem are not supported (strips stay in HBM)");
+        if (p.rbc) throw Fail(2, "input error: rbc (random boundary) is not supported; strips are used");
+        if (p.cpu_mem && p.iointerval < 1) throw Fail(2, "input error: cpu_mem needs iointerval >= 1 (steps per chunk)");
         const size_t nm = (size_t)p.allnx * p.allnz;
         std::vector<float> vp(nm), vs(nm), rho(nm, 2000.0f);
         ck_input(sfwi_model_read(p.vp, p.allnx, p.allnz, vp.data()));
@@ -126,6 +129,8 @@ int main(int argc, char **argv) {
             q.dx = p.dx; q.dz = p.dz; q.dt = dt; q.nx = w; q.nz = p.allnz; q.pml = p.pml; q.nt = nt; q.nop = 4;
             q.ntr = ntr_max; q.allnx = p.allnx; q.allnz = p.allnz; q.rr = 0.001f; q.adjoint_dir = 1;
             q.max_batch = std::min<int>(max_batch, (int)members[w].size());
+            q.cpu_mem = p.cpu_mem ? 1 : 0;      // strips in pinned host memory, iointerval steps per chunk
+            q.strip_chunk = p.cpu_mem ? p.iointerval : 0;
             sfwi_ctx *c = nullptr;
             ck(sfwi_ctx_create(&q, &c), nullptr);
             ctxs.push_back(c);

@@ -40,7 +40,8 @@ def _params(**kw):
 
 
 @pytest.mark.parametrize("bad", [dict(nop=6), dict(pml=4), dict(nz=30), dict(adjoint_dir=0),
-                                 dict(dt=0.0), dict(max_batch=0), dict(mode=7), dict(nx=100)])
+                                 dict(dt=0.0), dict(max_batch=0), dict(mode=7), dict(nx=100),
+                                 dict(cpu_mem=2), dict(cpu_mem=1, strip_chunk=-1)])
 def test_create_rejects_bad_parameters(bad):
     L = _lib.load()
     h = ctypes.c_void_p()

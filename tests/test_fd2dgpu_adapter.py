@@ -131,3 +131,12 @@ def test_driver_stage1_iterations_match_python_path(tmp_path):
     for name, ref in (("vel_p", vp), ("vel_s", vs)):
         got = np.fromfile(tmp_path / ("o_%s.bin" % name), np.float32).reshape(ref.shape)
         assert np.array_equal(got, ref), name
+    # the same with cpu_mem = true, ntt = 50 (strips in host memory; the adapter
+    # rebuilds its context on the first gradient call): identical output
+    r2 = _run(str(path), str(tmp_path / "h_"), "30", "150", "2", "4", "50")
+    assert r2.returncode == 0, r2.stderr
+    assert r2.stdout == r.stdout
+    for name in ("vel_p", "vel_s"):
+        a = np.fromfile(tmp_path / ("o_%s.bin" % name), np.float32)
+        b = np.fromfile(tmp_path / ("h_%s.bin" % name), np.float32)
+        assert np.array_equal(a, b), name

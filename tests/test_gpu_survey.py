@@ -163,3 +163,17 @@ def test_stage1_overlapping_windows_in_batches(tmp_path):
     assert np.array_equal(got.reshape(vp.shape), vp)
     got = np.fromfile(tmp_path / "out" / ("vel_s_1_%d_%d.dat" % (sv.allnx, sv.allnz)), np.float32)
     assert np.array_equal(got.reshape(vs.shape), vs)
+
+    # cpu_mem = 1, iointerval = 50 in the file: strips in host memory in chunks of
+    # 50 steps (the last one partial); same output bit for bit
+    para2 = tmp_path / "para2"
+    para2.write_text(para.read_text().replace("1 0 0 1000\n", "1 0 1 50\n").replace(str(tmp_path / "out"),
+                                                                                    str(tmp_path / "out2")))
+    assert S.read_para(para2).as_dict()["cpu_mem"] == 1
+    r2 = subprocess.run([PROG, str(para2), "2", "3", str(CUT)], capture_output=True, text=True, timeout=300)
+    assert r2.returncode == 0, r2.stderr
+    assert r2.stdout == r.stdout
+    for name in ("vel_p_1", "vel_s_1"):
+        f = "%s_%d_%d.dat" % (name, sv.allnx, sv.allnz)
+        assert np.array_equal(np.fromfile(tmp_path / "out" / f, np.float32),
+                              np.fromfile(tmp_path / "out2" / f, np.float32)), name
