@@ -464,7 +464,7 @@ int ot_batch(sfwi_ctx *c, int n, bool born) {
 int strip_chunk_copy(sfwi_ctx *c, int n, int k, bool to_host) {
     const size_t per = 2 * (size_t)c->g.sf;   // values of one step
     const size_t t0 = (size_t)k * c->chunk;
-    const size_t steps = std::min<size_t>(c->chunk, (size_t)c->p.nt - t0);
+    const size_t steps = std::min<size_t>(c->chunk, (size_t)c->g.nt - t0);   // g.nt: the solve's record length
     const size_t r0 = (size_t)(k & 1) * c->chunk;
     const hipMemcpyKind kd = to_host ? hipMemcpyDeviceToHost : hipMemcpyHostToDevice;
     auto cp = [&](void *dev, void *host, size_t bytes) {
