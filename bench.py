@@ -109,6 +109,8 @@ def main():
     ap.add_argument("--nt", type=int, default=0, help="time samples (0: the workload's)")
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"],
                     help="boundary-strip storage of the reconstruction")
+    ap.add_argument("--misfit", default="l2", choices=["l2", "ot_trace", "ot_power", "ot_gather"],
+                    help="misfit of the gradient (l2: the reference's; ot_*: optimal-transport variants)")
     ap.add_argument("--cpu-mem", type=int, default=0, metavar="CHUNK",
                     help="boundary strips in pinned host memory, CHUNK steps per chunk (0: strips in HBM)")
     ap.add_argument("--mode", default="fast", choices=["fast", "exact"])
@@ -164,7 +166,7 @@ def main():
     ctx = Context(dx=sv.dx, dz=sv.dz, dt=sv.dt, nx=sv.nx, nz=sv.allnz, pml=sv.pml, nt=sv.nt, ntr=sv.ntr,
                   allnx=sv.allnx, allnz=sv.allnz, max_batch=len(sv.shots), mode=args.mode, storage=args.storage,
                   march_cols=args.march_cols, march_depth=args.march_depth, march_order=args.march_order,
-                  cpu_mem=args.cpu_mem > 0, strip_chunk=max(args.cpu_mem, 0))
+                  cpu_mem=args.cpu_mem > 0, strip_chunk=max(args.cpu_mem, 0), misfit=args.misfit)
     ctx.set_wavelet(w)
     ctx.set_model(sv.vp, sv.vs, sv.rho)
     t0 = time.perf_counter()
@@ -261,7 +263,7 @@ def main():
     # accuracy of two shots against the FP64 oracle (and the CPU baseline)
     acc = None
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and args.misfit == "l2":     # the oracle leg is the L2 gradient
         from oracle import oracle as O
         ncpu = max(1, min(args.cpu_shots, nsh))
         sub = shots[:ncpu]
@@ -299,7 +301,7 @@ def main():
         "config": {"workload": "%s: %dx%d dx=dz=%gm, pml %d, dt %gms, nt %d, %d shots, %g Hz, "
                                "%d receivers, batched gradient" % (sv.name, sv.nx, sv.allnz, sv.dx, sv.pml,
                                                                     sv.dt * 1e3, sv.nt, nsh, sv.f0, sv.ntr),
-                   "mode": args.mode, "storage": "boundary strips %s%s" % (
+                   "mode": args.mode, "misfit": args.misfit, "storage": "boundary strips %s%s" % (
                        args.storage.upper(),
                        ", pinned host memory in chunks of %d steps" % args.cpu_mem if args.cpu_mem > 0 else "")},
         "gcell_updates_per_s": round(gcells, 2),

@@ -151,12 +151,17 @@ k_ot(const OtJob *__restrict__ jobs, int njobs, double *__restrict__ ws, size_t 
             continue;
         }
         if (t == 0) { P[0] = 0.0; Q[0] = 0.0; }
+        // The running sums and the totals add the same terms in different
+        // orders, so a fraction can round above 1 where the mass still to come
+        // is below the rounding of the total (a trace whose energy has ended):
+        // clamp to 1, which the last cell holds exactly.  The merge walk below
+        // relies on every P[i] <= Q[n] = 1.
         for (int j = j0; j < j1; ++j) {
             const double s = J.syn[j], o = J.obs[j];
             cu += J.power ? s * s : s + c;
             cv += J.power ? o * o : o + c;
-            P[j + 1] = cu / Mu;
-            Q[j + 1] = cv / Mv;
+            P[j + 1] = fmin(cu / Mu, 1.0);
+            Q[j + 1] = fmin(cv / Mv, 1.0);
         }
         if (j0 < j1 && j1 == n) { P[n] = 1.0; Q[n] = 1.0; }   // owner of the last cell
         __syncthreads();
@@ -169,7 +174,9 @@ k_ot(const OtJob *__restrict__ jobs, int njobs, double *__restrict__ ws, size_t 
             if (pp > 0.0) {
                 int k = ot_floor_index(Q, n, Pi);
                 double ya = Pi;
-                while (ya < Pe) {
+                // k rises on every pass that does not end the cell (Pe <= Q[n]),
+                // so n passes bound the walk
+                for (int pass = 0; ya < Pe && pass <= n; ++pass) {
                     const double Qk = Q[k], Qn = Q[k + 1];
                     const double yb = fmin(Qn, Pe);
                     if (yb > ya) {
