@@ -8,7 +8,10 @@ oracle at that size:
     one at a time, bit for bit;
   * the 1200 x 600 mixed-precision grid (nt 6000, 40-cell PML), 2 shots: the
     16-bit boundary-strip storage within the accuracy bar (gradient 1e-3) of
-    the FP32 strips, which are within 2e-6 of FP64 on this grid.
+    the FP32 strips, which are within 2e-6 of FP64 on this grid;
+  * the 401 x 301 batch with each optimal-transport misfit: the gradient
+    completes with finite images and its per-shot misfits equal the
+    standalone OT misfit of the same traces.
 """
 import numpy as np
 import pytest
@@ -85,3 +88,31 @@ def test_mixed_precision_full_size_strip_storage():
         g = out[st][1]
         assert rel_l2(g[0], ref[0]) <= bar and rel_l2(g[1], ref[1]) <= bar, st
         assert np.array_equal(out[st][0], out["f32"][0])     # the forward pass and its misfit do not use strips
+
+
+@pytest.mark.parametrize("misfit,variant", [("ot_trace", 0), ("ot_power", 1), ("ot_gather", 2)])
+def test_shot_batch_full_size_ot_misfits(misfit, variant):
+    """The 401 x 301 shot batch (nt 3000), 3 shots, with each optimal-transport
+    misfit: the batched gradient runs to the end on these full-length traces
+    (whose signal has ended before the record in some traces), gives finite,
+    nonzero images, and its per-shot misfits equal the standalone OT misfit of
+    the same synthetic and observed traces."""
+    sv = W.shot_batch(nshots=3)
+    w = W.ricker(sv.nt, sv.dt, sv.f0)
+    obs = _records(sv, w)
+    shots = _shots(sv, obs)
+    with _ctx(sv, 3, misfit=misfit) as ctx:
+        ctx.set_wavelet(w)
+        ctx.set_model(sv.vp0, sv.vs0, sv.rho0)
+        m = ctx.batch_gradient(shots)
+        g = ctx.flush()
+        ctx.set_model(sv.vp0, sv.vs0, sv.rho0)
+        syn = ctx.batch_forward(_shots(sv))
+        for k, s in enumerate(sv.shots):
+            n = len(s.gc)
+            sm, _, _ = ctx.ot_misfit(obs[1][k][:n], obs[0][k][:n], syn[1][k][:n], syn[0][k][:n],
+                                     float(np.float32(sv.dt)), variant)
+            assert sm == pytest.approx(m[k], rel=1e-12)
+    for a in g:
+        assert np.all(np.isfinite(a))
+    assert np.abs(g[0]).max() > 0 and np.abs(g[1]).max() > 0
