@@ -11,7 +11,9 @@ oracle at that size:
     the FP32 strips, which are within 2e-6 of FP64 on this grid;
   * the 401 x 301 batch with each optimal-transport misfit: the gradient
     completes with finite images and its per-shot misfits equal the
-    standalone OT misfit of the same traces.
+    standalone OT misfit of the same traces;
+  * the 401 x 301 batch's stage-2 Born gradient and density image: FAST
+    within its bar of EXACT, batch = alone bitwise.
 """
 import numpy as np
 import pytest
@@ -116,3 +118,68 @@ def test_shot_batch_full_size_ot_misfits(misfit, variant):
     for a in g:
         assert np.all(np.isfinite(a))
     assert np.abs(g[0]).max() > 0 and np.abs(g[1]).max() > 0
+
+
+def test_shot_batch_full_size_born_gradient():
+    """The stage-2 Born gradient on the 401 x 301 batch (nt 3000), 3 shots,
+    rho := 2000 and adjoint direction -1 as in stage 2, with a smooth model
+    perturbation: FAST within the FAST bar (1e-4) of EXACT, and the batch equal
+    to its shots run one at a time, bit for bit."""
+    sv = W.shot_batch(nshots=3)
+    w = W.ricker(sv.nt, sv.dt, sv.f0)
+    kw = dict(rho_const=2000.0, adjoint_dir=-1)
+    with _ctx(sv, 3, **kw) as ctx:
+        ctx.set_wavelet(w)
+        ctx.set_model(sv.vp, sv.vs, sv.rho)
+        obs = ctx.batch_forward(_shots(sv))
+    shots = _shots(sv, obs)
+    dvp = (0.5 * (sv.vp - sv.vp0)).astype(np.float32)
+    dvs = (0.5 * (sv.vs - sv.vs0)).astype(np.float32)
+    res = {}
+    for mode in ("exact", "fast"):
+        with _ctx(sv, 3, mode=mode, **kw) as ctx:
+            ctx.set_wavelet(w)
+            ctx.set_model(sv.vp0, sv.vs0, sv.rho0)
+            ctx.set_perturbation(dvp, dvs)
+            m = ctx.batch_gradient(shots, born=True)
+            g = ctx.flush()
+            if mode == "fast":
+                alone = [ctx.batch_gradient([s], born=True)[0] for s in shots]
+                g1 = ctx.flush()
+                assert np.array_equal(m, np.array(alone))
+                for a, b in zip(g, g1):
+                    assert np.array_equal(a, b)
+        res[mode] = (m, g)
+    (me, ge), (mf, gf) = res["exact"], res["fast"]
+    assert np.abs(ge[0]).max() > 0
+    for a, b in zip(gf, ge):
+        assert rel_l2(a, b) <= 1e-4
+    assert np.all(np.abs(mf - me) <= 1e-4 * np.abs(me))
+
+
+def test_shot_batch_full_size_density_gradient():
+    """The density image on the 401 x 301 batch (nt 3000), 3 shots: FAST (the
+    column-march path with stored derivatives) within 1e-3 of EXACT for the
+    density and 1e-4 for Vp / Vs, and the batch equal to its shots run one at
+    a time, bit for bit."""
+    sv = W.shot_batch(nshots=3)
+    w = W.ricker(sv.nt, sv.dt, sv.f0)
+    shots = _shots(sv, _records(sv, w))
+    res = {}
+    for mode in ("exact", "fast"):
+        with _ctx(sv, 3, mode=mode, grad_rho=True) as ctx:
+            ctx.set_wavelet(w)
+            ctx.set_model(sv.vp0, sv.vs0, sv.rho0)
+            ctx.batch_gradient(shots)
+            g = list(ctx.flush()) + [ctx.flush_rho()]
+            if mode == "fast":
+                for s in shots:
+                    ctx.batch_gradient([s])
+                g1 = list(ctx.flush()) + [ctx.flush_rho()]
+                for a, b in zip(g, g1):
+                    assert np.array_equal(a, b)
+        res[mode] = g
+    ge, gf = res["exact"], res["fast"]
+    assert np.abs(ge[3]).max() > 0
+    assert rel_l2(gf[0], ge[0]) <= 1e-4 and rel_l2(gf[1], ge[1]) <= 1e-4
+    assert rel_l2(gf[3], ge[3]) <= 1e-3
