@@ -125,6 +125,9 @@ def main():
                     help="run the serving-loop benchmark (Poisson request stream) instead")
     ap.add_argument("--uneven", action="store_true",
                     help="run the uneven-batch benchmark (120 shots over four grids, autotuned, BF16 strips)")
+    ap.add_argument("--inversion", type=int, default=0, metavar="NITER",
+                    help="run NITER stage-1 iterations of the sfwi_stage1 program on the reference's example-1 "
+                         "model (376x201, 38 shots) from synthetic SU records instead")
     ap.add_argument("--requests", type=int, default=24)
     ap.add_argument("--rate", type=float, default=1.0, help="serving: requests per second")
     ap.add_argument("--scale", type=float, default=1.0, help="serving: request size factor")
@@ -150,6 +153,8 @@ def main():
         return serving(args)
     if args.uneven:
         return uneven(args)
+    if args.inversion:
+        return inversion(args)
 
     kw = {}
     if args.shots:
@@ -361,6 +366,80 @@ def _run_alone(r):
             else:
                 ctx.born_gradient(s)
         ctx.flush()
+
+
+def inversion(args):
+    """Stage-1 inversion on the reference's example-1 model (example-1/
+    vel_sync_376x201.bin as the true model, _init.bin as the start, vs = 0,
+    rho := 2000 as stage 1 forces it; the parameter file of example-1/
+    para_stage1 with its 10 m grid, 50-cell PML and 10 Hz wavelet).  The
+    example's SU records are not part of the reference (Git LFS pointers), so
+    the records of 38 shots evenly spaced across the model, each recorded by
+    376 receivers on the surface, come from forward solves of the true model
+    (nt 2000, dt 1 ms) and are written in the reference's SU layout; then the
+    sfwi_stage1 program runs NITER iterations on those files."""
+    import subprocess
+    import tempfile
+    from sfwi_amd import _lib
+    from sfwi_amd import survey_io as S
+    from sfwi_amd.elastic import Context, Shot
+    here = os.path.dirname(os.path.abspath(__file__))
+    gold = os.path.join(here, "tests", "golden")
+    allnx, allnz, dx, pml, nt, dt, f0, nshots = 376, 201, 10.0, 50, 2000, 0.001, 10.0, 38
+    vp = np.fromfile(os.path.join(gold, "example1_vel_sync_376x201.bin"), np.float32).reshape(allnx, allnz)
+    vp0 = np.fromfile(os.path.join(gold, "example1_vel_sync_376x201_init.bin"), np.float32).reshape(allnx, allnz)
+    vs = np.zeros_like(vp)
+    rho = np.full_like(vp, 2000.0)
+    w = _lib.ricker(nt, np.float32(dt), f0)
+    gx = (np.arange(allnx) * dx).astype(np.int32)
+    sxs = np.round(np.linspace(5, allnx - 6, nshots)).astype(int) * int(dx)
+    tmp = tempfile.mkdtemp(prefix="sfwi_inv_")
+    t0 = time.perf_counter()
+    with Context(dx=dx, dz=dx, dt=dt, nx=allnx, nz=allnz, pml=pml, nt=nt, ntr=allnx, allnx=allnx, allnz=allnz,
+                 max_batch=nshots, rho_const=2000.0) as ctx:
+        ctx.set_wavelet(w)
+        ctx.set_model(vp, vs, rho)
+        shots = [Shot(0, pml + int(sx / dx), pml, pml, 0, gx) for sx in sxs]
+        oz, ox = ctx.batch_forward(shots)
+    for k, sx in enumerate(sxs):
+        S.write_su(os.path.join(tmp, "z.su"), k + 1, int(sx), gx, oz[k], dt, append=k > 0)
+        S.write_su(os.path.join(tmp, "x.su"), k + 1, int(sx), gx, ox[k], dt, append=k > 0)
+    vp0.tofile(os.path.join(tmp, "vp.bin"))
+    vs.tofile(os.path.join(tmp, "vs.bin"))
+    para = os.path.join(tmp, "para")
+    with open(para, "w") as f:
+        f.write("10 10 %d %d %d 1 %d\n%s\n%s\n%s\n%s\n%s\n%s\n%g %d 0\n0 0\n1 0 0 1000\n" % (
+            pml, allnx, allnz, nshots, os.path.join(tmp, "vp.bin"), os.path.join(tmp, "vs.bin"),
+            os.path.join(tmp, "vs.bin"), os.path.join(tmp, "z.su"), os.path.join(tmp, "x.su"),
+            os.path.join(tmp, "out"), f0, args.inversion))
+    t_data = time.perf_counter() - t0
+    print("bench: example-1 records (%d shots, nt %d) written in %.1f s" % (nshots, nt, t_data),
+          file=sys.stderr, flush=True)
+    prog = os.path.join(here, "sfwi_amd", "sfwi_stage1")
+    t0 = time.perf_counter()
+    p = subprocess.Popen([prog, para, str(args.inversion), str(nshots)], stdout=subprocess.PIPE,
+                         stderr=subprocess.PIPE, text=True)
+    its = []
+    for line in p.stdout:
+        f = line.split()
+        its.append({"iter": int(f[1]), "misfit": float(f[3]), "beta": float(f[5]), "alpha": float(f[7]),
+                    "t_s": round(time.perf_counter() - t0, 3)})
+        print("bench: " + line.strip(), file=sys.stderr, flush=True)
+    err = p.stderr.read()
+    if p.wait() != 0:
+        print(err, file=sys.stderr)
+        sys.exit(3)
+    total = time.perf_counter() - t0
+    per = total / max(len(its), 1)
+    print(json.dumps({
+        "metric": "stage-1 inversion iterations per second (example-1 model)", "value": round(1.0 / per, 4),
+        "unit": "iterations/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32",
+        "data": "the reference's example-1 true and initial Vp (vs 0, rho 2000); synthetic SU records of "
+                "38 shots from forward solves of the true model",
+        "config": {"workload": "sfwi_stage1 on example-1: 376x201 dx=dz=10m, pml 50, nt 2000, dt 1ms, "
+                               "38 shots x 376 receivers, 10 Hz, %d iterations" % len(its)},
+        "s_per_iteration": round(per, 3), "iterations": its,
+        "misfit_ratio": round(its[-1]["misfit"] / its[0]["misfit"], 6) if its else None}), flush=True)
 
 
 def uneven(args):

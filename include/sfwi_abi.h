@@ -165,6 +165,14 @@ int sfwi_shot_born_forward(sfwi_ctx *ctx, const sfwi_shot *shot, float *b_z, flo
  * forward solves at vel, vel - step1*dir and vel - step2*dir. */
 int sfwi_linesearch_parabola(sfwi_ctx *ctx, const sfwi_shot *shot, const float *dir_vp,
                              const float *dir_vs, float step1, float step2, double m[3]);
+/* The same line search over a list of shots (the workers' per-shot
+ * CALCULATE_ALPHA_parabola_mb calls, sfwi-stage1/main.cpp:1180, over the
+ * iteration's shots): each test model is built once and its windows once per window
+ * position, the forward solves run in batches of max_batch, and the per-shot
+ * misfits are added to m[0..2] in shot order, so the result equals nshots
+ * calls of sfwi_linesearch_parabola bit for bit. */
+int sfwi_batch_linesearch_parabola(sfwi_ctx *ctx, int nshots, const sfwi_shot *shots, const float *dir_vp,
+                                   const float *dir_vs, float step1, float step2, double m[3]);
 /* Stage-2 exact step, CALCULATE_ALPHA (sfwi-stage2/FD2DGPU.cu:4576-4722,
  * declared sfwi-stage2/FD2DGPU.cuh:457-458): fenzi += sum L(cg).r,
  * fenmu += sum L(cg)^2 with r from the current perturbation. */

@@ -303,7 +303,8 @@ int receiver_tables(sfwi_ctx *c, const sfwi_shot &s, int k, Slot &S) {
 }
 
 // Prepare n shots in slots 0..n-1: medium windows, tables, records, zeroed state.
-int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium *override_m) {
+int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium *override_m,
+            const std::map<int, Window> *override_w = nullptr) {
     const bool born = kind == K_BORN_FWD || kind == K_BORN_GRAD;
     const bool grad = kind == K_GRAD || kind == K_BORN_GRAD;
     const bool obs = grad || kind == K_FWD_MISFIT;
@@ -323,6 +324,11 @@ int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium 
         Slot &S = c->slots[k];
         if ((r = receiver_tables(c, s, k, S))) return r;
         const Medium *m = override_m;
+        if (!m && override_w) {   // a medium per window position (batched line search)
+            const auto ow = override_w->find(s.left);
+            if (ow == override_w->end()) return fail(c, SFWI_EINVAL, "shot %d: no test-model window", k);
+            m = &ow->second.m;
+        }
         if (!m && (r = model_window(c, s.left, born, &m))) return r;
         DevShot d{};
         d.m = *m;
@@ -968,6 +974,53 @@ int sfwi_linesearch_parabola(sfwi_ctx *c, const sfwi_shot *shot, const float *di
     }
     hipFree(w.base);
     return 0;
+}
+
+int sfwi_batch_linesearch_parabola(sfwi_ctx *c, int nshots, const sfwi_shot *shots, const float *dir_vp,
+                                   const float *dir_vs, float step1, float step2, double m[3]) {
+    if (!c || (nshots > 0 && !shots) || nshots < 0 || !dir_vp || !dir_vs || !m)
+        return fail(c, SFWI_EINVAL, "NULL argument or negative shot count");
+    if (!c->have_model) return fail(c, SFWI_ESTATE, "model not set");
+    if (nshots == 0) return 0;
+    const size_t b = c->nmodel * sizeof(float);
+    float *tvp = c->tmp_model, *tvs = tvp + c->nmodel, *dvp = tvp + 2 * c->nmodel, *dvs = tvp + 3 * c->nmodel;
+    HIPCK(c, hipMemcpyAsync(dvp, dir_vp, b, hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipMemcpyAsync(dvs, dir_vs, b, hipMemcpyHostToDevice, c->stream));
+    const float steps[3] = {0.0f, step1, step2};
+    std::map<int, Window> tw;   // one window of each test model per window position in the shot list
+    auto done = [&](int r) {
+        hipStreamSynchronize(c->stream);
+        for (auto &kv : tw) hipFree(kv.second.base);
+        return r;
+    };
+    std::vector<double> v(c->p.max_batch);
+    for (int k = 0; k < 3; ++k) {
+        const float *mp = c->mvp, *ms = c->mvs;
+        if (k > 0) {
+            const dim3 grd((unsigned)((c->nmodel + 255) / 256));
+            hipLaunchKernelGGL(k_axpy_model, grd, dim3(256), 0, c->stream, (int)c->nmodel, c->mvp, dvp, steps[k], tvp);
+            hipLaunchKernelGGL(k_axpy_model, grd, dim3(256), 0, c->stream, (int)c->nmodel, c->mvs, dvs, steps[k], tvs);
+            mp = tvp;
+            ms = tvs;
+        }
+        std::map<int, bool> built;
+        for (int s = 0; s < nshots; ++s) {
+            if (built.count(shots[s].left)) continue;
+            built[shots[s].left] = true;
+            if (int r = build_window(c, tw[shots[s].left], shots[s].left, mp, ms, c->mrho, nullptr, nullptr))
+                return done(r);
+        }
+        for (int b0 = 0; b0 < nshots; b0 += c->p.max_batch) {
+            const int n = std::min(c->p.max_batch, nshots - b0);
+            int r = prepare(c, K_FWD_MISFIT, n, shots + b0, nullptr, &tw);
+            if (r || (r = run(c, K_FWD_MISFIT, n))) return done(r);
+            if (hipMemcpyAsync(v.data(), c->d_out, n * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+                return done(fail(c, SFWI_EHIP, "copy failed"));
+            if ((r = finish(c))) return done(r);
+            for (int j = 0; j < n; ++j) m[k] += v[j];   // shot order, as the per-shot calls add
+        }
+    }
+    return done(0);
 }
 
 int sfwi_linesearch_linear(sfwi_ctx *c, const sfwi_shot *shot, const float *cg_vp, const float *cg_vs,

@@ -157,6 +157,16 @@ class Context:
                                                  m.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return m
 
+    def batch_linesearch_parabola(self, shots, dir_vp, dir_vs, step1, step2, m=None):
+        """linesearch_parabola over a list of shots in one call (batched forward
+        solves; equal to the per-shot calls in shot order, bitwise)."""
+        m = np.zeros(3, np.float64) if m is None else np.array(m, np.float64)
+        arr = (SfwiShot * len(shots))(*[s.c for s in shots])
+        self._ck(self.L.sfwi_batch_linesearch_parabola(self.h, len(shots), arr, _f32(dir_vp), _f32(dir_vs),
+                                                       step1, step2,
+                                                       m.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return m
+
     def linesearch_linear(self, shot, cg_vp, cg_vs, fenzi=0.0, fenmu=0.0):
         """This shot's terms added to (fenzi, fenmu); returns the sums."""
         a, b = ctypes.c_double(fenzi), ctypes.c_double(fenmu)

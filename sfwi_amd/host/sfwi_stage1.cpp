@@ -167,11 +167,13 @@ int main(int argc, char **argv) {
                                  &beta),
                nullptr);
             double ls[3] = {0.0, 0.0, 0.0};
-            for (size_t gi = 0; gi < widths.size(); ++gi)
-                for (int k : members[widths[gi]])
-                    ck(sfwi_linesearch_parabola(ctxs[gi], &shots[k].s, dir_pp.data(), dir_ps.data(), step1, step2,
-                                                ls),
-                       ctxs[gi]);
+            for (size_t gi = 0; gi < widths.size(); ++gi) {   // batched; the sums of per-shot calls in shot order
+                std::vector<sfwi_shot> grp;
+                for (int k : members[widths[gi]]) grp.push_back(shots[k].s);
+                ck(sfwi_batch_linesearch_parabola(ctxs[gi], (int)grp.size(), grp.data(), dir_pp.data(), dir_ps.data(),
+                                                  step1, step2, ls),
+                   ctxs[gi]);
+            }
             const double alpha = sfwi_s1_alpha(ls[0], ls[1], ls[2], step1, step2);
             ck(sfwi_s1_update(p.allnx, p.allnz, cut_layer, alpha, dir_pp.data(), dir_ps.data(), vp.data(), vs.data()),
                nullptr);

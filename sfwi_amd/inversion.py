@@ -95,9 +95,8 @@ class Stage1(_CG):
             G = g if G is None else tuple((a + b).astype(np.float32) for a, b in zip(G, g))
         d1, d2, beta = self.direction(*G)
         m = np.zeros(3)
-        for ctx, shots in groups:
-            for s in shots:
-                m = ctx.linesearch_parabola(s, d1, d2, self.step1, self.step2, m)
+        for ctx, shots in groups:   # batched forward solves; the same sums as per-shot calls in shot order
+            m = ctx.batch_linesearch_parabola(shots, d1, d2, self.step1, self.step2, m)
         alpha = self.alpha(*m)
         self.update(velp, vels, d1, d2, alpha)
         rec = dict(iter=self.iter, misfit=misfit, rms=self._rms(misfit), beta=beta, alpha=alpha,

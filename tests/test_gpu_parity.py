@@ -418,3 +418,32 @@ def test_overlapping_windows_batch(depth, order):
         assert np.array_equal(ma, mb)
         for x, y in zip(ga, gb):
             assert np.array_equal(x, y), "grad_rho %s: max |d| = %g" % (grad_rho, np.abs(x - y).max())
+
+
+def test_batch_linesearch_equals_per_shot_calls():
+    """sfwi_batch_linesearch_parabola over shots with overlapping windows at
+    three offsets, in batches of 2 + 1 (max_batch 2), equals the per-shot
+    sfwi_linesearch_parabola calls in shot order, bit for bit, in both modes;
+    a non-zero starting m is added to."""
+    sv, ntr = _overlap_case()
+    w = W.ricker(sv.nt, sv.dt, sv.f0)
+    lib = [lib_shot(sv, s, *_observed(sv, s, w)) for s in sv.shots]
+    rng = np.random.default_rng(5)
+    d1 = (rng.standard_normal(sv.vp.shape) * 0.1).astype(np.float32)
+    d2 = (rng.standard_normal(sv.vp.shape) * 0.05).astype(np.float32)
+    for mode in ("fast", "exact"):
+        with context(sv, mode, max_batch=2, ntr=ntr) as ctx:
+            ctx.set_wavelet(w)
+            ctx.set_model(sv.vp0, sv.vs0, sv.rho0)
+            per0 = np.zeros(3)
+            for s in lib:
+                per0 = ctx.linesearch_parabola(s, d1, d2, 30.0, 150.0, per0)
+            m0 = 0.5 * per0                          # a starting m on the misfits' scale
+            per = m0.copy()
+            for s in lib:
+                per = ctx.linesearch_parabola(s, d1, d2, 30.0, 150.0, per)
+            bat0 = ctx.batch_linesearch_parabola(lib, d1, d2, 30.0, 150.0)
+            bat = ctx.batch_linesearch_parabola(lib, d1, d2, 30.0, 150.0, m0)
+        assert np.all(per0 > 0)
+        assert np.array_equal(per0, bat0), (mode, per0, bat0)
+        assert np.array_equal(per, bat), (mode, per, bat)
