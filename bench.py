@@ -126,8 +126,10 @@ def main():
     ap.add_argument("--uneven", action="store_true",
                     help="run the uneven-batch benchmark (120 shots over four grids, autotuned, BF16 strips)")
     ap.add_argument("--inversion", type=int, default=0, metavar="NITER",
-                    help="run NITER stage-1 iterations of the sfwi_stage1 program on the reference's example-1 "
-                         "model (376x201, 38 shots) from synthetic SU records instead")
+                    help="run NITER inversion iterations of the sfwi_stage1 (or, with --stage 2, sfwi_stage2) "
+                         "program on the reference's example-1 model (376x201, 38 shots) from synthetic SU "
+                         "records instead")
+    ap.add_argument("--stage", type=int, default=1, choices=(1, 2), help="inversion: which stage's program")
     ap.add_argument("--requests", type=int, default=24)
     ap.add_argument("--rate", type=float, default=1.0, help="serving: requests per second")
     ap.add_argument("--scale", type=float, default=1.0, help="serving: request size factor")
@@ -369,15 +371,16 @@ def _run_alone(r):
 
 
 def inversion(args):
-    """Stage-1 inversion on the reference's example-1 model (example-1/
-    vel_sync_376x201.bin as the true model, _init.bin as the start, vs = 0,
-    rho := 2000 as stage 1 forces it; the parameter file of example-1/
-    para_stage1 with its 10 m grid, 50-cell PML and 10 Hz wavelet).  The
-    example's SU records are not part of the reference (Git LFS pointers), so
-    the records of 38 shots evenly spaced across the model, each recorded by
-    376 receivers on the surface, come from forward solves of the true model
-    (nt 2000, dt 1 ms) and are written in the reference's SU layout; then the
-    sfwi_stage1 program runs NITER iterations on those files."""
+    """Stage-1 (or stage-2) inversion on the reference's example-1 model
+    (example-1/vel_sync_376x201.bin as the true model, _init.bin as the start
+    or, for stage 2, the background; vs = 0, rho := 2000 as both stages force
+    it; the parameter file of example-1/para_stage1 with its 10 m grid,
+    50-cell PML and 10 Hz wavelet).  The example's SU records are not part of
+    the reference (Git LFS pointers), so the records of 38 shots evenly spaced
+    across the model, each recorded by 376 receivers on the surface, come from
+    forward solves of the true model (nt 2000, dt 1 ms) and are written in the
+    reference's SU layout; then the sfwi_stage1 (sfwi_stage2) program runs
+    NITER iterations on those files."""
     import subprocess
     import tempfile
     from sfwi_amd import _lib
@@ -415,7 +418,7 @@ def inversion(args):
     t_data = time.perf_counter() - t0
     print("bench: example-1 records (%d shots, nt %d) written in %.1f s" % (nshots, nt, t_data),
           file=sys.stderr, flush=True)
-    prog = os.path.join(here, "sfwi_amd", "sfwi_stage1")
+    prog = os.path.join(here, "sfwi_amd", "sfwi_stage%d" % args.stage)
     t0 = time.perf_counter()
     p = subprocess.Popen([prog, para, str(args.inversion), str(nshots)], stdout=subprocess.PIPE,
                          stderr=subprocess.PIPE, text=True)
@@ -424,6 +427,8 @@ def inversion(args):
         f = line.split()
         its.append({"iter": int(f[1]), "misfit": float(f[3]), "beta": float(f[5]), "alpha": float(f[7]),
                     "t_s": round(time.perf_counter() - t0, 3)})
+        if args.stage == 2:
+            its[-1].update(fenzi=float(f[9]), fenmu=float(f[11]))
         print("bench: " + line.strip(), file=sys.stderr, flush=True)
     err = p.stderr.read()
     if p.wait() != 0:
@@ -432,12 +437,13 @@ def inversion(args):
     total = time.perf_counter() - t0
     per = total / max(len(its), 1)
     print(json.dumps({
-        "metric": "stage-1 inversion iterations per second (example-1 model)", "value": round(1.0 / per, 4),
+        "metric": "stage-%d inversion iterations per second (example-1 model)" % args.stage,
+        "value": round(1.0 / per, 4),
         "unit": "iterations/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32",
         "data": "the reference's example-1 true and initial Vp (vs 0, rho 2000); synthetic SU records of "
                 "38 shots from forward solves of the true model",
-        "config": {"workload": "sfwi_stage1 on example-1: 376x201 dx=dz=10m, pml 50, nt 2000, dt 1ms, "
-                               "38 shots x 376 receivers, 10 Hz, %d iterations" % len(its)},
+        "config": {"workload": "sfwi_stage%d on example-1: 376x201 dx=dz=10m, pml 50, nt 2000, dt 1ms, "
+                               "38 shots x 376 receivers, 10 Hz, %d iterations" % (args.stage, len(its))},
         "s_per_iteration": round(per, 3), "iterations": its,
         "misfit_ratio": round(its[-1]["misfit"] / its[0]["misfit"], 6) if its else None}), flush=True)
 

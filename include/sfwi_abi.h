@@ -178,6 +178,15 @@ int sfwi_batch_linesearch_parabola(sfwi_ctx *ctx, int nshots, const sfwi_shot *s
  * fenmu += sum L(cg)^2 with r from the current perturbation. */
 int sfwi_linesearch_linear(sfwi_ctx *ctx, const sfwi_shot *shot, const float *cg_vp,
                            const float *cg_vs, double *fenzi, double *fenmu);
+/* The stage-2 driver's loop of CALCULATE_ALPHA calls over its shots
+ * (sfwi-stage2/main.cpp:766-859) as batched solves: both Born solves run over
+ * batches of up to max_batch shots, one window of the direction per window
+ * position.  Each shot's per-trace FP64 products are added to fenzi / fenmu in
+ * shot order, traces in trace order, so the result equals nshots calls of
+ * sfwi_linesearch_linear bit for bit.  The observed traces of shot i of a
+ * batch stay resident in its slot, as after a gradient call. */
+int sfwi_batch_linesearch_linear(sfwi_ctx *ctx, int nshots, const sfwi_shot *shots, const float *cg_vp,
+                                 const float *cg_vs, double *fenzi, double *fenmu);
 
 /* Gradient flush, FD2DGPU_ELASTIC::imagebuffer_resettozero
  * (sfwi-stage1/FD2DGPU.cu:3646-3673): copies the accumulated Vp and Vs

@@ -107,6 +107,7 @@ def load():
     L.sfwi_batch_linesearch_parabola.argtypes = [vp, ctypes.c_int, P(SfwiShot), _f32p, _f32p, ctypes.c_float,
                                                  ctypes.c_float, dp]
     L.sfwi_linesearch_linear.argtypes = [vp, P(SfwiShot), _f32p, _f32p, dp, dp]
+    L.sfwi_batch_linesearch_linear.argtypes = [vp, ctypes.c_int, P(SfwiShot), _f32p, _f32p, dp, dp]
     L.sfwi_flush.argtypes = [vp, vp, vp, vp]
     L.sfwi_flush_rho.argtypes = [vp, vp]
     L.sfwi_ot_misfit.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,

@@ -65,6 +65,9 @@ struct sfwi_ctx {
     float *tmp_model = nullptr;   // 4 full-model scratch arrays (line-search models)
     float *Gvp = nullptr, *Gvs = nullptr, *Gil = nullptr, *Grho = nullptr;
     float *res_keep = nullptr;    // 2 record arrays: residual kept across two Born solves
+    float *res_batch = nullptr;   // the same for each shot of a batched linear line search
+    double *dot_batch = nullptr;  // its per-trace FP64 sums
+    size_t res_batch_cap = 0, dot_batch_cap = 0;
     std::vector<Slot> slots;
     DevShot *d_shots = nullptr;
     std::vector<DevShot> h_shots;
@@ -850,7 +853,7 @@ void sfwi_ctx_destroy(sfwi_ctx *c) {
     drop_windows(c);
     for (hipEvent_t e : c->ev) hipEventDestroy(e);
     hipFree(c->wav); hipFree(c->mvp); hipFree(c->tmp_model); hipFree(c->Gvp); hipFree(c->res_keep);
-    hipFree(c->d_shots); hipFree(c->d_out); hipFree(c->d_num);
+    hipFree(c->d_shots); hipFree(c->d_out); hipFree(c->d_num); hipFree(c->res_batch); hipFree(c->dot_batch);
     hipFree(c->ot_jobs); hipFree(c->ot_ws); hipFree(c->ot_mis); hipFree(c->ot_first); hipFree(c->ot_io);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -1034,6 +1037,7 @@ int sfwi_linesearch_linear(sfwi_ctx *c, const sfwi_shot *shot, const float *cg_v
     const size_t rb = (size_t)shot->ntr * c->p.nt * sizeof(float);
     HIPCK(c, hipMemcpyAsync(S.oz, shot->obs_z, rb, hipMemcpyHostToDevice, c->stream));
     HIPCK(c, hipMemcpyAsync(S.ox, shot->obs_x, rb, hipMemcpyHostToDevice, c->stream));
+    S.obs_ntr = shot->ntr;   // slot 0 now holds this shot's records
     hipLaunchKernelGGL(k_residual<true>, dim3(256, 1), dim3(BLK), 0, c->stream, c->g, c->d_shots);
     HIPCK(c, hipMemcpyAsync(c->res_keep, S.rz, rb, hipMemcpyDeviceToDevice, c->stream));
     HIPCK(c, hipMemcpyAsync(c->res_keep + c->nrec, S.rx, rb, hipMemcpyDeviceToDevice, c->stream));
@@ -1059,6 +1063,81 @@ int sfwi_linesearch_linear(sfwi_ctx *c, const sfwi_shot *shot, const float *cg_v
     // The per-trace FP64 partial sums are added in trace order.
     for (int tr = 0; tr < shot->ntr; ++tr) { *fenzi += num[tr]; *fenmu += den[tr]; }
     return 0;
+}
+
+int sfwi_batch_linesearch_linear(sfwi_ctx *c, int nshots, const sfwi_shot *shots, const float *cg_vp,
+                                 const float *cg_vs, double *fenzi, double *fenmu) {
+    if (!c || (nshots > 0 && !shots) || nshots < 0 || !cg_vp || !cg_vs || !fenzi || !fenmu)
+        return fail(c, SFWI_EINVAL, "NULL argument or negative shot count");
+    if (!c->have_model) return fail(c, SFWI_ESTATE, "model not set");
+    if (!c->have_pert) return fail(c, SFWI_ESTATE, "perturbation not set (sfwi_set_perturbation)");
+    for (int s = 0; s < nshots; ++s)
+        if (!shots[s].obs_z || !shots[s].obs_x) return fail(c, SFWI_EINVAL, "shot %d: observed traces missing", s);
+    if (nshots == 0) return 0;
+    const int nb = std::min(c->p.max_batch, nshots);
+    // residuals of a batch ([2 nb][nrec]: z, x per slot) and per-trace sums ([2 nb][ntr])
+    int r = grow(c, (void **)&c->res_batch, &c->res_batch_cap, 2 * (size_t)nb * c->nrec * sizeof(float));
+    if (r || (r = grow(c, (void **)&c->dot_batch, &c->dot_batch_cap, 2 * (size_t)nb * c->p.ntr * sizeof(double))))
+        return r;
+    const size_t b = c->nmodel * sizeof(float);
+    float *cvp = c->tmp_model + 2 * c->nmodel, *cvs = c->tmp_model + 3 * c->nmodel;
+    HIPCK(c, hipMemcpyAsync(cvp, cg_vp, b, hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipMemcpyAsync(cvs, cg_vs, b, hipMemcpyHostToDevice, c->stream));
+    std::map<int, Window> tw;   // one window of the direction per window position in the shot list
+    auto done = [&](int rc) {
+        hipStreamSynchronize(c->stream);
+        for (auto &kv : tw) hipFree(kv.second.base);
+        return rc;
+    };
+    for (int s = 0; s < nshots; ++s) {
+        if (tw.count(shots[s].left)) continue;
+        if ((r = build_window(c, tw[shots[s].left], shots[s].left, c->mvp, c->mvs, c->mrho, cvp, cvs)))
+            return done(r);
+    }
+    std::vector<double> sums(2 * (size_t)nb * c->p.ntr);
+    for (int b0 = 0; b0 < nshots; b0 += nb) {
+        const int n = std::min(nb, nshots - b0);
+        // Solve 1 over the batch: residual r = born(dm) - obs + background, kept per slot.
+        if ((r = prepare(c, K_BORN_FWD, n, shots + b0, nullptr)) || (r = run(c, K_BORN_FWD, n))) return done(r);
+        for (int k = 0; k < n; ++k) {
+            Slot &S = c->slots[k];
+            const size_t rb = (size_t)shots[b0 + k].ntr * c->p.nt * sizeof(float);
+            if (hipMemcpyAsync(S.oz, shots[b0 + k].obs_z, rb, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+                hipMemcpyAsync(S.ox, shots[b0 + k].obs_x, rb, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+                return done(fail(c, SFWI_EHIP, "copy failed"));
+            S.obs_ntr = shots[b0 + k].ntr;
+        }
+        hipLaunchKernelGGL(k_residual<true>, dim3(256, n), dim3(BLK), 0, c->stream, c->g, c->d_shots);
+        for (int k = 0; k < n; ++k) {
+            Slot &S = c->slots[k];
+            const size_t rb = (size_t)shots[b0 + k].ntr * c->p.nt * sizeof(float);
+            float *keep = c->res_batch + 2 * (size_t)k * c->nrec;
+            if (hipMemcpyAsync(keep, S.rz, rb, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+                hipMemcpyAsync(keep + c->nrec, S.rx, rb, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+                return done(fail(c, SFWI_EHIP, "copy failed"));
+        }
+        // Solve 2 over the batch: L(cg), then the per-trace products of each shot.
+        if ((r = prepare(c, K_BORN_FWD, n, shots + b0, nullptr, &tw)) || (r = run(c, K_BORN_FWD, n)))
+            return done(r);
+        for (int k = 0; k < n; ++k) {
+            Slot &S = c->slots[k];
+            const int ntr = shots[b0 + k].ntr;
+            const float *keep = c->res_batch + 2 * (size_t)k * c->nrec;
+            double *num = c->dot_batch + 2 * (size_t)k * c->p.ntr;
+            hipLaunchKernelGGL(k_trace_dot, dim3((ntr + 63) / 64), dim3(64), 0, c->stream, c->g, ntr, S.rz, S.rx,
+                               keep, keep + c->nrec, num, num + c->p.ntr);
+        }
+        if (hipMemcpyAsync(sums.data(), c->dot_batch, 2 * (size_t)n * c->p.ntr * sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+            return done(fail(c, SFWI_EHIP, "copy failed"));
+        if ((r = finish(c))) return done(r);
+        // shot order, each shot's traces in trace order: the per-shot calls' sums
+        for (int k = 0; k < n; ++k) {
+            const double *num = sums.data() + 2 * (size_t)k * c->p.ntr, *den = num + c->p.ntr;
+            for (int tr = 0; tr < shots[b0 + k].ntr; ++tr) { *fenzi += num[tr]; *fenmu += den[tr]; }
+        }
+    }
+    return done(0);
 }
 
 int sfwi_flush_rho(sfwi_ctx *c, float *grho) {

@@ -174,6 +174,15 @@ class Context:
                                                ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
+    def batch_linesearch_linear(self, shots, cg_vp, cg_vs, fenzi=0.0, fenmu=0.0):
+        """linesearch_linear over a list of shots in one call (batched Born
+        solves; equal to the per-shot calls in shot order, bitwise)."""
+        a, b = ctypes.c_double(fenzi), ctypes.c_double(fenmu)
+        arr = (SfwiShot * len(shots))(*[s.c for s in shots])
+        self._ck(self.L.sfwi_batch_linesearch_linear(self.h, len(shots), arr, _f32(cg_vp), _f32(cg_vs),
+                                                     ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
     def flush(self):
         sh = (self.allnx, self.allnz)
         g = [np.zeros(sh, np.float32) for _ in range(3)]

@@ -19,48 +19,12 @@
 // and, as the reference writes them, <outdir>/misfit.txt lines and
 // <outdir>/vel_p_<k>_<allnx>_<allnz>.dat, vel_s_<k>_<allnx>_<allnz>.dat.
 // Exit status: 0 success, 2 input error, 3 library error.
-#include <sys/stat.h>
-
 #include <cstdio>
 #include <cstdlib>
-#include <map>
-#include <stdexcept>
-#include <string>
-#include <vector>
 
-#include "sfwi_abi.h"
+#include "survey_in.hpp"
 
-namespace {
-
-struct Fail : std::runtime_error {
-    int status;
-    Fail(int s, const std::string &m) : std::runtime_error(m), status(s) {}
-};
-
-void ck(int rc, const sfwi_ctx *ctx) {
-    if (rc != SFWI_OK) throw Fail(3, std::string("sfwi: ") + sfwi_last_error(ctx));
-}
-void ck_input(int rc) {
-    if (rc != SFWI_OK) throw Fail(2, std::string("input error: ") + sfwi_last_error(nullptr));
-}
-
-struct ShotIn {
-    sfwi_shot s;
-    std::vector<int> gc;
-    std::vector<float> rz, rx;
-    int width;
-};
-
-void write_model(const std::string &path, const std::vector<float> &a) {
-    FILE *f = std::fopen(path.c_str(), "wb");
-    if (!f || std::fwrite(a.data(), sizeof(float), a.size(), f) != a.size()) {
-        if (f) std::fclose(f);
-        throw Fail(2, "cannot write " + path);
-    }
-    std::fclose(f);
-}
-
-}  // namespace
+using namespace sfwi_host;
 
 int main(int argc, char **argv) {
     if (argc < 2 || argc > 5) {
@@ -69,68 +33,19 @@ int main(int argc, char **argv) {
     }
     std::vector<sfwi_ctx *> ctxs;
     try {
-        sfwi_para p;
-        ck_input(sfwi_para_read(argv[1], &p));
+        const Survey sv = read_survey(argv[1]);
+        const sfwi_para &p = sv.p;
         const int niter = argc >= 3 ? std::atoi(argv[2]) : p.maxiter;
         const int max_batch = argc >= 4 ? std::atoi(argv[3]) : 64;
         const int cut_layer = argc >= 5 ? std::atoi(argv[4]) : 26;   // sfwi-stage1/main.cpp:410
         if (niter < 0 || max_batch < 1 || cut_layer < 0)
             throw Fail(2, "input error: niter must be >= 0, max_batch >= 1, cut_layer >= 0");
-        if (p.rbc) throw Fail(2, "input error: rbc (random boundary) is not supported; strips are used");
-        if (p.cpu_mem && p.iointerval < 1) throw Fail(2, "input error: cpu_mem needs iointerval >= 1 (steps per chunk)");
         const size_t nm = (size_t)p.allnx * p.allnz;
-        std::vector<float> vp(nm), vs(nm), rho(nm, 2000.0f);
-        ck_input(sfwi_model_read(p.vp, p.allnx, p.allnz, vp.data()));
-        ck_input(sfwi_model_read(p.vs, p.allnx, p.allnz, vs.data()));
-        // shot index of the record files
-        int nt = 0, ns = 0, nt_x = 0, ns_x = 0;
-        float dt = 0, dt_x = 0;
-        ck_input(sfwi_su_index(p.rec_z, 1, 0, &nt, &dt, &ns, nullptr));
-        std::vector<int> table(9 * (size_t)ns), table_x(9 * (size_t)ns);
-        ck_input(sfwi_su_index(p.rec_z, 1, ns, &nt, &dt, &ns, table.data()));
-        ck_input(sfwi_su_index(p.rec_x, 1, ns, &nt_x, &dt_x, &ns_x, table_x.data()));
-        if (nt_x != nt || ns_x != ns || table_x != table) throw Fail(2, "input error: record files index differently");
-        int all_left = 1 << 30;
-        for (int k = 0; k < ns; ++k) all_left = std::min(all_left, table[9 * k + 4]);
-        std::vector<ShotIn> shots;
-        for (int k = 0; k < ns; ++k) {
-            const int *row = &table[9 * (size_t)k];
-            if (row[0] < p.minshot || row[0] > p.maxshot) continue;
-            ShotIn in;
-            ck_input(sfwi_shot_from_table(row, p.dx, p.pml, p.sy, p.gy, all_left, &in.s, &in.width));
-            in.gc.resize(in.s.ntr);
-            in.rz.resize((size_t)in.s.ntr * nt);
-            in.rx.resize((size_t)in.s.ntr * nt);
-            std::vector<int> gx(in.s.ntr);
-            ck_input(sfwi_su_read_gather(p.rec_z, row[8], in.s.ntr, nt, 1, in.rz.data(), in.gc.data()));
-            ck_input(sfwi_su_read_gather(p.rec_x, row[8], in.s.ntr, nt, 1, in.rx.data(), gx.data()));
-            if (gx != in.gc) throw Fail(2, "input error: receiver coordinates differ between the record files");
-            shots.push_back(std::move(in));
-        }
-        if (shots.empty()) throw Fail(2, "input error: no shot between minshot and maxshot");
-        for (ShotIn &in : shots) {
-            in.s.gc = in.gc.data();
-            in.s.obs_z = in.rz.data();
-            in.s.obs_x = in.rx.data();
-        }
-        // one context per aperture width, in order of first appearance
-        std::vector<int> widths;
-        std::map<int, std::vector<int>> members;
-        int ntr_max = 1;
-        for (size_t k = 0; k < shots.size(); ++k) {
-            if (!members.count(shots[k].width)) widths.push_back(shots[k].width);
-            members[shots[k].width].push_back((int)k);
-            ntr_max = std::max(ntr_max, shots[k].s.ntr);
-        }
-        std::vector<float> wav(nt);
-        sfwi_ricker(wav.data(), nt, dt, p.f0);
-        for (int w : widths) {
-            sfwi_params q = {};
-            q.dx = p.dx; q.dz = p.dz; q.dt = dt; q.nx = w; q.nz = p.allnz; q.pml = p.pml; q.nt = nt; q.nop = 4;
-            q.ntr = ntr_max; q.allnx = p.allnx; q.allnz = p.allnz; q.rr = 0.001f; q.adjoint_dir = 1;
-            q.max_batch = std::min<int>(max_batch, (int)members[w].size());
-            q.cpu_mem = p.cpu_mem ? 1 : 0;      // strips in pinned host memory, iointerval steps per chunk
-            q.strip_chunk = p.cpu_mem ? p.iointerval : 0;
+        std::vector<float> vp = sv.vp, vs = sv.vs, rho(nm, 2000.0f);
+        std::vector<float> wav(sv.nt);
+        sfwi_ricker(wav.data(), sv.nt, sv.dt, p.f0);
+        for (int w : sv.widths) {   // one context per aperture width, in order of first appearance
+            const sfwi_params q = width_params(sv, w, max_batch);
             sfwi_ctx *c = nullptr;
             ck(sfwi_ctx_create(&q, &c), nullptr);
             ctxs.push_back(c);
@@ -144,16 +59,15 @@ int main(int argc, char **argv) {
         const float step1 = 30.0f, step2 = 150.0f;   // sfwi-stage1/main.cpp:373-374
         for (int iter = 0; iter < niter; ++iter) {
             double misfit = 0.0;
-            for (size_t gi = 0; gi < widths.size(); ++gi) {
+            for (size_t gi = 0; gi < sv.widths.size(); ++gi) {
                 sfwi_ctx *c = ctxs[gi];
-                const std::vector<int> &mem = members[widths[gi]];
+                const std::vector<sfwi_shot> grp = group_shots(sv, sv.widths[gi]);
                 ck(sfwi_set_model(c, vp.data(), vs.data(), rho.data()), c);
-                const int nb = std::min<int>(max_batch, (int)mem.size());
-                for (size_t b0 = 0; b0 < mem.size(); b0 += nb) {
-                    std::vector<sfwi_shot> batch;
-                    for (size_t k = b0; k < std::min(mem.size(), b0 + nb); ++k) batch.push_back(shots[mem[k]].s);
-                    std::vector<double> m(batch.size(), 0.0);
-                    ck(sfwi_batch_gradient(c, (int)batch.size(), batch.data(), m.data()), c);
+                const int nb = std::min<int>(max_batch, (int)grp.size());
+                for (size_t b0 = 0; b0 < grp.size(); b0 += nb) {
+                    const int n = (int)std::min<size_t>(nb, grp.size() - b0);
+                    std::vector<double> m(n, 0.0);
+                    ck(sfwi_batch_gradient(c, n, grp.data() + b0, m.data()), c);
                     for (double v : m) misfit += v;
                 }
                 ck(sfwi_flush(c, t[0].data(), t[1].data(), t[2].data()), c);
@@ -167,9 +81,8 @@ int main(int argc, char **argv) {
                                  &beta),
                nullptr);
             double ls[3] = {0.0, 0.0, 0.0};
-            for (size_t gi = 0; gi < widths.size(); ++gi) {   // batched; the sums of per-shot calls in shot order
-                std::vector<sfwi_shot> grp;
-                for (int k : members[widths[gi]]) grp.push_back(shots[k].s);
+            for (size_t gi = 0; gi < sv.widths.size(); ++gi) {   // batched; the sums of per-shot calls in shot order
+                const std::vector<sfwi_shot> grp = group_shots(sv, sv.widths[gi]);
                 ck(sfwi_batch_linesearch_parabola(ctxs[gi], (int)grp.size(), grp.data(), dir_pp.data(), dir_ps.data(),
                                                   step1, step2, ls),
                    ctxs[gi]);
@@ -184,11 +97,8 @@ int main(int argc, char **argv) {
                 std::fprintf(f, "i = %d,alpha = %5.3e\n", iter, alpha);
                 std::fclose(f);
             }
-            char name[256];
-            std::snprintf(name, sizeof(name), "/vel_p_%d_%d_%d.dat", iter, p.allnx, p.allnz);
-            write_model(outdir + name, vp);
-            std::snprintf(name, sizeof(name), "/vel_s_%d_%d_%d.dat", iter, p.allnx, p.allnz);
-            write_model(outdir + name, vs);
+            write_model(out_name(sv, "vel_p", iter), vp);
+            write_model(out_name(sv, "vel_s", iter), vs);
         }
     } catch (const Fail &e) {
         std::fprintf(stderr, "%s\n", e.what());

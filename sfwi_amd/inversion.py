@@ -143,16 +143,31 @@ class Stage2(_CG):
     def iterate(self, ctx, shots, vp, vs, rho, image_pp, image_ps):
         """One iteration: the images (float32, allnx x allnz) are the Born
         perturbation and are updated in place."""
-        ctx.set_model(vp, vs, rho)
-        ctx.set_perturbation(image_pp, image_ps)
+        return self.iterate_groups([(ctx, shots)], vp, vs, rho, image_pp, image_ps)
+
+    def iterate_groups(self, groups, vp, vs, rho, image_pp, image_ps, per_shot=False):
+        """One iteration over several (context, shots) groups, e.g. one
+        context per shot aperture width: gradients summed over the groups in
+        list order, misfits and the step's sums over the shots in order.
+        per_shot: the step's Born solves one shot per call (the reference's
+        loop of CALCULATE_ALPHA calls; the same sums bit for bit)."""
         misfit = 0.0
-        for v in _batched(ctx, shots, lambda b: ctx.batch_gradient(b, born=True)):
-            misfit += float(v)
-        gvp, gvs, gil = ctx.flush()
-        c1, c2, beta, _, _ = self.direction(gvp, gvs, gil)
+        G = None
+        for ctx, shots in groups:
+            ctx.set_model(vp, vs, rho)
+            ctx.set_perturbation(image_pp, image_ps)
+            for v in _batched(ctx, shots, lambda b: ctx.batch_gradient(b, born=True)):
+                misfit += float(v)
+            g = ctx.flush()
+            G = g if G is None else tuple((a + b).astype(np.float32) for a, b in zip(G, g))
+        c1, c2, beta, _, _ = self.direction(*G)
         fenzi = fenmu = 0.0
-        for s in shots:
-            fenzi, fenmu = ctx.linesearch_linear(s, c1, c2, fenzi, fenmu)
+        for ctx, shots in groups:
+            if per_shot:
+                for s in shots:
+                    fenzi, fenmu = ctx.linesearch_linear(s, c1, c2, fenzi, fenmu)
+            else:
+                fenzi, fenmu = ctx.batch_linesearch_linear(shots, c1, c2, fenzi, fenmu)
         alpha = self.update(image_pp, image_ps, c1, c2, fenzi, fenmu)
         rec = dict(iter=self.iter, misfit=misfit, rms=self._rms(misfit), beta=beta, alpha=alpha,
                    fenzi=fenzi, fenmu=fenmu)

@@ -420,6 +420,47 @@ def test_overlapping_windows_batch(depth, order):
             assert np.array_equal(x, y), "grad_rho %s: max |d| = %g" % (grad_rho, np.abs(x - y).max())
 
 
+def test_batch_linesearch_linear_equals_per_shot_calls():
+    """sfwi_batch_linesearch_linear over shots with overlapping windows at
+    three offsets, in batches of 2 + 1 (max_batch 2), equals the per-shot
+    sfwi_linesearch_linear calls in shot order, bit for bit, in both modes,
+    starting from zero and from non-zero sums; the observed records stay
+    resident in their slots for a following gradient call."""
+    sv, ntr = _overlap_case()
+    w = W.ricker(sv.nt, sv.dt, sv.f0)
+    lib = [lib_shot(sv, s, *_observed(sv, s, w)) for s in sv.shots]
+    dvp = (0.5 * (sv.vp - sv.vp0)).astype(np.float32)
+    dvs = (0.5 * (sv.vs - sv.vs0)).astype(np.float32)
+    rng = np.random.default_rng(6)
+    c1 = (rng.standard_normal(sv.vp.shape) * 0.1).astype(np.float32)
+    c2 = (rng.standard_normal(sv.vp.shape) * 0.05).astype(np.float32)
+    for mode in ("fast", "exact"):
+        with context(sv, mode, max_batch=2, ntr=ntr, rho_const=2000.0, adjoint_dir=-1) as ctx:
+            ctx.set_wavelet(w)
+            ctx.set_model(sv.vp0, sv.vs0, None)
+            ctx.set_perturbation(dvp, dvs)
+            per0 = (0.0, 0.0)
+            for s in lib:
+                per0 = ctx.linesearch_linear(s, c1, c2, *per0)
+            start = (0.25 * per0[0], 0.5 * per0[1])
+            per = start
+            for s in lib:
+                per = ctx.linesearch_linear(s, c1, c2, *per)
+            bat0 = ctx.batch_linesearch_linear(lib, c1, c2)
+            bat = ctx.batch_linesearch_linear(lib, c1, c2, *start)
+            # the last batch was shot 2 alone, in slot 0: its records are resident there
+            m_obs = ctx.batch_gradient([lib[2]], born=True)
+            g_obs = ctx.flush()
+            m_res = ctx.batch_gradient([lib_shot(sv, sv.shots[2])], born=True)
+            g_res = ctx.flush()
+        assert per0[0] != 0.0 and per0[1] > 0.0
+        assert bat0 == per0, (mode, per0, bat0)
+        assert bat == per, (mode, per, bat)
+        assert np.array_equal(m_obs, m_res)
+        for x, y in zip(g_obs, g_res):
+            assert np.array_equal(x, y)
+
+
 def test_batch_linesearch_equals_per_shot_calls():
     """sfwi_batch_linesearch_parabola over shots with overlapping windows at
     three offsets, in batches of 2 + 1 (max_batch 2), equals the per-shot

@@ -16,12 +16,13 @@ from sfwi_amd import _lib
 from sfwi_amd import survey_io as S
 from sfwi_amd import workloads as W
 from sfwi_amd.elastic import Context, Shot
-from sfwi_amd.inversion import Stage1
+from sfwi_amd.inversion import Stage1, Stage2
 
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PROG = os.path.join(ROOT, "sfwi_amd", "sfwi_stage1")
+PROG2 = os.path.join(ROOT, "sfwi_amd", "sfwi_stage2")
 DX, PML, NT, DT, F0, SY, GY, CUT = 10.0, 8, 220, 0.001, 25.0, 2, 0, 4
 # the windows [0, 30), [25, 60), [20, 45) cover the model: a cell outside every window has zero
 # illumination, and the preconditioning divides by it, as in the reference
@@ -40,9 +41,9 @@ def _shot(sx, gx, all_left=0):
     return Shot(left, PML + int((sx - xmin) / DX), PML + SY, PML + GY, xmin, gx), width
 
 
-def _ctx(sv, width, nshots):
+def _ctx(sv, width, nshots, **kw):
     return Context(dx=DX, dz=DX, dt=DT, nx=width, nz=sv.allnz, pml=PML, nt=NT, ntr=35, allnx=sv.allnx,
-                   allnz=sv.allnz, max_batch=nshots)
+                   allnz=sv.allnz, max_batch=nshots, **kw)
 
 
 def test_stage1_from_su_files(tmp_path):
@@ -177,3 +178,70 @@ def test_stage1_overlapping_windows_in_batches(tmp_path):
         f = "%s_%d_%d.dat" % (name, sv.allnx, sv.allnz)
         assert np.array_equal(np.fromfile(tmp_path / "out" / f, np.float32),
                               np.fromfile(tmp_path / "out2" / f, np.float32)), name
+
+
+def test_stage2_from_su_files(tmp_path):
+    """The stage-2 program on SU records of shots with three aperture widths
+    (three contexts, density 2000, stage-2 adjoint direction) and the Python
+    stage-2 loop over the same files give the same images, misfits, beta,
+    alpha and step sums bit for bit; the batched step equals the per-shot
+    CALCULATE_ALPHA loop bit for bit; the misfit falls (Born data are affine
+    in the images, so the exact step along a descent direction lowers it)."""
+    sv = _model()
+    w = _lib.ricker(NT, np.float32(DT), F0)
+    rho = np.full_like(sv.vp, 2000.0)
+    for k, (sx, gx) in enumerate(SPREADS):
+        sh, width = _shot(sx, gx)
+        with _ctx(sv, width, 1) as ctx:
+            ctx.set_wavelet(w)
+            ctx.set_model(sv.vp, sv.vs, rho)
+            oz, ox = ctx.forward(sh)
+        S.write_su(str(tmp_path / "z.su"), k + 1, sx, gx, oz, DT, append=k > 0)
+        S.write_su(str(tmp_path / "x.su"), k + 1, sx, gx, ox, DT, append=k > 0)
+    sv.vp0.tofile(tmp_path / "vp.bin")
+    sv.vs0.tofile(tmp_path / "vs.bin")
+    sv.vs0.tofile(tmp_path / "rho.bin")
+    para = tmp_path / "para"
+    para.write_text("%g %g %d %d %d 1 3\n%s\n%s\n%s\n%s\n%s\n%s\n%g 2 0\n%d %d\n1 0 0 1000\n" % (
+        DX, DX, PML, sv.allnx, sv.allnz, tmp_path / "vp.bin", tmp_path / "vs.bin", tmp_path / "rho.bin",
+        tmp_path / "z.su", tmp_path / "x.su", tmp_path / "out", F0, SY, GY))
+
+    su = S.SuSurvey(para)
+    assert len(su.groups()) == 3
+    results = []
+    for per_shot in (False, True):
+        ctxs = {wd: _ctx(sv, wd, len(ix), rho_const=2000.0, adjoint_dir=-1) for wd, ix in su.groups().items()}
+        for c in ctxs.values():
+            c.set_wavelet(w)
+        groups = [(ctxs[wd], [su.shots[k] for k in ix]) for wd, ix in su.groups().items()]
+        st = Stage2(sv.allnx, sv.allnz, cut_layer=CUT)
+        ip, is_ = np.zeros_like(su.vp), np.zeros_like(su.vp)
+        its = [st.iterate_groups(groups, su.vp, su.vs, None, ip, is_, per_shot=per_shot) for _ in range(2)]
+        for c in ctxs.values():
+            c.close()
+        results.append((ip, is_, its))
+    (ip, is_, its), (ip1, is1, its1) = results
+    assert np.abs(ip).max() > 0 and np.abs(is_).max() > 0
+    assert np.array_equal(ip, ip1) and np.array_equal(is_, is1)
+    key = ("misfit", "beta", "alpha", "fenzi", "fenmu")
+    assert [[i[k] for k in key] for i in its] == [[i[k] for k in key] for i in its1]
+    assert its[1]["misfit"] < its[0]["misfit"] and its[0]["beta"] == 0.0
+
+    r = subprocess.run([PROG2, str(para), "2", "64", str(CUT)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    lines = [l.split() for l in r.stdout.strip().splitlines()]
+    assert len(lines) == 2
+    for l, it in zip(lines, its):
+        assert [float(l[j]) for j in (3, 5, 7, 9, 11)] == [it[k] for k in key]
+    out = tmp_path / "out"
+    for name, ref in (("sum_image_pp", ip), ("sum_image_ps", is_)):
+        got = np.fromfile(out / ("%s_1_%d_%d.dat" % (name, sv.allnx, sv.allnz)), np.float32)
+        assert np.array_equal(got.reshape(ref.shape), ref), name
+    for name in ("sum_grad_pp", "sum_grad_ps", "sum_cggrad_pp", "sum_cggrad_ps", "sum_illum"):
+        a = np.fromfile(out / ("%s_0_%d_%d.dat" % (name, sv.allnx, sv.allnz)), np.float32)
+        assert a.size == sv.allnx * sv.allnz and np.abs(a).max() > 0, name
+    # the first direction: image_1 = -alpha_0 cg_0
+    cg = np.fromfile(out / ("sum_cggrad_pp_0_%d_%d.dat" % (sv.allnx, sv.allnz)), np.float32).reshape(ip.shape)
+    i0 = np.fromfile(out / ("sum_image_pp_0_%d_%d.dat" % (sv.allnx, sv.allnz)), np.float32).reshape(ip.shape)
+    assert np.array_equal(i0, (0.0 - its[0]["alpha"] * cg.astype(np.float64)).astype(np.float32))
+    assert (out / "misfit.txt").read_text().count("misfit") == 2

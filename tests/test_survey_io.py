@@ -136,15 +136,22 @@ def test_su_survey_assembly(tmp_path):
     assert np.array_equal(s.obs_z, dz[2]) and np.array_equal(s.obs_x, dx_[2])
 
 
-def test_stage1_program_input_errors(tmp_path):
+@pytest.mark.parametrize("stage", [1, 2])
+def test_stage_program_input_errors(tmp_path, stage):
     import subprocess
-    prog = os.path.join(os.path.dirname(GOLDEN), "..", "sfwi_amd", "sfwi_stage1")
+    prog = os.path.join(os.path.dirname(GOLDEN), "..", "sfwi_amd", "sfwi_stage%d" % stage)
     assert os.path.exists(prog), "run __graft_entry__.build()"
     r = subprocess.run([prog], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "usage" in r.stderr
     r = subprocess.run([prog, str(tmp_path / "missing")], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "cannot open parameter file" in r.stderr
     # the example file points at /path/to/..., so the model read fails cleanly
-    r = subprocess.run([prog, os.path.join(GOLDEN, "para_stage1_example1"), "1"], capture_output=True, text=True,
-                       timeout=60)
+    r = subprocess.run([prog, os.path.join(GOLDEN, "para_stage%d_example1" % stage), "1"], capture_output=True,
+                       text=True, timeout=60)
     assert r.returncode == 2 and "cannot open model file /path/to/" in r.stderr
+    # rbc = 1 is refused before anything is read
+    para = tmp_path / "para"
+    para.write_text(open(os.path.join(GOLDEN, "para_stage%d_example1" % stage)).read().replace("1 0 0 1000",
+                                                                                             "1 1 0 1000"))
+    r = subprocess.run([prog, str(para), "1"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "rbc" in r.stderr
