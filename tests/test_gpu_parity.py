@@ -446,6 +446,9 @@ def test_batch_linesearch_linear_equals_per_shot_calls():
             per = start
             for s in lib:
                 per = ctx.linesearch_linear(s, c1, c2, *per)
+            # a per-shot call leaves its shot's records in slot 0
+            m_one = ctx.batch_gradient([lib_shot(sv, sv.shots[2])], born=True)
+            g_one = ctx.flush()
             bat0 = ctx.batch_linesearch_linear(lib, c1, c2)
             bat = ctx.batch_linesearch_linear(lib, c1, c2, *start)
             # the last batch was shot 2 alone, in slot 0: its records are resident there
@@ -456,9 +459,9 @@ def test_batch_linesearch_linear_equals_per_shot_calls():
         assert per0[0] != 0.0 and per0[1] > 0.0
         assert bat0 == per0, (mode, per0, bat0)
         assert bat == per, (mode, per, bat)
-        assert np.array_equal(m_obs, m_res)
-        for x, y in zip(g_obs, g_res):
-            assert np.array_equal(x, y)
+        assert np.array_equal(m_obs, m_res) and np.array_equal(m_obs, m_one)
+        for x, y, z in zip(g_obs, g_res, g_one):
+            assert np.array_equal(x, y) and np.array_equal(x, z)
 
 
 def test_batch_linesearch_equals_per_shot_calls():

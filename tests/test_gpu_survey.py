@@ -245,3 +245,13 @@ def test_stage2_from_su_files(tmp_path):
     i0 = np.fromfile(out / ("sum_image_pp_0_%d_%d.dat" % (sv.allnx, sv.allnz)), np.float32).reshape(ip.shape)
     assert np.array_equal(i0, (0.0 - its[0]["alpha"] * cg.astype(np.float64)).astype(np.float32))
     assert (out / "misfit.txt").read_text().count("misfit") == 2
+
+    # cpu_mem = 1, iointerval = 50: boundary strips in pinned host memory, same output bit for bit
+    para2 = tmp_path / "para2"
+    para2.write_text(para.read_text().replace("1 0 0 1000\n", "1 0 1 50\n").replace(str(out), str(tmp_path / "out2")))
+    r2 = subprocess.run([PROG2, str(para2), "2", "64", str(CUT)], capture_output=True, text=True, timeout=300)
+    assert r2.returncode == 0, r2.stderr
+    assert r2.stdout == r.stdout
+    for name in ("sum_image_pp_1", "sum_image_ps_1"):
+        f = "%s_%d_%d.dat" % (name, sv.allnx, sv.allnz)
+        assert np.array_equal(np.fromfile(out / f, np.float32), np.fromfile(tmp_path / "out2" / f, np.float32)), name
