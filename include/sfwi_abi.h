@@ -143,7 +143,9 @@ int sfwi_shot_gradient(sfwi_ctx *ctx, const sfwi_shot *shot, double *misfit);
  * residual = born - obs + background, perturbation from sfwi_set_perturbation. */
 int sfwi_shot_born_gradient(sfwi_ctx *ctx, const sfwi_shot *shot, double *misfit);
 /* Many shots at once (the reference farms them over MPI ranks,
- * sfwi-stage1/main.cpp:417-634); misfits[i] receives shot i's misfit (+=). */
+ * sfwi-stage1/main.cpp:417-634); misfits[i] receives shot i's misfit (+=).
+ * The sfwi_batch_* calls solve the shots in batches of up to max_batch; an
+ * empty list (nshots = 0, shots may be NULL) does nothing and returns 0. */
 int sfwi_batch_gradient(sfwi_ctx *ctx, int nshots, const sfwi_shot *shots, double *misfits);
 int sfwi_batch_born_gradient(sfwi_ctx *ctx, int nshots, const sfwi_shot *shots, double *misfits);
 

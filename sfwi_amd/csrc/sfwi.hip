@@ -716,7 +716,7 @@ int finish(sfwi_ctx *c) {
 
 // Batched solve with misfit output; shots processed in groups of max_batch.
 int solve_misfits(sfwi_ctx *c, Kind kind, int nshots, const sfwi_shot *shots, double *misfits) {
-    if (!shots || nshots < 0) return fail(c, SFWI_EINVAL, "bad shot list");
+    if (nshots < 0 || (nshots > 0 && !shots)) return fail(c, SFWI_EINVAL, "bad shot list");
     for (int b0 = 0; b0 < nshots; b0 += c->p.max_batch) {
         const int n = std::min(c->p.max_batch, nshots - b0);
         int r = prepare(c, kind, n, shots + b0, nullptr);
@@ -921,7 +921,7 @@ int sfwi_shot_forward(sfwi_ctx *c, const sfwi_shot *shot, float *syn_z, float *s
 }
 
 int sfwi_batch_forward(sfwi_ctx *c, int nshots, const sfwi_shot *shots, float *syn_z, float *syn_x) {
-    if (!c || !shots || nshots < 0) return fail(c, SFWI_EINVAL, "bad shot list");
+    if (!c || nshots < 0 || (nshots > 0 && !shots)) return fail(c, SFWI_EINVAL, "bad shot list");
     for (int b0 = 0; b0 < nshots; b0 += c->p.max_batch) {
         const int n = std::min(c->p.max_batch, nshots - b0);
         int r = prepare(c, K_FWD, n, shots + b0, nullptr);

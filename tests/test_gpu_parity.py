@@ -491,3 +491,36 @@ def test_batch_linesearch_equals_per_shot_calls():
         assert np.all(per0 > 0)
         assert np.array_equal(per0, bat0), (mode, per0, bat0)
         assert np.array_equal(per, bat), (mode, per, bat)
+
+
+def test_empty_shot_lists():
+    """Every batched call takes an empty shot list (also as a NULL pointer) as
+    a no-op: nothing is solved, sums are returned unchanged and the gradient
+    accumulators stay as they were."""
+    import ctypes
+    sv = W.tiny(nt=120)
+    w = W.ricker(sv.nt, sv.dt, sv.f0)
+    lib = [lib_shot(sv, sh, *_observed(sv, sh, w)) for sh in sv.shots[:1]]
+    d = np.zeros_like(sv.vp)
+    with context(sv, "fast", max_batch=2) as ctx:
+        ctx.set_wavelet(w)
+        ctx.set_model(sv.vp0, sv.vs0, sv.rho0)
+        ctx.set_perturbation(d, d)
+        m = ctx.batch_gradient(lib)
+        g = ctx.flush()
+        ctx.batch_gradient(lib)
+        assert ctx.batch_gradient([]).size == 0 and ctx.batch_gradient([], born=True).size == 0
+        z, x = ctx.batch_forward([])
+        assert z.shape[0] == 0
+        assert np.array_equal(ctx.batch_linesearch_parabola([], d, d, 30.0, 150.0, [1.0, 2.0, 3.0]), [1.0, 2.0, 3.0])
+        assert ctx.batch_linesearch_linear([], d, d, 4.0, 5.0) == (4.0, 5.0)
+        L, h = ctx.L, ctx.h
+        mis = (ctypes.c_double * 1)()
+        assert L.sfwi_batch_gradient(h, 0, None, mis) == 0
+        assert L.sfwi_batch_born_gradient(h, 0, None, mis) == 0
+        assert L.sfwi_batch_forward(h, 0, None, None, None) == 0
+        assert L.sfwi_batch_gradient(h, -1, None, mis) != 0
+        g2 = ctx.flush()             # only the one call with shots accumulated
+    assert m[0] > 0
+    for a, b in zip(g, g2):
+        assert np.array_equal(a, b)
