@@ -130,6 +130,8 @@ def main():
                          "program on the reference's example-1 model (376x201, 38 shots) from synthetic SU "
                          "records instead")
     ap.add_argument("--stage", type=int, default=1, choices=(1, 2), help="inversion: which stage's program")
+    ap.add_argument("--born", action="store_true",
+                    help="time the stage-2 (Born) gradient of the workload's shot batch instead")
     ap.add_argument("--requests", type=int, default=24)
     ap.add_argument("--rate", type=float, default=1.0, help="serving: requests per second")
     ap.add_argument("--scale", type=float, default=1.0, help="serving: request size factor")
@@ -157,6 +159,8 @@ def main():
         return uneven(args)
     if args.inversion:
         return inversion(args)
+    if args.born:
+        return born(args)
 
     kw = {}
     if args.shots:
@@ -323,6 +327,107 @@ def main():
         "kernels": kernels,
     }
     print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+def born(args):
+    """Stage-2 gradient throughput: the workload's shot batch solved as Born
+    gradients (sfwi-stage2/FD2DGPU.cu:3978-4570) at the initial model with the
+    perturbation = true - initial model, density 2000 and the stage-2 adjoint
+    direction; observed traces from a forward solve of the true model.  Timed
+    like the default line (records resident in HBM); the gradient of
+    --cpu-shots shots is checked against the FP64 oracle, which is also the CPU
+    baseline.  The two forward kernels are the Born variants (background and
+    scattered fields in one launch); the reverse step is the stage-1 one on
+    the background field.  No roofline entry."""
+    from sfwi_amd import workloads as W
+    from sfwi_amd import _lib
+    from sfwi_amd.elastic import Context, Shot
+    kw = {}
+    if args.shots:
+        kw["nshots"] = args.shots
+    if args.nt:
+        kw["nt"] = args.nt
+    sv = W.BY_NAME[args.workload](**kw)
+    w = W.ricker(sv.nt, sv.dt, sv.f0)
+    geo = [W.shot_geometry(sv, s) for s in sv.shots]
+    nsh = len(sv.shots)
+    print("bench: Born gradient, %s, %d shots, grid %dx%d pml %d, nt %d" % (
+        sv.name, nsh, sv.nx, sv.allnz, sv.pml, sv.nt), file=sys.stderr, flush=True)
+    ctx = Context(dx=sv.dx, dz=sv.dz, dt=sv.dt, nx=sv.nx, nz=sv.allnz, pml=sv.pml, nt=sv.nt, ntr=sv.ntr,
+                  allnx=sv.allnx, allnz=sv.allnz, max_batch=nsh, mode=args.mode, storage=args.storage,
+                  rho_const=2000.0, adjoint_dir=-1)
+    ctx.set_wavelet(w)
+    ctx.set_model(sv.vp, sv.vs, None)
+    obs_z, obs_x = ctx.batch_forward([Shot(s.left, *g, s.ngx_left, s.gc) for s, g in zip(sv.shots, geo)])
+    dvp = (sv.vp - sv.vp0).astype(np.float32)
+    dvs = (sv.vs - sv.vs0).astype(np.float32)
+    ctx.set_model(sv.vp0, sv.vs0, None)
+    ctx.set_perturbation(dvp, dvs)
+    shots = [Shot(s.left, *g, s.ngx_left, s.gc, obs_z[k], obs_x[k]) for k, (s, g) in enumerate(zip(sv.shots, geo))]
+    resident = [Shot(s.left, *g, s.ngx_left, s.gc) for s, g in zip(sv.shots, geo)]
+    for k in range(max(args.warmup, 1)):
+        t0 = time.perf_counter()
+        ctx.batch_gradient(shots, born=True)
+        ctx.flush()
+        print("bench: warmup %d %.3f s" % (k, time.perf_counter() - t0), file=sys.stderr, flush=True)
+    ctx.set_timing(args.timing_every)
+    times = []
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        m = ctx.batch_gradient(resident, born=True)
+        times.append(time.perf_counter() - t0)
+        print("bench: step %d %.3f s" % (k, times[-1]), file=sys.stderr, flush=True)
+    kstats = {kid: ctx.kernel_time(kid) for kid in range(_lib.K_COUNT)}
+    ctx.set_timing(0)
+    ctx.flush()
+    kernels = {}
+    for kid in range(_lib.K_COUNT):
+        ms, n = kstats[kid]
+        if n:
+            kernels[_lib.load().sfwi_kernel_name(kid).decode()] = {"avg_ms": round(ms, 5), "samples": n}
+    acc = cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        ncpu = max(1, min(args.cpu_shots, nsh))
+        ctx.batch_gradient(shots[:ncpu], born=True)
+        g_vp, g_vs, _ = ctx.flush()
+        threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+        r_vp = np.zeros((sv.allnx, sv.allnz))
+        r_vs = np.zeros_like(r_vp)
+        t0 = time.perf_counter()
+        for s, g, sh in zip(sv.shots[:ncpu], geo[:ncpu], shots[:ncpu]):
+            c = O.config(allnx=sv.allnx, allnz=sv.allnz, nx=sv.nx, nz=sv.allnz, pml=sv.pml, nt=sv.nt,
+                         ntr=sv.ntr, dx=sv.dx, dz=sv.dz, dt=sv.dt, left=s.left, isx=g[0], isz=g[1],
+                         igz=g[2], ngx_left=s.ngx_left, nthreads=threads, rho_const=2000.0, adjoint_dir=-1)
+            a = O.born_gradient(c, sv.vp0, sv.vs0, sv.rho0, dvp, dvs, w, s.gc, sh.obs_z, sh.obs_x, "f64")
+            r_vp += a[0]
+            r_vs += a[1]
+        tc = time.perf_counter() - t0
+
+        def rel(x, y):
+            return float(np.linalg.norm(x.astype(np.float64) - y) / np.linalg.norm(y))
+
+        acc = {"grad_vp": rel(g_vp, r_vp), "grad_vs": rel(g_vs, r_vs), "shots": ncpu, "bar": 1e-3}
+        cpu = {"value": round(ncpu / tc, 5), "unit": "shot-gradients/s", "cores": threads, "kind": "port",
+               "sample": "FP64 oracle Born gradient, %d of the %d shots of the same workload, %.1f s" % (
+                   ncpu, nsh, tc)}
+    total = sum(times)
+    lat = sorted(times)
+    print(json.dumps({
+        "metric": "stage-2 (Born) shot-gradients/s", "value": round(nsh * args.steps / total, 4),
+        "unit": "shot-gradients/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * total / args.steps, 2), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded model; perturbation = true - initial model; observed traces from a forward "
+                "solve of the true model)",
+        "config": {"workload": "%s: %dx%d dx=dz=%gm, pml %d, dt %gms, nt %d, %d shots, %g Hz, %d receivers, "
+                               "batched Born gradient, density 2000" % (sv.name, sv.nx, sv.allnz, sv.dx, sv.pml,
+                                                                       sv.dt * 1e3, sv.nt, nsh, sv.f0, sv.ntr),
+                   "mode": args.mode, "storage": "boundary strips %s" % args.storage.upper()},
+        "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 2),
+                       "p99": round(1e3 * float(np.percentile(lat, 99)), 2), "unit": "per batch step"},
+        "misfit": float(m.sum()), "grad_err_vs_fp64": acc, "cpu_baseline": cpu, "kernels": kernels}), flush=True)
     ctx.close()
 
 
