@@ -356,6 +356,7 @@ def born(args):
         sv.name, nsh, sv.nx, sv.allnz, sv.pml, sv.nt), file=sys.stderr, flush=True)
     ctx = Context(dx=sv.dx, dz=sv.dz, dt=sv.dt, nx=sv.nx, nz=sv.allnz, pml=sv.pml, nt=sv.nt, ntr=sv.ntr,
                   allnx=sv.allnx, allnz=sv.allnz, max_batch=nsh, mode=args.mode, storage=args.storage,
+                  march_cols=args.march_cols, march_depth=args.march_depth, march_order=args.march_order,
                   rho_const=2000.0, adjoint_dir=-1)
     ctx.set_wavelet(w)
     ctx.set_model(sv.vp, sv.vs, None)

@@ -80,7 +80,9 @@ typedef struct sfwi_params {
                            /* march kernels; < 0 selects per-cell kernels; 0 picks */
                            /* per-cell kernels for batches of fewer than 7168 wave */
                            /* tiles (too few to fill the GPU) and 8 columns above  */
-    int march_depth;       /* columns of operands prefetched ahead (1, 2, 4; 0 = 4) */
+    int march_depth;       /* columns of operands prefetched ahead (1, 2, 4; 0: 4, and */
+                           /* 2 for the two Born forward kernels, which hold twice */
+                           /* the fields)                                           */
     int misfit;            /* SFWI_MISFIT_*                                          */
     int march_order;       /* column-march block order: 0 default (= 3); 1 one shot  */
                            /* after another, split over the XCDs; 2 the shots of a  */

@@ -495,7 +495,12 @@ int strip_chunk_copy(sfwi_ctx *c, int n, int k, bool to_host) {
     return 0;
 }
 
-template <bool E, int PD>
+// PD: operand prefetch depth of the column-march kernels; HPD: the same for
+// the two Born forward kernels, which carry two fields and twice the registers
+// (at depth 4 they fit 2 waves per SIMD and run 6-11 % slower than at depth 2;
+// the imaging reconstruction kernels gain 1 % at depth 2, which the next
+// kernel loses again, and the others are 1-4 % faster at depth 4).
+template <bool E, int PD, int HPD>
 int run_batch(sfwi_ctx *c, Kind kind, int n) {
     const Geo &g = c->g;
     const int nt = g.nt;
@@ -545,8 +550,8 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
             }
         } else {
             if (born) {
-                if (grad) hipLaunchKernelGGL((km_vel<true, true, PD>), tiles, blk, 0, st, gm, D, sp, 1);
-                else hipLaunchKernelGGL((km_vel<false, true, PD>), tiles, blk, 0, st, gm, D, sp, 1);
+                if (grad) hipLaunchKernelGGL((km_vel<true, true, HPD>), tiles, blk, 0, st, gm, D, sp, 1);
+                else hipLaunchKernelGGL((km_vel<false, true, HPD>), tiles, blk, 0, st, gm, D, sp, 1);
             } else {
                 if (grad) hipLaunchKernelGGL((km_vel<true, false, PD>), tiles, blk, 0, st, gm, D, sp, 1);
                 else hipLaunchKernelGGL((km_vel<false, false, PD>), tiles, blk, 0, st, gm, D, sp, 1);
@@ -559,7 +564,7 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
             if (born) hipLaunchKernelGGL((k_str<E, false, true>), cells, blk, 0, st, g, D, 1);
             else hipLaunchKernelGGL((k_str<E, false, false>), cells, blk, 0, st, g, D, 1);
         } else {
-            if (born) hipLaunchKernelGGL((km_str<false, true, PD>), tiles, blk, 0, st, gm, D, 1);
+            if (born) hipLaunchKernelGGL((km_str<false, true, HPD>), tiles, blk, 0, st, gm, D, 1);
             else hipLaunchKernelGGL((km_str<false, false, PD>), tiles, blk, 0, st, gm, D, 1);
         }
         tend(c, t);
@@ -663,12 +668,12 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
 }
 
 int run_direct(sfwi_ctx *c, Kind kind, int n) {
-    if (c->p.mode == SFWI_MODE_EXACT) return run_batch<true, 1>(c, kind, n);
+    if (c->p.mode == SFWI_MODE_EXACT) return run_batch<true, 1, 1>(c, kind, n);
     switch (c->p.march_depth) {
-    case 1: return run_batch<false, 1>(c, kind, n);
-    case 2: return run_batch<false, 2>(c, kind, n);
-    case 4: return run_batch<false, 4>(c, kind, n);
-    default: return run_batch<false, 4>(c, kind, n);
+    case 1: return run_batch<false, 1, 1>(c, kind, n);
+    case 2: return run_batch<false, 2, 2>(c, kind, n);
+    case 4: return run_batch<false, 4, 4>(c, kind, n);
+    default: return run_batch<false, 4, 2>(c, kind, n);   // automatic
     }
 }
 
