@@ -113,6 +113,8 @@ def main():
                     help="misfit of the gradient (l2: the reference's; ot_*: optimal-transport variants)")
     ap.add_argument("--cpu-mem", type=int, default=0, metavar="CHUNK",
                     help="boundary strips in pinned host memory, CHUNK steps per chunk (0: strips in HBM)")
+    ap.add_argument("--rbc", action="store_true",
+                    help="random boundary instead of the boundary strips (the reference's rbc)")
     ap.add_argument("--mode", default="fast", choices=["fast", "exact"])
     ap.add_argument("--timing-every", type=int, default=20)
     ap.add_argument("--march-cols", type=int, default=0,
@@ -177,7 +179,8 @@ def main():
     ctx = Context(dx=sv.dx, dz=sv.dz, dt=sv.dt, nx=sv.nx, nz=sv.allnz, pml=sv.pml, nt=sv.nt, ntr=sv.ntr,
                   allnx=sv.allnx, allnz=sv.allnz, max_batch=len(sv.shots), mode=args.mode, storage=args.storage,
                   march_cols=args.march_cols, march_depth=args.march_depth, march_order=args.march_order,
-                  cpu_mem=args.cpu_mem > 0, strip_chunk=max(args.cpu_mem, 0), misfit=args.misfit)
+                  cpu_mem=args.cpu_mem > 0, strip_chunk=max(args.cpu_mem, 0), misfit=args.misfit,
+                  rbc=args.rbc, rbc_seed=1)
     ctx.set_wavelet(w)
     ctx.set_model(sv.vp, sv.vs, sv.rho)
     t0 = time.perf_counter()
@@ -240,7 +243,7 @@ def main():
     roofline = None
     if dom:
         d = kernels[dom]
-        measured = (args.workload == "shot_batch" and args.mode == "fast" and args.march_cols in (0, 8)
+        measured = (not args.rbc and args.workload == "shot_batch" and args.mode == "fast" and args.march_cols in (0, 8)
                     and args.march_depth in (0, 4) and args.march_order in (0, 3))
         traffic = round(MEASURED_HBM_BYTES_PER_SHOT[dom] * nsh) if measured and dom in MEASURED_HBM_BYTES_PER_SHOT \
             else None
@@ -287,7 +290,7 @@ def main():
         for s, g, sh in zip(sv.shots[:ncpu], geo[:ncpu], sub):
             c = O.config(allnx=sv.allnx, allnz=sv.allnz, nx=sv.nx, nz=sv.allnz, pml=sv.pml, nt=sv.nt,
                          ntr=sv.ntr, dx=sv.dx, dz=sv.dz, dt=sv.dt, left=s.left, isx=g[0], isz=g[1],
-                         igz=g[2], ngx_left=s.ngx_left, nthreads=threads)
+                         igz=g[2], ngx_left=s.ngx_left, nthreads=threads, rbc=int(args.rbc), rbc_seed=1)
             a = O.gradient(c, sv.vp0, sv.vs0, sv.rho0, w, s.gc, sh.obs_z, sh.obs_x, "f64")
             r_vp += a[0]
             r_vs += a[1]
@@ -312,7 +315,8 @@ def main():
         "config": {"workload": "%s: %dx%d dx=dz=%gm, pml %d, dt %gms, nt %d, %d shots, %g Hz, "
                                "%d receivers, batched gradient" % (sv.name, sv.nx, sv.allnz, sv.dx, sv.pml,
                                                                     sv.dt * 1e3, sv.nt, nsh, sv.f0, sv.ntr),
-                   "mode": args.mode, "misfit": args.misfit, "storage": "boundary strips %s%s" % (
+                   "mode": args.mode, "misfit": args.misfit, "storage": "random boundary, no strips" if args.rbc
+                   else "boundary strips %s%s" % (
                        args.storage.upper(),
                        ", pinned host memory in chunks of %d steps" % args.cpu_mem if args.cpu_mem > 0 else "")},
         "gcell_updates_per_s": round(gcells, 2),

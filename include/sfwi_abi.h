@@ -97,6 +97,13 @@ typedef struct sfwi_params {
                            /* strips bit for bit.  Solves with it run without graphs. */
     int strip_chunk;       /* steps per host chunk (the reference's ntt = iointerval; */
                            /* 0 = 100); nt <= 2 strip_chunk keeps every step in HBM   */
+    int rbc;               /* 1: random boundary (the reference's rbc, sfwi-stage1/   */
+                           /* FD2DGPU.cu:3245-3305, 4330-4334): random Vp / Vs in the */
+                           /* PML bands of every window, no damping, no strips; the   */
+                           /* reverse pass reconstructs by time reversal alone        */
+                           /* (cpu_mem then has nothing to hold)                      */
+    unsigned int rbc_seed; /* seed of the random boundary's generator (the reference  */
+                           /* seeds cuRAND from time(0); here a run repeats)          */
 } sfwi_params;
 
 /* One shot: the fields the reference driver writes into FD2DGPU_ELASTIC

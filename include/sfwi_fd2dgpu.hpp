@@ -16,7 +16,9 @@
 //     which sets the adjoint direction (+1 / -1) and the stage-2 density
 //     (2000 kg/m^3) that the two reference builds hard-code;
 //   * errors throw sfwi_ref::Error (the reference does not check errors);
-//   * rbc = true (random boundary) is rejected: boundary strips are used;
+//   * rbc = true (random boundary) draws it from a seeded project-owned
+//     generator (FD2DGPU_ELASTIC::rbc_seed, default 1) where the reference
+//     seeds cuRAND from the clock, and runs without strips, as the reference;
 //     cpu_mem = true keeps the strips in pinned host memory in chunks of ntt
 //     steps (sfwi_params.cpu_mem); the first gradient call after a flush may
 //     switch the setting (the context is rebuilt), a later one may not;
@@ -104,14 +106,19 @@ public:
     // arguments of the reference): a change rebuilds the context with the
     // model, perturbation and wavelet it holds, only while no gradient has
     // been accumulated since the last flush.
-    void use_host_strips(bool cpu_mem, int ntt) {
+    // The random boundary (the per-call rbc argument) is switched the same way;
+    // its generator is seeded with rbc_seed (default 1), where the reference
+    // seeds cuRAND from the clock.
+    void use_host_strips(bool cpu_mem, int ntt, bool rbc = false) {
         if (cpu_mem && ntt < 1) throw Error(SFWI_EINVAL, "sfwi: cpu_mem needs ntt >= 1 (steps per chunk)");
-        const int want = cpu_mem ? 1 : 0, chunk = cpu_mem ? ntt : 0;
-        if (want == p_.cpu_mem && chunk == p_.strip_chunk) return;
-        if (dirty_) throw Error(SFWI_ESTATE, "sfwi: cpu_mem / ntt changed with gradients accumulated");
+        const int want = cpu_mem ? 1 : 0, chunk = cpu_mem ? ntt : 0, rb = rbc ? 1 : 0;
+        if (want == p_.cpu_mem && chunk == p_.strip_chunk && rb == p_.rbc) return;
+        if (dirty_) throw Error(SFWI_ESTATE, "sfwi: cpu_mem / ntt / rbc changed with gradients accumulated");
         sfwi_params q = p_;
         q.cpu_mem = want;
         q.strip_chunk = chunk;
+        q.rbc = rb;
+        if (!q.rbc_seed) q.rbc_seed = rbc_seed;
         sfwi_ctx *c = nullptr;
         check(sfwi_ctx_create(&q, &c), nullptr);
         try {
@@ -126,6 +133,9 @@ public:
         ctx_ = c;
         p_ = q;
     }
+    // Random boundary of the next forward-only call, the strip settings kept.
+    void use_boundary(bool rbc) { use_host_strips(p_.cpu_mem != 0, p_.cpu_mem ? p_.strip_chunk : 0, rbc); }
+    unsigned rbc_seed = 1;
     void mark_accumulated() { dirty_ = true; }
     // Observed records of the next shot (sfwi-stage1/FD2DGPU.cuh:387): kept by
     // pointer, read by the next per-shot call.
@@ -178,9 +188,6 @@ private:
     const float *dvp_ = nullptr, *dvs_ = nullptr;
 };
 
-inline void reject_unsupported(bool rbc) {
-    if (rbc) throw Error(SFWI_EINVAL, "sfwi: random boundary (rbc) is not available; strips are used");
-}
 
 // Stage-1 per-shot gradient (sfwi-stage1/FD2DGPU.cuh:466-467).
 inline void IMAGE_SINGLESHOT(FD2DGPU_ELASTIC &fd, const int /*sxnum*/, const int /*nDim*/, const int /*myid*/,
@@ -188,8 +195,7 @@ inline void IMAGE_SINGLESHOT(FD2DGPU_ELASTIC &fd, const int /*sxnum*/, const int
                              int /*iter*/, int /*iter_round1*/, int /*maxiter*/, char * /*snapdir*/,
                              float * /*image_pp*/, float * /*image_ps*/, float * /*cpu_pp_grad_old*/,
                              float * /*cpu_ps_grad_old*/, double &misfit, float *misfit_ot) {
-    reject_unsupported(rbc);
-    fd.use_host_strips(cpu_mem, ntt);
+    fd.use_host_strips(cpu_mem, ntt, rbc);
     const sfwi_shot s = fd.shot(record, record_x, gc);
     double m = 0.0;
     check(sfwi_shot_gradient(fd.ctx(), &s, &m), fd.ctx());
@@ -203,8 +209,7 @@ inline void IMAGE_SINGLESHOT(FD2DGPU_ELASTIC &fd, const int /*sxnum*/, const int
                              float *record, float *record_x, int *gc, bool rbc, bool cpu_mem, int ntt,
                              int /*iter*/, char * /*snapdir*/, float * /*image_pp*/, float * /*image_ps*/,
                              float * /*cpu_pp_grad_old*/, float * /*cpu_ps_grad_old*/, double &misfit3) {
-    reject_unsupported(rbc);
-    fd.use_host_strips(cpu_mem, ntt);
+    fd.use_host_strips(cpu_mem, ntt, rbc);
     const sfwi_shot s = fd.shot(record, record_x, gc);
     check(sfwi_shot_born_gradient(fd.ctx(), &s, &misfit3), fd.ctx());
     fd.mark_accumulated();
@@ -219,7 +224,7 @@ inline void CALCULATE_ALPHA_parabola_mb(FD2DGPU_ELASTIC &fd, const int /*sxnum*/
                                         double &misfit_step0, double &misfit_step1, double &misfit_step2,
                                         float step_1, float step_2, float *misfit_step0_ot,
                                         float *misfit_step1_ot, float *misfit_step2_ot) {
-    reject_unsupported(rbc);   // forward solves only: no strips, cpu_mem has no effect
+    fd.use_boundary(rbc);   // forward solves only: no strips, cpu_mem has no effect
     const sfwi_shot s = fd.shot(record, record_x, gc);
     double m[3] = {0.0, 0.0, 0.0};
     check(sfwi_linesearch_parabola(fd.ctx(), &s, cpu_pp_grad_old, cpu_ps_grad_old, step_1, step_2, m), fd.ctx());
@@ -239,7 +244,7 @@ inline void CALCULATE_ALPHA(FD2DGPU_ELASTIC &fd, const int /*sxnum*/, const int 
                             float *record, float *record_x, int *gc, char * /*snapdir*/, bool rbc, bool /*cpu_mem*/,
                             int /*ntt*/, int /*iter*/, float * /*image_pp*/, float * /*image_ps*/,
                             float *cpu_pp_grad_old, float *cpu_ps_grad_old, double &fenzi, double &fenmu) {
-    reject_unsupported(rbc);   // forward solves only: no strips, cpu_mem has no effect
+    fd.use_boundary(rbc);   // forward solves only: no strips, cpu_mem has no effect
     const sfwi_shot s = fd.shot(record, record_x, gc);
     check(sfwi_linesearch_linear(fd.ctx(), &s, cpu_pp_grad_old, cpu_ps_grad_old, &fenzi, &fenmu), fd.ctx());
 }

@@ -34,6 +34,8 @@ class OrConfig(ctypes.Structure):
         ("rho_const", ctypes.c_float),
         ("pml_off", ctypes.c_int),
         ("nthreads", ctypes.c_int),
+        ("rbc", ctypes.c_int),
+        ("rbc_seed", ctypes.c_uint),
     ]
 
 

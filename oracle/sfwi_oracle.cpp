@@ -201,6 +201,52 @@ void damping(const Grid &g, float rr, bool off, const std::vector<T> &vp, std::v
         }
 }
 
+// Random boundary, the scheme of random_vel / set_random_boundary
+// (sfwi-stage1/FD2DGPU.cu:162-220, 3245-3290) with a project-owned generator:
+// k = vmax / pml (vmax the window's largest Vp); a pass over the z bands (the
+// top pml rows and their mirror rows at the bottom, depth d = pml - iz) and
+// then one over the x bands (columns, d = pml - ix; corners are drawn twice,
+// as in the reference) sets vp' = vp + (2u - 1) k d, redrawn until
+// 0 <= vp' <= vmax (64 draws, then vp clamped to [0, vmax]), and
+// vs' = (vs / vp) vp'.  The mirror cell is nzp-1-iz (nxp-1-ix); the reference
+// writes nzpml-iz, one past the band.  u = (h >> 8) 2^-24 of the lowbias32
+// hash h; every operation in FP32.
+inline unsigned rbc_hash(unsigned x) {
+    x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
+    return x;
+}
+inline float rbc_cell(unsigned seed, unsigned cell, int pass, int d, float kk, float vmax, float v0) {
+    const unsigned key = rbc_hash(seed ^ rbc_hash(cell * 2u + (unsigned)pass));
+    for (unsigned a = 0; a < 64; ++a) {
+        const float u = (float)(rbc_hash(key + a) >> 8) * (1.0f / 16777216.0f);
+        const float vn = v0 + ((2.0f * u - 1.0f) * kk) * (float)d;
+        if (vn >= 0.0f && vn <= vmax) return vn;
+    }
+    return std::fmin(std::fmax(v0, 0.0f), vmax);
+}
+template <typename T>
+void random_boundary(const Grid &g, unsigned seed, std::vector<T> &vp, std::vector<T> &vs) {
+    float vmax = 0.0f;
+    for (long i = 0; i < g.n; ++i) vmax = std::fmax(vmax, (float)vp[i]);
+    const float kk = vmax / (float)g.pml;
+    auto set = [&](long i, int pass, int d) {
+        const float v0 = (float)vp[i], s0 = (float)vs[i];
+        const float vn = rbc_cell(seed, (unsigned)i, pass, d, kk, vmax, v0);
+        vs[i] = (T)(v0 > 0.0f ? (s0 / v0) * vn : s0);
+        vp[i] = (T)vn;
+    };
+    for (int ix = 0; ix < g.nxp; ++ix)
+        for (int iz = 0; iz < g.pml; ++iz) {
+            set((long)ix * g.nzp + iz, 0, g.pml - iz);
+            set((long)ix * g.nzp + g.nzp - 1 - iz, 0, g.pml - iz);
+        }
+    for (int ix = 0; ix < g.pml; ++ix)
+        for (int iz = 0; iz < g.nzp; ++iz) {
+            set((long)ix * g.nzp + iz, 1, g.pml - ix);
+            set((long)(g.nxp - 1 - ix) * g.nzp + iz, 1, g.pml - ix);
+        }
+}
+
 // lambda = rho (vp^2 - 2 vs^2), mu = rho vs^2: sfwi-stage1/FD2DGPU.cu:1116-1124;
 // stage 2 forces rho := 2000 inside the same kernel (sfwi-stage2/FD2DGPU.cu:1111).
 template <typename T>
@@ -211,7 +257,8 @@ void build_medium(const or_config *c, const Grid &g, const float *vp, const floa
     window_pad(g, c->allnz, c->left, rho, m.rho);
     if (c->rho_const > 0)
         for (long i = 0; i < g.n; ++i) m.rho[i] = (T)c->rho_const;
-    damping(g, c->rr, c->pml_off != 0, m.vp, m.dampx, m.dampz);
+    if (c->rbc) random_boundary(g, c->rbc_seed, m.vp, m.vs);
+    damping(g, c->rr, c->pml_off != 0 || c->rbc != 0, m.vp, m.dampx, m.dampz);
     m.lam.resize(g.n);
     m.mu.resize(g.n);
     for (long i = 0; i < g.n; ++i) {
@@ -530,6 +577,7 @@ template <typename T> struct Run {
     void forward(const float *wav, std::vector<T> &rz, std::vector<T> &rx, Wave<T> *keep = nullptr,
                  Strips<T> *strips = nullptr) {
         Wave<T> w(g.n);
+        if (c->rbc == 1) strips = nullptr;   // random boundary: nothing saved (FD2DGPU.cu:4543)
         for (int it = 0; it < g.nt; ++it) {
             add_source(w, wav, it);
             update_v(g, st, m, w, +1, nth);
@@ -546,6 +594,7 @@ template <typename T> struct Run {
     void born_forward(const float *wav, std::vector<T> &bz, std::vector<T> &bx, std::vector<T> &gz,
                       std::vector<T> &gx, Wave<T> *keep = nullptr, Strips<T> *strips = nullptr) {
         Wave<T> w(g.n), b(g.n);
+        if (c->rbc == 1) strips = nullptr;
         for (int it = 0; it < g.nt; ++it) {
             add_source(w, wav, it);
             update_v(g, st, m, w, +1, nth);
@@ -572,7 +621,7 @@ template <typename T> struct Run {
         std::vector<T> vx_gx(g.n, T(0)), vz_gz(g.n, T(0)), vx_gz(g.n, T(0)), vz_gx(g.n, T(0));
         std::vector<T> ovx, ovz;
         for (int it = g.nt - 1; it >= 0; --it) {
-            strips.template move<false>(fw.vx.data(), fw.vz.data(), it, g.nt);
+            if (c->rbc != 1) strips.template move<false>(fw.vx.data(), fw.vz.data(), it, g.nt);
             update_s(g, st, m, fw, -1, nth, vx_gx.data(), vz_gz.data(), vx_gz.data(), vz_gx.data());
             if (grho) {
                 ovx = fw.vx;

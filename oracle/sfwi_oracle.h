@@ -24,6 +24,17 @@
  * lines only): the rest of the orchestration of the faithful gradient (strip
  * restore with corners skipped, the reconstruction without source
  * subtraction, imaging after the adjoint stress-like half step).
+ *
+ * Random boundary (rbc): the reference draws it from cuRAND states seeded with
+ * time(0) (sfwi-stage1/FD2DGPU.cu:155-230, 3245-3305), so no run repeats it;
+ * this oracle and the library share a project-owned generator instead (the
+ * lowbias32 integer hash of C. Wellons, 2018, on seed, cell and attempt),
+ * computed in FP32 on both sides.  Pinned by the band values' range and vs/vp
+ * ratio, an untouched interior and zero damping.  Parity unpinned: the
+ * gradient's orchestration with it (no strips saved or restored,
+ * sfwi-stage1/FD2DGPU.cu:4543, 4602), a restatement; with the source left in
+ * the reconstruction (Q5) time reversal does not return the forward field, so
+ * no reversal check applies to the gradient itself.
  */
 #ifndef SFWI_ORACLE_H
 #define SFWI_ORACLE_H
@@ -44,6 +55,11 @@ typedef struct or_config {
     float rho_const;      /* > 0: padded rho forced to this value (stage-2 kernel)       */
     int pml_off;          /* test hook: damping profiles forced to zero                   */
     int nthreads;         /* OpenMP threads, 0 = runtime default                          */
+    int rbc;              /* 1: random boundary (random Vp / Vs in the PML bands, no      */
+                          /*    damping, no strips: the reverse pass reconstructs by time */
+                          /*    reversal alone); 2: the same medium with the strips kept  */
+                          /*    (test hook)                                               */
+    unsigned rbc_seed;    /* seed of the random boundary                                  */
 } or_config;
 
 /* FD coefficients c1..c4 of the 8th-order staggered stencil (index 0 unused). */

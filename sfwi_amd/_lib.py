@@ -34,6 +34,7 @@ class SfwiParams(ctypes.Structure):
         ("march_cols", ctypes.c_int), ("march_depth", ctypes.c_int), ("misfit", ctypes.c_int),
         ("march_order", ctypes.c_int), ("grad_rho", ctypes.c_int),
         ("cpu_mem", ctypes.c_int), ("strip_chunk", ctypes.c_int),
+        ("rbc", ctypes.c_int), ("rbc_seed", ctypes.c_uint),
     ]
 
 

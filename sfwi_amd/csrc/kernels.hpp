@@ -987,6 +987,53 @@ __global__ void k_medium(Geo g, float lg, float rho_const, float *vp, float *vs,
     ics[i] = 2.0f * s * r;
 }
 
+// Random boundary (rbc): the scheme of random_vel / set_random_boundary
+// (sfwi-stage1/FD2DGPU.cu:162-220, 3245-3290) with a project-owned generator,
+// the lowbias32 integer hash (C. Wellons, 2018) of seed, cell and attempt,
+// where the reference seeds cuRAND with time(0).  k = vmax / pml; PASS 0 sets
+// the z bands (top rows iz < pml and their mirrors nzp-1-iz, depth
+// d = pml - iz), PASS 1 then the x bands (columns, d = pml - ix; corners drawn
+// twice, as in the reference): vp' = vp + (2u - 1) k d redrawn until
+// 0 <= vp' <= vmax (64 draws, then vp clamped), vs' = (vs / vp) vp'.  FP32
+// throughout; the mirror is one cell inside the reference's nzpml-iz.
+__device__ inline unsigned rbc_hash(unsigned x) {
+    x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
+    return x;
+}
+// Largest (non-negative) Vp of the window: float bits order like unsigned ints.
+__global__ void k_vmax(int n, const float *__restrict__ vp, unsigned *vmax) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicMax(vmax, __float_as_uint(fmaxf(vp[i], 0.0f)));
+}
+__device__ inline void rbc_set(unsigned seed, int i, int pass, int d, float kk, float vmax, float *vp, float *vs) {
+    const float v0 = vp[i], s0 = vs[i];
+    const unsigned key = rbc_hash(seed ^ rbc_hash((unsigned)i * 2u + (unsigned)pass));
+    float vn = fminf(fmaxf(v0, 0.0f), vmax);
+    for (unsigned a = 0; a < 64; ++a) {
+        const float u = __uint2float_rz(rbc_hash(key + a) >> 8) * (1.0f / 16777216.0f);
+        const float t = __fadd_rn(v0, __fmul_rn(__fmul_rn(__fsub_rn(__fmul_rn(2.0f, u), 1.0f), kk), (float)d));
+        if (t >= 0.0f && t <= vmax) { vn = t; break; }
+    }
+    vs[i] = v0 > 0.0f ? __fmul_rn(__fdiv_rn(s0, v0), vn) : s0;
+    vp[i] = vn;
+}
+template <int PASS>
+__global__ void k_rbc_band(Geo g, unsigned seed, const unsigned *vmax_bits, float *vp, float *vs) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const float vmax = __uint_as_float(*vmax_bits), kk = __fdiv_rn(vmax, (float)g.pml);
+    if (PASS == 0) {
+        if (t >= g.nxp * g.pml) return;
+        const int ix = t / g.pml, iz = t - ix * g.pml;
+        rbc_set(seed, ix * g.nzp + iz, 0, g.pml - iz, kk, vmax, vp, vs);
+        rbc_set(seed, ix * g.nzp + g.nzp - 1 - iz, 0, g.pml - iz, kk, vmax, vp, vs);
+    } else {
+        if (t >= g.pml * g.nzp) return;
+        const int ix = t / g.nzp, iz = t - ix * g.nzp;
+        rbc_set(seed, ix * g.nzp + iz, 1, g.pml - ix, kk, vmax, vp, vs);
+        rbc_set(seed, (g.nxp - 1 - ix) * g.nzp + iz, 1, g.pml - ix, kk, vmax, vp, vs);
+    }
+}
+
 // Test model of the stage-1 line search: v - step*d (host order,
 // sfwi-stage1/FD2DGPU.cu:4902-4907).
 __global__ void k_axpy_model(int n, const float *__restrict__ v, const float *__restrict__ d, float step,

@@ -73,6 +73,7 @@ struct sfwi_ctx {
     std::vector<DevShot> h_shots;
     std::map<int, Window> windows;
     double *d_out = nullptr;      // per-shot misfits [max_batch] + 2 scalars
+    unsigned *d_vmax = nullptr;   // largest Vp of a window (random boundary)
     double *d_num = nullptr, *d_den = nullptr;
     bool have_model = false, have_wav = false, have_pert = false;
     // optimal-transport misfit: job table, FP64 workspace, per-job misfits, I/O staging
@@ -174,7 +175,15 @@ int build_window(sfwi_ctx *c, Window &w, int left, const float *vp, const float 
     hipLaunchKernelGGL(k_window, grd, blk, 0, c->stream, c->g, vp, left, a[0]);
     hipLaunchKernelGGL(k_window, grd, blk, 0, c->stream, c->g, vs, left, a[1]);
     hipLaunchKernelGGL(k_window, grd, blk, 0, c->stream, c->g, rho, left, a[2]);
-    const float lg = (float)std::log10((double)(1.0f / c->p.rr));
+    float lg = (float)std::log10((double)(1.0f / c->p.rr));
+    if (c->p.rbc) {   // random Vp / Vs in the PML bands, no damping
+        HIPCK(c, hipMemsetAsync(c->d_vmax, 0, sizeof(unsigned), c->stream));
+        hipLaunchKernelGGL(k_vmax, grd, blk, 0, c->stream, (int)n, a[0], c->d_vmax);
+        const dim3 gz((unsigned)((c->g.nxp * c->g.pml + 255) / 256)), gx((unsigned)((c->g.pml * c->g.nzp + 255) / 256));
+        hipLaunchKernelGGL(k_rbc_band<0>, gz, blk, 0, c->stream, c->g, c->p.rbc_seed, c->d_vmax, a[0], a[1]);
+        hipLaunchKernelGGL(k_rbc_band<1>, gx, blk, 0, c->stream, c->g, c->p.rbc_seed, c->d_vmax, a[0], a[1]);
+        lg = 0.0f;
+    }
     hipLaunchKernelGGL(k_medium, grd, blk, 0, c->stream, c->g, lg, c->p.rho_const, a[0], a[1], a[2], a[3],
                        a[4], a[5], a[6], a[7], a[8], a[9]);
     w.has_pert = dvp && dvs;
@@ -315,7 +324,7 @@ int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium 
     if (!c->have_wav) return fail(c, SFWI_ESTATE, "wavelet not set (sfwi_set_wavelet)");
     if (born && !override_m && !c->have_pert) return fail(c, SFWI_ESTATE, "perturbation not set (sfwi_set_perturbation)");
     if (n < 1 || n > c->p.max_batch) return fail(c, SFWI_EINVAL, "batch of %d shots outside [1,%d]", n, c->p.max_batch);
-    int r = ensure_slots(c, n, born, grad);
+    int r = ensure_slots(c, n, born, grad && !c->p.rbc);
     if (r) return r;
     c->h_shots.resize(n);
     for (int k = 0; k < n; ++k) {
@@ -533,7 +542,10 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
     // previous chunk has left.  Reverse: the last two chunks are resident;
     // once a chunk is consumed its half takes the chunk two below it, copied
     // while the chunk in between is computed.
-    const bool hs = grad && c->ring < nt;
+    // With the random boundary (rbc) nothing is saved: the reverse pass
+    // reconstructs by time reversal alone (sfwi-stage1/FD2DGPU.cu:4543, 4602).
+    const bool save = grad && !c->p.rbc;
+    const bool hs = save && c->ring < nt;
     const int ch = c->chunk, klast = (nt - 1) / ch;
     for (int it = 0; it < nt; ++it) {
         const int sp = hs ? it % c->ring : it;
@@ -542,23 +554,23 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
         int t = tbeg(c, SFWI_K_FWD_VEL, it);
         if (percell) {
             if (born) {
-                if (grad) hipLaunchKernelGGL((k_vel<E, true, true>), cells, blk, 0, st, g, D, sp, 1);
+                if (save) hipLaunchKernelGGL((k_vel<E, true, true>), cells, blk, 0, st, g, D, sp, 1);
                 else hipLaunchKernelGGL((k_vel<E, false, true>), cells, blk, 0, st, g, D, sp, 1);
             } else {
-                if (grad) hipLaunchKernelGGL((k_vel<E, true, false>), cells, blk, 0, st, g, D, sp, 1);
+                if (save) hipLaunchKernelGGL((k_vel<E, true, false>), cells, blk, 0, st, g, D, sp, 1);
                 else hipLaunchKernelGGL((k_vel<E, false, false>), cells, blk, 0, st, g, D, sp, 1);
             }
         } else {
             if (born) {
-                if (grad) hipLaunchKernelGGL((km_vel<true, true, HPD>), tiles, blk, 0, st, gm, D, sp, 1);
+                if (save) hipLaunchKernelGGL((km_vel<true, true, HPD>), tiles, blk, 0, st, gm, D, sp, 1);
                 else hipLaunchKernelGGL((km_vel<false, true, HPD>), tiles, blk, 0, st, gm, D, sp, 1);
             } else {
-                if (grad) hipLaunchKernelGGL((km_vel<true, false, PD>), tiles, blk, 0, st, gm, D, sp, 1);
+                if (save) hipLaunchKernelGGL((km_vel<true, false, PD>), tiles, blk, 0, st, gm, D, sp, 1);
                 else hipLaunchKernelGGL((km_vel<false, false, PD>), tiles, blk, 0, st, gm, D, sp, 1);
             }
         }
         tend(c, t);
-        if (grad && g.store) hipLaunchKernelGGL(k_strip_pack, dim3(2, (unsigned)n), dim3(256), 0, st, g, D, sp);
+        if (save && g.store) hipLaunchKernelGGL(k_strip_pack, dim3(2, (unsigned)n), dim3(256), 0, st, g, D, sp);
         t = tbeg(c, SFWI_K_FWD_STR, it);
         if (percell) {
             if (born) hipLaunchKernelGGL((k_str<E, false, true>), cells, blk, 0, st, g, D, 1);
@@ -599,7 +611,7 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
     auto rev_begin = [&](int it) -> int {   // the step's strips restored (after their chunk has arrived)
         if (hs && it % ch == ch - 1 && it / ch <= klast - 2)
             HIPCK(c, hipStreamWaitEvent(st, c->ev_copy[(it / ch) & 1], 0));
-        hipLaunchKernelGGL(k_strip_load, stripg, blk, 0, st, g, D, hs ? it % c->ring : it);
+        if (save) hipLaunchKernelGGL(k_strip_load, stripg, blk, 0, st, g, D, hs ? it % c->ring : it);
         return 0;
     };
     auto rev_end = [&](int it) -> int {     // chunk consumed: its ring half takes the chunk two below
@@ -774,6 +786,8 @@ int sfwi_ctx_create(const sfwi_params *p, sfwi_ctx **out) {
     if (p->misfit < SFWI_MISFIT_L2 || p->misfit > SFWI_MISFIT_OT_GATHER)
         return fail(nullptr, SFWI_EINVAL, "unknown misfit %d", p->misfit);
     if (p->cpu_mem != 0 && p->cpu_mem != 1) return fail(nullptr, SFWI_EINVAL, "cpu_mem must be 0 or 1");
+    if (p->rbc != 0 && p->rbc != 1) return fail(nullptr, SFWI_EINVAL, "rbc must be 0 or 1");
+    if (p->rbc && p->pml < 1) return fail(nullptr, SFWI_EINVAL, "rbc needs pml >= 1 (the random bands)");
     if (p->strip_chunk < 0) return fail(nullptr, SFWI_EINVAL, "strip_chunk must be >= 0");
     const long nxp = p->nx + 2L * p->pml, nzp = p->nz + 2L * p->pml;
     if (nxp * nzp >= (1L << 31) / 16) return fail(nullptr, SFWI_EINVAL, "grid too large for 32-bit cell indices");
@@ -829,6 +843,7 @@ int sfwi_ctx_create(const sfwi_params *p, sfwi_ctx **out) {
     if ((r = dalloc(c, (void **)&c->d_shots, p->max_batch * sizeof(DevShot)))) return bail(r);
     if ((r = dalloc(c, (void **)&c->d_out, (p->max_batch + 4) * sizeof(double)))) return bail(r);
     if ((r = dalloc(c, (void **)&c->d_num, 2 * (size_t)p->ntr * sizeof(double)))) return bail(r);
+    if ((r = dalloc(c, (void **)&c->d_vmax, 256))) return bail(r);
     c->d_den = c->d_num + p->ntr;
     if (hipMemsetAsync(c->Gvp, 0, 4 * c->nmodel * sizeof(float), c->stream) != hipSuccess ||
         hipMemsetAsync(c->mvp, 0, 5 * c->nmodel * sizeof(float), c->stream) != hipSuccess ||
@@ -858,7 +873,7 @@ void sfwi_ctx_destroy(sfwi_ctx *c) {
     drop_windows(c);
     for (hipEvent_t e : c->ev) hipEventDestroy(e);
     hipFree(c->wav); hipFree(c->mvp); hipFree(c->tmp_model); hipFree(c->Gvp); hipFree(c->res_keep);
-    hipFree(c->d_shots); hipFree(c->d_out); hipFree(c->d_num); hipFree(c->res_batch); hipFree(c->dot_batch);
+    hipFree(c->d_shots); hipFree(c->d_out); hipFree(c->d_num); hipFree(c->res_batch); hipFree(c->dot_batch); hipFree(c->d_vmax);
     hipFree(c->ot_jobs); hipFree(c->ot_ws); hipFree(c->ot_mis); hipFree(c->ot_first); hipFree(c->ot_io);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;

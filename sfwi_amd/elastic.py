@@ -41,7 +41,8 @@ class Shot:
 
 def make_params(*, dx, dz, dt, nx, nz, pml, nt, ntr, allnx, allnz, rr=0.001, rho_const=0.0, adjoint_dir=1,
                 max_batch=1, mode="fast", storage="f32", use_graph=False, march_cols=0, march_depth=0,
-                misfit="l2", march_order=0, grad_rho=False, cpu_mem=False, strip_chunk=0):
+                misfit="l2", march_order=0, grad_rho=False, cpu_mem=False, strip_chunk=0, rbc=False,
+                rbc_seed=0):
     """``sfwi_params`` of a context or request (keyword names as in the C struct)."""
     p = SfwiParams()
     p.dx, p.dz, p.dt = dx, dz, dt
@@ -59,6 +60,8 @@ def make_params(*, dx, dz, dt, nx, nz, pml, nt, ntr, allnx, allnz, rr=0.001, rho
     p.grad_rho = int(bool(grad_rho))
     p.cpu_mem = int(bool(cpu_mem))
     p.strip_chunk = int(strip_chunk)
+    p.rbc = 1 if rbc else 0
+    p.rbc_seed = int(rbc_seed) & 0xFFFFFFFF
     return p
 
 

@@ -14,7 +14,9 @@
 // (main.cpp:610-617), every aperture width here gets a context of its own;
 // shots of a width are solved in batches of max_batch (default 64).  With
 // cpu_mem = 1 in the file the boundary strips live in pinned host memory in
-// chunks of iointerval steps (the reference's ntt), as the reference does.
+// chunks of iointerval steps (the reference's ntt), as the reference does;
+// with rbc = 1 the PML bands take random velocities (seed 1) and no strips
+// are kept.
 // Per iteration: one line "iter <k> misfit <m> beta <b> alpha <a>" on stdout,
 // and, as the reference writes them, <outdir>/misfit.txt lines and
 // <outdir>/vel_p_<k>_<allnx>_<allnz>.dat, vel_s_<k>_<allnx>_<allnz>.dat.

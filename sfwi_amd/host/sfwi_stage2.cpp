@@ -16,7 +16,8 @@
 // are solved in batches of max_batch (default 64), the step's sums equal the
 // reference's per-shot CALCULATE_ALPHA calls added in shot order.  With
 // cpu_mem = 1 in the file the boundary strips live in pinned host memory in
-// chunks of iointerval steps.
+// chunks of iointerval steps; with rbc = 1 the PML bands take random
+// velocities (seed 1) and no strips are kept.
 // Per iteration: one line "iter <k> misfit <m> beta <b> alpha <a> fenzi <z>
 // fenmu <u>" on stdout and, as the reference writes them, <outdir>/misfit.txt
 // lines and <outdir>/sum_grad_{pp,ps}, sum_cggrad_{pp,ps}, sum_illum and

@@ -52,7 +52,6 @@ inline Survey read_survey(const char *para) {
     Survey sv;
     sfwi_para &p = sv.p;
     ck_input(sfwi_para_read(para, &p));
-    if (p.rbc) throw Fail(2, "input error: rbc (random boundary) is not supported; strips are used");
     if (p.cpu_mem && p.iointerval < 1) throw Fail(2, "input error: cpu_mem needs iointerval >= 1 (steps per chunk)");
     const size_t nm = (size_t)p.allnx * p.allnz;
     sv.vp.resize(nm);
@@ -106,6 +105,8 @@ inline sfwi_params width_params(const Survey &sv, int width, int max_batch) {
     q.max_batch = std::min<int>(max_batch, (int)sv.members.at(width).size());
     q.cpu_mem = p.cpu_mem ? 1 : 0;      // strips in pinned host memory, iointerval steps per chunk
     q.strip_chunk = p.cpu_mem ? p.iointerval : 0;
+    q.rbc = p.rbc ? 1 : 0;              // random boundary, no strips (cpu_mem then holds nothing)
+    q.rbc_seed = 1;                     // the reference seeds from the clock; a run here repeats
     return q;
 }
 

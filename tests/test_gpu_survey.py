@@ -165,6 +165,24 @@ def test_stage1_overlapping_windows_in_batches(tmp_path):
     got = np.fromfile(tmp_path / "out" / ("vel_s_1_%d_%d.dat" % (sv.allnx, sv.allnz)), np.float32)
     assert np.array_equal(got.reshape(vs.shape), vs)
 
+    # rbc = 1 in the file: the random boundary (seed 1), no strips; the program
+    # equals the Python loop on contexts with the same boundary, bit for bit
+    para3 = tmp_path / "para3"
+    para3.write_text(para.read_text().replace("1 0 0 1000\n", "1 1 0 1000\n").replace(str(tmp_path / "out"),
+                                                                                    str(tmp_path / "out3")))
+    r3 = subprocess.run([PROG, str(para3), "2", "3", str(CUT)], capture_output=True, text=True, timeout=300)
+    assert r3.returncode == 0, r3.stderr
+    st3 = Stage1(sv.allnx, sv.allnz, cut_layer=CUT)
+    vp3, vs3 = sv.vp0.copy(), sv.vs0.copy()
+    with _ctx(sv, 30, 3, rbc=True, rbc_seed=1) as c:
+        c.set_wavelet(w)
+        its3 = [st3.iterate_groups([(c, shots)], vp3, vs3, rho) for _ in range(2)]
+    for l, it in zip([l.split() for l in r3.stdout.strip().splitlines()], its3):
+        assert float(l[3]) == it["misfit"] and float(l[7]) == it["alpha"]
+    got = np.fromfile(tmp_path / "out3" / ("vel_p_1_%d_%d.dat" % (sv.allnx, sv.allnz)), np.float32)
+    assert np.array_equal(got.reshape(vp.shape), vp3)
+    assert not np.array_equal(vp3, vp)          # the boundary changed the inversion
+
     # cpu_mem = 1, iointerval = 50 in the file: strips in host memory in chunks of
     # 50 steps (the last one partial); same output bit for bit
     para2 = tmp_path / "para2"

@@ -21,6 +21,12 @@ operands) would break:
     recomputed from independent forward solves.
   * Adjoint injection: the residual overwrites the adjoint velocity and the
     last of two coincident traces wins (neither additive nor first-wins).
+  * Random boundary (rbc): band velocities within [0, vmax] with the vs / vp
+    ratio kept, the interior and density untouched, no damping, a repeatable
+    seed.  Its reconstruction without strips is the propagator's inverse
+    (pinned above with the PML off); that the reference then skips the strips
+    and, as everywhere, leaves the source in the reconstruction is only
+    restated (parity unpinned).
   * Imaging: with kept forward fields, additive injection and imaging between
     the adjoint half steps, the imaging step gives the exact gradient of the
     L2 misfit (PML off): central differences converge to it at second order
@@ -324,3 +330,36 @@ def test_imaging_formula_is_the_discrete_gradient(param):
         errs.append(abs((jp - jm) / 2 - an) / abs(an))
     assert errs[-1] < 5e-4, errs
     assert 3.0 < errs[0] / errs[1] < 5.0 and 3.0 < errs[1] / errs[2] < 5.0, errs     # second order
+
+
+def test_random_boundary_medium():
+    """rbc = 1: every PML-band cell's Vp is redrawn within [0, vmax] (vmax the
+    window's largest Vp) with vs / vp kept; the interior, the density and the
+    seed's repeatability hold; damping is zero; another seed gives another
+    boundary."""
+    sv = W.tiny(nt=10, seed=4)
+    c0 = _cfg(nx=40, nz=36, pml=8)
+    vp, vs, rho = sv.vp[:45, :36].copy(), sv.vs[:45, :36].copy(), sv.rho[:45, :36].copy()
+    base = O.setup(c0, vp, vs, rho)
+    a = O.setup(_cfg(nx=40, nz=36, pml=8, rbc=1, rbc_seed=11), vp, vs, rho)
+    b = O.setup(_cfg(nx=40, nz=36, pml=8, rbc=1, rbc_seed=11), vp, vs, rho)
+    d = O.setup(_cfg(nx=40, nz=36, pml=8, rbc=1, rbc_seed=12), vp, vs, rho)
+    assert np.array_equal(a, b) and not np.array_equal(a[0], d[0])
+    p = 8
+    band = np.ones(a[0].shape, bool)
+    band[p:-p, p:-p] = False
+    inner = ~band
+    assert np.array_equal(a[0][inner], base[0][inner]) and np.array_equal(a[1][inner], base[1][inner])
+    assert np.array_equal(a[2], base[2])
+    assert not a[5].any() and not a[6].any()
+    vmax = base[0].max()
+    assert a[0][band].min() >= 0 and a[0][band].max() <= vmax
+    assert np.mean(a[0][band] != base[0][band]) > 0.95          # redrawn, not copied
+    ratio0 = base[1][band] / base[0][band]
+    assert np.allclose(a[1][band] / a[0][band], ratio0, rtol=1e-5)
+    # the Lame parameters follow the new velocities
+    assert np.allclose(a[4], a[2] * a[1] ** 2, rtol=1e-6)
+    # the perturbation grows with the depth into the band: outer rows spread more
+    dev = np.abs(a[0] - base[0])
+    assert dev[p:-p, 0].mean() > dev[p:-p, p - 1].mean()
+
