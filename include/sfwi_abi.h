@@ -198,6 +198,20 @@ int sfwi_linesearch_linear(sfwi_ctx *ctx, const sfwi_shot *shot, const float *cg
  * batch stay resident in its slot, as after a gradient call. */
 int sfwi_batch_linesearch_linear(sfwi_ctx *ctx, int nshots, const sfwi_shot *shots, const float *cg_vp,
                                  const float *cg_vs, double *fenzi, double *fenmu);
+/* No reference counterpart: the stage-2 step's first Born solve recomputes the
+ * residual born(dm) - obs + background that the Born gradient call at the
+ * same model and images has just formed.  sfwi_batch_born_gradient_res is
+ * sfwi_batch_born_gradient (L2 misfit) that also returns each shot's residual
+ * (shot i at res_z + i*ntr_max*nt, res_x likewise, ntr_max = sfwi_params.ntr);
+ * sfwi_batch_linesearch_linear_res takes those residuals in place of that
+ * solve (observed traces not needed) and returns the same sums as
+ * sfwi_batch_linesearch_linear bit for bit, one Born solve per shot fewer.
+ * The caller keeps model, images and shots unchanged in between. */
+int sfwi_batch_born_gradient_res(sfwi_ctx *ctx, int nshots, const sfwi_shot *shots, double *misfits,
+                                 float *res_z, float *res_x);
+int sfwi_batch_linesearch_linear_res(sfwi_ctx *ctx, int nshots, const sfwi_shot *shots, const float *cg_vp,
+                                     const float *cg_vs, const float *res_z, const float *res_x,
+                                     double *fenzi, double *fenmu);
 
 /* Gradient flush, FD2DGPU_ELASTIC::imagebuffer_resettozero
  * (sfwi-stage1/FD2DGPU.cu:3646-3673): copies the accumulated Vp and Vs

@@ -109,6 +109,8 @@ def load():
                                                  ctypes.c_float, dp]
     L.sfwi_linesearch_linear.argtypes = [vp, P(SfwiShot), _f32p, _f32p, dp, dp]
     L.sfwi_batch_linesearch_linear.argtypes = [vp, ctypes.c_int, P(SfwiShot), _f32p, _f32p, dp, dp]
+    L.sfwi_batch_born_gradient_res.argtypes = [vp, ctypes.c_int, P(SfwiShot), dp, vp, vp]
+    L.sfwi_batch_linesearch_linear_res.argtypes = [vp, ctypes.c_int, P(SfwiShot), _f32p, _f32p, vp, vp, dp, dp]
     L.sfwi_flush.argtypes = [vp, vp, vp, vp]
     L.sfwi_flush_rho.argtypes = [vp, vp]
     L.sfwi_ot_misfit.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,

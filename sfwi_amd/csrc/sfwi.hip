@@ -732,7 +732,10 @@ int finish(sfwi_ctx *c) {
 }
 
 // Batched solve with misfit output; shots processed in groups of max_batch.
-int solve_misfits(sfwi_ctx *c, Kind kind, int nshots, const sfwi_shot *shots, double *misfits) {
+int copy_records(sfwi_ctx *c, const float *src, float *dst, int ntr);
+
+int solve_misfits(sfwi_ctx *c, Kind kind, int nshots, const sfwi_shot *shots, double *misfits,
+                  float *res_z = nullptr, float *res_x = nullptr) {
     if (nshots < 0 || (nshots > 0 && !shots)) return fail(c, SFWI_EINVAL, "bad shot list");
     for (int b0 = 0; b0 < nshots; b0 += c->p.max_batch) {
         const int n = std::min(c->p.max_batch, nshots - b0);
@@ -741,6 +744,12 @@ int solve_misfits(sfwi_ctx *c, Kind kind, int nshots, const sfwi_shot *shots, do
         if ((r = run(c, kind, n))) return r;
         std::vector<double> m(n);
         HIPCK(c, hipMemcpyAsync(m.data(), c->d_out, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        for (int k = 0; k < n && res_z; ++k) {   // each shot's residual (the adjoint source)
+            const size_t o = (size_t)(b0 + k) * c->nrec;
+            if ((r = copy_records(c, c->slots[k].rz, res_z + o, shots[b0 + k].ntr)) ||
+                (r = copy_records(c, c->slots[k].rx, res_x + o, shots[b0 + k].ntr)))
+                return r;
+        }
         if ((r = finish(c))) return r;
         if (misfits)
             for (int k = 0; k < n; ++k) misfits[b0 + k] += m[k];
@@ -930,6 +939,13 @@ int sfwi_batch_born_gradient(sfwi_ctx *c, int nshots, const sfwi_shot *shots, do
     if (!c) return SFWI_EINVAL;
     return solve_misfits(c, K_BORN_GRAD, nshots, shots, misfits);
 }
+int sfwi_batch_born_gradient_res(sfwi_ctx *c, int nshots, const sfwi_shot *shots, double *misfits, float *res_z,
+                                 float *res_x) {
+    if (!c) return SFWI_EINVAL;
+    if (!res_z || !res_x) return fail(c, SFWI_EINVAL, "NULL residuals");
+    if (c->p.misfit != SFWI_MISFIT_L2) return fail(c, SFWI_EINVAL, "residuals are those of the L2 misfit");
+    return solve_misfits(c, K_BORN_GRAD, nshots, shots, misfits, res_z, res_x);
+}
 
 int sfwi_shot_forward(sfwi_ctx *c, const sfwi_shot *shot, float *syn_z, float *syn_x) {
     if (!c || !shot) return fail(c, SFWI_EINVAL, "NULL argument");
@@ -1085,13 +1101,22 @@ int sfwi_linesearch_linear(sfwi_ctx *c, const sfwi_shot *shot, const float *cg_v
     return 0;
 }
 
-int sfwi_batch_linesearch_linear(sfwi_ctx *c, int nshots, const sfwi_shot *shots, const float *cg_vp,
-                                 const float *cg_vs, double *fenzi, double *fenmu) {
+}  // extern "C"
+
+namespace {
+
+// The batched linear step; with res_z / res_x (the residuals of a Born
+// gradient call at the same model and images, [nshots][ntr][nt]) the first
+// Born solve is replaced by their upload.
+int linesearch_linear_batch(sfwi_ctx *c, int nshots, const sfwi_shot *shots, const float *cg_vp,
+                            const float *cg_vs, const float *res_z, const float *res_x, double *fenzi,
+                            double *fenmu) {
     if (!c || (nshots > 0 && !shots) || nshots < 0 || !cg_vp || !cg_vs || !fenzi || !fenmu)
         return fail(c, SFWI_EINVAL, "NULL argument or negative shot count");
     if (!c->have_model) return fail(c, SFWI_ESTATE, "model not set");
     if (!c->have_pert) return fail(c, SFWI_ESTATE, "perturbation not set (sfwi_set_perturbation)");
-    for (int s = 0; s < nshots; ++s)
+    if (!res_z != !res_x) return fail(c, SFWI_EINVAL, "res_z and res_x must both be given or both be NULL");
+    for (int s = 0; s < nshots && !res_z; ++s)
         if (!shots[s].obs_z || !shots[s].obs_x) return fail(c, SFWI_EINVAL, "shot %d: observed traces missing", s);
     if (nshots == 0) return 0;
     const int nb = std::min(c->p.max_batch, nshots);
@@ -1117,9 +1142,20 @@ int sfwi_batch_linesearch_linear(sfwi_ctx *c, int nshots, const sfwi_shot *shots
     std::vector<double> sums(2 * (size_t)nb * c->p.ntr);
     for (int b0 = 0; b0 < nshots; b0 += nb) {
         const int n = std::min(nb, nshots - b0);
+        if (res_z) {   // residuals handed over by the Born gradient call
+            for (int k = 0; k < n; ++k) {
+                const size_t rb = (size_t)shots[b0 + k].ntr * c->p.nt * sizeof(float);
+                const size_t o = (size_t)(b0 + k) * c->nrec;
+                float *keep = c->res_batch + 2 * (size_t)k * c->nrec;
+                if (hipMemcpyAsync(keep, res_z + o, rb, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+                    hipMemcpyAsync(keep + c->nrec, res_x + o, rb, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+                    return done(fail(c, SFWI_EHIP, "copy failed"));
+            }
+        }
         // Solve 1 over the batch: residual r = born(dm) - obs + background, kept per slot.
-        if ((r = prepare(c, K_BORN_FWD, n, shots + b0, nullptr)) || (r = run(c, K_BORN_FWD, n))) return done(r);
-        for (int k = 0; k < n; ++k) {
+        if (!res_z && ((r = prepare(c, K_BORN_FWD, n, shots + b0, nullptr)) || (r = run(c, K_BORN_FWD, n))))
+            return done(r);
+        for (int k = 0; k < n && !res_z; ++k) {
             Slot &S = c->slots[k];
             const size_t rb = (size_t)shots[b0 + k].ntr * c->p.nt * sizeof(float);
             if (hipMemcpyAsync(S.oz, shots[b0 + k].obs_z, rb, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
@@ -1127,8 +1163,8 @@ int sfwi_batch_linesearch_linear(sfwi_ctx *c, int nshots, const sfwi_shot *shots
                 return done(fail(c, SFWI_EHIP, "copy failed"));
             S.obs_ntr = shots[b0 + k].ntr;
         }
-        hipLaunchKernelGGL(k_residual<true>, dim3(256, n), dim3(BLK), 0, c->stream, c->g, c->d_shots);
-        for (int k = 0; k < n; ++k) {
+        if (!res_z) hipLaunchKernelGGL(k_residual<true>, dim3(256, n), dim3(BLK), 0, c->stream, c->g, c->d_shots);
+        for (int k = 0; k < n && !res_z; ++k) {
             Slot &S = c->slots[k];
             const size_t rb = (size_t)shots[b0 + k].ntr * c->p.nt * sizeof(float);
             float *keep = c->res_batch + 2 * (size_t)k * c->nrec;
@@ -1158,6 +1194,22 @@ int sfwi_batch_linesearch_linear(sfwi_ctx *c, int nshots, const sfwi_shot *shots
         }
     }
     return done(0);
+}
+
+}  // namespace
+
+extern "C" {
+
+int sfwi_batch_linesearch_linear(sfwi_ctx *c, int nshots, const sfwi_shot *shots, const float *cg_vp,
+                                 const float *cg_vs, double *fenzi, double *fenmu) {
+    return linesearch_linear_batch(c, nshots, shots, cg_vp, cg_vs, nullptr, nullptr, fenzi, fenmu);
+}
+
+int sfwi_batch_linesearch_linear_res(sfwi_ctx *c, int nshots, const sfwi_shot *shots, const float *cg_vp,
+                                     const float *cg_vs, const float *res_z, const float *res_x, double *fenzi,
+                                     double *fenmu) {
+    if (!res_z || !res_x) return fail(c, SFWI_EINVAL, "NULL residuals");
+    return linesearch_linear_batch(c, nshots, shots, cg_vp, cg_vs, res_z, res_x, fenzi, fenmu);
 }
 
 int sfwi_flush_rho(sfwi_ctx *c, float *grho) {

@@ -186,6 +186,31 @@ class Context:
                                                      ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
+    def batch_born_gradient_res(self, shots):
+        """batch_gradient(born=True) that also returns each shot's residual:
+        (misfits, res_z, res_x), the residuals (nshots, ntr_max, nt)."""
+        arr = (SfwiShot * len(shots))(*[s.c for s in shots])
+        m = np.zeros(len(shots), np.float64)
+        z = np.zeros((len(shots), self.p.ntr, self.nt), np.float32)
+        x = np.zeros_like(z)
+        self._ck(self.L.sfwi_batch_born_gradient_res(self.h, len(shots), arr,
+                                                     m.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                     _lib.ptr(z), _lib.ptr(x)))
+        return m, z, x
+
+    def batch_linesearch_linear_res(self, shots, cg_vp, cg_vs, res_z, res_x, fenzi=0.0, fenmu=0.0):
+        """batch_linesearch_linear with the residuals of batch_born_gradient_res
+        at the same model and images in place of its first Born solve."""
+        a, b = ctypes.c_double(fenzi), ctypes.c_double(fenmu)
+        arr = (SfwiShot * len(shots))(*[s.c for s in shots])
+        rz = np.ascontiguousarray(res_z, np.float32)
+        rx = np.ascontiguousarray(res_x, np.float32)
+        assert rz.shape == (len(shots), self.p.ntr, self.nt) and rx.shape == rz.shape
+        self._ck(self.L.sfwi_batch_linesearch_linear_res(self.h, len(shots), arr, _f32(cg_vp), _f32(cg_vs),
+                                                         _lib.ptr(rz), _lib.ptr(rx), ctypes.byref(a),
+                                                         ctypes.byref(b)))
+        return a.value, b.value
+
     def flush(self):
         sh = (self.allnx, self.allnz)
         g = [np.zeros(sh, np.float32) for _ in range(3)]

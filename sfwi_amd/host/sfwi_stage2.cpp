@@ -14,7 +14,9 @@
 // and the adjoint runs in the stage-2 direction, as the stage-2 kernels hard
 // code them.  Every aperture width gets a context of its own; shots of a width
 // are solved in batches of max_batch (default 64), the step's sums equal the
-// reference's per-shot CALCULATE_ALPHA calls added in shot order.  With
+// reference's per-shot CALCULATE_ALPHA calls added in shot order; the step
+// takes each shot's residual from the gradient call in place of its first
+// Born solve.  With
 // cpu_mem = 1 in the file the boundary strips live in pinned host memory in
 // chunks of iointerval steps; with rbc = 1 the PML bands take random
 // velocities (seed 1) and no strips are kept.
@@ -63,6 +65,10 @@ int main(int argc, char **argv) {
         std::vector<float> pp_cg(nm), ps_cg(nm), pp_cg_old(nm, 0.0f), ps_cg_old(nm, 0.0f);
         for (int k = 0; k < 3; ++k) { g[k].resize(nm); t[k].resize(nm); }
         double beta_fenmu = 0.0, max_misfit = 0.0;
+        // each width's residuals from its gradient calls, handed to the step
+        // in place of its first Born solve (the same sums bit for bit)
+        const size_t rec = (size_t)sv.ntr_max * sv.nt;
+        std::vector<std::vector<float>> res_z(sv.widths.size()), res_x(sv.widths.size());
         for (int iter = 0; iter < niter; ++iter) {
             double misfit = 0.0;
             for (size_t gi = 0; gi < sv.widths.size(); ++gi) {
@@ -70,11 +76,15 @@ int main(int argc, char **argv) {
                 const std::vector<sfwi_shot> grp = group_shots(sv, sv.widths[gi]);
                 ck(sfwi_set_model(c, sv.vp.data(), sv.vs.data(), nullptr), c);
                 ck(sfwi_set_perturbation(c, image_pp.data(), image_ps.data()), c);
+                res_z[gi].assign(grp.size() * rec, 0.0f);
+                res_x[gi].assign(grp.size() * rec, 0.0f);
                 const int nb = std::min<int>(max_batch, (int)grp.size());
                 for (size_t b0 = 0; b0 < grp.size(); b0 += nb) {
                     const int n = (int)std::min<size_t>(nb, grp.size() - b0);
                     std::vector<double> m(n, 0.0);
-                    ck(sfwi_batch_born_gradient(c, n, grp.data() + b0, m.data()), c);
+                    ck(sfwi_batch_born_gradient_res(c, n, grp.data() + b0, m.data(), res_z[gi].data() + b0 * rec,
+                                                    res_x[gi].data() + b0 * rec),
+                       c);
                     for (double v : m) misfit += v;
                 }
                 ck(sfwi_flush(c, t[0].data(), t[1].data(), t[2].data()), c);
@@ -98,8 +108,9 @@ int main(int argc, char **argv) {
             double fenzi = 0.0, fenmu = 0.0;
             for (size_t gi = 0; gi < sv.widths.size(); ++gi) {   // batched; the sums of per-shot calls in shot order
                 const std::vector<sfwi_shot> grp = group_shots(sv, sv.widths[gi]);
-                ck(sfwi_batch_linesearch_linear(ctxs[gi], (int)grp.size(), grp.data(), pp_cg.data(), ps_cg.data(),
-                                                &fenzi, &fenmu),
+                ck(sfwi_batch_linesearch_linear_res(ctxs[gi], (int)grp.size(), grp.data(), pp_cg.data(),
+                                                    ps_cg.data(), res_z[gi].data(), res_x[gi].data(), &fenzi,
+                                                    &fenmu),
                    ctxs[gi]);
             }
             double alpha = 0.0;

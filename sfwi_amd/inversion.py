@@ -150,24 +150,35 @@ class Stage2(_CG):
         context per shot aperture width: gradients summed over the groups in
         list order, misfits and the step's sums over the shots in order.
         per_shot: the step's Born solves one shot per call (the reference's
-        loop of CALCULATE_ALPHA calls; the same sums bit for bit)."""
+        loop of CALCULATE_ALPHA calls); otherwise batched, with the gradient
+        call's residuals in place of the step's first Born solve (the same
+        sums bit for bit)."""
         misfit = 0.0
         G = None
+        res = []   # batched step: each group's residuals from its gradient call
         for ctx, shots in groups:
             ctx.set_model(vp, vs, rho)
             ctx.set_perturbation(image_pp, image_ps)
-            for v in _batched(ctx, shots, lambda b: ctx.batch_gradient(b, born=True)):
-                misfit += float(v)
+            if per_shot:
+                for v in _batched(ctx, shots, lambda b: ctx.batch_gradient(b, born=True)):
+                    misfit += float(v)
+            else:
+                nb = max(int(ctx.p.max_batch), 1)
+                parts = [ctx.batch_born_gradient_res(shots[i:i + nb]) for i in range(0, len(shots), nb)]
+                for m, _, _ in parts:
+                    for v in m:
+                        misfit += float(v)
+                res.append((np.concatenate([p[1] for p in parts]), np.concatenate([p[2] for p in parts])))
             g = ctx.flush()
             G = g if G is None else tuple((a + b).astype(np.float32) for a, b in zip(G, g))
         c1, c2, beta, _, _ = self.direction(*G)
         fenzi = fenmu = 0.0
-        for ctx, shots in groups:
+        for k, (ctx, shots) in enumerate(groups):
             if per_shot:
                 for s in shots:
                     fenzi, fenmu = ctx.linesearch_linear(s, c1, c2, fenzi, fenmu)
-            else:
-                fenzi, fenmu = ctx.batch_linesearch_linear(shots, c1, c2, fenzi, fenmu)
+            else:   # the residuals stand in for the step's first Born solve (same sums bit for bit)
+                fenzi, fenmu = ctx.batch_linesearch_linear_res(shots, c1, c2, res[k][0], res[k][1], fenzi, fenmu)
         alpha = self.update(image_pp, image_ps, c1, c2, fenzi, fenmu)
         rec = dict(iter=self.iter, misfit=misfit, rms=self._rms(misfit), beta=beta, alpha=alpha,
                    fenzi=fenzi, fenmu=fenmu)

@@ -464,6 +464,39 @@ def test_batch_linesearch_linear_equals_per_shot_calls():
             assert np.array_equal(x, y) and np.array_equal(x, z)
 
 
+def test_linear_step_with_gradient_residuals():
+    """The Born gradient's residuals handed to the linear step in place of its
+    first Born solve give the sums of the step that recomputes them, bit for
+    bit (batches of 2 + 1 over overlapping windows, both modes), and the same
+    misfits and gradients as the plain Born gradient call."""
+    sv, ntr = _overlap_case()
+    w = W.ricker(sv.nt, sv.dt, sv.f0)
+    lib = [lib_shot(sv, s, *_observed(sv, s, w)) for s in sv.shots]
+    dvp = (0.5 * (sv.vp - sv.vp0)).astype(np.float32)
+    dvs = (0.5 * (sv.vs - sv.vs0)).astype(np.float32)
+    rng = np.random.default_rng(8)
+    c1 = (rng.standard_normal(sv.vp.shape) * 0.1).astype(np.float32)
+    c2 = (rng.standard_normal(sv.vp.shape) * 0.05).astype(np.float32)
+    for mode in ("fast", "exact"):
+        with context(sv, mode, max_batch=2, ntr=ntr, rho_const=2000.0, adjoint_dir=-1) as ctx:
+            ctx.set_wavelet(w)
+            ctx.set_model(sv.vp0, sv.vs0, None)
+            ctx.set_perturbation(dvp, dvs)
+            m0 = np.concatenate([ctx.batch_gradient(lib[:2], born=True), ctx.batch_gradient(lib[2:], born=True)])
+            g0 = ctx.flush()
+            parts = [ctx.batch_born_gradient_res(lib[:2]), ctx.batch_born_gradient_res(lib[2:])]
+            g1 = ctx.flush()
+            rz = np.concatenate([p[1] for p in parts])
+            rx = np.concatenate([p[2] for p in parts])
+            ref = ctx.batch_linesearch_linear(lib, c1, c2)
+            got = ctx.batch_linesearch_linear_res(lib, c1, c2, rz, rx)
+        assert np.array_equal(np.concatenate([p[0] for p in parts]), m0)
+        for a, b in zip(g0, g1):
+            assert np.array_equal(a, b)
+        assert np.abs(rz).max() > 0 and ref[1] > 0
+        assert got == ref, (mode, got, ref)
+
+
 def test_batch_linesearch_equals_per_shot_calls():
     """sfwi_batch_linesearch_parabola over shots with overlapping windows at
     three offsets, in batches of 2 + 1 (max_batch 2), equals the per-shot
