@@ -1,7 +1,7 @@
 """Host steps of the reference's inversion loops -- CPU oracle (numpy).
 
-TEST INFRASTRUCTURE ONLY: the checker for the library's sfwi_s1_* / sfwi_s2_*
-host functions and for the inversion iterations built on them.  Only tests/,
+TEST INFRASTRUCTURE ONLY: the oracle of the library's sfwi_s1_* / sfwi_s2_*
+host functions and of the inversion iterations built on them.  Only tests/,
 the smoke check and bench.py's cpu_baseline leg may import it.
 
 Each function restates the reference's arithmetic type by type: FP32 arrays

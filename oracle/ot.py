@@ -1,6 +1,6 @@
 """Optimal-transport trace misfit and adjoint source -- CPU oracle (numpy, FP64).
 
-TEST INFRASTRUCTURE ONLY: the checker for the HIP implementation
+TEST INFRASTRUCTURE ONLY: the oracle of the HIP implementation
 (sfwi_ot_misfit and the OT misfit option of the gradient path).  Only tests/,
 the smoke check and bench.py's cpu_baseline leg may import it.
 

@@ -1,6 +1,6 @@
 // CPU oracle for SFWI's per-shot elastic modeling + gradient path.
 //
-// TEST INFRASTRUCTURE ONLY: this file is the checker for the HIP library in
+// TEST INFRASTRUCTURE ONLY: this file is the test oracle of the HIP library in
 // sfwi_amd/.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
 // leg load it; the product never does.
 //

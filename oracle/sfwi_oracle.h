@@ -1,7 +1,7 @@
 /*
  * CPU oracle for SFWI's per-shot elastic modeling + gradient path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker: tests/, the smoke
+ * TEST INFRASTRUCTURE ONLY.  This library is the test oracle: tests/, the smoke
  * check in __graft_entry__.py and bench.py's cpu_baseline leg may load it,
  * nothing in sfwi_amd/ may.  It restates the reference algorithm on the CPU in
  * two precisions:

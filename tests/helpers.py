@@ -1,5 +1,5 @@
 """Shared helpers of the test suite: oracle configs for a workload shot and
-tolerance checks.  The oracle is the checker; the HIP library is the product."""
+tolerance checks.  The oracle is what results are compared with; the HIP library is the product."""
 import numpy as np
 
 from oracle import oracle as O
