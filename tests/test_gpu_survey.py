@@ -264,6 +264,27 @@ def test_stage2_from_su_files(tmp_path):
     assert np.array_equal(i0, (0.0 - its[0]["alpha"] * cg.astype(np.float64)).astype(np.float32))
     assert (out / "misfit.txt").read_text().count("misfit") == 2
 
+    # rbc = 1: the random boundary (seed 1) in the program and in the Python loop, bit for bit
+    para3 = tmp_path / "para3"
+    para3.write_text(para.read_text().replace("1 0 0 1000\n", "1 1 0 1000\n").replace(str(out), str(tmp_path / "out3")))
+    r3 = subprocess.run([PROG2, str(para3), "2", "64", str(CUT)], capture_output=True, text=True, timeout=300)
+    assert r3.returncode == 0, r3.stderr
+    ctxs = {wd: _ctx(sv, wd, len(ix), rho_const=2000.0, adjoint_dir=-1, rbc=True, rbc_seed=1)
+            for wd, ix in su.groups().items()}
+    for c in ctxs.values():
+        c.set_wavelet(w)
+    groups = [(ctxs[wd], [su.shots[k] for k in ix]) for wd, ix in su.groups().items()]
+    st3 = Stage2(sv.allnx, sv.allnz, cut_layer=CUT)
+    ip3, is3 = np.zeros_like(su.vp), np.zeros_like(su.vp)
+    its3 = [st3.iterate_groups(groups, su.vp, su.vs, None, ip3, is3) for _ in range(2)]
+    for c in ctxs.values():
+        c.close()
+    for l, it in zip([l.split() for l in r3.stdout.strip().splitlines()], its3):
+        assert [float(l[j]) for j in (3, 5, 7, 9, 11)] == [it[k] for k in key]
+    f = "sum_image_pp_1_%d_%d.dat" % (sv.allnx, sv.allnz)
+    assert np.array_equal(np.fromfile(tmp_path / "out3" / f, np.float32).reshape(ip3.shape), ip3)
+    assert not np.array_equal(ip3, ip)
+
     # cpu_mem = 1, iointerval = 50: boundary strips in pinned host memory, same output bit for bit
     para2 = tmp_path / "para2"
     para2.write_text(para.read_text().replace("1 0 0 1000\n", "1 0 1 50\n").replace(str(out), str(tmp_path / "out2")))
