@@ -129,7 +129,8 @@ typedef struct sfwi_shot {
 int sfwi_ctx_create(const sfwi_params *p, sfwi_ctx **out);
 /* Replaces ~FD2DGPU_ELASTIC. */
 void sfwi_ctx_destroy(sfwi_ctx *ctx);
-/* Text of the last error on this context (or of a failed create when ctx is NULL). */
+/* Text of the last error on this context; with ctx NULL, of the calling
+ * thread's last failed call without a context (create, server calls). */
 const char *sfwi_last_error(const sfwi_ctx *ctx);
 
 /* Source wavelet (nt floats): the `sou` constructor argument, uploaded in
@@ -257,7 +258,9 @@ int sfwi_ot_misfit(sfwi_ctx *ctx, int variant, int ntr, int nt, float dt, const 
  * least recently used first when memory is needed.  Each request's result is
  * the one the same solve gives alone.  All buffers of a request are
  * caller-owned and must stay valid until sfwi_server_wait returns for it.
- * Errors: negative codes; the message is sfwi_last_error(NULL).
+ * Errors: negative codes; the message is sfwi_last_error(NULL) on the calling
+ * thread (each host thread keeps its own; a failed request's message is
+ * returned to the thread that waits for it).
  * ------------------------------------------------------------------------- */
 #define SFWI_REQ_FORWARD 0          /* sfwi_batch_forward                       */
 #define SFWI_REQ_BORN_FORWARD 1     /* sfwi_shot_born_forward per shot          */

@@ -7,7 +7,10 @@
  * two precisions:
  *   *_f32 : the reference's FP32 operation order, including the places where a
  *           double literal promotes an expression to FP64 (built with
- *           -ffp-contract=off so no FMA is formed that the C source lacks);
+ *           -ffp-contract=off so no FMA is formed that the C source lacks).
+ *           This is the uncontracted reading of the source text: the
+ *           reference's own nvcc -O3 build contracts a*b+c into FMAs by
+ *           default, so ulp-level parity with its binary is unpinned;
  *   *_f64 : every array and every operation in FP64 (accuracy oracle and timed
  *           CPU baseline).
  *

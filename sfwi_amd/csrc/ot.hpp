@@ -20,6 +20,7 @@
 // workspace of 4 (n + 1) doubles; the scans over chunk totals run on one
 // thread (256 terms) so the result does not depend on timing.
 #pragma once
+#include "ot_walk.hpp"
 
 namespace sfwi {
 
@@ -104,19 +105,8 @@ __device__ inline double ot_block_suffix(double v, double *red) {
     return p;
 }
 
-// Largest k in [0, n-1] with Q[k] <= y (Q[0] = 0 <= y).
-__device__ inline int ot_floor_index(const double *Q, int n, double y) {
-    int lo = 0, hi = n;   // invariant: Q[lo] <= y, answer in [lo, hi)
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (Q[mid] <= y) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
 __global__ void __launch_bounds__(OT_THREADS)
-k_ot(const OtJob *__restrict__ jobs, int njobs, double *__restrict__ ws, size_t ws_stride, double dt2) {
+k_ot(const OtJob *__restrict__ jobs, int njobs, double *__restrict__ ws, size_t ws_stride, double dt2, int *bad) {
     __shared__ double red[OT_THREADS + 1];
     __shared__ float redf[OT_THREADS + 1];
     const int t = threadIdx.x;
@@ -172,27 +162,8 @@ k_ot(const OtJob *__restrict__ jobs, int njobs, double *__restrict__ ws, size_t 
             const double Pi = P[i], Pe = P[i + 1], pp = Pe - Pi;
             double a = 0.0, b = 0.0;
             if (pp > 0.0) {
-                int k = ot_floor_index(Q, n, Pi);
-                double ya = Pi;
-                // k rises on every pass that does not end the cell (Pe <= Q[n]),
-                // so n passes bound the walk
-                for (int pass = 0; ya < Pe && pass <= n; ++pass) {
-                    const double Qk = Q[k], Qn = Q[k + 1];
-                    const double yb = fmin(Qn, Pe);
-                    if (yb > ya) {
-                        const double qq = Qn - Qk;
-                        const double xa = i + (ya - Pi) / pp, xb = i + (yb - Pi) / pp;
-                        const double ha = 2.0 * (xa - (k + (ya - Qk) / qq));
-                        const double hb = 2.0 * (xb - (k + (yb - Qk) / qq));
-                        const double dx = xb - xa, ua = xa - i, ub = xb - i;
-                        w += (yb - ya) * (ha * ha + ha * hb + hb * hb);
-                        a += dx * (ha + hb) / 2.0;
-                        b += dx * (ha * (2.0 * ua + ub) + hb * (ua + 2.0 * ub)) / 6.0;
-                    }
-                    ya = yb;
-                    if (ya >= Pe) break;
-                    while (k < n - 1 && Q[k + 1] <= ya) ++k;
-                }
+                // a walk that does not end the cell means levels above Q[n]: flag it
+                if (!ot_cell_walk(Q, n, i, Pi, Pe, w, a, b)) *bad = 1;
             } else {
                 // empty cell: T = G^-1 just above the level Pi
                 const int k = ot_floor_index(Q, n, Pi);

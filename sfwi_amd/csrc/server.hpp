@@ -81,7 +81,6 @@ struct sfwi_server {
     std::map<long long, Job> jobs;
     std::list<Cached> cache;
     std::vector<std::thread> workers;
-    std::string err;
 
     // Context of shape p for a job of `bytes` (lock held).  Reuses an idle
     // cached context, else evicts idle ones until the new one fits.
@@ -257,10 +256,7 @@ int sfwi_server_wait(sfwi_server *s, long long ticket, double *queue_ms, double 
     if (queue_ms) *queue_ms = ms_since(J.t_submit, J.t_start);
     if (run_ms) *run_ms = ms_since(J.t_start, J.t_end);
     const int r = J.status;
-    if (r) {
-        s->err = J.err;
-        fail(nullptr, r, "request %lld: %s", ticket, J.err.c_str());
-    }
+    if (r) fail(nullptr, r, "request %lld: %s", ticket, J.err.c_str());   // this thread's error slot
     s->jobs.erase(it);
     return r;
 }

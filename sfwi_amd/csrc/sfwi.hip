@@ -24,7 +24,9 @@ using namespace sfwi;
 
 namespace {
 
-std::string g_create_error;
+// Message of the last failed call without a context (create, server calls):
+// one per host thread, so concurrent callers never share or clobber it.
+thread_local std::string g_create_error;
 
 struct Slot {
     char *block = nullptr;      // fw(15) + bk(15) + derivatives(4) fields
@@ -81,6 +83,8 @@ struct sfwi_ctx {
     double *ot_ws = nullptr, *ot_mis = nullptr;
     int *ot_first = nullptr;
     float *ot_io = nullptr;
+    int *ot_bad = nullptr;        // set by k_ot when a merge walk stops at its bound
+    bool ot_armed = false;        // an OT launch since the flag was last read
     size_t ot_jobs_cap = 0, ot_ws_cap = 0, ot_mis_cap = 0, ot_first_cap = 0, ot_io_cap = 0;
     std::map<std::pair<int, int>, hipGraphExec_t> graphs;   // captured solves by (kind, batch size)
     // boundary strips in host memory (cpu_mem): the device keeps `ring` steps
@@ -422,8 +426,10 @@ int ot_launch(sfwi_ctx *c, size_t n_max, double dt2) {
     if ((r = grow(c, (void **)&c->ot_jobs, &c->ot_jobs_cap, nj * sizeof(OtJob)))) return r;
     if ((r = grow(c, (void **)&c->ot_ws, &c->ot_ws_cap, blocks * stride * sizeof(double)))) return r;
     HIPCK(c, hipMemcpyAsync(c->ot_jobs, c->ot_h.data(), nj * sizeof(OtJob), hipMemcpyHostToDevice, c->stream));
+    if (!c->ot_armed) HIPCK(c, hipMemsetAsync(c->ot_bad, 0, sizeof(int), c->stream));
+    c->ot_armed = true;
     hipLaunchKernelGGL(k_ot, dim3(blocks), dim3(OT_THREADS), 0, c->stream, c->ot_jobs, (int)nj, c->ot_ws, stride,
-                       dt2);
+                       dt2, c->ot_bad);
     HIPCK(c, hipGetLastError());
     return 0;
 }
@@ -725,10 +731,22 @@ int run(sfwi_ctx *c, Kind kind, int n) {
     return 0;
 }
 
+// After the stream has drained: an OT merge walk that stopped at its bound
+// (levels above the last cumulative level, which the clamping rules out)
+// makes the solve fail instead of returning a truncated misfit.
+int check_ot(sfwi_ctx *c) {
+    if (!c->ot_armed) return 0;
+    int bad = 0;
+    c->ot_armed = false;
+    HIPCK(c, hipMemcpy(&bad, c->ot_bad, sizeof(int), hipMemcpyDeviceToHost));
+    if (bad) return fail(c, SFWI_EHIP, "OT misfit: a merge walk did not end within its bound (invalid CDF levels)");
+    return 0;
+}
+
 int finish(sfwi_ctx *c) {
     HIPCK(c, hipStreamSynchronize(c->stream));
     tcollect(c);
-    return 0;
+    return check_ot(c);
 }
 
 // Batched solve with misfit output; shots processed in groups of max_batch.
@@ -852,7 +870,8 @@ int sfwi_ctx_create(const sfwi_params *p, sfwi_ctx **out) {
     if ((r = dalloc(c, (void **)&c->d_shots, p->max_batch * sizeof(DevShot)))) return bail(r);
     if ((r = dalloc(c, (void **)&c->d_out, (p->max_batch + 4) * sizeof(double)))) return bail(r);
     if ((r = dalloc(c, (void **)&c->d_num, 2 * (size_t)p->ntr * sizeof(double)))) return bail(r);
-    if ((r = dalloc(c, (void **)&c->d_vmax, 256))) return bail(r);
+    if ((r = dalloc(c, (void **)&c->d_vmax, 256))) return bail(r);   // vmax word + OT flag
+    c->ot_bad = (int *)((char *)c->d_vmax + 128);
     c->d_den = c->d_num + p->ntr;
     if (hipMemsetAsync(c->Gvp, 0, 4 * c->nmodel * sizeof(float), c->stream) != hipSuccess ||
         hipMemsetAsync(c->mvp, 0, 5 * c->nmodel * sizeof(float), c->stream) != hipSuccess ||
@@ -1385,6 +1404,7 @@ int sfwi_ot_misfit(sfwi_ctx *c, int variant, int ntr, int nt, float dt, const fl
     HIPCK(c, hipMemcpyAsync(res_x, rx, m * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIPCK(c, hipMemcpyAsync(res_z, rz, m * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIPCK(c, hipStreamSynchronize(c->stream));
+    if ((r = check_ot(c))) return r;
     double total = 0.0;
     for (double v : mis) total += v;
     *misfit += total;
