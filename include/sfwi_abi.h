@@ -52,6 +52,29 @@ extern "C" {
 #define SFWI_STORE_STRIPS_F32 0   /* FP32                                        */
 #define SFWI_STORE_STRIPS_F16 1   /* FP16 with per-step max-abs scale            */
 #define SFWI_STORE_STRIPS_BF16 2  /* BF16 with per-step max-abs scale            */
+/* The same codes select the storage of the velocity snapshots of
+ * SFWI_RECON_SNAPSHOTS (one scale per snapshot and field). */
+
+/* How a gradient solve obtains the forward wavefield for its imaging.
+ * STRIPS is the reference's scheme (IMAGE_SINGLESHOT, sfwi-stage1/
+ * FD2DGPU.cu:4321-4695): boundary strips saved in the forward loop, the
+ * forward field rebuilt by reverse-time stepping (source not subtracted),
+ * overwrite injection of the residual, imaging after both adjoint half steps.
+ * SNAPSHOTS has no reference counterpart: the forward velocity after the
+ * velocity update of every snap_every-th step is kept on the interior plus a
+ * 4-cell halo (FP32, or FP16 / BF16 with a max-abs scale per snapshot and
+ * field), and the reverse pass runs the adjoint only: the adjoint source
+ * (2 r for the L2 misfit, the OT adjoint source as it is) divided by rho is
+ * added at the receivers, then the adjoint velocity half step, on snapshot
+ * steps the zero-lag imaging from the kept velocity, then the adjoint stress
+ * half step, direction +1 (adjoint_dir is not used).  This is the consistent
+ * discrete gradient of the misfit with the imaging sampled every snap_every
+ * steps (weight snap_every, times dt); with snap_every = 1 and FP32 storage
+ * it is the exact gradient of the discrete scheme.  Stage 2: the gradient of
+ * the Born misfit with respect to the perturbation.  FAST mode only, no
+ * density image, no cpu_mem. */
+#define SFWI_RECON_STRIPS 0
+#define SFWI_RECON_SNAPSHOTS 1
 
 typedef struct sfwi_ctx sfwi_ctx;
 
@@ -104,6 +127,8 @@ typedef struct sfwi_params {
                            /* (cpu_mem then has nothing to hold)                      */
     unsigned int rbc_seed; /* seed of the random boundary's generator (the reference  */
                            /* seeds cuRAND from time(0); here a run repeats)          */
+    int recon;             /* SFWI_RECON_*                                          */
+    int snap_every;        /* SFWI_RECON_SNAPSHOTS: steps between snapshots (0 = 4) */
 } sfwi_params;
 
 /* One shot: the fields the reference driver writes into FD2DGPU_ELASTIC
@@ -397,7 +422,9 @@ int sfwi_s2_update(int allnx, int allnz, double fenzi, double fenmu, const float
 #define SFWI_K_REC_STR_IMG 6  /* reconstruction stress step + Vp / Vs imaging  */
 #define SFWI_K_REC_VEL_ILL 7  /* reconstruction velocity step + illumination   */
 #define SFWI_K_ADJ_VEL_ONLY 8 /* adjoint velocity step                         */
-#define SFWI_K_COUNT 9
+#define SFWI_K_SNAP_IMAGE 9   /* snapshot gradient: imaging of one snapshot    */
+#define SFWI_K_SNAP_STORE 10  /* snapshot gradient: max-abs scale + packing    */
+#define SFWI_K_COUNT 11
 int sfwi_set_timing(sfwi_ctx *ctx, int sample_every);
 int sfwi_kernel_time(sfwi_ctx *ctx, int kernel_id, double *avg_ms, int *launches);
 const char *sfwi_kernel_name(int kernel_id);

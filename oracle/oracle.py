@@ -36,6 +36,7 @@ class OrConfig(ctypes.Structure):
         ("nthreads", ctypes.c_int),
         ("rbc", ctypes.c_int),
         ("rbc_seed", ctypes.c_uint),
+        ("snap_every", ctypes.c_int),
     ]
 
 
@@ -67,6 +68,9 @@ def lib():
                                           _f32p, _f32p, _f32p]
         L.or_gradient_consistent_f64.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _i32p, _f32p, _f32p,
                                                  _f64p, _f64p, _f64p, dp]
+        _nf32 = ctypes.c_void_p   # nullable float arrays
+        L.or_gradient_snapshot_f64.argtypes = [cfg, _f32p, _f32p, _f32p, _nf32, _nf32, _f32p, _i32p, _f32p, _f32p,
+                                               _nf32, _nf32, _f64p, _f64p, _f64p, dp]
         L.or_born_forward_f32.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _i32p,
                                           _f32p, _f32p, _f32p, _f32p]
         L.or_born_forward_f64.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _i32p,
@@ -172,6 +176,22 @@ def gradient_consistent(c, vp, vs, rho, wav, gc, obs_z, obs_x):
     lib().or_gradient_consistent_f64(ctypes.byref(c), _f32(vp), _f32(vs), _f32(rho), _f32(wav),
                                      np.ascontiguousarray(gc, np.int32), _f32(obs_z), _f32(obs_x), *g,
                                      ctypes.byref(m))
+    sh = (c.allnx, c.allnz)
+    return g[0].reshape(sh), g[1].reshape(sh), g[2].reshape(sh), m.value
+
+
+def gradient_snapshot(c, vp, vs, rho, wav, gc, obs_z, obs_x, dvp=None, dvs=None, adj_z=None, adj_x=None):
+    """Snapshot gradient (FP64), imaging every c.snap_every-th step:
+    (grad_vp, grad_vs, illum, misfit).  dvp/dvs: stage-2 Born forward;
+    adj_z/adj_x: adjoint source of another misfit in place of 2 r."""
+    n = c.allnx * c.allnz
+    g = [np.zeros(n, np.float64) for _ in range(3)]
+    m = ctypes.c_double(0.0)
+    keep = [None if a is None else _f32(a) for a in (dvp, dvs, adj_z, adj_x)]
+    ptr = [None if a is None else a.ctypes.data for a in keep]
+    lib().or_gradient_snapshot_f64(ctypes.byref(c), _f32(vp), _f32(vs), _f32(rho), ptr[0], ptr[1], _f32(wav),
+                                   np.ascontiguousarray(gc, np.int32), _f32(obs_z), _f32(obs_x), ptr[2], ptr[3],
+                                   *g, ctypes.byref(m))
     sh = (c.allnx, c.allnz)
     return g[0].reshape(sh), g[1].reshape(sh), g[2].reshape(sh), m.value
 

@@ -819,11 +819,133 @@ static void gradient_consistent_impl(const or_config *c, const float *vp, const 
     }
 }
 
+// Snapshot gradient (no reference counterpart; the oracle of the library's
+// recon = snapshots mode).  The consistent gradient above with the imaging
+// decimated in time: the forward velocity after the velocity update of every
+// step it with it % K == 0 is kept on the imaged interior plus a 4-cell halo
+// (what its derivatives need), and the reverse pass images only at those
+// steps, each with weight K.  Per reverse step: add the adjoint source at the
+// receivers (2 r / rho for the L2 misfit, or a caller-supplied source adj / rho,
+// the derivative of another misfit with respect to the synthetic traces),
+// adjoint velocity half step, imaging from the kept velocity and the adjoint
+// stresses, adjoint stress half step; direction +1.  With K = 1 the Vp / Vs
+// images equal gradient_consistent_impl's (a CPU test checks it), which
+// finite differences of the misfit pin.  Born (dvp, dvs given): the forward
+// solve is the stage-2 one and the residual born - obs + background; the
+// background velocity is kept.  Images scaled by K dt; illumination
+// K sum (vx^2 + vz^2) of the kept velocity.
+template <typename T>
+static void gradient_snapshot_impl(const or_config *c, const float *vp, const float *vs, const float *rho,
+                                   const float *dvp, const float *dvs, const float *wav, const int *gc,
+                                   const float *obs_z, const float *obs_x, const float *adj_z,
+                                   const float *adj_x, T *gvp, T *gvs, T *gil, double *misfit) {
+    const bool born = dvp && dvs;
+    Run<T> r(c, vp, vs, rho, born ? dvp : nullptr, born ? dvs : nullptr, gc);
+    const Grid &g = r.g;
+    const long n = g.n, nr = (long)c->ntr * c->nt;
+    const int K = c->snap_every > 1 ? c->snap_every : 1;
+    const int nsnap = (c->nt + K - 1) / K;
+    // kept region: columns [pml-4, pml+nx+4), rows [pml-4, pml+nz+4)
+    const int x0 = g.pml - NOP, z0 = g.pml - NOP, RX = g.nx + 2 * NOP, RZ = g.nz + 2 * NOP;
+    const long RN = (long)RX * RZ;
+    std::vector<T> keep((size_t)nsnap * 2 * RN);
+    auto save = [&](const Wave<T> &w, int k) {
+        T *o = keep.data() + (size_t)k * 2 * RN;
+        for (int x = 0; x < RX; ++x)
+            for (int z = 0; z < RZ; ++z) {
+                const long i = r.at(x0 + x, z0 + z), q = (long)x * RZ + z;
+                o[q] = w.vx[i];
+                o[RN + q] = w.vz[i];
+            }
+    };
+    std::vector<T> sz(nr), sx(nr), bz, bx;
+    Wave<T> w(n), b(born ? n : 1);
+    if (born) { bz.resize(nr); bx.resize(nr); }
+    for (int it = 0; it < c->nt; ++it) {
+        r.add_source(w, wav, it);
+        update_v(g, r.st, r.m, w, +1, r.nth);
+        if (it % K == 0) save(w, it / K);
+        update_s(g, r.st, r.m, w, +1, r.nth);
+        if (born) {
+            update_v(g, r.st, r.m, b, +1, r.nth);
+            born_source(g, r.st, r.m, w, b, +1, r.nth);
+            update_s(g, r.st, r.m, b, +1, r.nth);
+            r.record(b, it, bz, bx);
+        }
+        r.record(w, it, sz, sx);
+    }
+    for (long q = 0; q < nr; ++q) {   // residual (stage 2: born - obs + background)
+        if (born) {
+            sz[q] = (bz[q] - (T)obs_z[q]) + sz[q];
+            sx[q] = (bx[q] - (T)obs_x[q]) + sx[q];
+        } else {
+            sz[q] = sz[q] - (T)obs_z[q];
+            sx[q] = sx[q] - (T)obs_x[q];
+        }
+    }
+    *misfit += l2_sum(sz, sx, c->ntr, c->nt);
+    Wave<T> bk(n);
+    std::vector<T> fvx(n, T(0)), fvz(n, T(0));
+    const size_t nm = (size_t)c->allnx * c->allnz;
+    std::vector<T> ivp(nm, T(0)), ivs(nm, T(0)), ill(nm, T(0));
+    const T dx = r.st.dx, dz = r.st.dz;
+    for (int it = c->nt - 1; it >= 0; --it) {
+        for (int tr = 0; tr < g.ntr; ++tr) {   // the split parts keep summing to the totals
+            const long i = r.at(r.igx[tr], c->igz), o = (long)tr * c->nt + it;
+            const T az = (adj_z ? (T)adj_z[o] : 2 * sz[o]) / r.m.rho[i];
+            const T ax = (adj_x ? (T)adj_x[o] : 2 * sx[o]) / r.m.rho[i];
+            bk.vz[i] += az;
+            bk.vz_x[i] += az;
+            bk.vx[i] += ax;
+            bk.vx_x[i] += ax;
+        }
+        adjoint_v(g, r.st, r.m, bk, +1, r.nth);
+        if (it % K == 0) {
+            const T *o = keep.data() + (size_t)(it / K) * 2 * RN;
+            for (int x = 0; x < RX; ++x)
+                for (int z = 0; z < RZ; ++z) {
+                    const long i = r.at(x0 + x, z0 + z), q = (long)x * RZ + z;
+                    fvx[i] = o[q];
+                    fvz[i] = o[RN + q];
+                }
+#pragma omp parallel for schedule(static) num_threads(r.nth)
+            for (int ix = 0; ix < g.nx; ++ix)
+                for (int iz = 0; iz < g.nz; ++iz) {
+                    const long p = r.at(ix + g.pml, iz + g.pml);
+                    const long q = (long)(c->left + ix) * g.nz + iz;
+                    const T gx = d_minus(fvx.data(), p, (long)g.nzp) / dx;   // the stress update's derivatives
+                    const T gz = d_minus(fvz.data(), p, 1L) / dz;
+                    const T gxz = d_plus(fvx.data(), p, 1L) / dz;
+                    const T gzx = d_plus(fvz.data(), p, (long)g.nzp) / dx;
+                    const T lg = -((gx + gz) * (bk.txx[p] + bk.tzz[p]));
+                    const T mg = -2 * (gx * bk.txx[p] + gz * bk.tzz[p]) - (gxz + gzx) * bk.txz[p];
+                    ivp[q] += 2 * r.m.vp[p] * r.m.rho[p] * lg;
+                    ivs[q] += -4 * r.m.vs[p] * r.m.rho[p] * lg + 2 * r.m.vs[p] * r.m.rho[p] * mg;
+                    ill[q] += fvx[p] * fvx[p] + fvz[p] * fvz[p];
+                }
+        }
+        adjoint_s(g, r.st, r.m, bk, +1, r.nth);
+    }
+    const T wk = (T)K * r.st.dt;
+    for (size_t q = 0; q < nm; ++q) {
+        gvp[q] += wk * ivp[q];
+        gvs[q] += wk * ivs[q];
+        gil[q] += (T)K * ill[q];
+    }
+}
+
 extern "C" {
 void or_gradient_consistent_f64(const or_config *c, const float *vp, const float *vs, const float *rho,
                                 const float *wav, const int *gc, const float *obs_z, const float *obs_x,
                                 double *gvp, double *gvs, double *grho, double *misfit) {
     gradient_consistent_impl<double>(c, vp, vs, rho, wav, gc, obs_z, obs_x, gvp, gvs, grho, misfit);
+}
+void or_gradient_snapshot_f64(const or_config *c, const float *vp, const float *vs, const float *rho,
+                              const float *dvp, const float *dvs, const float *wav, const int *gc,
+                              const float *obs_z, const float *obs_x, const float *adj_z, const float *adj_x,
+                              double *gvp, double *gvs, double *gil, double *misfit) {
+    gradient_snapshot_impl<double>(c, vp, vs, rho, dvp, dvs, wav, gc, obs_z, obs_x, adj_z, adj_x, gvp, gvs, gil,
+                                   misfit);
 }
 }  // extern "C"
 

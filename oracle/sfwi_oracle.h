@@ -63,6 +63,7 @@ typedef struct or_config {
                           /*    reversal alone); 2: the same medium with the strips kept  */
                           /*    (test hook)                                               */
     unsigned rbc_seed;    /* seed of the random boundary                                  */
+    int snap_every;       /* snapshot gradient: image every K-th step (<= 1: every step)  */
 } or_config;
 
 /* FD coefficients c1..c4 of the 8th-order staggered stencil (index 0 unused). */
@@ -116,6 +117,22 @@ void or_gradient_rho_f64(const or_config *c, const float *vp, const float *vs, c
 void or_gradient_consistent_f64(const or_config *c, const float *vp, const float *vs, const float *rho,
                                 const float *wav, const int *gc, const float *obs_z, const float *obs_x,
                                 double *gvp, double *gvs, double *grho, double *misfit);
+/* Snapshot gradient (FP64; no reference counterpart; the oracle of the
+ * library's snapshot reconstruction mode): the consistent gradient above with
+ * the imaging decimated to the steps it % K == 0 (K = snap_every), each
+ * weighted by K, from the forward velocity after that step's velocity update
+ * kept on the interior plus a 4-cell halo.  Reverse step: adjoint source at
+ * the receivers (2 r / rho, or adj_* / rho when adj_z, adj_x are given:
+ * the derivative of another misfit with respect to the synthetic traces),
+ * adjoint velocity half step, imaging, adjoint stress half step, direction
+ * +1.  dvp, dvs non-NULL: stage-2 Born forward, residual born - obs +
+ * background, background velocity kept.  Accumulates K dt times the Vp / Vs
+ * images into gvp, gvs, K sum (vx^2 + vz^2) into gil (allnx*allnz) and the
+ * L2 misfit.  With K = 1 the Vp / Vs images equal or_gradient_consistent_f64's. */
+void or_gradient_snapshot_f64(const or_config *c, const float *vp, const float *vs, const float *rho,
+                              const float *dvp, const float *dvs, const float *wav, const int *gc,
+                              const float *obs_z, const float *obs_x, const float *adj_z, const float *adj_x,
+                              double *gvp, double *gvs, double *gil, double *misfit);
 void or_gradient_res_f32(const or_config *c, const float *vp, const float *vs, const float *rho,
                          const float *wav, const int *gc, const float *res_z, const float *res_x,
                          float *grad_vp, float *grad_vs, float *illum);

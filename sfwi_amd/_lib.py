@@ -18,8 +18,9 @@ MODE_FAST, MODE_EXACT = 0, 1
 STORE_STRIPS_F32, STORE_STRIPS_F16, STORE_STRIPS_BF16 = 0, 1, 2
 MISFIT_L2, MISFIT_OT_TRACE, MISFIT_OT_POWER, MISFIT_OT_GATHER = 0, 1, 2, 3
 REQ_FORWARD, REQ_BORN_FORWARD, REQ_GRADIENT, REQ_BORN_GRADIENT = 0, 1, 2, 3
+RECON_STRIPS, RECON_SNAPSHOTS = 0, 1
 (K_FWD_VEL, K_FWD_STR, K_REC_STR, K_REC_VEL, K_ADJ_STR, K_ADJ_VEL, K_REC_STR_IMG, K_REC_VEL_ILL, K_ADJ_VEL_ONLY,
- K_COUNT) = range(10)
+ K_SNAP_IMAGE, K_SNAP_STORE, K_COUNT) = range(12)
 
 
 class SfwiParams(ctypes.Structure):
@@ -35,6 +36,7 @@ class SfwiParams(ctypes.Structure):
         ("march_order", ctypes.c_int), ("grad_rho", ctypes.c_int),
         ("cpu_mem", ctypes.c_int), ("strip_chunk", ctypes.c_int),
         ("rbc", ctypes.c_int), ("rbc_seed", ctypes.c_uint),
+        ("recon", ctypes.c_int), ("snap_every", ctypes.c_int),
     ]
 
 

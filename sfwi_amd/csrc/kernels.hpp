@@ -84,11 +84,15 @@ struct DevShot {
     fptr r2z, r2x;                         // background traces of a Born solve
     cfptr oz, ox;                          // observed traces
     gptr<const int> igx;                   // receiver columns [ntr]
-    gptr<const int> inj;                   // traces injected by the adjoint [ninj]
+    gptr<const int> inj;                   // traces grouped by receiver cell, trace order within a cell [ntr]
+    gptr<const int> inj_off;               // group k = inj[inj_off[k] .. inj_off[k+1]) [ninj + 1]
     fptr strip;                            // boundary strips, [nt][2 fields][SF] (FP32 storage)
     fptr stage;                            // one step's strips [2][SF] before packing (16-bit storage)
     gptr<unsigned short> strip_h;          // packed strips [nt][2][SF] (FP16 / BF16 storage)
     fptr strip_scale;                      // max-abs scale per step and field [nt][2]
+    fptr snap32;                           // velocity snapshots, FP32 [nsnap][2][snap_n]
+    gptr<unsigned short> snap16;           // velocity snapshots, FP16 / BF16 of v / scale [nsnap][2][snap_n]
+    gptr<unsigned> snap_max;               // max |v| per snapshot and field as FP32 bits [nsnap][2]
     fptr gvp, gvs, gil;                    // per-shot gradient accumulators [nx][nz]
     fptr grho;                             // per-shot density gradient [nx][nz] (grad_rho)
     fptr vold_x, vold_z;                   // v(it+1) before the reconstruction velocity step (grad_rho)
@@ -110,6 +114,8 @@ struct Geo {
     int sf;                 // strip floats per field and step
     int store;              // strip storage: 0 FP32, 1 FP16, 2 BF16 (per-step max-abs scale)
     int grad_rho;           // 1: also image the density
+    int snap_k;             // snapshot reconstruction: steps between snapshots (0: strips)
+    int snap_rz, snap_n;    // snapshot region height nz + 8 and cells (nx + 8)(nz + 8), origin (pml-4, pml-4)
     float inv_nzd;
     float dt, dx, dz, dtdx, dtdz, idx, idz;
 };
@@ -478,6 +484,18 @@ __device__ inline void image_fast(const DevShot &d, int i, int q) {
     d.gil[q] = __builtin_fmaf(vx, vx, __builtin_fmaf(vz, vz, d.gil[q]));
 }
 
+// Vp / Vs image cofactors of one cell (the arithmetic of image_fast; k_merge
+// applies the coefficients 2 vp rho, 2 vs rho): velocity derivatives
+// gx = Dx- vx / dx, gz = Dz- vz / dz, gxz = Dz+ vx / dz, gzx = Dx+ vz / dx,
+// adjoint stresses and the sums so far; returns the updated sums in op, os.
+__device__ inline void image_vpvs(float gx, float gz, float gxz, float gzx, float sxx, float szz, float sxz,
+                                  float ip, float is, float &op, float &os) {
+    const float lg = -((gx + gz) * (sxx + szz));
+    const float mg = __builtin_fmaf(-(gxz + gzx), sxz, -2.0f * __builtin_fmaf(gx, sxx, gz * szz));
+    op = ip + lg;
+    os = is + __builtin_fmaf(-2.0f, lg, mg);
+}
+
 // Density term of one interior cell (FAST; both FAST kernel families call
 // it with the same operands):  -(vx_adj dvx + vz_adj dvz) / dt
 // + (vp^2 - 2 vs^2) lg + vs^2 mg, dv = v(it+1) - v(it) of the reconstruction.
@@ -844,10 +862,31 @@ __global__ void k_inject(Geo g, const DevShot *__restrict__ shots, int it) {
     const DevShot &d = shots[blockIdx.y];
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= d.ninj) return;
-    const int tr = d.inj[k];
+    const int tr = d.inj[d.inj_off[k + 1] - 1];
     const int i = d.igx[tr] * g.nzp + d.igz;
     d.bk.vz[i] = d.rz[(size_t)tr * g.nt + it];
     d.bk.vx[i] = d.rx[(size_t)tr * g.nt + it];
+}
+
+// Adjoint source of the snapshot (consistent) gradient: fac r / rho added at
+// the receivers, to the totals and to the x parts of the split adjoint
+// velocity (the receiver row is split in the adjoint velocity step), so the
+// parts keep summing to the totals.  fac = 2 for the L2 misfit sum r^2 (r the
+// residual), 1 when r is already the derivative of the misfit (OT).  The
+// traces of one receiver cell are added in trace order by one thread.
+__global__ void k_inject_add(Geo g, const DevShot *__restrict__ shots, int it, float fac) {
+    const DevShot &d = shots[blockIdx.y];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d.ninj) return;
+    const int i = d.igx[d.inj[d.inj_off[k]]] * g.nzp + d.igz;
+    const float rho = d.m.rho[i];
+    float vz = d.bk.vz[i], vzx = d.bk.vz_x[i], vx = d.bk.vx[i], vxx = d.bk.vx_x[i];
+    for (int j = d.inj_off[k]; j < d.inj_off[k + 1]; ++j) {
+        const size_t o = (size_t)d.inj[j] * g.nt + it;
+        const float az = fac * d.rz[o] / rho, ax = fac * d.rx[o] / rho;
+        vz += az; vzx += az; vx += ax; vxx += ax;
+    }
+    d.bk.vz[i] = vz; d.bk.vz_x[i] = vzx; d.bk.vx[i] = vx; d.bk.vx_x[i] = vxx;
 }
 
 template <bool EXACT>
@@ -860,8 +899,10 @@ __global__ void __launch_bounds__(BLK) k_adj_str(Geo g, const DevShot *__restric
 }
 
 // Adjoint velocity step fused with the zero-lag imaging of this reverse step
-// (sfwi-stage1/FD2DGPU.cu:3350-3382 doimage_gradient, :3324-3331 doimage_illum).
-template <bool EXACT>
+// (sfwi-stage1/FD2DGPU.cu:3350-3382 doimage_gradient, :3324-3331 doimage_illum);
+// IMAGE = false: the adjoint velocity step alone (snapshot gradient, which
+// images in k_snap_image).
+template <bool EXACT, bool IMAGE = true>
 __global__ void __launch_bounds__(BLK) k_adj_vel_image(Geo g, const DevShot *__restrict__ shots, int dir) {
     int ix, iz;
     if (!cell_of(g, ix, iz)) return;
@@ -869,6 +910,7 @@ __global__ void __launch_bounds__(BLK) k_adj_vel_image(Geo g, const DevShot *__r
     const int i = ix * g.nzp + iz;
     const bool split = in_band(g, ix, iz) || iz == d.igz;
     adj_vel_cell<EXACT>(g, d.m, d.bk, i, split, dir);
+    if (!IMAGE) return;
     const int jx = ix - g.pml, jz = iz - g.pml;
     if (jx < 0 || jx >= g.nx || jz < 0 || jz >= g.nz) return;
     const int q = jx * g.nz + jz;
@@ -909,9 +951,12 @@ __global__ void __launch_bounds__(BLK) k_adj_vel_image(Geo g, const DevShot *__r
 // the shots merged one at a time, bitwise.  SCALE (FAST mode): the Vp and Vs
 // accumulators hold cofactors, multiplied here by the imaging coefficients
 // 2 vp rho and 2 vs rho of the shot's window (a rounded product, then the sum).
+// wv, wi: weights of the Vp / Vs and illumination sums (1 for the strip
+// gradient, which leaves every product unchanged; K dt and K for the
+// snapshot gradient, which images every K-th step and includes dt).
 template <bool SCALE>
 __global__ void k_merge(Geo g, const DevShot *__restrict__ shots, int nshots, int allnx, float *gvp, float *gvs,
-                        float *gil, float *grho) {
+                        float *gil, float *grho, float wv, float wi) {
     const int o = blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= allnx * g.nz) return;
     const int ox = o / g.nz, jz = o - ox * g.nz;
@@ -923,19 +968,123 @@ __global__ void k_merge(Geo g, const DevShot *__restrict__ shots, int nshots, in
         const int q = jx * g.nz + jz;
         if (SCALE) {
             const int i = (jx + g.pml) * g.nzp + jz + g.pml;
-            a += d.m.icp[i] * d.gvp[q];
-            b += d.m.ics[i] * d.gvs[q];
+            a += (d.m.icp[i] * wv) * d.gvp[q];
+            b += (d.m.ics[i] * wv) * d.gvs[q];
         } else {
             a += d.gvp[q];
             b += d.gvs[q];
         }
-        c += d.gil[q];
+        c += wi * d.gil[q];
         if (g.grad_rho) r += d.grho[q];
     }
     gvp[o] = a;
     gvs[o] = b;
     gil[o] = c;
     if (g.grad_rho) grho[o] = r;
+}
+
+// ----------------------------------------------------- velocity snapshots ---
+// The snapshot gradient keeps the forward velocity after the velocity update
+// of every step it with it % snap_k == 0 on the region the imaging's
+// derivatives read: the nx x nz interior plus a 4-cell halo, columns
+// [pml-4, pml+nx+4) x rows [pml-4, pml+nz+4), index x * snap_rz + z in region
+// coordinates.  FP32, or FP16 / BF16 of v / scale with one FP32 max-abs scale
+// per snapshot and field (rounded to nearest even).
+
+__device__ inline int snap_cell(const Geo &g, int t) {   // padded index of region cell t
+    const int x = t / g.snap_rz, z = t - x * g.snap_rz;
+    return (x + g.pml - NOP) * g.nzp + (z + g.pml - NOP);
+}
+
+// Max |v| of snapshot k (16-bit storage): a wave maximum, then one atomic per
+// wave and field on the FP32 bits (non-negative floats order as integers).
+__global__ void __launch_bounds__(BLK) k_snap_max(Geo g, const DevShot *__restrict__ shots, int k) {
+    const DevShot &d = shots[blockIdx.y];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    float mx = 0.0f, mz = 0.0f;
+    if (t < g.snap_n) {
+        const int i = snap_cell(g, t);
+        mx = fabsf(d.fw.vx[i]);
+        mz = fabsf(d.fw.vz[i]);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        mx = fmaxf(mx, __shfl_xor(mx, o));
+        mz = fmaxf(mz, __shfl_xor(mz, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        unsigned *m = (unsigned *)d.snap_max + 2 * (size_t)k;
+        atomicMax(m, __float_as_uint(mx));
+        atomicMax(m + 1, __float_as_uint(mz));
+    }
+}
+
+// Store snapshot k: FP32, or v / scale in 16 bits.
+__global__ void __launch_bounds__(BLK) k_snap_pack(Geo g, const DevShot *__restrict__ shots, int k) {
+    const DevShot &d = shots[blockIdx.y];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.snap_n) return;
+    const int i = snap_cell(g, t);
+    const size_t o = (size_t)k * 2 * g.snap_n + t;
+    if (g.store == 0) {
+        d.snap32[o] = d.fw.vx[i];
+        d.snap32[o + g.snap_n] = d.fw.vz[i];
+    } else {
+        const float sx = __uint_as_float(d.snap_max[2 * (size_t)k]), sz = __uint_as_float(d.snap_max[2 * (size_t)k + 1]);
+        const float ix = sx > 0.0f ? 1.0f / sx : 0.0f, iz = sz > 0.0f ? 1.0f / sz : 0.0f;
+        d.snap16[o] = pack16(g.store, d.fw.vx[i] * ix);
+        d.snap16[o + g.snap_n] = pack16(g.store, d.fw.vz[i] * iz);
+    }
+}
+
+// Value of snapshot field f (0 vx, 1 vz) at region cell t, unscaled.
+__device__ inline float snap_val(const Geo &g, const DevShot &d, size_t base, int t) {
+    return g.store == 0 ? d.snap32[base + t] : unpack16(g.store, d.snap16[base + t]);
+}
+
+// Zero-lag imaging of the snapshot gradient at snapshot k, one thread per
+// interior cell: the stress update's velocity derivatives of the kept
+// velocity (Dx- vx, Dz- vz, Dz+ vx, Dx+ vz), the adjoint stresses of the
+// current reverse step (between its adjoint velocity and stress half steps)
+// and the Vp / Vs cofactor sums (k_merge applies 2 vp rho, 2 vs rho and the
+// weight); illumination vx^2 + vz^2 of the kept velocity.
+__global__ void __launch_bounds__(BLK) k_snap_image(Geo g, const DevShot *__restrict__ shots, int k) {
+    const DevShot &d = shots[blockIdx.y];
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.nx * g.nz) return;
+    const int jx = q / g.nz, jz = q - jx * g.nz;
+    const int R = g.snap_rz;
+    const int t = (jx + NOP) * R + (jz + NOP);   // region cell of this interior cell
+    const size_t bx = (size_t)k * 2 * g.snap_n, bz = bx + g.snap_n;
+    float sx = 1.0f, sz = 1.0f;
+    if (g.store != 0) {
+        sx = __uint_as_float(d.snap_max[2 * (size_t)k]);
+        sz = __uint_as_float(d.snap_max[2 * (size_t)k + 1]);
+    }
+    // sum_k c_k (f[t+(k-1)s] - f[t-ks]) and sum_k c_k (f[t+ks] - f[t-(k-1)s])
+    auto dm = [&](size_t b, int s) {
+        float a = kC[1] * (snap_val(g, d, b, t) - snap_val(g, d, b, t - s));
+        a = __builtin_fmaf(kC[2], snap_val(g, d, b, t + s) - snap_val(g, d, b, t - 2 * s), a);
+        a = __builtin_fmaf(kC[3], snap_val(g, d, b, t + 2 * s) - snap_val(g, d, b, t - 3 * s), a);
+        a = __builtin_fmaf(kC[4], snap_val(g, d, b, t + 3 * s) - snap_val(g, d, b, t - 4 * s), a);
+        return a;
+    };
+    auto dp = [&](size_t b, int s) {
+        float a = kC[1] * (snap_val(g, d, b, t + s) - snap_val(g, d, b, t));
+        a = __builtin_fmaf(kC[2], snap_val(g, d, b, t + 2 * s) - snap_val(g, d, b, t - s), a);
+        a = __builtin_fmaf(kC[3], snap_val(g, d, b, t + 3 * s) - snap_val(g, d, b, t - 2 * s), a);
+        a = __builtin_fmaf(kC[4], snap_val(g, d, b, t + 4 * s) - snap_val(g, d, b, t - 3 * s), a);
+        return a;
+    };
+    const float gx = dm(bx, R) * (sx * g.idx), gz = dm(bz, 1) * (sz * g.idz);
+    const float gxz = dp(bx, 1) * (sx * g.idz), gzx = dp(bz, R) * (sz * g.idx);
+    const float vx = snap_val(g, d, bx, t) * sx, vz = snap_val(g, d, bz, t) * sz;
+    const int i = (jx + g.pml) * g.nzp + (jz + g.pml);
+    float op, os;
+    image_vpvs(gx, gz, gxz, gzx, d.bk.txx[i], d.bk.tzz[i], d.bk.txz[i], d.gvp[q], d.gvs[q], op, os);
+    d.gvp[q] = op;
+    d.gvs[q] = os;
+    d.gil[q] = __builtin_fmaf(vx, vx, __builtin_fmaf(vz, vz, d.gil[q]));
 }
 
 // ------------------------------------------------------------- set-up ---

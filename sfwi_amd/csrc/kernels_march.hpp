@@ -172,18 +172,6 @@ __device__ inline int img_index(const Geo &g, const Lane &l, int c) {
     return (l.act && jx >= 0 && jx < g.nx && jz >= 0 && jz < g.nz) ? jx * g.nz + jz : -1;
 }
 
-// Vp / Vs image cofactors of one cell (the arithmetic of image_fast; k_merge
-// applies the coefficients 2 vp rho, 2 vs rho): velocity derivatives
-// gx = Dx- vx / dx, gz = Dz- vz / dz, gxz = Dz+ vx / dz, gzx = Dx+ vz / dx,
-// adjoint stresses and the sums so far; returns the updated sums in op, os.
-__device__ inline void image_vpvs(float gx, float gz, float gxz, float gzx, float sxx, float szz, float sxz,
-                                  float ip, float is, float &op, float &os) {
-    const float lg = -((gx + gz) * (sxx + szz));
-    const float mg = __builtin_fmaf(-(gxz + gzx), sxz, -2.0f * __builtin_fmaf(gx, sxx, gz * szz));
-    op = ip + lg;
-    os = is + __builtin_fmaf(-2.0f, lg, mg);
-}
-
 // Operands of the zero-lag imaging of one cell (loaded a column ahead, only
 // when some lane of the wave images).  sxx and szz are the adjoint stresses
 // the velocity update loads anyway; the caller fills them from its ring.

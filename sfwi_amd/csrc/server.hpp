@@ -30,11 +30,17 @@ size_t solve_bytes(const sfwi_params &p, int kind, int nshots) {
     const bool grad = kind == SFWI_REQ_GRADIENT || kind == SFWI_REQ_BORN_GRADIENT;
     const size_t sf = 16 * (size_t)(p.nx + p.nz);
     size_t fixed = p.nt * f + 12 * nmodel * f + 2 * nrec * f + p.max_batch * 1024 + 2 * p.ntr * 8 + 4096;
-    size_t slot = (34 + 2 * (p.grad_rho != 0)) * ncell * f + 6 * nrec * f + 4 * nint * f + 16 * (size_t)p.ntr + 64;
+    size_t slot = (34 + 2 * (p.grad_rho != 0)) * ncell * f + 6 * nrec * f + 4 * nint * f + 20 * (size_t)p.ntr + 68;
     if (born) slot += 15 * ncell * f;
     const size_t chunk = p.strip_chunk > 0 ? (size_t)p.strip_chunk : 100;
     const size_t ring = p.cpu_mem && 2 * chunk < (size_t)p.nt ? 2 * chunk : (size_t)p.nt;   // strip steps in HBM
-    if (grad) slot += p.storage ? ring * 2 * sf * 2 + ring * 8 + 2 * sf * f + 256 : ring * 2 * sf * f;
+    if (grad && p.recon == SFWI_RECON_SNAPSHOTS) {   // velocity snapshots instead of strips
+        const size_t k = p.snap_every > 0 ? (size_t)p.snap_every : 4, nsnap = (p.nt + k - 1) / k;
+        const size_t rn = (size_t)(p.nx + 8) * (p.nz + 8);
+        slot += ((nsnap * 8 + 255) & ~(size_t)255) + nsnap * 2 * rn * (p.storage ? 2 : 4);
+    } else if (grad) {
+        slot += p.storage ? ring * 2 * sf * 2 + ring * 8 + 2 * sf * f + 256 : ring * 2 * sf * f;
+    }
     const size_t window = 12 * ncell * f;
     size_t ot = 0;
     if (p.misfit != SFWI_MISFIT_L2 && grad) {

@@ -34,15 +34,16 @@ struct Slot {
     char *rec = nullptr;        // rz rx r2z r2x oz ox, ints, accumulators, trace sums
     float *strip = nullptr;     // FP32 [ring][2][sf], or 16-bit: scales [ring][2], stage [2][sf], [ring][2][sf]
     char *hstrip = nullptr;     // host strips (cpu_mem, pinned): FP32 [nt][2][sf], or 16-bit [nt][2][sf] + scales [nt][2]
+    char *snap = nullptr;       // velocity snapshots: max bits [nsnap][2] (256-B aligned), then FP32 or 16-bit values
     Fields fw{}, bk{}, bn{};
     float *dg[4]{};
     float *rz, *rx, *r2z, *r2x, *oz, *ox;
-    int *igx, *inj;
+    int *igx, *inj, *inj_off;
     float *gvp, *gvs, *gil, *grho;
     float *vold[2]{};
     double *trace_sum;
     int obs_ntr = 0;                 // traces of the observed records resident in oz/ox (0: none)
-    std::vector<int> h_igx, h_inj;   // host copies kept alive for async uploads
+    std::vector<int> h_igx, h_inj, h_off;   // host copies kept alive for async uploads
 };
 
 struct Window {
@@ -224,7 +225,13 @@ int model_window(sfwi_ctx *c, int left, bool born, const Medium **out) {
 
 // -------------------------------------------------------------- slots ---
 
-int ensure_slots(sfwi_ctx *c, int n, bool born, bool strips) {
+// Bytes before the snapshot values of a slot: the max-abs words, 256-B aligned.
+size_t snap_head(const sfwi_ctx *c) {
+    const size_t nsnap = (c->p.nt + c->g.snap_k - 1) / c->g.snap_k;
+    return (nsnap * 2 * sizeof(unsigned) + 255) & ~(size_t)255;
+}
+
+int ensure_slots(sfwi_ctx *c, int n, bool born, bool strips, bool snaps) {
     const size_t cell = c->ncell * sizeof(float);
     while ((int)c->slots.size() < n) c->slots.emplace_back();
     for (int s = 0; s < n; ++s) {
@@ -245,7 +252,7 @@ int ensure_slots(sfwi_ctx *c, int n, bool born, bool strips) {
             if (c->g.grad_rho)
                 for (int k = 0; k < 2; ++k) S.vold[k] = f + (34 + k) * c->ncell;
             // records, receiver tables, accumulators, per-trace sums
-            const size_t rbytes = 6 * c->nrec * sizeof(float) + 2 * (size_t)c->p.ntr * sizeof(int) +
+            const size_t rbytes = 6 * c->nrec * sizeof(float) + (3 * (size_t)c->p.ntr + 1) * sizeof(int) +
                                   4 * c->nint * sizeof(float) + (size_t)c->p.ntr * sizeof(double) + 64;
             r = dalloc(c, (void **)&S.rec, rbytes);
             if (r) return r;
@@ -258,6 +265,7 @@ int ensure_slots(sfwi_ctx *c, int n, bool born, bool strips) {
             S.gvp = acc; S.gvs = acc + c->nint; S.gil = acc + 2 * c->nint; S.grho = acc + 3 * c->nint;
             S.igx = (int *)(acc + 4 * c->nint);
             S.inj = S.igx + c->p.ntr;
+            S.inj_off = S.inj + c->p.ntr;
         }
         if (born && !S.born) {
             int r = dalloc(c, (void **)&S.born, 15 * cell);
@@ -267,6 +275,12 @@ int ensure_slots(sfwi_ctx *c, int n, bool born, bool strips) {
                               &S.bn.vz_x, &S.bn.vz_z, &S.bn.txx_x, &S.bn.txx_z, &S.bn.tzz_x, &S.bn.tzz_z,
                               &S.bn.txz_x, &S.bn.txz_z};
             for (int k = 0; k < 15; ++k) *bn[k] = f + k * c->ncell;
+        }
+        if (snaps && !S.snap) {
+            const size_t nsnap = (c->p.nt + c->g.snap_k - 1) / c->g.snap_k;
+            const size_t vals = nsnap * 2 * (size_t)c->g.snap_n * (c->g.store ? 2 : 4);
+            int r = dalloc(c, (void **)&S.snap, snap_head(c) + vals);
+            if (r) return r;
         }
         if (strips && !S.strip) {
             const size_t nv = (size_t)c->ring * 2 * c->g.sf;
@@ -310,11 +324,20 @@ int receiver_tables(sfwi_ctx *c, const sfwi_shot &s, int k, Slot &S) {
             return fail(c, SFWI_EINVAL, "shot %d trace %d: receiver column %d outside the padded grid", k, tr, x);
         S.h_igx[tr] = x;
     }
-    std::vector<int> last(c->g.nxp, -1);
-    for (int tr = 0; tr < s.ntr; ++tr) last[S.h_igx[tr]] = tr;
-    S.h_inj.clear();
-    for (int tr = 0; tr < s.ntr; ++tr)
-        if (last[S.h_igx[tr]] == tr) S.h_inj.push_back(tr);
+    // traces grouped by receiver cell (cells in order of their first trace,
+    // traces in order within a cell): the overwrite injection takes the last
+    // of each group, the additive one sums the group in trace order
+    std::vector<int> group(c->g.nxp, -1), count;
+    for (int tr = 0; tr < s.ntr; ++tr) {
+        int &gi = group[S.h_igx[tr]];
+        if (gi < 0) { gi = (int)count.size(); count.push_back(0); }
+        ++count[gi];
+    }
+    S.h_off.assign(count.size() + 1, 0);
+    for (size_t k = 0; k < count.size(); ++k) S.h_off[k + 1] = S.h_off[k] + count[k];
+    std::vector<int> fill(S.h_off.begin(), S.h_off.end() - 1);
+    S.h_inj.assign(s.ntr, 0);
+    for (int tr = 0; tr < s.ntr; ++tr) S.h_inj[fill[group[S.h_igx[tr]]]++] = tr;
     return 0;
 }
 
@@ -328,7 +351,8 @@ int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium 
     if (!c->have_wav) return fail(c, SFWI_ESTATE, "wavelet not set (sfwi_set_wavelet)");
     if (born && !override_m && !c->have_pert) return fail(c, SFWI_ESTATE, "perturbation not set (sfwi_set_perturbation)");
     if (n < 1 || n > c->p.max_batch) return fail(c, SFWI_EINVAL, "batch of %d shots outside [1,%d]", n, c->p.max_batch);
-    int r = ensure_slots(c, n, born, grad && !c->p.rbc);
+    const bool snaps = grad && c->g.snap_k > 0;
+    int r = ensure_slots(c, n, born, grad && !c->p.rbc && !snaps, snaps);
     if (r) return r;
     c->h_shots.resize(n);
     for (int k = 0; k < n; ++k) {
@@ -351,7 +375,13 @@ int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium 
         d.fw = S.fw; d.bk = S.bk; d.bn = S.bn;
         d.dgx = S.dg[0]; d.dgz = S.dg[1]; d.dgxz = S.dg[2]; d.dgzx = S.dg[3];
         d.rz = S.rz; d.rx = S.rx; d.r2z = S.r2z; d.r2x = S.r2x; d.oz = S.oz; d.ox = S.ox;
-        d.igx = S.igx; d.inj = S.inj; d.strip = S.strip;
+        d.igx = S.igx; d.inj = S.inj; d.inj_off = S.inj_off; d.strip = S.strip;
+        if (snaps) {
+            const size_t head = snap_head(c);
+            d.snap_max = (unsigned *)S.snap;
+            d.snap32 = (float *)(S.snap + head);
+            d.snap16 = (unsigned short *)(S.snap + head);
+        }
         if (c->g.store && S.strip) {
             const size_t nv = (size_t)c->p.nt * 2 * c->g.sf;
             char *q = (char *)S.strip;
@@ -362,7 +392,7 @@ int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium 
         }
         d.gvp = S.gvp; d.gvs = S.gvs; d.gil = S.gil; d.grho = S.grho; d.trace_sum = S.trace_sum;
         d.vold_x = S.vold[0]; d.vold_z = S.vold[1];
-        d.isx = s.isx; d.isz = s.isz; d.igz = s.igz; d.ntr = s.ntr; d.ninj = (int)S.h_inj.size();
+        d.isx = s.isx; d.isz = s.isz; d.igz = s.igz; d.ntr = s.ntr; d.ninj = (int)S.h_off.size() - 1;
         d.left = s.left;
         c->h_shots[k] = d;
         const size_t cell = c->ncell * sizeof(float);
@@ -370,6 +400,9 @@ int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium 
         if (born) HIPCK(c, hipMemsetAsync(S.born, 0, 15 * cell, c->stream));
         HIPCK(c, hipMemcpyAsync(S.igx, S.h_igx.data(), s.ntr * sizeof(int), hipMemcpyHostToDevice, c->stream));
         HIPCK(c, hipMemcpyAsync(S.inj, S.h_inj.data(), S.h_inj.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        HIPCK(c, hipMemcpyAsync(S.inj_off, S.h_off.data(), S.h_off.size() * sizeof(int), hipMemcpyHostToDevice,
+                                c->stream));
+        if (snaps) HIPCK(c, hipMemsetAsync(S.snap, 0, snap_head(c), c->stream));   // max-abs words start at 0
         if (obs && s.obs_z) {          // NULL: the records resident in this slot are reused
             const size_t rb = (size_t)s.ntr * c->p.nt * sizeof(float);
             HIPCK(c, hipMemcpyAsync(S.oz, s.obs_z, rb, hipMemcpyHostToDevice, c->stream));
@@ -550,8 +583,10 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
     // while the chunk in between is computed.
     // With the random boundary (rbc) nothing is saved: the reverse pass
     // reconstructs by time reversal alone (sfwi-stage1/FD2DGPU.cu:4543, 4602).
-    const bool save = grad && !c->p.rbc;
+    const bool snap = grad && g.snap_k > 0;   // velocity snapshots instead of strips
+    const bool save = grad && !c->p.rbc && !snap;
     const bool hs = save && c->ring < nt;
+    const dim3 snapg((unsigned)((g.snap_n + BLK - 1) / BLK), (unsigned)n);
     const int ch = c->chunk, klast = (nt - 1) / ch;
     for (int it = 0; it < nt; ++it) {
         const int sp = hs ? it % c->ring : it;
@@ -577,6 +612,12 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
         }
         tend(c, t);
         if (save && g.store) hipLaunchKernelGGL(k_strip_pack, dim3(2, (unsigned)n), dim3(256), 0, st, g, D, sp);
+        if (snap && it % g.snap_k == 0) {   // keep the new velocity (snapshot it / snap_k)
+            t = tbeg(c, SFWI_K_SNAP_STORE, it);
+            if (g.store) hipLaunchKernelGGL(k_snap_max, snapg, blk, 0, st, g, D, it / g.snap_k);
+            hipLaunchKernelGGL(k_snap_pack, snapg, blk, 0, st, g, D, it / g.snap_k);
+            tend(c, t);
+        }
         t = tbeg(c, SFWI_K_FWD_STR, it);
         if (percell) {
             if (born) hipLaunchKernelGGL((k_str<E, false, true>), cells, blk, 0, st, g, D, 1);
@@ -613,6 +654,38 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
     if (!grad) return 0;
 
     const int adir = c->p.adjoint_dir;
+    const dim3 mg((unsigned)((c->nmodel + 255) / 256));
+    if (snap) {
+        // Snapshot (consistent) gradient: per reverse step the adjoint source
+        // fac r / rho is added at the receivers, then the adjoint velocity half
+        // step, on snapshot steps the imaging from the kept velocity and the
+        // adjoint stresses, then the adjoint stress half step, direction +1;
+        // no reconstruction.  Images weighted by snap_k dt, illumination by
+        // snap_k, at the merge.
+        const float fac = c->p.misfit == SFWI_MISFIT_L2 ? 2.0f : 1.0f;
+        const dim3 ig((unsigned)((c->nint + BLK - 1) / BLK), (unsigned)n);
+        for (int it = nt - 1; it >= 0; --it) {
+            hipLaunchKernelGGL(k_inject_add, trg, dim3(64), 0, st, g, D, it, fac);
+            int t = tbeg(c, SFWI_K_ADJ_VEL_ONLY, it);
+            if (percell) hipLaunchKernelGGL((k_adj_vel_image<false, false>), cells, blk, 0, st, g, D, 1);
+            else hipLaunchKernelGGL((km_adj_vel_image<PD, false, false, 1>), tiles, blk, 0, st, gm, D, 1);
+            tend(c, t);
+            if (it % g.snap_k == 0) {
+                t = tbeg(c, SFWI_K_SNAP_IMAGE, it);
+                hipLaunchKernelGGL(k_snap_image, ig, blk, 0, st, g, D, it / g.snap_k);
+                tend(c, t);
+            }
+            t = tbeg(c, SFWI_K_ADJ_STR, it);
+            if (percell) hipLaunchKernelGGL(k_adj_str<false>, cells, blk, 0, st, g, D, 1);
+            else hipLaunchKernelGGL(km_adj_str<PD>, tiles, blk, 0, st, gm, D, 1);
+            tend(c, t);
+        }
+        const float K = (float)g.snap_k;
+        hipLaunchKernelGGL(k_merge<true>, mg, blk, 0, st, g, D, n, c->p.allnx, c->Gvp, c->Gvs, c->Gil, c->Grho,
+                           K * g.dt, K);
+        HIPCK(c, hipGetLastError());
+        return 0;
+    }
     const dim3 stripg((unsigned)((g.sf + 255) / 256), (unsigned)n);
     auto rev_begin = [&](int it) -> int {   // the step's strips restored (after their chunk has arrived)
         if (hs && it % ch == ch - 1 && it / ch <= klast - 2)
@@ -679,8 +752,8 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
         tend(c, t);
         if (int r = rev_end(it)) return r;
     }
-    const dim3 mg((unsigned)((c->nmodel + 255) / 256));
-    hipLaunchKernelGGL(k_merge<!E>, mg, blk, 0, st, g, D, n, c->p.allnx, c->Gvp, c->Gvs, c->Gil, c->Grho);
+    hipLaunchKernelGGL(k_merge<!E>, mg, blk, 0, st, g, D, n, c->p.allnx, c->Gvp, c->Gvs, c->Gil, c->Grho, 1.0f,
+                       1.0f);
     HIPCK(c, hipGetLastError());
     return 0;
 }
@@ -816,6 +889,15 @@ int sfwi_ctx_create(const sfwi_params *p, sfwi_ctx **out) {
     if (p->rbc != 0 && p->rbc != 1) return fail(nullptr, SFWI_EINVAL, "rbc must be 0 or 1");
     if (p->rbc && p->pml < 1) return fail(nullptr, SFWI_EINVAL, "rbc needs pml >= 1 (the random bands)");
     if (p->strip_chunk < 0) return fail(nullptr, SFWI_EINVAL, "strip_chunk must be >= 0");
+    if (p->recon != SFWI_RECON_STRIPS && p->recon != SFWI_RECON_SNAPSHOTS)
+        return fail(nullptr, SFWI_EINVAL, "unknown recon %d", p->recon);
+    if (p->snap_every < 0) return fail(nullptr, SFWI_EINVAL, "snap_every must be >= 0");
+    if (p->recon == SFWI_RECON_SNAPSHOTS) {
+        if (p->mode != SFWI_MODE_FAST)
+            return fail(nullptr, SFWI_EINVAL, "recon=snapshots runs in FAST mode only (EXACT restates the strips)");
+        if (p->grad_rho) return fail(nullptr, SFWI_EINVAL, "recon=snapshots does not image the density");
+        if (p->cpu_mem) return fail(nullptr, SFWI_EINVAL, "recon=snapshots keeps its snapshots in HBM (no cpu_mem)");
+    }
     const long nxp = p->nx + 2L * p->pml, nzp = p->nz + 2L * p->pml;
     if (nxp * nzp >= (1L << 31) / 16) return fail(nullptr, SFWI_EINVAL, "grid too large for 32-bit cell indices");
     int dev = 0;
@@ -835,6 +917,11 @@ int sfwi_ctx_create(const sfwi_params *p, sfwi_ctx **out) {
     g.sf = 4 * NSTRIP * (p->nx + p->nz);
     g.store = p->storage;
     g.grad_rho = p->grad_rho ? 1 : 0;
+    if (p->recon == SFWI_RECON_SNAPSHOTS) {
+        g.snap_k = p->snap_every > 0 ? p->snap_every : 4;
+        g.snap_rz = p->nz + 2 * NOP;
+        g.snap_n = (p->nx + 2 * NOP) * g.snap_rz;
+    }
     g.inv_nzd = 1.0f / (float)g.nzd;
     g.dt = p->dt; g.dx = p->dx; g.dz = p->dz;
     g.dtdx = p->dt / p->dx; g.dtdz = p->dt / p->dz;
@@ -1350,7 +1437,8 @@ int sfwi_kernel_time(sfwi_ctx *c, int kid, double *avg_ms, int *launches) {
 const char *sfwi_kernel_name(int kid) {
     static const char *names[SFWI_K_COUNT] = {
         "k_vel(forward)", "k_str(forward)", "k_str(reconstruct)", "k_vel(reconstruct)", "k_adj_str",
-        "k_adj_vel_image", "k_str(reconstruct+image)", "k_vel(reconstruct+illum)", "k_adj_vel"};
+        "k_adj_vel_image", "k_str(reconstruct+image)", "k_vel(reconstruct+illum)", "k_adj_vel",
+        "k_snap_image", "k_snap_store"};
     return (kid >= 0 && kid < SFWI_K_COUNT) ? names[kid] : "?";
 }
 

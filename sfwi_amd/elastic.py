@@ -42,7 +42,7 @@ class Shot:
 def make_params(*, dx, dz, dt, nx, nz, pml, nt, ntr, allnx, allnz, rr=0.001, rho_const=0.0, adjoint_dir=1,
                 max_batch=1, mode="fast", storage="f32", use_graph=False, march_cols=0, march_depth=0,
                 misfit="l2", march_order=0, grad_rho=False, cpu_mem=False, strip_chunk=0, rbc=False,
-                rbc_seed=0):
+                rbc_seed=0, recon="strips", snap_every=0):
     """``sfwi_params`` of a context or request (keyword names as in the C struct)."""
     p = SfwiParams()
     p.dx, p.dz, p.dt = dx, dz, dt
@@ -62,6 +62,8 @@ def make_params(*, dx, dz, dt, nx, nz, pml, nt, ntr, allnx, allnz, rr=0.001, rho
     p.strip_chunk = int(strip_chunk)
     p.rbc = 1 if rbc else 0
     p.rbc_seed = int(rbc_seed) & 0xFFFFFFFF
+    p.recon = {"strips": _lib.RECON_STRIPS, "snapshots": _lib.RECON_SNAPSHOTS}[recon]
+    p.snap_every = int(snap_every)
     return p
 
 
