@@ -1,26 +1,28 @@
 #!/usr/bin/env python3
 """Benchmark of the per-shot elastic FWI gradient path on one MI355X.
 
-Workload (default): the shot-batch survey -- 401 x 301 grid, dx = dz = 10 m,
-seeded 6-layer model with a 15-degree dipping interface, dt = 1 ms,
-nt = 3000, 30-cell PML, 64 shots with a 12 Hz Ricker wavelet and 401
-two-component receivers each.  Synthetic data: observed traces come from a
-forward solve through the true model (untimed); gradients are taken at the
-smoothed starting model.
+Workload (default): the mixed-precision survey -- 1200 x 600 grid,
+dx = dz = 5 m, a seeded random 8-layer model with three Gaussian inclusions of
++-8 %, dt = 0.5 ms, nt = 6000, 40-cell PML, 96 shots with a 15 Hz Ricker
+wavelet and 1200 two-component receivers each.  Gradients are taken with the
+snapshot reconstruction: the forward velocity is kept every 4 steps in BF16
+with one max-abs scale per snapshot and field, and the reverse pass runs the
+adjoint and images from the snapshots (FP32 stencils and accumulation).  The
+snapshots of all 96 shots (4.4 GB each) exceed one GPU's 288 GB, so the survey
+runs in waves of 24 shots: one step = one batched gradient of one wave, the
+waves rotating through the survey.  Synthetic data: observed traces come from
+a forward solve through the true model (untimed) and stay resident in HBM
+(sfwi_obs_put); gradients are taken at the smoothed starting model.
 
-One step = one batched gradient of all shots (forward with strip storage,
-misfit, reverse reconstruction + adjoint + zero-lag imaging), inputs resident
-in HBM (the observed traces stay resident from the warm-up call).  value =
-shot-gradients per second over the timed steps; pcie_inclusive_value is one
-more step that uploads the observed traces from host memory.
-
-Also reported: Gcell-updates/s (3 propagations per gradient), p50/p99 of the
-step latency, the gradient error of two shots against the FP64 oracle, a
-roofline entry for the dominant kernel (HIP-event timings taken inside the
-timed steps on the library's stream), the plain one-shot path on the same GPU
-(one shot per call, per-cell kernels; FP32 and reference operation order) and
-a CPU baseline (the FP64 oracle on the host cores, bounded 2-shot sample of
-the same workload).
+value = shot-gradients per second over the timed steps.  Also reported:
+Gcell-updates/s (the propagations actually run: 2 per snapshot gradient,
+3 per strip gradient), p50/p99 of the step latency, the trace, misfit and
+gradient error of one shot against the FP64 oracle of the same algorithm, a
+roofline entry for the dominant kernel (HIP-event timings inside the timed
+steps on the library's stream; bytes per the scope table's unit costs) and
+for the whole time step, the plain one-shot path on the same GPU (one shot
+per call, per-cell kernels, FP32 boundary strips: the reference's scheme),
+and a CPU baseline (the same FP64 oracle run, timed on the host cores).
 """
 import argparse
 import json
@@ -37,19 +39,12 @@ if ROOT not in sys.path:
 METRIC = "shot-gradients/s & Gcell-updates/s at ≤1e-3 grad err vs FP64; p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes per launch of the default (column-march, 8 columns, depth 4, whole shots per XCD) FAST
-# kernels on this workload's 461 x 361 padded grid, per shot in the launch:
-# FETCH_SIZE x 2 + WRITE_SIZE x 1 from rocprofv3 --pmc passes over a 64-shot
-# batch, the factors from a plain 1 GiB copy (profiles/shot_batch_march_hbm_traffic.md).
-MEASURED_HBM_BYTES_PER_SHOT = {
-    "k_vel(forward)": 486.7e6 / 64,
-    "k_str(forward)": 636.6e6 / 64,
-    "k_adj_str": 468.4e6 / 64,
-    "k_str(reconstruct+image)": 866.2e6 / 64,
-    "k_vel(reconstruct+illum)": 543.3e6 / 64,
-    "k_adj_vel": 627.9e6 / 64,
-}
-
+# HBM bytes per launch (FETCH_SIZE x 2 + WRITE_SIZE x 1 from two rocprofv3
+# --pmc passes, the factors from a plain copy of known size) of the default
+# benchmark configuration's kernels, per shot of the launch; see
+# profiles/mixed_precision_snapshot_hbm_traffic.md.  Keyed by
+# (workload, recon, storage, snap_every).
+MEASURED_HBM_BYTES_PER_SHOT = {}
 
 NOP, NSTRIP = 4, 4   # stencil half-width; strip half-width (split band = PML + NSTRIP)
 
@@ -67,29 +62,25 @@ def cell_classes(nx, nz, pml):
 
 
 def alg_bytes(kernel, nx, nz, pml):
-    """Algorithmic bytes of one launch over one shot: every array the kernel
-    needs read once and every array it produces written once, FP32.  An
-    unsplit cell reads its own totals; a split cell reads and writes its split
-    parts and the damping, writes the totals, and does not read its own
-    totals; stencil inputs are totals everywhere; imaging terms count on the
-    imaged cells."""
+    """Bytes of one launch over one shot by the kernels' own cell classes:
+    every array the kernel needs read once and every array it produces written
+    once, FP32.  An unsplit cell reads its own totals; a split cell reads and
+    writes its split parts and the damping, writes the totals, and does not
+    read its own totals; stencil inputs are totals everywhere; imaging terms
+    count on the imaged cells."""
     per = {  # (unsplit B/cell, split B/cell, imaged B/cell)
-        # R txx tzz txz, 1/rho, vx vz | split: R 4 parts + 2 damping, W 4 parts; W vx vz
         "k_vel(forward)": (32, 64, 0),
-        # R vx vz, lam mu, txx tzz txz | split: R 6 parts + 2 damping, W 6 parts; W 3 stresses
         "k_str(forward)": (40, 84, 0),
-        "k_str(reconstruct)": (56, 100, 0),      # as forward + W 4 velocity derivatives
+        "k_str(reconstruct)": (56, 100, 0),
         "k_vel(reconstruct)": (32, 64, 0),
-        # R vx vz, txx tzz txz | split: R 4 parts + 2 damping, W 4 parts; W 3 stresses
         "k_adj_str": (32, 60, 0),
-        # R txx tzz txz, lam mu 1/rho, vx vz | split: R 4 parts + 2 damping, W 4 parts; W vx vz;
-        # imaging: R 4 derivatives, reconstructed vx vz, RW 3 images
         "k_adj_vel_image": (40, 72, 48),
-        # column-march reverse step without the density image
-        "k_str(reconstruct+image)": (40, 84, 28),   # imaging: R 3 adjoint stresses, RW 2 images
-        "k_vel(reconstruct+illum)": (32, 64, 8),    # imaging: RW illumination
+        "k_str(reconstruct+image)": (40, 84, 28),
+        "k_vel(reconstruct+illum)": (32, 64, 8),
         "k_adj_vel": (40, 72, 0),
-    }[kernel]
+    }.get(kernel)
+    if per is None:
+        return None
     un, sp, im = cell_classes(nx, nz, pml)
     b = per[0] * un + per[1] * sp + per[2] * im
     if kernel == "k_vel(forward)":
@@ -97,18 +88,76 @@ def alg_bytes(kernel, nx, nz, pml):
     return b
 
 
+# Unit costs of the hot-path scope table (FP32, one read + one write of the
+# minimal state, no temporal reuse): a full cell-update is 52 B on an interior
+# cell (5 fields read and written, 3 medium parameters) and 100 B on a PML cell
+# (10 split components, 3 parameters, 2 damping profiles); imaging adds 24 B per
+# interior cell per gradient step.  Per half-step kernel the update splits as
+# velocity 20 / 44 B (vx vz read+write, 1/rho; PML: 4 split parts, rho, 2
+# damping) and stress 32 / 64 B (3 stresses read+write, lambda mu; PML: 6 split
+# parts, 2 parameters, 2 damping); interior = nx*nz cells, PML = the rest of
+# the padded grid.
+UNIT_8D = {"vel": (20, 44), "str": (32, 64)}
+
+
+def alg_bytes_8d(kernel, nx, nz, pml, store_bytes=2):
+    """Scope-table bytes of one launch of `kernel` over one shot."""
+    inter = nx * nz
+    pmlc = (nx + 2 * pml) * (nz + 2 * pml) - inter
+    half = {"k_vel(forward)": "vel", "k_adj_vel": "vel", "k_vel(reconstruct+illum)": "vel",
+            "k_vel(reconstruct)": "vel", "k_adj_vel_image": "vel",
+            "k_str(forward)": "str", "k_adj_str": "str", "k_str(reconstruct+image)": "str",
+            "k_str(reconstruct)": "str"}.get(kernel)
+    if half:
+        a, b = UNIT_8D[half]
+        extra = {"k_str(reconstruct+image)": 16, "k_vel(reconstruct+illum)": 8, "k_adj_vel_image": 24}.get(kernel, 0)
+        return a * inter + b * pmlc + extra * inter
+    rn = (nx + 2 * NOP) * (nz + 2 * NOP)
+    if kernel == "k_snap_image":       # R the snapshot (2 fields) and 3 adjoint stresses, RW 3 accumulators
+        return inter * (2 * store_bytes + 12 + 24)
+    if kernel == "k_snap_store":       # R vx vz, W the snapshot of the region
+        return rn * (8 + 2 * store_bytes)
+    return None
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _survey(args, W):
+    kw = {}
+    if args.shots:
+        kw["nshots"] = args.shots
+    if args.nt:
+        kw["nt"] = args.nt
+    return W.BY_NAME[args.workload](**kw)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="shot_batch", choices=["shot_batch", "mixed_precision", "plain_small"],
-                    help="shot_batch: 401x301, nt 3000, 64 shots (default); mixed_precision: 1200x600, "
-                         "dx 5 m, nt 6000, 96 shots; plain_small: 376x201, nt 2000, 16 shots")
+    ap.add_argument("--workload", default="mixed_precision", choices=["mixed_precision", "shot_batch", "plain_small"],
+                    help="mixed_precision: 1200x600, dx 5 m, nt 6000, 96 shots (default); shot_batch: 401x301, "
+                         "nt 3000, 64 shots; plain_small: 376x201, nt 2000, 16 shots")
     ap.add_argument("--shots", type=int, default=0, help="shots (0: the workload's)")
+    ap.add_argument("--wave", type=int, default=0,
+                    help="shots per step (0: 24 for mixed_precision, whose snapshots of 96 shots exceed HBM; "
+                         "all shots otherwise)")
     ap.add_argument("--nt", type=int, default=0, help="time samples (0: the workload's)")
-    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"],
-                    help="boundary-strip storage of the reconstruction")
+    ap.add_argument("--recon", default=None, choices=["strips", "snapshots"],
+                    help="forward-field reconstruction (default: snapshots for mixed_precision, strips otherwise)")
+    ap.add_argument("--snap-every", type=int, default=4, help="snapshots: steps between snapshots")
+    ap.add_argument("--storage", default=None, choices=["f32", "f16", "bf16"],
+                    help="strip or snapshot storage (default: bf16 for mixed_precision, f32 otherwise)")
     ap.add_argument("--misfit", default="l2", choices=["l2", "ot_trace", "ot_power", "ot_gather"],
                     help="misfit of the gradient (l2: the reference's; ot_*: optimal-transport variants)")
     ap.add_argument("--cpu-mem", type=int, default=0, metavar="CHUNK",
@@ -134,12 +183,14 @@ def main():
     ap.add_argument("--stage", type=int, default=1, choices=(1, 2), help="inversion: which stage's program")
     ap.add_argument("--born", action="store_true",
                     help="time the stage-2 (Born) gradient of the workload's shot batch instead")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--requests", type=int, default=24)
     ap.add_argument("--rate", type=float, default=1.0, help="serving: requests per second")
     ap.add_argument("--scale", type=float, default=1.0, help="serving: request size factor")
     ap.add_argument("--workers", type=int, default=4, help="serving: concurrent solves")
-    ap.add_argument("--cpu-shots", type=int, default=2)
-    ap.add_argument("--plain-shots", type=int, default=4,
+    ap.add_argument("--cpu-shots", type=int, default=1)
+    ap.add_argument("--plain-shots", type=int, default=3,
                     help="shots timed through the plain one-shot path (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -155,6 +206,8 @@ def main():
     from sfwi_amd import _lib
     from sfwi_amd.elastic import Context, Shot
 
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
     if args.serving:
         return serving(args)
     if args.uneven:
@@ -164,147 +217,197 @@ def main():
     if args.born:
         return born(args)
 
-    kw = {}
-    if args.shots:
-        kw["nshots"] = args.shots
-    if args.nt:
-        kw["nt"] = args.nt
-    sv = W.BY_NAME[args.workload](**kw)
+    mp = args.workload == "mixed_precision"
+    recon = args.recon or ("snapshots" if mp else "strips")
+    storage = args.storage or ("bf16" if mp else "f32")
+    snap = recon == "snapshots"
+    sv = _survey(args, W)
+    nsh = len(sv.shots)
+    wave = min(args.wave or (24 if mp else nsh), nsh)
+    waves = [list(range(i, min(i + wave, nsh))) for i in range(0, nsh, wave)]
     w = W.ricker(sv.nt, sv.dt, sv.f0)
     geo = [W.shot_geometry(sv, s) for s in sv.shots]
     n_cells = sv.cells()
-    print("bench: %s, %d shots, grid %dx%d pml %d (%d cells), nt %d" % (
-        sv.name, len(sv.shots), sv.nx, sv.allnz, sv.pml, n_cells, sv.nt), file=sys.stderr, flush=True)
+    print("bench: %s, %d shots in waves of %d, grid %dx%d pml %d (%d cells), nt %d, %s %s" % (
+        sv.name, nsh, wave, sv.nx, sv.allnz, sv.pml, n_cells, sv.nt, recon, storage), file=sys.stderr, flush=True)
 
     ctx = Context(dx=sv.dx, dz=sv.dz, dt=sv.dt, nx=sv.nx, nz=sv.allnz, pml=sv.pml, nt=sv.nt, ntr=sv.ntr,
-                  allnx=sv.allnx, allnz=sv.allnz, max_batch=len(sv.shots), mode=args.mode, storage=args.storage,
+                  allnx=sv.allnx, allnz=sv.allnz, max_batch=wave, mode=args.mode, storage=storage,
                   march_cols=args.march_cols, march_depth=args.march_depth, march_order=args.march_order,
                   cpu_mem=args.cpu_mem > 0, strip_chunk=max(args.cpu_mem, 0), misfit=args.misfit,
-                  rbc=args.rbc, rbc_seed=1)
+                  rbc=args.rbc, rbc_seed=1, recon=recon, snap_every=args.snap_every)
     ctx.set_wavelet(w)
     ctx.set_model(sv.vp, sv.vs, sv.rho)
     t0 = time.perf_counter()
     obs_z, obs_x = ctx.batch_forward([Shot(s.left, *g, s.ngx_left, s.gc) for s, g in zip(sv.shots, geo)])
+    for k in range(nsh):   # the survey's observed traces stay resident in HBM, by shot id
+        ctx.obs_put(k + 1, obs_z[k][:len(sv.shots[k].gc)], obs_x[k][:len(sv.shots[k].gc)])
     print("bench: observed data %.2f s" % (time.perf_counter() - t0), file=sys.stderr, flush=True)
     ctx.set_model(sv.vp0, sv.vs0, sv.rho0)
-    shots = [Shot(s.left, *g, s.ngx_left, s.gc, obs_z[k], obs_x[k])
-             for k, (s, g) in enumerate(zip(sv.shots, geo))]
+    shots = [Shot(s.left, *g, s.ngx_left, s.gc, obs_id=k + 1) for k, (s, g) in enumerate(zip(sv.shots, geo))]
+
+    def step(j):
+        return ctx.batch_gradient([shots[k] for k in waves[j % len(waves)]])
 
     for k in range(args.warmup):
         t0 = time.perf_counter()
-        ctx.batch_gradient(shots)
+        step(k)
         ctx.flush()
         print("bench: warmup %d %.3f s" % (k, time.perf_counter() - t0), file=sys.stderr, flush=True)
-
-    # timed steps: observed traces already resident in HBM (uploaded by the
-    # warm-up call); the shots pass NULL records and reuse their batch slot's
-    resident = [Shot(s.left, *g, s.ngx_left, s.gc) for s, g in zip(sv.shots, geo)]
-    if args.warmup == 0:
-        ctx.batch_gradient(shots)
-        ctx.flush()
     ctx.set_timing(args.timing_every)
     times = []
     misfit = 0.0
     for k in range(args.steps):
         t0 = time.perf_counter()
-        m = ctx.batch_gradient(resident)       # returns after the stream is drained
+        m = step(args.warmup + k)             # returns after the stream is drained
         times.append(time.perf_counter() - t0)
         misfit = float(m.sum())
         print("bench: step %d %.3f s" % (k, times[-1]), file=sys.stderr, flush=True)
     kstats = {kid: ctx.kernel_time(kid) for kid in range(_lib.K_COUNT)}
     ctx.set_timing(0)
     ctx.flush()
-    # one step that uploads the observed traces from host memory (PCIe-inclusive rate)
+    # one step that uploads its observed traces from host memory (PCIe-inclusive rate)
+    host_shots = [Shot(sv.shots[k].left, *geo[k], sv.shots[k].ngx_left, sv.shots[k].gc, obs_z[k], obs_x[k])
+                  for k in waves[0]]
     t0 = time.perf_counter()
-    ctx.batch_gradient(shots)
+    ctx.batch_gradient(host_shots)
     t_pcie = time.perf_counter() - t0
     ctx.flush()
 
     total = sum(times)
-    nsh = len(shots)
-    value = nsh * args.steps / total
-    gcells = nsh * n_cells * sv.nt * 3 * args.steps / total / 1e9
+    shot_steps = sum(len(waves[(args.warmup + k) % len(waves)]) for k in range(args.steps))
+    value = shot_steps / total
+    props = 2 if snap else 3
+    gcells = shot_steps * n_cells * sv.nt * props / total / 1e9
 
     # per-kernel roofline from the HIP events recorded inside the timed steps
+    sb = 4 if storage == "f32" else 2
+    avg_wave = shot_steps / args.steps
     kernels = {}
     for kid in range(_lib.K_COUNT):
         ms, n = kstats[kid]
         if n == 0:
             continue
         name = _lib.load().sfwi_kernel_name(kid).decode()
-        b = alg_bytes(name, sv.nx, sv.allnz, sv.pml) * nsh
-        kernels[name] = {"avg_ms": round(ms, 5), "samples": n, "alg_bytes": b,
-                         "GBps": round(b / (ms * 1e-3) / 1e9, 1),
-                         "share": 0.0}
-    tot_ms = sum(v["avg_ms"] for v in kernels.values())
-    for v in kernels.values():
-        v["share"] = round(v["avg_ms"] / tot_ms, 3) if tot_ms else 0.0
-    dom = max(kernels, key=lambda k: kernels[k]["avg_ms"]) if kernels else None
+        b8 = alg_bytes_8d(name, sv.nx, sv.allnz, sv.pml, sb)
+        bc = alg_bytes(name, sv.nx, sv.allnz, sv.pml)
+        e = {"avg_ms": round(ms, 5), "samples": n}
+        if b8 is not None:
+            e["alg_bytes"] = round(b8 * avg_wave)
+            e["GBps"] = round(b8 * avg_wave / (ms * 1e-3) / 1e9, 1)
+        if bc is not None:
+            e["cell_class_bytes"] = round(bc * avg_wave)
+        kernels[name] = e
+    # launches per time step: every kernel once, the snapshot kernels once per snap_every steps
+    per_step = {k: (1.0 / args.snap_every if k.startswith("k_snap") else 1.0) for k in kernels}
+    step_ms = sum(v["avg_ms"] * per_step[k] for k, v in kernels.items())
+    for k, v in kernels.items():
+        v["share"] = round(v["avg_ms"] * per_step[k] / step_ms, 3) if step_ms else 0.0
+    dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * per_step[k]) if kernels else None
     roofline = None
-    if dom:
+    key = (args.workload, recon, storage, args.snap_every if snap else 0)
+    if dom and "GBps" in kernels[dom]:
         d = kernels[dom]
-        measured = (not args.rbc and args.workload == "shot_batch" and args.mode == "fast" and args.march_cols in (0, 8)
-                    and args.march_depth in (0, 4) and args.march_order in (0, 3))
-        traffic = round(MEASURED_HBM_BYTES_PER_SHOT[dom] * nsh) if measured and dom in MEASURED_HBM_BYTES_PER_SHOT \
-            else None
+        measured = MEASURED_HBM_BYTES_PER_SHOT.get(key, {}) if (
+            args.mode == "fast" and args.march_cols in (0, 8) and args.march_depth in (0, 4)
+            and args.march_order in (0, 3) and not args.rbc and args.cpu_mem == 0 and wave == (24 if mp else nsh)) \
+            else {}
+        traffic = round(measured[dom] * avg_wave) if dom in measured else None
+        step_bytes = sum(v.get("alg_bytes", 0) * per_step[k] for k, v in kernels.items())
         roofline = {"kernel": dom, "bound": "hbm", "achieved": d["GBps"], "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(d["GBps"] / HBM_PEAK_GBS, 4), "traffic": traffic,
-                    "alg_bytes_per_launch": d["alg_bytes"], "avg_launch_ms": d["avg_ms"]}
+                    "alg_bytes_per_launch": d["alg_bytes"], "avg_launch_ms": d["avg_ms"],
+                    "bytes_model": "scope-table unit costs (interior 52 B, PML 100 B per cell-update, "
+                                   "split per half step; imaging operands)",
+                    "step": {"alg_bytes": round(step_bytes), "kernel_ms": round(step_ms, 5),
+                             "achieved": round(step_bytes / (step_ms * 1e-3) / 1e9, 1),
+                             "frac": round(step_bytes / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
+        if measured:
+            tb = sum(measured.get(k, 0) * avg_wave * per_step[k] for k in kernels)
+            roofline["step"]["traffic"] = round(tb)
+            roofline["step"]["traffic_over_alg"] = round(tb / step_bytes, 3) if step_bytes else None
 
-    # the plain path on the same GPU: one shot per call, per-cell kernels
+    # the plain path on the same GPU: one shot per call, per-cell kernels, FP32 strips
     plain = None
     if args.plain_shots > 0:
         plain = {}
+        host0 = [Shot(sv.shots[k].left, *geo[k], sv.shots[k].ngx_left, sv.shots[k].gc, obs_z[k], obs_x[k])
+                 for k in range(min(args.plain_shots, nsh))]
         for label, mode in (("fp32_percell", "fast"), ("reference_order", "exact")):
             pc = Context(dx=sv.dx, dz=sv.dz, dt=sv.dt, nx=sv.nx, nz=sv.allnz, pml=sv.pml, nt=sv.nt,
                          ntr=sv.ntr, allnx=sv.allnx, allnz=sv.allnz, max_batch=1, mode=mode, march_cols=-1)
             pc.set_wavelet(w)
             pc.set_model(sv.vp0, sv.vs0, sv.rho0)
-            pc.gradient(shots[0])                      # warm-up
-            k = min(args.plain_shots, nsh)
+            pc.gradient(host0[0])                      # warm-up
             t0 = time.perf_counter()
-            for sh in shots[:k]:
+            for sh in host0:
                 pc.gradient(sh)
             tp = time.perf_counter() - t0
             pc.flush()
             pc.close()
-            plain[label] = {"value": round(k / tp, 4), "shots": k}
-            print("bench: plain one-shot path (%s) %.3f s per shot" % (label, tp / k), file=sys.stderr,
+            plain[label] = {"value": round(len(host0) / tp, 4), "shots": len(host0)}
+            print("bench: plain one-shot path (%s) %.3f s per shot" % (label, tp / len(host0)), file=sys.stderr,
                   flush=True)
         plain["unit"] = "shot-gradients/s"
+        plain["path"] = "one shot per call, per-cell kernels, FP32 boundary strips (the reference's scheme)"
         plain["speedup_of_batched"] = round(value / plain["fp32_percell"]["value"], 2)
 
-    # accuracy of two shots against the FP64 oracle (and the CPU baseline)
+    # accuracy of one shot against the FP64 oracle of the same algorithm (also the CPU baseline)
     acc = None
     cpu = None
-    if not args.no_cpu and args.misfit == "l2":     # the oracle leg is the L2 gradient
+    if not args.no_cpu and args.misfit == "l2":
         from oracle import oracle as O
         ncpu = max(1, min(args.cpu_shots, nsh))
-        sub = shots[:ncpu]
-        ctx.batch_gradient(sub)
+        sub = [shots[k] for k in range(ncpu)]
+        m_gpu = ctx.batch_gradient(sub)
         g_vp, g_vs, g_il = ctx.flush()
-        threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        threads = int(os.environ.get("OMP_NUM_THREADS") or threads)
         r_vp = np.zeros((sv.allnx, sv.allnz))
         r_vs = np.zeros_like(r_vp)
+        r_il = np.zeros_like(r_vp)
+        r_m = 0.0
         t0 = time.perf_counter()
-        for s, g, sh in zip(sv.shots[:ncpu], geo[:ncpu], sub):
+        for k in range(ncpu):
+            s, g = sv.shots[k], geo[k]
             c = O.config(allnx=sv.allnx, allnz=sv.allnz, nx=sv.nx, nz=sv.allnz, pml=sv.pml, nt=sv.nt,
-                         ntr=sv.ntr, dx=sv.dx, dz=sv.dz, dt=sv.dt, left=s.left, isx=g[0], isz=g[1],
-                         igz=g[2], ngx_left=s.ngx_left, nthreads=threads, rbc=int(args.rbc), rbc_seed=1)
-            a = O.gradient(c, sv.vp0, sv.vs0, sv.rho0, w, s.gc, sh.obs_z, sh.obs_x, "f64")
+                         ntr=len(s.gc), dx=sv.dx, dz=sv.dz, dt=sv.dt, left=s.left, isx=g[0], isz=g[1],
+                         igz=g[2], ngx_left=s.ngx_left, nthreads=threads, rbc=int(args.rbc), rbc_seed=1,
+                         snap_every=args.snap_every if snap else 0)
+            oz_k, ox_k = obs_z[k][:len(s.gc)], obs_x[k][:len(s.gc)]
+            if snap:
+                a = O.gradient_snapshot(c, sv.vp0, sv.vs0, sv.rho0, w, s.gc, oz_k, ox_k)
+            else:
+                a = O.gradient(c, sv.vp0, sv.vs0, sv.rho0, w, s.gc, oz_k, ox_k, "f64")
             r_vp += a[0]
             r_vs += a[1]
+            r_il += a[2]
+            r_m += a[3]
         tc = time.perf_counter() - t0
+        # traces: the GPU's forward solve of the true model against the FP64 oracle's, shot 0
+        s, g = sv.shots[0], geo[0]
+        c = O.config(allnx=sv.allnx, allnz=sv.allnz, nx=sv.nx, nz=sv.allnz, pml=sv.pml, nt=sv.nt,
+                     ntr=len(s.gc), dx=sv.dx, dz=sv.dz, dt=sv.dt, left=s.left, isx=g[0], isz=g[1],
+                     igz=g[2], ngx_left=s.ngx_left, nthreads=threads)
+        t1 = time.perf_counter()
+        fz, fx = O.forward(c, sv.vp, sv.vs, sv.rho, w, s.gc, "f64")
+        tf = time.perf_counter() - t1
 
         def rel(x, y):
-            return float(np.linalg.norm(x.astype(np.float64) - y) / np.linalg.norm(y))
+            return float(np.linalg.norm(np.asarray(x, np.float64) - y) / np.linalg.norm(y))
 
-        acc = {"grad_vp": rel(g_vp, r_vp), "grad_vs": rel(g_vs, r_vs), "shots": ncpu,
-               "bar": 1e-3}
+        nt0 = len(s.gc)
+        acc = {"grad_vp": rel(g_vp, r_vp), "grad_vs": rel(g_vs, r_vs), "illum": rel(g_il, r_il),
+               "misfit": abs(float(m_gpu.sum()) - r_m) / abs(r_m),
+               "traces": rel(np.stack([obs_z[0][:nt0], obs_x[0][:nt0]]), np.stack([fz, fx])),
+               "shots": ncpu, "bar": 1e-3,
+               "reference": "FP64 oracle of the same algorithm (%s)" % (
+                   "snapshot gradient, imaging every %d steps" % args.snap_every if snap else "strip gradient")}
         cpu = {"value": round(ncpu / tc, 5), "unit": "shot-gradients/s", "cores": threads,
-               "kind": "port",
-               "sample": "FP64 oracle (C++ restatement, OpenMP over x columns), %d of the %d shots "
-                         "of the same workload, %.1f s" % (ncpu, nsh, tc)}
+               "cpu": cpu_model(), "nproc": os.cpu_count(), "kind": "port",
+               "sample": "FP64 oracle (C++ restatement, OpenMP over x columns) of the same gradient, %d of the "
+                         "%d shots of the same workload, %.1f s (the FP64 forward solve for the trace check, "
+                         "%.1f s, is not counted)" % (ncpu, nsh, tc, tf)}
 
     lat = sorted(times)
     line = {
@@ -312,17 +415,20 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * total / args.steps, 2),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded model, observed traces from a forward solve of the true model)",
-        "config": {"workload": "%s: %dx%d dx=dz=%gm, pml %d, dt %gms, nt %d, %d shots, %g Hz, "
+        "config": {"workload": "%s: %dx%d dx=dz=%gm, pml %d, dt %gms, nt %d, %d shots in waves of %d, %g Hz, "
                                "%d receivers, batched gradient" % (sv.name, sv.nx, sv.allnz, sv.dx, sv.pml,
-                                                                    sv.dt * 1e3, sv.nt, nsh, sv.f0, sv.ntr),
-                   "mode": args.mode, "misfit": args.misfit, "storage": "random boundary, no strips" if args.rbc
-                   else "boundary strips %s%s" % (
-                       args.storage.upper(),
-                       ", pinned host memory in chunks of %d steps" % args.cpu_mem if args.cpu_mem > 0 else "")},
+                                                                    sv.dt * 1e3, sv.nt, nsh, wave, sv.f0, sv.ntr),
+                   "mode": args.mode, "misfit": args.misfit,
+                   "storage": "random boundary, no strips" if args.rbc else (
+                       "velocity snapshots every %d steps, %s with a max-abs scale per snapshot" % (
+                           args.snap_every, storage.upper()) if snap else
+                       "boundary strips %s%s" % (storage.upper(), ", pinned host memory in chunks of %d steps" %
+                                                 args.cpu_mem if args.cpu_mem > 0 else ""))},
         "gcell_updates_per_s": round(gcells, 2),
-        "pcie_inclusive_value": round(len(shots) / t_pcie, 4),
+        "gcell_propagations_per_gradient": props,
+        "pcie_inclusive_value": round(len(host_shots) / t_pcie, 4),
         "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 2),
-                       "p99": round(1e3 * float(np.percentile(lat, 99)), 2), "unit": "per batch step"},
+                       "p99": round(1e3 * float(np.percentile(lat, 99)), 2), "unit": "per batch step (one wave)"},
         "grad_err_vs_fp64": acc,
         "misfit": misfit,
         "roofline": roofline,
@@ -332,6 +438,22 @@ def main():
     }
     print(json.dumps(line), flush=True)
     ctx.close()
+
+
+def copy_calibration(mib):
+    """Counter calibration: one plain elementwise kernel that reads `mib` MiB and
+    writes `mib` MiB (torch's out = a * 1, after a fill).  FETCH_SIZE and
+    WRITE_SIZE of that dispatch, against its known bytes, give the scale
+    factors applied to the stencil kernels' counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.empty(n, dtype=torch.float32, device="cuda")
+    a.fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"read_bytes": n * 4, "write_bytes": n * 4}), flush=True)
 
 
 def born(args):

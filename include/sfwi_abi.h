@@ -147,7 +147,20 @@ typedef struct sfwi_shot {
                            /* the previous call for the same batch position (shot   */
                            /* i of a batch of <= max_batch shots sits in slot i);   */
                            /* the trace count must match.                          */
+    int obs_id;            /* > 0: the observed traces stored on the device under   */
+                           /* this id by sfwi_obs_put (obs_z, obs_x are ignored and */
+                           /* nothing is copied); 0: obs_z / obs_x as above         */
 } sfwi_shot;
+
+/* Observed traces resident on the device (no reference counterpart: the
+ * reference uploads each shot's records before every IMAGE_SINGLESHOT call,
+ * record_copytoGPU, sfwi-stage1/FD2DGPU.cu:1327-1333).  sfwi_obs_put stores a
+ * shot's traces ([ntr][nt] per component) under obs_id > 0 (replacing what the
+ * id held); a shot with that obs_id then reads them in place, so an inversion
+ * or a benchmark keeps a whole survey's records in HBM and a solve copies
+ * none.  sfwi_obs_clear frees every stored set. */
+int sfwi_obs_put(sfwi_ctx *ctx, int obs_id, int ntr, const float *obs_z, const float *obs_x);
+int sfwi_obs_clear(sfwi_ctx *ctx);
 
 /* Replaces the FD2DGPU_ELASTIC constructor (sfwi-stage1/FD2DGPU.cu:1366-1655)
  * minus the source wavelet, which sfwi_set_wavelet uploads. */

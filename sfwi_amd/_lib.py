@@ -46,6 +46,7 @@ class SfwiShot(ctypes.Structure):
         ("igz", ctypes.c_int), ("ngx_left", ctypes.c_int), ("ntr", ctypes.c_int),
         ("gc", ctypes.POINTER(ctypes.c_int)),
         ("obs_z", ctypes.POINTER(ctypes.c_float)), ("obs_x", ctypes.POINTER(ctypes.c_float)),
+        ("obs_id", ctypes.c_int),
     ]
 
 
@@ -114,6 +115,8 @@ def load():
     L.sfwi_batch_born_gradient_res.argtypes = [vp, ctypes.c_int, P(SfwiShot), dp, vp, vp]
     L.sfwi_batch_linesearch_linear_res.argtypes = [vp, ctypes.c_int, P(SfwiShot), _f32p, _f32p, vp, vp, dp, dp]
     L.sfwi_flush.argtypes = [vp, vp, vp, vp]
+    L.sfwi_obs_put.argtypes = [vp, ctypes.c_int, ctypes.c_int, _f32p, _f32p]
+    L.sfwi_obs_clear.argtypes = [vp]
     L.sfwi_flush_rho.argtypes = [vp, vp]
     L.sfwi_ot_misfit.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                                  _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, dp]

@@ -93,6 +93,7 @@ struct DevShot {
     fptr snap32;                           // velocity snapshots, FP32 [nsnap][2][snap_n]
     gptr<unsigned short> snap16;           // velocity snapshots, FP16 / BF16 of v / scale [nsnap][2][snap_n]
     gptr<unsigned> snap_max;               // max |v| per snapshot and field as FP32 bits [nsnap][2]
+    fptr snap_part;                        // partial maxima of the snapshot being stored [2][SNAP_PART]
     fptr gvp, gvs, gil;                    // per-shot gradient accumulators [nx][nz]
     fptr grho;                             // per-shot density gradient [nx][nz] (grad_rho)
     fptr vold_x, vold_z;                   // v(it+1) before the reconstruction velocity step (grad_rho)
@@ -116,6 +117,8 @@ struct Geo {
     int grad_rho;           // 1: also image the density
     int snap_k;             // snapshot reconstruction: steps between snapshots (0: strips)
     int snap_rz, snap_n;    // snapshot region height nz + 8 and cells (nx + 8)(nz + 8), origin (pml-4, pml-4)
+    int im_nbands, im_waves, im_wg, im_per_xcd;   // snapshot imaging march: row bands, waves and workgroups
+                                                  // per shot over the interior, workgroups per XCD (launch)
     float inv_nzd;
     float dt, dx, dz, dtdx, dtdz, idx, idz;
 };
@@ -996,26 +999,57 @@ __device__ inline int snap_cell(const Geo &g, int t) {   // padded index of regi
     return (x + g.pml - NOP) * g.nzp + (z + g.pml - NOP);
 }
 
-// Max |v| of snapshot k (16-bit storage): a wave maximum, then one atomic per
-// wave and field on the FP32 bits (non-negative floats order as integers).
-__global__ void __launch_bounds__(BLK) k_snap_max(Geo g, const DevShot *__restrict__ shots, int k) {
-    const DevShot &d = shots[blockIdx.y];
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    float mx = 0.0f, mz = 0.0f;
-    if (t < g.snap_n) {
-        const int i = snap_cell(g, t);
-        mx = fabsf(d.fw.vx[i]);
-        mz = fabsf(d.fw.vz[i]);
-    }
+// Max |v| of snapshot k (16-bit storage) in two passes without atomics:
+// k_snap_max gives block b of a shot the region cells b BLK + j + m SNAP_PART
+// BLK and writes that block's maximum of each field to the shot's partials;
+// k_snap_scale reduces a shot's SNAP_PART partials to the snapshot's scales.
+constexpr int SNAP_PART = 256;   // blocks (partial maxima per field) of k_snap_max per shot
+
+__device__ inline float wave_max(float v) {
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        mx = fmaxf(mx, __shfl_xor(mx, o));
-        mz = fmaxf(mz, __shfl_xor(mz, o));
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+
+__global__ void __launch_bounds__(BLK) k_snap_max(Geo g, const DevShot *__restrict__ shots) {
+    __shared__ float red[2][BLK / 64];
+    const DevShot &d = shots[blockIdx.y];
+    float mx = 0.0f, mz = 0.0f;
+    for (int t = blockIdx.x * BLK + threadIdx.x; t < g.snap_n; t += SNAP_PART * BLK) {
+        const int i = snap_cell(g, t);
+        mx = fmaxf(mx, fabsf(d.fw.vx[i]));
+        mz = fmaxf(mz, fabsf(d.fw.vz[i]));
     }
+    mx = wave_max(mx);
+    mz = wave_max(mz);
+    const int wv = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        unsigned *m = (unsigned *)d.snap_max + 2 * (size_t)k;
-        atomicMax(m, __float_as_uint(mx));
-        atomicMax(m + 1, __float_as_uint(mz));
+        red[0][wv] = mx;
+        red[1][wv] = mz;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        float m = red[threadIdx.x][0];
+#pragma unroll
+        for (int k = 1; k < BLK / 64; ++k) m = fmaxf(m, red[threadIdx.x][k]);
+        d.snap_part[threadIdx.x * SNAP_PART + blockIdx.x] = m;
+    }
+}
+
+// One wave per shot: the scales of snapshot k from the partial maxima, stored
+// as FP32 bits (non-negative floats order as integers).
+__global__ void __launch_bounds__(64) k_snap_scale(const DevShot *__restrict__ shots, int k) {
+    const DevShot &d = shots[blockIdx.x];
+    float mx = 0.0f, mz = 0.0f;
+    for (int j = threadIdx.x; j < SNAP_PART; j += 64) {
+        mx = fmaxf(mx, d.snap_part[j]);
+        mz = fmaxf(mz, d.snap_part[SNAP_PART + j]);
+    }
+    mx = wave_max(mx);
+    mz = wave_max(mz);
+    if (threadIdx.x == 0) {
+        d.snap_max[2 * (size_t)k] = __float_as_uint(mx);
+        d.snap_max[2 * (size_t)k + 1] = __float_as_uint(mz);
     }
 }
 
@@ -1025,21 +1059,25 @@ __global__ void __launch_bounds__(BLK) k_snap_pack(Geo g, const DevShot *__restr
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= g.snap_n) return;
     const int i = snap_cell(g, t);
-    const size_t o = (size_t)k * 2 * g.snap_n + t;
+    // snapshot k starts at a 64-bit offset; indices within it stay below 2^31
+    const size_t o = (size_t)k * 2 * g.snap_n;
     if (g.store == 0) {
-        d.snap32[o] = d.fw.vx[i];
-        d.snap32[o + g.snap_n] = d.fw.vz[i];
+        const fptr s = d.snap32 + o;
+        s[t] = d.fw.vx[i];
+        s[g.snap_n + t] = d.fw.vz[i];
     } else {
+        const gptr<unsigned short> s = d.snap16 + o;
         const float sx = __uint_as_float(d.snap_max[2 * (size_t)k]), sz = __uint_as_float(d.snap_max[2 * (size_t)k + 1]);
         const float ix = sx > 0.0f ? 1.0f / sx : 0.0f, iz = sz > 0.0f ? 1.0f / sz : 0.0f;
-        d.snap16[o] = pack16(g.store, d.fw.vx[i] * ix);
-        d.snap16[o + g.snap_n] = pack16(g.store, d.fw.vz[i] * iz);
+        s[t] = pack16(g.store, d.fw.vx[i] * ix);
+        s[g.snap_n + t] = pack16(g.store, d.fw.vz[i] * iz);
     }
 }
 
-// Value of snapshot field f (0 vx, 1 vz) at region cell t, unscaled.
-__device__ inline float snap_val(const Geo &g, const DevShot &d, size_t base, int t) {
-    return g.store == 0 ? d.snap32[base + t] : unpack16(g.store, d.snap16[base + t]);
+// Value of snapshot k's field f (0 vx, 1 vz) at region cell t, unscaled.
+__device__ inline float snap_val(const Geo &g, const DevShot &d, int k, int f, int t) {
+    const size_t o = ((size_t)k * 2 + f) * g.snap_n;
+    return g.store == 0 ? (d.snap32 + o)[t] : unpack16(g.store, (d.snap16 + o)[t]);
 }
 
 // Zero-lag imaging of the snapshot gradient at snapshot k, one thread per
@@ -1055,30 +1093,30 @@ __global__ void __launch_bounds__(BLK) k_snap_image(Geo g, const DevShot *__rest
     const int jx = q / g.nz, jz = q - jx * g.nz;
     const int R = g.snap_rz;
     const int t = (jx + NOP) * R + (jz + NOP);   // region cell of this interior cell
-    const size_t bx = (size_t)k * 2 * g.snap_n, bz = bx + g.snap_n;
+    const int bx = 0, bz = 1;   // fields of snapshot k
     float sx = 1.0f, sz = 1.0f;
     if (g.store != 0) {
         sx = __uint_as_float(d.snap_max[2 * (size_t)k]);
         sz = __uint_as_float(d.snap_max[2 * (size_t)k + 1]);
     }
     // sum_k c_k (f[t+(k-1)s] - f[t-ks]) and sum_k c_k (f[t+ks] - f[t-(k-1)s])
-    auto dm = [&](size_t b, int s) {
-        float a = kC[1] * (snap_val(g, d, b, t) - snap_val(g, d, b, t - s));
-        a = __builtin_fmaf(kC[2], snap_val(g, d, b, t + s) - snap_val(g, d, b, t - 2 * s), a);
-        a = __builtin_fmaf(kC[3], snap_val(g, d, b, t + 2 * s) - snap_val(g, d, b, t - 3 * s), a);
-        a = __builtin_fmaf(kC[4], snap_val(g, d, b, t + 3 * s) - snap_val(g, d, b, t - 4 * s), a);
+    auto dm = [&](int b, int s) {
+        float a = kC[1] * (snap_val(g, d, k, b, t) - snap_val(g, d, k, b, t - s));
+        a = __builtin_fmaf(kC[2], snap_val(g, d, k, b, t + s) - snap_val(g, d, k, b, t - 2 * s), a);
+        a = __builtin_fmaf(kC[3], snap_val(g, d, k, b, t + 2 * s) - snap_val(g, d, k, b, t - 3 * s), a);
+        a = __builtin_fmaf(kC[4], snap_val(g, d, k, b, t + 3 * s) - snap_val(g, d, k, b, t - 4 * s), a);
         return a;
     };
-    auto dp = [&](size_t b, int s) {
-        float a = kC[1] * (snap_val(g, d, b, t + s) - snap_val(g, d, b, t));
-        a = __builtin_fmaf(kC[2], snap_val(g, d, b, t + 2 * s) - snap_val(g, d, b, t - s), a);
-        a = __builtin_fmaf(kC[3], snap_val(g, d, b, t + 3 * s) - snap_val(g, d, b, t - 2 * s), a);
-        a = __builtin_fmaf(kC[4], snap_val(g, d, b, t + 4 * s) - snap_val(g, d, b, t - 3 * s), a);
+    auto dp = [&](int b, int s) {
+        float a = kC[1] * (snap_val(g, d, k, b, t + s) - snap_val(g, d, k, b, t));
+        a = __builtin_fmaf(kC[2], snap_val(g, d, k, b, t + 2 * s) - snap_val(g, d, k, b, t - s), a);
+        a = __builtin_fmaf(kC[3], snap_val(g, d, k, b, t + 3 * s) - snap_val(g, d, k, b, t - 2 * s), a);
+        a = __builtin_fmaf(kC[4], snap_val(g, d, k, b, t + 4 * s) - snap_val(g, d, k, b, t - 3 * s), a);
         return a;
     };
     const float gx = dm(bx, R) * (sx * g.idx), gz = dm(bz, 1) * (sz * g.idz);
     const float gxz = dp(bx, 1) * (sx * g.idz), gzx = dp(bz, R) * (sz * g.idx);
-    const float vx = snap_val(g, d, bx, t) * sx, vz = snap_val(g, d, bz, t) * sz;
+    const float vx = snap_val(g, d, k, bx, t) * sx, vz = snap_val(g, d, k, bz, t) * sz;
     const int i = (jx + g.pml) * g.nzp + (jz + g.pml);
     float op, os;
     image_vpvs(gx, gz, gxz, gzx, d.bk.txx[i], d.bk.tzz[i], d.bk.txz[i], d.gvp[q], d.gvs[q], op, os);

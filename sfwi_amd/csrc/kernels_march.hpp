@@ -587,4 +587,79 @@ __global__ void __launch_bounds__(BLK) km_adj_vel_image(Geo g, const DevShot *__
     }
 }
 
+
+// Zero-lag imaging of the snapshot gradient at snapshot k as a column march
+// over the interior (the arithmetic of k_snap_image, so both kernel families
+// give identical images).  A wave owns 56 interior rows of one shot (plus 4
+// halo rows each side, which the region holds) and 8 interior columns; the
+// x derivatives come from 8-column register windows of the kept vx (columns
+// c-4..c+3, Dx-) and vz (c-3..c+4, Dx+), the z derivatives from DPP shifts of
+// the lane's own column value.  ST: snapshot storage (0 FP32, 1 FP16, 2 BF16).
+// Block -> (shot, tile): each XCD owns a contiguous range of whole shots.
+template <int ST>
+__global__ void __launch_bounds__(BLK) km_snap_image(Geo g, const DevShot *__restrict__ shots, int nshots, int k) {
+    const int b = blockIdx.x;
+    const int L = (b & 7) * g.im_per_xcd + (b >> 3);
+    const int shot = L / g.im_wg;
+    if (shot >= nshots) return;
+    const int W = (L - shot * g.im_wg) * (BLK / 64) + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (W >= g.im_waves) return;
+    const int seg = W / g.im_nbands, band = W - seg * g.im_nbands;
+    const int x0 = seg * 8, x1 = min(x0 + 8, g.nx);   // interior columns
+    const DevShot d = shots[shot];
+    const int lane = threadIdx.x & 63;
+    const int jz = band * CM_ROWS - 4 + lane;           // interior row of this lane (halo lanes outside)
+    const int R = g.snap_rz;
+    const int rz = min(jz + NOP, R - 1);                 // region row (clamped load address)
+    const bool act = lane >= 4 && lane < 60 && jz < g.nz;
+    const size_t bx = (size_t)k * 2 * g.snap_n, bz = bx + g.snap_n;   // 64-bit offsets of the two fields
+    float sx = 1.0f, sz = 1.0f;
+    if (ST != 0) {
+        sx = __uint_as_float(d.snap_max[2 * (size_t)k]);
+        sz = __uint_as_float(d.snap_max[2 * (size_t)k + 1]);
+    }
+    auto val = [&](size_t base, int rx) -> float {   // unscaled snapshot value at region column rx
+        const int o = rx * R + rz;   // within one field of one snapshot
+        if (ST == 0) return (d.snap32 + base)[o];
+        return unpack16(ST, (d.snap16 + base)[o]);
+    };
+    // region column of interior column c is c + 4: vx window of column c covers
+    // region columns c .. c+7, vz window c+1 .. c+8
+    float wx[8], wz[8];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        wx[j] = val(bx, x0 + j);
+        wz[j] = val(bz, x0 + 1 + j);
+    }
+    float nx_ = val(bx, x0 + 7), nz_ = val(bz, x0 + 8);
+    for (int c = x0; c < x1; ++c) {
+        wx[7] = nx_;
+        wz[7] = nz_;
+        if (c + 1 < x1) {   // next column's window entries
+            nx_ = val(bx, c + 8);
+            nz_ = val(bz, c + 9);
+        }
+        const int i = (c + g.pml) * g.nzp + (jz + g.pml);
+        const int q = c * g.nz + jz;
+        float sxx = 0.0f, szz = 0.0f, sxz = 0.0f, ip = 0.0f, is = 0.0f, il = 0.0f;
+        if (act) {
+            sxx = d.bk.txx[i]; szz = d.bk.tzz[i]; sxz = d.bk.txz[i];
+            ip = d.gvp[q]; is = d.gvs[q]; il = d.gil[q];
+        }
+        const float vxc = wx[4], vzc = wz[3];             // kept velocity at column c
+        const float gx = x_win(wx) * (sx * g.idx), gz = z_minus(vzc) * (sz * g.idz);
+        const float gxz = z_plus(vxc) * (sx * g.idz), gzx = x_win(wz) * (sz * g.idx);
+        shift(wx);
+        shift(wz);
+        if (act) {
+            const float vx = vxc * sx, vz = vzc * sz;
+            float op, os;
+            image_vpvs(gx, gz, gxz, gzx, sxx, szz, sxz, ip, is, op, os);
+            d.gvp[q] = op;
+            d.gvs[q] = os;
+            d.gil[q] = __builtin_fmaf(vx, vx, __builtin_fmaf(vz, vz, il));
+        }
+    }
+}
+
 }  // namespace sfwi

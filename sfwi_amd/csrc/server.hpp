@@ -37,7 +37,7 @@ size_t solve_bytes(const sfwi_params &p, int kind, int nshots) {
     if (grad && p.recon == SFWI_RECON_SNAPSHOTS) {   // velocity snapshots instead of strips
         const size_t k = p.snap_every > 0 ? (size_t)p.snap_every : 4, nsnap = (p.nt + k - 1) / k;
         const size_t rn = (size_t)(p.nx + 8) * (p.nz + 8);
-        slot += ((nsnap * 8 + 255) & ~(size_t)255) + nsnap * 2 * rn * (p.storage ? 2 : 4);
+        slot += ((nsnap * 8 + 255) & ~(size_t)255) + 2048 + nsnap * 2 * rn * (p.storage ? 2 : 4);
     } else if (grad) {
         slot += p.storage ? ring * 2 * sf * 2 + ring * 8 + 2 * sf * f + 256 : ring * 2 * sf * f;
     }

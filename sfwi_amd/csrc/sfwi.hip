@@ -46,6 +46,11 @@ struct Slot {
     std::vector<int> h_igx, h_inj, h_off;   // host copies kept alive for async uploads
 };
 
+struct ObsSet {        // observed traces resident under an id (sfwi_obs_put)
+    float *base = nullptr;   // [ntr][nt] vz, then [ntr][nt] vx
+    int ntr = 0;
+};
+
 struct Window {
     float *base = nullptr;
     Medium m{};
@@ -75,6 +80,7 @@ struct sfwi_ctx {
     DevShot *d_shots = nullptr;
     std::vector<DevShot> h_shots;
     std::map<int, Window> windows;
+    std::map<int, ObsSet> obs;    // resident observed traces by id
     double *d_out = nullptr;      // per-shot misfits [max_batch] + 2 scalars
     unsigned *d_vmax = nullptr;   // largest Vp of a window (random boundary)
     double *d_num = nullptr, *d_den = nullptr;
@@ -225,11 +231,13 @@ int model_window(sfwi_ctx *c, int left, bool born, const Medium **out) {
 
 // -------------------------------------------------------------- slots ---
 
-// Bytes before the snapshot values of a slot: the max-abs words, 256-B aligned.
-size_t snap_head(const sfwi_ctx *c) {
+// Bytes before the snapshot values of a slot: the max-abs words (256-B
+// aligned), then the partial maxima of the snapshot being stored.
+size_t snap_words(const sfwi_ctx *c) {
     const size_t nsnap = (c->p.nt + c->g.snap_k - 1) / c->g.snap_k;
     return (nsnap * 2 * sizeof(unsigned) + 255) & ~(size_t)255;
 }
+size_t snap_head(const sfwi_ctx *c) { return snap_words(c) + 2 * SNAP_PART * sizeof(float); }
 
 int ensure_slots(sfwi_ctx *c, int n, bool born, bool strips, bool snaps) {
     const size_t cell = c->ncell * sizeof(float);
@@ -358,7 +366,15 @@ int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium 
     for (int k = 0; k < n; ++k) {
         const sfwi_shot &s = shots[k];
         if ((r = check_shot(c, s, k))) return r;
-        if (obs && (!s.obs_z || !s.obs_x) && (s.obs_z || s.obs_x || c->slots[k].obs_ntr != s.ntr))
+        const ObsSet *os = nullptr;
+        if (obs && s.obs_id > 0) {
+            const auto f = c->obs.find(s.obs_id);
+            if (f == c->obs.end()) return fail(c, SFWI_EINVAL, "shot %d: no observed traces stored under id %d", k, s.obs_id);
+            if (f->second.ntr != s.ntr)
+                return fail(c, SFWI_EINVAL, "shot %d: id %d holds %d traces, the shot has %d", k, s.obs_id,
+                            f->second.ntr, s.ntr);
+            os = &f->second;
+        } else if (obs && (!s.obs_z || !s.obs_x) && (s.obs_z || s.obs_x || c->slots[k].obs_ntr != s.ntr))
             return fail(c, SFWI_EINVAL, "shot %d: observed traces missing (and no resident records of %d traces "
                         "in batch slot %d)", k, s.ntr, k);
         Slot &S = c->slots[k];
@@ -375,10 +391,15 @@ int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium 
         d.fw = S.fw; d.bk = S.bk; d.bn = S.bn;
         d.dgx = S.dg[0]; d.dgz = S.dg[1]; d.dgxz = S.dg[2]; d.dgzx = S.dg[3];
         d.rz = S.rz; d.rx = S.rx; d.r2z = S.r2z; d.r2x = S.r2x; d.oz = S.oz; d.ox = S.ox;
+        if (os) {   // read in place from the resident set
+            d.oz = os->base;
+            d.ox = os->base + (size_t)os->ntr * c->p.nt;
+        }
         d.igx = S.igx; d.inj = S.inj; d.inj_off = S.inj_off; d.strip = S.strip;
         if (snaps) {
             const size_t head = snap_head(c);
             d.snap_max = (unsigned *)S.snap;
+            d.snap_part = (float *)(S.snap + snap_words(c));
             d.snap32 = (float *)(S.snap + head);
             d.snap16 = (unsigned short *)(S.snap + head);
         }
@@ -402,8 +423,8 @@ int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium 
         HIPCK(c, hipMemcpyAsync(S.inj, S.h_inj.data(), S.h_inj.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
         HIPCK(c, hipMemcpyAsync(S.inj_off, S.h_off.data(), S.h_off.size() * sizeof(int), hipMemcpyHostToDevice,
                                 c->stream));
-        if (snaps) HIPCK(c, hipMemsetAsync(S.snap, 0, snap_head(c), c->stream));   // max-abs words start at 0
-        if (obs && s.obs_z) {          // NULL: the records resident in this slot are reused
+        if (snaps) HIPCK(c, hipMemsetAsync(S.snap, 0, snap_words(c), c->stream));   // scales of FP32 snapshots: 0
+        if (obs && !os && s.obs_z) {   // NULL: the records resident in this slot are reused
             const size_t rb = (size_t)s.ntr * c->p.nt * sizeof(float);
             HIPCK(c, hipMemcpyAsync(S.oz, s.obs_z, rb, hipMemcpyHostToDevice, c->stream));
             HIPCK(c, hipMemcpyAsync(S.ox, s.obs_x, rb, hipMemcpyHostToDevice, c->stream));
@@ -614,7 +635,10 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
         if (save && g.store) hipLaunchKernelGGL(k_strip_pack, dim3(2, (unsigned)n), dim3(256), 0, st, g, D, sp);
         if (snap && it % g.snap_k == 0) {   // keep the new velocity (snapshot it / snap_k)
             t = tbeg(c, SFWI_K_SNAP_STORE, it);
-            if (g.store) hipLaunchKernelGGL(k_snap_max, snapg, blk, 0, st, g, D, it / g.snap_k);
+            if (g.store) {   // max-abs scales: per-block partials, then one wave per shot
+                hipLaunchKernelGGL(k_snap_max, dim3(SNAP_PART, (unsigned)n), blk, 0, st, g, D);
+                hipLaunchKernelGGL(k_snap_scale, dim3((unsigned)n), dim3(64), 0, st, D, it / g.snap_k);
+            }
             hipLaunchKernelGGL(k_snap_pack, snapg, blk, 0, st, g, D, it / g.snap_k);
             tend(c, t);
         }
@@ -664,6 +688,9 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
         // snap_k, at the merge.
         const float fac = c->p.misfit == SFWI_MISFIT_L2 ? 2.0f : 1.0f;
         const dim3 ig((unsigned)((c->nint + BLK - 1) / BLK), (unsigned)n);
+        Geo gi = g;   // imaging march: whole shots per XCD
+        gi.im_per_xcd = (int)(((long)g.im_wg * n + 7) / 8);
+        const dim3 igm((unsigned)(8 * gi.im_per_xcd));
         for (int it = nt - 1; it >= 0; --it) {
             hipLaunchKernelGGL(k_inject_add, trg, dim3(64), 0, st, g, D, it, fac);
             int t = tbeg(c, SFWI_K_ADJ_VEL_ONLY, it);
@@ -672,7 +699,11 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
             tend(c, t);
             if (it % g.snap_k == 0) {
                 t = tbeg(c, SFWI_K_SNAP_IMAGE, it);
-                hipLaunchKernelGGL(k_snap_image, ig, blk, 0, st, g, D, it / g.snap_k);
+                const int k = it / g.snap_k;
+                if (percell) hipLaunchKernelGGL(k_snap_image, ig, blk, 0, st, g, D, k);
+                else if (g.store == 0) hipLaunchKernelGGL(km_snap_image<0>, igm, blk, 0, st, gi, D, n, k);
+                else if (g.store == 1) hipLaunchKernelGGL(km_snap_image<1>, igm, blk, 0, st, gi, D, n, k);
+                else hipLaunchKernelGGL(km_snap_image<2>, igm, blk, 0, st, gi, D, n, k);
                 tend(c, t);
             }
             t = tbeg(c, SFWI_K_ADJ_STR, it);
@@ -822,6 +853,26 @@ int finish(sfwi_ctx *c) {
     return check_ot(c);
 }
 
+// A shot's observed traces into its slot: from the resident set of its
+// obs_id, else from the host arrays.
+int load_obs(sfwi_ctx *c, const sfwi_shot &s, Slot &S) {
+    const size_t rb = (size_t)s.ntr * c->p.nt * sizeof(float);
+    if (s.obs_id > 0) {
+        const auto f = c->obs.find(s.obs_id);
+        if (f == c->obs.end() || f->second.ntr != s.ntr)
+            return fail(c, SFWI_EINVAL, "no observed traces of %d traces stored under id %d", s.ntr, s.obs_id);
+        const float *b = f->second.base;
+        HIPCK(c, hipMemcpyAsync(S.oz, b, rb, hipMemcpyDeviceToDevice, c->stream));
+        HIPCK(c, hipMemcpyAsync(S.ox, b + (size_t)s.ntr * c->p.nt, rb, hipMemcpyDeviceToDevice, c->stream));
+    } else {
+        if (!s.obs_z || !s.obs_x) return fail(c, SFWI_EINVAL, "observed traces missing");
+        HIPCK(c, hipMemcpyAsync(S.oz, s.obs_z, rb, hipMemcpyHostToDevice, c->stream));
+        HIPCK(c, hipMemcpyAsync(S.ox, s.obs_x, rb, hipMemcpyHostToDevice, c->stream));
+    }
+    S.obs_ntr = s.ntr;
+    return 0;
+}
+
 // Batched solve with misfit output; shots processed in groups of max_batch.
 int copy_records(sfwi_ctx *c, const float *src, float *dst, int ntr);
 
@@ -921,6 +972,9 @@ int sfwi_ctx_create(const sfwi_params *p, sfwi_ctx **out) {
         g.snap_k = p->snap_every > 0 ? p->snap_every : 4;
         g.snap_rz = p->nz + 2 * NOP;
         g.snap_n = (p->nx + 2 * NOP) * g.snap_rz;
+        g.im_nbands = (p->nz + CM_ROWS - 1) / CM_ROWS;
+        g.im_waves = ((p->nx + 7) / 8) * g.im_nbands;
+        g.im_wg = (g.im_waves + BLK / 64 - 1) / (BLK / 64);
     }
     g.inv_nzd = 1.0f / (float)g.nzd;
     g.dt = p->dt; g.dx = p->dx; g.dz = p->dz;
@@ -986,6 +1040,7 @@ void sfwi_ctx_destroy(sfwi_ctx *c) {
     }
     if (c->cstream) hipStreamDestroy(c->cstream);
     drop_windows(c);
+    for (auto &kv : c->obs) hipFree(kv.second.base);
     for (hipEvent_t e : c->ev) hipEventDestroy(e);
     hipFree(c->wav); hipFree(c->mvp); hipFree(c->tmp_model); hipFree(c->Gvp); hipFree(c->res_keep);
     hipFree(c->d_shots); hipFree(c->d_out); hipFree(c->d_num); hipFree(c->res_batch); hipFree(c->dot_batch); hipFree(c->d_vmax);
@@ -1026,6 +1081,39 @@ int sfwi_set_perturbation(sfwi_ctx *c, const float *dvp, const float *dvs) {
     HIPCK(c, hipMemcpyAsync(c->mdvs, dvs, b, hipMemcpyHostToDevice, c->stream));
     HIPCK(c, hipStreamSynchronize(c->stream));
     c->have_pert = true;
+    return 0;
+}
+
+int sfwi_obs_put(sfwi_ctx *c, int id, int ntr, const float *obs_z, const float *obs_x) {
+    if (!c || !obs_z || !obs_x) return fail(c, SFWI_EINVAL, "NULL argument");
+    if (id <= 0) return fail(c, SFWI_EINVAL, "obs_id must be positive (got %d)", id);
+    if (ntr < 1 || ntr > c->p.ntr) return fail(c, SFWI_EINVAL, "ntr=%d outside [1,%d]", ntr, c->p.ntr);
+    ObsSet &o = c->obs[id];
+    if (o.base && o.ntr != ntr) {   // a different size: reallocate
+        HIPCK(c, hipStreamSynchronize(c->stream));
+        hipFree(o.base);
+        o.base = nullptr;
+    }
+    const size_t rb = (size_t)ntr * c->p.nt * sizeof(float);
+    if (!o.base) {
+        int r = dalloc(c, (void **)&o.base, 2 * rb);
+        if (r) {
+            c->obs.erase(id);
+            return r;
+        }
+    }
+    o.ntr = ntr;
+    HIPCK(c, hipMemcpyAsync(o.base, obs_z, rb, hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipMemcpyAsync(o.base + (size_t)ntr * c->p.nt, obs_x, rb, hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int sfwi_obs_clear(sfwi_ctx *c) {
+    if (!c) return fail(c, SFWI_EINVAL, "NULL context");
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    for (auto &kv : c->obs) hipFree(kv.second.base);
+    c->obs.clear();
     return 0;
 }
 
@@ -1171,15 +1259,13 @@ int sfwi_batch_linesearch_parabola(sfwi_ctx *c, int nshots, const sfwi_shot *sho
 int sfwi_linesearch_linear(sfwi_ctx *c, const sfwi_shot *shot, const float *cg_vp, const float *cg_vs,
                            double *fenzi, double *fenmu) {
     if (!c || !shot || !cg_vp || !cg_vs || !fenzi || !fenmu) return fail(c, SFWI_EINVAL, "NULL argument");
-    if (!shot->obs_z || !shot->obs_x) return fail(c, SFWI_EINVAL, "observed traces missing");
+    if ((!shot->obs_z || !shot->obs_x) && shot->obs_id <= 0) return fail(c, SFWI_EINVAL, "observed traces missing");
     // Solve 1: residual r = born(dm) - obs + background.
     int r = prepare(c, K_BORN_FWD, 1, shot, nullptr);
     if (r || (r = run(c, K_BORN_FWD, 1))) return r;
     Slot &S = c->slots[0];
+    if ((r = load_obs(c, *shot, S))) return r;   // slot 0 now holds this shot's records
     const size_t rb = (size_t)shot->ntr * c->p.nt * sizeof(float);
-    HIPCK(c, hipMemcpyAsync(S.oz, shot->obs_z, rb, hipMemcpyHostToDevice, c->stream));
-    HIPCK(c, hipMemcpyAsync(S.ox, shot->obs_x, rb, hipMemcpyHostToDevice, c->stream));
-    S.obs_ntr = shot->ntr;   // slot 0 now holds this shot's records
     hipLaunchKernelGGL(k_residual<true>, dim3(256, 1), dim3(BLK), 0, c->stream, c->g, c->d_shots);
     HIPCK(c, hipMemcpyAsync(c->res_keep, S.rz, rb, hipMemcpyDeviceToDevice, c->stream));
     HIPCK(c, hipMemcpyAsync(c->res_keep + c->nrec, S.rx, rb, hipMemcpyDeviceToDevice, c->stream));
@@ -1223,7 +1309,8 @@ int linesearch_linear_batch(sfwi_ctx *c, int nshots, const sfwi_shot *shots, con
     if (!c->have_pert) return fail(c, SFWI_ESTATE, "perturbation not set (sfwi_set_perturbation)");
     if (!res_z != !res_x) return fail(c, SFWI_EINVAL, "res_z and res_x must both be given or both be NULL");
     for (int s = 0; s < nshots && !res_z; ++s)
-        if (!shots[s].obs_z || !shots[s].obs_x) return fail(c, SFWI_EINVAL, "shot %d: observed traces missing", s);
+        if ((!shots[s].obs_z || !shots[s].obs_x) && shots[s].obs_id <= 0)
+            return fail(c, SFWI_EINVAL, "shot %d: observed traces missing", s);
     if (nshots == 0) return 0;
     const int nb = std::min(c->p.max_batch, nshots);
     // residuals of a batch ([2 nb][nrec]: z, x per slot) and per-trace sums ([2 nb][ntr])
@@ -1261,14 +1348,8 @@ int linesearch_linear_batch(sfwi_ctx *c, int nshots, const sfwi_shot *shots, con
         // Solve 1 over the batch: residual r = born(dm) - obs + background, kept per slot.
         if (!res_z && ((r = prepare(c, K_BORN_FWD, n, shots + b0, nullptr)) || (r = run(c, K_BORN_FWD, n))))
             return done(r);
-        for (int k = 0; k < n && !res_z; ++k) {
-            Slot &S = c->slots[k];
-            const size_t rb = (size_t)shots[b0 + k].ntr * c->p.nt * sizeof(float);
-            if (hipMemcpyAsync(S.oz, shots[b0 + k].obs_z, rb, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-                hipMemcpyAsync(S.ox, shots[b0 + k].obs_x, rb, hipMemcpyHostToDevice, c->stream) != hipSuccess)
-                return done(fail(c, SFWI_EHIP, "copy failed"));
-            S.obs_ntr = shots[b0 + k].ntr;
-        }
+        for (int k = 0; k < n && !res_z; ++k)
+            if ((r = load_obs(c, shots[b0 + k], c->slots[k]))) return done(r);
         if (!res_z) hipLaunchKernelGGL(k_residual<true>, dim3(256, n), dim3(BLK), 0, c->stream, c->g, c->d_shots);
         for (int k = 0; k < n && !res_z; ++k) {
             Slot &S = c->slots[k];

@@ -219,6 +219,7 @@ int sfwi_shot_from_table(const int row[9], float dx, int pml, int sy, int gy, in
     out->igz = pml + gy;
     out->ngx_left = ngx_left;
     out->ntr = row[1];
+    out->obs_id = 0;
     if (nx_shot) *nx_shot = nx_s;
     return 0;
 }

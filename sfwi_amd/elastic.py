@@ -22,7 +22,7 @@ def _f32(a):
 class Shot:
     """One shot's geometry and observed traces (host arrays kept alive here)."""
 
-    def __init__(self, left, isx, isz, igz, ngx_left, gc, obs_z=None, obs_x=None):
+    def __init__(self, left, isx, isz, igz, ngx_left, gc, obs_z=None, obs_x=None, obs_id=0):
         self.gc = np.ascontiguousarray(gc, dtype=np.int32)
         self.obs_z = None if obs_z is None else _f32(obs_z)
         self.obs_x = None if obs_x is None else _f32(obs_x)
@@ -33,6 +33,7 @@ class Shot:
         fp = ctypes.POINTER(ctypes.c_float)
         self.c.obs_z = self.obs_z.ctypes.data_as(fp) if self.obs_z is not None else fp()
         self.c.obs_x = self.obs_x.ctypes.data_as(fp) if self.obs_x is not None else fp()
+        self.c.obs_id = int(obs_id)
 
     @property
     def ntr(self):
@@ -112,6 +113,15 @@ class Context:
         self._vp, self._vs = _f32(vp), _f32(vs)
         self._rho = None if rho is None else _f32(rho)
         self._ck(self.L.sfwi_set_model(self.h, _lib.ptr(self._vp), _lib.ptr(self._vs), _lib.ptr(self._rho)))
+
+    def obs_put(self, obs_id, obs_z, obs_x):
+        """Keep a shot's observed traces on the device under obs_id > 0."""
+        z, x = _f32(obs_z), _f32(obs_x)
+        assert z.shape == x.shape and z.shape[-1] == self.nt
+        self._ck(self.L.sfwi_obs_put(self.h, int(obs_id), int(z.size // self.nt), z, x))
+
+    def obs_clear(self):
+        self._ck(self.L.sfwi_obs_clear(self.h))
 
     def set_perturbation(self, dvp, dvs):
         self._ck(self.L.sfwi_set_perturbation(self.h, _f32(dvp), _f32(dvs)))

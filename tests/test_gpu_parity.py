@@ -318,6 +318,47 @@ def test_march_kernels_match_per_cell(cols, depth, order):
         assert np.array_equal(x, y), "max |d| = %g" % np.abs(x - y).max()
 
 
+@pytest.mark.parametrize("cols,depth,order", [(8, 4, 3), (8, 1, 2), (16, 2, 1), (37, 4, 3), (8, 0, 0)])
+def test_march_kernels_match_per_cell_multiband(cols, depth, order):
+    """As above on a grid tall enough for several row bands per column strip:
+    nz = 134 with an 8-cell PML gives a 142-row update domain, three 56-row
+    march bands of which the last holds 30 rows (and three interior bands of
+    56, 56 and 22 rows for the snapshot imaging march), so the seams between
+    bands -- halo lanes, band offsets, a partial last band -- are compared.
+    Strip gradients, Born traces and gradients, and snapshot gradients (BF16)."""
+    sv = W.tiny(nx=44, nz=134, pml=8, nt=200, seed=23)
+    dvp = (sv.vp - sv.vp0).astype(np.float32)
+    dvs = (sv.vs - sv.vs0).astype(np.float32)
+    w = W.ricker(sv.nt, sv.dt, sv.f0)
+    shots = []
+    for sh in sv.shots:
+        oz, ox = _observed(sv, sh, w)
+        shots.append(lib_shot(sv, sh, oz, ox))
+    out = {}
+    for mc in (-1, cols):
+        with context(sv, "fast", max_batch=2, march_cols=mc, march_depth=depth, march_order=order) as ctx:
+            ctx.set_wavelet(w)
+            ctx.set_model(sv.vp0, sv.vs0, sv.rho0)
+            m = ctx.batch_gradient(shots)
+            g = ctx.flush()
+            ctx.set_perturbation(dvp, dvs)
+            b = ctx.born_forward(shots[0])
+            mb = ctx.batch_gradient(shots, born=True)
+            gb = ctx.flush()
+        with context(sv, "fast", max_batch=2, march_cols=mc, march_depth=depth, march_order=order,
+                     recon="snapshots", snap_every=3, storage="bf16") as ctx:
+            ctx.set_wavelet(w)
+            ctx.set_model(sv.vp0, sv.vs0, sv.rho0)
+            ms = ctx.batch_gradient(shots)
+            gs = ctx.flush()
+        out[mc] = (m, g, b, mb, gb, ms, gs)
+    a, b = out[-1], out[cols]
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[3], b[3]) and np.array_equal(a[5], b[5])
+    assert np.abs(a[1][0]).max() > 0 and np.abs(a[6][0]).max() > 0
+    for x, y in zip(a[1] + a[2] + a[4] + a[6], b[1] + b[2] + b[4] + b[6]):
+        assert np.array_equal(x, y), "max |d| = %g" % np.abs(x - y).max()
+
+
 def test_resident_observed_records():
     """Shots with NULL observed traces reuse the records left in their batch
     slot by the previous call: same misfits and gradients, bitwise."""
