@@ -44,7 +44,12 @@ HBM_PEAK_GBS = 8000.0
 # benchmark configuration's kernels, per shot of the launch; see
 # profiles/mixed_precision_snapshot_hbm_traffic.md.  Keyed by
 # (workload, recon, storage, snap_every).
-MEASURED_HBM_BYTES_PER_SHOT = {}
+MEASURED_HBM_BYTES_PER_SHOT = {
+    ("mixed_precision", "snapshots", "bf16", 4): {
+        "k_vel(forward)": 35341303, "k_str(forward)": 46382541, "k_adj_str": 34575718,
+        "k_adj_vel": 48396983, "k_snap_image": 30830349, "k_snap_store": 16727339,
+    },
+}
 
 NOP, NSTRIP = 4, 4   # stencil half-width; strip half-width (split band = PML + NSTRIP)
 
