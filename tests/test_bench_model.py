@@ -1,8 +1,11 @@
-"""The benchmark's algorithmic-byte model (bench.alg_bytes) on the CPU: its
-cell classes equal a brute-force count over the padded grid of the split-band
-rule the kernels use (a cell of the update domain [4, n - 4) is split when it
-lies within pml + 4 of an edge), and each kernel's bytes add up from those
-counts.  No GPU needed."""
+"""The benchmark's byte models on the CPU.  The kernels' own cell-class model
+(bench.alg_bytes): its cell classes equal a brute-force count over the padded
+grid of the split-band rule the kernels use (a cell of the update domain
+[4, n - 4) is split when it lies within pml + 4 of an edge), and each
+kernel's bytes add up from those counts.  The scope-table model
+(bench.alg_bytes_8d): the two half steps of a propagation add up to 52 B per
+interior cell and 100 B per PML cell, and each kernel of the measured
+benchmark path has a figure.  No GPU needed."""
 import numpy as np
 import pytest
 
@@ -32,6 +35,19 @@ def test_kernel_bytes_from_cell_classes():
     assert bench.alg_bytes("k_str(reconstruct+image)", 401, 301, 30) == 40 * un + 84 * sp + 28 * im
     strips = 2 * 4 * 4 * (401 + 301) * 4          # vx, vz of 16 (nx + nz) strip cells, FP32
     assert bench.alg_bytes("k_vel(forward)", 401, 301, 30) == 32 * un + 64 * sp + strips
-    # every kernel of the measured path has a traffic figure
-    for k in bench.MEASURED_HBM_BYTES_PER_SHOT:
-        assert bench.alg_bytes(k, 401, 301, 30) > 0
+
+
+def test_scope_table_bytes():
+    nx, nz, pml = 1200, 600, 40
+    inter = nx * nz
+    pmlc = (nx + 2 * pml) * (nz + 2 * pml) - inter
+    v = bench.alg_bytes_8d("k_vel(forward)", nx, nz, pml)
+    s = bench.alg_bytes_8d("k_str(forward)", nx, nz, pml)
+    assert v + s == 52 * inter + 100 * pmlc
+    assert bench.alg_bytes_8d("k_adj_vel", nx, nz, pml) == v
+    assert bench.alg_bytes_8d("k_snap_image", nx, nz, pml, 2) == 40 * inter
+    assert bench.alg_bytes_8d("k_snap_store", nx, nz, pml, 2) == 12 * (nx + 8) * (nz + 8)
+    # every kernel of the measured path has a traffic figure and a byte figure
+    for key, table in bench.MEASURED_HBM_BYTES_PER_SHOT.items():
+        for k in table:
+            assert bench.alg_bytes_8d(k, nx, nz, pml) > 0

@@ -2,13 +2,16 @@
 oracle at that size:
 
   * the 401 x 301 shot batch (nt 3000, 30-cell PML), 3 of its shots: the FAST
-    batched column-march path against the EXACT path (the reference's
-    operation order, bit-identical to the oracle at small sizes) within the
-    FAST bar (gradients and misfit 1e-4), and a batch equal to its shots run
-    one at a time, bit for bit;
-  * the 1200 x 600 mixed-precision grid (nt 6000, 40-cell PML), 2 shots: the
-    16-bit boundary-strip storage within the accuracy bar (gradient 1e-3) of
-    the FP32 strips, which are within 2e-6 of FP64 on this grid;
+    batched path with the column-march kernels (forced: 3 shots are below the
+    automatic family switch, which would pick the per-cell kernels) against
+    the EXACT path (the reference's operation order, bit-identical to the
+    oracle at small sizes) within the FAST bar (gradients and misfit 1e-4),
+    the per-cell FAST kernels equal to the column march bit for bit, and a
+    batch equal to its shots run one at a time, bit for bit;
+  * the 1200 x 600 mixed-precision grid (nt 6000, 40-cell PML), 2 shots, with
+    the column-march kernels: the 16-bit boundary-strip storage within the
+    accuracy bar (gradient 1e-3) of the FP32 strips, which are within 2e-6 of
+    FP64 on this grid;
   * the 401 x 301 batch with each optimal-transport misfit: the gradient
     completes with finite images and its per-shot misfits equal the
     standalone OT misfit of the same traces;
@@ -51,8 +54,8 @@ def test_shot_batch_full_size():
     w = W.ricker(sv.nt, sv.dt, sv.f0)
     shots = _shots(sv, _records(sv, w))
     res = {}
-    for mode in ("exact", "fast"):
-        with _ctx(sv, 3, mode=mode) as ctx:
+    for mode, cols in (("exact", 0), ("fast", 8), ("percell", -1)):
+        with _ctx(sv, 3, mode="exact" if mode == "exact" else "fast", march_cols=cols) as ctx:
             ctx.set_wavelet(w)
             ctx.set_model(sv.vp0, sv.vs0, sv.rho0)
             m = ctx.batch_gradient(shots)
@@ -72,6 +75,9 @@ def test_shot_batch_full_size():
     for a, b in zip(gf, ge):
         assert rel_l2(a, b) <= 1e-4
     assert np.all(np.abs(mf - me) <= 1e-4 * np.abs(me))
+    assert np.array_equal(res["percell"][0], mf)
+    for a, b in zip(res["percell"][1], gf):
+        assert np.array_equal(a, b)
 
 
 def test_mixed_precision_full_size_strip_storage():
@@ -80,7 +86,7 @@ def test_mixed_precision_full_size_strip_storage():
     shots = _shots(sv, _records(sv, w))
     out = {}
     for st in ("f32", "f16", "bf16"):
-        with _ctx(sv, 2, storage=st) as ctx:
+        with _ctx(sv, 2, storage=st, march_cols=8) as ctx:
             ctx.set_wavelet(w)
             ctx.set_model(sv.vp0, sv.vs0, sv.rho0)
             m = ctx.batch_gradient(shots)
