@@ -37,6 +37,7 @@ class OrConfig(ctypes.Structure):
         ("rbc", ctypes.c_int),
         ("rbc_seed", ctypes.c_uint),
         ("snap_every", ctypes.c_int),
+        ("plant", ctypes.c_int),
     ]
 
 
@@ -71,6 +72,8 @@ def lib():
         _nf32 = ctypes.c_void_p   # nullable float arrays
         L.or_gradient_snapshot_f64.argtypes = [cfg, _f32p, _f32p, _f32p, _nf32, _nf32, _f32p, _i32p, _f32p, _f32p,
                                                _nf32, _nf32, _f64p, _f64p, _f64p, dp]
+        L.or_reconstruct_f64.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, ctypes.c_void_p, ctypes.c_int,
+                                         ctypes.c_void_p, _f64p]
         L.or_born_forward_f32.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _i32p,
                                           _f32p, _f32p, _f32p, _f32p]
         L.or_born_forward_f64.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _i32p,
@@ -194,6 +197,19 @@ def gradient_snapshot(c, vp, vs, rho, wav, gc, obs_z, obs_x, dvp=None, dvs=None,
                                    *g, ctypes.byref(m))
     sh = (c.allnx, c.allnz)
     return g[0].reshape(sh), g[1].reshape(sh), g[2].reshape(sh), m.value
+
+
+def reconstruct(c, vp, vs, rho, wav, kind, init=None, errs=False):
+    """The faithful gradient's forward loop and reverse reconstruction (test
+    device, FP64): (state after reverse step 0 [15, nxpml, nzpml], errs or None)."""
+    nxp, nzp = c.nx + 2 * c.pml, c.nz + 2 * c.pml
+    out = np.zeros(15 * nxp * nzp, np.float64)
+    e = np.zeros(c.nt, np.float64) if errs else None
+    ini = None if init is None else np.ascontiguousarray(init, np.float64)
+    lib().or_reconstruct_f64(ctypes.byref(c), _f32(vp), _f32(vs), _f32(rho), _f32(wav),
+                             None if ini is None else ini.ctypes.data, int(kind),
+                             None if e is None else e.ctypes.data, out)
+    return out.reshape(15, nxp, nzp), e
 
 
 def gradient_res(c, vp, vs, rho, wav, gc, res_z, res_x):

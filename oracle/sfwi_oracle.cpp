@@ -19,9 +19,11 @@
 
 #include "sfwi_oracle.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -447,9 +449,12 @@ std::vector<int> receiver_columns(const or_config *c, const int *gc) {
 // sfwi-stage1/FD2DGPU.cu:3842-3869 (save) and :3873-3899 (restore).
 template <typename T> struct Strips {
     int nx, nz, pml, nxp, nzp, N;
+    int su = 0, sl = 0;                     // test hook (or_config.plant): up / left windows moved by a cell
     std::vector<T> up, down, left, right;   // per field: [field][it][...]
-    void init(const Grid &g) {
+    void init(const Grid &g, int plant = 0) {
         nx = g.nx; nz = g.nz; pml = g.pml; nxp = g.nxp; nzp = g.nzp; N = NOP;
+        su = (plant & 1) ? 1 : 0;
+        sl = (plant & 2) ? 1 : 0;
         long a = 2L * (long)g.nt * 2 * N * nx, b = 2L * (long)g.nt * 2 * N * nz;
         up.assign(a, T(0)); down.assign(a, T(0)); left.assign(b, T(0)); right.assign(b, T(0));
     }
@@ -459,14 +464,14 @@ template <typename T> struct Strips {
             long ou = (long)k * nt * 2 * N * nx + (long)it * nx * 2 * N;
             long ol = (long)k * nt * 2 * N * nz + (long)it * 2 * N * nz;
             for (int ix = pml; ix < nxp - pml; ++ix) {
-                for (int iz = pml - N; iz < pml + N; ++iz)
-                    sw<SAVE>(f[k][(long)ix * nzp + iz], up[ou + (long)(ix - pml) * 2 * N + (iz - (pml - N))]);
+                for (int iz = pml - N + su; iz < pml + N + su; ++iz)
+                    sw<SAVE>(f[k][(long)ix * nzp + iz], up[ou + (long)(ix - pml) * 2 * N + (iz - (pml - N + su))]);
                 for (int iz = nzp - pml - N; iz < nzp - pml + N; ++iz)
                     sw<SAVE>(f[k][(long)ix * nzp + iz], down[ou + (long)(ix - pml) * 2 * N + (iz - (nzp - pml - N))]);
             }
-            for (int ix = pml - N; ix < pml + N; ++ix)
+            for (int ix = pml - N + sl; ix < pml + N + sl; ++ix)
                 for (int iz = pml; iz < nzp - pml; ++iz)
-                    sw<SAVE>(f[k][(long)ix * nzp + iz], left[ol + (long)(ix - (pml - N)) * nz + (iz - pml)]);
+                    sw<SAVE>(f[k][(long)ix * nzp + iz], left[ol + (long)(ix - (pml - N + sl)) * nz + (iz - pml)]);
             for (int ix = nxp - pml - N; ix < nxp - pml + N; ++ix)
                 for (int iz = pml; iz < nzp - pml; ++iz)
                     sw<SAVE>(f[k][(long)ix * nzp + iz], right[ol + (long)(ix - (nxp - pml - N)) * nz + (iz - pml)]);
@@ -547,6 +552,7 @@ template <typename T> struct Run {
         if (gc) igx = receiver_columns(cfg, gc);
     }
     long at(int ix, int iz) const { return (long)ix * g.nzp + iz; }
+    const float *wav = nullptr;   // the forward solve's wavelet
     // Explosive source on txx and tzz before the velocity update,
     // sfwi-stage1/FD2DGPU.cu:2104-2114.
     void add_source(Wave<T> &w, const float *wav, int it) {
@@ -574,18 +580,56 @@ template <typename T> struct Run {
 
     // Background forward modeling, born_modeling mode 0
     // (sfwi-stage1/FD2DGPU.cu:1864-1874) inside the loop :4535-4564.
-    void forward(const float *wav, std::vector<T> &rz, std::vector<T> &rx, Wave<T> *keep = nullptr,
-                 Strips<T> *strips = nullptr) {
+    // init: starting state (zero when NULL); before: called with each step's
+    // index and the state before it (test hooks).  or_config.plant & 4 (test
+    // hook) adds the source after the velocity update instead.
+    void forward(const float *wav_, std::vector<T> &rz, std::vector<T> &rx, Wave<T> *keep = nullptr,
+                 Strips<T> *strips = nullptr, const Wave<T> *init = nullptr,
+                 const std::function<void(int, const Wave<T> &)> *before = nullptr) {
+        wav = wav_;
         Wave<T> w(g.n);
+        if (init) w = *init;
         if (c->rbc == 1) strips = nullptr;   // random boundary: nothing saved (FD2DGPU.cu:4543)
+        const bool late = (c->plant & 4) != 0;
         for (int it = 0; it < g.nt; ++it) {
-            add_source(w, wav, it);
+            if (before) (*before)(it, w);
+            if (!late) add_source(w, wav, it);
             update_v(g, st, m, w, +1, nth);
+            if (late) add_source(w, wav, it);
             update_s(g, st, m, w, +1, nth);
-            record(w, it, rz, rx);
+            if (!igx.empty()) record(w, it, rz, rx);
             if (strips) strips->template move<true>(w.vx.data(), w.vz.data(), it, g.nt);
         }
         if (keep) *keep = w;
+    }
+
+    // One reverse step of the forward field, born_modeling mode 6 inside
+    // sfwi-stage1/FD2DGPU.cu:4600-4640: the strips of step it restored (not
+    // with the random boundary), the stress step with the velocity
+    // derivatives, the velocity step, direction -1; the source is not
+    // subtracted (quirk Q5).  The forward step adds its source to the stress
+    // totals only, and the stress step rebuilds the totals from the split
+    // parts, so the source acts on that step's velocity update alone; the
+    // exact inverse therefore lets the reverse velocity step see the source in
+    // the totals again.  src_sign (test hooks): +1 does that (the exact
+    // inverse), -1 lets it see the negated source (the difference of the
+    // faithful and the exact reconstructions, by linearity); or_config.plant
+    // & 8 makes the gradient's reverse loop use the exact inverse.  ovx, ovz
+    // receive v(it+1) before the velocity step.
+    void reconstruct(Wave<T> &fw, Strips<T> *strips, int it, T *gx = nullptr, T *gz = nullptr,
+                     T *gxz = nullptr, T *gzx = nullptr, std::vector<T> *ovx = nullptr,
+                     std::vector<T> *ovz = nullptr, int src_sign = 0) {
+        if (strips && c->rbc != 1) strips->template move<false>(fw.vx.data(), fw.vz.data(), it, g.nt);
+        update_s(g, st, m, fw, -1, nth, gx, gz, gxz, gzx);
+        if (ovx) {
+            *ovx = fw.vx;
+            *ovz = fw.vz;
+        }
+        const long i = at(c->isx, c->isz);
+        const T s = (src_sign && wav) ? (T)src_sign * (T)wav[it] : T(0);
+        if (s != T(0)) { fw.txx[i] += s; fw.tzz[i] += s; }
+        update_v(g, st, m, fw, -1, nth);
+        if (s != T(0)) { fw.txx[i] -= s; fw.tzz[i] -= s; }
     }
 
     // Stage-2 forward, mode 1 (sfwi-stage2/FD2DGPU.cu:1814-1845) inside
@@ -621,13 +665,8 @@ template <typename T> struct Run {
         std::vector<T> vx_gx(g.n, T(0)), vz_gz(g.n, T(0)), vx_gz(g.n, T(0)), vz_gx(g.n, T(0));
         std::vector<T> ovx, ovz;
         for (int it = g.nt - 1; it >= 0; --it) {
-            if (c->rbc != 1) strips.template move<false>(fw.vx.data(), fw.vz.data(), it, g.nt);
-            update_s(g, st, m, fw, -1, nth, vx_gx.data(), vz_gz.data(), vx_gz.data(), vz_gx.data());
-            if (grho) {
-                ovx = fw.vx;
-                ovz = fw.vz;
-            }
-            update_v(g, st, m, fw, -1, nth);
+            reconstruct(fw, &strips, it, vx_gx.data(), vz_gz.data(), vx_gz.data(), vz_gx.data(),
+                        grho ? &ovx : nullptr, grho ? &ovz : nullptr, (c->plant & 8) ? +1 : 0);
             inject(bk, it, rz, rx);
             adjoint_s(g, st, m, bk, c->adjoint_dir, nth);
             adjoint_v(g, st, m, bk, c->adjoint_dir, nth);
@@ -644,7 +683,7 @@ void gradient_impl(const or_config *c, const float *vp, const float *vs, const f
     const long nr = (long)c->ntr * c->nt;
     std::vector<T> sz(nr), sx(nr);
     Strips<T> strips;
-    strips.init(r.g);
+    strips.init(r.g, c->plant);
     Wave<T> fw(1);
     r.forward(wav, sz, sx, &fw, &strips);
     // r = 1.0*syn - obs  (sfwi-stage1/FD2DGPU.cu:3181-3195 with alpha = 1)
@@ -668,7 +707,7 @@ void gradient_res_impl(const or_config *c, const float *vp, const float *vs, con
     const long nr = (long)c->ntr * c->nt;
     std::vector<T> sz(nr), sx(nr);
     Strips<T> strips;
-    strips.init(r.g);
+    strips.init(r.g, c->plant);
     Wave<T> fw(1);
     r.forward(wav, sz, sx, &fw, &strips);
     for (long k = 0; k < nr; ++k) {
@@ -686,7 +725,7 @@ void born_gradient_impl(const or_config *c, const float *vp, const float *vs, co
     const long nr = (long)c->ntr * c->nt;
     std::vector<T> bz(nr), bx(nr), gz(nr), gx(nr);
     Strips<T> strips;
-    strips.init(r.g);
+    strips.init(r.g, c->plant);
     Wave<T> fw(1);
     r.born_forward(wav, bz, bx, gz, gx, &fw, &strips);
     // r = (b - obs) then r = 1*r + background: sfwi-stage2/FD2DGPU.cu:3119-3130,
@@ -818,6 +857,84 @@ static void gradient_consistent_impl(const or_config *c, const float *vp, const 
         grho[q] = r.st.dt * irho[q];
     }
 }
+
+// Test device (no reference counterpart) for the faithful gradient's
+// orchestration: its forward loop (Run::forward, strips saved) from `init`
+// (zero when NULL), then its reverse reconstruction steps (Run::reconstruct)
+// with no recording and no adjoint.  kind 0: as the gradient (the source
+// left out of the reverse velocity steps, quirk Q5); 1: the exact inverse
+// (each reverse velocity step sees its step's source, as the forward one did);
+// 2: their difference by linearity -- from a zero state with zero strips,
+// each reverse velocity step seeing the negated source.  errs[it] (may be NULL): the largest difference
+// between the reconstructed totals after reverse step it and the forward
+// totals before forward step it -- velocities on the cells at least pml + 4
+// from every edge, stresses on the imaged interior (at least pml from every
+// edge) -- divided by the largest forward total there.  out: the state after
+// reverse step 0 (15 arrays of nxpml*nzpml).
+template <typename T>
+static void reconstruct_impl(const or_config *c, const float *vp, const float *vs, const float *rho,
+                             const float *wav, const double *init, int kind, double *errs, double *out) {
+    Run<T> r(c, vp, vs, rho, nullptr, nullptr, nullptr);
+    const Grid &g = r.g;
+    const long n = g.n;
+    Strips<T> strips;
+    strips.init(g, c->plant);
+    Wave<T> w(n);
+    // checked cells: velocities at least pml + 4 from every edge, stresses on
+    // the imaged interior (at least pml from every edge)
+    auto inside = [&](long i, int f) {
+        const int lo = f < 2 ? g.pml + NOP : g.pml;
+        const int ix = (int)(i / g.nzp), iz = (int)(i % g.nzp);
+        return ix >= lo && ix < g.nxp - lo && iz >= lo && iz < g.nzp - lo;
+    };
+    std::vector<std::vector<T>> keep;
+    if (kind != 2) {
+        Wave<T> w0(n);
+        if (init) {
+            std::vector<std::vector<T> *> a = w0.all();
+            for (int k = 0; k < 15; ++k)
+                for (long i = 0; i < n; ++i) (*a[k])[i] = (T)init[k * n + i];
+        }
+        if (errs) keep.resize(g.nt);
+        std::function<void(int, const Wave<T> &)> hook = [&](int it, const Wave<T> &x) {
+            std::vector<T> &k = keep[it];
+            k.resize(5 * n);
+            const std::vector<T> *t[5] = {&x.vx, &x.vz, &x.txx, &x.tzz, &x.txz};
+            for (int f = 0; f < 5; ++f) std::copy(t[f]->begin(), t[f]->end(), k.begin() + f * n);
+        };
+        std::vector<T> rz, rx;
+        r.forward(wav, rz, rx, &w, &strips, init ? &w0 : nullptr, errs ? &hook : nullptr);
+    } else {
+        r.wav = wav;
+    }
+    T scale = 0;
+    if (errs)
+        for (const std::vector<T> &k : keep)
+            for (long q = 0; q < 5 * n; ++q)
+                if (inside(q % n, (int)(q / n))) scale = std::max(scale, std::abs(k[q]));
+    const int sign = kind == 1 ? +1 : kind == 2 ? -1 : ((c->plant & 8) ? +1 : 0);
+    for (int it = g.nt - 1; it >= 0; --it) {
+        r.reconstruct(w, &strips, it, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, sign);
+        if (errs) {
+            const std::vector<T> *t[5] = {&w.vx, &w.vz, &w.txx, &w.tzz, &w.txz};
+            T e = 0;
+            for (int f = 0; f < 5; ++f)
+                for (long i = 0; i < n; ++i)
+                    if (inside(i, f)) e = std::max(e, std::abs((*t[f])[i] - keep[it][f * n + i]));
+            errs[it] = scale > 0 ? (double)(e / scale) : (double)e;
+        }
+    }
+    std::vector<std::vector<T> *> a = w.all();
+    for (int k = 0; k < 15; ++k)
+        for (long i = 0; i < n; ++i) out[k * n + i] = (double)(*a[k])[i];
+}
+
+extern "C" {
+void or_reconstruct_f64(const or_config *c, const float *vp, const float *vs, const float *rho, const float *wav,
+                        const double *init, int kind, double *errs, double *out) {
+    reconstruct_impl<double>(c, vp, vs, rho, wav, init, kind, errs, out);
+}
+}  // extern "C"
 
 // Snapshot gradient (no reference counterpart; the oracle of the library's
 // recon = snapshots mode).  The consistent gradient above with the imaging

@@ -23,10 +23,16 @@
  * misfit and line-search sums, the overwrite injection with the last of
  * coincident traces winning, and the imaging step (Vp, Vs and density terms)
  * through or_gradient_consistent_f64, whose images must equal central
- * differences of the misfit.  Parity unpinned (restatement of the cited
- * lines only): the rest of the orchestration of the faithful gradient (strip
- * restore with corners skipped, the reconstruction without source
- * subtraction, imaging after the adjoint stress-like half step).
+ * differences of the misfit.  The faithful gradient's orchestration
+ * (tests/test_oracle_orchestration.py, through or_reconstruct_f64 and the
+ * planted mistakes of or_config.plant): the strip save / restore geometry
+ * with corners skipped (the reconstruction returns the forward field on the
+ * imaged interior at every step, and a window one cell off fails), the source
+ * timing and scale (closed form of the first sample), the reconstruction
+ * without source subtraction (linearity against the exact inverse, which
+ * time reversal pins).  Parity unpinned (restatement of the cited lines
+ * only): the imaging position in the reverse loop (after the adjoint
+ * stress-like half step) and the stage-2 density override.
  *
  * Random boundary (rbc): the reference draws it from cuRAND states seeded with
  * time(0) (sfwi-stage1/FD2DGPU.cu:155-230, 3245-3305), so no run repeats it;
@@ -64,6 +70,11 @@ typedef struct or_config {
                           /*    (test hook)                                               */
     unsigned rbc_seed;    /* seed of the random boundary                                  */
     int snap_every;       /* snapshot gradient: image every K-th step (<= 1: every step)  */
+    int plant;            /* test hook, 0 in use: planted mistakes that the pins must     */
+                          /*   catch -- 1: the up strips one row deeper, 2: the left      */
+                          /*   strips one column further in, 4: the forward source after  */
+                          /*   the velocity update, 8: the reconstruction undoes the      */
+                          /*   source (the exact inverse)                                 */
 } or_config;
 
 /* FD coefficients c1..c4 of the 8th-order staggered stencil (index 0 unused). */
@@ -165,6 +176,20 @@ void or_linear_alpha_f32(const or_config *c, const float *vp, const float *vs, c
                          const float *dvp, const float *dvs, const float *cg_vp, const float *cg_vs,
                          const float *wav, const int *gc, const float *obs_z, const float *obs_x,
                          double *fenzi, double *fenmu);
+
+/* Test device for the faithful gradient's orchestration (FP64): its forward
+ * loop with strips saved, from init (15 arrays, NULL: zero), then its reverse
+ * reconstruction steps, no recording or adjoint.  kind 0: as the gradient
+ * (the source left out of the reverse steps); 1: the exact inverse (each
+ * reverse velocity step sees its step's source in the stress totals, as the
+ * forward one did); 2: the difference of 0 and 1 by linearity (zero state
+ * and zero strips, the negated source).  errs[nt] (NULL: skipped): largest difference of
+ * the totals after reverse step it from the forward totals before forward
+ * step it (velocities on cells at least pml + 4 from every edge, stresses on
+ * the imaged interior, at least pml from every edge), relative to the
+ * largest forward total there.  out: the state after reverse step 0. */
+void or_reconstruct_f64(const or_config *c, const float *vp, const float *vs, const float *rho, const float *wav,
+                        const double *init, int kind, double *errs, double *out);
 
 /* Test hooks (FP64): the four stencils of one field, and free propagation of
  * a given state.  state = 15 arrays of nxpml*nzpml in the order
