@@ -285,6 +285,18 @@ int sfwi_ot_misfit(sfwi_ctx *ctx, int variant, int ntr, int nt, float dt, const 
                    const float *obs_z, const float *syn_x, const float *syn_z, float *res_x,
                    float *res_z, double *misfit);
 
+/* sfwi_ot_misfit without any device allocation (no reference counterpart:
+ * for callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results, bit for bit; the workspace (staging, job table,
+ * FP64 merge-walk workspace) is one static device array per GPU, about
+ * 21 MB, held until the process ends.  Limits: ntr * nt <= 262144 samples per
+ * component and ntr <= 4096, else SFWI_EINVAL.  Calls that overlap on one GPU
+ * (from any context) share that array and race: serialize them. */
+int sfwi_ot_misfit_static(sfwi_ctx *ctx, int variant, int ntr, int nt, float dt, const float *obs_x,
+                          const float *obs_z, const float *syn_x, const float *syn_z, float *res_x,
+                          float *res_z, double *misfit);
+
 /* ---------------------------------------------------------------------------
  * Serving loop (no reference counterpart: the reference runs one shot per MPI
  * rank at a time).  Requests of any grid, record length and kind enter a FIFO

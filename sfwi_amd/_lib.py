@@ -120,6 +120,8 @@ def load():
     L.sfwi_flush_rho.argtypes = [vp, vp]
     L.sfwi_ot_misfit.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                                  _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, dp]
+    L.sfwi_ot_misfit_static.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                        _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, dp]
     L.sfwi_ricker.argtypes = [_f32p, ctypes.c_int, ctypes.c_float, ctypes.c_float]
     L.sfwi_ricker.restype = None
     L.sfwi_set_timing.argtypes = [vp, ctypes.c_int]

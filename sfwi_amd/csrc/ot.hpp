@@ -198,6 +198,38 @@ k_ot(const OtJob *__restrict__ jobs, int njobs, double *__restrict__ ws, size_t 
     }
 }
 
+// Job table of one gather on the device (the layout sfwi_ot_misfit builds on
+// the host): components x then z; per trace (variants 0, 1) or one job of the
+// whole flattened gather (variant 2); the adjoint source overwrites the
+// synthetic traces; job j's misfit goes to mis[j].
+__global__ void k_ot_jobs(OtJob *jobs, int variant, int ntr, int nt, float *syn_x, float *syn_z,
+                          const float *obs_x, const float *obs_z, double *mis) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int per = variant == 2 ? 1 : ntr;
+    if (j >= 2 * per) return;
+    const int comp = j / per, tr = j - comp * per;
+    float *syn = comp ? syn_z : syn_x;
+    const float *obs = comp ? obs_z : obs_x;
+    OtJob J;
+    if (variant == 2) {
+        J.syn = syn; J.obs = obs; J.n = ntr * nt; J.power = 0;
+    } else {
+        const size_t o = (size_t)tr * nt;
+        J.syn = syn + o; J.obs = obs + o; J.n = nt; J.power = variant == 1;
+    }
+    J.res = const_cast<float *>(J.syn);
+    J.misfit = mis + j;
+    jobs[j] = J;
+}
+
+// Sum of n values in index order by one thread (the order of a host loop).
+__global__ void k_sum_in_order(const double *v, int n, double *out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double s = 0.0;
+    for (int k = 0; k < n; ++k) s += v[k];
+    *out = s;
+}
+
 // Stage-2 synthetic traces for the OT misfit: born + background, in place
 // (the L2 residual is born - obs + background, sfwi-stage2/FD2DGPU.cu:3119-3158).
 __global__ void k_born_total(Geo g, const DevShot *__restrict__ shots) {

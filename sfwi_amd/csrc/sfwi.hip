@@ -17,12 +17,39 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
 using namespace sfwi;
 
+// --------------------------------------------- static OT workspace ---
+// The workspace of sfwi_ot_misfit_static, which allocates nothing: one
+// file-scope device array per GPU, held until the process ends.  Layout:
+// [flag int, total double] | job table | per-job misfits | staging of the four
+// gathers | FP64 merge-walk workspace.  Limits: at most NA_MAX_SAMPLES samples
+// per component (ntr * nt) and NA_MAX_TRACES traces.
 namespace {
+constexpr size_t NA_MAX_SAMPLES = size_t(1) << 18;
+constexpr int NA_MAX_TRACES = 4096;
+constexpr size_t na_align(size_t x) { return (x + 255) & ~size_t(255); }
+constexpr size_t NA_JOBS = 256;
+constexpr size_t NA_MIS = NA_JOBS + na_align(2 * NA_MAX_TRACES * sizeof(OtJob));
+constexpr size_t NA_IO = NA_MIS + na_align(2 * NA_MAX_TRACES * sizeof(double));
+constexpr size_t NA_WS = NA_IO + na_align(4 * NA_MAX_SAMPLES * sizeof(float));
+constexpr size_t NA_WS_BYTES = na_align(2 * 4 * (NA_MAX_SAMPLES + 1) * sizeof(double));
+constexpr size_t NA_BYTES = NA_WS + NA_WS_BYTES;
+}  // namespace
+
+static __device__ __align__(256) char scratch[NA_BYTES];
+
+namespace {
+
+// Device address of `scratch` per GPU, from hipGetSymbolAddress with that GPU
+// current (the array's name is no device address in host code).
+std::mutex g_scratch_m;
+constexpr int NA_MAX_DEVICES = 64;
+char *g_scratch[NA_MAX_DEVICES] = {};
 
 // Message of the last failed call without a context (create, server calls):
 // one per host thread, so concurrent callers never share or clobber it.
@@ -128,6 +155,20 @@ int fail(sfwi_ctx *c, int code, const char *fmt, ...) {
         hipError_t e_ = (expr);                                                             \
         if (e_ != hipSuccess) return fail(c, SFWI_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
+
+// Resolve the static workspace's address on `dev` (once per device; the
+// caller has made `dev` current).
+int scratch_init(sfwi_ctx *c, int dev) {
+    if (dev < 0 || dev >= NA_MAX_DEVICES) return fail(c, SFWI_EINVAL, "device %d beyond the static workspace table", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_m);
+    if (g_scratch[dev]) return 0;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(c, SFWI_EHIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch[dev] = (char *)p;
+    return 0;
+}
 
 int dalloc(sfwi_ctx *c, void **p, size_t bytes) {
     size_t fr = 0, tot = 0;
@@ -991,6 +1032,7 @@ int sfwi_ctx_create(const sfwi_params *p, sfwi_ctx **out) {
     auto bail = [&](int code) { g_create_error = c->err; sfwi_ctx_destroy(c); return code; };
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(c, SFWI_EHIP, "hipStreamCreate failed"));
+    if (int rs = scratch_init(c, dev)) return bail(rs);   // this device is current
     if (c->ring < p->nt) {
         if (hipStreamCreateWithFlags(&c->cstream, hipStreamNonBlocking) != hipSuccess)
             return bail(fail(c, SFWI_EHIP, "hipStreamCreate failed (strip copies)"));
@@ -1576,6 +1618,59 @@ int sfwi_ot_misfit(sfwi_ctx *c, int variant, int ntr, int nt, float dt, const fl
     if ((r = check_ot(c))) return r;
     double total = 0.0;
     for (double v : mis) total += v;
+    *misfit += total;
+    return 0;
+}
+
+int sfwi_ot_misfit_static(sfwi_ctx *c, int variant, int ntr, int nt, float dt, const float *obs_x,
+                          const float *obs_z, const float *syn_x, const float *syn_z, float *res_x, float *res_z,
+                          double *misfit) {
+    if (!c) return fail(c, SFWI_EINVAL, "NULL context");
+    if (!obs_x || !obs_z || !syn_x || !syn_z || !res_x || !res_z || !misfit)
+        return fail(c, SFWI_EINVAL, "NULL argument");
+    if (variant < 0 || variant > 2) return fail(c, SFWI_EINVAL, "unknown OT variant %d", variant);
+    if (ntr < 1 || nt < 1) return fail(c, SFWI_EINVAL, "ntr=%d nt=%d must be positive", ntr, nt);
+    if (!(dt > 0)) return fail(c, SFWI_EINVAL, "dt must be positive");
+    const size_t m = (size_t)ntr * nt;
+    if (m > NA_MAX_SAMPLES || ntr > NA_MAX_TRACES)
+        return fail(c, SFWI_EINVAL, "gather of %d x %d exceeds the static workspace (at most %zu samples per "
+                    "component and %d traces)", ntr, nt, NA_MAX_SAMPLES, NA_MAX_TRACES);
+    char *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_m);
+        base = g_scratch[c->device];
+    }
+    if (!base) return fail(c, SFWI_ESTATE, "static workspace not resolved on device %d", c->device);
+    int *bad = (int *)base;
+    double *total_d = (double *)(base + 8);
+    OtJob *jobs = (OtJob *)(base + NA_JOBS);
+    double *mis = (double *)(base + NA_MIS);
+    float *ox = (float *)(base + NA_IO), *oz = ox + m, *rx = oz + m, *rz = rx + m;
+    double *ws = (double *)(base + NA_WS);
+    const size_t n_max = variant == 2 ? m : (size_t)nt;
+    const size_t nj = variant == 2 ? 2 : 2 * (size_t)ntr;
+    const size_t stride = 4 * (n_max + 1);
+    const size_t fit = NA_WS_BYTES / (stride * sizeof(double));   // >= 2: n_max <= NA_MAX_SAMPLES
+    const int blocks = (int)std::min(std::min(nj, fit), (size_t)OT_MAX_BLOCKS);
+    HIPCK(c, hipMemcpyAsync(ox, obs_x, m * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipMemcpyAsync(oz, obs_z, m * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipMemcpyAsync(rx, syn_x, m * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipMemcpyAsync(rz, syn_z, m * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipMemsetAsync(bad, 0, sizeof(int), c->stream));   // the walk flag starts clear on every call
+    hipLaunchKernelGGL(k_ot_jobs, dim3((unsigned)((nj + 63) / 64)), dim3(64), 0, c->stream, jobs, variant, ntr, nt,
+                       rx, rz, ox, oz, mis);
+    hipLaunchKernelGGL(k_ot, dim3(blocks), dim3(OT_THREADS), 0, c->stream, jobs, (int)nj, ws, stride,
+                       (double)dt * (double)dt, bad);
+    hipLaunchKernelGGL(k_sum_in_order, dim3(1), dim3(64), 0, c->stream, mis, (int)nj, total_d);
+    HIPCK(c, hipGetLastError());
+    double total = 0.0;
+    int flag = 0;
+    HIPCK(c, hipMemcpyAsync(&total, total_d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipMemcpyAsync(&flag, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipMemcpyAsync(res_x, rx, m * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipMemcpyAsync(res_z, rz, m * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    if (flag) return fail(c, SFWI_EHIP, "OT misfit: a merge walk did not end within its bound (invalid CDF levels)");
     *misfit += total;
     return 0;
 }
