@@ -116,6 +116,8 @@ def alg_bytes_8d(kernel, nx, nz, pml, store_bytes=2):
     """Scope-table bytes of one launch of `kernel` over one shot."""
     inter = nx * nz
     pmlc = (nx + 2 * pml) * (nz + 2 * pml) - inter
+    if kernel == "k_step(forward)":   # a whole cell-update
+        return 52 * inter + 100 * pmlc
     half = {"k_vel(forward)": "vel", "k_adj_vel": "vel", "k_vel(reconstruct+illum)": "vel",
             "k_vel(reconstruct)": "vel", "k_adj_vel_image": "vel",
             "k_str(forward)": "str", "k_adj_str": "str", "k_str(reconstruct+image)": "str",
@@ -182,6 +184,9 @@ def main():
                     help="FAST kernels: columns per wave segment (0: 8, the column-march kernels, also for the "
                          "accuracy leg's single shot; <0: per-cell kernels)")
     ap.add_argument("--march-depth", type=int, default=0, help="FAST march kernels: prefetch depth")
+    ap.add_argument("--fuse-step", type=int, default=0,
+                    help="FAST column-march forward: 1 one fused launch per time step, 0 / -1 two half-step launches")
+    ap.add_argument("--step-cols", type=int, default=0, help="fused step: columns per wave segment (0: 16)")
     ap.add_argument("--march-order", type=int, default=0,
                     help="FAST march kernels: block order (0 default, 1 shot after shot, 2 shots interleaved, "
                          "3 whole shots per XCD)")
@@ -248,7 +253,8 @@ def main():
                   allnx=sv.allnx, allnz=sv.allnz, max_batch=wave, mode=args.mode, storage=storage,
                   march_cols=args.march_cols or 8, march_depth=args.march_depth, march_order=args.march_order,
                   cpu_mem=args.cpu_mem > 0, strip_chunk=max(args.cpu_mem, 0), misfit=args.misfit,
-                  rbc=args.rbc, rbc_seed=1, recon=recon, snap_every=args.snap_every)
+                  rbc=args.rbc, rbc_seed=1, recon=recon, snap_every=args.snap_every, fuse_step=args.fuse_step,
+                  step_cols=args.step_cols)
     ctx.set_wavelet(w)
     ctx.set_model(sv.vp, sv.vs, sv.rho)
     t0 = time.perf_counter()

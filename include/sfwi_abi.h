@@ -129,6 +129,12 @@ typedef struct sfwi_params {
                            /* seeds cuRAND from time(0); here a run repeats)          */
     int recon;             /* SFWI_RECON_*                                          */
     int snap_every;        /* SFWI_RECON_SNAPSHOTS: steps between snapshots (0 = 4) */
+    int fuse_step;         /* FAST column-march forward solves of the background    */
+                           /* field: > 0 runs each time step as one fused launch    */
+                           /* (velocity and stress together, state double-buffered, */
+                           /* identical results); <= 0 as two half-step launches    */
+                           /* (0: the default, which sfwi_autotune may change)      */
+    int step_cols;         /* columns per wave segment of the fused step (0 = 16)   */
 } sfwi_params;
 
 /* One shot: the fields the reference driver writes into FD2DGPU_ELASTIC
@@ -449,7 +455,8 @@ int sfwi_s2_update(int allnx, int allnz, double fenzi, double fenmu, const float
 #define SFWI_K_ADJ_VEL_ONLY 8 /* adjoint velocity step                         */
 #define SFWI_K_SNAP_IMAGE 9   /* snapshot gradient: imaging of one snapshot    */
 #define SFWI_K_SNAP_STORE 10  /* snapshot gradient: max-abs scale + packing    */
-#define SFWI_K_COUNT 11
+#define SFWI_K_FWD_STEP 11    /* fused forward time step (velocity + stress)   */
+#define SFWI_K_COUNT 12
 int sfwi_set_timing(sfwi_ctx *ctx, int sample_every);
 int sfwi_kernel_time(sfwi_ctx *ctx, int kernel_id, double *avg_ms, int *launches);
 const char *sfwi_kernel_name(int kernel_id);
@@ -457,13 +464,19 @@ const char *sfwi_kernel_name(int kernel_id);
 /* Kernel-family autotuner of the FAST mode (no reference counterpart): for a
  * batch of nshots shots of this context's grid, times a gradient solve of
  * `steps` time steps for each candidate -- per-cell kernels; column march with
- * 8 or 16 columns per segment, prefetch depth 1 or 2, shot-major or
- * shot-interleaved block order -- and keeps the fastest in the context.
+ * 8 or 16 columns per segment, prefetch depth 1, 2 or 4, shot-major,
+ * shot-interleaved or whole-shots-per-XCD block order; the forward solve in
+ * half-step launches or in fused whole-step launches (the time-blocking
+ * depth, fuse_step) -- and keeps the fastest in the context (read it back
+ * with sfwi_get_params).
  * Needs the model and wavelet set.  Accumulated gradients are preserved; the
  * observed records resident in the batch slots are invalidated.  Outputs may
  * be NULL; best_ms is the winner's time for the `steps`-step solve. */
 int sfwi_autotune(sfwi_ctx *ctx, int nshots, int steps, int *march_cols, int *march_depth, int *march_order,
                   double *best_ms);
+
+/* The context's current parameters, including what sfwi_autotune chose. */
+int sfwi_get_params(const sfwi_ctx *ctx, sfwi_params *out);
 
 /* Library version string. */
 const char *sfwi_version(void);

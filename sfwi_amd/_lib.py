@@ -20,7 +20,7 @@ MISFIT_L2, MISFIT_OT_TRACE, MISFIT_OT_POWER, MISFIT_OT_GATHER = 0, 1, 2, 3
 REQ_FORWARD, REQ_BORN_FORWARD, REQ_GRADIENT, REQ_BORN_GRADIENT = 0, 1, 2, 3
 RECON_STRIPS, RECON_SNAPSHOTS = 0, 1
 (K_FWD_VEL, K_FWD_STR, K_REC_STR, K_REC_VEL, K_ADJ_STR, K_ADJ_VEL, K_REC_STR_IMG, K_REC_VEL_ILL, K_ADJ_VEL_ONLY,
- K_SNAP_IMAGE, K_SNAP_STORE, K_COUNT) = range(12)
+ K_SNAP_IMAGE, K_SNAP_STORE, K_FWD_STEP, K_COUNT) = range(13)
 
 
 class SfwiParams(ctypes.Structure):
@@ -37,6 +37,7 @@ class SfwiParams(ctypes.Structure):
         ("cpu_mem", ctypes.c_int), ("strip_chunk", ctypes.c_int),
         ("rbc", ctypes.c_int), ("rbc_seed", ctypes.c_uint),
         ("recon", ctypes.c_int), ("snap_every", ctypes.c_int),
+        ("fuse_step", ctypes.c_int), ("step_cols", ctypes.c_int),
     ]
 
 
@@ -128,6 +129,7 @@ def load():
     L.sfwi_kernel_time.argtypes = [vp, ctypes.c_int, dp, P(ctypes.c_int)]
     L.sfwi_kernel_name.restype = ctypes.c_char_p
     L.sfwi_kernel_name.argtypes = [ctypes.c_int]
+    L.sfwi_get_params.argtypes = [vp, P(SfwiParams)]
     L.sfwi_autotune.argtypes = [vp, ctypes.c_int, ctypes.c_int, P(ctypes.c_int), P(ctypes.c_int),
                                 P(ctypes.c_int), dp]
     i = ctypes.c_int

@@ -79,6 +79,8 @@ struct Medium {
 struct DevShot {
     Medium m;
     Fields fw, bk, bn;                     // forward/reconstructed, adjoint, Born scattered
+    Fields fw2;                            // second buffer of the forward state for the fused step march:
+                                           // its own totals and velocity split parts, fw's stress split parts
     fptr dgx, dgz, dgxz, dgzx;             // velocity derivatives for imaging
     fptr rz, rx;                           // synthetic (then residual) traces [ntr][nt]
     fptr r2z, r2x;                         // background traces of a Born solve
@@ -119,6 +121,8 @@ struct Geo {
     int snap_rz, snap_n;    // snapshot region height nz + 8 and cells (nx + 8)(nz + 8), origin (pml-4, pml-4)
     int im_nbands, im_waves, im_wg, im_per_xcd;   // snapshot imaging march: row bands, waves and workgroups
                                                   // per shot over the interior, workgroups per XCD (launch)
+    int fs_xs, fs_nbands, fs_waves, fs_wg, fs_per_xcd;   // fused step march: segment width, 48-row bands,
+                                                         // waves and workgroups per shot, per XCD (launch)
     float inv_nzd;
     float dt, dx, dz, dtdx, dtdz, idx, idz;
 };
@@ -662,14 +666,16 @@ __device__ inline void strip_slots(const Geo &g, int ix, int iz, int s[4]) {
 }
 
 // Explosive source on txx and tzz of every shot (sfwi-stage1/FD2DGPU.cu:2104-2114).
+// buf: the forward-state buffer the step reads (1: DevShot::fw2, fused step march).
 __global__ void k_source(Geo g, const DevShot *__restrict__ shots, const float *__restrict__ wav,
-                         int it, int nshots) {
+                         int it, int nshots, int buf) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nshots) return;
     const DevShot &d = shots[s];
+    const Fields &F = buf ? d.fw2 : d.fw;
     const int i = d.isx * g.nzp + d.isz;
-    d.fw.txx[i] += wav[it];
-    d.fw.tzz[i] += wav[it];
+    F.txx[i] += wav[it];
+    F.tzz[i] += wav[it];
 }
 
 // Velocity half step of every shot.  FWD also stores the boundary strips of
@@ -799,21 +805,23 @@ __global__ void __launch_bounds__(256) k_strip_pack(Geo g, const DevShot *__rest
 // Receiver sampling (sfwi-stage1/FD2DGPU.cu:3158-3166): vz, vx of the
 // forward field into rz, rx; with BORN the scattered field goes to rz, rx and
 // the background to r2z, r2x (record_scatter, sfwi-stage2/FD2DGPU.cu:3511-3519).
+// buf: the forward-state buffer holding the step's result (1: DevShot::fw2).
 template <bool BORN>
-__global__ void k_record(Geo g, const DevShot *__restrict__ shots, int it) {
+__global__ void k_record(Geo g, const DevShot *__restrict__ shots, int it, int buf) {
     const int tr = blockIdx.x * blockDim.x + threadIdx.x;
     const DevShot &d = shots[blockIdx.y];
     if (tr >= d.ntr) return;
+    const Fields &F = buf ? d.fw2 : d.fw;
     const int i = d.igx[tr] * g.nzp + d.igz;
     const size_t o = (size_t)tr * g.nt + it;
     if (BORN) {
         d.rz[o] = d.bn.vz[i];
         d.rx[o] = d.bn.vx[i];
-        d.r2z[o] = d.fw.vz[i];
-        d.r2x[o] = d.fw.vx[i];
+        d.r2z[o] = F.vz[i];
+        d.r2x[o] = F.vx[i];
     } else {
-        d.rz[o] = d.fw.vz[i];
-        d.rx[o] = d.fw.vx[i];
+        d.rz[o] = F.vz[i];
+        d.rx[o] = F.vx[i];
     }
 }
 
@@ -1011,14 +1019,15 @@ __device__ inline float wave_max(float v) {
     return v;
 }
 
-__global__ void __launch_bounds__(BLK) k_snap_max(Geo g, const DevShot *__restrict__ shots) {
+__global__ void __launch_bounds__(BLK) k_snap_max(Geo g, const DevShot *__restrict__ shots, int buf) {
     __shared__ float red[2][BLK / 64];
     const DevShot &d = shots[blockIdx.y];
+    const Fields &F = buf ? d.fw2 : d.fw;
     float mx = 0.0f, mz = 0.0f;
     for (int t = blockIdx.x * BLK + threadIdx.x; t < g.snap_n; t += SNAP_PART * BLK) {
         const int i = snap_cell(g, t);
-        mx = fmaxf(mx, fabsf(d.fw.vx[i]));
-        mz = fmaxf(mz, fabsf(d.fw.vz[i]));
+        mx = fmaxf(mx, fabsf(F.vx[i]));
+        mz = fmaxf(mz, fabsf(F.vz[i]));
     }
     mx = wave_max(mx);
     mz = wave_max(mz);
@@ -1053,9 +1062,11 @@ __global__ void __launch_bounds__(64) k_snap_scale(const DevShot *__restrict__ s
     }
 }
 
-// Store snapshot k: FP32, or v / scale in 16 bits.
-__global__ void __launch_bounds__(BLK) k_snap_pack(Geo g, const DevShot *__restrict__ shots, int k) {
+// Store snapshot k: FP32, or v / scale in 16 bits.  buf: the forward-state
+// buffer holding the new velocity (1: DevShot::fw2).
+__global__ void __launch_bounds__(BLK) k_snap_pack(Geo g, const DevShot *__restrict__ shots, int k, int buf) {
     const DevShot &d = shots[blockIdx.y];
+    const Fields &F = buf ? d.fw2 : d.fw;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= g.snap_n) return;
     const int i = snap_cell(g, t);
@@ -1063,14 +1074,14 @@ __global__ void __launch_bounds__(BLK) k_snap_pack(Geo g, const DevShot *__restr
     const size_t o = (size_t)k * 2 * g.snap_n;
     if (g.store == 0) {
         const fptr s = d.snap32 + o;
-        s[t] = d.fw.vx[i];
-        s[g.snap_n + t] = d.fw.vz[i];
+        s[t] = F.vx[i];
+        s[g.snap_n + t] = F.vz[i];
     } else {
         const gptr<unsigned short> s = d.snap16 + o;
         const float sx = __uint_as_float(d.snap_max[2 * (size_t)k]), sz = __uint_as_float(d.snap_max[2 * (size_t)k + 1]);
         const float ix = sx > 0.0f ? 1.0f / sx : 0.0f, iz = sz > 0.0f ? 1.0f / sz : 0.0f;
-        s[t] = pack16(g.store, d.fw.vx[i] * ix);
-        s[g.snap_n + t] = pack16(g.store, d.fw.vz[i] * iz);
+        s[t] = pack16(g.store, F.vx[i] * ix);
+        s[g.snap_n + t] = pack16(g.store, F.vz[i] * iz);
     }
 }
 

@@ -1,0 +1,185 @@
+// Fused forward time step of the FAST mode: velocity and stress updates of
+// one time step in a single column march (time blocking of depth one step).
+//
+// The two half steps of a time step are split into two launches elsewhere
+// (km_vel, km_str), and each re-reads what the other wrote: the velocity half
+// reads the stresses as stencil inputs, the stress half the velocities, so an
+// unsplit cell moves 72 B per step instead of the 52 B of one update.  Here a
+// wave owns 48 rows (lanes 8..55 of 64) and a segment of columns [x0, x1),
+// and marches x from x0 - 4 to x1 + 4: at column c it forms the new velocity
+// on lanes 4..59 from the old stresses (8-column register windows of txx and
+// txz, DPP z taps), keeps the new velocity in 9-column windows, and forms the
+// new stresses of column c - 4, whose stencil needs the new velocity of
+// columns c - 8 .. c.  The velocity of the 4 halo columns on each side and
+// of the 4 halo rows is computed again by the neighbouring waves, so every
+// wave reads only old values: the velocity totals, the velocity split parts
+// and the stress totals live in two buffers (DevShot::fw and fw2), read from
+// the buffer of the step's parity and written to the other.  The stress
+// split parts are read and written by their owner wave only and stay in
+// place (fw2's stress split pointers are fw's).  The old stresses a cell's
+// stress update needs are the stencil windows' own values, so an unsplit
+// cell's old state is fetched once.  The per-cell arithmetic is vel_math /
+// str_math on the same stencil sums as km_vel / km_str, so the results equal
+// those kernels' bit for bit.
+#pragma once
+#include "kernels_march.hpp"
+
+namespace sfwi {
+
+constexpr int FS_ROWS = 48;   // output rows per wave of the fused step (64 lanes minus 2 x 8 halo)
+
+// The buffer holding a shot's forward state after `steps` fused steps.
+__device__ inline const Fields &fw_buf(const DevShot &d, int b) { return b ? d.fw2 : d.fw; }
+
+template <bool SAVE, int PD>
+__global__ void __launch_bounds__(BLK) km_fwd_step(Geo g, const DevShot *__restrict__ shots, int nshots, int it,
+                                                   int par) {
+    const int b = blockIdx.x;
+    const int L = (b & 7) * g.fs_per_xcd + (b >> 3);
+    const int shot = L / g.fs_wg;
+    if (shot >= nshots) return;
+    const int W = (L - shot * g.fs_wg) * (BLK / 64) + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (W >= g.fs_waves) return;
+    const int seg = W / g.fs_nbands, band = W - seg * g.fs_nbands;
+    const int x0 = NOP + seg * g.fs_xs, x1 = min(x0 + g.fs_xs, g.nxp - NOP);
+    const int zb = NOP + band * FS_ROWS;
+    const DevShot d = shots[shot];   // by value: the field pointers stay in registers
+    const Fields &R = par ? d.fw2 : d.fw;    // the state before the step
+    const Fields &Wt = par ? d.fw : d.fw2;   // the state after it
+    const int lane = threadIdx.x & 63;
+    const int z = zb - 8 + lane;
+    const int zc = min(max(z, 0), g.nzp - 1);
+    const bool zin = z >= 0 && z < g.nzp;
+    const bool vrow = z >= NOP && z < g.nzp - NOP;              // rows the update domain holds
+    const bool vlane = lane >= 4 && lane < 60 && vrow;          // velocity formed here (halo rows included)
+    const bool own = lane >= 8 && lane < 56 && z < g.nzp - NOP;  // rows this wave writes
+    auto colc = [&](int c) { return min(max(c, 0), g.nxp - 1); };
+    auto colin = [&](int c) { return c >= 0 && c < g.nxp; };
+    auto in_dom = [&](int c) { return c >= NOP && c < g.nxp - NOP; };
+    auto ld = [&](cfptr f, int c) { return f[colc(c) * g.nzp + zc]; };
+    // per-column operands, PD columns ahead: old stresses entering the windows
+    // (txx at c + 4, txz at c + 3, tzz at c), the old velocity state of
+    // column c, and the stress split state and moduli of column c - 4
+    float qxx[PD], qxz[PD], qzz[PD];
+    VelCell qv[PD] = {};
+    StrCell qs[PD] = {};
+    auto fetch = [&](int j, int c) {
+        qxx[j] = ld(R.txx, c + 4);
+        qxz[j] = ld(R.txz, c + 3);
+        qzz[j] = ld(R.tzz, c);
+        const int i = colc(c) * g.nzp + zc;
+        vel_load_w(d.m, R, i, vlane && in_dom(c), in_band(g, c, z), qv[j]);
+        const int cp = c - 4;
+        const bool sown = own && cp >= x0 && cp < x1;
+        const bool sp = in_band(g, cp, z) || (cp == d.isx && z == d.isz);
+        const int ip = colc(cp) * g.nzp + zc;
+        if (sown) { qs[j].lam = d.m.lam[ip]; qs[j].mu = d.m.mu[ip]; }
+        if (sown && sp) {
+            qs[j].xx_x = R.txx_x[ip]; qs[j].xx_z = R.txx_z[ip]; qs[j].zz_x = R.tzz_x[ip];
+            qs[j].zz_z = R.tzz_z[ip]; qs[j].xz_x = R.txz_x[ip]; qs[j].xz_z = R.txz_z[ip];
+            qs[j].ax = d.m.dampx[ip]; qs[j].az = d.m.dampz[ip];
+        }
+    };
+    auto zs = [&](int c, float v) { return (zin && colin(c)) ? v : 0.0f; };
+    const int cs = x0 - 4, ce = x1 + 4;
+    float wxx[8], wxz[8], tzh[5] = {}, vxn[9] = {}, vzn[9] = {};
+    float txx_m4 = 0.0f;   // txx of column c - 4 (left the txx window a column ago)
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        wxx[j] = zs(cs - 3 + j, ld(R.txx, cs - 3 + j));
+        wxz[j] = zs(cs - 4 + j, ld(R.txz, cs - 4 + j));
+    }
+#pragma unroll
+    for (int j = 0; j < PD; ++j)
+        if (cs + j < ce) fetch(j, cs + j);
+    const BornSrc none{};
+    for (int c0 = cs; c0 < ce; c0 += PD) {
+#pragma unroll
+        for (int j = 0; j < PD; ++j) {
+            const int c = c0 + j;
+            if (c >= ce) break;
+            wxx[7] = zs(c + 4, qxx[j]);
+            wxz[7] = zs(c + 3, qxz[j]);
+            tzh[4] = zs(c, qzz[j]);
+            VelCell vs = qv[j];
+            StrCell ss = qs[j];
+            if (c + PD < ce) fetch(j, c + PD);
+            // velocity of column c
+            const float a = x_win(wxx);        // Dx+ txx
+            const float e = x_win(wxz);        // Dx- txz
+            const float cz = z_plus(tzh[4]);   // Dz+ tzz
+            const float bz = z_minus(wxz[4]);  // Dz- txz (txz at column c)
+            float vx_c = 0.0f, vz_c = 0.0f;
+            if (vlane && in_dom(c)) {
+                const bool sp = in_band(g, c, z);
+                vel_math(g, vs, sp, 1, a, bz, cz, e);
+                vx_c = vs.vx;
+                vz_c = vs.vz;
+                if (own && c >= x0 && c < x1) {
+                    const int i = c * g.nzp + z;
+                    vel_store(Wt, i, vs, sp);
+                    if (SAVE && sp) {
+                        int sl[4];
+                        strip_slots(g, c, z, sl);
+                        const fptr base = g.store ? d.stage : d.strip + (size_t)it * 2 * g.sf;
+#pragma unroll
+                        for (int k = 0; k < 4; ++k)
+                            if (sl[k] >= 0) {
+                                base[sl[k]] = vs.vx;
+                                base[g.sf + sl[k]] = vs.vz;
+                            }
+                    }
+                }
+            }
+            // old stresses of column c - 4, then the windows move on
+            const float sxx4 = txx_m4, sxz4 = wxz[0], szz4 = tzh[0];
+            txx_m4 = wxx[0];
+            shift(wxx);
+            shift(wxz);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) tzh[k] = tzh[k + 1];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                vxn[k] = vxn[k + 1];
+                vzn[k] = vzn[k + 1];
+            }
+            vxn[8] = vx_c;
+            vzn[8] = vz_c;
+            // stresses of column c - 4 from the new velocities of columns c - 8 .. c
+            const int cp = c - 4;
+            if (cp >= x0) {
+                VGrad v;
+                v.xx = x_win(&vxn[0]);       // Dx- vx
+                v.zx = x_win(&vzn[1]);       // Dx+ vz
+                v.zz = z_minus(vzn[4]);      // Dz- vz (vz at column c - 4)
+                v.xz = z_plus(vxn[4]);       // Dz+ vx (vx at column c - 4)
+                if (own && cp < x1) {
+                    const bool sp = in_band(g, cp, z) || (cp == d.isx && z == d.isz);
+                    if (!sp) {
+                        ss.txx = sxx4;
+                        ss.tzz = szz4;
+                        ss.txz = sxz4;
+                    }
+                    str_math<false>(g, ss, v, sp, 1, none);
+                    str_store(Wt, cp * g.nzp + z, ss, sp);
+                }
+            }
+        }
+    }
+}
+
+// Copy a shot's forward state (velocity and stress totals, velocity split
+// parts) from the second buffer to the first, after an odd number of fused
+// steps, so the kernels that follow find it in DevShot::fw.
+__global__ void k_fw_from_fw2(Geo g, const DevShot *__restrict__ shots) {
+    const DevShot &d = shots[blockIdx.y];
+    const int n = g.nxp * g.nzp;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        d.fw.vx[i] = d.fw2.vx[i]; d.fw.vz[i] = d.fw2.vz[i];
+        d.fw.txx[i] = d.fw2.txx[i]; d.fw.tzz[i] = d.fw2.tzz[i]; d.fw.txz[i] = d.fw2.txz[i];
+        d.fw.vx_x[i] = d.fw2.vx_x[i]; d.fw.vx_z[i] = d.fw2.vx_z[i];
+        d.fw.vz_x[i] = d.fw2.vz_x[i]; d.fw.vz_z[i] = d.fw2.vz_z[i];
+    }
+}
+
+}  // namespace sfwi

@@ -9,6 +9,7 @@
 #include "sfwi_abi.h"
 #include "kernels.hpp"
 #include "kernels_march.hpp"
+#include "kernels_step.hpp"
 #include "ot.hpp"
 
 #include <algorithm>
@@ -62,7 +63,7 @@ struct Slot {
     float *strip = nullptr;     // FP32 [ring][2][sf], or 16-bit: scales [ring][2], stage [2][sf], [ring][2][sf]
     char *hstrip = nullptr;     // host strips (cpu_mem, pinned): FP32 [nt][2][sf], or 16-bit [nt][2][sf] + scales [nt][2]
     char *snap = nullptr;       // velocity snapshots: max bits [nsnap][2] (256-B aligned), then FP32 or 16-bit values
-    Fields fw{}, bk{}, bn{};
+    Fields fw{}, fw2{}, bk{}, bn{};
     float *dg[4]{};
     float *rz, *rx, *r2z, *r2x, *oz, *ox;
     int *igx, *inj, *inj_off;
@@ -286,7 +287,7 @@ int ensure_slots(sfwi_ctx *c, int n, bool born, bool strips, bool snaps) {
     for (int s = 0; s < n; ++s) {
         Slot &S = c->slots[s];
         if (!S.block) {
-            int r = dalloc(c, (void **)&S.block, (34 + 2 * c->g.grad_rho) * cell);
+            int r = dalloc(c, (void **)&S.block, (43 + 2 * c->g.grad_rho) * cell);
             if (r) return r;
             float *f = (float *)S.block;
             fptr *fw[15] = {&S.fw.vx, &S.fw.vz, &S.fw.txx, &S.fw.tzz, &S.fw.txz, &S.fw.vx_x, &S.fw.vx_z,
@@ -295,11 +296,16 @@ int ensure_slots(sfwi_ctx *c, int n, bool born, bool strips, bool snaps) {
             fptr *bk[15] = {&S.bk.vx, &S.bk.vz, &S.bk.txx, &S.bk.tzz, &S.bk.txz, &S.bk.vx_x, &S.bk.vx_z,
                               &S.bk.vz_x, &S.bk.vz_z, &S.bk.txx_x, &S.bk.txx_z, &S.bk.tzz_x, &S.bk.tzz_z,
                               &S.bk.txz_x, &S.bk.txz_z};
+            // fw (15 fields) | fw2 (9: totals and velocity split parts) | bk (15) | derivatives (4) | v(it+1) (2)
             for (int k = 0; k < 15; ++k) *fw[k] = f + k * c->ncell;
-            for (int k = 0; k < 15; ++k) *bk[k] = f + (15 + k) * c->ncell;
-            for (int k = 0; k < 4; ++k) S.dg[k] = f + (30 + k) * c->ncell;
+            S.fw2 = S.fw;   // the stress split parts are shared
+            fptr *f2[9] = {&S.fw2.vx, &S.fw2.vz, &S.fw2.txx, &S.fw2.tzz, &S.fw2.txz, &S.fw2.vx_x, &S.fw2.vx_z,
+                           &S.fw2.vz_x, &S.fw2.vz_z};
+            for (int k = 0; k < 9; ++k) *f2[k] = f + (15 + k) * c->ncell;
+            for (int k = 0; k < 15; ++k) *bk[k] = f + (24 + k) * c->ncell;
+            for (int k = 0; k < 4; ++k) S.dg[k] = f + (39 + k) * c->ncell;
             if (c->g.grad_rho)
-                for (int k = 0; k < 2; ++k) S.vold[k] = f + (34 + k) * c->ncell;
+                for (int k = 0; k < 2; ++k) S.vold[k] = f + (43 + k) * c->ncell;
             // records, receiver tables, accumulators, per-trace sums
             const size_t rbytes = 6 * c->nrec * sizeof(float) + (3 * (size_t)c->p.ntr + 1) * sizeof(int) +
                                   4 * c->nint * sizeof(float) + (size_t)c->p.ntr * sizeof(double) + 64;
@@ -429,7 +435,7 @@ int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium 
         if (!m && (r = model_window(c, s.left, born, &m))) return r;
         DevShot d{};
         d.m = *m;
-        d.fw = S.fw; d.bk = S.bk; d.bn = S.bn;
+        d.fw = S.fw; d.fw2 = S.fw2; d.bk = S.bk; d.bn = S.bn;
         d.dgx = S.dg[0]; d.dgz = S.dg[1]; d.dgxz = S.dg[2]; d.dgzx = S.dg[3];
         d.rz = S.rz; d.rx = S.rx; d.r2z = S.r2z; d.r2x = S.r2x; d.oz = S.oz; d.ox = S.ox;
         if (os) {   // read in place from the resident set
@@ -458,7 +464,7 @@ int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium 
         d.left = s.left;
         c->h_shots[k] = d;
         const size_t cell = c->ncell * sizeof(float);
-        HIPCK(c, hipMemsetAsync(S.block, 0, (grad ? 34 : 15) * cell, c->stream));
+        HIPCK(c, hipMemsetAsync(S.block, 0, (grad ? 43 : 24) * cell, c->stream));
         if (born) HIPCK(c, hipMemsetAsync(S.born, 0, 15 * cell, c->stream));
         HIPCK(c, hipMemcpyAsync(S.igx, S.h_igx.data(), s.ntr * sizeof(int), hipMemcpyHostToDevice, c->stream));
         HIPCK(c, hipMemcpyAsync(S.inj, S.h_inj.data(), S.h_inj.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
@@ -610,7 +616,7 @@ int strip_chunk_copy(sfwi_ctx *c, int n, int k, bool to_host) {
 // (at depth 4 they fit 2 waves per SIMD and run 6-11 % slower than at depth 2;
 // the imaging reconstruction kernels gain 1 % at depth 2, which the next
 // kernel loses again, and the others are 1-4 % faster at depth 4).
-template <bool E, int PD, int HPD>
+template <bool E, int PD, int HPD, int FPD = 2>
 int run_batch(sfwi_ctx *c, Kind kind, int n) {
     const Geo &g = c->g;
     const int nt = g.nt;
@@ -650,10 +656,47 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
     const bool hs = save && c->ring < nt;
     const dim3 snapg((unsigned)((g.snap_n + BLK - 1) / BLK), (unsigned)n);
     const int ch = c->chunk, klast = (nt - 1) / ch;
-    for (int it = 0; it < nt; ++it) {
+    // Fused step march (velocity and stress of a step in one launch, the state
+    // double-buffered) for FAST column-march forward solves of the background
+    // field when fuse_step > 0 (sfwi_autotune sets it where it is faster).
+    const bool fused = !percell && !born && c->p.fuse_step > 0;
+    Geo gs = g;
+    gs.fs_per_xcd = (int)(((long)g.fs_wg * n + 7) / 8);
+    const dim3 steps_grid((unsigned)(8 * gs.fs_per_xcd));
+    for (int it = 0; it < nt && fused; ++it) {
+        const int sp = hs ? it % c->ring : it;
+        const int par = it & 1;   // buffer read by this step; the step writes the other
+        if (hs && it % ch == 0 && it / ch >= 2) HIPCK(c, hipStreamWaitEvent(st, c->ev_copy[(it / ch) & 1], 0));
+        hipLaunchKernelGGL(k_source, src1, dim3(64), 0, st, g, D, c->wav, it, n, par);
+        int t = tbeg(c, SFWI_K_FWD_STEP, it);
+        if (save) hipLaunchKernelGGL((km_fwd_step<true, FPD>), steps_grid, blk, 0, st, gs, D, n, sp, par);
+        else hipLaunchKernelGGL((km_fwd_step<false, FPD>), steps_grid, blk, 0, st, gs, D, n, sp, par);
+        tend(c, t);
+        if (save && g.store) hipLaunchKernelGGL(k_strip_pack, dim3(2, (unsigned)n), dim3(256), 0, st, g, D, sp);
+        if (snap && it % g.snap_k == 0) {   // keep the new velocity (snapshot it / snap_k)
+            t = tbeg(c, SFWI_K_SNAP_STORE, it);
+            if (g.store) {
+                hipLaunchKernelGGL(k_snap_max, dim3(SNAP_PART, (unsigned)n), blk, 0, st, g, D, 1 - par);
+                hipLaunchKernelGGL(k_snap_scale, dim3((unsigned)n), dim3(64), 0, st, D, it / g.snap_k);
+            }
+            hipLaunchKernelGGL(k_snap_pack, snapg, blk, 0, st, g, D, it / g.snap_k, 1 - par);
+            tend(c, t);
+        }
+        hipLaunchKernelGGL(k_record<false>, trg, dim3(64), 0, st, g, D, it, 1 - par);
+        if (hs && it % ch == ch - 1 && it / ch <= klast - 2) {   // chunk written: to the host
+            const int k = it / ch;
+            HIPCK(c, hipEventRecord(c->ev_ring[k & 1], st));
+            HIPCK(c, hipStreamWaitEvent(c->cstream, c->ev_ring[k & 1], 0));
+            if (int r = strip_chunk_copy(c, n, k, true)) return r;
+            HIPCK(c, hipEventRecord(c->ev_copy[k & 1], c->cstream));
+        }
+    }
+    if (fused && (nt & 1) && grad && !snap)   // the final state is in the second buffer: the reverse reads fw
+        hipLaunchKernelGGL(k_fw_from_fw2, dim3(1024, (unsigned)n), blk, 0, st, g, D);
+    for (int it = 0; it < nt && !fused; ++it) {
         const int sp = hs ? it % c->ring : it;
         if (hs && it % ch == 0 && it / ch >= 2) HIPCK(c, hipStreamWaitEvent(st, c->ev_copy[(it / ch) & 1], 0));
-        hipLaunchKernelGGL(k_source, src1, dim3(64), 0, st, g, D, c->wav, it, n);
+        hipLaunchKernelGGL(k_source, src1, dim3(64), 0, st, g, D, c->wav, it, n, 0);
         int t = tbeg(c, SFWI_K_FWD_VEL, it);
         if (percell) {
             if (born) {
@@ -677,10 +720,10 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
         if (snap && it % g.snap_k == 0) {   // keep the new velocity (snapshot it / snap_k)
             t = tbeg(c, SFWI_K_SNAP_STORE, it);
             if (g.store) {   // max-abs scales: per-block partials, then one wave per shot
-                hipLaunchKernelGGL(k_snap_max, dim3(SNAP_PART, (unsigned)n), blk, 0, st, g, D);
+                hipLaunchKernelGGL(k_snap_max, dim3(SNAP_PART, (unsigned)n), blk, 0, st, g, D, 0);
                 hipLaunchKernelGGL(k_snap_scale, dim3((unsigned)n), dim3(64), 0, st, D, it / g.snap_k);
             }
-            hipLaunchKernelGGL(k_snap_pack, snapg, blk, 0, st, g, D, it / g.snap_k);
+            hipLaunchKernelGGL(k_snap_pack, snapg, blk, 0, st, g, D, it / g.snap_k, 0);
             tend(c, t);
         }
         t = tbeg(c, SFWI_K_FWD_STR, it);
@@ -692,8 +735,8 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
             else hipLaunchKernelGGL((km_str<false, false, PD>), tiles, blk, 0, st, gm, D, 1);
         }
         tend(c, t);
-        if (born) hipLaunchKernelGGL(k_record<true>, trg, dim3(64), 0, st, g, D, it);
-        else hipLaunchKernelGGL(k_record<false>, trg, dim3(64), 0, st, g, D, it);
+        if (born) hipLaunchKernelGGL(k_record<true>, trg, dim3(64), 0, st, g, D, it, 0);
+        else hipLaunchKernelGGL(k_record<false>, trg, dim3(64), 0, st, g, D, it, 0);
         if (hs && it % ch == ch - 1 && it / ch <= klast - 2) {   // chunk written: to the host
             const int k = it / ch;
             HIPCK(c, hipEventRecord(c->ev_ring[k & 1], st));
@@ -833,10 +876,10 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
 int run_direct(sfwi_ctx *c, Kind kind, int n) {
     if (c->p.mode == SFWI_MODE_EXACT) return run_batch<true, 1, 1>(c, kind, n);
     switch (c->p.march_depth) {
-    case 1: return run_batch<false, 1, 1>(c, kind, n);
-    case 2: return run_batch<false, 2, 2>(c, kind, n);
-    case 4: return run_batch<false, 4, 4>(c, kind, n);
-    default: return run_batch<false, 4, 2>(c, kind, n);   // automatic
+    case 1: return run_batch<false, 1, 1, 1>(c, kind, n);
+    case 2: return run_batch<false, 2, 2, 2>(c, kind, n);
+    case 4: return run_batch<false, 4, 4, 4>(c, kind, n);
+    default: return run_batch<false, 4, 2, 2>(c, kind, n);   // automatic
     }
 }
 
@@ -984,6 +1027,7 @@ int sfwi_ctx_create(const sfwi_params *p, sfwi_ctx **out) {
     if (p->recon != SFWI_RECON_STRIPS && p->recon != SFWI_RECON_SNAPSHOTS)
         return fail(nullptr, SFWI_EINVAL, "unknown recon %d", p->recon);
     if (p->snap_every < 0) return fail(nullptr, SFWI_EINVAL, "snap_every must be >= 0");
+    if (p->step_cols < 0) return fail(nullptr, SFWI_EINVAL, "step_cols must be >= 0");
     if (p->recon == SFWI_RECON_SNAPSHOTS) {
         if (p->mode != SFWI_MODE_FAST)
             return fail(nullptr, SFWI_EINVAL, "recon=snapshots runs in FAST mode only (EXACT restates the strips)");
@@ -1017,6 +1061,10 @@ int sfwi_ctx_create(const sfwi_params *p, sfwi_ctx **out) {
         g.im_waves = ((p->nx + 7) / 8) * g.im_nbands;
         g.im_wg = (g.im_waves + BLK / 64 - 1) / (BLK / 64);
     }
+    g.fs_xs = p->step_cols > 0 ? p->step_cols : 16;
+    g.fs_nbands = (g.nzd + FS_ROWS - 1) / FS_ROWS;
+    g.fs_waves = ((g.nxp - 2 * NOP + g.fs_xs - 1) / g.fs_xs) * g.fs_nbands;
+    g.fs_wg = (g.fs_waves + BLK / 64 - 1) / (BLK / 64);
     g.inv_nzd = 1.0f / (float)g.nzd;
     g.dt = p->dt; g.dx = p->dx; g.dz = p->dz;
     g.dtdx = p->dt / p->dx; g.dtdz = p->dt / p->dz;
@@ -1501,8 +1549,11 @@ int sfwi_autotune(sfwi_ctx *c, int nshots, int steps, int *march_cols, int *marc
     }
     const size_t acc = 4 * c->nmodel * sizeof(float);
     HIPCK(c, hipMemcpyAsync(c->tmp_model, c->Gvp, acc, hipMemcpyDeviceToDevice, c->stream));
-    struct Cand { int cols, depth, order; };
-    const Cand cands[] = {{-1, 1, 1}, {8, 4, 3}, {8, 2, 3}, {8, 1, 3}, {16, 2, 3}, {8, 4, 1}, {8, 2, 2}};
+    // kernel family, segment, prefetch depth, block order, and the time-blocking
+    // depth of the forward solve: half steps (fuse 0) or whole fused steps (1)
+    struct Cand { int cols, depth, order, fuse; };
+    const Cand cands[] = {{-1, 1, 1, 0}, {8, 4, 3, 0}, {8, 2, 3, 0}, {8, 1, 3, 0}, {16, 2, 3, 0}, {8, 4, 1, 0},
+                          {8, 2, 2, 0}, {8, 2, 3, 1}, {8, 1, 3, 1}};
     const sfwi_params keep = c->p;
     const int nt_keep = c->g.nt;
     c->g.nt = ns;                              // a shorter record inside the same buffers
@@ -1516,6 +1567,7 @@ int sfwi_autotune(sfwi_ctx *c, int nshots, int steps, int *march_cols, int *marc
         c->p.march_cols = cd.cols;
         c->p.march_depth = cd.depth;
         c->p.march_order = cd.order;
+        c->p.fuse_step = cd.fuse;
         set_march_geo(c->g, cd.cols);
         float ms = 0.0f;
         for (int rep = 0; rep < 2 && !r; ++rep) {
@@ -1537,6 +1589,7 @@ int sfwi_autotune(sfwi_ctx *c, int nshots, int steps, int *march_cols, int *marc
     c->p.march_cols = win.cols;
     c->p.march_depth = win.depth;
     c->p.march_order = win.order;
+    c->p.fuse_step = win.fuse;
     set_march_geo(c->g, win.cols);
     drop_graphs(c);                            // captured with another kernel family
     for (Slot &S : c->slots) S.obs_ntr = 0;
@@ -1547,6 +1600,12 @@ int sfwi_autotune(sfwi_ctx *c, int nshots, int steps, int *march_cols, int *marc
     if (march_depth) *march_depth = win.depth;
     if (march_order) *march_order = win.order;
     if (best_ms) *best_ms = best;
+    return 0;
+}
+
+int sfwi_get_params(const sfwi_ctx *c, sfwi_params *out) {
+    if (!c || !out) return SFWI_EINVAL;
+    *out = c->p;
     return 0;
 }
 
@@ -1561,7 +1620,7 @@ const char *sfwi_kernel_name(int kid) {
     static const char *names[SFWI_K_COUNT] = {
         "k_vel(forward)", "k_str(forward)", "k_str(reconstruct)", "k_vel(reconstruct)", "k_adj_str",
         "k_adj_vel_image", "k_str(reconstruct+image)", "k_vel(reconstruct+illum)", "k_adj_vel",
-        "k_snap_image", "k_snap_store"};
+        "k_snap_image", "k_snap_store", "k_step(forward)"};
     return (kid >= 0 && kid < SFWI_K_COUNT) ? names[kid] : "?";
 }
 

@@ -43,7 +43,7 @@ class Shot:
 def make_params(*, dx, dz, dt, nx, nz, pml, nt, ntr, allnx, allnz, rr=0.001, rho_const=0.0, adjoint_dir=1,
                 max_batch=1, mode="fast", storage="f32", use_graph=False, march_cols=0, march_depth=0,
                 misfit="l2", march_order=0, grad_rho=False, cpu_mem=False, strip_chunk=0, rbc=False,
-                rbc_seed=0, recon="strips", snap_every=0):
+                rbc_seed=0, recon="strips", snap_every=0, fuse_step=0, step_cols=0):
     """``sfwi_params`` of a context or request (keyword names as in the C struct)."""
     p = SfwiParams()
     p.dx, p.dz, p.dt = dx, dz, dt
@@ -65,6 +65,8 @@ def make_params(*, dx, dz, dt, nx, nz, pml, nt, ntr, allnx, allnz, rr=0.001, rho
     p.rbc_seed = int(rbc_seed) & 0xFFFFFFFF
     p.recon = {"strips": _lib.RECON_STRIPS, "snapshots": _lib.RECON_SNAPSHOTS}[recon]
     p.snap_every = int(snap_every)
+    p.fuse_step = int(fuse_step)
+    p.step_cols = int(step_cols)
     return p
 
 
@@ -243,6 +245,12 @@ class Context:
         ms, n = ctypes.c_double(0.0), ctypes.c_int(0)
         self._ck(self.L.sfwi_kernel_time(self.h, kid, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+    def params(self):
+        """The context's current sfwi_params (including the autotuner's choices)."""
+        p = SfwiParams()
+        self._ck(self.L.sfwi_get_params(self.h, ctypes.byref(p)))
+        return p
 
     def autotune(self, nshots, steps=100):
         """Pick the fastest FAST kernel family for batches of nshots; returns

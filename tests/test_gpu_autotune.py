@@ -23,11 +23,13 @@ def test_autotune_choice_and_preserved_gradients():
         ctx.set_model(sv.vp0, sv.vs0, sv.rho0)
         m0 = ctx.batch_gradient(shots[:1])            # accumulated before tuning
         cols, depth, order, ms = ctx.autotune(2, steps=60)
-        assert (cols, depth, order) in {(-1, 1, 1), (8, 1, 1), (8, 2, 1), (16, 1, 1), (8, 1, 2)}
+        fuse = ctx.params().fuse_step
+        assert (cols, depth, order, fuse) in {(-1, 1, 1, 0), (8, 4, 3, 0), (8, 2, 3, 0), (8, 1, 3, 0), (16, 2, 3, 0),
+                                              (8, 4, 1, 0), (8, 2, 2, 0), (8, 2, 3, 1), (8, 1, 3, 1)}
         assert ms > 0
         m1 = ctx.batch_gradient(shots)
         g = ctx.flush()
-    with context(sv, max_batch=2, march_cols=cols, march_depth=depth, march_order=order) as chk:
+    with context(sv, max_batch=2, march_cols=cols, march_depth=depth, march_order=order, fuse_step=fuse) as chk:
         chk.set_wavelet(w)
         chk.set_model(sv.vp0, sv.vs0, sv.rho0)
         n0 = chk.batch_gradient(shots[:1])

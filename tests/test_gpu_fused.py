@@ -1,0 +1,81 @@
+"""The fused forward time step (velocity and stress of a step in one column
+march, double-buffered state) against the two half-step column-march kernels
+and the per-cell kernels: forward traces, strip gradients (FP32 and BF16
+strips, strips in host memory), snapshot gradients and line-search misfits
+are equal bit for bit, for even and odd record lengths (the final state ends
+in either buffer), segment widths that do and do not divide the grid, grids
+with one and with several row bands, and graph replays."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from sfwi_amd import workloads as W
+from tests.helpers import context, lib_shot, oracle_cfg
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(nz, nt, seed):
+    sv = W.tiny(nx=52, nz=nz, pml=8, nt=nt, nshots=2, seed=seed)
+    w = W.ricker(sv.nt, sv.dt, sv.f0)
+    obs = [O.forward(oracle_cfg(sv, sh), sv.vp, sv.vs, sv.rho, w, sh.gc, "f32") for sh in sv.shots]
+    return sv, w, obs
+
+
+def _run(sv, w, obs, **kw):
+    out = []
+    with context(sv, "fast", max_batch=2, **kw) as ctx:
+        ctx.set_wavelet(w)
+        ctx.set_model(sv.vp, sv.vs, sv.rho)
+        out.append(ctx.batch_forward([lib_shot(sv, sh) for sh in sv.shots]))
+        ctx.set_model(sv.vp0, sv.vs0, sv.rho0)
+        out.append(ctx.batch_gradient([lib_shot(sv, sh, *obs[k]) for k, sh in enumerate(sv.shots)]))
+        out.append(ctx.flush())
+        d = (sv.vp - sv.vp0).astype(np.float32)
+        out.append(ctx.batch_linesearch_parabola([lib_shot(sv, sh, *obs[k]) for k, sh in enumerate(sv.shots)],
+                                                 d, 0.5 * d, 0.1, 0.3))
+    return out
+
+
+def _flat(out):
+    for x in out:
+        if isinstance(x, (tuple, list)):
+            yield from _flat(x)
+        else:
+            yield np.asarray(x)
+
+
+def _same(a, b):
+    fa, fb = list(_flat(a)), list(_flat(b))
+    assert len(fa) == len(fb)
+    for x, y in zip(fa, fb):
+        assert np.array_equal(x, y), "max |d| = %g" % np.abs(x.astype(np.float64) - y).max()
+
+
+@pytest.mark.parametrize("nz,nt,cols", [(40, 200, 16), (40, 201, 8), (134, 201, 37), (134, 200, 16)])
+def test_fused_step_equals_half_steps(nz, nt, cols):
+    sv, w, obs = _case(nz, nt, seed=nz + nt)
+    ref = _run(sv, w, obs, march_cols=8, fuse_step=-1)
+    assert np.abs(ref[0][0]).max() > 0 and np.abs(ref[2][0]).max() > 0
+    _same(_run(sv, w, obs, march_cols=8, fuse_step=1, step_cols=cols), ref)
+    _same(_run(sv, w, obs, march_cols=-1), ref)
+
+
+@pytest.mark.parametrize("extra", [dict(storage="bf16"), dict(cpu_mem=True, strip_chunk=30),
+                                   dict(recon="snapshots", storage="f16", snap_every=3),
+                                   dict(use_graph=True)])
+def test_fused_step_with_storage_and_graphs(extra):
+    sv, w, obs = _case(134, 201, seed=5)
+    ref = _run(sv, w, obs, march_cols=8, fuse_step=-1, **extra)
+    _same(_run(sv, w, obs, march_cols=8, fuse_step=1, **extra), ref)
+    if extra.get("use_graph"):   # replays of the captured solves
+        with context(sv, "fast", max_batch=2, march_cols=8, fuse_step=1, use_graph=True) as ctx:
+            ctx.set_wavelet(w)
+            ctx.set_model(sv.vp0, sv.vs0, sv.rho0)
+            shots = [lib_shot(sv, sh, *obs[k]) for k, sh in enumerate(sv.shots)]
+            for _ in range(3):
+                m = ctx.batch_gradient(shots)
+                g = ctx.flush()
+                assert np.array_equal(m, ref[1])
+                for a, b in zip(g, ref[2]):
+                    assert np.array_equal(a, b)
