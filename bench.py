@@ -201,6 +201,8 @@ def main():
     ap.add_argument("--stage", type=int, default=1, choices=(1, 2), help="inversion: which stage's program")
     ap.add_argument("--born", action="store_true",
                     help="time the stage-2 (Born) gradient of the workload's shot batch instead")
+    ap.add_argument("--hessian", action="store_true",
+                    help="time the stage-1 Gauss-Newton Hessian-vector product of the workload's shot batch instead")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--requests", type=int, default=24)
@@ -232,7 +234,7 @@ def main():
         return uneven(args)
     if args.inversion:
         return inversion(args)
-    if args.born:
+    if args.born or args.hessian:
         return born(args)
 
     mp = args.workload == "mixed_precision"
@@ -484,7 +486,13 @@ def born(args):
     --cpu-shots shots is checked against the FP64 oracle, which is also the CPU
     baseline.  The two forward kernels are the Born variants (background and
     scattered fields in one launch); the reverse step is the stage-1 one on
-    the background field.  No roofline entry."""
+    the background field.  No roofline entry.
+    --hessian: the stage-1 Gauss-Newton Hessian-vector product
+    (sfwi-stage1/FD2DGPU.cu:5236-5359) of the same batch at the initial model,
+    the vector p = true - initial model, the model's density and the stage-1
+    direction: the same launches with the scattered traces as the adjoint
+    source (no observed traces, no misfit)."""
+    hess = args.hessian
     from sfwi_amd import workloads as W
     from sfwi_amd import _lib
     from sfwi_amd.elastic import Context, Shot
@@ -497,31 +505,40 @@ def born(args):
     w = W.ricker(sv.nt, sv.dt, sv.f0)
     geo = [W.shot_geometry(sv, s) for s in sv.shots]
     nsh = len(sv.shots)
-    print("bench: Born gradient, %s, %d shots, grid %dx%d pml %d, nt %d" % (
-        sv.name, nsh, sv.nx, sv.allnz, sv.pml, sv.nt), file=sys.stderr, flush=True)
+    storage = args.storage or "f32"
+    print("bench: %s, %s, %d shots, grid %dx%d pml %d, nt %d" % (
+        "Hessian-vector product" if hess else "Born gradient", sv.name, nsh, sv.nx, sv.allnz, sv.pml, sv.nt),
+        file=sys.stderr, flush=True)
     ctx = Context(dx=sv.dx, dz=sv.dz, dt=sv.dt, nx=sv.nx, nz=sv.allnz, pml=sv.pml, nt=sv.nt, ntr=sv.ntr,
-                  allnx=sv.allnx, allnz=sv.allnz, max_batch=nsh, mode=args.mode, storage=args.storage,
+                  allnx=sv.allnx, allnz=sv.allnz, max_batch=nsh, mode=args.mode, storage=storage,
                   march_cols=args.march_cols, march_depth=args.march_depth, march_order=args.march_order,
-                  rho_const=2000.0, adjoint_dir=-1)
+                  rho_const=0.0 if hess else 2000.0, adjoint_dir=1 if hess else -1)
     ctx.set_wavelet(w)
-    ctx.set_model(sv.vp, sv.vs, None)
+    rho_m = sv.rho0 if hess else None
+    ctx.set_model(sv.vp, sv.vs, rho_m)
     obs_z, obs_x = ctx.batch_forward([Shot(s.left, *g, s.ngx_left, s.gc) for s, g in zip(sv.shots, geo)])
     dvp = (sv.vp - sv.vp0).astype(np.float32)
     dvs = (sv.vs - sv.vs0).astype(np.float32)
-    ctx.set_model(sv.vp0, sv.vs0, None)
-    ctx.set_perturbation(dvp, dvs)
+    ctx.set_model(sv.vp0, sv.vs0, rho_m)
+    ctx.set_perturbation(dvp, dvs, region="s1" if hess else "s2")
     shots = [Shot(s.left, *g, s.ngx_left, s.gc, obs_z[k], obs_x[k]) for k, (s, g) in enumerate(zip(sv.shots, geo))]
     resident = [Shot(s.left, *g, s.ngx_left, s.gc) for s, g in zip(sv.shots, geo)]
+
+    def solve(sh):
+        if hess:
+            ctx.hessian_vector(sh)
+            return np.zeros(len(sh))
+        return ctx.batch_gradient(sh, born=True)
     for k in range(max(args.warmup, 1)):
         t0 = time.perf_counter()
-        ctx.batch_gradient(shots, born=True)
+        solve(shots)
         ctx.flush()
         print("bench: warmup %d %.3f s" % (k, time.perf_counter() - t0), file=sys.stderr, flush=True)
     ctx.set_timing(args.timing_every)
     times = []
     for k in range(args.steps):
         t0 = time.perf_counter()
-        m = ctx.batch_gradient(resident, born=True)
+        m = solve(resident)
         times.append(time.perf_counter() - t0)
         print("bench: step %d %.3f s" % (k, times[-1]), file=sys.stderr, flush=True)
     kstats = {kid: ctx.kernel_time(kid) for kid in range(_lib.K_COUNT)}
@@ -536,7 +553,7 @@ def born(args):
     if not args.no_cpu:
         from oracle import oracle as O
         ncpu = max(1, min(args.cpu_shots, nsh))
-        ctx.batch_gradient(shots[:ncpu], born=True)
+        solve(shots[:ncpu])
         g_vp, g_vs, _ = ctx.flush()
         threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
         r_vp = np.zeros((sv.allnx, sv.allnz))
@@ -545,8 +562,12 @@ def born(args):
         for s, g, sh in zip(sv.shots[:ncpu], geo[:ncpu], shots[:ncpu]):
             c = O.config(allnx=sv.allnx, allnz=sv.allnz, nx=sv.nx, nz=sv.allnz, pml=sv.pml, nt=sv.nt,
                          ntr=sv.ntr, dx=sv.dx, dz=sv.dz, dt=sv.dt, left=s.left, isx=g[0], isz=g[1],
-                         igz=g[2], ngx_left=s.ngx_left, nthreads=threads, rho_const=2000.0, adjoint_dir=-1)
-            a = O.born_gradient(c, sv.vp0, sv.vs0, sv.rho0, dvp, dvs, w, s.gc, sh.obs_z, sh.obs_x, "f64")
+                         igz=g[2], ngx_left=s.ngx_left, nthreads=threads, rho_const=0.0 if hess else 2000.0,
+                         adjoint_dir=1 if hess else -1)
+            if hess:
+                a = O.hessian_vector(c, sv.vp0, sv.vs0, sv.rho0, dvp, dvs, w, s.gc, "f64")
+            else:
+                a = O.born_gradient(c, sv.vp0, sv.vs0, sv.rho0, dvp, dvs, w, s.gc, sh.obs_z, sh.obs_x, "f64")
             r_vp += a[0]
             r_vs += a[1]
         tc = time.perf_counter() - t0
@@ -555,22 +576,26 @@ def born(args):
             return float(np.linalg.norm(x.astype(np.float64) - y) / np.linalg.norm(y))
 
         acc = {"grad_vp": rel(g_vp, r_vp), "grad_vs": rel(g_vs, r_vs), "shots": ncpu, "bar": 1e-3}
-        cpu = {"value": round(ncpu / tc, 5), "unit": "shot-gradients/s", "cores": threads, "kind": "port",
-               "sample": "FP64 oracle Born gradient, %d of the %d shots of the same workload, %.1f s" % (
-                   ncpu, nsh, tc)}
+        unit = "shot-products/s" if hess else "shot-gradients/s"
+        cpu = {"value": round(ncpu / tc, 5), "unit": unit, "cores": threads, "kind": "port",
+               "sample": "FP64 oracle %s, %d of the %d shots of the same workload, %.1f s" % (
+                   "Hessian-vector product" if hess else "Born gradient", ncpu, nsh, tc)}
     total = sum(times)
     lat = sorted(times)
     print(json.dumps({
-        "metric": "stage-2 (Born) shot-gradients/s", "value": round(nsh * args.steps / total, 4),
-        "unit": "shot-gradients/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "metric": ("stage-1 Gauss-Newton Hessian-vector shot-products/s" if hess
+                   else "stage-2 (Born) shot-gradients/s"), "value": round(nsh * args.steps / total, 4),
+        "unit": "shot-products/s" if hess else "shot-gradients/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup,
         "ms_per_step": round(1e3 * total / args.steps, 2), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded model; perturbation = true - initial model; observed traces from a forward "
                 "solve of the true model)",
         "config": {"workload": "%s: %dx%d dx=dz=%gm, pml %d, dt %gms, nt %d, %d shots, %g Hz, %d receivers, "
-                               "batched Born gradient, density 2000" % (sv.name, sv.nx, sv.allnz, sv.dx, sv.pml,
-                                                                       sv.dt * 1e3, sv.nt, nsh, sv.f0, sv.ntr),
-                   "mode": args.mode, "storage": "boundary strips %s" % args.storage.upper()},
+                               "%s" % (sv.name, sv.nx, sv.allnz, sv.dx, sv.pml, sv.dt * 1e3, sv.nt, nsh, sv.f0,
+                                       sv.ntr, "batched Hessian-vector product p = true - initial model" if hess
+                                       else "batched Born gradient, density 2000"),
+                   "mode": args.mode, "storage": "boundary strips %s" % storage.upper()},
         "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 2),
                        "p99": round(1e3 * float(np.percentile(lat, 99)), 2), "unit": "per batch step"},
         "misfit": float(m.sum()), "grad_err_vs_fp64": acc, "cpu_baseline": cpu, "kernels": kernels}), flush=True)

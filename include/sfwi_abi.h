@@ -184,8 +184,16 @@ int sfwi_set_wavelet(sfwi_ctx *ctx, const float *wavelet);
  * (sfwi-stage1/FD2DGPU.cu:1322-1327).  rho may be NULL when rho_const > 0. */
 int sfwi_set_model(sfwi_ctx *ctx, const float *vp, const float *vs, const float *rho);
 /* Model perturbation for Born modeling: FD2DGPU_ELASTIC::GPUbufferM
- * (sfwi-stage1/FD2DGPU.cu:1330-1334). */
+ * (sfwi-stage1/FD2DGPU.cu:1330-1334) and its window (buffer_image_extrapolation,
+ * :1336-1346).  The two stages' Born sources act on different cells
+ * (add_born_source_components): stage 2 on every cell at least nop from the
+ * grid edge (sfwi-stage2/born.cu:749-755), stage 1 on the interior only, at
+ * least pml from the edge (sfwi-stage1/born.cu:830-836).  sfwi_set_perturbation
+ * selects the stage-2 cells, sfwi_set_perturbation_region either. */
+#define SFWI_BORN_S2 0
+#define SFWI_BORN_S1 1
 int sfwi_set_perturbation(sfwi_ctx *ctx, const float *dvp, const float *dvs);
+int sfwi_set_perturbation_region(sfwi_ctx *ctx, const float *dvp, const float *dvs, int region);
 
 /* Stage-1 per-shot gradient: IMAGE_SINGLESHOT (sfwi-stage1/FD2DGPU.cu:4321-4695,
  * declared sfwi-stage1/FD2DGPU.cuh:466-467).  Adds the L2 misfit to *misfit and
@@ -215,6 +223,36 @@ int sfwi_batch_forward(sfwi_ctx *ctx, int nshots, const sfwi_shot *shots, float 
  * and background traces bg_* (either pair may be NULL). */
 int sfwi_shot_born_forward(sfwi_ctx *ctx, const sfwi_shot *shot, float *b_z, float *b_x,
                            float *bg_z, float *bg_x);
+
+/* Born forward solve with double scattering (born_modeling mode 1 then mode
+ * 5 each step, sfwi-stage1/FD2DGPU.cu:1855-1860, 1888-1903; the reference
+ * keeps the mode-5 calls commented out, e.g. :4742, :5307): the second-order
+ * scattered field is driven by the Born source of the updated first-order
+ * scattered velocity.  b_*: first-order scattered traces (record_scatter),
+ * b2_*: second-order ones (record_scatter with double_scatter, :3610-3617);
+ * the reference's Lm + Lm2 (scatter_add, :3236-3241) is b + b2.  Traces of
+ * shot i at b_z + i*ntr_max*nt (ntr_max = sfwi_params.ntr); any pointer may
+ * be NULL.  Uses the current perturbation and its region; per-cell kernels. */
+int sfwi_batch_born2_forward(sfwi_ctx *ctx, int nshots, const sfwi_shot *shots, float *b_z, float *b_x,
+                             float *b2_z, float *b2_x);
+
+/* Gauss-Newton Hessian-vector product, ELSRTM_CALCULATE_Hessian_vector
+ * (sfwi-stage1/FD2DGPU.cu:5236-5359, declared sfwi-stage1/FD2DGPU.cuh:477-478),
+ * called per shot by the inner CG loop of sfwi-stage1/main.cpp:846-935.  The
+ * vector p (p_vp, p_vs: allnx*allnz) becomes the perturbation with the stage-1
+ * region (as the reference copies it into image_pp_m / image_ps_m); NULL keeps
+ * the current perturbation.  Per shot: stage-1 Born forward of p with the
+ * background's reconstruction state saved (strips, or snapshots with
+ * SFWI_RECON_SNAPSHOTS), the scattered traces injected as the adjoint source,
+ * reverse reconstruction and adjoint, Vp / Vs imaging without illumination
+ * (hessian_vector_product, :3469-3473).  H p is accumulated into the Vp / Vs
+ * gradient accumulators until sfwi_flush (illumination and density untouched).
+ * With SFWI_RECON_SNAPSHOTS, snap_every = 1 and FP32 storage it is
+ * 2 J^T J p, J the Jacobian of the traces with respect to (vp, vs), up to
+ * rounding: symmetric. */
+int sfwi_shot_hessian_vector(sfwi_ctx *ctx, const sfwi_shot *shot, const float *p_vp, const float *p_vs);
+int sfwi_batch_hessian_vector(sfwi_ctx *ctx, int nshots, const sfwi_shot *shots, const float *p_vp,
+                              const float *p_vs);
 
 /* Stage-1 line search, CALCULATE_ALPHA_parabola_mb (sfwi-stage1/FD2DGPU.cu:4819-5028,
  * declared sfwi-stage1/FD2DGPU.cuh:471-472): adds to m[0..2] the misfits of
@@ -418,6 +456,26 @@ int sfwi_shot_from_table(const int row[9], float dx, int pml, int sy, int gy, in
 int sfwi_s1_direction(int allnx, int allnz, int iter, int cut_layer, const float *pp_grad,
                       const float *ps_grad, const float *illum, float *pp_cg_old, float *ps_cg_old,
                       double *beta_fenmu, float *dir_pp, float *dir_ps, double *beta);
+/* Stage-1 Gauss-Newton inner CG (sfwi-stage1/main.cpp:784-1063; it runs
+ * when max_inner > 0, and the reference ships max_inner = 0, :408).  The
+ * caller owns the state: r, p, delta, grad (pp and ps arrays, allnx*allnz
+ * each).  sfwi_s1_hcg_init (:786-839): r = normalize_all(cg), p = -r,
+ * grad = normalize_all(cg), delta = normalize_all(-cg), cg the CG direction
+ * of the iteration (pp_cg_old / ps_cg_old after sfwi_s1_direction).  Per inner
+ * iteration the caller sums H p over the shots (sfwi_batch_hessian_vector
+ * with p, then sfwi_flush) and calls sfwi_s1_hcg_step (:957-1024): with
+ * cut_grad, Hp rows z < cut_layer zeroed (in hp_*); alpha = sum r.r / sum p.Hp;
+ * delta += alpha p; grad = -delta; r += alpha Hp; beta = sum r.r (new) /
+ * sum r.r (old); p = -r + beta p.  rr_old and rr_new (the reference's
+ * hessian_misfit before and after), alpha and beta may be NULL.
+ * sfwi_s1_hcg_finish (:1057-1063) normalizes delta and grad; grad is then the
+ * search direction of the line search and the update (:1180, 1252). */
+int sfwi_s1_hcg_init(int allnx, int allnz, const float *pp_cg, const float *ps_cg, float *r_pp, float *r_ps,
+                     float *p_pp, float *p_ps, float *delta_pp, float *delta_ps, float *grad_pp, float *grad_ps);
+int sfwi_s1_hcg_step(int allnx, int allnz, int cut_grad, int cut_layer, float *hp_pp, float *hp_ps, float *r_pp,
+                     float *r_ps, float *p_pp, float *p_ps, float *delta_pp, float *delta_ps, float *grad_pp,
+                     float *grad_ps, double *rr_old, double *rr_new, double *alpha, double *beta);
+int sfwi_s1_hcg_finish(int allnx, int allnz, float *delta_pp, float *delta_ps, float *grad_pp, float *grad_ps);
 /* Stage-1 step length from the three line-search misfits at 0, step1, step2
  * (cal_step2, sfwi-stage1/main.cpp:65-72), clamped to [-100, 300] (1229-1238). */
 double sfwi_s1_alpha(double m0, double m1, double m2, float step1, float step2);

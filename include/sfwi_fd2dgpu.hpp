@@ -10,6 +10,7 @@
 //   IMAGE_SINGLESHOT (stage 2)     sfwi-stage2/FD2DGPU.cuh:455-456
 //   CALCULATE_ALPHA_parabola_mb    sfwi-stage1/FD2DGPU.cuh:471-472
 //   CALCULATE_ALPHA                sfwi-stage2/FD2DGPU.cuh:457-458
+//   ELSRTM_CALCULATE_Hessian_vector sfwi-stage1/FD2DGPU.cuh:477-478
 //
 // Differences a maintainer sees:
 //   * the constructor takes one more, defaulted argument: the stage (1 or 2),
@@ -247,6 +248,20 @@ inline void CALCULATE_ALPHA(FD2DGPU_ELASTIC &fd, const int /*sxnum*/, const int 
     fd.use_boundary(rbc);   // forward solves only: no strips, cpu_mem has no effect
     const sfwi_shot s = fd.shot(record, record_x, gc);
     check(sfwi_linesearch_linear(fd.ctx(), &s, cpu_pp_grad_old, cpu_ps_grad_old, &fenzi, &fenmu), fd.ctx());
+}
+
+// Gauss-Newton Hessian-vector product of one shot (sfwi-stage1/FD2DGPU.cuh:477-478),
+// the inner CG's per-shot call (sfwi-stage1/main.cpp:928): H p for
+// p = (grad_pp, grad_ps) is accumulated with the gradients and read back by
+// the flush (imagebuffer_resettozero), as in the reference; the observed
+// records are not used.
+inline void ELSRTM_CALCULATE_Hessian_vector(FD2DGPU_ELASTIC &fd, const int /*sxnum*/, const int /*nDim*/,
+                                            const int /*myid*/, float *record, float *record_x, int *gc, bool rbc,
+                                            bool cpu_mem, int ntt, int /*iter*/, float *grad_pp, float *grad_ps) {
+    fd.use_host_strips(cpu_mem, ntt, rbc);
+    const sfwi_shot s = fd.shot(record, record_x, gc);
+    check(sfwi_shot_hessian_vector(fd.ctx(), &s, grad_pp, grad_ps), fd.ctx());
+    fd.mark_accumulated();
 }
 
 }  // namespace sfwi_ref

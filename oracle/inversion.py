@@ -110,3 +110,67 @@ def s2_update(fenzi, fenmu, pp_cg, ps_cg, image_pp, image_ps):
     ip = (np.asarray(image_pp, F32).astype(np.float64) - alpha * np.asarray(pp_cg, F32).astype(np.float64)).astype(F32)
     is_ = (np.asarray(image_ps, F32).astype(np.float64) - alpha * np.asarray(ps_cg, F32).astype(np.float64)).astype(F32)
     return ip, is_, alpha
+
+
+# ------------------------------------------- stage-1 Gauss-Newton inner CG ---
+# sfwi-stage1/main.cpp:784-1063 (run when max_inner > 0).  dt = float32 is the
+# reference's arithmetic; dt = float64 runs the same steps in FP64 (the pins).
+
+def _seqsum_dt(t, dt):
+    t = np.ascontiguousarray(t, dt).ravel().astype(np.float64)
+    return float(np.cumsum(t)[-1]) if t.size else 0.0
+
+
+def _normalize_dt(a, b, dt):
+    if dt == F32:
+        return normalize_all(a, b)
+    mx = max(float(np.abs(a).max()), float(np.abs(b).max()))
+    return np.asarray(a, dt) / mx, np.asarray(b, dt) / mx
+
+
+def s1_hcg_init(pp_cg, ps_cg, dt=F32):
+    """Set-up of the inner loop (main.cpp:786-839): returns the state dict with
+    r = normalize_all(cg), p = -r, grad = normalize_all(cg),
+    delta = normalize_all(-cg)."""
+    c1, c2 = np.asarray(pp_cg, dt), np.asarray(ps_cg, dt)
+    r = _normalize_dt(c1, c2, dt)
+    p = (-r[0], -r[1])
+    delta = _normalize_dt(-c1, -c2, dt)
+    grad = _normalize_dt(c1, c2, dt)
+    return dict(r=r, p=p, delta=delta, grad=grad)
+
+
+def s1_hcg_step(st, hp_pp, hp_ps, cut_grad, cut_layer, dt=F32, plant=0):
+    """One inner iteration's host step (main.cpp:957-1024) on the state dict
+    (updated in place); returns (rr_old, rr_new, alpha, beta).  plant (test
+    hook): 1 takes beta with the wrong sign."""
+    h1, h2 = np.array(hp_pp, dt), np.array(hp_ps, dt)
+    if cut_grad:
+        h1[:, :cut_layer] = 0
+        h2[:, :cut_layer] = 0
+    r1, r2 = st["r"]
+    p1, p2 = st["p"]
+    d1, d2 = st["delta"]
+    a_num = _seqsum_dt((r1 * r1).astype(dt) + (r2 * r2).astype(dt), dt)
+    a_den = _seqsum_dt((p1 * h1).astype(dt) + (p2 * h2).astype(dt), dt)
+    alpha = a_num / a_den
+    f = np.float64
+    d1 = (d1.astype(f) + alpha * p1.astype(f)).astype(dt)
+    d2 = (d2.astype(f) + alpha * p2.astype(f)).astype(dt)
+    r1 = (r1.astype(f) + alpha * h1.astype(f)).astype(dt)
+    r2 = (r2.astype(f) + alpha * h2.astype(f)).astype(dt)
+    b_num = _seqsum_dt((r1 * r1).astype(dt) + (r2 * r2).astype(dt), dt)
+    beta = b_num / a_num
+    if plant & 1:
+        beta = -beta
+    p1 = (-r1.astype(f) + beta * p1.astype(f)).astype(dt)
+    p2 = (-r2.astype(f) + beta * p2.astype(f)).astype(dt)
+    st.update(r=(r1, r2), p=(p1, p2), delta=(d1, d2), grad=(-d1, -d2))
+    return a_num, b_num, alpha, beta
+
+
+def s1_hcg_finish(st, dt=F32):
+    """main.cpp:1057-1063: delta and grad normalized; grad is the search direction."""
+    st["delta"] = _normalize_dt(*st["delta"], dt)
+    st["grad"] = _normalize_dt(*st["grad"], dt)
+    return st

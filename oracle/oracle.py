@@ -82,6 +82,13 @@ def lib():
                                            _f32p, _f32p, _f32p, _f32p, _f32p, dp]
         L.or_born_gradient_f64.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _i32p,
                                            _f32p, _f32p, _f64p, _f64p, _f64p, dp]
+        for prec, fp in (("f32", _f32p), ("f64", _f64p)):
+            getattr(L, "or_born2_forward_" + prec).argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p,
+                                                               _i32p, fp, fp, fp, fp]
+            getattr(L, "or_hessian_vector_" + prec).argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p,
+                                                                _i32p, fp, fp]
+        L.or_hessian_vector_snapshot_f64.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _i32p,
+                                                     _f64p, _f64p]
         L.or_parabola_misfits_f32.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _f32p,
                                               ctypes.c_float, ctypes.c_float, _f32p, _i32p,
                                               _f32p, _f32p, _f64p]
@@ -242,6 +249,40 @@ def born_gradient(c, vp, vs, rho, dvp, dvs, wav, gc, obs_z, obs_x, prec="f32"):
                                                _f32(obs_z), _f32(obs_x), g1, g2, g3, ctypes.byref(m))
     sh = (c.allnx, c.allnz)
     return g1.reshape(sh), g2.reshape(sh), g3.reshape(sh), m.value
+
+
+def born2_forward(c, vp, vs, rho, dvp, dvs, wav, gc, prec="f32"):
+    """Stage-1 Born forward with double scattering: (b_z, b_x, b2_z, b2_x),
+    the first- and second-order scattered traces."""
+    dt = np.float32 if prec == "f32" else np.float64
+    out = [np.zeros((c.ntr, c.nt), dt) for _ in range(4)]
+    getattr(lib(), "or_born2_forward_" + prec)(ctypes.byref(c), _f32(vp), _f32(vs), _f32(rho),
+                                               _f32(dvp), _f32(dvs), _f32(wav),
+                                               np.ascontiguousarray(gc, np.int32), *out)
+    return tuple(out)
+
+
+def hessian_vector(c, vp, vs, rho, p_vp, p_vs, wav, gc, prec="f32"):
+    """Gauss-Newton Hessian-vector product of one shot (the reference's
+    strip reconstruction): (Hp_vp, Hp_vs) on the allnx x allnz model."""
+    dt = np.float32 if prec == "f32" else np.float64
+    n = c.allnx * c.allnz
+    g1, g2 = np.zeros(n, dt), np.zeros(n, dt)
+    getattr(lib(), "or_hessian_vector_" + prec)(ctypes.byref(c), _f32(vp), _f32(vs), _f32(rho), _f32(p_vp),
+                                                _f32(p_vs), _f32(wav), np.ascontiguousarray(gc, np.int32), g1, g2)
+    sh = (c.allnx, c.allnz)
+    return g1.reshape(sh), g2.reshape(sh)
+
+
+def hessian_vector_snapshot(c, vp, vs, rho, p_vp, p_vs, wav, gc):
+    """Hessian-vector product through the snapshot (consistent) gradient,
+    FP64, imaging every c.snap_every-th step: (Hp_vp, Hp_vs)."""
+    n = c.allnx * c.allnz
+    g1, g2 = np.zeros(n, np.float64), np.zeros(n, np.float64)
+    lib().or_hessian_vector_snapshot_f64(ctypes.byref(c), _f32(vp), _f32(vs), _f32(rho), _f32(p_vp), _f32(p_vs),
+                                         _f32(wav), np.ascontiguousarray(gc, np.int32), g1, g2)
+    sh = (c.allnx, c.allnz)
+    return g1.reshape(sh), g2.reshape(sh)
 
 
 def parabola_misfits(c, vp, vs, rho, dir_vp, dir_vs, step1, step2, wav, gc, obs_z, obs_x):

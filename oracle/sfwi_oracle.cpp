@@ -408,17 +408,23 @@ void adjoint_v(const Grid &g, const Steps<T> &st, const Medium<T> &m, Wave<T> &w
 }
 
 // Born virtual source into the scattered split stresses from the updated
-// background velocity, stage-2 live kernel sfwi-stage2/born.cu:749-801
-// (add_born_source_components; bounds [nop, n-nop), quirk Q9).
+// velocity of the field it scatters from (the background, or the first-order
+// scattered field for the second order).  Stage-2 live kernel
+// sfwi-stage2/born.cu:749-801 (add_born_source_components; bounds
+// [nop, n - nop), quirk Q9): lo = NOP.  Stage 1 (sfwi-stage1/born.cu:830-880,
+// the Born solves of the Hessian-vector product and of double scattering):
+// bounds [pml, n - pml), the interior only: lo = pml.  plant & 16 (test hook)
+// drops the -4 rho vs dvs term of txx_z and tzz_x.
 template <typename T>
 void born_source(const Grid &g, const Steps<T> &st, const Medium<T> &m, const Wave<T> &bg, Wave<T> &b,
-                 int dir, int nth) {
+                 int dir, int nth, int lo = NOP, int plant = 0) {
     const long S = g.nzp;
     const T *vx = bg.vx.data(), *vz = bg.vz.data();
     const T D = (T)dir;
+    const T k4 = (plant & 16) ? T(0) : T(4);
 #pragma omp parallel for schedule(static) num_threads(nth)
-    for (int ix = NOP; ix < g.nxp - NOP; ++ix)
-        for (int iz = NOP; iz < g.nzp - NOP; ++iz) {
+    for (int ix = lo; ix < g.nxp - lo; ++ix)
+        for (int iz = lo; iz < g.nzp - lo; ++iz) {
             const long i = (long)ix * S + iz;
             T s_vx_x = d_minus(vx, i, S);
             T s_vz_z = d_minus(vz, i, 1);
@@ -426,8 +432,8 @@ void born_source(const Grid &g, const Steps<T> &st, const Medium<T> &m, const Wa
             T s_vz_x = d_plus(vz, i, S);
             const T rho = m.rho[i], vp = m.vp[i], vs = m.vs[i], dp = m.dvp[i], ds = m.dvs[i];
             b.txx_x[i] = b.txx_x[i] + D * st.dt * (2 * (rho * vp) * (dp) * (s_vx_x) / st.dx);
-            b.txx_z[i] = b.txx_z[i] + D * st.dt * ((2 * (rho * vp) * (dp) - 4 * (rho * vs) * (ds)) * (s_vz_z) / st.dz);
-            b.tzz_x[i] = b.tzz_x[i] + D * st.dt * ((2 * (rho * vp) * (dp) - 4 * (rho * vs) * (ds)) * (s_vx_x) / st.dx);
+            b.txx_z[i] = b.txx_z[i] + D * st.dt * ((2 * (rho * vp) * (dp) - k4 * (rho * vs) * (ds)) * (s_vz_z) / st.dz);
+            b.tzz_x[i] = b.tzz_x[i] + D * st.dt * ((2 * (rho * vp) * (dp) - k4 * (rho * vs) * (ds)) * (s_vx_x) / st.dx);
             b.tzz_z[i] = b.tzz_z[i] + D * st.dt * (2 * (rho * vp) * (dp) * (s_vz_z) / st.dz);
             b.txz_x[i] = b.txz_x[i] + D * st.dt * ((2 * (rho * vs) * (ds)) * ((s_vz_x) / st.dx));
             b.txz_z[i] = b.txz_z[i] + D * st.dt * ((2 * (rho * vs) * (ds)) * ((s_vx_z) / st.dz));
@@ -635,17 +641,31 @@ template <typename T> struct Run {
     // Stage-2 forward, mode 1 (sfwi-stage2/FD2DGPU.cu:1814-1845) inside
     // sfwi-stage2/FD2DGPU.cu:4223-4314: background step, scattered velocity,
     // Born source from the updated background velocity, scattered stress.
+    // lo: the Born source's bound (NOP stage 2, pml stage 1).  With b2z / b2x
+    // (stage 1, double scattering) each step continues with mode 5
+    // (sfwi-stage1/FD2DGPU.cu:1888-1903): the second-order scattered velocity,
+    // the Born source from the updated first-order scattered velocity, the
+    // second-order scattered stress; its traces are recorded as well
+    // (record_scatter with double_scatter, :3610-3617).  plant & 32 (test
+    // hook) drives the second order from the background instead.
     void born_forward(const float *wav, std::vector<T> &bz, std::vector<T> &bx, std::vector<T> &gz,
-                      std::vector<T> &gx, Wave<T> *keep = nullptr, Strips<T> *strips = nullptr) {
-        Wave<T> w(g.n), b(g.n);
+                      std::vector<T> &gx, Wave<T> *keep = nullptr, Strips<T> *strips = nullptr, int lo = NOP,
+                      std::vector<T> *b2z = nullptr, std::vector<T> *b2x = nullptr) {
+        Wave<T> w(g.n), b(g.n), b2(b2z ? g.n : 1);
         if (c->rbc == 1) strips = nullptr;
         for (int it = 0; it < g.nt; ++it) {
             add_source(w, wav, it);
             update_v(g, st, m, w, +1, nth);
             update_s(g, st, m, w, +1, nth);
             update_v(g, st, m, b, +1, nth);
-            born_source(g, st, m, w, b, +1, nth);
+            born_source(g, st, m, w, b, +1, nth, lo, c->plant);
             update_s(g, st, m, b, +1, nth);
+            if (b2z) {
+                update_v(g, st, m, b2, +1, nth);
+                born_source(g, st, m, (c->plant & 32) ? w : b, b2, +1, nth, lo, c->plant);
+                update_s(g, st, m, b2, +1, nth);
+                record(b2, it, *b2z, *b2x);
+            }
             record(b, it, bz, bx);   // record_scatter: scatter = b, scatter2 = background
             record(w, it, gz, gx);
             if (strips) strips->template move<true>(w.vx.data(), w.vz.data(), it, g.nt);
@@ -738,6 +758,28 @@ void born_gradient_impl(const or_config *c, const float *vp, const float *vs, co
     }
     *misfit += l2_sum(bz, bx, c->ntr, c->nt);
     r.reverse(fw, strips, bz, bx, gvp, gvs, ill);
+}
+
+// Gauss-Newton Hessian-vector product of one shot, ELSRTM_CALCULATE_Hessian_vector
+// (sfwi-stage1/FD2DGPU.cu:5236-5359): the vector p is the perturbation of a
+// stage-1 Born forward solve (mode 1, source on the interior only) that
+// saves the background strips and records the scattered traces; the reverse
+// loop of IMAGE_SINGLESHOT then runs with those traces as the adjoint source
+// (overwrite injection, stack = false, :3425-3447) and images with
+// doimage_gradient alone (hessian_vector_product, :3469-3473: no
+// illumination).  Accumulates H p into gvp, gvs.
+template <typename T, typename G>
+void hessian_impl(const or_config *c, const float *vp, const float *vs, const float *rho, const float *pvp,
+                  const float *pvs, const float *wav, const int *gc, G *gvp, G *gvs) {
+    Run<T> r(c, vp, vs, rho, pvp, pvs, gc);
+    const long nr = (long)c->ntr * c->nt;
+    std::vector<T> bz(nr), bx(nr), gz(nr), gx(nr);
+    Strips<T> strips;
+    strips.init(r.g, c->plant);
+    Wave<T> fw(1);
+    r.born_forward(wav, bz, bx, gz, gx, &fw, &strips, c->pml);
+    std::vector<G> ill((size_t)c->allnx * c->allnz, G(0));   // not part of the product
+    r.reverse(fw, strips, bz, bx, gvp, gvs, ill.data());
 }
 
 }  // namespace
@@ -955,7 +997,8 @@ template <typename T>
 static void gradient_snapshot_impl(const or_config *c, const float *vp, const float *vs, const float *rho,
                                    const float *dvp, const float *dvs, const float *wav, const int *gc,
                                    const float *obs_z, const float *obs_x, const float *adj_z,
-                                   const float *adj_x, T *gvp, T *gvs, T *gil, double *misfit) {
+                                   const float *adj_x, T *gvp, T *gvs, T *gil, double *misfit,
+                                   bool hess = false) {
     const bool born = dvp && dvs;
     Run<T> r(c, vp, vs, rho, born ? dvp : nullptr, born ? dvs : nullptr, gc);
     const Grid &g = r.g;
@@ -985,14 +1028,17 @@ static void gradient_snapshot_impl(const or_config *c, const float *vp, const fl
         update_s(g, r.st, r.m, w, +1, r.nth);
         if (born) {
             update_v(g, r.st, r.m, b, +1, r.nth);
-            born_source(g, r.st, r.m, w, b, +1, r.nth);
+            born_source(g, r.st, r.m, w, b, +1, r.nth, hess ? g.pml : NOP, c->plant);
             update_s(g, r.st, r.m, b, +1, r.nth);
             r.record(b, it, bz, bx);
         }
         r.record(w, it, sz, sx);
     }
     for (long q = 0; q < nr; ++q) {   // residual (stage 2: born - obs + background)
-        if (born) {
+        if (hess) {                   // Hessian-vector product: the scattered traces
+            sz[q] = bz[q];
+            sx[q] = bx[q];
+        } else if (born) {
             sz[q] = (bz[q] - (T)obs_z[q]) + sz[q];
             sx[q] = (bx[q] - (T)obs_x[q]) + sx[q];
         } else {
@@ -1000,7 +1046,7 @@ static void gradient_snapshot_impl(const or_config *c, const float *vp, const fl
             sx[q] = sx[q] - (T)obs_x[q];
         }
     }
-    *misfit += l2_sum(sz, sx, c->ntr, c->nt);
+    if (!hess) *misfit += l2_sum(sz, sx, c->ntr, c->nt);
     Wave<T> bk(n);
     std::vector<T> fvx(n, T(0)), fvz(n, T(0));
     const size_t nm = (size_t)c->allnx * c->allnz;
@@ -1047,7 +1093,7 @@ static void gradient_snapshot_impl(const or_config *c, const float *vp, const fl
     for (size_t q = 0; q < nm; ++q) {
         gvp[q] += wk * ivp[q];
         gvs[q] += wk * ivs[q];
-        gil[q] += (T)K * ill[q];
+        if (gil) gil[q] += (T)K * ill[q];
     }
 }
 
@@ -1063,6 +1109,13 @@ void or_gradient_snapshot_f64(const or_config *c, const float *vp, const float *
                               double *gvp, double *gvs, double *gil, double *misfit) {
     gradient_snapshot_impl<double>(c, vp, vs, rho, dvp, dvs, wav, gc, obs_z, obs_x, adj_z, adj_x, gvp, gvs, gil,
                                    misfit);
+}
+void or_hessian_vector_snapshot_f64(const or_config *c, const float *vp, const float *vs, const float *rho,
+                                    const float *p_vp, const float *p_vs, const float *wav, const int *gc,
+                                    double *gvp, double *gvs) {
+    double unused = 0.0;
+    gradient_snapshot_impl<double>(c, vp, vs, rho, p_vp, p_vs, wav, gc, nullptr, nullptr, nullptr, nullptr, gvp,
+                                   gvs, nullptr, &unused, true);
 }
 }  // extern "C"
 
@@ -1120,7 +1173,41 @@ static void born_forward_impl(const or_config *c, const float *vp, const float *
     std::memcpy(bg_x, gx.data(), nr * sizeof(T));
 }
 
+template <typename T>
+static void born2_forward_impl(const or_config *c, const float *vp, const float *vs, const float *rho,
+                               const float *dvp, const float *dvs, const float *wav, const int *gc,
+                               T *b_z, T *b_x, T *b2_z, T *b2_x) {
+    Run<T> r(c, vp, vs, rho, dvp, dvs, gc);
+    const long nr = (long)c->ntr * c->nt;
+    std::vector<T> bz(nr), bx(nr), gz(nr), gx(nr), cz(nr), cx(nr);
+    r.born_forward(wav, bz, bx, gz, gx, nullptr, nullptr, c->pml, &cz, &cx);
+    std::memcpy(b_z, bz.data(), nr * sizeof(T));
+    std::memcpy(b_x, bx.data(), nr * sizeof(T));
+    std::memcpy(b2_z, cz.data(), nr * sizeof(T));
+    std::memcpy(b2_x, cx.data(), nr * sizeof(T));
+}
+
 extern "C" {
+void or_born2_forward_f32(const or_config *c, const float *vp, const float *vs, const float *rho,
+                          const float *dvp, const float *dvs, const float *wav, const int *gc,
+                          float *b_z, float *b_x, float *b2_z, float *b2_x) {
+    born2_forward_impl<float>(c, vp, vs, rho, dvp, dvs, wav, gc, b_z, b_x, b2_z, b2_x);
+}
+void or_born2_forward_f64(const or_config *c, const float *vp, const float *vs, const float *rho,
+                          const float *dvp, const float *dvs, const float *wav, const int *gc,
+                          double *b_z, double *b_x, double *b2_z, double *b2_x) {
+    born2_forward_impl<double>(c, vp, vs, rho, dvp, dvs, wav, gc, b_z, b_x, b2_z, b2_x);
+}
+void or_hessian_vector_f32(const or_config *c, const float *vp, const float *vs, const float *rho,
+                           const float *p_vp, const float *p_vs, const float *wav, const int *gc,
+                           float *gvp, float *gvs) {
+    hessian_impl<float, float>(c, vp, vs, rho, p_vp, p_vs, wav, gc, gvp, gvs);
+}
+void or_hessian_vector_f64(const or_config *c, const float *vp, const float *vs, const float *rho,
+                           const float *p_vp, const float *p_vs, const float *wav, const int *gc,
+                           double *gvp, double *gvs) {
+    hessian_impl<double, double>(c, vp, vs, rho, p_vp, p_vs, wav, gc, gvp, gvs);
+}
 void or_born_forward_f32(const or_config *c, const float *vp, const float *vs, const float *rho,
                          const float *dvp, const float *dvs, const float *wav, const int *gc,
                          float *b_z, float *b_x, float *bg_z, float *bg_x) {

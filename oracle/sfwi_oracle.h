@@ -30,9 +30,14 @@
  * imaged interior at every step, and a window one cell off fails), the source
  * timing and scale (closed form of the first sample), the reconstruction
  * without source subtraction (linearity against the exact inverse, which
- * time reversal pins).  Parity unpinned (restatement of the cited lines
- * only): the imaging position in the reverse loop (after the adjoint
- * stress-like half step) and the stage-2 density override.
+ * time reversal pins).  The Hessian-vector product and double scattering
+ * (tests/test_hessian_oracle.py): the consistent product's symmetry, its
+ * quadratic form against the Born energy, central differences of the
+ * consistent gradient, the reference's product as the gradient loop on the
+ * scattered traces, the stage-1 Born cells, and double scattering as the
+ * second-order term of the traces' expansion.  Parity unpinned (restatement
+ * of the cited lines only): the imaging position in the reverse loop (after
+ * the adjoint stress-like half step) and the stage-2 density override.
  *
  * Random boundary (rbc): the reference draws it from cuRAND states seeded with
  * time(0) (sfwi-stage1/FD2DGPU.cu:155-230, 3245-3305), so no run repeats it;
@@ -74,7 +79,9 @@ typedef struct or_config {
                           /*   catch -- 1: the up strips one row deeper, 2: the left      */
                           /*   strips one column further in, 4: the forward source after  */
                           /*   the velocity update, 8: the reconstruction undoes the      */
-                          /*   source (the exact inverse)                                 */
+                          /*   source (the exact inverse), 16: the Born source without    */
+                          /*   the -4 rho vs dvs term, 32: the second-order scattered     */
+                          /*   field driven by the background instead of the first order */
 } or_config;
 
 /* FD coefficients c1..c4 of the 8th-order staggered stencil (index 0 unused). */
@@ -165,6 +172,42 @@ void or_born_gradient_f64(const or_config *c, const float *vp, const float *vs, 
                           const float *dvp, const float *dvs, const float *wav, const int *gc,
                           const float *obs_z, const float *obs_x,
                           double *grad_vp, double *grad_vs, double *illum, double *misfit);
+
+/* Stage-1 Born forward with double scattering (dead code of the reference,
+ * kept as its own commented-out calls place it): born_modeling mode 1 then
+ * mode 5 each step (sfwi-stage1/FD2DGPU.cu:1855-1860, 1888-1903), both Born
+ * sources on the interior only (sfwi-stage1/born.cu:830-880), the second-order
+ * field driven by the updated first-order scattered velocity.  b_*: first-order
+ * scattered traces, b2_*: second-order ones (record_scatter with
+ * double_scatter, :3610-3617). */
+void or_born2_forward_f32(const or_config *c, const float *vp, const float *vs, const float *rho,
+                          const float *dvp, const float *dvs, const float *wav, const int *gc,
+                          float *b_z, float *b_x, float *b2_z, float *b2_x);
+void or_born2_forward_f64(const or_config *c, const float *vp, const float *vs, const float *rho,
+                          const float *dvp, const float *dvs, const float *wav, const int *gc,
+                          double *b_z, double *b_x, double *b2_z, double *b2_x);
+
+/* Gauss-Newton Hessian-vector product of one shot, ELSRTM_CALCULATE_Hessian_vector
+ * (sfwi-stage1/FD2DGPU.cu:5236-5359): stage-1 Born forward of the vector p
+ * (interior source) with background strips saved and scattered traces
+ * recorded; the gradient's reverse loop with those traces as the adjoint
+ * source; Vp / Vs imaging without illumination.  Accumulates H p into
+ * gvp, gvs (allnx*allnz). */
+void or_hessian_vector_f32(const or_config *c, const float *vp, const float *vs, const float *rho,
+                           const float *p_vp, const float *p_vs, const float *wav, const int *gc,
+                           float *gvp, float *gvs);
+void or_hessian_vector_f64(const or_config *c, const float *vp, const float *vs, const float *rho,
+                           const float *p_vp, const float *p_vs, const float *wav, const int *gc,
+                           double *gvp, double *gvs);
+/* The same product through the snapshot gradient (FP64, imaging every
+ * snap_every-th step, no reference counterpart): the adjoint source 2 b / rho
+ * of the scattered traces b.  With snap_every <= 1 and pml_off = 1 it is
+ * 2 J^T J p, J the Jacobian of the traces with respect to (vp, vs) on the
+ * interior: symmetric, and equal to central differences of the consistent
+ * gradient at a model whose own traces are the observed ones. */
+void or_hessian_vector_snapshot_f64(const or_config *c, const float *vp, const float *vs, const float *rho,
+                                    const float *p_vp, const float *p_vs, const float *wav, const int *gc,
+                                    double *gvp, double *gvs);
 
 /* Stage-1 line search: misfits at vel, vel - step1*dir, vel - step2*dir. */
 void or_parabola_misfits_f32(const or_config *c, const float *vp, const float *vs, const float *rho,

@@ -17,6 +17,7 @@ SFWI_OK, SFWI_EINVAL, SFWI_EHIP, SFWI_ENOMEM, SFWI_ESTATE = 0, -1, -2, -3, -4
 MODE_FAST, MODE_EXACT = 0, 1
 STORE_STRIPS_F32, STORE_STRIPS_F16, STORE_STRIPS_BF16 = 0, 1, 2
 MISFIT_L2, MISFIT_OT_TRACE, MISFIT_OT_POWER, MISFIT_OT_GATHER = 0, 1, 2, 3
+SFWI_BORN_S2, SFWI_BORN_S1 = 0, 1
 REQ_FORWARD, REQ_BORN_FORWARD, REQ_GRADIENT, REQ_BORN_GRADIENT = 0, 1, 2, 3
 RECON_STRIPS, RECON_SNAPSHOTS = 0, 1
 (K_FWD_VEL, K_FWD_STR, K_REC_STR, K_REC_VEL, K_ADJ_STR, K_ADJ_VEL, K_REC_STR_IMG, K_REC_VEL_ILL, K_ADJ_VEL_ONLY,
@@ -100,6 +101,10 @@ def load():
     L.sfwi_set_wavelet.argtypes = [vp, _f32p]
     L.sfwi_set_model.argtypes = [vp, vp, vp, vp]
     L.sfwi_set_perturbation.argtypes = [vp, _f32p, _f32p]
+    L.sfwi_set_perturbation_region.argtypes = [vp, _f32p, _f32p, ctypes.c_int]
+    L.sfwi_batch_born2_forward.argtypes = [vp, ctypes.c_int, P(SfwiShot), vp, vp, vp, vp]
+    L.sfwi_shot_hessian_vector.argtypes = [vp, P(SfwiShot), vp, vp]
+    L.sfwi_batch_hessian_vector.argtypes = [vp, ctypes.c_int, P(SfwiShot), vp, vp]
     L.sfwi_shot_gradient.argtypes = [vp, P(SfwiShot), dp]
     L.sfwi_shot_born_gradient.argtypes = [vp, P(SfwiShot), dp]
     L.sfwi_batch_gradient.argtypes = [vp, ctypes.c_int, P(SfwiShot), dp]
@@ -134,6 +139,9 @@ def load():
                                 P(ctypes.c_int), dp]
     i = ctypes.c_int
     L.sfwi_s1_direction.argtypes = [i, i, i, i, _f32p, _f32p, _f32p, _f32p, _f32p, dp, _f32p, _f32p, dp]
+    L.sfwi_s1_hcg_init.argtypes = [i, i, _f32p, _f32p] + [_f32p] * 8
+    L.sfwi_s1_hcg_step.argtypes = [i, i, i, i] + [_f32p] * 10 + [dp] * 4
+    L.sfwi_s1_hcg_finish.argtypes = [i, i] + [_f32p] * 4
     L.sfwi_s1_alpha.restype = ctypes.c_double
     L.sfwi_s1_alpha.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_float, ctypes.c_float]
     L.sfwi_s1_update.argtypes = [i, i, i, ctypes.c_double, _f32p, _f32p, _f32p, _f32p]

@@ -92,6 +92,94 @@ int sfwi_s1_direction(int allnx, int allnz, int iter, int cut_layer, const float
     return 0;
 }
 
+// Gauss-Newton inner CG of stage 1 (sfwi-stage1/main.cpp:784-1063, run when
+// max_inner > 0).  The state arrays are allnx*allnz FP32 each, in the order
+// r, p, delta, grad (pp then ps of each).
+int sfwi_s1_hcg_init(int allnx, int allnz, const float *pp_cg, const float *ps_cg, float *r_pp, float *r_ps,
+                     float *p_pp, float *p_ps, float *delta_pp, float *delta_ps, float *grad_pp, float *grad_ps) {
+    if (!inv_dims_ok(allnx, allnz)) return fail(nullptr, SFWI_EINVAL, "sfwi_s1_hcg_init: allnx=%d allnz=%d", allnx, allnz);
+    if (!pp_cg || !ps_cg || !r_pp || !r_ps || !p_pp || !p_ps || !delta_pp || !delta_ps || !grad_pp || !grad_ps)
+        return fail(nullptr, SFWI_EINVAL, "sfwi_s1_hcg_init: NULL array");
+    const size_t n = (size_t)allnx * allnz;
+    // r = cg normalized, p = -r (:809-822)
+    std::memcpy(r_pp, pp_cg, n * sizeof(float));
+    std::memcpy(r_ps, ps_cg, n * sizeof(float));
+    inv_normalize_all(r_pp, r_ps, n);
+    for (size_t k = 0; k < n; ++k) {
+        p_pp[k] = -r_pp[k];
+        p_ps[k] = -r_ps[k];
+    }
+    // grad = cg, delta = -grad, both normalized (:826-839)
+    std::memcpy(grad_pp, pp_cg, n * sizeof(float));
+    std::memcpy(grad_ps, ps_cg, n * sizeof(float));
+    for (size_t k = 0; k < n; ++k) {
+        delta_pp[k] = -grad_pp[k];
+        delta_ps[k] = -grad_ps[k];
+    }
+    inv_normalize_all(delta_pp, delta_ps, n);
+    inv_normalize_all(grad_pp, grad_ps, n);
+    return 0;
+}
+
+int sfwi_s1_hcg_step(int allnx, int allnz, int cut_grad, int cut_layer, float *hp_pp, float *hp_ps, float *r_pp,
+                     float *r_ps, float *p_pp, float *p_ps, float *delta_pp, float *delta_ps, float *grad_pp,
+                     float *grad_ps, double *rr_old, double *rr_new, double *alpha_out, double *beta_out) {
+    if (!inv_dims_ok(allnx, allnz) || cut_layer < 0)
+        return fail(nullptr, SFWI_EINVAL, "sfwi_s1_hcg_step: allnx=%d allnz=%d cut_layer=%d", allnx, allnz, cut_layer);
+    if (!hp_pp || !hp_ps || !r_pp || !r_ps || !p_pp || !p_ps || !delta_pp || !delta_ps || !grad_pp || !grad_ps)
+        return fail(nullptr, SFWI_EINVAL, "sfwi_s1_hcg_step: NULL array");
+    const size_t n = (size_t)allnx * allnz;
+    // Hp above the cut layer zeroed; alpha = r.r / p.Hp (:961-985)
+    double a_num = 0.0, a_den = 0.0;
+    for (int i = 0; i < allnx; ++i)
+        for (int j = 0; j < allnz; ++j) {
+            const size_t k = (size_t)i * allnz + j;
+            if (cut_grad && j < cut_layer) {
+                hp_pp[k] = 0.0f;
+                hp_ps[k] = 0.0f;
+            }
+            const float rr = r_pp[k] * r_pp[k] + r_ps[k] * r_ps[k];
+            a_num += rr;
+            const float ph = p_pp[k] * hp_pp[k] + p_ps[k] * hp_ps[k];
+            a_den += ph;
+        }
+    const double alpha = a_num / a_den;
+    // delta += alpha p, grad = -delta, r += alpha Hp, beta = r.r / r.r_old (:999-1017)
+    double b_num = 0.0;
+    for (int i = 0; i < allnx; ++i)
+        for (int j = 0; j < allnz; ++j) {
+            const size_t k = (size_t)i * allnz + j;
+            delta_pp[k] = (float)((double)delta_pp[k] + alpha * (double)p_pp[k]);
+            delta_ps[k] = (float)((double)delta_ps[k] + alpha * (double)p_ps[k]);
+            grad_pp[k] = -delta_pp[k];
+            grad_ps[k] = -delta_ps[k];
+            r_pp[k] = (float)((double)r_pp[k] + alpha * (double)hp_pp[k]);
+            r_ps[k] = (float)((double)r_ps[k] + alpha * (double)hp_ps[k]);
+            const float rr = r_pp[k] * r_pp[k] + r_ps[k] * r_ps[k];
+            b_num += rr;
+        }
+    const double beta = b_num / a_num;
+    // p = -r + beta p (:1019-1024)
+    for (size_t k = 0; k < n; ++k) {
+        p_pp[k] = (float)(-(double)r_pp[k] + beta * (double)p_pp[k]);
+        p_ps[k] = (float)(-(double)r_ps[k] + beta * (double)p_ps[k]);
+    }
+    if (rr_old) *rr_old = a_num;
+    if (rr_new) *rr_new = b_num;
+    if (alpha_out) *alpha_out = alpha;
+    if (beta_out) *beta_out = beta;
+    return 0;
+}
+
+int sfwi_s1_hcg_finish(int allnx, int allnz, float *delta_pp, float *delta_ps, float *grad_pp, float *grad_ps) {
+    if (!inv_dims_ok(allnx, allnz)) return fail(nullptr, SFWI_EINVAL, "sfwi_s1_hcg_finish: allnx=%d allnz=%d", allnx, allnz);
+    if (!delta_pp || !delta_ps || !grad_pp || !grad_ps) return fail(nullptr, SFWI_EINVAL, "sfwi_s1_hcg_finish: NULL array");
+    const size_t n = (size_t)allnx * allnz;
+    inv_normalize_all(delta_pp, delta_ps, n);
+    inv_normalize_all(grad_pp, grad_ps, n);
+    return 0;
+}
+
 double sfwi_s1_alpha(double m0, double m1, double m2, float step1, float step2) {
     const double fenzi = (step1 * step1 - step2 * step2) * m0 + (step2 * step2) * m1 + (-step1 * step1) * m2;
     const double fenmu = (step1 - step2) * m0 + (step2)*m1 + (-step1) * m2;

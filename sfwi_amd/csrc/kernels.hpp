@@ -81,6 +81,7 @@ struct DevShot {
     Fields fw, bk, bn;                     // forward/reconstructed, adjoint, Born scattered
     Fields fw2;                            // second buffer of the forward state for the fused step march:
                                            // its own totals and velocity split parts, fw's stress split parts
+    Fields bn2;                            // second-order scattered field (double scattering)
     fptr dgx, dgz, dgxz, dgzx;             // velocity derivatives for imaging
     fptr rz, rx;                           // synthetic (then residual) traces [ntr][nt]
     fptr r2z, r2x;                         // background traces of a Born solve
@@ -679,8 +680,9 @@ __global__ void k_source(Geo g, const DevShot *__restrict__ shots, const float *
 }
 
 // Velocity half step of every shot.  FWD also stores the boundary strips of
-// the new velocity; BORN also advances the scattered field.
-template <bool EXACT, bool SAVE, bool BORN>
+// the new velocity; BORN (1 or 2) also advances the scattered field, and
+// BORN == 2 the second-order scattered field.
+template <bool EXACT, bool SAVE, int BORN>
 __global__ void __launch_bounds__(BLK) k_vel(Geo g, const DevShot *__restrict__ shots, int it, int dir) {
     int ix, iz;
     if (!cell_of(g, ix, iz)) return;
@@ -693,6 +695,7 @@ __global__ void __launch_bounds__(BLK) k_vel(Geo g, const DevShot *__restrict__ 
     }
     vel_cell<EXACT>(g, d.m, d.fw, i, split, dir);
     if (BORN) vel_cell<EXACT>(g, d.m, d.bn, i, split, dir);
+    if (BORN == 2) vel_cell<EXACT>(g, d.m, d.bn2, i, split, dir);
     if (SAVE && split) {
         int sl[4];
         strip_slots(g, ix, iz, sl);
@@ -711,8 +714,12 @@ __global__ void __launch_bounds__(BLK) k_vel(Geo g, const DevShot *__restrict__ 
 
 // Stress half step of every shot.  GRAD stores the four velocity derivatives
 // (update_txx_tzz_txz_grad, sfwi-stage1/born.cu:451-499); BORN also advances
-// the scattered stresses with the Born source of the background velocity.
-template <bool EXACT, bool GRAD, bool BORN>
+// the scattered stresses with the Born source of the background velocity;
+// BORN == 2 then the second-order scattered stresses with the Born source of
+// the updated first-order scattered velocity (born_modeling mode 5,
+// sfwi-stage1/FD2DGPU.cu:1888-1903: its velocity step, the source from b_vx,
+// b_vz, its stress step).
+template <bool EXACT, bool GRAD, int BORN>
 __global__ void __launch_bounds__(BLK) k_str(Geo g, const DevShot *__restrict__ shots, int dir) {
     int ix, iz;
     if (!cell_of(g, ix, iz)) return;
@@ -739,6 +746,11 @@ __global__ void __launch_bounds__(BLK) k_str(Geo g, const DevShot *__restrict__ 
         const BornSrc s = born_src<EXACT>(g, d.m, v, i, dir);
         const VGrad vb = vgrad<EXACT>(g, d.bn, i);
         str_cell<EXACT, true>(g, d.m, d.bn, vb, i, in_band(g, ix, iz), dir, s);
+        if (BORN == 2) {   // the first-order velocity is not changed by its stress step
+            const BornSrc s2 = born_src<EXACT>(g, d.m, vb, i, dir);
+            const VGrad vb2 = vgrad<EXACT>(g, d.bn2, i);
+            str_cell<EXACT, true>(g, d.m, d.bn2, vb2, i, in_band(g, ix, iz), dir, s2);
+        }
     }
 }
 
@@ -804,9 +816,11 @@ __global__ void __launch_bounds__(256) k_strip_pack(Geo g, const DevShot *__rest
 
 // Receiver sampling (sfwi-stage1/FD2DGPU.cu:3158-3166): vz, vx of the
 // forward field into rz, rx; with BORN the scattered field goes to rz, rx and
-// the background to r2z, r2x (record_scatter, sfwi-stage2/FD2DGPU.cu:3511-3519).
+// the background to r2z, r2x (record_scatter, sfwi-stage2/FD2DGPU.cu:3511-3519);
+// BORN == 2 puts the second-order scattered field in r2z, r2x instead
+// (record_scatter with double_scatter, sfwi-stage1/FD2DGPU.cu:3610-3617).
 // buf: the forward-state buffer holding the step's result (1: DevShot::fw2).
-template <bool BORN>
+template <int BORN>
 __global__ void k_record(Geo g, const DevShot *__restrict__ shots, int it, int buf) {
     const int tr = blockIdx.x * blockDim.x + threadIdx.x;
     const DevShot &d = shots[blockIdx.y];
@@ -815,10 +829,11 @@ __global__ void k_record(Geo g, const DevShot *__restrict__ shots, int it, int b
     const int i = d.igx[tr] * g.nzp + d.igz;
     const size_t o = (size_t)tr * g.nt + it;
     if (BORN) {
+        const Fields &S = BORN == 2 ? d.bn2 : F;
         d.rz[o] = d.bn.vz[i];
         d.rx[o] = d.bn.vx[i];
-        d.r2z[o] = F.vz[i];
-        d.r2x[o] = F.vx[i];
+        d.r2z[o] = S.vz[i];
+        d.r2x[o] = S.vx[i];
     } else {
         d.rz[o] = F.vz[i];
         d.rx[o] = F.vx[i];
@@ -965,13 +980,16 @@ __global__ void __launch_bounds__(BLK) k_adj_vel_image(Geo g, const DevShot *__r
 // wv, wi: weights of the Vp / Vs and illumination sums (1 for the strip
 // gradient, which leaves every product unchanged; K dt and K for the
 // snapshot gradient, which images every K-th step and includes dt).
-template <bool SCALE>
+// ALL = false (the Hessian-vector product, doimage_gradient alone): the
+// Vp / Vs images only.
+template <bool SCALE, bool ALL = true>
 __global__ void k_merge(Geo g, const DevShot *__restrict__ shots, int nshots, int allnx, float *gvp, float *gvs,
                         float *gil, float *grho, float wv, float wi) {
     const int o = blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= allnx * g.nz) return;
     const int ox = o / g.nz, jz = o - ox * g.nz;
-    float a = gvp[o], b = gvs[o], c = gil[o], r = g.grad_rho ? grho[o] : 0.0f;
+    const bool rho = ALL && g.grad_rho;
+    float a = gvp[o], b = gvs[o], c = ALL ? gil[o] : 0.0f, r = rho ? grho[o] : 0.0f;
     for (int s = 0; s < nshots; ++s) {
         const DevShot &d = shots[s];
         const int jx = ox - d.left;
@@ -985,13 +1003,13 @@ __global__ void k_merge(Geo g, const DevShot *__restrict__ shots, int nshots, in
             a += d.gvp[q];
             b += d.gvs[q];
         }
-        c += wi * d.gil[q];
-        if (g.grad_rho) r += d.grho[q];
+        if (ALL) c += wi * d.gil[q];
+        if (rho) r += d.grho[q];
     }
     gvp[o] = a;
     gvs[o] = b;
-    gil[o] = c;
-    if (g.grad_rho) grho[o] = r;
+    if (ALL) gil[o] = c;
+    if (rho) grho[o] = r;
 }
 
 // ----------------------------------------------------- velocity snapshots ---
@@ -1148,6 +1166,17 @@ __global__ void k_window(Geo g, const float *__restrict__ all, int left, float *
     const int sx = min(max(ix, g.pml), g.nxp - g.pml - 1) - g.pml;
     const int sz = min(max(iz, g.pml), g.nzp - g.pml - 1) - g.pml;
     out[i] = all[(size_t)(left + sx) * g.allnz + sz];
+}
+
+// Window of the perturbation for the stage-1 Born source, which acts on the
+// interior cells only (sfwi-stage1/born.cu:830-836, bounds [pml, n - pml)):
+// the interior as k_window, zero on the PML cells, so the source there is 0.
+__global__ void k_window_interior(Geo g, const float *__restrict__ all, int left, float *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.nxp * g.nzp) return;
+    const int ix = i / g.nzp, iz = i - ix * g.nzp;
+    const bool in = ix >= g.pml && ix < g.nxp - g.pml && iz >= g.pml && iz < g.nzp - g.pml;
+    out[i] = in ? all[(size_t)(left + ix - g.pml) * g.allnz + (iz - g.pml)] : 0.0f;
 }
 
 // PML profile (sfwi-stage1/FD2DGPU.cu:443-493) and Lame parameters

@@ -59,11 +59,12 @@ thread_local std::string g_create_error;
 struct Slot {
     char *block = nullptr;      // fw(15) + bk(15) + derivatives(4) fields
     char *born = nullptr;       // bn(15) fields
+    char *born2 = nullptr;      // bn2(15) fields (double scattering)
     char *rec = nullptr;        // rz rx r2z r2x oz ox, ints, accumulators, trace sums
     float *strip = nullptr;     // FP32 [ring][2][sf], or 16-bit: scales [ring][2], stage [2][sf], [ring][2][sf]
     char *hstrip = nullptr;     // host strips (cpu_mem, pinned): FP32 [nt][2][sf], or 16-bit [nt][2][sf] + scales [nt][2]
     char *snap = nullptr;       // velocity snapshots: max bits [nsnap][2] (256-B aligned), then FP32 or 16-bit values
-    Fields fw{}, fw2{}, bk{}, bn{};
+    Fields fw{}, fw2{}, bk{}, bn{}, bn2{};
     float *dg[4]{};
     float *rz, *rx, *r2z, *r2x, *oz, *ox;
     int *igx, *inj, *inj_off;
@@ -85,7 +86,12 @@ struct Window {
     bool has_pert = false;
 };
 
-enum Kind { K_FWD, K_FWD_MISFIT, K_GRAD, K_BORN_FWD, K_BORN_GRAD };
+// Solves: forward, forward + misfit, gradient, Born forward, Born gradient,
+// Born forward with double scattering, Gauss-Newton Hessian-vector product.
+enum Kind { K_FWD, K_FWD_MISFIT, K_GRAD, K_BORN_FWD, K_BORN_GRAD, K_BORN2_FWD, K_HESS };
+inline bool kind_born(Kind k) { return k == K_BORN_FWD || k == K_BORN_GRAD || k == K_BORN2_FWD || k == K_HESS; }
+inline bool kind_grad(Kind k) { return k == K_GRAD || k == K_BORN_GRAD || k == K_HESS; }
+inline bool kind_forward_only(Kind k) { return k == K_FWD || k == K_BORN_FWD || k == K_BORN2_FWD; }
 
 }  // namespace
 
@@ -113,6 +119,7 @@ struct sfwi_ctx {
     unsigned *d_vmax = nullptr;   // largest Vp of a window (random boundary)
     double *d_num = nullptr, *d_den = nullptr;
     bool have_model = false, have_wav = false, have_pert = false;
+    int pert_region = SFWI_BORN_S2;   // where the Born source acts (sfwi_set_perturbation_region)
     // optimal-transport misfit: job table, FP64 workspace, per-job misfits, I/O staging
     OtJob *ot_jobs = nullptr;
     double *ot_ws = nullptr, *ot_mis = nullptr;
@@ -216,7 +223,7 @@ void tcollect(sfwi_ctx *c) {
 // ------------------------------------------------------------ windows ---
 
 int build_window(sfwi_ctx *c, Window &w, int left, const float *vp, const float *vs, const float *rho,
-                 const float *dvp, const float *dvs) {
+                 const float *dvp, const float *dvs, bool interior = false) {
     const size_t n = c->ncell;
     if (!w.base) {
         int r = dalloc(c, (void **)&w.base, 12 * n * sizeof(float));
@@ -240,7 +247,10 @@ int build_window(sfwi_ctx *c, Window &w, int left, const float *vp, const float 
     hipLaunchKernelGGL(k_medium, grd, blk, 0, c->stream, c->g, lg, c->p.rho_const, a[0], a[1], a[2], a[3],
                        a[4], a[5], a[6], a[7], a[8], a[9]);
     w.has_pert = dvp && dvs;
-    if (w.has_pert) {
+    if (w.has_pert && interior) {   // stage-1 Born source: interior cells only
+        hipLaunchKernelGGL(k_window_interior, grd, blk, 0, c->stream, c->g, dvp, left, a[10]);
+        hipLaunchKernelGGL(k_window_interior, grd, blk, 0, c->stream, c->g, dvs, left, a[11]);
+    } else if (w.has_pert) {
         hipLaunchKernelGGL(k_window, grd, blk, 0, c->stream, c->g, dvp, left, a[10]);
         hipLaunchKernelGGL(k_window, grd, blk, 0, c->stream, c->g, dvs, left, a[11]);
     }
@@ -265,7 +275,7 @@ int model_window(sfwi_ctx *c, int left, bool born, const Medium **out) {
     if (c->windows.size() >= 256 && it == c->windows.end()) drop_windows(c);
     Window &w = c->windows[left];
     int r = build_window(c, w, left, c->mvp, c->mvs, c->mrho, c->have_pert ? c->mdvp : nullptr,
-                         c->have_pert ? c->mdvs : nullptr);
+                         c->have_pert ? c->mdvs : nullptr, c->pert_region == SFWI_BORN_S1);
     if (r) return r;
     *out = &w.m;
     return 0;
@@ -281,7 +291,7 @@ size_t snap_words(const sfwi_ctx *c) {
 }
 size_t snap_head(const sfwi_ctx *c) { return snap_words(c) + 2 * SNAP_PART * sizeof(float); }
 
-int ensure_slots(sfwi_ctx *c, int n, bool born, bool strips, bool snaps) {
+int ensure_slots(sfwi_ctx *c, int n, bool born, bool strips, bool snaps, bool born2 = false) {
     const size_t cell = c->ncell * sizeof(float);
     while ((int)c->slots.size() < n) c->slots.emplace_back();
     for (int s = 0; s < n; ++s) {
@@ -329,6 +339,15 @@ int ensure_slots(sfwi_ctx *c, int n, bool born, bool strips, bool snaps) {
             fptr *bn[15] = {&S.bn.vx, &S.bn.vz, &S.bn.txx, &S.bn.tzz, &S.bn.txz, &S.bn.vx_x, &S.bn.vx_z,
                               &S.bn.vz_x, &S.bn.vz_z, &S.bn.txx_x, &S.bn.txx_z, &S.bn.tzz_x, &S.bn.tzz_z,
                               &S.bn.txz_x, &S.bn.txz_z};
+            for (int k = 0; k < 15; ++k) *bn[k] = f + k * c->ncell;
+        }
+        if (born2 && !S.born2) {
+            int r = dalloc(c, (void **)&S.born2, 15 * cell);
+            if (r) return r;
+            float *f = (float *)S.born2;
+            fptr *bn[15] = {&S.bn2.vx, &S.bn2.vz, &S.bn2.txx, &S.bn2.tzz, &S.bn2.txz, &S.bn2.vx_x, &S.bn2.vx_z,
+                            &S.bn2.vz_x, &S.bn2.vz_z, &S.bn2.txx_x, &S.bn2.txx_z, &S.bn2.tzz_x, &S.bn2.tzz_z,
+                            &S.bn2.txz_x, &S.bn2.txz_z};
             for (int k = 0; k < 15; ++k) *bn[k] = f + k * c->ncell;
         }
         if (snaps && !S.snap) {
@@ -399,15 +418,15 @@ int receiver_tables(sfwi_ctx *c, const sfwi_shot &s, int k, Slot &S) {
 // Prepare n shots in slots 0..n-1: medium windows, tables, records, zeroed state.
 int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium *override_m,
             const std::map<int, Window> *override_w = nullptr) {
-    const bool born = kind == K_BORN_FWD || kind == K_BORN_GRAD;
-    const bool grad = kind == K_GRAD || kind == K_BORN_GRAD;
-    const bool obs = grad || kind == K_FWD_MISFIT;
+    const bool born = kind_born(kind), born2 = kind == K_BORN2_FWD;
+    const bool grad = kind_grad(kind);
+    const bool obs = kind == K_GRAD || kind == K_BORN_GRAD || kind == K_FWD_MISFIT;
     if (!c->have_model) return fail(c, SFWI_ESTATE, "model not set (sfwi_set_model)");
     if (!c->have_wav) return fail(c, SFWI_ESTATE, "wavelet not set (sfwi_set_wavelet)");
     if (born && !override_m && !c->have_pert) return fail(c, SFWI_ESTATE, "perturbation not set (sfwi_set_perturbation)");
     if (n < 1 || n > c->p.max_batch) return fail(c, SFWI_EINVAL, "batch of %d shots outside [1,%d]", n, c->p.max_batch);
     const bool snaps = grad && c->g.snap_k > 0;
-    int r = ensure_slots(c, n, born, grad && !c->p.rbc && !snaps, snaps);
+    int r = ensure_slots(c, n, born, grad && !c->p.rbc && !snaps, snaps, born2);
     if (r) return r;
     c->h_shots.resize(n);
     for (int k = 0; k < n; ++k) {
@@ -435,7 +454,7 @@ int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium 
         if (!m && (r = model_window(c, s.left, born, &m))) return r;
         DevShot d{};
         d.m = *m;
-        d.fw = S.fw; d.fw2 = S.fw2; d.bk = S.bk; d.bn = S.bn;
+        d.fw = S.fw; d.fw2 = S.fw2; d.bk = S.bk; d.bn = S.bn; d.bn2 = S.bn2;
         d.dgx = S.dg[0]; d.dgz = S.dg[1]; d.dgxz = S.dg[2]; d.dgzx = S.dg[3];
         d.rz = S.rz; d.rx = S.rx; d.r2z = S.r2z; d.r2x = S.r2x; d.oz = S.oz; d.ox = S.ox;
         if (os) {   // read in place from the resident set
@@ -466,6 +485,7 @@ int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium 
         const size_t cell = c->ncell * sizeof(float);
         HIPCK(c, hipMemsetAsync(S.block, 0, (grad ? 43 : 24) * cell, c->stream));
         if (born) HIPCK(c, hipMemsetAsync(S.born, 0, 15 * cell, c->stream));
+        if (born2) HIPCK(c, hipMemsetAsync(S.born2, 0, 15 * cell, c->stream));
         HIPCK(c, hipMemcpyAsync(S.igx, S.h_igx.data(), s.ntr * sizeof(int), hipMemcpyHostToDevice, c->stream));
         HIPCK(c, hipMemcpyAsync(S.inj, S.h_inj.data(), S.h_inj.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
         HIPCK(c, hipMemcpyAsync(S.inj_off, S.h_off.data(), S.h_off.size() * sizeof(int), hipMemcpyHostToDevice,
@@ -620,8 +640,8 @@ template <bool E, int PD, int HPD, int FPD = 2>
 int run_batch(sfwi_ctx *c, Kind kind, int n) {
     const Geo &g = c->g;
     const int nt = g.nt;
-    const bool born = kind == K_BORN_FWD || kind == K_BORN_GRAD;
-    const bool grad = kind == K_GRAD || kind == K_BORN_GRAD;
+    const bool born = kind_born(kind), born2 = kind == K_BORN2_FWD, hess = kind == K_HESS;
+    const bool grad = kind_grad(kind);
     const dim3 blk(BLK);
     const dim3 cells((unsigned)(8 * g.per_xcd), (unsigned)n);
     const dim3 src1((unsigned)((n + 63) / 64));
@@ -642,7 +662,8 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
     // per-cell kernels on request (march_cols < 0), or automatically (march_cols 0)
     // when the batch has too few wave tiles to fill the GPU: measured crossover
     // near 7 march waves per SIMD (1024 SIMDs)
-    const bool percell = E || c->p.march_cols < 0 || (c->p.march_cols == 0 && (long)n * g.cm_waves < 7168);
+    // (double scattering has per-cell kernels only)
+    const bool percell = E || born2 || c->p.march_cols < 0 || (c->p.march_cols == 0 && (long)n * g.cm_waves < 7168);
     // Strips in host memory (cpu_mem): step it lives at ring position it % ring
     // (two chunks).  Forward: a chunk is copied to the host once written (the
     // last two stay on the device), and a ring half is rewritten only after its
@@ -699,7 +720,9 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
         hipLaunchKernelGGL(k_source, src1, dim3(64), 0, st, g, D, c->wav, it, n, 0);
         int t = tbeg(c, SFWI_K_FWD_VEL, it);
         if (percell) {
-            if (born) {
+            if (born2) {
+                hipLaunchKernelGGL((k_vel<E, false, 2>), cells, blk, 0, st, g, D, sp, 1);
+            } else if (born) {
                 if (save) hipLaunchKernelGGL((k_vel<E, true, true>), cells, blk, 0, st, g, D, sp, 1);
                 else hipLaunchKernelGGL((k_vel<E, false, true>), cells, blk, 0, st, g, D, sp, 1);
             } else {
@@ -728,15 +751,17 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
         }
         t = tbeg(c, SFWI_K_FWD_STR, it);
         if (percell) {
-            if (born) hipLaunchKernelGGL((k_str<E, false, true>), cells, blk, 0, st, g, D, 1);
+            if (born2) hipLaunchKernelGGL((k_str<E, false, 2>), cells, blk, 0, st, g, D, 1);
+            else if (born) hipLaunchKernelGGL((k_str<E, false, true>), cells, blk, 0, st, g, D, 1);
             else hipLaunchKernelGGL((k_str<E, false, false>), cells, blk, 0, st, g, D, 1);
         } else {
             if (born) hipLaunchKernelGGL((km_str<false, true, HPD>), tiles, blk, 0, st, gm, D, 1);
             else hipLaunchKernelGGL((km_str<false, false, PD>), tiles, blk, 0, st, gm, D, 1);
         }
         tend(c, t);
-        if (born) hipLaunchKernelGGL(k_record<true>, trg, dim3(64), 0, st, g, D, it, 0);
-        else hipLaunchKernelGGL(k_record<false>, trg, dim3(64), 0, st, g, D, it, 0);
+        if (born2) hipLaunchKernelGGL(k_record<2>, trg, dim3(64), 0, st, g, D, it, 0);
+        else if (born) hipLaunchKernelGGL(k_record<1>, trg, dim3(64), 0, st, g, D, it, 0);
+        else hipLaunchKernelGGL(k_record<0>, trg, dim3(64), 0, st, g, D, it, 0);
         if (hs && it % ch == ch - 1 && it / ch <= klast - 2) {   // chunk written: to the host
             const int k = it / ch;
             HIPCK(c, hipEventRecord(c->ev_ring[k & 1], st));
@@ -746,9 +771,12 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
         }
     }
     HIPCK(c, hipGetLastError());
-    if (kind == K_FWD || kind == K_BORN_FWD) return 0;
+    if (kind_forward_only(kind)) return 0;
 
-    if (c->p.misfit == SFWI_MISFIT_L2) {
+    // The Hessian-vector product injects the scattered traces as they are
+    // recorded (rz, rx): no residual, no misfit.
+    if (hess) {
+    } else if (c->p.misfit == SFWI_MISFIT_L2) {
         const dim3 resg(256, (unsigned)n);
         if (born) hipLaunchKernelGGL(k_residual<true>, resg, blk, 0, st, g, D);
         else hipLaunchKernelGGL(k_residual<false>, resg, blk, 0, st, g, D);
@@ -770,7 +798,7 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
         // adjoint stresses, then the adjoint stress half step, direction +1;
         // no reconstruction.  Images weighted by snap_k dt, illumination by
         // snap_k, at the merge.
-        const float fac = c->p.misfit == SFWI_MISFIT_L2 ? 2.0f : 1.0f;
+        const float fac = (hess || c->p.misfit == SFWI_MISFIT_L2) ? 2.0f : 1.0f;
         const dim3 ig((unsigned)((c->nint + BLK - 1) / BLK), (unsigned)n);
         Geo gi = g;   // imaging march: whole shots per XCD
         gi.im_per_xcd = (int)(((long)g.im_wg * n + 7) / 8);
@@ -796,8 +824,12 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
             tend(c, t);
         }
         const float K = (float)g.snap_k;
-        hipLaunchKernelGGL(k_merge<true>, mg, blk, 0, st, g, D, n, c->p.allnx, c->Gvp, c->Gvs, c->Gil, c->Grho,
-                           K * g.dt, K);
+        if (hess)
+            hipLaunchKernelGGL((k_merge<true, false>), mg, blk, 0, st, g, D, n, c->p.allnx, c->Gvp, c->Gvs, c->Gil,
+                               c->Grho, K * g.dt, K);
+        else
+            hipLaunchKernelGGL((k_merge<true>), mg, blk, 0, st, g, D, n, c->p.allnx, c->Gvp, c->Gvs, c->Gil, c->Grho,
+                               K * g.dt, K);
         HIPCK(c, hipGetLastError());
         return 0;
     }
@@ -867,8 +899,12 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
         tend(c, t);
         if (int r = rev_end(it)) return r;
     }
-    hipLaunchKernelGGL(k_merge<!E>, mg, blk, 0, st, g, D, n, c->p.allnx, c->Gvp, c->Gvs, c->Gil, c->Grho, 1.0f,
-                       1.0f);
+    if (hess)
+        hipLaunchKernelGGL((k_merge<!E, false>), mg, blk, 0, st, g, D, n, c->p.allnx, c->Gvp, c->Gvs, c->Gil,
+                           c->Grho, 1.0f, 1.0f);
+    else
+        hipLaunchKernelGGL((k_merge<!E>), mg, blk, 0, st, g, D, n, c->p.allnx, c->Gvp, c->Gvs, c->Gil, c->Grho, 1.0f,
+                           1.0f);
     HIPCK(c, hipGetLastError());
     return 0;
 }
@@ -895,7 +931,7 @@ int run_direct(sfwi_ctx *c, Kind kind, int n) {
 // solves whose strips live in host memory (copies on a second stream) and
 // timed solves run directly.
 int run(sfwi_ctx *c, Kind kind, int n) {
-    const bool host_work = c->p.misfit != SFWI_MISFIT_L2 && kind != K_FWD && kind != K_BORN_FWD;
+    const bool host_work = c->p.misfit != SFWI_MISFIT_L2 && !kind_forward_only(kind) && kind != K_HESS;
     if (!c->p.use_graph || host_work || c->timing_every > 0 || c->ring < c->p.nt) return run_direct(c, kind, n);
     const std::pair<int, int> key((int)kind, n);
     auto it = c->graphs.find(key);
@@ -1162,8 +1198,9 @@ int sfwi_set_model(sfwi_ctx *c, const float *vp, const float *vs, const float *r
     return 0;
 }
 
-int sfwi_set_perturbation(sfwi_ctx *c, const float *dvp, const float *dvs) {
+int sfwi_set_perturbation_region(sfwi_ctx *c, const float *dvp, const float *dvs, int region) {
     if (!c || !dvp || !dvs) return fail(c, SFWI_EINVAL, "NULL perturbation");
+    if (region != SFWI_BORN_S1 && region != SFWI_BORN_S2) return fail(c, SFWI_EINVAL, "unknown Born region %d", region);
     const size_t b = c->nmodel * sizeof(float);
     HIPCK(c, hipStreamSynchronize(c->stream));
     drop_windows(c);
@@ -1171,7 +1208,52 @@ int sfwi_set_perturbation(sfwi_ctx *c, const float *dvp, const float *dvs) {
     HIPCK(c, hipMemcpyAsync(c->mdvs, dvs, b, hipMemcpyHostToDevice, c->stream));
     HIPCK(c, hipStreamSynchronize(c->stream));
     c->have_pert = true;
+    c->pert_region = region;
     return 0;
+}
+
+int sfwi_set_perturbation(sfwi_ctx *c, const float *dvp, const float *dvs) {
+    return sfwi_set_perturbation_region(c, dvp, dvs, SFWI_BORN_S2);
+}
+
+int sfwi_batch_born2_forward(sfwi_ctx *c, int nshots, const sfwi_shot *shots, float *b_z, float *b_x, float *b2_z,
+                             float *b2_x) {
+    if (!c || nshots < 0 || (nshots > 0 && !shots)) return fail(c, SFWI_EINVAL, "bad shot list");
+    for (int b0 = 0; b0 < nshots; b0 += c->p.max_batch) {
+        const int n = std::min(c->p.max_batch, nshots - b0);
+        int r = prepare(c, K_BORN2_FWD, n, shots + b0, nullptr);
+        if (r || (r = run(c, K_BORN2_FWD, n))) return r;
+        for (int k = 0; k < n; ++k) {
+            const size_t o = (size_t)(b0 + k) * c->nrec;
+            const Slot &S = c->slots[k];
+            const int ntr = shots[b0 + k].ntr;
+            if ((r = copy_records(c, S.rz, b_z ? b_z + o : nullptr, ntr)) ||
+                (r = copy_records(c, S.rx, b_x ? b_x + o : nullptr, ntr)) ||
+                (r = copy_records(c, S.r2z, b2_z ? b2_z + o : nullptr, ntr)) ||
+                (r = copy_records(c, S.r2x, b2_x ? b2_x + o : nullptr, ntr)))
+                return r;
+        }
+        if ((r = finish(c))) return r;
+    }
+    return 0;
+}
+
+int sfwi_batch_hessian_vector(sfwi_ctx *c, int nshots, const sfwi_shot *shots, const float *p_vp,
+                              const float *p_vs) {
+    if (!c) return SFWI_EINVAL;
+    if (!p_vp != !p_vs) return fail(c, SFWI_EINVAL, "p_vp and p_vs must both be given or both be NULL");
+    if (p_vp) {
+        int r = sfwi_set_perturbation_region(c, p_vp, p_vs, SFWI_BORN_S1);
+        if (r) return r;
+    } else if (!c->have_pert) {
+        return fail(c, SFWI_ESTATE, "no vector: give p_vp, p_vs or set the perturbation");
+    }
+    return solve_misfits(c, K_HESS, nshots, shots, nullptr);
+}
+
+int sfwi_shot_hessian_vector(sfwi_ctx *c, const sfwi_shot *shot, const float *p_vp, const float *p_vs) {
+    if (!c || !shot) return fail(c, SFWI_EINVAL, "NULL argument");
+    return sfwi_batch_hessian_vector(c, 1, shot, p_vp, p_vs);
 }
 
 int sfwi_obs_put(sfwi_ctx *c, int id, int ntr, const float *obs_z, const float *obs_x) {

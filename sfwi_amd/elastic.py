@@ -125,8 +125,28 @@ class Context:
     def obs_clear(self):
         self._ck(self.L.sfwi_obs_clear(self.h))
 
-    def set_perturbation(self, dvp, dvs):
-        self._ck(self.L.sfwi_set_perturbation(self.h, _f32(dvp), _f32(dvs)))
+    def set_perturbation(self, dvp, dvs, region="s2"):
+        """Born perturbation; region "s2" (every cell at least 4 from the grid
+        edge, the stage-2 Born source) or "s1" (the interior only, stage 1)."""
+        reg = {"s2": _lib.SFWI_BORN_S2, "s1": _lib.SFWI_BORN_S1}[region]
+        self._ck(self.L.sfwi_set_perturbation_region(self.h, _f32(dvp), _f32(dvs), reg))
+
+    def batch_born2_forward(self, shots):
+        """Born forward with double scattering: (b_z, b_x, b2_z, b2_x), the first-
+        and second-order scattered traces, arrays (nshots, ntr_max, nt)."""
+        arr = (SfwiShot * len(shots))(*[s.c for s in shots])
+        out = [np.zeros((len(shots), self.p.ntr, self.nt), np.float32) for _ in range(4)]
+        self._ck(self.L.sfwi_batch_born2_forward(self.h, len(shots), arr, *[_lib.ptr(a) for a in out]))
+        return tuple(out)
+
+    def hessian_vector(self, shots, p_vp=None, p_vs=None):
+        """Accumulate the Gauss-Newton Hessian-vector product H p of the shots
+        (read it with flush()); p_vp, p_vs None: the current perturbation."""
+        arr = (SfwiShot * len(shots))(*[s.c for s in shots])
+        a = None if p_vp is None else _f32(p_vp)
+        b = None if p_vs is None else _f32(p_vs)
+        self._ck(self.L.sfwi_batch_hessian_vector(self.h, len(shots), arr, None if a is None else _lib.ptr(a),
+                                                  None if b is None else _lib.ptr(b)))
 
     # ---------------------------------------------------------- solves ---
     def forward(self, shot):

@@ -5,9 +5,14 @@
 // iterations (batched gradients, preconditioned CG direction, parabolic line
 // search along it, clamped step, model update with the top-row reset).
 //
-//   sfwi_stage1 <para file> [niter [max_batch [cut_layer]]]
+//   sfwi_stage1 <para file> [niter [max_batch [cut_layer [max_inner]]]]
 //
-// niter defaults to the file's maxiter, cut_layer to the reference's 26; shots whose number lies outside
+// niter defaults to the file's maxiter, cut_layer to the reference's 26,
+// max_inner to the reference's 0 (main.cpp:408); with max_inner > 0 each
+// iteration runs the Gauss-Newton inner CG (main.cpp:784-1063: batched
+// Hessian-vector products of every shot, the host step, the normalized
+// -delta as the search direction) and logs its hessian_misfit lines to
+// misfit.txt as the reference does.  Shots whose number lies outside
 // [minshot, maxshot] are skipped.  The density is forced to 2000 as the
 // reference does (sfwi-stage1/main.cpp:1476-1479).  Where the reference sizes
 // its one context by shot 0's aperture and runs other apertures on it
@@ -29,8 +34,8 @@
 using namespace sfwi_host;
 
 int main(int argc, char **argv) {
-    if (argc < 2 || argc > 5) {
-        std::fprintf(stderr, "usage: %s para_file [niter [max_batch [cut_layer]]]\n", argv[0]);
+    if (argc < 2 || argc > 6) {
+        std::fprintf(stderr, "usage: %s para_file [niter [max_batch [cut_layer [max_inner]]]]\n", argv[0]);
         return 2;
     }
     std::vector<sfwi_ctx *> ctxs;
@@ -40,8 +45,9 @@ int main(int argc, char **argv) {
         const int niter = argc >= 3 ? std::atoi(argv[2]) : p.maxiter;
         const int max_batch = argc >= 4 ? std::atoi(argv[3]) : 64;
         const int cut_layer = argc >= 5 ? std::atoi(argv[4]) : 26;   // sfwi-stage1/main.cpp:410
-        if (niter < 0 || max_batch < 1 || cut_layer < 0)
-            throw Fail(2, "input error: niter must be >= 0, max_batch >= 1, cut_layer >= 0");
+        const int max_inner = argc >= 6 ? std::atoi(argv[5]) : 0;    // sfwi-stage1/main.cpp:408
+        if (niter < 0 || max_batch < 1 || cut_layer < 0 || max_inner < 0)
+            throw Fail(2, "input error: niter must be >= 0, max_batch >= 1, cut_layer >= 0, max_inner >= 0");
         const size_t nm = (size_t)p.allnx * p.allnz;
         std::vector<float> vp = sv.vp, vs = sv.vs, rho(nm, 2000.0f);
         std::vector<float> wav(sv.nt);
@@ -82,6 +88,39 @@ int main(int argc, char **argv) {
                                  pp_cg_old.data(), ps_cg_old.data(), &beta_fenmu, dir_pp.data(), dir_ps.data(),
                                  &beta),
                nullptr);
+            if (max_inner > 0) {   // Gauss-Newton inner CG: its direction replaces the CG one
+                std::vector<float> h[8], hp[2], ht[3];
+                for (auto &a : h) a.resize(nm);
+                for (auto &a : hp) a.resize(nm);
+                for (auto &a : ht) a.resize(nm);
+                float *r0 = h[0].data(), *r1 = h[1].data(), *q0 = h[2].data(), *q1 = h[3].data();
+                float *d0 = h[4].data(), *d1 = h[5].data(), *e0 = h[6].data(), *e1 = h[7].data();
+                ck(sfwi_s1_hcg_init(p.allnx, p.allnz, pp_cg_old.data(), ps_cg_old.data(), r0, r1, q0, q1, d0, d1, e0,
+                                    e1),
+                   nullptr);
+                for (int inner = 0; inner < max_inner; ++inner) {
+                    for (size_t gi = 0; gi < sv.widths.size(); ++gi) {   // H p summed over every shot
+                        sfwi_ctx *c = ctxs[gi];
+                        const std::vector<sfwi_shot> grp = group_shots(sv, sv.widths[gi]);
+                        ck(sfwi_batch_hessian_vector(c, (int)grp.size(), grp.data(), q0, q1), c);
+                        ck(sfwi_flush(c, ht[0].data(), ht[1].data(), ht[2].data()), c);
+                        for (int k = 0; k < 2; ++k)
+                            for (size_t i = 0; i < nm; ++i) hp[k][i] = gi == 0 ? ht[k][i] : hp[k][i] + ht[k][i];
+                    }
+                    double rr0 = 0.0, rr1 = 0.0, ha = 0.0, hb = 0.0;
+                    ck(sfwi_s1_hcg_step(p.allnx, p.allnz, 1, cut_layer, hp[0].data(), hp[1].data(), r0, r1, q0, q1, d0,
+                                        d1, e0, e1, &rr0, &rr1, &ha, &hb),
+                       nullptr);
+                    if (FILE *f = std::fopen((outdir + "/misfit.txt").c_str(), "a")) {   // main.cpp:987-1037
+                        if (inner == 0) std::fprintf(f, "i = %d,inner = %d,hessian_misfit = %5.3e\n", iter, inner, rr0);
+                        std::fprintf(f, "i = %d,inner = %d,hessian_misfit = %5.3e\n", iter, inner + 1, rr1);
+                        std::fclose(f);
+                    }
+                }
+                ck(sfwi_s1_hcg_finish(p.allnx, p.allnz, d0, d1, e0, e1), nullptr);
+                dir_pp = h[6];
+                dir_ps = h[7];
+            }
             double ls[3] = {0.0, 0.0, 0.0};
             for (size_t gi = 0; gi < sv.widths.size(); ++gi) {   // batched; the sums of per-shot calls in shot order
                 const std::vector<sfwi_shot> grp = group_shots(sv, sv.widths[gi]);

@@ -1,11 +1,12 @@
 """The reference drivers' inversion loops over the per-shot solves.
 
 ``Stage1`` is the stage-1 iteration of sfwi-stage1/main.cpp (the
-configuration the reference runs: update_mp, no Gauss-Newton inner loop,
-conjugate gradient): batched gradient of every shot, illumination
-preconditioning with the cut layer, CG direction normalized by its joint
-max-abs, parabolic line search at two trial steps, clamped step, model update
-and top-row reset.  ``Stage2`` is the stage-2 iteration of
+configuration the reference runs: update_mp, conjugate gradient, and by
+default no Gauss-Newton inner loop): batched gradient of every shot,
+illumination preconditioning with the cut layer, CG direction normalized by
+its joint max-abs, with max_inner > 0 the Gauss-Newton inner CG on batched
+Hessian-vector products (main.cpp:784-1063), parabolic line search at two
+trial steps, clamped step, model update and top-row reset.  ``Stage2`` is the stage-2 iteration of
 sfwi-stage2/main.cpp: Born gradient, preconditioned CG direction, exact
 linear step, image update.  The host steps run in the library
 (sfwi_s1_* / sfwi_s2_*, CPU code with the reference's arithmetic types); the
@@ -50,9 +51,53 @@ class Stage1(_CG):
     """Stage-1 loop state and steps (sfwi-stage1/main.cpp:366-416 locals:
     cut_layer 26, step1 30, step2 150)."""
 
-    def __init__(self, allnx, allnz, cut_layer=26, step1=30.0, step2=150.0):
+    def __init__(self, allnx, allnz, cut_layer=26, step1=30.0, step2=150.0, max_inner=0, cut_grad=True):
         super().__init__(allnx, allnz, cut_layer)
         self.step1, self.step2 = float(step1), float(step2)
+        self.max_inner, self.cut_grad = int(max_inner), bool(cut_grad)
+
+    # Gauss-Newton inner CG (main.cpp:784-1063); the state is a dict of
+    # float32 (pp, ps) pairs r, p, delta, grad.
+    def hcg_init(self, pp_cg=None, ps_cg=None):
+        """Inner-loop set-up from the CG direction (default: the one the last
+        direction() call formed)."""
+        c1 = _f32c(self.pp_cg_old if pp_cg is None else pp_cg)
+        c2 = _f32c(self.ps_cg_old if ps_cg is None else ps_cg)
+        a = [np.zeros((self.allnx, self.allnz), np.float32) for _ in range(8)]
+        self._ck(self.L.sfwi_s1_hcg_init(self.allnx, self.allnz, c1, c2, *a))
+        return dict(r=(a[0], a[1]), p=(a[2], a[3]), delta=(a[4], a[5]), grad=(a[6], a[7]))
+
+    def hcg_step(self, st, hp_pp, hp_ps):
+        """One inner iteration from the summed H p; st updated in place, hp_*
+        copied (the rows above the cut are zeroed in the copies).  Returns
+        (rr_old, rr_new, alpha, beta)."""
+        h1, h2 = np.array(hp_pp, np.float32), np.array(hp_ps, np.float32)
+        out = [ctypes.c_double(0.0) for _ in range(4)]
+        self._ck(self.L.sfwi_s1_hcg_step(self.allnx, self.allnz, int(self.cut_grad), self.cut_layer, h1, h2,
+                                         *st["r"], *st["p"], *st["delta"], *st["grad"],
+                                         *[ctypes.byref(o) for o in out]))
+        return tuple(o.value for o in out)
+
+    def hcg_finish(self, st):
+        self._ck(self.L.sfwi_s1_hcg_finish(self.allnx, self.allnz, *st["delta"], *st["grad"]))
+        return st
+
+    def gauss_newton(self, groups, st=None):
+        """The inner loop over (context, shots) groups: max_inner Hessian-vector
+        products summed over every shot (groups in list order), one host step
+        each.  Returns (grad_pp, grad_ps, history): the search direction and the
+        (rr_old, rr_new, alpha, beta) of every inner iteration."""
+        st = self.hcg_init() if st is None else st
+        hist = []
+        for _ in range(self.max_inner):
+            H = None
+            for ctx, shots in groups:
+                ctx.hessian_vector(shots, *st["p"])   # in batches of max_batch inside the library
+                h = ctx.flush()
+                H = h[:2] if H is None else tuple((a + b).astype(np.float32) for a, b in zip(H, h[:2]))
+            hist.append(self.hcg_step(st, *H))
+        self.hcg_finish(st)
+        return st["grad"][0], st["grad"][1], hist
 
     def direction(self, pp_grad, ps_grad, illum):
         """Search direction from the summed gradients; advances the CG state.
@@ -94,13 +139,16 @@ class Stage1(_CG):
             g = ctx.flush()
             G = g if G is None else tuple((a + b).astype(np.float32) for a, b in zip(G, g))
         d1, d2, beta = self.direction(*G)
+        inner = []
+        if self.max_inner > 0:   # the Gauss-Newton direction replaces the CG one
+            d1, d2, inner = self.gauss_newton(groups)
         m = np.zeros(3)
         for ctx, shots in groups:   # batched forward solves; the same sums as per-shot calls in shot order
             m = ctx.batch_linesearch_parabola(shots, d1, d2, self.step1, self.step2, m)
         alpha = self.alpha(*m)
         self.update(velp, vels, d1, d2, alpha)
         rec = dict(iter=self.iter, misfit=misfit, rms=self._rms(misfit), beta=beta, alpha=alpha,
-                   linesearch=tuple(m))
+                   linesearch=tuple(m), inner=inner)
         self.iter += 1
         return rec
 

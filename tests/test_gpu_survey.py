@@ -112,6 +112,54 @@ def test_stage1_from_su_files(tmp_path):
     assert (tmp_path / "out" / "misfit.txt").read_text().count("misfit") == 2
 
 
+def test_stage1_gauss_newton_program_matches_python(tmp_path):
+    """Two stage-1 iterations with the Gauss-Newton inner CG (max_inner = 2,
+    sfwi-stage1/main.cpp:784-1063): the C++ program on the SU files and the
+    Python loop over the same survey give the same models, misfits, beta and
+    alpha bit for bit, and the program logs the reference's hessian_misfit
+    lines."""
+    sv = _model()
+    w = _lib.ricker(NT, np.float32(DT), F0)
+    rho = np.full_like(sv.vp, 2000.0)
+    for k, (sx, gx) in enumerate(SPREADS):
+        sh, width = _shot(sx, gx)
+        with _ctx(sv, width, 1) as ctx:
+            ctx.set_wavelet(w)
+            ctx.set_model(sv.vp, sv.vs, rho)
+            oz, ox = ctx.forward(sh)
+        S.write_su(str(tmp_path / "z.su"), k + 1, sx, gx, oz, DT, append=k > 0)
+        S.write_su(str(tmp_path / "x.su"), k + 1, sx, gx, ox, DT, append=k > 0)
+    sv.vp0.tofile(tmp_path / "vp.bin")
+    sv.vs0.tofile(tmp_path / "vs.bin")
+    sv.vs0.tofile(tmp_path / "rho.bin")
+    para = tmp_path / "para"
+    para.write_text("%g %g %d %d %d 1 3\n%s\n%s\n%s\n%s\n%s\n%s\n%g 2 0\n%d %d\n1 0 0 1000\n" % (
+        DX, DX, PML, sv.allnx, sv.allnz, tmp_path / "vp.bin", tmp_path / "vs.bin", tmp_path / "rho.bin",
+        tmp_path / "z.su", tmp_path / "x.su", tmp_path / "out", F0, SY, GY))
+    su = S.SuSurvey(para)
+    ctxs = {wd: _ctx(sv, wd, len(ix)) for wd, ix in su.groups().items()}
+    for c in ctxs.values():
+        c.set_wavelet(w)
+    groups = [(ctxs[wd], [su.shots[k] for k in ix]) for wd, ix in su.groups().items()]
+    st = Stage1(sv.allnx, sv.allnz, cut_layer=CUT, max_inner=2)
+    vp, vs = su.vp.copy(), su.vs.copy()
+    its = [st.iterate_groups(groups, vp, vs, su.rho) for _ in range(2)]
+    for c in ctxs.values():
+        c.close()
+    assert all(len(i["inner"]) == 2 for i in its) and np.all(np.isfinite(vp))
+    r = subprocess.run([PROG, str(para), "2", "64", str(CUT), "2"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    lines = [l.split() for l in r.stdout.strip().splitlines()]
+    for l, it in zip(lines, its):
+        assert float(l[3]) == it["misfit"] and float(l[5]) == it["beta"] and float(l[7]) == it["alpha"]
+    got = np.fromfile(tmp_path / "out" / ("vel_p_1_%d_%d.dat" % (sv.allnx, sv.allnz)), np.float32)
+    assert np.array_equal(got.reshape(vp.shape), vp)
+    log = (tmp_path / "out" / "misfit.txt").read_text()
+    assert log.count("hessian_misfit") == 2 * 3
+    first = float(log.split("hessian_misfit = ")[1].split()[0])
+    assert first == pytest.approx(its[0]["inner"][0][0], rel=1e-3)   # printed with %5.3e
+
+
 def test_stage1_overlapping_windows_in_batches(tmp_path):
     """Four shots of one aperture width whose windows overlap at offsets 0, 10,
     20 and 30 of a 60-column model: the stage-1 program solving them in

@@ -105,7 +105,76 @@ def test_s2_direction_cuts_and_scales():
     assert np.array_equal(k1, c1 * 2 + c1 * 4)
 
 
+def _spd(n, seed):
+    r = np.random.default_rng(seed)
+    a = r.standard_normal((n, n)) / np.sqrt(n)
+    return a.T @ a + np.eye(n)      # well conditioned, symmetric positive definite
+
+
+@pytest.mark.parametrize("plant", [0, 1])
+def test_hcg_solves_the_newton_system(plant):
+    """The inner CG in FP64 on an explicit SPD matrix: after as many steps as
+    there are unknowns below the cut layer, r = 0 and delta = -r0 - H^-1 r0
+    there (CG from zero solves H d = -r0; the reference starts delta at
+    normalize(-cg) = -r0 and adds the CG steps), zero above the cut; grad =
+    -delta.  Beta with the wrong sign does not converge."""
+    nx, nz, cut = 3, 5, 2
+    act = np.zeros((nx, nz), bool)
+    act[:, cut:] = True
+    m = 2 * int(act.sum())
+    H = _spd(m, 11)
+    r = np.random.default_rng(12)
+    g1 = np.where(act, r.standard_normal((nx, nz)), 0.0)
+    g2 = np.where(act, r.standard_normal((nx, nz)), 0.0)
+    st = I.s1_hcg_init(g1, g2, dt=np.float64)
+    r0 = np.concatenate([st["r"][0][act], st["r"][1][act]])
+    def hv(p1, p2):
+        y = H @ np.concatenate([p1[act], p2[act]])
+        h1, h2 = np.zeros((nx, nz)), np.zeros((nx, nz))
+        h1[act], h2[act] = y[:m // 2], y[m // 2:]
+        h1[~act] = 7.0      # cut by the step
+        return h1, h2
+    for _ in range(m):
+        I.s1_hcg_step(st, *hv(*st["p"]), True, cut, dt=np.float64, plant=plant)
+    d = np.concatenate([st["delta"][0][act], st["delta"][1][act]])
+    want = -r0 - np.linalg.solve(H, r0)
+    if plant:
+        assert np.linalg.norm(d - want) > 1e-3 * np.linalg.norm(want)
+        return
+    assert np.linalg.norm(d - want) <= 1e-10 * np.linalg.norm(want)
+    assert np.abs(st["delta"][0][~act]).max() == 0 and np.abs(st["delta"][1][~act]).max() == 0
+    assert np.linalg.norm(np.concatenate([st["r"][0][act], st["r"][1][act]])) <= 1e-10
+    assert np.array_equal(st["grad"][0], -st["delta"][0])
+    I.s1_hcg_finish(st, dt=np.float64)
+    assert max(np.abs(st["grad"][0]).max(), np.abs(st["grad"][1]).max()) == 1.0
+
+
 # ------------------------------------------------- library equals oracle ---
+
+@pytest.mark.parametrize("seed", [13, 14])
+def test_library_hcg_steps_equal_oracle(seed):
+    g1, g2, il = _fields(seed)
+    st = Stage1(NX, NZ, cut_layer=4)
+    st.direction(g1, g2, il)
+    lib = st.hcg_init()
+    orc = I.s1_hcg_init(st.pp_cg_old, st.ps_cg_old)
+    for k in ("r", "p", "delta", "grad"):
+        assert np.array_equal(lib[k][0], orc[k][0]) and np.array_equal(lib[k][1], orc[k][1]), k
+    r = np.random.default_rng(seed)
+    A = (r.standard_normal((NZ, NZ)) * 0.2 + np.eye(NZ) * 2).astype(np.float32)
+    for it in range(3):
+        h1 = (lib["p"][0] @ A).astype(np.float32)
+        h2 = (lib["p"][1] @ A.T).astype(np.float32)
+        a = st.hcg_step(lib, h1, h2)
+        b = I.s1_hcg_step(orc, h1, h2, True, 4)
+        assert a == b, (a, b)
+        for k in ("r", "p", "delta", "grad"):
+            assert np.array_equal(lib[k][0], orc[k][0]) and np.array_equal(lib[k][1], orc[k][1]), (it, k)
+    st.hcg_finish(lib)
+    I.s1_hcg_finish(orc)
+    for k in ("delta", "grad"):
+        assert np.array_equal(lib[k][0], orc[k][0]) and np.array_equal(lib[k][1], orc[k][1]), k
+
 
 @pytest.mark.parametrize("seed", [5, 6])
 def test_library_s1_steps_equal_oracle(seed):
