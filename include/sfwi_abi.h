@@ -223,6 +223,9 @@ int sfwi_batch_forward(sfwi_ctx *ctx, int nshots, const sfwi_shot *shots, float 
  * and background traces bg_* (either pair may be NULL). */
 int sfwi_shot_born_forward(sfwi_ctx *ctx, const sfwi_shot *shot, float *b_z, float *b_x,
                            float *bg_z, float *bg_x);
+/* Born forward solves of many shots at once (the same kernels over batches of
+ * max_batch shots): scattered traces of shot i at b_z + i*ntr_max*nt. */
+int sfwi_batch_born_forward(sfwi_ctx *ctx, int nshots, const sfwi_shot *shots, float *b_z, float *b_x);
 
 /* Born forward solve with double scattering (born_modeling mode 1 then mode
  * 5 each step, sfwi-stage1/FD2DGPU.cu:1855-1860, 1888-1903; the reference
@@ -384,6 +387,14 @@ size_t sfwi_solve_bytes(const sfwi_params *p, int kind, int nshots);
 /* hbm_budget 0: 15/16 of the free HBM; workers in [1, 16]. */
 int sfwi_server_create(size_t hbm_budget, int workers, sfwi_server **out);
 int sfwi_server_submit(sfwi_server *s, const sfwi_request *req, long long *ticket);
+/* Cross-request batching (no reference counterpart), on by default: requests
+ * of the same shape and kind (every params field, the kind and the autotune
+ * flag equal) queued when a worker takes one join its batch, up to
+ * params.max_batch shots in all; each shot keeps its own request's model,
+ * perturbation and wavelet, and each request's images are merged over its
+ * own shots, so every result equals the request solved alone.  on = 0 runs
+ * one request per batch. */
+int sfwi_server_batching(sfwi_server *s, int on);
 /* Blocks until the request is done; queue_ms = submit -> start, run_ms = start -> end. */
 int sfwi_server_wait(sfwi_server *s, long long ticket, double *queue_ms, double *run_ms);
 /* Call after every submitted request has been waited for. */

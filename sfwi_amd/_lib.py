@@ -103,6 +103,7 @@ def load():
     L.sfwi_set_perturbation.argtypes = [vp, _f32p, _f32p]
     L.sfwi_set_perturbation_region.argtypes = [vp, _f32p, _f32p, ctypes.c_int]
     L.sfwi_batch_born2_forward.argtypes = [vp, ctypes.c_int, P(SfwiShot), vp, vp, vp, vp]
+    L.sfwi_batch_born_forward.argtypes = [vp, ctypes.c_int, P(SfwiShot), vp, vp]
     L.sfwi_shot_hessian_vector.argtypes = [vp, P(SfwiShot), vp, vp]
     L.sfwi_batch_hessian_vector.argtypes = [vp, ctypes.c_int, P(SfwiShot), vp, vp]
     L.sfwi_shot_gradient.argtypes = [vp, P(SfwiShot), dp]
@@ -152,6 +153,7 @@ def load():
     L.sfwi_server_create.argtypes = [ctypes.c_size_t, ctypes.c_int, P(vp)]
     L.sfwi_server_submit.argtypes = [vp, P(SfwiRequest), P(ctypes.c_longlong)]
     L.sfwi_server_wait.argtypes = [vp, ctypes.c_longlong, dp, dp]
+    L.sfwi_server_batching.argtypes = [vp, ctypes.c_int]
     L.sfwi_server_destroy.argtypes = [vp]
     L.sfwi_server_destroy.restype = None
     _LIB = L

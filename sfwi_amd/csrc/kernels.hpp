@@ -82,6 +82,7 @@ struct DevShot {
     Fields fw2;                            // second buffer of the forward state for the fused step march:
                                            // its own totals and velocity split parts, fw's stress split parts
     Fields bn2;                            // second-order scattered field (double scattering)
+    cfptr wav;                             // the shot's own wavelet [nt] (NULL: the context's)
     fptr dgx, dgz, dgxz, dgzx;             // velocity derivatives for imaging
     fptr rz, rx;                           // synthetic (then residual) traces [ntr][nt]
     fptr r2z, r2x;                         // background traces of a Born solve
@@ -675,8 +676,9 @@ __global__ void k_source(Geo g, const DevShot *__restrict__ shots, const float *
     const DevShot &d = shots[s];
     const Fields &F = buf ? d.fw2 : d.fw;
     const int i = d.isx * g.nzp + d.isz;
-    F.txx[i] += wav[it];
-    F.tzz[i] += wav[it];
+    const float w = d.wav ? d.wav[it] : wav[it];
+    F.txx[i] += w;
+    F.tzz[i] += w;
 }
 
 // Velocity half step of every shot.  FWD also stores the boundary strips of

@@ -5,7 +5,12 @@
 // of worker threads, each driving its own context and stream, takes the
 // first queued request whose device memory fits the free part of the HBM
 // budget, so requests that arrive while others run are admitted as workers
-// and memory free up, and their kernels overlap on the GPU.  Contexts are
+// and memory free up, and their kernels overlap on the GPU.  Requests of the
+// same shape and kind (every sfwi_params field equal) that are queued when a
+// worker takes one join its batch, up to max_batch shots in all: they share
+// every launch, each shot with its own request's model and wavelet, and
+// each request's images are merged over its own shots (solve_merged), so the
+// results equal the requests solved one by one.  Contexts are
 // cached by shape (every sfwi_params field) and request kind and stay
 // resident with their wavefield slots; an idle cached context is evicted
 // (least recently used first) when a new request needs its memory.  A request's result does not depend on what else runs:
@@ -30,7 +35,9 @@ size_t solve_bytes(const sfwi_params &p, int kind, int nshots) {
     const bool grad = kind == SFWI_REQ_GRADIENT || kind == SFWI_REQ_BORN_GRADIENT;
     const size_t sf = 16 * (size_t)(p.nx + p.nz);
     size_t fixed = p.nt * f + 12 * nmodel * f + 2 * nrec * f + p.max_batch * 1024 + 2 * p.ntr * 8 + 4096;
-    size_t slot = (43 + 2 * (p.grad_rho != 0)) * ncell * f + 6 * nrec * f + 4 * nint * f + 20 * (size_t)p.ntr + 68;
+    // per shot: the slot, its window and, in a merged batch, its request's model
+    size_t slot = (43 + 2 * (p.grad_rho != 0)) * ncell * f + 6 * nrec * f + 4 * nint * f + 20 * (size_t)p.ntr + 68 +
+                  5 * nmodel * f + p.nt * f;
     if (born) slot += 15 * ncell * f;
     const size_t chunk = p.strip_chunk > 0 ? (size_t)p.strip_chunk : 100;
     const size_t ring = p.cpu_mem && 2 * chunk < (size_t)p.nt ? 2 * chunk : (size_t)p.nt;   // strip steps in HBM
@@ -80,6 +87,7 @@ struct sfwi_server {
     size_t budget = 0, in_use = 0;   // in_use: running solves + cached idle contexts
     int device = 0;
     bool stop = false;
+    bool batching = true;   // queued requests of one shape and kind join a worker's batch
     long long next_ticket = 1, use_clock = 0;
     std::mutex m;
     std::condition_variable cv_work, cv_done;
@@ -113,6 +121,41 @@ struct sfwi_server {
         return true;
     }
 
+    static Kind kind_of(int req) {
+        switch (req) {
+        case SFWI_REQ_FORWARD: return K_FWD;
+        case SFWI_REQ_BORN_FORWARD: return K_BORN_FWD;
+        case SFWI_REQ_GRADIENT: return K_GRAD;
+        default: return K_BORN_GRAD;
+        }
+    }
+
+    // Several requests of one shape and kind as one merged batch.
+    int run_group(const std::vector<Job *> &group, sfwi_ctx *c, bool fresh) {
+        const sfwi_request &q0 = group[0]->req;
+        int total = 0;
+        for (const Job *J : group) total += J->req.nshots;
+        if (fresh && q0.autotune && q0.params.mode == SFWI_MODE_FAST) {
+            int r = sfwi_set_wavelet(c, q0.wavelet);
+            if (!r) r = sfwi_set_model(c, q0.vp, q0.vs, q0.rho);
+            if (!r) r = sfwi_autotune(c, total, 100, nullptr, nullptr, nullptr, nullptr);
+            if (r) return r;
+        }
+        std::vector<Part> parts;
+        for (const Job *J : group) {
+            const sfwi_request &q = J->req;
+            Part p{};
+            p.vp = q.vp; p.vs = q.vs; p.rho = q.rho; p.dvp = q.dvp; p.dvs = q.dvs; p.wav = q.wavelet;
+            p.nshots = q.nshots;
+            p.shots = J->shots.data();
+            p.syn_z = q.syn_z; p.syn_x = q.syn_x;
+            p.misfits = q.misfits;
+            p.gvp = q.grad_vp; p.gvs = q.grad_vs; p.gil = q.illum;
+            parts.push_back(p);
+        }
+        return solve_merged(c, kind_of(q0.kind), parts);
+    }
+
     int run_job(Job &J, sfwi_ctx *c, bool fresh) {
         const sfwi_request &q = J.req;
         int r = sfwi_set_wavelet(c, q.wavelet);
@@ -128,10 +171,7 @@ struct sfwi_server {
         case SFWI_REQ_FORWARD:
             return sfwi_batch_forward(c, n, J.shots.data(), q.syn_z, q.syn_x);
         case SFWI_REQ_BORN_FORWARD:
-            for (int k = 0; k < n && !r; ++k)
-                r = sfwi_shot_born_forward(c, &J.shots[k], q.syn_z ? q.syn_z + k * rec : nullptr,
-                                           q.syn_x ? q.syn_x + k * rec : nullptr, nullptr, nullptr);
-            return r;
+            return sfwi_batch_born_forward(c, n, J.shots.data(), q.syn_z, q.syn_x);
         case SFWI_REQ_GRADIENT:
         case SFWI_REQ_BORN_GRADIENT: {
             std::vector<double> mis(n, 0.0);
@@ -168,6 +208,22 @@ struct sfwi_server {
             }
             J->status = 2;
             J->t_start = std::chrono::steady_clock::now();
+            // queued requests of the same shape and kind join this batch
+            std::vector<Job *> group{J};
+            int total = J->req.nshots;
+            for (auto it = queue.begin(); batching && it != queue.end();) {
+                Job &K = jobs[*it];
+                if (K.req.kind == J->req.kind && K.req.autotune == J->req.autotune &&
+                    same_params(K.req.params, J->req.params) && total + K.req.nshots <= J->req.params.max_batch) {
+                    total += K.req.nshots;
+                    K.status = 2;
+                    K.t_start = J->t_start;
+                    group.push_back(&K);
+                    it = queue.erase(it);
+                } else {
+                    ++it;
+                }
+            }
             sfwi_ctx *c = slot->ctx;
             lk.unlock();
             int r = 0;
@@ -178,7 +234,7 @@ struct sfwi_server {
                 if (r) msg = sfwi_last_error(nullptr);
             }
             if (!r) {
-                r = run_job(*J, c, fresh);
+                r = group.size() == 1 ? run_job(*J, c, fresh) : run_group(group, c, fresh);
                 if (r) msg = sfwi_last_error(c);
             }
             lk.lock();
@@ -188,9 +244,12 @@ struct sfwi_server {
                 in_use -= slot->bytes;
                 cache.erase(slot);
             }
-            J->t_end = std::chrono::steady_clock::now();
-            J->status = r ? r : 0;
-            J->err = msg;
+            const auto t_end = std::chrono::steady_clock::now();
+            for (Job *G : group) {
+                G->t_end = t_end;
+                G->status = r ? r : 0;
+                G->err = msg;
+            }
             cv_done.notify_all();
             cv_work.notify_all();
         }
@@ -219,6 +278,13 @@ int sfwi_server_create(size_t hbm_budget, int nworkers, sfwi_server **out) {
     s->device = dev;
     for (int k = 0; k < nworkers; ++k) s->workers.emplace_back([s] { s->worker(); });
     *out = s;
+    return 0;
+}
+
+int sfwi_server_batching(sfwi_server *s, int on) {
+    if (!s) return fail(nullptr, SFWI_EINVAL, "NULL server");
+    std::lock_guard<std::mutex> lk(s->m);
+    s->batching = on != 0;
     return 0;
 }
 

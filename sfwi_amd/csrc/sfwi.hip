@@ -120,6 +120,9 @@ struct sfwi_ctx {
     double *d_num = nullptr, *d_den = nullptr;
     bool have_model = false, have_wav = false, have_pert = false;
     int pert_region = SFWI_BORN_S2;   // where the Born source acts (sfwi_set_perturbation_region)
+    bool defer_merge = false;          // run_batch leaves the per-shot images unmerged (solve_merged)
+    float *part_buf = nullptr;         // models and wavelets of the parts of a merged solve
+    size_t part_buf_cap = 0;
     // optimal-transport misfit: job table, FP64 workspace, per-job misfits, I/O staging
     OtJob *ot_jobs = nullptr;
     double *ot_ws = nullptr, *ot_mis = nullptr;
@@ -416,14 +419,18 @@ int receiver_tables(sfwi_ctx *c, const sfwi_shot &s, int k, Slot &S) {
 }
 
 // Prepare n shots in slots 0..n-1: medium windows, tables, records, zeroed state.
+// shot_m / shot_wav (n each, a merged solve): every shot's own medium window
+// and wavelet.
 int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium *override_m,
-            const std::map<int, Window> *override_w = nullptr) {
+            const std::map<int, Window> *override_w = nullptr, const Medium *const *shot_m = nullptr,
+            const float *const *shot_wav = nullptr) {
     const bool born = kind_born(kind), born2 = kind == K_BORN2_FWD;
     const bool grad = kind_grad(kind);
     const bool obs = kind == K_GRAD || kind == K_BORN_GRAD || kind == K_FWD_MISFIT;
-    if (!c->have_model) return fail(c, SFWI_ESTATE, "model not set (sfwi_set_model)");
-    if (!c->have_wav) return fail(c, SFWI_ESTATE, "wavelet not set (sfwi_set_wavelet)");
-    if (born && !override_m && !c->have_pert) return fail(c, SFWI_ESTATE, "perturbation not set (sfwi_set_perturbation)");
+    if (!c->have_model && !shot_m) return fail(c, SFWI_ESTATE, "model not set (sfwi_set_model)");
+    if (!c->have_wav && !shot_wav) return fail(c, SFWI_ESTATE, "wavelet not set (sfwi_set_wavelet)");
+    if (born && !override_m && !shot_m && !c->have_pert)
+        return fail(c, SFWI_ESTATE, "perturbation not set (sfwi_set_perturbation)");
     if (n < 1 || n > c->p.max_batch) return fail(c, SFWI_EINVAL, "batch of %d shots outside [1,%d]", n, c->p.max_batch);
     const bool snaps = grad && c->g.snap_k > 0;
     int r = ensure_slots(c, n, born, grad && !c->p.rbc && !snaps, snaps, born2);
@@ -445,7 +452,7 @@ int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium 
                         "in batch slot %d)", k, s.ntr, k);
         Slot &S = c->slots[k];
         if ((r = receiver_tables(c, s, k, S))) return r;
-        const Medium *m = override_m;
+        const Medium *m = override_m ? override_m : shot_m ? shot_m[k] : nullptr;
         if (!m && override_w) {   // a medium per window position (batched line search)
             const auto ow = override_w->find(s.left);
             if (ow == override_w->end()) return fail(c, SFWI_EINVAL, "shot %d: no test-model window", k);
@@ -481,6 +488,7 @@ int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium 
         d.vold_x = S.vold[0]; d.vold_z = S.vold[1];
         d.isx = s.isx; d.isz = s.isz; d.igz = s.igz; d.ntr = s.ntr; d.ninj = (int)S.h_off.size() - 1;
         d.left = s.left;
+        d.wav = shot_wav ? shot_wav[k] : nullptr;
         c->h_shots[k] = d;
         const size_t cell = c->ncell * sizeof(float);
         HIPCK(c, hipMemsetAsync(S.block, 0, (grad ? 43 : 24) * cell, c->stream));
@@ -629,6 +637,26 @@ int strip_chunk_copy(sfwi_ctx *c, int n, int k, bool to_host) {
         }
     }
     return 0;
+}
+
+// Merge the per-shot images of batch slots [off, off + n) into the context's
+// accumulators, as the end of a gradient solve of `kind` does: FAST mode
+// applies the imaging coefficients, the snapshot gradient weights its sums
+// by snap_k dt and snap_k, the Hessian product merges the Vp / Vs images only.
+void merge_range(sfwi_ctx *c, Kind kind, int off, int n) {
+    const Geo &g = c->g;
+    const DevShot *D = c->d_shots + off;
+    const dim3 mg((unsigned)((c->nmodel + 255) / 256)), blk(BLK);
+    const bool fast = c->p.mode != SFWI_MODE_EXACT, snap = g.snap_k > 0, hess = kind == K_HESS;
+    const float K = (float)g.snap_k;
+    const float wv = snap ? K * g.dt : 1.0f, wi = snap ? K : 1.0f;
+    hipStream_t st = c->stream;
+    float *a = c->Gvp, *b = c->Gvs, *il = c->Gil, *r = c->Grho;
+    const int ax = c->p.allnx;
+    if (fast && hess) hipLaunchKernelGGL((k_merge<true, false>), mg, blk, 0, st, g, D, n, ax, a, b, il, r, wv, wi);
+    else if (fast) hipLaunchKernelGGL((k_merge<true>), mg, blk, 0, st, g, D, n, ax, a, b, il, r, wv, wi);
+    else if (hess) hipLaunchKernelGGL((k_merge<false, false>), mg, blk, 0, st, g, D, n, ax, a, b, il, r, wv, wi);
+    else hipLaunchKernelGGL((k_merge<false>), mg, blk, 0, st, g, D, n, ax, a, b, il, r, wv, wi);
 }
 
 // PD: operand prefetch depth of the column-march kernels; HPD: the same for
@@ -823,13 +851,7 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
             else hipLaunchKernelGGL(km_adj_str<PD>, tiles, blk, 0, st, gm, D, 1);
             tend(c, t);
         }
-        const float K = (float)g.snap_k;
-        if (hess)
-            hipLaunchKernelGGL((k_merge<true, false>), mg, blk, 0, st, g, D, n, c->p.allnx, c->Gvp, c->Gvs, c->Gil,
-                               c->Grho, K * g.dt, K);
-        else
-            hipLaunchKernelGGL((k_merge<true>), mg, blk, 0, st, g, D, n, c->p.allnx, c->Gvp, c->Gvs, c->Gil, c->Grho,
-                               K * g.dt, K);
+        if (!c->defer_merge) merge_range(c, kind, 0, n);
         HIPCK(c, hipGetLastError());
         return 0;
     }
@@ -899,12 +921,7 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
         tend(c, t);
         if (int r = rev_end(it)) return r;
     }
-    if (hess)
-        hipLaunchKernelGGL((k_merge<!E, false>), mg, blk, 0, st, g, D, n, c->p.allnx, c->Gvp, c->Gvs, c->Gil,
-                           c->Grho, 1.0f, 1.0f);
-    else
-        hipLaunchKernelGGL((k_merge<!E>), mg, blk, 0, st, g, D, n, c->p.allnx, c->Gvp, c->Gvs, c->Gil, c->Grho, 1.0f,
-                           1.0f);
+    if (!c->defer_merge) merge_range(c, kind, 0, n);
     HIPCK(c, hipGetLastError());
     return 0;
 }
@@ -1023,6 +1040,108 @@ int copy_records(sfwi_ctx *c, const float *src, float *dst, int ntr) {
     if (dst)
         HIPCK(c, hipMemcpyAsync(dst, src, (size_t)ntr * c->p.nt * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     return 0;
+}
+
+// One part of a merged solve: a request's shots with its own model,
+// perturbation (Born kinds) and wavelet, and its outputs.
+struct Part {
+    const float *vp, *vs, *rho, *dvp, *dvs, *wav;
+    int nshots;
+    const sfwi_shot *shots;
+    float *syn_z, *syn_x;                  // forward kinds: traces of shot i at i*ntr*nt
+    double *misfits;                       // gradient kinds: [nshots]
+    float *gvp, *gvs, *gil;                // gradient kinds: allnx*allnz, the part's shots summed
+};
+
+// Several requests of one shape and kind solved as one batch: every shot
+// gets the medium window of its own request's model and its request's
+// wavelet; each part's images are merged over its own shots only, into
+// accumulators that start at zero, so a part's results equal the same
+// request solved alone bit for bit.  The context's accumulators are zero on
+// return (its model, wavelet and perturbation are not used or changed).
+int solve_merged(sfwi_ctx *c, Kind kind, const std::vector<Part> &parts) {
+    int n = 0;
+    for (const Part &q : parts) n += q.nshots;
+    if (n < 1 || n > c->p.max_batch) return fail(c, SFWI_EINVAL, "merged batch of %d shots outside [1,%d]", n, c->p.max_batch);
+    const bool born = kind_born(kind), grad = kind_grad(kind);
+    const size_t nm = c->nmodel, per = 5 * nm + (size_t)c->p.nt;
+    int r = grow(c, (void **)&c->part_buf, &c->part_buf_cap, parts.size() * per * sizeof(float));
+    if (r) return r;
+    std::vector<std::map<int, Window>> tw(parts.size());
+    auto done = [&](int rc) {
+        hipStreamSynchronize(c->stream);
+        for (auto &mw : tw)
+            for (auto &kv : mw) hipFree(kv.second.base);
+        c->defer_merge = false;
+        return rc;
+    };
+    std::vector<sfwi_shot> shots;
+    std::vector<const Medium *> sm;
+    std::vector<const float *> sw;
+    for (size_t p = 0; p < parts.size(); ++p) {
+        const Part &q = parts[p];
+        if (!q.vp || !q.vs || !q.wav || (!q.rho && !(c->p.rho_const > 0)) || (born && (!q.dvp || !q.dvs)))
+            return done(fail(c, SFWI_EINVAL, "part %zu: model, wavelet or perturbation missing", p));
+        float *b = c->part_buf + p * per;
+        const size_t mb = nm * sizeof(float);
+        HIPCK(c, hipMemcpyAsync(b, q.vp, mb, hipMemcpyHostToDevice, c->stream));
+        HIPCK(c, hipMemcpyAsync(b + nm, q.vs, mb, hipMemcpyHostToDevice, c->stream));
+        if (q.rho) HIPCK(c, hipMemcpyAsync(b + 2 * nm, q.rho, mb, hipMemcpyHostToDevice, c->stream));
+        else HIPCK(c, hipMemsetAsync(b + 2 * nm, 0, mb, c->stream));
+        if (born) {
+            HIPCK(c, hipMemcpyAsync(b + 3 * nm, q.dvp, mb, hipMemcpyHostToDevice, c->stream));
+            HIPCK(c, hipMemcpyAsync(b + 4 * nm, q.dvs, mb, hipMemcpyHostToDevice, c->stream));
+        }
+        HIPCK(c, hipMemcpyAsync(b + 5 * nm, q.wav, c->p.nt * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        for (int k = 0; k < q.nshots; ++k) {
+            const int left = q.shots[k].left;
+            if (left < 0 || left + c->g.nx > c->p.allnx)
+                return done(fail(c, SFWI_EINVAL, "part %zu shot %d: window outside the model", p, k));
+            Window &w = tw[p][left];
+            if (!w.base && (r = build_window(c, w, left, b, b + nm, b + 2 * nm, born ? b + 3 * nm : nullptr,
+                                             born ? b + 4 * nm : nullptr)))
+                return done(r);
+            shots.push_back(q.shots[k]);
+            sm.push_back(&w.m);
+            sw.push_back(b + 5 * nm);
+        }
+    }
+    if ((r = prepare(c, kind, n, shots.data(), nullptr, nullptr, sm.data(), sw.data()))) return done(r);
+    c->defer_merge = true;
+    if ((r = run_direct(c, kind, n))) return done(r);
+    c->defer_merge = false;
+    std::vector<double> mis(n, 0.0);
+    if (grad) HIPCK(c, hipMemcpyAsync(mis.data(), c->d_out, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    const size_t ib = c->nmodel * sizeof(float);
+    int off = 0;
+    for (const Part &q : parts) {
+        if (grad) {   // this part's images alone
+            HIPCK(c, hipMemsetAsync(c->Gvp, 0, 4 * ib, c->stream));
+            merge_range(c, kind, off, q.nshots);
+            HIPCK(c, hipGetLastError());
+            if (q.gvp) HIPCK(c, hipMemcpyAsync(q.gvp, c->Gvp, ib, hipMemcpyDeviceToHost, c->stream));
+            if (q.gvs) HIPCK(c, hipMemcpyAsync(q.gvs, c->Gvs, ib, hipMemcpyDeviceToHost, c->stream));
+            if (q.gil) HIPCK(c, hipMemcpyAsync(q.gil, c->Gil, ib, hipMemcpyDeviceToHost, c->stream));
+        } else {
+            for (int k = 0; k < q.nshots; ++k) {
+                const size_t o = (size_t)k * c->nrec;
+                const Slot &S = c->slots[off + k];
+                if ((r = copy_records(c, S.rz, q.syn_z ? q.syn_z + o : nullptr, q.shots[k].ntr)) ||
+                    (r = copy_records(c, S.rx, q.syn_x ? q.syn_x + o : nullptr, q.shots[k].ntr)))
+                    return done(r);
+            }
+        }
+        off += q.nshots;
+    }
+    if (grad) HIPCK(c, hipMemsetAsync(c->Gvp, 0, 4 * ib, c->stream));
+    if ((r = finish(c))) return done(r);
+    off = 0;
+    for (const Part &q : parts) {
+        if (grad && q.misfits)
+            for (int k = 0; k < q.nshots; ++k) q.misfits[k] = mis[off + k];
+        off += q.nshots;
+    }
+    return done(0);
 }
 
 }  // namespace
@@ -1214,6 +1333,23 @@ int sfwi_set_perturbation_region(sfwi_ctx *c, const float *dvp, const float *dvs
 
 int sfwi_set_perturbation(sfwi_ctx *c, const float *dvp, const float *dvs) {
     return sfwi_set_perturbation_region(c, dvp, dvs, SFWI_BORN_S2);
+}
+
+int sfwi_batch_born_forward(sfwi_ctx *c, int nshots, const sfwi_shot *shots, float *b_z, float *b_x) {
+    if (!c || nshots < 0 || (nshots > 0 && !shots)) return fail(c, SFWI_EINVAL, "bad shot list");
+    for (int b0 = 0; b0 < nshots; b0 += c->p.max_batch) {
+        const int n = std::min(c->p.max_batch, nshots - b0);
+        int r = prepare(c, K_BORN_FWD, n, shots + b0, nullptr);
+        if (r || (r = run(c, K_BORN_FWD, n))) return r;
+        for (int k = 0; k < n; ++k) {
+            const size_t o = (size_t)(b0 + k) * c->nrec;
+            if ((r = copy_records(c, c->slots[k].rz, b_z ? b_z + o : nullptr, shots[b0 + k].ntr)) ||
+                (r = copy_records(c, c->slots[k].rx, b_x ? b_x + o : nullptr, shots[b0 + k].ntr)))
+                return r;
+        }
+        if ((r = finish(c))) return r;
+    }
+    return 0;
 }
 
 int sfwi_batch_born2_forward(sfwi_ctx *c, int nshots, const sfwi_shot *shots, float *b_z, float *b_x, float *b2_z,

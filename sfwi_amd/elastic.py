@@ -163,6 +163,14 @@ class Context:
         self._ck(self.L.sfwi_batch_forward(self.h, len(shots), arr, _lib.ptr(z), _lib.ptr(x)))
         return z, x
 
+    def batch_born_forward(self, shots):
+        """Scattered traces of every shot: arrays (nshots, ntr_max, nt)."""
+        arr = (SfwiShot * len(shots))(*[s.c for s in shots])
+        z = np.zeros((len(shots), self.p.ntr, self.nt), np.float32)
+        x = np.zeros_like(z)
+        self._ck(self.L.sfwi_batch_born_forward(self.h, len(shots), arr, _lib.ptr(z), _lib.ptr(x)))
+        return z, x
+
     def born_forward(self, shot):
         out = [np.zeros((shot.ntr, self.nt), np.float32) for _ in range(4)]
         self._ck(self.L.sfwi_shot_born_forward(self.h, ctypes.byref(shot.c), *[_lib.ptr(a) for a in out]))
@@ -300,7 +308,7 @@ class Server:
     KINDS = {"forward": _lib.REQ_FORWARD, "born_forward": _lib.REQ_BORN_FORWARD,
              "gradient": _lib.REQ_GRADIENT, "born_gradient": _lib.REQ_BORN_GRADIENT}
 
-    def __init__(self, budget_bytes=0, workers=4):
+    def __init__(self, budget_bytes=0, workers=4, batching=True):
         self.L = _lib.load()
         h = ctypes.c_void_p()
         rc = self.L.sfwi_server_create(int(budget_bytes), int(workers), ctypes.byref(h))
@@ -308,6 +316,14 @@ class Server:
             raise SfwiError(rc, self.L.sfwi_last_error(None).decode())
         self.h = h
         self._live = {}
+        self.set_batching(batching)
+
+    def set_batching(self, on):
+        """Cross-request batching: queued requests of one shape and kind join a
+        worker's batch (up to max_batch shots)."""
+        rc = self.L.sfwi_server_batching(self.h, int(bool(on)))
+        if rc != 0:
+            raise SfwiError(rc, self.L.sfwi_last_error(None).decode())
 
     @staticmethod
     def solve_bytes(kind, nshots, **params):

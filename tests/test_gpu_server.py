@@ -98,3 +98,49 @@ def test_budget_serialises_admission_and_refuses_oversized():
                              ntr=sv.ntr)
         with pytest.raises(SfwiError, match="budget"):
             _submit(srv, big)
+
+
+def _same_shape_requests(kind, n, max_batch=6):
+    """n requests of one shape (grid, record length, max_batch) with their own
+    models, wavelets, perturbations and shot counts."""
+    reqs = []
+    for j in range(n):
+        sv = W.tiny(nx=48, nz=40, nt=200, seed=40 + j, f0=20.0 + 2 * j, nshots=1 + j % 2)
+        w = W.ricker(sv.nt, sv.dt, sv.f0)
+        born = kind.startswith("born")
+        params = dict(dx=sv.dx, dz=sv.dz, dt=sv.dt, nx=sv.nx, nz=sv.allnz, pml=sv.pml, nt=sv.nt, ntr=sv.ntr,
+                      allnx=sv.allnx, allnz=sv.allnz, max_batch=max_batch,
+                      rho_const=2000.0 if born else 0.0, adjoint_dir=-1 if born else 1)
+        with Context(**params) as ctx:
+            ctx.set_wavelet(w)
+            ctx.set_model(sv.vp, sv.vs, sv.rho)
+            oz, ox = ctx.batch_forward([lib_shot(sv, s) for s in sv.shots])
+        shots = [lib_shot(sv, s, oz[k][:len(s.gc)], ox[k][:len(s.gc)]) for k, s in enumerate(sv.shots)]
+        dvp, dvs = ((sv.vp - sv.vp0) * 0.5, (sv.vs - sv.vs0) * 0.5) if born else (None, None)
+        reqs.append(dict(kind=kind, params=params, shots=shots, wavelet=w, vp=sv.vp0, vs=sv.vs0, rho=sv.rho0,
+                         dvp=dvp, dvs=dvs))
+    return reqs
+
+
+@pytest.mark.parametrize("kind", ["forward", "born_forward", "gradient", "born_gradient"])
+def test_same_shape_requests_share_a_batch(kind):
+    """Requests of one shape and kind queued behind a running one join one
+    batch (one worker: the first runs alone, the next ones queue and merge up
+    to max_batch = 6 shots); every request's result equals the request solved
+    alone bit for bit, each with its own model, wavelet and perturbation."""
+    reqs = _same_shape_requests(kind, 6)
+    alone = [_alone(r) for r in reqs]
+    with Server(workers=1) as srv:
+        tickets = [_submit(srv, r) for r in reqs]
+        outs = [srv.wait(t) for t in tickets]
+    for o, a in zip(outs, alone):
+        _same(o, a)
+    # merged requests start and end together
+    runs = [o["run_ms"] for o in outs]
+    assert any(runs.count(x) >= 2 for x in runs), runs
+    with Server(workers=1, batching=False) as srv:   # one request per batch: the same results
+        outs = [srv.wait(t) for t in [_submit(srv, r) for r in reqs]]
+    for o, a in zip(outs, alone):
+        _same(o, a)
+    runs = [o["run_ms"] for o in outs]
+    assert len(set(runs)) == len(runs)
