@@ -212,6 +212,10 @@ def main():
     ap.add_argument("--rate", type=float, default=1.0, help="serving: requests per second")
     ap.add_argument("--scale", type=float, default=1.0, help="serving: request size factor")
     ap.add_argument("--workers", type=int, default=4, help="serving: concurrent solves")
+    ap.add_argument("--shapes", type=int, default=0,
+                    help="serving: requests drawn from this many (nx, nz, nt) shapes (0: every request its own)")
+    ap.add_argument("--merge-cap", type=int, default=16,
+                    help="serving: shots a batch of same-shape requests may hold (the requests' max_batch)")
     ap.add_argument("--cpu-shots", type=int, default=1)
     ap.add_argument("--plain-shots", type=int, default=3,
                     help="shots timed through the plain one-shot path (0: skip)")
@@ -611,8 +615,9 @@ def born(args):
     ctx.close()
 
 
-def _request(kind, sv):
-    """Arguments of one serving request (zero observed traces for gradients)."""
+def _request(kind, sv, max_batch=0):
+    """Arguments of one serving request (zero observed traces for gradients);
+    max_batch: room for same-shape requests to join its batch."""
     from sfwi_amd import workloads as W
     from sfwi_amd.elastic import Shot
     born = kind.startswith("born")
@@ -622,7 +627,7 @@ def _request(kind, sv):
         z = np.zeros((len(s.gc), sv.nt), np.float32) if "gradient" in kind else None
         shots.append(Shot(s.left, isx, isz, igz, s.ngx_left, s.gc, z, z))
     params = dict(dx=sv.dx, dz=sv.dz, dt=sv.dt, nx=sv.nx, nz=sv.allnz, pml=sv.pml, nt=sv.nt, ntr=sv.ntr,
-                  allnx=sv.allnx, allnz=sv.allnz, max_batch=len(shots),
+                  allnx=sv.allnx, allnz=sv.allnz, max_batch=max(len(shots), max_batch),
                   rho_const=2000.0 if born else 0.0, adjoint_dir=-1 if born else 1)
     return dict(kind=kind, shots=shots, wavelet=W.ricker(sv.nt, sv.dt, sv.f0), vp=sv.vp0, vs=sv.vs0,
                 rho=sv.rho0, dvp=(sv.vp - sv.vp0) if born else None, dvs=(sv.vs - sv.vs0) if born else None,
@@ -836,12 +841,16 @@ def serving(args):
     shots, 10-20 Hz) submitted in real time to the library's server, which
     admits them against the HBM budget and overlaps them on the GPU.  Reports
     p50/p99 of request latency (submit to done) and throughput, beside the same
-    requests run one at a time through the plain one-shot path."""
+    requests run one at a time through the plain one-shot path.  Requests of
+    one shape and kind queued together share a batch (up to --merge-cap
+    shots); the saturated pass is also run with that cross-request batching
+    off.  --shapes K draws every request's grid and record length from K
+    shapes (the default stream gives every request its own)."""
     import threading
     from sfwi_amd import workloads as W
     from sfwi_amd.elastic import Server
-    stream = W.serving_requests(args.requests, rate=args.rate, scale=args.scale)
-    reqs = [(t, _request(k, sv), sv) for t, k, sv in stream]
+    stream = W.serving_requests(args.requests, rate=args.rate, scale=args.scale, shapes=args.shapes)
+    reqs = [(t, _request(k, sv, args.merge_cap), sv) for t, k, sv in stream]
     cells = sum(sv.cells() * sv.nt * len(sv.shots) * {"forward": 1, "born_forward": 2, "gradient": 3,
                                                         "born_gradient": 4}[r["kind"]] for _, r, sv in reqs)
     shots = sum(len(r["shots"]) for _, r, _ in reqs)
@@ -874,13 +883,18 @@ def serving(args):
             th.join()
         makespan = max(done_at) - t0
         print("serving: paced makespan %.2f s" % makespan, file=sys.stderr, flush=True)
-        # saturated: the same requests all queued at once (throughput)
-        t0 = time.perf_counter()
-        tks = [_submit(srv, r) for _, r, _ in reqs]
-        for tk in tks:
-            srv.wait(tk)
-        t_sat = time.perf_counter() - t0
-    print("serving: saturated makespan %.2f s" % t_sat, file=sys.stderr, flush=True)
+        # saturated: the same requests all queued at once (throughput), with and
+        # without cross-request batching (the cached contexts are reused)
+        sat = {}
+        for on in (False, True):
+            srv.set_batching(on)
+            t0 = time.perf_counter()
+            tks = [_submit(srv, r) for _, r, _ in reqs]
+            outs = [srv.wait(tk) for tk in tks]
+            sat[on] = (time.perf_counter() - t0, len(set(o["run_ms"] for o in outs)))
+        t_sat = sat[True][0]
+    print("serving: saturated makespan %.2f s (batching off: %.2f s)" % (t_sat, sat[False][0]), file=sys.stderr,
+          flush=True)
     alone = []
     for _, r, _ in reqs:
         t1 = time.perf_counter()
@@ -904,6 +918,10 @@ def serving(args):
         "gcell_updates_per_s": round(cells / t_sat / 1e9, 2),
         "paced_makespan_s": round(makespan, 3),
         "saturated_makespan_s": round(t_sat, 3),
+        "cross_request_batching": {"merge_cap_shots": args.merge_cap, "shapes": args.shapes or "per request",
+                                   "batches": sat[True][1],
+                                   "off": {"value": round(shots / sat[False][0], 4), "unit": "shot-solves/s",
+                                           "makespan_s": round(sat[False][0], 3), "batches": sat[False][1]}},
         "one_at_a_time": {"value": round(shots / t_alone, 4), "unit": "shot-solves/s", "seconds": round(t_alone, 3),
                           "request_ms": {"p50": round(float(np.percentile(ams, 50)), 1),
                                          "p99": round(float(np.percentile(ams, 99)), 1)},

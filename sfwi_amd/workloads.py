@@ -236,16 +236,22 @@ def uneven_batch(shots_per_model=30):
             mixed_precision(nshots=n, nt=6000)]
 
 
-def serving_requests(n=24, seed=1005, rate=1.0, scale=1.0):
+def serving_requests(n=24, seed=1005, rate=1.0, scale=1.0, shapes=0):
     """Serving-loop request stream: Poisson arrivals at `rate` per second of
     forward, Born and gradient solves; per request nx uniform in 200-1500, nz
     in 150-800, nt in 1500-6000 (all multiplied by `scale`, for quick runs), a
     seeded layered Vp/Vs/rho model, 1-8 shots and a 10-20 Hz Ricker wavelet;
     dx = dz = 10 m, dt = 1 ms, 20-cell PML.  Gradient requests invert zero
     observed traces (the misfit is the synthetic energy), Born requests
-    perturb Vp/Vs by a Gaussian anomaly.  Returns [(arrival_s, kind, survey)]."""
+    perturb Vp/Vs by a Gaussian anomaly.  shapes > 0: the (nx, nz, nt) of
+    each request is one of `shapes` triples drawn the same way (a service
+    whose clients share a few grids; models, shots and wavelets stay per
+    request).  Returns [(arrival_s, kind, survey)]."""
     rng = np.random.default_rng(seed)
     kinds = ["forward", "born_forward", "gradient", "born_gradient"]
+    srng = np.random.default_rng(seed + 1)
+    table = [(max(32, int(srng.integers(200, 1501) * scale)), max(32, int(srng.integers(150, 801) * scale)),
+              max(64, int(srng.integers(1500, 6001) * scale))) for _ in range(shapes)]
     t = 0.0
     out = []
     for k in range(n):
@@ -253,6 +259,8 @@ def serving_requests(n=24, seed=1005, rate=1.0, scale=1.0):
         nx = max(32, int(rng.integers(200, 1501) * scale))
         nz = max(32, int(rng.integers(150, 801) * scale))
         nt = max(64, int(rng.integers(1500, 6001) * scale))
+        if shapes:
+            nx, nz, nt = table[int(srng.integers(0, shapes))]
         nsh = int(rng.integers(1, 9))
         f0 = float(rng.uniform(10.0, 20.0))
         nl = int(rng.integers(3, 7))
