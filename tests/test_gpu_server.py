@@ -100,7 +100,7 @@ def test_budget_serialises_admission_and_refuses_oversized():
             _submit(srv, big)
 
 
-def _same_shape_requests(kind, n, max_batch=6):
+def _same_shape_requests(kind, n, max_batch=6, **extra):
     """n requests of one shape (grid, record length, max_batch) with their own
     models, wavelets, perturbations and shot counts."""
     reqs = []
@@ -110,7 +110,7 @@ def _same_shape_requests(kind, n, max_batch=6):
         born = kind.startswith("born")
         params = dict(dx=sv.dx, dz=sv.dz, dt=sv.dt, nx=sv.nx, nz=sv.allnz, pml=sv.pml, nt=sv.nt, ntr=sv.ntr,
                       allnx=sv.allnx, allnz=sv.allnz, max_batch=max_batch,
-                      rho_const=2000.0 if born else 0.0, adjoint_dir=-1 if born else 1)
+                      rho_const=2000.0 if born else 0.0, adjoint_dir=-1 if born else 1, **extra)
         with Context(**params) as ctx:
             ctx.set_wavelet(w)
             ctx.set_model(sv.vp, sv.vs, sv.rho)
@@ -122,13 +122,18 @@ def _same_shape_requests(kind, n, max_batch=6):
     return reqs
 
 
-@pytest.mark.parametrize("kind", ["forward", "born_forward", "gradient", "born_gradient"])
-def test_same_shape_requests_share_a_batch(kind):
+@pytest.mark.parametrize("kind,extra", [("forward", {}), ("born_forward", {}), ("gradient", {}),
+                                        ("born_gradient", {}),
+                                        ("gradient", dict(recon="snapshots", storage="f16", snap_every=3)),
+                                        ("gradient", dict(mode="exact")),
+                                        ("gradient", dict(misfit="ot_trace")),
+                                        ("born_gradient", dict(march_cols=8, storage="bf16"))])
+def test_same_shape_requests_share_a_batch(kind, extra):
     """Requests of one shape and kind queued behind a running one join one
     batch (one worker: the first runs alone, the next ones queue and merge up
     to max_batch = 6 shots); every request's result equals the request solved
     alone bit for bit, each with its own model, wavelet and perturbation."""
-    reqs = _same_shape_requests(kind, 6)
+    reqs = _same_shape_requests(kind, 6, **extra)
     alone = [_alone(r) for r in reqs]
     with Server(workers=1) as srv:
         tickets = [_submit(srv, r) for r in reqs]
