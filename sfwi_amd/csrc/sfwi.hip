@@ -818,7 +818,6 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
     if (!grad) return 0;
 
     const int adir = c->p.adjoint_dir;
-    const dim3 mg((unsigned)((c->nmodel + 255) / 256));
     if (snap) {
         // Snapshot (consistent) gradient: per reverse step the adjoint source
         // fac r / rho is added at the receivers, then the adjoint velocity half
@@ -1275,8 +1274,10 @@ void sfwi_ctx_destroy(sfwi_ctx *c) {
     for (Slot &S : c->slots) {
         hipFree(S.block);
         hipFree(S.born);
+        hipFree(S.born2);
         hipFree(S.rec);
         hipFree(S.strip);
+        hipFree(S.snap);
         if (S.hstrip) hipHostFree(S.hstrip);
     }
     for (int k = 0; k < 2; ++k) {
@@ -1290,6 +1291,7 @@ void sfwi_ctx_destroy(sfwi_ctx *c) {
     hipFree(c->wav); hipFree(c->mvp); hipFree(c->tmp_model); hipFree(c->Gvp); hipFree(c->res_keep);
     hipFree(c->d_shots); hipFree(c->d_out); hipFree(c->d_num); hipFree(c->res_batch); hipFree(c->dot_batch); hipFree(c->d_vmax);
     hipFree(c->ot_jobs); hipFree(c->ot_ws); hipFree(c->ot_mis); hipFree(c->ot_first); hipFree(c->ot_io);
+    hipFree(c->part_buf);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
