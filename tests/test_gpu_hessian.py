@@ -161,3 +161,21 @@ def test_gauss_newton_inner_loop_exact():
     assert hist == ohist
     assert np.array_equal(d1, orc["grad"][0]) and np.array_equal(d2, orc["grad"][1])
     assert all(h[2] > 0 for h in hist)   # p.Hp > 0: the Gauss-Newton Hessian is positive along p
+
+
+@pytest.mark.parametrize("extra,okw", [(dict(rbc=True, rbc_seed=11), dict(rbc=1, rbc_seed=11)),
+                                       (dict(cpu_mem=True, strip_chunk=30), {}),
+                                       (dict(use_graph=True), {})])
+def test_hessian_vector_storage_variants_exact(extra, okw):
+    """The random boundary (no strips: time reversal, the oracle's rbc), strips
+    in pinned host memory and graph replays: EXACT = the oracle bit for bit."""
+    sv, w = _case()
+    p = _vec(sv, 8, edge=True)
+    want = _oracle_hp(sv, w, p, **okw)
+    with context(sv, "exact", max_batch=len(sv.shots), **extra) as ctx:
+        ctx.set_wavelet(w)
+        ctx.set_model(sv.vp0, sv.vs0, sv.rho0)
+        for _ in range(3):   # with use_graph the later calls replay the captured solve
+            ctx.hessian_vector([lib_shot(sv, sh) for sh in sv.shots], *p)
+            gvp, gvs, _ = ctx.flush()
+            assert np.array_equal(gvp, want[0]) and np.array_equal(gvs, want[1])
