@@ -250,14 +250,17 @@ void random_boundary(const Grid &g, unsigned seed, std::vector<T> &vp, std::vect
 }
 
 // lambda = rho (vp^2 - 2 vs^2), mu = rho vs^2: sfwi-stage1/FD2DGPU.cu:1116-1124;
-// stage 2 forces rho := 2000 inside the same kernel (sfwi-stage2/FD2DGPU.cu:1111).
+// stage 2 forces rho := 2000 on every padded cell inside the same kernel,
+// before lambda and mu are formed (sfwi-stage2/FD2DGPU.cu:1106-1113).
+// or_config.plant & 64 (test hook) forces it after them instead.
 template <typename T>
 void build_medium(const or_config *c, const Grid &g, const float *vp, const float *vs,
                   const float *rho, const float *dvp, const float *dvs, Medium<T> &m) {
     window_pad(g, c->allnz, c->left, vp, m.vp);
     window_pad(g, c->allnz, c->left, vs, m.vs);
     window_pad(g, c->allnz, c->left, rho, m.rho);
-    if (c->rho_const > 0)
+    const bool late_rho = (c->plant & 64) != 0;
+    if (c->rho_const > 0 && !late_rho)
         for (long i = 0; i < g.n; ++i) m.rho[i] = (T)c->rho_const;
     if (c->rbc) random_boundary(g, c->rbc_seed, m.vp, m.vs);
     damping(g, c->rr, c->pml_off != 0 || c->rbc != 0, m.vp, m.dampx, m.dampz);
@@ -267,6 +270,8 @@ void build_medium(const or_config *c, const Grid &g, const float *vp, const floa
         m.lam[i] = m.rho[i] * (m.vp[i] * m.vp[i] - 2 * m.vs[i] * m.vs[i]);
         m.mu[i] = m.rho[i] * m.vs[i] * m.vs[i];
     }
+    if (c->rho_const > 0 && late_rho)
+        for (long i = 0; i < g.n; ++i) m.rho[i] = (T)c->rho_const;
     if (dvp) window_pad(g, c->allnz, c->left, dvp, m.dvp);   // buffer_image_extrapolation :1336-1345
     if (dvs) window_pad(g, c->allnz, c->left, dvs, m.dvs);
 }

@@ -35,9 +35,11 @@
  * quadratic form against the Born energy, central differences of the
  * consistent gradient, the reference's product as the gradient loop on the
  * scattered traces, the stage-1 Born cells, and double scattering as the
- * second-order term of the traces' expansion.  Parity unpinned (restatement
- * of the cited lines only): the imaging position in the reverse loop (after
- * the adjoint stress-like half step) and the stage-2 density override.
+ * second-order term of the traces' expansion.  The stage-2 density override
+ * (tests/test_oracle_orchestration.py): a run with it equals, bit for bit,
+ * the run whose density file is the constant itself, also on the padding.
+ * Parity unpinned (restatement of the cited lines only): the imaging position
+ * in the reverse loop (after the adjoint stress-like half step).
  *
  * Random boundary (rbc): the reference draws it from cuRAND states seeded with
  * time(0) (sfwi-stage1/FD2DGPU.cu:155-230, 3245-3305), so no run repeats it;
@@ -81,7 +83,8 @@ typedef struct or_config {
                           /*   the velocity update, 8: the reconstruction undoes the      */
                           /*   source (the exact inverse), 16: the Born source without    */
                           /*   the -4 rho vs dvs term, 32: the second-order scattered     */
-                          /*   field driven by the background instead of the first order */
+                          /*   field driven by the background instead of the first order,*/
+                          /*   64: the density override after lambda and mu are formed   */
 } or_config;
 
 /* FD coefficients c1..c4 of the 8th-order staggered stencil (index 0 unused). */

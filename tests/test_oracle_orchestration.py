@@ -126,3 +126,30 @@ def test_reconstruction_that_subtracts_the_source_fails():
     S, _ = O.reconstruct(c, sv.vp, sv.vs, sv.rho, w, 1)
     A, _ = O.reconstruct(c, sv.vp, sv.vs, sv.rho, w, 2)
     assert rel_l2(F - S, A) > 0.5
+
+
+def _rho_case(rho_const, plant=0, const_file=False):
+    sv = W.tiny(nx=48, nz=40, nt=240, seed=33)
+    sh = sv.shots[0]
+    c = oracle_cfg(sv, sh, rho_const=rho_const, adjoint_dir=-1, plant=plant)
+    rho = np.full_like(sv.rho, 2000.0) if const_file else sv.rho
+    dvp, dvs = (sv.vp - sv.vp0) * 0.5, (sv.vs - sv.vs0) * 0.5
+    w = W.ricker(sv.nt, sv.dt, sv.f0)
+    return sv, sh, c, rho, dvp, dvs, w
+
+
+def test_density_override_equals_a_constant_density_file():
+    """Stage 2's rho := 2000 (sfwi-stage2/FD2DGPU.cu:1106-1113) replaces the
+    density on every padded cell before lambda and mu are formed, so a run
+    with it equals, bit for bit, the run whose density file holds 2000
+    everywhere -- background traces and Born traces, FP32 and FP64.  An
+    override formed after lambda and mu (plant 64) fails."""
+    assert not np.all(_rho_case(0.0)[3] == 2000.0)   # the file's own density differs
+    for prec in ("f32", "f64"):
+        outs = []
+        for rho_const, const_file, plant in ((2000.0, False, 0), (0.0, True, 0), (2000.0, False, 64)):
+            sv, sh, c, rho, dvp, dvs, w = _rho_case(rho_const, plant, const_file)
+            outs.append(O.born_forward(c, sv.vp, sv.vs, rho, dvp, dvs, w, sh.gc, prec))
+        for a, b in zip(outs[0], outs[1]):
+            assert np.array_equal(a, b), prec
+        assert not all(np.array_equal(a, b) for a, b in zip(outs[2], outs[1])), prec
