@@ -67,6 +67,8 @@ def lib():
                                           _f64p, _f64p, _f64p, _f64p, dp]
         L.or_gradient_res_f32.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _i32p, _f32p, _f32p,
                                           _f32p, _f32p, _f32p]
+        L.or_gradient_res_f64.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _i32p, _f32p, _f32p,
+                                          _f64p, _f64p, _f64p]
         L.or_gradient_consistent_f64.argtypes = [cfg, _f32p, _f32p, _f32p, _f32p, _i32p, _f32p, _f32p,
                                                  _f64p, _f64p, _f64p, dp]
         _nf32 = ctypes.c_void_p   # nullable float arrays
@@ -219,12 +221,14 @@ def reconstruct(c, vp, vs, rho, wav, kind, init=None, errs=False):
     return out.reshape(15, nxp, nzp), e
 
 
-def gradient_res(c, vp, vs, rho, wav, gc, res_z, res_x):
-    """Gradient (FP32 order) of a misfit whose adjoint source is res_z / res_x."""
+def gradient_res(c, vp, vs, rho, wav, gc, res_z, res_x, prec="f32"):
+    """Gradient (FP32 order, or FP64) of a misfit whose adjoint source is res_z / res_x."""
     n = c.allnx * c.allnz
-    g1, g2, g3 = np.zeros(n, np.float32), np.zeros(n, np.float32), np.zeros(n, np.float32)
-    lib().or_gradient_res_f32(ctypes.byref(c), _f32(vp), _f32(vs), _f32(rho), _f32(wav),
-                              np.ascontiguousarray(gc, np.int32), _f32(res_z), _f32(res_x), g1, g2, g3)
+    dt = np.float32 if prec == "f32" else np.float64
+    g1, g2, g3 = np.zeros(n, dt), np.zeros(n, dt), np.zeros(n, dt)
+    getattr(lib(), "or_gradient_res_" + prec)(ctypes.byref(c), _f32(vp), _f32(vs), _f32(rho), _f32(wav),
+                                              np.ascontiguousarray(gc, np.int32), _f32(res_z), _f32(res_x),
+                                              g1, g2, g3)
     sh = (c.allnx, c.allnz)
     return g1.reshape(sh), g2.reshape(sh), g3.reshape(sh)
 

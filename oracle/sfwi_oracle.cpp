@@ -581,11 +581,25 @@ template <typename T> struct Run {
     }
     // Adjoint source: the residual overwrites the adjoint velocity at the
     // receivers (stack=false), sfwi-stage1/FD2DGPU.cu:3425-3441 (quirk Q6).
+    // Only the totals are written, and the adjoint velocity step rebuilds
+    // them from the split parts, so the residual reaches the adjoint field
+    // through the stress-like step that follows alone.  or_config.plant & 128
+    // (test hook) adds it to the totals and the x split parts instead, as the
+    // consistent gradient does.
     void inject(Wave<T> &bk, int it, const std::vector<T> &rz, const std::vector<T> &rx) {
+        const bool add = (c->plant & 128) != 0;
         for (int tr = 0; tr < g.ntr; ++tr) {
             long i = at(igx[tr], c->igz);
-            bk.vz[i] = rz[(long)tr * g.nt + it];
-            bk.vx[i] = rx[(long)tr * g.nt + it];
+            if (add) {
+                const T az = rz[(long)tr * g.nt + it], ax = rx[(long)tr * g.nt + it];
+                bk.vz[i] += az;
+                bk.vz_x[i] += az;
+                bk.vx[i] += ax;
+                bk.vx_x[i] += ax;
+            } else {
+                bk.vz[i] = rz[(long)tr * g.nt + it];
+                bk.vx[i] = rx[(long)tr * g.nt + it];
+            }
         }
     }
 
@@ -682,7 +696,8 @@ template <typename T> struct Run {
     // stage 2 :4379-4510): restore strips, reconstruct (mode 6: stress with
     // derivatives, then velocity, direction -1, source not subtracted, Q5),
     // inject the residual, adjoint step (mode 8) with the stage's direction
-    // (Q7), image.
+    // (Q7), image.  or_config.plant & 256 (test hook) images before the
+    // adjoint step instead.
     template <typename G>
     void reverse(Wave<T> &fw, Strips<T> &strips, const std::vector<T> &rz, const std::vector<T> &rx,
                  G *gvp, G *gvs, G *ill, G *grho = nullptr) {
@@ -693,9 +708,13 @@ template <typename T> struct Run {
             reconstruct(fw, &strips, it, vx_gx.data(), vz_gz.data(), vx_gz.data(), vz_gx.data(),
                         grho ? &ovx : nullptr, grho ? &ovz : nullptr, (c->plant & 8) ? +1 : 0);
             inject(bk, it, rz, rx);
+            const bool early = (c->plant & 256) != 0;
+            if (early)
+                image_step(c, g, m, fw, bk, vx_gx, vz_gz, vx_gz, vz_gx, gvp, gvs, ill, nth, grho, &ovx, &ovz, st.dt);
             adjoint_s(g, st, m, bk, c->adjoint_dir, nth);
             adjoint_v(g, st, m, bk, c->adjoint_dir, nth);
-            image_step(c, g, m, fw, bk, vx_gx, vz_gz, vx_gz, vz_gx, gvp, gvs, ill, nth, grho, &ovx, &ovz, st.dt);
+            if (!early)
+                image_step(c, g, m, fw, bk, vx_gx, vz_gz, vx_gz, vz_gx, gvp, gvs, ill, nth, grho, &ovx, &ovz, st.dt);
         }
     }
 };
@@ -1160,6 +1179,11 @@ void or_gradient_res_f32(const or_config *c, const float *vp, const float *vs, c
                          const float *wav, const int *gc, const float *res_z, const float *res_x,
                          float *gvp, float *gvs, float *ill) {
     gradient_res_impl<float>(c, vp, vs, rho, wav, gc, res_z, res_x, gvp, gvs, ill);
+}
+void or_gradient_res_f64(const or_config *c, const float *vp, const float *vs, const float *rho,
+                         const float *wav, const int *gc, const float *res_z, const float *res_x,
+                         double *gvp, double *gvs, double *ill) {
+    gradient_res_impl<double>(c, vp, vs, rho, wav, gc, res_z, res_x, gvp, gvs, ill);
 }
 
 }  // extern "C"

@@ -38,19 +38,25 @@
  * second-order term of the traces' expansion.  The stage-2 density override
  * (tests/test_oracle_orchestration.py): a run with it equals, bit for bit,
  * the run whose density file is the constant itself, also on the padding.
- * Parity unpinned (restatement of the cited lines only): the imaging position
- * in the reverse loop (after the adjoint stress-like half step).
+ * The imaging position in the faithful reverse loop (same file): with the
+ * reconstruction made exact and the residual added (plants 8 and 128), the
+ * loop's Vp / Vs images equal, to FP64 round-off, the consistent gradient of
+ * the wavelet advanced by one sample (imaging after the adjoint stress-like
+ * step pairs the derivatives of v(it + 1) with the consistent gradient's
+ * adjoint stresses of one iteration later); imaging before that step (plant
+ * 256) equals the consistent gradient of the wavelet itself instead.
  *
  * Random boundary (rbc): the reference draws it from cuRAND states seeded with
  * time(0) (sfwi-stage1/FD2DGPU.cu:155-230, 3245-3305), so no run repeats it;
  * this oracle and the library share a project-owned generator instead (the
  * lowbias32 integer hash of C. Wellons, 2018, on seed, cell and attempt),
  * computed in FP32 on both sides.  Pinned by the band values' range and vs/vp
- * ratio, an untouched interior and zero damping.  Parity unpinned: the
- * gradient's orchestration with it (no strips saved or restored,
- * sfwi-stage1/FD2DGPU.cu:4543, 4602), a restatement; with the source left in
- * the reconstruction (Q5) time reversal does not return the forward field, so
- * no reversal check applies to the gradient itself.
+ * ratio, an untouched interior and zero damping.  The gradient's
+ * orchestration with it (no strips saved or restored,
+ * sfwi-stage1/FD2DGPU.cu:4543, 4602): the exact inverse returns the zero start
+ * on the whole grid by reversal alone, the faithful reconstruction minus it
+ * is the source alone run backwards (Q5, linearity), and the faithful loop on
+ * the random medium meets the same advanced-wavelet identity as above.
  */
 #ifndef SFWI_ORACLE_H
 #define SFWI_ORACLE_H
@@ -84,7 +90,10 @@ typedef struct or_config {
                           /*   source (the exact inverse), 16: the Born source without    */
                           /*   the -4 rho vs dvs term, 32: the second-order scattered     */
                           /*   field driven by the background instead of the first order,*/
-                          /*   64: the density override after lambda and mu are formed   */
+                          /*   64: the density override after lambda and mu are formed,  */
+                          /*   128: the faithful reverse loop adds the residual to the    */
+                          /*   totals and the x split parts instead of overwriting the    */
+                          /*   totals, 256: it images before the adjoint step             */
 } or_config;
 
 /* FD coefficients c1..c4 of the 8th-order staggered stencil (index 0 unused). */
@@ -157,6 +166,9 @@ void or_gradient_snapshot_f64(const or_config *c, const float *vp, const float *
 void or_gradient_res_f32(const or_config *c, const float *vp, const float *vs, const float *rho,
                          const float *wav, const int *gc, const float *res_z, const float *res_x,
                          float *grad_vp, float *grad_vs, float *illum);
+void or_gradient_res_f64(const or_config *c, const float *vp, const float *vs, const float *rho,
+                         const float *wav, const int *gc, const float *res_z, const float *res_x,
+                         double *grad_vp, double *grad_vs, double *illum);
 
 /* Stage-2 Born forward: scattered traces b_* and background traces bg_*. */
 void or_born_forward_f32(const or_config *c, const float *vp, const float *vs, const float *rho,
