@@ -29,6 +29,10 @@ extern "C" {
 #define SFWI_EHIP (-2)     /* HIP runtime error                              */
 #define SFWI_ENOMEM (-3)   /* device allocation failed or over HBM budget    */
 #define SFWI_ESTATE (-4)   /* call order (model/wavelet not set, ...)        */
+#define SFWI_ENUMERIC (-5) /* a shot's misfit is not finite (NaN or Inf in    */
+                           /* its observed traces)                           */
+#define SFWI_ECANCELED (-6) /* the request was cancelled before it started   */
+#define SFWI_ETIMEOUT (-7) /* sfwi_server_wait_for: not done in time         */
 
 /* Numerical modes. */
 #define SFWI_MODE_FAST 0       /* FP32, unsplit interior, FMA stencils          */
@@ -181,8 +185,16 @@ const char *sfwi_last_error(const sfwi_ctx *ctx);
  * sfwi-stage1/FD2DGPU.cu:46-47. */
 int sfwi_set_wavelet(sfwi_ctx *ctx, const float *wavelet);
 /* Full models (allnx*allnz each): FD2DGPU_ELASTIC::GPUbufferVPVS
- * (sfwi-stage1/FD2DGPU.cu:1322-1327).  rho may be NULL when rho_const > 0. */
+ * (sfwi-stage1/FD2DGPU.cu:1322-1327).  rho may be NULL when rho_const > 0.
+ * The model is checked first (sfwi_check_model); a bad one is refused with
+ * SFWI_EINVAL and the context keeps its previous model. */
 int sfwi_set_model(sfwi_ctx *ctx, const float *vp, const float *vs, const float *rho);
+/* Host check of a model (no reference counterpart: the reference reads its
+ * model files unchecked, sfwi-stage1/main.cpp:1470-1472): every vp finite
+ * and > 0, every vs finite and >= 0, every rho finite and > 0 (not checked
+ * when rho_const > 0, which replaces it).  SFWI_EINVAL names the first bad
+ * cell (ix, iz) and value in sfwi_last_error(NULL).  Needs no GPU. */
+int sfwi_check_model(const sfwi_params *p, const float *vp, const float *vs, const float *rho);
 /* Model perturbation for Born modeling: FD2DGPU_ELASTIC::GPUbufferM
  * (sfwi-stage1/FD2DGPU.cu:1330-1334) and its window (buffer_image_extrapolation,
  * :1336-1346).  The two stages' Born sources act on different cells
@@ -207,7 +219,11 @@ int sfwi_shot_born_gradient(sfwi_ctx *ctx, const sfwi_shot *shot, double *misfit
 /* Many shots at once (the reference farms them over MPI ranks,
  * sfwi-stage1/main.cpp:417-634); misfits[i] receives shot i's misfit (+=).
  * The sfwi_batch_* calls solve the shots in batches of up to max_batch; an
- * empty list (nshots = 0, shots may be NULL) does nothing and returns 0. */
+ * empty list (nshots = 0, shots may be NULL) does nothing and returns 0.
+ * The gradient solves (per-shot and batched) return SFWI_ENUMERIC, naming the
+ * first shot, when a misfit is not finite (NaN or Inf in the observed traces):
+ * every shot is still solved and its misfit added, so the accumulated images
+ * then hold non-finite values until the next sfwi_flush. */
 int sfwi_batch_gradient(sfwi_ctx *ctx, int nshots, const sfwi_shot *shots, double *misfits);
 int sfwi_batch_born_gradient(sfwi_ctx *ctx, int nshots, const sfwi_shot *shots, double *misfits);
 
@@ -395,8 +411,19 @@ int sfwi_server_submit(sfwi_server *s, const sfwi_request *req, long long *ticke
  * own shots, so every result equals the request solved alone.  on = 0 runs
  * one request per batch. */
 int sfwi_server_batching(sfwi_server *s, int on);
-/* Blocks until the request is done; queue_ms = submit -> start, run_ms = start -> end. */
+/* Blocks until the request is done; queue_ms = submit -> start, run_ms = start -> end.
+ * Returns the request's status: 0, or its error (SFWI_ENUMERIC for a
+ * gradient request whose misfit is not finite: its outputs then hold the
+ * non-finite values, and the other requests of its batch are unaffected;
+ * SFWI_ECANCELED after sfwi_server_cancel), and forgets the ticket. */
 int sfwi_server_wait(sfwi_server *s, long long ticket, double *queue_ms, double *run_ms);
+/* sfwi_server_wait with a time limit: SFWI_ETIMEOUT if the request is not done
+ * within timeout_ms; the ticket then stays valid (wait again, or cancel). */
+int sfwi_server_wait_for(sfwi_server *s, long long ticket, double timeout_ms, double *queue_ms, double *run_ms);
+/* Cancels a request that is still queued: it never runs, and waiting for it
+ * returns SFWI_ECANCELED.  A request that has started (or ended) cannot be
+ * cancelled: SFWI_ESTATE. */
+int sfwi_server_cancel(sfwi_server *s, long long ticket);
 /* Call after every submitted request has been waited for. */
 void sfwi_server_destroy(sfwi_server *s);
 

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libsfwi_amd.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "sfwi_abi.h")
 
 SFWI_OK, SFWI_EINVAL, SFWI_EHIP, SFWI_ENOMEM, SFWI_ESTATE = 0, -1, -2, -3, -4
+SFWI_ENUMERIC, SFWI_ECANCELED, SFWI_ETIMEOUT = -5, -6, -7
 MODE_FAST, MODE_EXACT = 0, 1
 STORE_STRIPS_F32, STORE_STRIPS_F16, STORE_STRIPS_BF16 = 0, 1, 2
 MISFIT_L2, MISFIT_OT_TRACE, MISFIT_OT_POWER, MISFIT_OT_GATHER = 0, 1, 2, 3
@@ -153,6 +154,9 @@ def load():
     L.sfwi_server_create.argtypes = [ctypes.c_size_t, ctypes.c_int, P(vp)]
     L.sfwi_server_submit.argtypes = [vp, P(SfwiRequest), P(ctypes.c_longlong)]
     L.sfwi_server_wait.argtypes = [vp, ctypes.c_longlong, dp, dp]
+    L.sfwi_server_wait_for.argtypes = [vp, ctypes.c_longlong, ctypes.c_double, dp, dp]
+    L.sfwi_server_cancel.argtypes = [vp, ctypes.c_longlong]
+    L.sfwi_check_model.argtypes = [P(SfwiParams), vp, vp, vp]
     L.sfwi_server_batching.argtypes = [vp, ctypes.c_int]
     L.sfwi_server_destroy.argtypes = [vp]
     L.sfwi_server_destroy.restype = None

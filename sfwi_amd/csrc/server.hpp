@@ -19,9 +19,11 @@
 
 #include <chrono>
 #include <condition_variable>
+#include <algorithm>
 #include <deque>
 #include <list>
 #include <mutex>
+#include <string>
 #include <thread>
 
 namespace {
@@ -130,8 +132,9 @@ struct sfwi_server {
         }
     }
 
-    // Several requests of one shape and kind as one merged batch.
-    int run_group(const std::vector<Job *> &group, sfwi_ctx *c, bool fresh) {
+    // Several requests of one shape and kind as one merged batch; bad[j]:
+    // request j's first shot whose misfit is not finite, or -1.
+    int run_group(const std::vector<Job *> &group, sfwi_ctx *c, bool fresh, std::vector<int> &bad) {
         const sfwi_request &q0 = group[0]->req;
         int total = 0;
         for (const Job *J : group) total += J->req.nshots;
@@ -153,7 +156,7 @@ struct sfwi_server {
             p.gvp = q.grad_vp; p.gvs = q.grad_vs; p.gil = q.illum;
             parts.push_back(p);
         }
-        return solve_merged(c, kind_of(q0.kind), parts);
+        return solve_merged(c, kind_of(q0.kind), parts, &bad);
     }
 
     int run_job(Job &J, sfwi_ctx *c, bool fresh) {
@@ -177,9 +180,15 @@ struct sfwi_server {
             std::vector<double> mis(n, 0.0);
             r = q.kind == SFWI_REQ_GRADIENT ? sfwi_batch_gradient(c, n, J.shots.data(), mis.data())
                                             : sfwi_batch_born_gradient(c, n, J.shots.data(), mis.data());
-            if (!r) r = sfwi_flush(c, q.grad_vp, q.grad_vs, q.illum);
-            if (!r && q.misfits)
-                for (int k = 0; k < n; ++k) q.misfits[k] = mis[k];
+            // a non-finite misfit still flushes, so the cached context starts clean
+            if (!r || r == SFWI_ENUMERIC) {
+                const std::string keep = r ? std::string(sfwi_last_error(c)) : std::string();
+                const int f = sfwi_flush(c, q.grad_vp, q.grad_vs, q.illum);
+                if (q.misfits)
+                    for (int k = 0; k < n; ++k) q.misfits[k] = mis[k];
+                if (f) return f;
+                if (r) return fail(c, r, "%s", keep.c_str());
+            }
             return r;
         }
         }
@@ -233,8 +242,9 @@ struct sfwi_server {
                 r = sfwi_ctx_create(&J->req.params, &c);
                 if (r) msg = sfwi_last_error(nullptr);
             }
+            std::vector<int> bad(group.size(), -1);
             if (!r) {
-                r = group.size() == 1 ? run_job(*J, c, fresh) : run_group(group, c, fresh);
+                r = group.size() == 1 ? run_job(*J, c, fresh) : run_group(group, c, fresh, bad);
                 if (r) msg = sfwi_last_error(c);
             }
             lk.lock();
@@ -245,10 +255,14 @@ struct sfwi_server {
                 cache.erase(slot);
             }
             const auto t_end = std::chrono::steady_clock::now();
-            for (Job *G : group) {
+            for (size_t j = 0; j < group.size(); ++j) {
+                Job *G = group[j];
                 G->t_end = t_end;
-                G->status = r ? r : 0;
+                G->status = r ? r : bad[j] >= 0 ? SFWI_ENUMERIC : 0;
                 G->err = msg;
+                if (!r && bad[j] >= 0)
+                    G->err = "shot " + std::to_string(bad[j]) +
+                             ": misfit is not finite (NaN or Inf in its observed traces)";
             }
             cv_done.notify_all();
             cv_work.notify_all();
@@ -301,10 +315,20 @@ int sfwi_server_submit(sfwi_server *s, const sfwi_request *q, long long *ticket)
     if (born && (!q->dvp || !q->dvs)) return fail(nullptr, SFWI_EINVAL, "Born request without perturbation");
     // a cached context keeps the slots of up to max_batch shots: account for all of them
     const size_t bytes = solve_bytes(q->params, q->kind, q->params.max_batch);
+    {
+        std::lock_guard<std::mutex> lk(s->m);
+        if (bytes > s->budget)
+            return fail(nullptr, SFWI_ENOMEM, "request needs %zu MiB, more than the server budget (%zu MiB)",
+                        bytes >> 20, s->budget >> 20);
+    }
+    // a bad model, wavelet or perturbation is refused here, before it can join a batch
+    if (int r = sfwi_check_model(&q->params, q->vp, q->vs, q->rho)) return r;
+    if (q->params.nt < 1 || bad_value(q->wavelet, q->params.nt) >= 0)
+        return fail(nullptr, SFWI_EINVAL, "the wavelet holds a value that is not finite");
+    const long nm = (long)q->params.allnx * q->params.allnz;
+    if (born && (bad_value(q->dvp, nm) >= 0 || bad_value(q->dvs, nm) >= 0))
+        return fail(nullptr, SFWI_EINVAL, "the perturbation holds a value that is not finite");
     std::lock_guard<std::mutex> lk(s->m);
-    if (bytes > s->budget)
-        return fail(nullptr, SFWI_ENOMEM, "request needs %zu MiB, more than the server budget (%zu MiB)",
-                    bytes >> 20, s->budget >> 20);
     const long long t = s->next_ticket++;
     sfwi_server::Job &J = s->jobs[t];
     J.ticket = t;
@@ -318,12 +342,18 @@ int sfwi_server_submit(sfwi_server *s, const sfwi_request *q, long long *ticket)
     return 0;
 }
 
-int sfwi_server_wait(sfwi_server *s, long long ticket, double *queue_ms, double *run_ms) {
+// Waits (at most timeout_ms when >= 0) for the request; see sfwi_server_wait.
+static int server_wait(sfwi_server *s, long long ticket, double timeout_ms, double *queue_ms, double *run_ms) {
     if (!s) return fail(nullptr, SFWI_EINVAL, "NULL server");
     std::unique_lock<std::mutex> lk(s->m);
     auto it = s->jobs.find(ticket);
     if (it == s->jobs.end()) return fail(nullptr, SFWI_EINVAL, "unknown ticket %lld", ticket);
-    s->cv_done.wait(lk, [&] { return it->second.status <= 0; });
+    auto done = [&] { return it->second.status <= 0; };
+    if (timeout_ms < 0) {
+        s->cv_done.wait(lk, done);
+    } else if (!s->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), done)) {
+        return fail(nullptr, SFWI_ETIMEOUT, "request %lld not done within %g ms", ticket, timeout_ms);
+    }
     sfwi_server::Job &J = it->second;
     if (queue_ms) *queue_ms = ms_since(J.t_submit, J.t_start);
     if (run_ms) *run_ms = ms_since(J.t_start, J.t_end);
@@ -331,6 +361,30 @@ int sfwi_server_wait(sfwi_server *s, long long ticket, double *queue_ms, double 
     if (r) fail(nullptr, r, "request %lld: %s", ticket, J.err.c_str());   // this thread's error slot
     s->jobs.erase(it);
     return r;
+}
+
+int sfwi_server_wait(sfwi_server *s, long long ticket, double *queue_ms, double *run_ms) {
+    return server_wait(s, ticket, -1.0, queue_ms, run_ms);
+}
+
+int sfwi_server_wait_for(sfwi_server *s, long long ticket, double timeout_ms, double *queue_ms, double *run_ms) {
+    if (!(timeout_ms >= 0)) return fail(nullptr, SFWI_EINVAL, "timeout_ms must be >= 0");
+    return server_wait(s, ticket, timeout_ms, queue_ms, run_ms);
+}
+
+int sfwi_server_cancel(sfwi_server *s, long long ticket) {
+    if (!s) return fail(nullptr, SFWI_EINVAL, "NULL server");
+    std::lock_guard<std::mutex> lk(s->m);
+    auto it = s->jobs.find(ticket);
+    if (it == s->jobs.end()) return fail(nullptr, SFWI_EINVAL, "unknown ticket %lld", ticket);
+    sfwi_server::Job &J = it->second;
+    if (J.status != 1) return fail(nullptr, SFWI_ESTATE, "request %lld has already started or ended", ticket);
+    s->queue.erase(std::find(s->queue.begin(), s->queue.end(), ticket));
+    J.t_start = J.t_end = std::chrono::steady_clock::now();
+    J.status = SFWI_ECANCELED;
+    J.err = "cancelled before it started";
+    s->cv_done.notify_all();
+    return 0;
 }
 
 void sfwi_server_destroy(sfwi_server *s) {

@@ -161,6 +161,34 @@ int fail(sfwi_ctx *c, int code, const char *fmt, ...) {
     return code;
 }
 
+// Index of the first value that is not finite, or -1.
+long bad_value(const float *v, long n) {
+    for (long i = 0; i < n; ++i)
+        if (!std::isfinite(v[i])) return i;
+    return -1;
+}
+
+// A model the propagator cannot use: some vp not finite or <= 0, vs not
+// finite or < 0, or (when the file's density is used) rho not finite or <= 0.
+// Writes the first such cell to msg.
+bool bad_model(const sfwi_params &p, const float *vp, const float *vs, const float *rho, char *msg, size_t cap) {
+    const long n = (long)p.allnx * p.allnz;
+    const bool use_rho = rho && !(p.rho_const > 0);
+    for (long q = 0; q < n; ++q) {
+        const char *what = nullptr;
+        float v = 0.0f;
+        if (!(std::isfinite(vp[q]) && vp[q] > 0.0f)) { what = "vp"; v = vp[q]; }
+        else if (!(std::isfinite(vs[q]) && vs[q] >= 0.0f)) { what = "vs"; v = vs[q]; }
+        else if (use_rho && !(std::isfinite(rho[q]) && rho[q] > 0.0f)) { what = "rho"; v = rho[q]; }
+        if (what) {
+            snprintf(msg, cap, "%s = %g at model cell (ix %ld, iz %ld): vp must be finite and > 0, vs finite "
+                     "and >= 0, rho finite and > 0", what, v, q / p.allnz, q % p.allnz);
+            return true;
+        }
+    }
+    return false;
+}
+
 #define HIPCK(c, expr)                                                                      \
     do {                                                                                    \
         hipError_t e_ = (expr);                                                             \
@@ -1015,6 +1043,8 @@ int copy_records(sfwi_ctx *c, const float *src, float *dst, int ntr);
 int solve_misfits(sfwi_ctx *c, Kind kind, int nshots, const sfwi_shot *shots, double *misfits,
                   float *res_z = nullptr, float *res_x = nullptr) {
     if (nshots < 0 || (nshots > 0 && !shots)) return fail(c, SFWI_EINVAL, "bad shot list");
+    int bad = -1;   // first shot whose misfit is not finite
+    double bad_m = 0.0;
     for (int b0 = 0; b0 < nshots; b0 += c->p.max_batch) {
         const int n = std::min(c->p.max_batch, nshots - b0);
         int r = prepare(c, kind, n, shots + b0, nullptr);
@@ -1031,7 +1061,12 @@ int solve_misfits(sfwi_ctx *c, Kind kind, int nshots, const sfwi_shot *shots, do
         if ((r = finish(c))) return r;
         if (misfits)
             for (int k = 0; k < n; ++k) misfits[b0 + k] += m[k];
+        for (int k = 0; k < n && bad < 0 && kind != K_HESS; ++k)
+            if (!std::isfinite(m[k])) { bad = b0 + k; bad_m = m[k]; }
     }
+    if (bad >= 0)
+        return fail(c, SFWI_ENUMERIC, "shot %d: misfit %g is not finite (NaN or Inf in its observed traces); "
+                                      "the accumulated images hold it until the next flush", bad, bad_m);
     return 0;
 }
 
@@ -1058,7 +1093,9 @@ struct Part {
 // accumulators that start at zero, so a part's results equal the same
 // request solved alone bit for bit.  The context's accumulators are zero on
 // return (its model, wavelet and perturbation are not used or changed).
-int solve_merged(sfwi_ctx *c, Kind kind, const std::vector<Part> &parts) {
+// bad (may be NULL) receives, per part, its first shot whose misfit is not
+// finite, or -1.
+int solve_merged(sfwi_ctx *c, Kind kind, const std::vector<Part> &parts, std::vector<int> *bad) {
     int n = 0;
     for (const Part &q : parts) n += q.nshots;
     if (n < 1 || n > c->p.max_batch) return fail(c, SFWI_EINVAL, "merged batch of %d shots outside [1,%d]", n, c->p.max_batch);
@@ -1135,9 +1172,13 @@ int solve_merged(sfwi_ctx *c, Kind kind, const std::vector<Part> &parts) {
     if (grad) HIPCK(c, hipMemsetAsync(c->Gvp, 0, 4 * ib, c->stream));
     if ((r = finish(c))) return done(r);
     off = 0;
-    for (const Part &q : parts) {
+    if (bad) bad->assign(parts.size(), -1);
+    for (size_t p = 0; p < parts.size(); ++p) {
+        const Part &q = parts[p];
         if (grad && q.misfits)
             for (int k = 0; k < q.nshots; ++k) q.misfits[k] = mis[off + k];
+        for (int k = q.nshots - 1; k >= 0 && grad && bad && kind != K_HESS; --k)
+            if (!std::isfinite(mis[off + k])) (*bad)[p] = k;
         off += q.nshots;
     }
     return done(0);
@@ -1153,6 +1194,15 @@ const char *sfwi_version(void) { return "sfwi_amd 0.1 (gfx950)"; }
 
 const char *sfwi_last_error(const sfwi_ctx *ctx) {
     return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+int sfwi_check_model(const sfwi_params *p, const float *vp, const float *vs, const float *rho) {
+    if (!p || !vp || !vs) return fail(nullptr, SFWI_EINVAL, "NULL argument");
+    if (p->allnx < 1 || p->allnz < 1) return fail(nullptr, SFWI_EINVAL, "model of %d x %d cells", p->allnx, p->allnz);
+    if (!rho && !(p->rho_const > 0)) return fail(nullptr, SFWI_EINVAL, "rho is NULL and rho_const is not set");
+    char msg[160];
+    if (bad_model(*p, vp, vs, rho, msg, sizeof msg)) return fail(nullptr, SFWI_EINVAL, "%s", msg);
+    return 0;
 }
 
 int sfwi_ctx_create(const sfwi_params *p, sfwi_ctx **out) {
@@ -1298,6 +1348,8 @@ void sfwi_ctx_destroy(sfwi_ctx *c) {
 
 int sfwi_set_wavelet(sfwi_ctx *c, const float *w) {
     if (!c || !w) return fail(c, SFWI_EINVAL, "NULL argument");
+    const long bw = bad_value(w, c->p.nt);
+    if (bw >= 0) return fail(c, SFWI_EINVAL, "wavelet sample %ld is %g, not finite", bw, w[bw]);
     HIPCK(c, hipMemcpyAsync(c->wav, w, c->p.nt * sizeof(float), hipMemcpyHostToDevice, c->stream));
     HIPCK(c, hipStreamSynchronize(c->stream));
     c->have_wav = true;
@@ -1307,6 +1359,8 @@ int sfwi_set_wavelet(sfwi_ctx *c, const float *w) {
 int sfwi_set_model(sfwi_ctx *c, const float *vp, const float *vs, const float *rho) {
     if (!c || !vp || !vs) return fail(c, SFWI_EINVAL, "NULL model");
     if (!rho && !(c->p.rho_const > 0)) return fail(c, SFWI_EINVAL, "rho is NULL and rho_const is not set");
+    char msg[160];
+    if (bad_model(c->p, vp, vs, rho, msg, sizeof msg)) return fail(c, SFWI_EINVAL, "%s", msg);
     const size_t b = c->nmodel * sizeof(float);
     HIPCK(c, hipStreamSynchronize(c->stream));
     drop_windows(c);
@@ -1322,6 +1376,10 @@ int sfwi_set_model(sfwi_ctx *c, const float *vp, const float *vs, const float *r
 int sfwi_set_perturbation_region(sfwi_ctx *c, const float *dvp, const float *dvs, int region) {
     if (!c || !dvp || !dvs) return fail(c, SFWI_EINVAL, "NULL perturbation");
     if (region != SFWI_BORN_S1 && region != SFWI_BORN_S2) return fail(c, SFWI_EINVAL, "unknown Born region %d", region);
+    const long bp = bad_value(dvp, (long)c->nmodel), bs = bad_value(dvs, (long)c->nmodel);
+    if (bp >= 0 || bs >= 0)
+        return fail(c, SFWI_EINVAL, "perturbation %s is not finite at cell %ld", bp >= 0 ? "dvp" : "dvs",
+                    bp >= 0 ? bp : bs);
     const size_t b = c->nmodel * sizeof(float);
     HIPCK(c, hipStreamSynchronize(c->stream));
     drop_windows(c);

@@ -187,10 +187,17 @@ class Context:
         return m.value
 
     def batch_gradient(self, shots, born=False):
+        """Misfits of the shots; the images accumulate until flush.  A
+        non-finite misfit raises SfwiError SFWI_ENUMERIC with the misfits as
+        ``misfits``."""
         arr = (SfwiShot * len(shots))(*[s.c for s in shots])
         m = np.zeros(len(shots), np.float64)
         fn = self.L.sfwi_batch_born_gradient if born else self.L.sfwi_batch_gradient
-        self._ck(fn(self.h, len(shots), arr, m.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        rc = fn(self.h, len(shots), arr, m.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        if rc != 0:
+            err = SfwiError(rc, self.L.sfwi_last_error(self.h).decode())
+            err.misfits = m
+            raise err
         return m
 
     def linesearch_parabola(self, shot, dir_vp, dir_vs, step1, step2, m=None):
@@ -299,6 +306,23 @@ class Context:
         return m.value, rx, rz
 
 
+def check_model(vp, vs, rho=None, rho_const=0.0):
+    """sfwi_check_model on (allnx, allnz) arrays: raise SfwiError (SFWI_EINVAL)
+    naming the first cell of the model that the propagator cannot use.
+    Needs no GPU."""
+    L = _lib.load()
+    vp, vs = _f32(vp), _f32(vs)
+    if vs.shape != vp.shape or (rho is not None and np.shape(rho) != vp.shape):
+        raise ValueError("vp, vs and rho must have one shape")
+    p = SfwiParams()
+    p.allnx, p.allnz = vp.shape
+    p.rho_const = rho_const
+    r = None if rho is None else _f32(rho)
+    rc = L.sfwi_check_model(ctypes.byref(p), _lib.ptr(vp), _lib.ptr(vs), _lib.ptr(r))
+    if rc != 0:
+        raise SfwiError(rc, L.sfwi_last_error(None).decode())
+
+
 class Server:
     """Serving loop (``sfwi_server_*``): submit forward / Born / gradient
     requests of any shape; they run on a pool of workers with their own
@@ -360,15 +384,35 @@ class Server:
         self._live[t.value] = (q, keep, arr, list(shots), out)
         return t.value
 
-    def wait(self, ticket):
+    def wait(self, ticket, timeout=None):
+        """The request's outputs once it is done.  timeout (seconds): raise
+        SfwiError SFWI_ETIMEOUT if it is not done by then (the ticket stays
+        valid).  A failed request raises its error; the outputs of a gradient
+        request with a non-finite misfit (SFWI_ENUMERIC) are on the error as
+        ``outputs``."""
         qm, rm = ctypes.c_double(0.0), ctypes.c_double(0.0)
-        rc = self.L.sfwi_server_wait(self.h, ticket, ctypes.byref(qm), ctypes.byref(rm))
+        if timeout is None:
+            rc = self.L.sfwi_server_wait(self.h, ticket, ctypes.byref(qm), ctypes.byref(rm))
+        else:
+            rc = self.L.sfwi_server_wait_for(self.h, ticket, 1e3 * float(timeout), ctypes.byref(qm),
+                                             ctypes.byref(rm))
+            if rc == _lib.SFWI_ETIMEOUT:
+                raise SfwiError(rc, self.L.sfwi_last_error(None).decode())
         rec = self._live.pop(ticket)
-        if rc != 0:
-            raise SfwiError(rc, self.L.sfwi_last_error(None).decode())
         out = rec[4]
         out["queue_ms"], out["run_ms"] = qm.value, rm.value
+        if rc != 0:
+            err = SfwiError(rc, self.L.sfwi_last_error(None).decode())
+            err.outputs = out
+            raise err
         return out
+
+    def cancel(self, ticket):
+        """Cancel a request that has not started; waiting for it then raises
+        SFWI_ECANCELED."""
+        rc = self.L.sfwi_server_cancel(self.h, ticket)
+        if rc != 0:
+            raise SfwiError(rc, self.L.sfwi_last_error(None).decode())
 
     def close(self):
         if getattr(self, "h", None):

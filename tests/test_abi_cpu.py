@@ -49,3 +49,39 @@ def test_create_rejects_bad_parameters(bad):
     assert rc == _lib.SFWI_EINVAL
     assert not h.value
     assert L.sfwi_last_error(None)
+
+
+def _model(nx=30, nz=20):
+    vp = np.full((nx, nz), 2500.0, np.float32)
+    return vp, vp / 1.8, np.full_like(vp, 2000.0)
+
+
+@pytest.mark.parametrize("field,cell,value,rho_const,ok", [
+    (None, None, None, 0.0, True),
+    ("vp", (3, 7), np.nan, 0.0, False), ("vp", (0, 0), 0.0, 0.0, False), ("vp", (29, 19), -1.0, 0.0, False),
+    ("vp", (5, 5), np.inf, 0.0, False), ("vs", (12, 2), -np.inf, 0.0, False), ("vs", (1, 1), -0.5, 0.0, False),
+    ("vs", (2, 3), 0.0, 0.0, True),                       # a fluid cell (Vs = 0) is allowed
+    ("rho", (4, 4), np.nan, 0.0, False), ("rho", (4, 4), 0.0, 0.0, False),
+    ("rho", (4, 4), np.nan, 2000.0, True)])               # rho_const replaces the file's density
+def test_check_model(field, cell, value, rho_const, ok):
+    """sfwi_check_model refuses a model the propagator cannot use and names
+    its first bad cell; it needs no GPU."""
+    from sfwi_amd.elastic import check_model
+    vp, vs, rho = _model()
+    if field:
+        {"vp": vp, "vs": vs, "rho": rho}[field][cell] = value
+    if ok:
+        check_model(vp, vs, rho, rho_const=rho_const)
+        return
+    with pytest.raises(_lib.SfwiError) as e:
+        check_model(vp, vs, rho, rho_const=rho_const)
+    assert e.value.code == _lib.SFWI_EINVAL
+    assert "%s = " % field in str(e.value) and "(ix %d, iz %d)" % cell in str(e.value), str(e.value)
+
+
+def test_check_model_without_density():
+    from sfwi_amd.elastic import check_model
+    vp, vs, _ = _model()
+    check_model(vp, vs, None, rho_const=2000.0)
+    with pytest.raises(_lib.SfwiError, match="rho_const"):
+        check_model(vp, vs, None)

@@ -96,6 +96,7 @@ def test_budget_serialises_admission_and_refuses_oversized():
         sv = W.tiny(nx=200, nz=150, nt=400, nshots=2)
         big["params"] = dict(reqs[0]["params"], nx=sv.nx, nz=sv.allnz, allnx=sv.allnx, allnz=sv.allnz, nt=400,
                              ntr=sv.ntr)
+        big.update(vp=sv.vp, vs=sv.vs, rho=sv.rho, wavelet=W.ricker(400, sv.dt, sv.f0))
         with pytest.raises(SfwiError, match="budget"):
             _submit(srv, big)
 
