@@ -378,8 +378,10 @@ int sfwi_server_cancel(sfwi_server *s, long long ticket) {
     auto it = s->jobs.find(ticket);
     if (it == s->jobs.end()) return fail(nullptr, SFWI_EINVAL, "unknown ticket %lld", ticket);
     sfwi_server::Job &J = it->second;
-    if (J.status != 1) return fail(nullptr, SFWI_ESTATE, "request %lld has already started or ended", ticket);
-    s->queue.erase(std::find(s->queue.begin(), s->queue.end(), ticket));
+    const auto q = std::find(s->queue.begin(), s->queue.end(), ticket);
+    if (J.status != 1 || q == s->queue.end())
+        return fail(nullptr, SFWI_ESTATE, "request %lld has already started or ended", ticket);
+    s->queue.erase(q);
     J.t_start = J.t_end = std::chrono::steady_clock::now();
     J.status = SFWI_ECANCELED;
     J.err = "cancelled before it started";
