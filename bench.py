@@ -519,13 +519,16 @@ def born(args):
     geo = [W.shot_geometry(sv, s) for s in sv.shots]
     nsh = len(sv.shots)
     storage = args.storage or "f32"
-    print("bench: %s, %s, %d shots, grid %dx%d pml %d, nt %d" % (
-        "Hessian-vector product" if hess else "Born gradient", sv.name, nsh, sv.nx, sv.allnz, sv.pml, sv.nt),
-        file=sys.stderr, flush=True)
+    recon = args.recon or "strips"
+    snap_k = args.snap_every if recon == "snapshots" else 0
+    print("bench: %s, %s, %d shots, grid %dx%d pml %d, nt %d, %s %s" % (
+        "Hessian-vector product" if hess else "Born gradient", sv.name, nsh, sv.nx, sv.allnz, sv.pml, sv.nt,
+        recon, storage), file=sys.stderr, flush=True)
     ctx = Context(dx=sv.dx, dz=sv.dz, dt=sv.dt, nx=sv.nx, nz=sv.allnz, pml=sv.pml, nt=sv.nt, ntr=sv.ntr,
                   allnx=sv.allnx, allnz=sv.allnz, max_batch=nsh, mode=args.mode, storage=storage,
                   march_cols=args.march_cols, march_depth=args.march_depth, march_order=args.march_order,
-                  rho_const=0.0 if hess else 2000.0, adjoint_dir=1 if hess else -1)
+                  rho_const=0.0 if hess else 2000.0, adjoint_dir=1 if hess else -1, recon=recon,
+                  snap_every=snap_k)
     ctx.set_wavelet(w)
     rho_m = sv.rho0 if hess else None
     ctx.set_model(sv.vp, sv.vs, rho_m)
@@ -576,9 +579,13 @@ def born(args):
             c = O.config(allnx=sv.allnx, allnz=sv.allnz, nx=sv.nx, nz=sv.allnz, pml=sv.pml, nt=sv.nt,
                          ntr=sv.ntr, dx=sv.dx, dz=sv.dz, dt=sv.dt, left=s.left, isx=g[0], isz=g[1],
                          igz=g[2], ngx_left=s.ngx_left, nthreads=threads, rho_const=0.0 if hess else 2000.0,
-                         adjoint_dir=1 if hess else -1)
-            if hess:
+                         adjoint_dir=1 if hess else -1, snap_every=snap_k)
+            if hess and snap_k:
+                a = O.hessian_vector_snapshot(c, sv.vp0, sv.vs0, sv.rho0, dvp, dvs, w, s.gc)
+            elif hess:
                 a = O.hessian_vector(c, sv.vp0, sv.vs0, sv.rho0, dvp, dvs, w, s.gc, "f64")
+            elif snap_k:
+                a = O.gradient_snapshot(c, sv.vp0, sv.vs0, sv.rho0, w, s.gc, sh.obs_z, sh.obs_x, dvp=dvp, dvs=dvs)
             else:
                 a = O.born_gradient(c, sv.vp0, sv.vs0, sv.rho0, dvp, dvs, w, s.gc, sh.obs_z, sh.obs_x, "f64")
             r_vp += a[0]
@@ -608,7 +615,9 @@ def born(args):
                                "%s" % (sv.name, sv.nx, sv.allnz, sv.dx, sv.pml, sv.dt * 1e3, sv.nt, nsh, sv.f0,
                                        sv.ntr, "batched Hessian-vector product p = true - initial model" if hess
                                        else "batched Born gradient, density 2000"),
-                   "mode": args.mode, "storage": "boundary strips %s" % storage.upper()},
+                   "mode": args.mode,
+                   "storage": ("velocity snapshots every %d steps, %s with a max-abs scale per snapshot"
+                               % (snap_k, storage.upper()) if snap_k else "boundary strips %s" % storage.upper())},
         "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 2),
                        "p99": round(1e3 * float(np.percentile(lat, 99)), 2), "unit": "per batch step"},
         "misfit": float(m.sum()), "grad_err_vs_fp64": acc, "cpu_baseline": cpu, "kernels": kernels}), flush=True)
