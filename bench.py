@@ -624,9 +624,11 @@ def born(args):
     ctx.close()
 
 
-def _request(kind, sv, max_batch=0):
+def _request(kind, sv, max_batch=0, recon=None, storage=None, snap_every=4):
     """Arguments of one serving request (zero observed traces for gradients);
-    max_batch: room for same-shape requests to join its batch."""
+    max_batch: room for same-shape requests to join its batch; recon /
+    storage: the gradient kinds' forward-field storage (default the
+    reference's FP32 strips)."""
     from sfwi_amd import workloads as W
     from sfwi_amd.elastic import Shot
     born = kind.startswith("born")
@@ -638,6 +640,9 @@ def _request(kind, sv, max_batch=0):
     params = dict(dx=sv.dx, dz=sv.dz, dt=sv.dt, nx=sv.nx, nz=sv.allnz, pml=sv.pml, nt=sv.nt, ntr=sv.ntr,
                   allnx=sv.allnx, allnz=sv.allnz, max_batch=max(len(shots), max_batch),
                   rho_const=2000.0 if born else 0.0, adjoint_dir=-1 if born else 1)
+    if "gradient" in kind and (recon or storage):
+        params.update(recon=recon or "strips", storage=storage or "f32",
+                      snap_every=snap_every if recon == "snapshots" else 0)
     return dict(kind=kind, shots=shots, wavelet=W.ricker(sv.nt, sv.dt, sv.f0), vp=sv.vp0, vs=sv.vs0,
                 rho=sv.rho0, dvp=(sv.vp - sv.vp0) if born else None, dvs=(sv.vs - sv.vs0) if born else None,
                 params=params)
@@ -649,9 +654,10 @@ def _submit(srv, r):
 
 
 def _run_alone(r):
-    """One request through the plain one-shot path: per-cell kernels, one shot per call."""
+    """One request through the plain one-shot path: per-cell kernels, one shot
+    per call, the reference's FP32 boundary strips."""
     from sfwi_amd.elastic import Context
-    params = dict(r["params"], max_batch=1, march_cols=-1)
+    params = dict(r["params"], max_batch=1, march_cols=-1, recon="strips", storage="f32", snap_every=0)
     with Context(**params) as ctx:
         ctx.set_wavelet(r["wavelet"])
         ctx.set_model(r["vp"], r["vs"], r["rho"])
@@ -859,7 +865,8 @@ def serving(args):
     from sfwi_amd import workloads as W
     from sfwi_amd.elastic import Server
     stream = W.serving_requests(args.requests, rate=args.rate, scale=args.scale, shapes=args.shapes)
-    reqs = [(t, _request(k, sv, args.merge_cap), sv) for t, k, sv in stream]
+    reqs = [(t, _request(k, sv, args.merge_cap, args.recon, args.storage, args.snap_every), sv)
+            for t, k, sv in stream]
     cells = sum(sv.cells() * sv.nt * len(sv.shots) * {"forward": 1, "born_forward": 2, "gradient": 3,
                                                         "born_gradient": 4}[r["kind"]] for _, r, sv in reqs)
     shots = sum(len(r["shots"]) for _, r, _ in reqs)
@@ -919,7 +926,11 @@ def serving(args):
         "data": "synthetic (seeded layered models, zero observed traces)",
         "config": {"workload": "serving: %d Poisson requests at %.2f/s, nx 200-1500, nz 150-800, nt 1500-6000 "
                                "(x%.2f), 1-8 shots, forward/Born/gradient mix" % (len(reqs), args.rate, args.scale),
-                   "workers": args.workers},
+                   "workers": args.workers,
+                   "gradient_storage": ("velocity snapshots every %d steps, %s" % (args.snap_every,
+                                                                                  (args.storage or "f32").upper())
+                                        if args.recon == "snapshots" else
+                                        "boundary strips %s" % (args.storage or "f32").upper())},
         "value_note": "saturated: all requests queued at once",
         "latency_ms": {"p50": round(float(np.percentile(lms, 50)), 1),
                        "p99": round(float(np.percentile(lms, 99)), 1),
