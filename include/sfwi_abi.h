@@ -136,9 +136,12 @@ typedef struct sfwi_params {
     int fuse_step;         /* FAST column-march forward solves of the background    */
                            /* field: > 0 runs each time step as one fused launch    */
                            /* (velocity and stress together, state double-buffered, */
-                           /* identical results); <= 0 as two half-step launches    */
-                           /* (0: the default, which sfwi_autotune may change)      */
-    int step_cols;         /* columns per wave segment of the fused step (0 = 16)   */
+                           /* identical results): 1 wave-level march (48 rows per  */
+                           /* wave), 2 workgroup march (240 rows per workgroup,    */
+                           /* z halo shared through LDS); <= 0 as two half-step    */
+                           /* launches (0: the default, which sfwi_autotune may    */
+                           /* change)                                               */
+    int step_cols;         /* columns per segment of the fused step (0 = 16)        */
 } sfwi_params;
 
 /* One shot: the fields the reference driver writes into FD2DGPU_ELASTIC
@@ -562,8 +565,9 @@ const char *sfwi_kernel_name(int kernel_id);
  * `steps` time steps for each candidate -- per-cell kernels; column march with
  * 8 or 16 columns per segment, prefetch depth 1, 2 or 4, shot-major,
  * shot-interleaved or whole-shots-per-XCD block order; the forward solve in
- * half-step launches or in fused whole-step launches (the time-blocking
- * depth, fuse_step) -- and keeps the fastest in the context (read it back
+ * half-step launches or in fused whole-step launches, wave-level or
+ * workgroup march (the time-blocking depth, fuse_step 1 or 2) -- and keeps
+ * the fastest in the context (read it back
  * with sfwi_get_params).
  * Needs the model and wavelet set.  Accumulated gradients are preserved; the
  * observed records resident in the batch slots are invalidated.  Outputs may

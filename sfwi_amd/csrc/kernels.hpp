@@ -125,6 +125,8 @@ struct Geo {
                                                   // per shot over the interior, workgroups per XCD (launch)
     int fs_xs, fs_nbands, fs_waves, fs_wg, fs_per_xcd;   // fused step march: segment width, 48-row bands,
                                                          // waves and workgroups per shot, per XCD (launch)
+    int ws_xs, ws_ntiles, ws_wg, ws_per_xcd;   // workgroup fused step: segment width, 240-row tiles,
+                                               // workgroups per shot, per XCD (launch)
     float inv_nzd;
     float dt, dx, dz, dtdx, dtdz, idx, idz;
 };

@@ -168,6 +168,197 @@ __global__ void __launch_bounds__(BLK) km_fwd_step(Geo g, const DevShot *__restr
     }
 }
 
+// z taps of a column held in LDS (col[t + 4] is lane t's row; 4 pad entries
+// each side): the sums of z_plus / z_minus in the same order, so a column
+// formed from LDS taps equals one formed from DPP taps bit for bit.
+__device__ inline float lz_plus(const float *col, int t, float f) {
+    const float p1 = col[t + 5], p2 = col[t + 6], p3 = col[t + 7], p4 = col[t + 8];
+    const float m1 = col[t + 3], m2 = col[t + 2], m3 = col[t + 1];
+    float r = kC[1] * (p1 - f);
+    r = __builtin_fmaf(kC[2], p2 - m1, r);
+    r = __builtin_fmaf(kC[3], p3 - m2, r);
+    r = __builtin_fmaf(kC[4], p4 - m3, r);
+    return r;
+}
+__device__ inline float lz_minus(const float *col, int t, float f) {
+    const float p1 = col[t + 5], p2 = col[t + 6], p3 = col[t + 7];
+    const float m1 = col[t + 3], m2 = col[t + 2], m3 = col[t + 1], m4 = col[t];
+    float r = kC[1] * (f - m1);
+    r = __builtin_fmaf(kC[2], p1 - m2, r);
+    r = __builtin_fmaf(kC[3], p2 - m3, r);
+    r = __builtin_fmaf(kC[4], p3 - m4, r);
+    return r;
+}
+
+constexpr int WS_ROWS = BLK - 16;   // output rows of the workgroup fused step (256 lanes minus 2 x 8 halo)
+constexpr int WS_PAD = BLK + 8;     // one LDS column: 4 pad entries each side
+
+// Fused forward time step with the z halo shared inside the workgroup.
+// The four waves of a workgroup stack along z: thread t owns row
+// Z0 - 8 + t of one column strip [x0, x1), so a workgroup covers 256
+// consecutive rows and writes the 240 rows Z0 .. Z0 + 239.  The z taps come
+// from LDS instead of DPP shifts: at column c every thread puts the old tzz
+// and txz of column c and the new vx, vz of column c - 1 into a
+// double-buffered LDS column, one barrier, then reads the eight neighbours of
+// its row, so the rows where two waves meet are formed once, and only the
+// workgroup's top and bottom 8 rows are halo (the wave-level fused march
+// spends 16 of every 64 lanes on it).  Velocity z derivatives are formed a
+// column after their velocity and wait three columns in registers for the
+// stress update of their column.  The x direction, the buffers and the
+// per-cell arithmetic are those of km_fwd_step, so the results equal the
+// half-step pair's bit for bit.
+template <bool SAVE, int PD>
+__global__ void __launch_bounds__(BLK) km_fwd_step_wg(Geo g, const DevShot *__restrict__ shots, int nshots,
+                                                      int it, int par) {
+    __shared__ float zt[2][4][WS_PAD];
+    const int b = blockIdx.x;
+    const int L = (b & 7) * g.ws_per_xcd + (b >> 3);
+    const int shot = L / g.ws_wg;
+    if (shot >= nshots) return;   // uniform over the workgroup: no barrier is left waiting
+    const int w = L - shot * g.ws_wg;
+    const int seg = w / g.ws_ntiles, tile = w - seg * g.ws_ntiles;
+    const int x0 = NOP + seg * g.ws_xs, x1 = min(x0 + g.ws_xs, g.nxp - NOP);
+    const int Z0 = NOP + tile * WS_ROWS;
+    const DevShot d = shots[shot];   // by value: the field pointers stay in registers
+    const Fields &R = par ? d.fw2 : d.fw;    // the state before the step
+    const Fields &Wt = par ? d.fw : d.fw2;   // the state after it
+    const int t = threadIdx.x;
+    const int z = Z0 - 8 + t;
+    const int zc = min(max(z, 0), g.nzp - 1);
+    const bool zin = z >= 0 && z < g.nzp;
+    const bool vrow = z >= NOP && z < g.nzp - NOP;            // rows the update domain holds
+    const bool vlane = t >= 4 && t < BLK - 4 && vrow;         // velocity formed here (halo rows included)
+    const bool own = t >= 8 && t < BLK - 8 && z < g.nzp - NOP;  // rows this workgroup writes
+    auto colc = [&](int c) { return min(max(c, 0), g.nxp - 1); };
+    auto colin = [&](int c) { return c >= 0 && c < g.nxp; };
+    auto in_dom = [&](int c) { return c >= NOP && c < g.nxp - NOP; };
+    auto ld = [&](cfptr f, int c) { return f[colc(c) * g.nzp + zc]; };
+    float qxx[PD], qxz[PD], qzz[PD];
+    VelCell qv[PD] = {};
+    StrCell qs[PD] = {};
+    auto fetch = [&](int j, int c) {
+        qxx[j] = ld(R.txx, c + 4);
+        qxz[j] = ld(R.txz, c + 3);
+        qzz[j] = ld(R.tzz, c);
+        const int i = colc(c) * g.nzp + zc;
+        vel_load_w(d.m, R, i, vlane && in_dom(c), in_band(g, c, z), qv[j]);
+        const int cp = c - 4;
+        const bool sown = own && cp >= x0 && cp < x1;
+        const bool sp = in_band(g, cp, z) || (cp == d.isx && z == d.isz);
+        const int ip = colc(cp) * g.nzp + zc;
+        if (sown) { qs[j].lam = d.m.lam[ip]; qs[j].mu = d.m.mu[ip]; }
+        if (sown && sp) {
+            qs[j].xx_x = R.txx_x[ip]; qs[j].xx_z = R.txx_z[ip]; qs[j].zz_x = R.tzz_x[ip];
+            qs[j].zz_z = R.tzz_z[ip]; qs[j].xz_x = R.txz_x[ip]; qs[j].xz_z = R.txz_z[ip];
+            qs[j].ax = d.m.dampx[ip]; qs[j].az = d.m.dampz[ip];
+        }
+    };
+    auto zs = [&](int c, float v) { return (zin && colin(c)) ? v : 0.0f; };
+    if (t < 8) {   // pad entries: read only by halo rows, whose results are dropped
+        const int e = t < 4 ? t : BLK + t;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) zt[k >> 2][k & 3][e] = 0.0f;
+    }
+    const int cs = x0 - 4, ce = x1 + 4;
+    float wxx[8], wxz[8], tzh[5] = {}, vxn[9] = {}, vzn[9] = {};
+    float hxz[4] = {}, hzz[4] = {};   // Dz+ vx, Dz- vz of columns c - 4 .. c - 1
+    float txx_m4 = 0.0f;   // txx of column c - 4 (left the txx window a column ago)
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        wxx[j] = zs(cs - 3 + j, ld(R.txx, cs - 3 + j));
+        wxz[j] = zs(cs - 4 + j, ld(R.txz, cs - 4 + j));
+    }
+#pragma unroll
+    for (int j = 0; j < PD; ++j)
+        if (cs + j < ce) fetch(j, cs + j);
+    const BornSrc none{};
+    for (int c0 = cs; c0 < ce; c0 += PD) {
+#pragma unroll
+        for (int j = 0; j < PD; ++j) {
+            const int c = c0 + j;
+            if (c >= ce) break;   // uniform over the workgroup
+            wxx[7] = zs(c + 4, qxx[j]);
+            wxz[7] = zs(c + 3, qxz[j]);
+            tzh[4] = zs(c, qzz[j]);
+            VelCell vs = qv[j];
+            StrCell ss = qs[j];
+            if (c + PD < ce) fetch(j, c + PD);
+            float(*col)[WS_PAD] = zt[c & 1];
+            col[0][t + 4] = tzh[4];   // old tzz, txz of column c
+            col[1][t + 4] = wxz[4];
+            col[2][t + 4] = vxn[8];   // new vx, vz of column c - 1
+            col[3][t + 4] = vzn[8];
+            __syncthreads();
+            // velocity of column c
+            const float a = x_win(wxx);                    // Dx+ txx
+            const float e = x_win(wxz);                    // Dx- txz
+            const float cz = lz_plus(col[0], t, tzh[4]);   // Dz+ tzz
+            const float bz = lz_minus(col[1], t, wxz[4]);  // Dz- txz (txz at column c)
+            hxz[3] = lz_plus(col[2], t, vxn[8]);           // Dz+ vx of column c - 1
+            hzz[3] = lz_minus(col[3], t, vzn[8]);          // Dz- vz of column c - 1
+            float vx_c = 0.0f, vz_c = 0.0f;
+            if (vlane && in_dom(c)) {
+                const bool sp = in_band(g, c, z);
+                vel_math(g, vs, sp, 1, a, bz, cz, e);
+                vx_c = vs.vx;
+                vz_c = vs.vz;
+                if (own && c >= x0 && c < x1) {
+                    const int i = c * g.nzp + z;
+                    vel_store(Wt, i, vs, sp);
+                    if (SAVE && sp) {
+                        int sl[4];
+                        strip_slots(g, c, z, sl);
+                        const fptr base = g.store ? d.stage : d.strip + (size_t)it * 2 * g.sf;
+#pragma unroll
+                        for (int k = 0; k < 4; ++k)
+                            if (sl[k] >= 0) {
+                                base[sl[k]] = vs.vx;
+                                base[g.sf + sl[k]] = vs.vz;
+                            }
+                    }
+                }
+            }
+            // old stresses of column c - 4, then the windows move on
+            const float sxx4 = txx_m4, sxz4 = wxz[0], szz4 = tzh[0];
+            const float dxz4 = hxz[0], dzz4 = hzz[0];
+            txx_m4 = wxx[0];
+            shift(wxx);
+            shift(wxz);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) tzh[k] = tzh[k + 1];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                hxz[k] = hxz[k + 1];
+                hzz[k] = hzz[k + 1];
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                vxn[k] = vxn[k + 1];
+                vzn[k] = vzn[k + 1];
+            }
+            vxn[8] = vx_c;
+            vzn[8] = vz_c;
+            // stresses of column c - 4 from the new velocities of columns c - 8 .. c
+            const int cp = c - 4;
+            if (cp >= x0 && own && cp < x1) {
+                VGrad v;
+                v.xx = x_win(&vxn[0]);   // Dx- vx
+                v.zx = x_win(&vzn[1]);   // Dx+ vz
+                v.zz = dzz4;             // Dz- vz (vz at column c - 4)
+                v.xz = dxz4;             // Dz+ vx (vx at column c - 4)
+                const bool sp = in_band(g, cp, z) || (cp == d.isx && z == d.isz);
+                if (!sp) {
+                    ss.txx = sxx4;
+                    ss.tzz = szz4;
+                    ss.txz = sxz4;
+                }
+                str_math<false>(g, ss, v, sp, 1, none);
+                str_store(Wt, cp * g.nzp + z, ss, sp);
+            }
+        }
+    }
+}
+
 // Copy a shot's forward state (velocity and stress totals, velocity split
 // parts) from the second buffer to the first, after an odd number of fused
 // steps, so the kernels that follow find it in DevShot::fw.

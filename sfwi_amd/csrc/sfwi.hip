@@ -737,17 +737,26 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
     // double-buffered) for FAST column-march forward solves of the background
     // field when fuse_step > 0 (sfwi_autotune sets it where it is faster).
     const bool fused = !percell && !born && c->p.fuse_step > 0;
+    // fuse_step 2: the workgroup form (z halo shared through LDS, 240-row tiles)
+    const bool fwg = fused && c->p.fuse_step == 2;
     Geo gs = g;
     gs.fs_per_xcd = (int)(((long)g.fs_wg * n + 7) / 8);
-    const dim3 steps_grid((unsigned)(8 * gs.fs_per_xcd));
+    gs.ws_per_xcd = (int)(((long)g.ws_wg * n + 7) / 8);
+    const dim3 steps_grid((unsigned)(8 * (fwg ? gs.ws_per_xcd : gs.fs_per_xcd)));
     for (int it = 0; it < nt && fused; ++it) {
         const int sp = hs ? it % c->ring : it;
         const int par = it & 1;   // buffer read by this step; the step writes the other
         if (hs && it % ch == 0 && it / ch >= 2) HIPCK(c, hipStreamWaitEvent(st, c->ev_copy[(it / ch) & 1], 0));
         hipLaunchKernelGGL(k_source, src1, dim3(64), 0, st, g, D, c->wav, it, n, par);
         int t = tbeg(c, SFWI_K_FWD_STEP, it);
-        if (save) hipLaunchKernelGGL((km_fwd_step<true, FPD>), steps_grid, blk, 0, st, gs, D, n, sp, par);
-        else hipLaunchKernelGGL((km_fwd_step<false, FPD>), steps_grid, blk, 0, st, gs, D, n, sp, par);
+        if (fwg) {
+            if (save) hipLaunchKernelGGL((km_fwd_step_wg<true, FPD>), steps_grid, blk, 0, st, gs, D, n, sp, par);
+            else hipLaunchKernelGGL((km_fwd_step_wg<false, FPD>), steps_grid, blk, 0, st, gs, D, n, sp, par);
+        } else if (save) {
+            hipLaunchKernelGGL((km_fwd_step<true, FPD>), steps_grid, blk, 0, st, gs, D, n, sp, par);
+        } else {
+            hipLaunchKernelGGL((km_fwd_step<false, FPD>), steps_grid, blk, 0, st, gs, D, n, sp, par);
+        }
         tend(c, t);
         if (save && g.store) hipLaunchKernelGGL(k_strip_pack, dim3(2, (unsigned)n), dim3(256), 0, st, g, D, sp);
         if (snap && it % g.snap_k == 0) {   // keep the new velocity (snapshot it / snap_k)
@@ -1269,6 +1278,9 @@ int sfwi_ctx_create(const sfwi_params *p, sfwi_ctx **out) {
     g.fs_nbands = (g.nzd + FS_ROWS - 1) / FS_ROWS;
     g.fs_waves = ((g.nxp - 2 * NOP + g.fs_xs - 1) / g.fs_xs) * g.fs_nbands;
     g.fs_wg = (g.fs_waves + BLK / 64 - 1) / (BLK / 64);
+    g.ws_xs = p->step_cols > 0 ? p->step_cols : 16;
+    g.ws_ntiles = (g.nzd + WS_ROWS - 1) / WS_ROWS;
+    g.ws_wg = ((g.nxp - 2 * NOP + g.ws_xs - 1) / g.ws_xs) * g.ws_ntiles;
     g.inv_nzd = 1.0f / (float)g.nzd;
     g.dt = p->dt; g.dx = p->dx; g.dz = p->dz;
     g.dtdx = p->dt / p->dx; g.dtdz = p->dt / p->dz;
@@ -1799,8 +1811,9 @@ int sfwi_set_timing(sfwi_ctx *c, int every) {
 // this context's grid, times a gradient solve of `steps` time steps (zero
 // observed traces, one warm-up run per candidate) for each candidate and keeps
 // the fastest in the context.  Candidates: per-cell kernels; column march with
-// 8 or 16 columns per segment, prefetch depth 1 or 2, shot-major or
-// shot-interleaved block order.  Accumulated gradients are preserved; the
+// 8 or 16 columns per segment, prefetch depth 1, 2 or 4, shot-major,
+// shot-interleaved or whole-shots-per-XCD block order; the forward solve in
+// half steps or in fused whole steps (wave-level or workgroup march).  Accumulated gradients are preserved; the
 // observed records resident in the batch slots are invalidated.
 int sfwi_autotune(sfwi_ctx *c, int nshots, int steps, int *march_cols, int *march_depth, int *march_order,
                   double *best_ms) {
@@ -1828,10 +1841,11 @@ int sfwi_autotune(sfwi_ctx *c, int nshots, int steps, int *march_cols, int *marc
     const size_t acc = 4 * c->nmodel * sizeof(float);
     HIPCK(c, hipMemcpyAsync(c->tmp_model, c->Gvp, acc, hipMemcpyDeviceToDevice, c->stream));
     // kernel family, segment, prefetch depth, block order, and the time-blocking
-    // depth of the forward solve: half steps (fuse 0) or whole fused steps (1)
+    // depth of the forward solve: half steps (fuse 0) or whole fused steps
+    // (1: wave-level march, 2: workgroup march with the z halo in LDS)
     struct Cand { int cols, depth, order, fuse; };
     const Cand cands[] = {{-1, 1, 1, 0}, {8, 4, 3, 0}, {8, 2, 3, 0}, {8, 1, 3, 0}, {16, 2, 3, 0}, {8, 4, 1, 0},
-                          {8, 2, 2, 0}, {8, 2, 3, 1}, {8, 1, 3, 1}};
+                          {8, 2, 2, 0}, {8, 2, 3, 1}, {8, 1, 3, 1}, {8, 2, 3, 2}, {8, 1, 3, 2}};
     const sfwi_params keep = c->p;
     const int nt_keep = c->g.nt;
     c->g.nt = ns;                              // a shorter record inside the same buffers

@@ -61,15 +61,28 @@ def test_fused_step_equals_half_steps(nz, nt, cols):
     _same(_run(sv, w, obs, march_cols=-1), ref)
 
 
+# Workgroup form: one 240-row tile (nz 40, 134), two (nz 300: 240 + 68 rows)
+# and three with a 16-row last tile (nz 488), segment widths that do and do
+# not divide the grid, and each prefetch depth.
+@pytest.mark.parametrize("nz,nt,cols,depth", [(40, 200, 32, 2), (134, 201, 37, 1), (300, 201, 32, 2),
+                                              (300, 200, 16, 4), (488, 200, 24, 2)])
+def test_workgroup_fused_step_equals_half_steps(nz, nt, cols, depth):
+    sv, w, obs = _case(nz, nt, seed=nz + nt)
+    ref = _run(sv, w, obs, march_cols=8, march_depth=depth, fuse_step=-1)
+    assert np.abs(ref[0][0]).max() > 0 and np.abs(ref[2][0]).max() > 0
+    _same(_run(sv, w, obs, march_cols=8, march_depth=depth, fuse_step=2, step_cols=cols), ref)
+
+
+@pytest.mark.parametrize("fuse", [1, 2])
 @pytest.mark.parametrize("extra", [dict(storage="bf16"), dict(cpu_mem=True, strip_chunk=30),
                                    dict(recon="snapshots", storage="f16", snap_every=3),
                                    dict(use_graph=True)])
-def test_fused_step_with_storage_and_graphs(extra):
-    sv, w, obs = _case(134, 201, seed=5)
+def test_fused_step_with_storage_and_graphs(extra, fuse):
+    sv, w, obs = _case(134 if fuse == 1 else 300, 201, seed=5)
     ref = _run(sv, w, obs, march_cols=8, fuse_step=-1, **extra)
-    _same(_run(sv, w, obs, march_cols=8, fuse_step=1, **extra), ref)
+    _same(_run(sv, w, obs, march_cols=8, fuse_step=fuse, **extra), ref)
     if extra.get("use_graph"):   # replays of the captured solves
-        with context(sv, "fast", max_batch=2, march_cols=8, fuse_step=1, use_graph=True) as ctx:
+        with context(sv, "fast", max_batch=2, march_cols=8, fuse_step=fuse, use_graph=True) as ctx:
             ctx.set_wavelet(w)
             ctx.set_model(sv.vp0, sv.vs0, sv.rho0)
             shots = [lib_shot(sv, sh, *obs[k]) for k, sh in enumerate(sv.shots)]
