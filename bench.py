@@ -51,7 +51,8 @@ HBM_PEAK_GBS = 8000.0
 _SNAP16 = {
     "k_vel(forward)": 35344157, "k_str(forward)": 46374932, "k_adj_str": 34583001,
     "k_adj_vel": 48443475, "k_snap_image": 30838385, "k_snap_store": 16727600,
-    "k_step(forward)": 78535057,   # the workgroup fused step, 16-column segments, prefetch depth 2
+    # the workgroup fused steps, 16-column segments, prefetch depth 1
+    "k_step(forward)": 79253210, "k_step(adjoint)": 93899550,
 }
 # measured with FP16 snapshots; BF16 snapshots move the same bytes with the same kernels (within 0.1 %)
 MEASURED_HBM_BYTES_PER_SHOT = {
@@ -117,7 +118,7 @@ def alg_bytes_8d(kernel, nx, nz, pml, store_bytes=2):
     """Scope-table bytes of one launch of `kernel` over one shot."""
     inter = nx * nz
     pmlc = (nx + 2 * pml) * (nz + 2 * pml) - inter
-    if kernel == "k_step(forward)":   # a whole cell-update
+    if kernel in ("k_step(forward)", "k_step(adjoint)"):   # a whole cell-update
         return 52 * inter + 100 * pmlc
     half = {"k_vel(forward)": "vel", "k_adj_vel": "vel", "k_vel(reconstruct+illum)": "vel",
             "k_vel(reconstruct)": "vel", "k_adj_vel_image": "vel",
@@ -187,8 +188,9 @@ def main():
     ap.add_argument("--march-depth", type=int, default=0, help="FAST march kernels: prefetch depth")
     ap.add_argument("--fuse-step", type=int, default=None,
                     help="FAST column-march forward: 2 one fused launch per time step as a workgroup march (the z "
-                         "halo shared through LDS; default for mixed_precision), 1 the same as a wave-level march, "
-                         "0 / -1 two half-step launches (default otherwise)")
+                         "halo shared through LDS), 3 the same and the snapshot gradient's adjoint steps fused too "
+                         "(default for mixed_precision), 1 a wave-level march, 0 / -1 two half-step launches "
+                         "(default otherwise)")
     ap.add_argument("--step-cols", type=int, default=0, help="fused step: columns per segment (0: 16)")
     ap.add_argument("--graph", action="store_true",
                     help="replay each solve from a captured hipGraph (the timed steps carry no kernel events; the "
@@ -249,7 +251,7 @@ def main():
 
     mp = args.workload == "mixed_precision"
     if args.fuse_step is None:
-        args.fuse_step = 2 if mp else 0
+        args.fuse_step = 3 if mp else 0
     recon = args.recon or ("snapshots" if mp else "strips")
     storage = args.storage or ("f16" if mp else "f32")
     snap = recon == "snapshots"
@@ -348,7 +350,7 @@ def main():
         measured = MEASURED_HBM_BYTES_PER_SHOT.get(key, {}) if (
             args.mode == "fast" and args.march_cols in (0, 8) and args.march_depth in (0, 4)
             and args.march_order in (0, 3) and not args.rbc and args.cpu_mem == 0 and wave == (24 if mp else nsh)
-            and args.fuse_step in (-1, 0, 2) and args.step_cols in (0, 16)) \
+            and args.fuse_step in (-1, 0, 2, 3) and args.step_cols in (0, 16)) \
             else {}
         traffic = round(measured[dom] * avg_wave) if dom in measured else None
         step_bytes = sum(v.get("alg_bytes", 0) * per_step[k] for k, v in kernels.items())
@@ -462,9 +464,10 @@ def main():
                            args.snap_every, storage.upper()) if snap else
                        "boundary strips %s%s" % (storage.upper(), ", pinned host memory in chunks of %d steps" %
                                                  args.cpu_mem if args.cpu_mem > 0 else "")),
-                   "forward": ("fused whole time steps, %s, %d-column segments" % (
-                       "workgroup march (z halo shared through LDS)" if args.fuse_step == 2 else "wave-level march",
-                       args.step_cols or 16) if args.fuse_step > 0 else "half-step launches (velocity, stress)"),
+                   "forward": ("fused whole time steps, %s, %d-column segments%s" % (
+                       "workgroup march (z halo shared through LDS)" if args.fuse_step >= 2 else "wave-level march",
+                       args.step_cols or 16, "; adjoint steps fused the same way" if args.fuse_step == 3 and snap
+                       else "") if args.fuse_step > 0 else "half-step launches (velocity, stress)"),
                    "launch": ("each solve replayed from a captured hipGraph; kernel times from one extra direct step"
                               if args.graph else "direct launches, kernel events every %d steps" % args.timing_every)},
         "gcell_updates_per_s": round(gcells, 2),

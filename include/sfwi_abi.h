@@ -138,9 +138,10 @@ typedef struct sfwi_params {
                            /* (velocity and stress together, state double-buffered, */
                            /* identical results): 1 wave-level march (48 rows per  */
                            /* wave), 2 workgroup march (240 rows per workgroup,    */
-                           /* z halo shared through LDS); <= 0 as two half-step    */
-                           /* launches (0: the default, which sfwi_autotune may    */
-                           /* change)                                               */
+                           /* z halo shared through LDS), 3 the same, and a        */
+                           /* snapshot gradient's adjoint steps fused the same     */
+                           /* way; <= 0 as two half-step launches (0: the default, */
+                           /* which sfwi_autotune may change)                      */
     int step_cols;         /* columns per segment of the fused step (0 = 16)        */
 } sfwi_params;
 
@@ -555,7 +556,8 @@ int sfwi_s2_update(int allnx, int allnz, double fenzi, double fenmu, const float
 #define SFWI_K_SNAP_IMAGE 9   /* snapshot gradient: imaging of one snapshot    */
 #define SFWI_K_SNAP_STORE 10  /* snapshot gradient: max-abs scale + packing    */
 #define SFWI_K_FWD_STEP 11    /* fused forward time step (velocity + stress)   */
-#define SFWI_K_COUNT 12
+#define SFWI_K_ADJ_STEP 12    /* fused adjoint time step (snapshot gradient)   */
+#define SFWI_K_COUNT 13
 int sfwi_set_timing(sfwi_ctx *ctx, int sample_every);
 int sfwi_kernel_time(sfwi_ctx *ctx, int kernel_id, double *avg_ms, int *launches);
 const char *sfwi_kernel_name(int kernel_id);
@@ -566,7 +568,7 @@ const char *sfwi_kernel_name(int kernel_id);
  * 8 or 16 columns per segment, prefetch depth 1, 2 or 4, shot-major,
  * shot-interleaved or whole-shots-per-XCD block order; the forward solve in
  * half-step launches or in fused whole-step launches, wave-level or
- * workgroup march (the time-blocking depth, fuse_step 1 or 2) -- and keeps
+ * workgroup march (the time-blocking depth, fuse_step 1, 2 or 3) -- and keeps
  * the fastest in the context (read it back
  * with sfwi_get_params).
  * Needs the model and wavelet set.  Accumulated gradients are preserved; the

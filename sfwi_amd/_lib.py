@@ -22,7 +22,7 @@ SFWI_BORN_S2, SFWI_BORN_S1 = 0, 1
 REQ_FORWARD, REQ_BORN_FORWARD, REQ_GRADIENT, REQ_BORN_GRADIENT = 0, 1, 2, 3
 RECON_STRIPS, RECON_SNAPSHOTS = 0, 1
 (K_FWD_VEL, K_FWD_STR, K_REC_STR, K_REC_VEL, K_ADJ_STR, K_ADJ_VEL, K_REC_STR_IMG, K_REC_VEL_ILL, K_ADJ_VEL_ONLY,
- K_SNAP_IMAGE, K_SNAP_STORE, K_FWD_STEP, K_COUNT) = range(13)
+ K_SNAP_IMAGE, K_SNAP_STORE, K_FWD_STEP, K_ADJ_STEP, K_COUNT) = range(14)
 
 
 class SfwiParams(ctypes.Structure):

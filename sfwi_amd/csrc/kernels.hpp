@@ -82,6 +82,9 @@ struct DevShot {
     Fields fw2;                            // second buffer of the forward state for the fused step march:
                                            // its own totals and velocity split parts, fw's stress split parts
     Fields bn2;                            // second-order scattered field (double scattering)
+    Fields bk2;                            // second buffer of the adjoint state for the fused adjoint step of
+                                           // the snapshot gradient (totals and velocity split parts; bk's
+                                           // stress split parts)
     cfptr wav;                             // the shot's own wavelet [nt] (NULL: the context's)
     fptr dgx, dgz, dgxz, dgzx;             // velocity derivatives for imaging
     fptr rz, rx;                           // synthetic (then residual) traces [ntr][nt]
@@ -904,19 +907,20 @@ __global__ void k_inject(Geo g, const DevShot *__restrict__ shots, int it) {
 // parts keep summing to the totals.  fac = 2 for the L2 misfit sum r^2 (r the
 // residual), 1 when r is already the derivative of the misfit (OT).  The
 // traces of one receiver cell are added in trace order by one thread.
-__global__ void k_inject_add(Geo g, const DevShot *__restrict__ shots, int it, float fac) {
+__global__ void k_inject_add(Geo g, const DevShot *__restrict__ shots, int it, float fac, int par) {
     const DevShot &d = shots[blockIdx.y];
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= d.ninj) return;
+    const Fields &B = par ? d.bk2 : d.bk;   // the buffer holding the current adjoint state
     const int i = d.igx[d.inj[d.inj_off[k]]] * g.nzp + d.igz;
     const float rho = d.m.rho[i];
-    float vz = d.bk.vz[i], vzx = d.bk.vz_x[i], vx = d.bk.vx[i], vxx = d.bk.vx_x[i];
+    float vz = B.vz[i], vzx = B.vz_x[i], vx = B.vx[i], vxx = B.vx_x[i];
     for (int j = d.inj_off[k]; j < d.inj_off[k + 1]; ++j) {
         const size_t o = (size_t)d.inj[j] * g.nt + it;
         const float az = fac * d.rz[o] / rho, ax = fac * d.rx[o] / rho;
         vz += az; vzx += az; vx += ax; vxx += ax;
     }
-    d.bk.vz[i] = vz; d.bk.vz_x[i] = vzx; d.bk.vx[i] = vx; d.bk.vx_x[i] = vxx;
+    B.vz[i] = vz; B.vz_x[i] = vzx; B.vx[i] = vx; B.vx_x[i] = vxx;
 }
 
 template <bool EXACT>
@@ -1119,8 +1123,9 @@ __device__ inline float snap_val(const Geo &g, const DevShot &d, int k, int f, i
 // current reverse step (between its adjoint velocity and stress half steps)
 // and the Vp / Vs cofactor sums (k_merge applies 2 vp rho, 2 vs rho and the
 // weight); illumination vx^2 + vz^2 of the kept velocity.
-__global__ void __launch_bounds__(BLK) k_snap_image(Geo g, const DevShot *__restrict__ shots, int k) {
+__global__ void __launch_bounds__(BLK) k_snap_image(Geo g, const DevShot *__restrict__ shots, int k, int par) {
     const DevShot &d = shots[blockIdx.y];
+    const Fields &B = par ? d.bk2 : d.bk;   // the buffer holding the current adjoint state
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= g.nx * g.nz) return;
     const int jx = q / g.nz, jz = q - jx * g.nz;
@@ -1152,7 +1157,7 @@ __global__ void __launch_bounds__(BLK) k_snap_image(Geo g, const DevShot *__rest
     const float vx = snap_val(g, d, k, bx, t) * sx, vz = snap_val(g, d, k, bz, t) * sz;
     const int i = (jx + g.pml) * g.nzp + (jz + g.pml);
     float op, os;
-    image_vpvs(gx, gz, gxz, gzx, d.bk.txx[i], d.bk.tzz[i], d.bk.txz[i], d.gvp[q], d.gvs[q], op, os);
+    image_vpvs(gx, gz, gxz, gzx, B.txx[i], B.tzz[i], B.txz[i], d.gvp[q], d.gvs[q], op, os);
     d.gvp[q] = op;
     d.gvs[q] = os;
     d.gil[q] = __builtin_fmaf(vx, vx, __builtin_fmaf(vz, vz, d.gil[q]));

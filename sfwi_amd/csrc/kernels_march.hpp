@@ -597,7 +597,8 @@ __global__ void __launch_bounds__(BLK) km_adj_vel_image(Geo g, const DevShot *__
 // the lane's own column value.  ST: snapshot storage (0 FP32, 1 FP16, 2 BF16).
 // Block -> (shot, tile): each XCD owns a contiguous range of whole shots.
 template <int ST>
-__global__ void __launch_bounds__(BLK) km_snap_image(Geo g, const DevShot *__restrict__ shots, int nshots, int k) {
+__global__ void __launch_bounds__(BLK) km_snap_image(Geo g, const DevShot *__restrict__ shots, int nshots, int k,
+                                                     int par) {
     const int b = blockIdx.x;
     const int L = (b & 7) * g.im_per_xcd + (b >> 3);
     const int shot = L / g.im_wg;
@@ -607,6 +608,7 @@ __global__ void __launch_bounds__(BLK) km_snap_image(Geo g, const DevShot *__res
     const int seg = W / g.im_nbands, band = W - seg * g.im_nbands;
     const int x0 = seg * 8, x1 = min(x0 + 8, g.nx);   // interior columns
     const DevShot d = shots[shot];
+    const Fields &B = par ? d.bk2 : d.bk;   // the buffer holding the current adjoint state
     const int lane = threadIdx.x & 63;
     const int jz = band * CM_ROWS - 4 + lane;           // interior row of this lane (halo lanes outside)
     const int R = g.snap_rz;
@@ -643,7 +645,7 @@ __global__ void __launch_bounds__(BLK) km_snap_image(Geo g, const DevShot *__res
         const int q = c * g.nz + jz;
         float sxx = 0.0f, szz = 0.0f, sxz = 0.0f, ip = 0.0f, is = 0.0f, il = 0.0f;
         if (act) {
-            sxx = d.bk.txx[i]; szz = d.bk.tzz[i]; sxz = d.bk.txz[i];
+            sxx = B.txx[i]; szz = B.tzz[i]; sxz = B.txz[i];
             ip = d.gvp[q]; is = d.gvs[q]; il = d.gil[q];
         }
         const float vxc = wx[4], vzc = wz[3];             // kept velocity at column c

@@ -359,6 +359,170 @@ __global__ void __launch_bounds__(BLK) km_fwd_step_wg(Geo g, const DevShot *__re
     }
 }
 
+// Fused adjoint time step of the snapshot gradient, the workgroup march of
+// km_fwd_step_wg applied to the adjoint pair: the adjoint velocity of column
+// c (km_adj_vel_image's arithmetic: x taps of p1 = (lam + 2 mu) txx + lam tzz
+// and p2 = mu txz from 8-column windows, z taps of p3 = (lam + 2 mu) tzz +
+// lam txx and of p2 from LDS), then the adjoint stress-like update of column
+// c - 4 (km_adj_str's arithmetic) from the new adjoint velocities of columns
+// c - 8 .. c.  Each column's lam, mu, txx, tzz are loaded once, four columns
+// ahead, and kept for p2 (mu, one column later) and p3 (four columns later).
+// The adjoint totals and velocity split parts live in two buffers
+// (DevShot::bk and bk2, the step's parity picks the one read), the stress
+// split parts stay in place; results equal km_adj_vel_image + km_adj_str bit
+// for bit.  Used by the snapshot gradient, whose imaging reads the adjoint
+// stresses before the step (neither half step of the pair had changed them
+// at that point either).
+template <int PD>
+__global__ void __launch_bounds__(BLK) km_adj_step_wg(Geo g, const DevShot *__restrict__ shots, int nshots,
+                                                      int par, int dir) {
+    __shared__ float zt[2][4][WS_PAD];
+    const int b = blockIdx.x;
+    const int L = (b & 7) * g.ws_per_xcd + (b >> 3);
+    const int shot = L / g.ws_wg;
+    if (shot >= nshots) return;   // uniform over the workgroup: no barrier is left waiting
+    const int w = L - shot * g.ws_wg;
+    const int seg = w / g.ws_ntiles, tile = w - seg * g.ws_ntiles;
+    const int x0 = NOP + seg * g.ws_xs, x1 = min(x0 + g.ws_xs, g.nxp - NOP);
+    const int Z0 = NOP + tile * WS_ROWS;
+    const DevShot d = shots[shot];   // by value: the field pointers stay in registers
+    const Fields &R = par ? d.bk2 : d.bk;    // the adjoint state before the step
+    const Fields &Wt = par ? d.bk : d.bk2;   // the state after it
+    const Medium &m = d.m;
+    const int t = threadIdx.x;
+    const int z = Z0 - 8 + t;
+    const int zc = min(max(z, 0), g.nzp - 1);
+    const bool zin = z >= 0 && z < g.nzp;
+    const bool vrow = z >= NOP && z < g.nzp - NOP;            // rows the update domain holds
+    const bool vlane = t >= 4 && t < BLK - 4 && vrow;         // velocity formed here (halo rows included)
+    const bool own = t >= 8 && t < BLK - 8 && z < g.nzp - NOP;  // rows this workgroup writes
+    auto colc = [&](int c) { return min(max(c, 0), g.nxp - 1); };
+    auto colin = [&](int c) { return c >= 0 && c < g.nxp; };
+    auto in_dom = [&](int c) { return c >= NOP && c < g.nxp - NOP; };
+    auto ok = [&](int c) { return zin && colin(c); };
+    struct Raw {
+        float lam, mu, xx, zz;
+    };
+    // per-column operands, PD columns ahead: lam, mu, txx, tzz of column c + 4,
+    // txz of column c + 3, the adjoint velocity state of column c, the split
+    // state of the stress column c - 4
+    Raw qr[PD];
+    float qxz[PD];
+    VelCell qv[PD] = {};
+    AdjStrCell qs[PD] = {};
+    auto fetch = [&](int j, int c) {
+        const int i4 = colc(c + 4) * g.nzp + zc;
+        qr[j].lam = m.lam[i4];
+        qr[j].mu = m.mu[i4];
+        qr[j].xx = R.txx[i4];
+        qr[j].zz = R.tzz[i4];
+        qxz[j] = R.txz[colc(c + 3) * g.nzp + zc];
+        const int i = colc(c) * g.nzp + zc;
+        vel_load_w(m, R, i, vlane && in_dom(c), in_band(g, c, z) || z == d.igz, qv[j]);
+        const int cp = c - 4;
+        const bool sown = own && cp >= x0 && cp < x1;
+        const int ip = colc(cp) * g.nzp + zc;
+        adj_str_load_w(m, R, ip, sown, in_band(g, cp, z), qs[j]);
+    };
+    auto P1 = [&](const Raw &r, int c) { return ok(c) ? adj_p1(r.lam, r.mu, r.xx, r.zz) : 0.0f; };
+    auto P3 = [&](const Raw &r, int c) { return ok(c) ? adj_p1(r.lam, r.mu, r.zz, r.xx) : 0.0f; };
+    auto P2 = [&](float mu, float xz, int c) { return ok(c) ? mu * xz : 0.0f; };
+    if (t < 8) {   // pad entries: read only by halo rows, whose results are dropped
+        const int e = t < 4 ? t : BLK + t;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) zt[k >> 2][k & 3][e] = 0.0f;
+    }
+    const int cs = x0 - 4, ce = x1 + 4;
+    // w1: p1 of c-3 .. c+4; w2: p2 of c-4 .. c+3; hist[k]: lam, mu, txx, tzz of
+    // column c+k (k = 0..3); vxn / vzn: new adjoint velocity of c-8 .. c.  The
+    // stress update of column c - 4 fetches that column's old stresses with
+    // its other operands (a 12 B re-read per cell; keeping them eight columns
+    // in registers costs a wave per SIMD).
+    float w1[8], w2[8], vxn[9] = {}, vzn[9] = {};
+    float hxz[4] = {}, hzz[4] = {};   // Dz+ vx, Dz- vz of columns c - 4 .. c - 1
+    Raw hist[4];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {   // p1 of cs-3 .. cs+3, p2 of cs-4 .. cs+2
+        const int x1c = cs - 3 + k, x2c = cs - 4 + k;
+        const int i1 = colc(x1c) * g.nzp + zc, i2 = colc(x2c) * g.nzp + zc;
+        Raw r{m.lam[i1], m.mu[i1], R.txx[i1], R.tzz[i1]};
+        if (!ok(x1c)) r = Raw{0.0f, 0.0f, 0.0f, 0.0f};
+        w1[k] = P1(r, x1c);
+        if (k >= 3) hist[k - 3] = r;   // columns cs .. cs+3
+        w2[k] = P2(m.mu[i2], R.txz[i2], x2c);
+    }
+#pragma unroll
+    for (int j = 0; j < PD; ++j)
+        if (cs + j < ce) fetch(j, cs + j);
+    for (int c0 = cs; c0 < ce; c0 += PD) {
+#pragma unroll
+        for (int j = 0; j < PD; ++j) {
+            const int c = c0 + j;
+            if (c >= ce) break;   // uniform over the workgroup
+            Raw r4 = qr[j];                       // column c + 4
+            if (!ok(c + 4)) r4 = Raw{0.0f, 0.0f, 0.0f, 0.0f};
+            w1[7] = P1(r4, c + 4);
+            w2[7] = P2(hist[3].mu, qxz[j], c + 3);   // mu, txz of column c + 3
+            const float p3 = P3(hist[0], c);         // column c
+            VelCell vs = qv[j];
+            AdjStrCell ss = qs[j];
+            if (c + PD < ce) fetch(j, c + PD);
+            float(*col)[WS_PAD] = zt[c & 1];
+            col[0][t + 4] = p3;       // p3, p2 of column c
+            col[1][t + 4] = w2[4];
+            col[2][t + 4] = vxn[8];   // new vx, vz of column c - 1
+            col[3][t + 4] = vzn[8];
+            __syncthreads();
+            // adjoint velocity of column c
+            const float sx = x_win(w1);                       // Dx+ p1
+            const float sxz_x = x_win(w2);                    // Dx- p2
+            const float sz = lz_plus(col[0], t, p3);          // Dz+ p3
+            const float sxz_z = lz_minus(col[1], t, w2[4]);   // Dz- p2 (p2 at column c)
+            hxz[3] = lz_plus(col[2], t, vxn[8]);              // Dz+ vx of column c - 1
+            hzz[3] = lz_minus(col[3], t, vzn[8]);             // Dz- vz of column c - 1
+            float vx_c = 0.0f, vz_c = 0.0f;
+            if (vlane && in_dom(c)) {
+                const bool sp = in_band(g, c, z) || z == d.igz;
+                vel_math(g, vs, sp, dir, sx, sxz_z, sz, sxz_x);
+                vx_c = vs.vx;
+                vz_c = vs.vz;
+                if (own && c >= x0 && c < x1) vel_store(Wt, c * g.nzp + z, vs, sp);
+            }
+            // the windows move on
+            const float dxz4 = hxz[0], dzz4 = hzz[0];
+            shift(w1);
+            shift(w2);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) hist[k] = hist[k + 1];
+            hist[3] = r4;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                hxz[k] = hxz[k + 1];
+                hzz[k] = hzz[k + 1];
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                vxn[k] = vxn[k + 1];
+                vzn[k] = vzn[k + 1];
+            }
+            vxn[8] = vx_c;
+            vzn[8] = vz_c;
+            // adjoint stresses of column c - 4 from the new velocities of columns c - 8 .. c
+            const int cp = c - 4;
+            if (cp >= x0 && own && cp < x1) {
+                VGrad v;
+                v.xx = x_win(&vxn[0]);   // Dx- vx
+                v.zx = x_win(&vzn[1]);   // Dx+ vz
+                v.zz = dzz4;             // Dz- vz (vz at column c - 4)
+                v.xz = dxz4;             // Dz+ vx (vx at column c - 4)
+                const bool sp = in_band(g, cp, z);
+                adj_str_math(g, ss, v, sp, dir);
+                adj_str_store(Wt, cp * g.nzp + z, ss, sp);
+            }
+        }
+    }
+}
+
 // Copy a shot's forward state (velocity and stress totals, velocity split
 // parts) from the second buffer to the first, after an odd number of fused
 // steps, so the kernels that follow find it in DevShot::fw.

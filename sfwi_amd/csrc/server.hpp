@@ -38,8 +38,9 @@ size_t solve_bytes(const sfwi_params &p, int kind, int nshots) {
     const size_t sf = 16 * (size_t)(p.nx + p.nz);
     size_t fixed = p.nt * f + 12 * nmodel * f + 2 * nrec * f + p.max_batch * 1024 + 2 * p.ntr * 8 + 4096;
     // per shot: the slot, its window and, in a merged batch, its request's model
-    size_t slot = (43 + 2 * (p.grad_rho != 0)) * ncell * f + 6 * nrec * f + 4 * nint * f + 20 * (size_t)p.ntr + 68 +
-                  5 * nmodel * f + p.nt * f;
+    // (a snapshot context's slot also holds the fused adjoint step's second buffer, 9 fields)
+    size_t slot = (43 + 2 * (p.grad_rho != 0) + (p.recon == SFWI_RECON_SNAPSHOTS ? 9 : 0)) * ncell * f +
+                  6 * nrec * f + 4 * nint * f + 20 * (size_t)p.ntr + 68 + 5 * nmodel * f + p.nt * f;
     if (born) slot += 15 * ncell * f;
     const size_t chunk = p.strip_chunk > 0 ? (size_t)p.strip_chunk : 100;
     const size_t ring = p.cpu_mem && 2 * chunk < (size_t)p.nt ? 2 * chunk : (size_t)p.nt;   // strip steps in HBM

@@ -57,14 +57,14 @@ char *g_scratch[NA_MAX_DEVICES] = {};
 thread_local std::string g_create_error;
 
 struct Slot {
-    char *block = nullptr;      // fw(15) + bk(15) + derivatives(4) fields
+    char *block = nullptr;      // fw(15) + fw2(9) + bk(15) + derivatives(4) [+ v(it+1)(2)] [+ bk2(9)] fields
     char *born = nullptr;       // bn(15) fields
     char *born2 = nullptr;      // bn2(15) fields (double scattering)
     char *rec = nullptr;        // rz rx r2z r2x oz ox, ints, accumulators, trace sums
     float *strip = nullptr;     // FP32 [ring][2][sf], or 16-bit: scales [ring][2], stage [2][sf], [ring][2][sf]
     char *hstrip = nullptr;     // host strips (cpu_mem, pinned): FP32 [nt][2][sf], or 16-bit [nt][2][sf] + scales [nt][2]
     char *snap = nullptr;       // velocity snapshots: max bits [nsnap][2] (256-B aligned), then FP32 or 16-bit values
-    Fields fw{}, fw2{}, bk{}, bn{}, bn2{};
+    Fields fw{}, fw2{}, bk{}, bn{}, bn2{}, bk2{};
     float *dg[4]{};
     float *rz, *rx, *r2z, *r2x, *oz, *ox;
     int *igx, *inj, *inj_off;
@@ -322,13 +322,18 @@ size_t snap_words(const sfwi_ctx *c) {
 }
 size_t snap_head(const sfwi_ctx *c) { return snap_words(c) + 2 * SNAP_PART * sizeof(float); }
 
+// Model-size fields of one batch slot's block: fw (15), fw2 (9), bk (15), the
+// velocity derivatives (4), v(it+1) with the density image (2), bk2 (9) with
+// snapshot gradients.
+int slot_fields(const sfwi_ctx *c) { return 43 + 2 * c->g.grad_rho + (c->g.snap_k > 0 ? 9 : 0); }
+
 int ensure_slots(sfwi_ctx *c, int n, bool born, bool strips, bool snaps, bool born2 = false) {
     const size_t cell = c->ncell * sizeof(float);
     while ((int)c->slots.size() < n) c->slots.emplace_back();
     for (int s = 0; s < n; ++s) {
         Slot &S = c->slots[s];
         if (!S.block) {
-            int r = dalloc(c, (void **)&S.block, (43 + 2 * c->g.grad_rho) * cell);
+            int r = dalloc(c, (void **)&S.block, slot_fields(c) * cell);
             if (r) return r;
             float *f = (float *)S.block;
             fptr *fw[15] = {&S.fw.vx, &S.fw.vz, &S.fw.txx, &S.fw.tzz, &S.fw.txz, &S.fw.vx_x, &S.fw.vx_z,
@@ -347,6 +352,12 @@ int ensure_slots(sfwi_ctx *c, int n, bool born, bool strips, bool snaps, bool bo
             for (int k = 0; k < 4; ++k) S.dg[k] = f + (39 + k) * c->ncell;
             if (c->g.grad_rho)
                 for (int k = 0; k < 2; ++k) S.vold[k] = f + (43 + k) * c->ncell;
+            if (c->g.snap_k > 0) {   // the fused adjoint step's second buffer: totals and velocity split parts
+                S.bk2 = S.bk;
+                fptr *b2[9] = {&S.bk2.vx, &S.bk2.vz, &S.bk2.txx, &S.bk2.tzz, &S.bk2.txz, &S.bk2.vx_x, &S.bk2.vx_z,
+                               &S.bk2.vz_x, &S.bk2.vz_z};
+                for (int k = 0; k < 9; ++k) *b2[k] = f + (43 + 2 * c->g.grad_rho + k) * c->ncell;
+            }
             // records, receiver tables, accumulators, per-trace sums
             const size_t rbytes = 6 * c->nrec * sizeof(float) + (3 * (size_t)c->p.ntr + 1) * sizeof(int) +
                                   4 * c->nint * sizeof(float) + (size_t)c->p.ntr * sizeof(double) + 64;
@@ -489,7 +500,7 @@ int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium 
         if (!m && (r = model_window(c, s.left, born, &m))) return r;
         DevShot d{};
         d.m = *m;
-        d.fw = S.fw; d.fw2 = S.fw2; d.bk = S.bk; d.bn = S.bn; d.bn2 = S.bn2;
+        d.fw = S.fw; d.fw2 = S.fw2; d.bk = S.bk; d.bn = S.bn; d.bn2 = S.bn2; d.bk2 = S.bk2;
         d.dgx = S.dg[0]; d.dgz = S.dg[1]; d.dgxz = S.dg[2]; d.dgzx = S.dg[3];
         d.rz = S.rz; d.rx = S.rx; d.r2z = S.r2z; d.r2x = S.r2x; d.oz = S.oz; d.ox = S.ox;
         if (os) {   // read in place from the resident set
@@ -520,6 +531,7 @@ int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium 
         c->h_shots[k] = d;
         const size_t cell = c->ncell * sizeof(float);
         HIPCK(c, hipMemsetAsync(S.block, 0, (grad ? 43 : 24) * cell, c->stream));
+        if (grad && snaps) HIPCK(c, hipMemsetAsync((float *)S.bk2.vx, 0, 9 * cell, c->stream));
         if (born) HIPCK(c, hipMemsetAsync(S.born, 0, 15 * cell, c->stream));
         if (born2) HIPCK(c, hipMemsetAsync(S.born2, 0, 15 * cell, c->stream));
         HIPCK(c, hipMemcpyAsync(S.igx, S.h_igx.data(), s.ntr * sizeof(int), hipMemcpyHostToDevice, c->stream));
@@ -691,8 +703,11 @@ void merge_range(sfwi_ctx *c, Kind kind, int off, int n) {
 // the two Born forward kernels, which carry two fields and twice the registers
 // (at depth 4 they fit 2 waves per SIMD and run 6-11 % slower than at depth 2;
 // the imaging reconstruction kernels gain 1 % at depth 2, which the next
-// kernel loses again, and the others are 1-4 % faster at depth 4).
-template <bool E, int PD, int HPD, int FPD = 2>
+// kernel loses again, and the others are 1-4 % faster at depth 4).  FPD: the
+// wave-level fused step's; WPD: the workgroup fused steps' (depth 1 keeps them
+// at 4 waves per SIMD, 115-127 VGPRs; at depth 2 the adjoint one needs 165 and
+// runs 3 % slower, the forward one the same).
+template <bool E, int PD, int HPD, int FPD = 2, int WPD = FPD>
 int run_batch(sfwi_ctx *c, Kind kind, int n) {
     const Geo &g = c->g;
     const int nt = g.nt;
@@ -737,12 +752,13 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
     // double-buffered) for FAST column-march forward solves of the background
     // field when fuse_step > 0 (sfwi_autotune sets it where it is faster).
     const bool fused = !percell && !born && c->p.fuse_step > 0;
-    // fuse_step 2: the workgroup form (z halo shared through LDS, 240-row tiles)
-    const bool fwg = fused && c->p.fuse_step == 2;
+    // fuse_step 2, 3: the workgroup form (z halo shared through LDS, 240-row tiles)
+    const bool fwg = fused && c->p.fuse_step >= 2;
     Geo gs = g;
     gs.fs_per_xcd = (int)(((long)g.fs_wg * n + 7) / 8);
     gs.ws_per_xcd = (int)(((long)g.ws_wg * n + 7) / 8);
     const dim3 steps_grid((unsigned)(8 * (fwg ? gs.ws_per_xcd : gs.fs_per_xcd)));
+    const dim3 ws_grid((unsigned)(8 * gs.ws_per_xcd));   // workgroup marches (forward or adjoint)
     for (int it = 0; it < nt && fused; ++it) {
         const int sp = hs ? it % c->ring : it;
         const int par = it & 1;   // buffer read by this step; the step writes the other
@@ -750,8 +766,8 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
         hipLaunchKernelGGL(k_source, src1, dim3(64), 0, st, g, D, c->wav, it, n, par);
         int t = tbeg(c, SFWI_K_FWD_STEP, it);
         if (fwg) {
-            if (save) hipLaunchKernelGGL((km_fwd_step_wg<true, FPD>), steps_grid, blk, 0, st, gs, D, n, sp, par);
-            else hipLaunchKernelGGL((km_fwd_step_wg<false, FPD>), steps_grid, blk, 0, st, gs, D, n, sp, par);
+            if (save) hipLaunchKernelGGL((km_fwd_step_wg<true, WPD>), steps_grid, blk, 0, st, gs, D, n, sp, par);
+            else hipLaunchKernelGGL((km_fwd_step_wg<false, WPD>), steps_grid, blk, 0, st, gs, D, n, sp, par);
         } else if (save) {
             hipLaunchKernelGGL((km_fwd_step<true, FPD>), steps_grid, blk, 0, st, gs, D, n, sp, par);
         } else {
@@ -867,8 +883,28 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
         Geo gi = g;   // imaging march: whole shots per XCD
         gi.im_per_xcd = (int)(((long)g.im_wg * n + 7) / 8);
         const dim3 igm((unsigned)(8 * gi.im_per_xcd));
-        for (int it = nt - 1; it >= 0; --it) {
-            hipLaunchKernelGGL(k_inject_add, trg, dim3(64), 0, st, g, D, it, fac);
+        // fuse_step 3: each reverse step is also one workgroup march (the
+        // adjoint state double-buffered, bk / bk2 by the step's parity); the
+        // imaging reads the adjoint stresses before it, which neither half step
+        // of the pair changes before the imaging either
+        const bool afused = !percell && c->p.fuse_step == 3;
+        for (int it = nt - 1; it >= 0 && afused; --it) {
+            const int apar = (nt - 1 - it) & 1;   // buffer holding the adjoint state before this step
+            hipLaunchKernelGGL(k_inject_add, trg, dim3(64), 0, st, g, D, it, fac, apar);
+            if (it % g.snap_k == 0) {
+                int t = tbeg(c, SFWI_K_SNAP_IMAGE, it);
+                const int k = it / g.snap_k;
+                if (g.store == 0) hipLaunchKernelGGL(km_snap_image<0>, igm, blk, 0, st, gi, D, n, k, apar);
+                else if (g.store == 1) hipLaunchKernelGGL(km_snap_image<1>, igm, blk, 0, st, gi, D, n, k, apar);
+                else hipLaunchKernelGGL(km_snap_image<2>, igm, blk, 0, st, gi, D, n, k, apar);
+                tend(c, t);
+            }
+            int t = tbeg(c, SFWI_K_ADJ_STEP, it);
+            hipLaunchKernelGGL((km_adj_step_wg<WPD>), ws_grid, blk, 0, st, gs, D, n, apar, 1);
+            tend(c, t);
+        }
+        for (int it = nt - 1; it >= 0 && !afused; --it) {
+            hipLaunchKernelGGL(k_inject_add, trg, dim3(64), 0, st, g, D, it, fac, 0);
             int t = tbeg(c, SFWI_K_ADJ_VEL_ONLY, it);
             if (percell) hipLaunchKernelGGL((k_adj_vel_image<false, false>), cells, blk, 0, st, g, D, 1);
             else hipLaunchKernelGGL((km_adj_vel_image<PD, false, false, 1>), tiles, blk, 0, st, gm, D, 1);
@@ -876,10 +912,10 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
             if (it % g.snap_k == 0) {
                 t = tbeg(c, SFWI_K_SNAP_IMAGE, it);
                 const int k = it / g.snap_k;
-                if (percell) hipLaunchKernelGGL(k_snap_image, ig, blk, 0, st, g, D, k);
-                else if (g.store == 0) hipLaunchKernelGGL(km_snap_image<0>, igm, blk, 0, st, gi, D, n, k);
-                else if (g.store == 1) hipLaunchKernelGGL(km_snap_image<1>, igm, blk, 0, st, gi, D, n, k);
-                else hipLaunchKernelGGL(km_snap_image<2>, igm, blk, 0, st, gi, D, n, k);
+                if (percell) hipLaunchKernelGGL(k_snap_image, ig, blk, 0, st, g, D, k, 0);
+                else if (g.store == 0) hipLaunchKernelGGL(km_snap_image<0>, igm, blk, 0, st, gi, D, n, k, 0);
+                else if (g.store == 1) hipLaunchKernelGGL(km_snap_image<1>, igm, blk, 0, st, gi, D, n, k, 0);
+                else hipLaunchKernelGGL(km_snap_image<2>, igm, blk, 0, st, gi, D, n, k, 0);
                 tend(c, t);
             }
             t = tbeg(c, SFWI_K_ADJ_STR, it);
@@ -968,7 +1004,7 @@ int run_direct(sfwi_ctx *c, Kind kind, int n) {
     case 1: return run_batch<false, 1, 1, 1>(c, kind, n);
     case 2: return run_batch<false, 2, 2, 2>(c, kind, n);
     case 4: return run_batch<false, 4, 4, 4>(c, kind, n);
-    default: return run_batch<false, 4, 2, 2>(c, kind, n);   // automatic
+    default: return run_batch<false, 4, 2, 2, 1>(c, kind, n);   // automatic
     }
 }
 
@@ -1842,10 +1878,11 @@ int sfwi_autotune(sfwi_ctx *c, int nshots, int steps, int *march_cols, int *marc
     HIPCK(c, hipMemcpyAsync(c->tmp_model, c->Gvp, acc, hipMemcpyDeviceToDevice, c->stream));
     // kernel family, segment, prefetch depth, block order, and the time-blocking
     // depth of the forward solve: half steps (fuse 0) or whole fused steps
-    // (1: wave-level march, 2: workgroup march with the z halo in LDS)
+    // (1: wave-level march, 2: workgroup march with the z halo in LDS, 3: the
+    // same for a snapshot gradient's adjoint steps too)
     struct Cand { int cols, depth, order, fuse; };
     const Cand cands[] = {{-1, 1, 1, 0}, {8, 4, 3, 0}, {8, 2, 3, 0}, {8, 1, 3, 0}, {16, 2, 3, 0}, {8, 4, 1, 0},
-                          {8, 2, 2, 0}, {8, 2, 3, 1}, {8, 1, 3, 1}, {8, 2, 3, 2}, {8, 1, 3, 2}};
+                          {8, 2, 2, 0}, {8, 2, 3, 1}, {8, 1, 3, 1}, {8, 2, 3, 2}, {8, 1, 3, 2}, {8, 1, 3, 3}};
     const sfwi_params keep = c->p;
     const int nt_keep = c->g.nt;
     c->g.nt = ns;                              // a shorter record inside the same buffers
@@ -1912,7 +1949,7 @@ const char *sfwi_kernel_name(int kid) {
     static const char *names[SFWI_K_COUNT] = {
         "k_vel(forward)", "k_str(forward)", "k_str(reconstruct)", "k_vel(reconstruct)", "k_adj_str",
         "k_adj_vel_image", "k_str(reconstruct+image)", "k_vel(reconstruct+illum)", "k_adj_vel",
-        "k_snap_image", "k_snap_store", "k_step(forward)"};
+        "k_snap_image", "k_snap_store", "k_step(forward)", "k_step(adjoint)"};
     return (kid >= 0 && kid < SFWI_K_COUNT) ? names[kid] : "?";
 }
 

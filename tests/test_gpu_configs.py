@@ -26,8 +26,9 @@
     differentiates the rounded velocity, about 20 cells per dominant
     wavelength, which amplifies BF16's 2^-8 rounding), so it is only held to
     5e-3; forward traces within 1e-5 and misfits within 1e-5 of FP64; the
-    FP16 gradient with the forward in fused whole steps (the workgroup march
-    that the benchmark runs) equal to the half-step pair's bit for bit.
+    FP16 gradient with the forward and adjoint in fused whole steps (the
+    workgroup marches that the benchmark runs) equal to the half-step pairs'
+    bit for bit.
   * The uneven batch's four grids (376 x 201 nt 2000, 401 x 301 nt 3000,
     801 x 401 nt 4000, 1200 x 600 nt 6000), 2 shots each: their gradient
     requests submitted together to the server (BF16 snapshots) give exactly
@@ -186,10 +187,11 @@ def test_mixed_precision_snapshot_gradient_within_fp64_bar():
         errs[storage] = [rel_l2(a, b) for a, b in zip(g, ref[:3])]
         mis[storage] = abs(m[0] - ref[3]) / abs(ref[3])
         out[storage] = (m, g)
-    # the benchmark's forward: whole time steps in the workgroup march (z halo
-    # in LDS), bit for bit the half-step pair at full size and nt
+    # the benchmark's configuration: forward and adjoint in whole time steps,
+    # workgroup marches (z halo in LDS), bit for bit the half-step pairs at
+    # full size and nt
     with Context(**_params(sv, 1, march_cols=8, recon="snapshots", snap_every=4, storage="f16",
-                           fuse_step=2)) as ctx:
+                           fuse_step=3)) as ctx:
         ctx.set_wavelet(w)
         ctx.set_model(sv.vp0, sv.vs0, sv.rho0)
         m = ctx.batch_gradient(_shots(sv, [(oz, ox)]))
