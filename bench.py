@@ -46,18 +46,23 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch (FETCH_SIZE x 2 + WRITE_SIZE x 1 from two rocprofv3
 # --pmc passes, the factors from a plain copy of known size) of the default
 # benchmark configuration's kernels, per shot of the launch; see
-# profiles/mixed_precision_snapshot_hbm_traffic.md.  Keyed by
-# (workload, recon, storage, snap_every).
+# profiles/mixed_precision_snapshot_*kernel_trace_and_hbm_traffic.md.  Keyed by
+# (workload, recon, storage, snap_every, time-blocking): half steps (0) or the
+# workgroup-fused forward and adjoint steps (3), whose forward step also forms
+# the snapshot maxima (the store is then scale + pack only).
 _SNAP16 = {
     "k_vel(forward)": 35344157, "k_str(forward)": 46374932, "k_adj_str": 34583001,
     "k_adj_vel": 48443475, "k_snap_image": 30838385, "k_snap_store": 16727600,
-    # the workgroup fused steps, 16-column segments, prefetch depth 1
-    "k_step(forward)": 79253210, "k_step(adjoint)": 93899550,
+}
+_SNAP16_FUSED = {   # 16-column segments, prefetch depth 1
+    "k_step(forward)": 79559206, "k_step(adjoint)": 93909871, "k_snap_image": 30838385, "k_snap_store": 9826381,
 }
 # measured with FP16 snapshots; BF16 snapshots move the same bytes with the same kernels (within 0.1 %)
 MEASURED_HBM_BYTES_PER_SHOT = {
-    ("mixed_precision", "snapshots", "bf16", 4): _SNAP16,
-    ("mixed_precision", "snapshots", "f16", 4): _SNAP16,
+    ("mixed_precision", "snapshots", "bf16", 4, 0): _SNAP16,
+    ("mixed_precision", "snapshots", "f16", 4, 0): _SNAP16,
+    ("mixed_precision", "snapshots", "bf16", 4, 3): _SNAP16_FUSED,
+    ("mixed_precision", "snapshots", "f16", 4, 3): _SNAP16_FUSED,
 }
 
 NOP, NSTRIP = 4, 4   # stencil half-width; strip half-width (split band = PML + NSTRIP)
@@ -181,7 +186,9 @@ def main():
     ap.add_argument("--rbc", action="store_true",
                     help="random boundary instead of the boundary strips (the reference's rbc)")
     ap.add_argument("--mode", default="fast", choices=["fast", "exact"])
-    ap.add_argument("--timing-every", type=int, default=20)
+    ap.add_argument("--timing-every", type=int, default=21,
+                    help="kernel events on every k-th time step (21: prime to the snapshot interval, so both kinds "
+                         "of steps are sampled)")
     ap.add_argument("--march-cols", type=int, default=0,
                     help="FAST kernels: columns per wave segment (0: 8, the column-march kernels, also for the "
                          "accuracy leg's single shot; <0: per-cell kernels)")
@@ -344,13 +351,13 @@ def main():
         v["share"] = round(v["avg_ms"] * per_step[k] / step_ms, 3) if step_ms else 0.0
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * per_step[k]) if kernels else None
     roofline = None
-    key = (args.workload, recon, storage, args.snap_every if snap else 0)
+    key = (args.workload, recon, storage, args.snap_every if snap else 0, max(args.fuse_step, 0))
     if dom and "GBps" in kernels[dom]:
         d = kernels[dom]
         measured = MEASURED_HBM_BYTES_PER_SHOT.get(key, {}) if (
-            args.mode == "fast" and args.march_cols in (0, 8) and args.march_depth in (0, 4)
+            args.mode == "fast" and args.march_cols in (0, 8)
             and args.march_order in (0, 3) and not args.rbc and args.cpu_mem == 0 and wave == (24 if mp else nsh)
-            and args.fuse_step in (-1, 0, 2, 3) and args.step_cols in (0, 16)) \
+            and args.step_cols in (0, 16) and args.march_depth in (0, 1 if args.fuse_step == 3 else 4)) \
             else {}
         traffic = round(measured[dom] * avg_wave) if dom in measured else None
         step_bytes = sum(v.get("alg_bytes", 0) * per_step[k] for k, v in kernels.items())

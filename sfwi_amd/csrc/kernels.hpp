@@ -93,6 +93,9 @@ struct DevShot {
     gptr<const int> igx;                   // receiver columns [ntr]
     gptr<const int> inj;                   // traces grouped by receiver cell, trace order within a cell [ntr]
     gptr<const int> inj_off;               // group k = inj[inj_off[k] .. inj_off[k+1]) [ninj + 1]
+    gptr<const int> rtr;                   // trace of each column of the receiver row, -1 none, -2 several [nxp]
+    fptr ainj;                             // adjoint source by step and column, [nt][2 (z, x)][nxp] (snapshot
+                                           // gradients whose receivers have a cell each)
     fptr strip;                            // boundary strips, [nt][2 fields][SF] (FP32 storage)
     fptr stage;                            // one step's strips [2][SF] before packing (16-bit storage)
     gptr<unsigned short> strip_h;          // packed strips [nt][2][SF] (FP16 / BF16 storage)
@@ -130,6 +133,7 @@ struct Geo {
                                                          // waves and workgroups per shot, per XCD (launch)
     int ws_xs, ws_ntiles, ws_wg, ws_per_xcd;   // workgroup fused step: segment width, 240-row tiles,
                                                // workgroups per shot, per XCD (launch)
+    int snap_np;            // partial maxima per field and shot of a snapshot being stored (stride)
     float inv_nzd;
     float dt, dx, dz, dtdx, dtdz, idx, idz;
 };
@@ -923,6 +927,28 @@ __global__ void k_inject_add(Geo g, const DevShot *__restrict__ shots, int it, f
     B.vz[i] = vz; B.vz_x[i] = vzx; B.vx[i] = vx; B.vx_x[i] = vxx;
 }
 
+// The adjoint source of every reverse step as a column table, fac r / rho at
+// the receiver row (the value k_inject_add adds), -0 elsewhere (adding it
+// changes nothing, not even a zero's sign): the fused adjoint step reads one
+// value per column and component with no index load and adds it on the whole
+// receiver row.  Needs every receiver in a cell of its own.
+__global__ void k_inject_table(Geo g, const DevShot *__restrict__ shots, float fac) {
+    const DevShot &d = shots[blockIdx.y];
+    const size_t n = (size_t)g.nt * 2 * g.nxp;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const int x = (int)(q % g.nxp), f = (int)((q / g.nxp) & 1);
+        const int it = (int)(q / (2 * (size_t)g.nxp));
+        const int tr = d.rtr[x];
+        float v = -0.0f;   // x + (-0) == x for every x, zeros of either sign included
+        if (tr >= 0) {
+            const float rho = d.m.rho[x * g.nzp + d.igz];
+            const size_t o = (size_t)tr * g.nt + it;
+            v = f ? fac * d.rx[o] / rho : fac * d.rz[o] / rho;
+        }
+        d.ainj[q] = v;
+    }
+}
+
 template <bool EXACT>
 __global__ void __launch_bounds__(BLK) k_adj_str(Geo g, const DevShot *__restrict__ shots, int dir) {
     int ix, iz;
@@ -1067,18 +1093,19 @@ __global__ void __launch_bounds__(BLK) k_snap_max(Geo g, const DevShot *__restri
         float m = red[threadIdx.x][0];
 #pragma unroll
         for (int k = 1; k < BLK / 64; ++k) m = fmaxf(m, red[threadIdx.x][k]);
-        d.snap_part[threadIdx.x * SNAP_PART + blockIdx.x] = m;
+        d.snap_part[threadIdx.x * g.snap_np + blockIdx.x] = m;
     }
 }
 
-// One wave per shot: the scales of snapshot k from the partial maxima, stored
-// as FP32 bits (non-negative floats order as integers).
-__global__ void __launch_bounds__(64) k_snap_scale(const DevShot *__restrict__ shots, int k) {
+// One wave per shot: the scales of snapshot k from the np partial maxima of
+// each field (written by k_snap_max or by the workgroup fused forward step),
+// stored as FP32 bits (non-negative floats order as integers).
+__global__ void __launch_bounds__(64) k_snap_scale(Geo g, const DevShot *__restrict__ shots, int k, int np) {
     const DevShot &d = shots[blockIdx.x];
     float mx = 0.0f, mz = 0.0f;
-    for (int j = threadIdx.x; j < SNAP_PART; j += 64) {
+    for (int j = threadIdx.x; j < np; j += 64) {
         mx = fmaxf(mx, d.snap_part[j]);
-        mz = fmaxf(mz, d.snap_part[SNAP_PART + j]);
+        mz = fmaxf(mz, d.snap_part[g.snap_np + j]);
     }
     mx = wave_max(mx);
     mz = wave_max(mz);

@@ -206,11 +206,15 @@ constexpr int WS_PAD = BLK + 8;     // one LDS column: 4 pad entries each side
 // column after their velocity and wait three columns in registers for the
 // stress update of their column.  The x direction, the buffers and the
 // per-cell arithmetic are those of km_fwd_step, so the results equal the
-// half-step pair's bit for bit.
+// half-step pair's bit for bit.  With wav set the step also adds the
+// explosive source (k_source) and records the receivers' new velocity
+// (k_record), so a time step of the forward solve is one launch.
 template <bool SAVE, int PD>
 __global__ void __launch_bounds__(BLK) km_fwd_step_wg(Geo g, const DevShot *__restrict__ shots, int nshots,
-                                                      int it, int par) {
+                                                      int it, int par, int smax, const float *__restrict__ wav,
+                                                      int ts) {
     __shared__ float zt[2][4][WS_PAD];
+    __shared__ float red[2][BLK / 64];
     const int b = blockIdx.x;
     const int L = (b & 7) * g.ws_per_xcd + (b >> 3);
     const int shot = L / g.ws_wg;
@@ -234,9 +238,16 @@ __global__ void __launch_bounds__(BLK) km_fwd_step_wg(Geo g, const DevShot *__re
     auto in_dom = [&](int c) { return c >= NOP && c < g.nxp - NOP; };
     auto ld = [&](cfptr f, int c) { return f[colc(c) * g.nzp + zc]; };
     float qxx[PD], qxz[PD], qzz[PD];
+    int qtr[PD];   // trace recorded at column c (-1: none), through the ring like every operand
     VelCell qv[PD] = {};
     StrCell qs[PD] = {};
     auto fetch = [&](int j, int c) {
+        // the column's trace index is the same on every lane; kept a per-lane
+        // value (the empty asm hides that), or the compiler moves it to a
+        // scalar register right at the load, which waits for the load
+        int cj = colc(c);
+        asm volatile("" : "+v"(cj));
+        qtr[j] = d.rtr[cj];
         qxx[j] = ld(R.txx, c + 4);
         qxz[j] = ld(R.txz, c + 3);
         qzz[j] = ld(R.tzz, c);
@@ -254,6 +265,12 @@ __global__ void __launch_bounds__(BLK) km_fwd_step_wg(Geo g, const DevShot *__re
         }
     };
     auto zs = [&](int c, float v) { return (zin && colin(c)) ? v : 0.0f; };
+    // the explosive source of step ts, added where the old txx, tzz of the
+    // source cell enter (k_source's addition, made by every reader instead of
+    // stored; a select, so no other value changes, not even a zero's sign)
+    const float wsrc = !wav ? 0.0f : d.wav ? d.wav[ts] : wav[ts];
+    const bool srow = wav && z == d.isz;   // wav NULL: k_source has added it
+    auto src = [&](int c, float v) { return (srow && c == d.isx) ? v + wsrc : v; };
     if (t < 8) {   // pad entries: read only by halo rows, whose results are dropped
         const int e = t < 4 ? t : BLK + t;
 #pragma unroll
@@ -265,23 +282,29 @@ __global__ void __launch_bounds__(BLK) km_fwd_step_wg(Geo g, const DevShot *__re
     float txx_m4 = 0.0f;   // txx of column c - 4 (left the txx window a column ago)
 #pragma unroll
     for (int j = 0; j < 7; ++j) {
-        wxx[j] = zs(cs - 3 + j, ld(R.txx, cs - 3 + j));
+        wxx[j] = src(cs - 3 + j, zs(cs - 3 + j, ld(R.txx, cs - 3 + j)));
         wxz[j] = zs(cs - 4 + j, ld(R.txz, cs - 4 + j));
     }
 #pragma unroll
     for (int j = 0; j < PD; ++j)
         if (cs + j < ce) fetch(j, cs + j);
     const BornSrc none{};
+    // smax: a snapshot step; keep the max |v| of the new velocity on the
+    // snapshot region (interior plus a 4-cell halo) over the cells this
+    // workgroup stores, for the snapshot's max-abs scale
+    const bool zreg = z >= g.pml - NOP && z < g.pml + g.nz + NOP;
+    float mvx = 0.0f, mvz = 0.0f;
     for (int c0 = cs; c0 < ce; c0 += PD) {
 #pragma unroll
         for (int j = 0; j < PD; ++j) {
             const int c = c0 + j;
             if (c >= ce) break;   // uniform over the workgroup
-            wxx[7] = zs(c + 4, qxx[j]);
+            wxx[7] = src(c + 4, zs(c + 4, qxx[j]));
             wxz[7] = zs(c + 3, qxz[j]);
-            tzh[4] = zs(c, qzz[j]);
+            tzh[4] = src(c, zs(c, qzz[j]));
             VelCell vs = qv[j];
             StrCell ss = qs[j];
+            const int trc = qtr[j];
             if (c + PD < ce) fetch(j, c + PD);
             float(*col)[WS_PAD] = zt[c & 1];
             col[0][t + 4] = tzh[4];   // old tzz, txz of column c
@@ -305,6 +328,15 @@ __global__ void __launch_bounds__(BLK) km_fwd_step_wg(Geo g, const DevShot *__re
                 if (own && c >= x0 && c < x1) {
                     const int i = c * g.nzp + z;
                     vel_store(Wt, i, vs, sp);
+                    if (wav && z == d.igz && trc >= 0) {   // the receiver here records (k_record)
+                        const size_t o = (size_t)trc * g.nt + ts;
+                        d.rz[o] = vs.vz;
+                        d.rx[o] = vs.vx;
+                    }
+                    if (smax && zreg && c >= g.pml - NOP && c < g.pml + g.nx + NOP) {
+                        mvx = fmaxf(mvx, fabsf(vs.vx));
+                        mvz = fmaxf(mvz, fabsf(vs.vz));
+                    }
                     if (SAVE && sp) {
                         int sl[4];
                         strip_slots(g, c, z, sl);
@@ -357,6 +389,21 @@ __global__ void __launch_bounds__(BLK) km_fwd_step_wg(Geo g, const DevShot *__re
             }
         }
     }
+    if (smax) {   // this workgroup's partial maxima (k_snap_scale reduces them)
+        mvx = wave_max(mvx);
+        mvz = wave_max(mvz);
+        if ((t & 63) == 0) {
+            red[0][t >> 6] = mvx;
+            red[1][t >> 6] = mvz;
+        }
+        __syncthreads();
+        if (t < 2) {
+            float mm = red[t][0];
+#pragma unroll
+            for (int k = 1; k < BLK / 64; ++k) mm = fmaxf(mm, red[t][k]);
+            d.snap_part[t * g.snap_np + w] = mm;
+        }
+    }
 }
 
 // Fused adjoint time step of the snapshot gradient, the workgroup march of
@@ -372,10 +419,12 @@ __global__ void __launch_bounds__(BLK) km_fwd_step_wg(Geo g, const DevShot *__re
 // split parts stay in place; results equal km_adj_vel_image + km_adj_str bit
 // for bit.  Used by the snapshot gradient, whose imaging reads the adjoint
 // stresses before the step (neither half step of the pair had changed them
-// at that point either).
+// at that point either).  With inj set it also adds the step's adjoint
+// source (k_inject_add) from the column table k_inject_table built, to the
+// split parts of the receiver row (its totals are recomputed from them).
 template <int PD>
 __global__ void __launch_bounds__(BLK) km_adj_step_wg(Geo g, const DevShot *__restrict__ shots, int nshots,
-                                                      int par, int dir) {
+                                                      int par, int dir, int it, int inj) {
     __shared__ float zt[2][4][WS_PAD];
     const int b = blockIdx.x;
     const int L = (b & 7) * g.ws_per_xcd + (b >> 3);
@@ -407,10 +456,20 @@ __global__ void __launch_bounds__(BLK) km_adj_step_wg(Geo g, const DevShot *__re
     // txz of column c + 3, the adjoint velocity state of column c, the split
     // state of the stress column c - 4
     Raw qr[PD];
-    float qxz[PD];
-    VelCell qv[PD] = {};
-    AdjStrCell qs[PD] = {};
+    float qxz[PD], qaz[PD], qax[PD];   // qaz, qax: the adjoint source at column c (inj)
+    // Own-cell state in shared slots: a lane uses either its totals or its
+    // split parts, never both, and a VGPR is per lane, so one set of registers
+    // holds whichever the lane needs (adjoint velocity: bu, then vx, vz or
+    // vx_x, vx_z, vz_x, vz_z, ax, az; stress column: txx, tzz, txz or xx_x,
+    // zz_z, xz_x, xz_z, ax, az).  Five registers fewer per ring slot.
+    float qv[PD][7] = {}, qs[PD][6] = {};
+    const bool rrow = inj && z == d.igz;   // the receiver row takes the adjoint source (k_inject_add)
+    const size_t ia = (size_t)it * 2 * g.nxp;
     auto fetch = [&](int j, int c) {
+        if (inj) {   // one address per wave and column
+            qaz[j] = d.ainj[ia + colc(c)];
+            qax[j] = d.ainj[ia + g.nxp + colc(c)];
+        }
         const int i4 = colc(c + 4) * g.nzp + zc;
         qr[j].lam = m.lam[i4];
         qr[j].mu = m.mu[i4];
@@ -418,11 +477,23 @@ __global__ void __launch_bounds__(BLK) km_adj_step_wg(Geo g, const DevShot *__re
         qr[j].zz = R.tzz[i4];
         qxz[j] = R.txz[colc(c + 3) * g.nzp + zc];
         const int i = colc(c) * g.nzp + zc;
-        vel_load_w(m, R, i, vlane && in_dom(c), in_band(g, c, z) || z == d.igz, qv[j]);
+        const bool va = vlane && in_dom(c), vsp = in_band(g, c, z) || z == d.igz;
+        float *u = qv[j];
+        if (va) u[6] = m.bu[i];
+        if (va && !vsp) { u[0] = R.vx[i]; u[1] = R.vz[i]; }
+        if (va && vsp) {
+            u[0] = R.vx_x[i]; u[1] = R.vx_z[i]; u[2] = R.vz_x[i]; u[3] = R.vz_z[i];
+            u[4] = m.dampx[i]; u[5] = m.dampz[i];
+        }
         const int cp = c - 4;
-        const bool sown = own && cp >= x0 && cp < x1;
+        const bool sown = own && cp >= x0 && cp < x1, ssp = in_band(g, cp, z);
         const int ip = colc(cp) * g.nzp + zc;
-        adj_str_load_w(m, R, ip, sown, in_band(g, cp, z), qs[j]);
+        float *q = qs[j];
+        if (sown && !ssp) { q[0] = R.txx[ip]; q[1] = R.tzz[ip]; q[2] = R.txz[ip]; }
+        if (sown && ssp) {
+            q[0] = R.txx_x[ip]; q[1] = R.tzz_z[ip]; q[2] = R.txz_x[ip]; q[3] = R.txz_z[ip];
+            q[4] = m.dampx[ip]; q[5] = m.dampz[ip];
+        }
     };
     auto P1 = [&](const Raw &r, int c) { return ok(c) ? adj_p1(r.lam, r.mu, r.xx, r.zz) : 0.0f; };
     auto P3 = [&](const Raw &r, int c) { return ok(c) ? adj_p1(r.lam, r.mu, r.zz, r.xx) : 0.0f; };
@@ -464,8 +535,20 @@ __global__ void __launch_bounds__(BLK) km_adj_step_wg(Geo g, const DevShot *__re
             w1[7] = P1(r4, c + 4);
             w2[7] = P2(hist[3].mu, qxz[j], c + 3);   // mu, txz of column c + 3
             const float p3 = P3(hist[0], c);         // column c
-            VelCell vs = qv[j];
-            AdjStrCell ss = qs[j];
+            // each lane's update reads only the variant its cell uses
+            VelCell vs;
+            vs.vx = vs.vx_x = qv[j][0];
+            vs.vz = vs.vx_z = qv[j][1];
+            vs.vz_x = qv[j][2]; vs.vz_z = qv[j][3]; vs.ax = qv[j][4]; vs.az = qv[j][5]; vs.bu = qv[j][6];
+            AdjStrCell ss;
+            ss.txx = ss.xx_x = qs[j][0];
+            ss.tzz = ss.zz_z = qs[j][1];
+            ss.txz = ss.xz_x = qs[j][2];
+            ss.xz_z = qs[j][3]; ss.ax = qs[j][4]; ss.az = qs[j][5];
+            if (rrow) {   // receiver row cells are split: their parts take the source (-0 where none)
+                vs.vz_x += qaz[j];
+                vs.vx_x += qax[j];
+            }
             if (c + PD < ce) fetch(j, c + PD);
             float(*col)[WS_PAD] = zt[c & 1];
             col[0][t + 4] = p3;       // p3, p2 of column c

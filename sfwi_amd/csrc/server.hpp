@@ -40,14 +40,19 @@ size_t solve_bytes(const sfwi_params &p, int kind, int nshots) {
     // per shot: the slot, its window and, in a merged batch, its request's model
     // (a snapshot context's slot also holds the fused adjoint step's second buffer, 9 fields)
     size_t slot = (43 + 2 * (p.grad_rho != 0) + (p.recon == SFWI_RECON_SNAPSHOTS ? 9 : 0)) * ncell * f +
-                  6 * nrec * f + 4 * nint * f + 20 * (size_t)p.ntr + 68 + 5 * nmodel * f + p.nt * f;
+                  6 * nrec * f + 4 * nint * f + 20 * (size_t)p.ntr + 68 + 4 * (ncell / (size_t)(p.nz + 2 * p.pml)) +
+                  5 * nmodel * f + p.nt * f;
     if (born) slot += 15 * ncell * f;
     const size_t chunk = p.strip_chunk > 0 ? (size_t)p.strip_chunk : 100;
     const size_t ring = p.cpu_mem && 2 * chunk < (size_t)p.nt ? 2 * chunk : (size_t)p.nt;   // strip steps in HBM
     if (grad && p.recon == SFWI_RECON_SNAPSHOTS) {   // velocity snapshots instead of strips
         const size_t k = p.snap_every > 0 ? (size_t)p.snap_every : 4, nsnap = (p.nt + k - 1) / k;
         const size_t rn = (size_t)(p.nx + 8) * (p.nz + 8);
-        slot += ((nsnap * 8 + 255) & ~(size_t)255) + 2048 + nsnap * 2 * rn * (p.storage ? 2 : 4);
+        // partial maxima: 2 x max(256, workgroups of the fused step per shot) floats
+        const size_t wc = p.step_cols > 0 ? (size_t)p.step_cols : 16;
+        const size_t wg = (ncell / (size_t)(p.nz + 2 * p.pml) - 8 + wc - 1) / wc * ((p.nz + 2 * p.pml - 8 + 239) / 240);
+        slot += ((nsnap * 8 + 255) & ~(size_t)255) + 8 * std::max<size_t>(256, wg) + nsnap * 2 * rn * (p.storage ? 2 : 4);
+        slot += 2 * (size_t)p.nt * (ncell / (size_t)(p.nz + 2 * p.pml)) * f;   // the adjoint source table
     } else if (grad) {
         slot += p.storage ? ring * 2 * sf * 2 + ring * 8 + 2 * sf * f + 256 : ring * 2 * sf * f;
     }

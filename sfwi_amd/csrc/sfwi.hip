@@ -64,15 +64,16 @@ struct Slot {
     float *strip = nullptr;     // FP32 [ring][2][sf], or 16-bit: scales [ring][2], stage [2][sf], [ring][2][sf]
     char *hstrip = nullptr;     // host strips (cpu_mem, pinned): FP32 [nt][2][sf], or 16-bit [nt][2][sf] + scales [nt][2]
     char *snap = nullptr;       // velocity snapshots: max bits [nsnap][2] (256-B aligned), then FP32 or 16-bit values
+    float *ainj = nullptr;      // adjoint source by step and column [nt][2][nxp] (fused adjoint of snapshot gradients)
     Fields fw{}, fw2{}, bk{}, bn{}, bn2{}, bk2{};
     float *dg[4]{};
     float *rz, *rx, *r2z, *r2x, *oz, *ox;
-    int *igx, *inj, *inj_off;
+    int *igx, *inj, *inj_off, *rtr;
     float *gvp, *gvs, *gil, *grho;
     float *vold[2]{};
     double *trace_sum;
     int obs_ntr = 0;                 // traces of the observed records resident in oz/ox (0: none)
-    std::vector<int> h_igx, h_inj, h_off;   // host copies kept alive for async uploads
+    std::vector<int> h_igx, h_inj, h_off, h_rtr;   // host copies kept alive for async uploads
 };
 
 struct ObsSet {        // observed traces resident under an id (sfwi_obs_put)
@@ -98,6 +99,9 @@ inline bool kind_forward_only(Kind k) { return k == K_FWD || k == K_BORN_FWD || 
 struct sfwi_ctx {
     sfwi_params p{};
     Geo g{};
+    bool fold = true;       // the prepared batch's sources and receivers lie in the update domain and no two
+                            // receivers share a cell: the workgroup marches add the source, record the traces
+                            // and inject the adjoint source themselves
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
@@ -320,7 +324,7 @@ size_t snap_words(const sfwi_ctx *c) {
     const size_t nsnap = (c->p.nt + c->g.snap_k - 1) / c->g.snap_k;
     return (nsnap * 2 * sizeof(unsigned) + 255) & ~(size_t)255;
 }
-size_t snap_head(const sfwi_ctx *c) { return snap_words(c) + 2 * SNAP_PART * sizeof(float); }
+size_t snap_head(const sfwi_ctx *c) { return snap_words(c) + 2 * (size_t)c->g.snap_np * sizeof(float); }
 
 // Model-size fields of one batch slot's block: fw (15), fw2 (9), bk (15), the
 // velocity derivatives (4), v(it+1) with the density image (2), bk2 (9) with
@@ -360,7 +364,8 @@ int ensure_slots(sfwi_ctx *c, int n, bool born, bool strips, bool snaps, bool bo
             }
             // records, receiver tables, accumulators, per-trace sums
             const size_t rbytes = 6 * c->nrec * sizeof(float) + (3 * (size_t)c->p.ntr + 1) * sizeof(int) +
-                                  4 * c->nint * sizeof(float) + (size_t)c->p.ntr * sizeof(double) + 64;
+                                  4 * c->nint * sizeof(float) + (size_t)c->p.ntr * sizeof(double) +
+                                  (size_t)c->g.nxp * sizeof(int) + 64;
             r = dalloc(c, (void **)&S.rec, rbytes);
             if (r) return r;
             char *q = S.rec;
@@ -373,6 +378,7 @@ int ensure_slots(sfwi_ctx *c, int n, bool born, bool strips, bool snaps, bool bo
             S.igx = (int *)(acc + 4 * c->nint);
             S.inj = S.igx + c->p.ntr;
             S.inj_off = S.inj + c->p.ntr;
+            S.rtr = S.inj_off + c->p.ntr + 1;
         }
         if (born && !S.born) {
             int r = dalloc(c, (void **)&S.born, 15 * cell);
@@ -396,6 +402,8 @@ int ensure_slots(sfwi_ctx *c, int n, bool born, bool strips, bool snaps, bool bo
             const size_t nsnap = (c->p.nt + c->g.snap_k - 1) / c->g.snap_k;
             const size_t vals = nsnap * 2 * (size_t)c->g.snap_n * (c->g.store ? 2 : 4);
             int r = dalloc(c, (void **)&S.snap, snap_head(c) + vals);
+            if (r) return r;
+            r = dalloc(c, (void **)&S.ainj, 2 * (size_t)c->p.nt * c->g.nxp * sizeof(float));
             if (r) return r;
         }
         if (strips && !S.strip) {
@@ -454,6 +462,12 @@ int receiver_tables(sfwi_ctx *c, const sfwi_shot &s, int k, Slot &S) {
     std::vector<int> fill(S.h_off.begin(), S.h_off.end() - 1);
     S.h_inj.assign(s.ntr, 0);
     for (int tr = 0; tr < s.ntr; ++tr) S.h_inj[fill[group[S.h_igx[tr]]]++] = tr;
+    // the trace of each column of the receiver row: -1 none, -2 several
+    S.h_rtr.assign(c->g.nxp, -1);
+    for (int tr = 0; tr < s.ntr; ++tr) {
+        int &v = S.h_rtr[S.h_igx[tr]];
+        v = v == -1 ? tr : -2;
+    }
     return 0;
 }
 
@@ -475,6 +489,8 @@ int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium 
     int r = ensure_slots(c, n, born, grad && !c->p.rbc && !snaps, snaps, born2);
     if (r) return r;
     c->h_shots.resize(n);
+    c->fold = true;
+    auto dom = [&](int x, int z) { return x >= NOP && x < c->g.nxp - NOP && z >= NOP && z < c->g.nzp - NOP; };
     for (int k = 0; k < n; ++k) {
         const sfwi_shot &s = shots[k];
         if ((r = check_shot(c, s, k))) return r;
@@ -507,7 +523,7 @@ int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium 
             d.oz = os->base;
             d.ox = os->base + (size_t)os->ntr * c->p.nt;
         }
-        d.igx = S.igx; d.inj = S.inj; d.inj_off = S.inj_off; d.strip = S.strip;
+        d.igx = S.igx; d.inj = S.inj; d.inj_off = S.inj_off; d.rtr = S.rtr; d.strip = S.strip; d.ainj = S.ainj;
         if (snaps) {
             const size_t head = snap_head(c);
             d.snap_max = (unsigned *)S.snap;
@@ -529,6 +545,12 @@ int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium 
         d.left = s.left;
         d.wav = shot_wav ? shot_wav[k] : nullptr;
         c->h_shots[k] = d;
+        // the workgroup marches add the source, record the receivers and inject
+        // the adjoint source themselves where these lie in their update domain
+        // and every receiver has a cell of its own
+        if (!dom(s.isx, s.isz)) c->fold = false;
+        for (int x : S.h_igx)
+            if (!dom(x, s.igz) || S.h_rtr[x] < 0) c->fold = false;
         const size_t cell = c->ncell * sizeof(float);
         HIPCK(c, hipMemsetAsync(S.block, 0, (grad ? 43 : 24) * cell, c->stream));
         if (grad && snaps) HIPCK(c, hipMemsetAsync((float *)S.bk2.vx, 0, 9 * cell, c->stream));
@@ -537,6 +559,8 @@ int prepare(sfwi_ctx *c, Kind kind, int n, const sfwi_shot *shots, const Medium 
         HIPCK(c, hipMemcpyAsync(S.igx, S.h_igx.data(), s.ntr * sizeof(int), hipMemcpyHostToDevice, c->stream));
         HIPCK(c, hipMemcpyAsync(S.inj, S.h_inj.data(), S.h_inj.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
         HIPCK(c, hipMemcpyAsync(S.inj_off, S.h_off.data(), S.h_off.size() * sizeof(int), hipMemcpyHostToDevice,
+                                c->stream));
+        HIPCK(c, hipMemcpyAsync(S.rtr, S.h_rtr.data(), S.h_rtr.size() * sizeof(int), hipMemcpyHostToDevice,
                                 c->stream));
         if (snaps) HIPCK(c, hipMemsetAsync(S.snap, 0, snap_words(c), c->stream));   // scales of FP32 snapshots: 0
         if (obs && !os && s.obs_z) {   // NULL: the records resident in this slot are reused
@@ -763,11 +787,22 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
         const int sp = hs ? it % c->ring : it;
         const int par = it & 1;   // buffer read by this step; the step writes the other
         if (hs && it % ch == 0 && it / ch >= 2) HIPCK(c, hipStreamWaitEvent(st, c->ev_copy[(it / ch) & 1], 0));
-        hipLaunchKernelGGL(k_source, src1, dim3(64), 0, st, g, D, c->wav, it, n, par);
+        // the workgroup march adds the source and records the receivers itself
+        // (when they lie in its update domain: else k_source / k_record do)
+        const bool fold = fwg && c->fold;
+        if (!fold) hipLaunchKernelGGL(k_source, src1, dim3(64), 0, st, g, D, c->wav, it, n, par);
         int t = tbeg(c, SFWI_K_FWD_STEP, it);
+        // a 16-bit snapshot step: the workgroup march also forms the partial
+        // maxima of the snapshot's max-abs scales (k_snap_max's pass)
+        const int smax = fwg && snap && g.store && it % g.snap_k == 0;
         if (fwg) {
-            if (save) hipLaunchKernelGGL((km_fwd_step_wg<true, WPD>), steps_grid, blk, 0, st, gs, D, n, sp, par);
-            else hipLaunchKernelGGL((km_fwd_step_wg<false, WPD>), steps_grid, blk, 0, st, gs, D, n, sp, par);
+            const float *wsrc = fold ? c->wav : nullptr;
+            if (save)
+                hipLaunchKernelGGL((km_fwd_step_wg<true, WPD>), steps_grid, blk, 0, st, gs, D, n, sp, par, smax,
+                                   wsrc, it);
+            else
+                hipLaunchKernelGGL((km_fwd_step_wg<false, WPD>), steps_grid, blk, 0, st, gs, D, n, sp, par, smax,
+                                   wsrc, it);
         } else if (save) {
             hipLaunchKernelGGL((km_fwd_step<true, FPD>), steps_grid, blk, 0, st, gs, D, n, sp, par);
         } else {
@@ -778,13 +813,14 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
         if (snap && it % g.snap_k == 0) {   // keep the new velocity (snapshot it / snap_k)
             t = tbeg(c, SFWI_K_SNAP_STORE, it);
             if (g.store) {
-                hipLaunchKernelGGL(k_snap_max, dim3(SNAP_PART, (unsigned)n), blk, 0, st, g, D, 1 - par);
-                hipLaunchKernelGGL(k_snap_scale, dim3((unsigned)n), dim3(64), 0, st, D, it / g.snap_k);
+                if (!smax) hipLaunchKernelGGL(k_snap_max, dim3(SNAP_PART, (unsigned)n), blk, 0, st, g, D, 1 - par);
+                hipLaunchKernelGGL(k_snap_scale, dim3((unsigned)n), dim3(64), 0, st, g, D, it / g.snap_k,
+                                   smax ? g.ws_wg : SNAP_PART);
             }
             hipLaunchKernelGGL(k_snap_pack, snapg, blk, 0, st, g, D, it / g.snap_k, 1 - par);
             tend(c, t);
         }
-        hipLaunchKernelGGL(k_record<false>, trg, dim3(64), 0, st, g, D, it, 1 - par);
+        if (!fold) hipLaunchKernelGGL(k_record<false>, trg, dim3(64), 0, st, g, D, it, 1 - par);
         if (hs && it % ch == ch - 1 && it / ch <= klast - 2) {   // chunk written: to the host
             const int k = it / ch;
             HIPCK(c, hipEventRecord(c->ev_ring[k & 1], st));
@@ -825,7 +861,7 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
             t = tbeg(c, SFWI_K_SNAP_STORE, it);
             if (g.store) {   // max-abs scales: per-block partials, then one wave per shot
                 hipLaunchKernelGGL(k_snap_max, dim3(SNAP_PART, (unsigned)n), blk, 0, st, g, D, 0);
-                hipLaunchKernelGGL(k_snap_scale, dim3((unsigned)n), dim3(64), 0, st, D, it / g.snap_k);
+                hipLaunchKernelGGL(k_snap_scale, dim3((unsigned)n), dim3(64), 0, st, g, D, it / g.snap_k, SNAP_PART);
             }
             hipLaunchKernelGGL(k_snap_pack, snapg, blk, 0, st, g, D, it / g.snap_k, 0);
             tend(c, t);
@@ -888,9 +924,15 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
         // imaging reads the adjoint stresses before it, which neither half step
         // of the pair changes before the imaging either
         const bool afused = !percell && c->p.fuse_step == 3;
+        // the adjoint source as a column table, added by the march itself
+        // (receivers in the update domain, one per cell); else k_inject_add
+        const int ainj = afused && c->fold;
+        if (ainj)
+            hipLaunchKernelGGL(k_inject_table, dim3((unsigned)std::min<size_t>(4096, ((size_t)nt * 2 * g.nxp + 255) / 256),
+                                                    (unsigned)n), dim3(256), 0, st, g, D, fac);
         for (int it = nt - 1; it >= 0 && afused; --it) {
             const int apar = (nt - 1 - it) & 1;   // buffer holding the adjoint state before this step
-            hipLaunchKernelGGL(k_inject_add, trg, dim3(64), 0, st, g, D, it, fac, apar);
+            if (!ainj) hipLaunchKernelGGL(k_inject_add, trg, dim3(64), 0, st, g, D, it, fac, apar);
             if (it % g.snap_k == 0) {
                 int t = tbeg(c, SFWI_K_SNAP_IMAGE, it);
                 const int k = it / g.snap_k;
@@ -900,7 +942,7 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
                 tend(c, t);
             }
             int t = tbeg(c, SFWI_K_ADJ_STEP, it);
-            hipLaunchKernelGGL((km_adj_step_wg<WPD>), ws_grid, blk, 0, st, gs, D, n, apar, 1);
+            hipLaunchKernelGGL((km_adj_step_wg<WPD>), ws_grid, blk, 0, st, gs, D, n, apar, 1, it, ainj);
             tend(c, t);
         }
         for (int it = nt - 1; it >= 0 && !afused; --it) {
@@ -1317,6 +1359,7 @@ int sfwi_ctx_create(const sfwi_params *p, sfwi_ctx **out) {
     g.ws_xs = p->step_cols > 0 ? p->step_cols : 16;
     g.ws_ntiles = (g.nzd + WS_ROWS - 1) / WS_ROWS;
     g.ws_wg = ((g.nxp - 2 * NOP + g.ws_xs - 1) / g.ws_xs) * g.ws_ntiles;
+    g.snap_np = std::max(SNAP_PART, g.ws_wg);   // k_snap_max's blocks or the fused step's workgroups
     g.inv_nzd = 1.0f / (float)g.nzd;
     g.dt = p->dt; g.dx = p->dx; g.dz = p->dz;
     g.dtdx = p->dt / p->dx; g.dtdz = p->dt / p->dz;
@@ -1376,6 +1419,7 @@ void sfwi_ctx_destroy(sfwi_ctx *c) {
         hipFree(S.rec);
         hipFree(S.strip);
         hipFree(S.snap);
+        hipFree(S.ainj);
         if (S.hstrip) hipHostFree(S.hstrip);
     }
     for (int k = 0; k < 2; ++k) {
