@@ -257,7 +257,7 @@ def main():
         return born(args)
 
     mp = args.workload == "mixed_precision"
-    if args.fuse_step is None:
+    if args.fuse_step is None:   # the library's automatic choice picks 3 on this grid, 0 on the batch's
         args.fuse_step = 3 if mp else 0
     recon = args.recon or ("snapshots" if mp else "strips")
     storage = args.storage or ("f16" if mp else "f32")

@@ -140,8 +140,10 @@ typedef struct sfwi_params {
                            /* wave), 2 workgroup march (240 rows per workgroup,    */
                            /* z halo shared through LDS), 3 the same, and a        */
                            /* snapshot gradient's adjoint steps fused the same     */
-                           /* way; <= 0 as two half-step launches (0: the default, */
-                           /* which sfwi_autotune may change)                      */
+                           /* way; < 0 as two half-step launches; 0 (default,      */
+                           /* which sfwi_autotune may change): 3 where the         */
+                           /* 240-row tiles take no more lanes than 56-row waves,  */
+                           /* else half steps (every choice gives the same bits)   */
     int step_cols;         /* columns per segment of the fused step (0 = 16)        */
 } sfwi_params;
 

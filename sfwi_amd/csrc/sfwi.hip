@@ -775,9 +775,16 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
     // Fused step march (velocity and stress of a step in one launch, the state
     // double-buffered) for FAST column-march forward solves of the background
     // field when fuse_step > 0 (sfwi_autotune sets it where it is faster).
-    const bool fused = !percell && !born && c->p.fuse_step > 0;
+    // fuse_step 0 (automatic): the workgroup marches (3: forward and, for a
+    // snapshot gradient, adjoint) where their 240-row tiles take no more lanes
+    // per column than the 56-row waves of the half-step kernels (1200 x 600
+    // grid: 3 x 256 = 12 x 64, 5.5 % faster; 401 x 301: 2 x 256 > 7 x 64, 9 %
+    // slower), the half steps elsewhere.  Every choice gives the same bits.
+    const int fstep = c->p.fuse_step != 0 ? c->p.fuse_step
+                      : (!percell && g.ws_ntiles * BLK <= g.cm_nbands * 64 ? 3 : 0);
+    const bool fused = !percell && !born && fstep > 0;
     // fuse_step 2, 3: the workgroup form (z halo shared through LDS, 240-row tiles)
-    const bool fwg = fused && c->p.fuse_step >= 2;
+    const bool fwg = fused && fstep >= 2;
     Geo gs = g;
     gs.fs_per_xcd = (int)(((long)g.fs_wg * n + 7) / 8);
     gs.ws_per_xcd = (int)(((long)g.ws_wg * n + 7) / 8);
@@ -923,7 +930,7 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
         // adjoint state double-buffered, bk / bk2 by the step's parity); the
         // imaging reads the adjoint stresses before it, which neither half step
         // of the pair changes before the imaging either
-        const bool afused = !percell && c->p.fuse_step == 3;
+        const bool afused = !percell && fstep == 3;
         // the adjoint source as a column table, added by the march itself
         // (receivers in the update domain, one per cell); else k_inject_add
         const int ainj = afused && c->fold;

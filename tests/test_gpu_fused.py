@@ -76,7 +76,7 @@ def test_workgroup_fused_step_equals_half_steps(nz, nt, cols, depth):
 @pytest.mark.parametrize("fuse", [1, 2])
 @pytest.mark.parametrize("extra", [dict(storage="bf16"), dict(cpu_mem=True, strip_chunk=30),
                                    dict(recon="snapshots", storage="f16", snap_every=3),
-                                   dict(use_graph=True)])
+                                   dict(use_graph=True), dict(rbc=True, rbc_seed=3)])
 def test_fused_step_with_storage_and_graphs(extra, fuse):
     sv, w, obs = _case(134 if fuse == 1 else 300, 201, seed=5)
     ref = _run(sv, w, obs, march_cols=8, fuse_step=-1, **extra)
