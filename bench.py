@@ -54,8 +54,8 @@ _SNAP16 = {
     "k_vel(forward)": 35344157, "k_str(forward)": 46374932, "k_adj_str": 34583001,
     "k_adj_vel": 48443475, "k_snap_image": 30838385, "k_snap_store": 16727600,
 }
-_SNAP16_FUSED = {   # 16-column segments, prefetch depth 1
-    "k_step(forward)": 79559206, "k_step(adjoint)": 93909871, "k_snap_image": 30838385, "k_snap_store": 9826381,
+_SNAP16_FUSED = {   # forward 16-column segments, adjoint 32, prefetch depth 1
+    "k_step(forward)": 79561574, "k_step(adjoint)": 83204275, "k_snap_image": 30847010, "k_snap_store": 9826389,
 }
 # measured with FP16 snapshots; BF16 snapshots move the same bytes with the same kernels (within 0.1 %)
 MEASURED_HBM_BYTES_PER_SHOT = {
@@ -198,7 +198,7 @@ def main():
                          "halo shared through LDS), 3 the same and the snapshot gradient's adjoint steps fused too "
                          "(default for mixed_precision), 1 a wave-level march, 0 / -1 two half-step launches "
                          "(default otherwise)")
-    ap.add_argument("--step-cols", type=int, default=0, help="fused step: columns per segment (0: 16)")
+    ap.add_argument("--step-cols", type=int, default=0, help="fused step: columns per segment of both marches (0: forward 16, adjoint 32)")
     ap.add_argument("--graph", action="store_true",
                     help="replay each solve from a captured hipGraph (the timed steps carry no kernel events; the "
                          "per-kernel times come from one extra step run directly afterwards)")
@@ -357,7 +357,7 @@ def main():
         measured = MEASURED_HBM_BYTES_PER_SHOT.get(key, {}) if (
             args.mode == "fast" and args.march_cols in (0, 8)
             and args.march_order in (0, 3) and not args.rbc and args.cpu_mem == 0 and wave == (24 if mp else nsh)
-            and args.step_cols in (0, 16) and args.march_depth in (0, 1 if args.fuse_step == 3 else 4)) \
+            and (args.step_cols == 0 or (args.step_cols == 16 and args.fuse_step != 3)) and args.march_depth in (0, 1 if args.fuse_step == 3 else 4)) \
             else {}
         traffic = round(measured[dom] * avg_wave) if dom in measured else None
         step_bytes = sum(v.get("alg_bytes", 0) * per_step[k] for k, v in kernels.items())
@@ -473,8 +473,9 @@ def main():
                                                  args.cpu_mem if args.cpu_mem > 0 else "")),
                    "forward": ("fused whole time steps, %s, %d-column segments%s" % (
                        "workgroup march (z halo shared through LDS)" if args.fuse_step >= 2 else "wave-level march",
-                       args.step_cols or 16, "; adjoint steps fused the same way" if args.fuse_step == 3 and snap
-                       else "") if args.fuse_step > 0 else "half-step launches (velocity, stress)"),
+                       args.step_cols or 16, "; adjoint steps fused the same way, %d-column segments"
+                       % (args.step_cols or 32) if args.fuse_step == 3 and snap else "")
+                       if args.fuse_step > 0 else "half-step launches (velocity, stress)"),
                    "launch": ("each solve replayed from a captured hipGraph; kernel times from one extra direct step"
                               if args.graph else "direct launches, kernel events every %d steps" % args.timing_every)},
         "gcell_updates_per_s": round(gcells, 2),

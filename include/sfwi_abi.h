@@ -144,7 +144,8 @@ typedef struct sfwi_params {
                            /* which sfwi_autotune may change): 3 where the         */
                            /* 240-row tiles take no more lanes than 56-row waves,  */
                            /* else half steps (every choice gives the same bits)   */
-    int step_cols;         /* columns per segment of the fused step (0 = 16)        */
+    int step_cols;         /* columns per segment of both fused steps (0 = forward
+                              16, adjoint 32)                                        */
 } sfwi_params;
 
 /* One shot: the fields the reference driver writes into FD2DGPU_ELASTIC
