@@ -193,6 +193,92 @@ __device__ inline float lz_minus(const float *col, int t, float f) {
 constexpr int WS_ROWS = BLK - 16;   // output rows of the workgroup fused step (256 lanes minus 2 x 8 halo)
 constexpr int WS_PAD = BLK + 8;     // one LDS column: 4 pad entries each side
 
+// The fields of one allocation addressed through a buffer resource: a field
+// is the byte offset of its start from the allocation's (the scalar offset
+// operand of each access), a cell its byte offset inside the field (one
+// vector register that every field at that cell shares).  The workgroup
+// marches address a shot's state block and its medium window this way: with a
+// 64-bit pointer per field they held more uniform values than a wave has
+// scalar registers, the compiler parked the rest in vector-register lanes and
+// read them back every column, and every load in a lane-masked branch formed
+// its own 64-bit address.  Word 3 (0x00020000) is the gfx9-family raw-buffer
+// setting (32-bit data format); the record count is the maximum, so no access
+// is range-checked away.  Every offset stays below 4 GiB: the host allows the
+// workgroup marches only where a shot's state block does.
+struct Buf {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ explicit Buf(const void *base)
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, -1, 0x00020000)) {}
+    __device__ float ld(unsigned f, int i) const {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (unsigned)i * 4u, f, 0));
+    }
+    __device__ void st(unsigned f, int i, float v) const {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (unsigned)i * 4u, f, 0);
+    }
+};
+__device__ inline unsigned boff(const void *p, const void *base) {
+    return (unsigned)((const char *)p - (const char *)base);
+}
+// Byte offsets of one state buffer's totals and velocity split parts, and of
+// the stress split parts (which have one buffer only), from the state block.
+struct FOff {
+    unsigned vx, vz, txx, tzz, txz, vx_x, vx_z, vz_x, vz_z;
+};
+struct SOff {
+    unsigned txx_x, txx_z, tzz_x, tzz_z, txz_x, txz_z;
+};
+__device__ inline FOff foff(const Fields &F, const void *b) {
+    return FOff{boff(F.vx, b),   boff(F.vz, b),   boff(F.txx, b),  boff(F.tzz, b), boff(F.txz, b),
+                boff(F.vx_x, b), boff(F.vx_z, b), boff(F.vz_x, b), boff(F.vz_z, b)};
+}
+__device__ inline SOff soff(const Fields &F, const void *b) {
+    return SOff{boff(F.txx_x, b), boff(F.txx_z, b), boff(F.tzz_x, b),
+                boff(F.tzz_z, b), boff(F.txz_x, b), boff(F.txz_z, b)};
+}
+// Byte offsets of the medium fields the marches read, from the window's start.
+struct MOff {
+    unsigned lam, mu, bu, dampx, dampz;
+};
+__device__ inline MOff moff(const Medium &m) {
+    return MOff{boff(m.lam, m.vp), boff(m.mu, m.vp), boff(m.bu, m.vp), boff(m.dampx, m.vp), boff(m.dampz, m.vp)};
+}
+// vel_load_w, vel_store, str_store and adj_str_store through the two bases.
+__device__ inline void vel_load_b(const Buf &S, const Buf &M, const MOff &mo, const FOff &F, int i, bool act,
+                                  bool sp, VelCell &s) {
+    if (act) s.bu = M.ld(mo.bu, i);
+    if (act && !sp) { s.vx = S.ld(F.vx, i); s.vz = S.ld(F.vz, i); }
+    if (act && sp) {
+        s.vx_x = S.ld(F.vx_x, i); s.vx_z = S.ld(F.vx_z, i); s.vz_x = S.ld(F.vz_x, i); s.vz_z = S.ld(F.vz_z, i);
+        s.ax = M.ld(mo.dampx, i); s.az = M.ld(mo.dampz, i);
+    }
+}
+__device__ inline void vel_store_b(const Buf &S, const FOff &F, int i, const VelCell &s, bool split) {
+    if (split) {
+        S.st(F.vx_x, i, s.vx_x); S.st(F.vx_z, i, s.vx_z); S.st(F.vz_x, i, s.vz_x); S.st(F.vz_z, i, s.vz_z);
+    }
+    S.st(F.vx, i, s.vx);
+    S.st(F.vz, i, s.vz);
+}
+__device__ inline void str_store_b(const Buf &S, const FOff &F, const SOff &P, int i, const StrCell &s,
+                                   bool split) {
+    if (split) {
+        S.st(P.txx_x, i, s.xx_x); S.st(P.txx_z, i, s.xx_z); S.st(P.tzz_x, i, s.zz_x);
+        S.st(P.tzz_z, i, s.zz_z); S.st(P.txz_x, i, s.xz_x); S.st(P.txz_z, i, s.xz_z);
+    }
+    S.st(F.txx, i, s.txx);
+    S.st(F.tzz, i, s.tzz);
+    S.st(F.txz, i, s.txz);
+}
+__device__ inline void adj_str_store_b(const Buf &S, const FOff &F, const SOff &P, int i, const AdjStrCell &s,
+                                       bool split) {
+    if (split) {
+        S.st(P.txx_x, i, s.xx_x); S.st(P.tzz_z, i, s.zz_z); S.st(P.txz_x, i, s.xz_x); S.st(P.txz_z, i, s.xz_z);
+    }
+    S.st(F.txx, i, s.txx);
+    S.st(F.tzz, i, s.tzz);
+    S.st(F.txz, i, s.txz);
+}
+
 // Fused forward time step with the z halo shared inside the workgroup.
 // The four waves of a workgroup stack along z: thread t owns row
 // Z0 - 8 + t of one column strip [x0, x1), so a workgroup covers 256
@@ -224,8 +310,11 @@ __global__ void __launch_bounds__(BLK) km_fwd_step_wg(Geo g, const DevShot *__re
     const int x0 = NOP + seg * g.ws_xs, x1 = min(x0 + g.ws_xs, g.nxp - NOP);
     const int Z0 = NOP + tile * WS_ROWS;
     const DevShot d = shots[shot];   // by value: the field pointers stay in registers
-    const Fields &R = par ? d.fw2 : d.fw;    // the state before the step
-    const Fields &Wt = par ? d.fw : d.fw2;   // the state after it
+    const Buf S(d.fw.vx), M(d.m.vp);   // the shot's state block (fw.vx leads it) and medium window
+    const FOff R = foff(par ? d.fw2 : d.fw, d.fw.vx);    // the state before the step
+    const FOff Wt = foff(par ? d.fw : d.fw2, d.fw.vx);   // the state after it
+    const SOff P = soff(d.fw, d.fw.vx);                  // the stress split parts (one buffer)
+    const MOff mo = moff(d.m);
     const int t = threadIdx.x;
     const int z = Z0 - 8 + t;
     const int zc = min(max(z, 0), g.nzp - 1);
@@ -236,7 +325,7 @@ __global__ void __launch_bounds__(BLK) km_fwd_step_wg(Geo g, const DevShot *__re
     auto colc = [&](int c) { return min(max(c, 0), g.nxp - 1); };
     auto colin = [&](int c) { return c >= 0 && c < g.nxp; };
     auto in_dom = [&](int c) { return c >= NOP && c < g.nxp - NOP; };
-    auto ld = [&](cfptr f, int c) { return f[colc(c) * g.nzp + zc]; };
+    auto ld = [&](unsigned f, int c) { return S.ld(f, colc(c) * g.nzp + zc); };
     float qxx[PD], qxz[PD], qzz[PD];
     int qtr[PD];   // trace recorded at column c (-1: none), through the ring like every operand
     VelCell qv[PD] = {};
@@ -252,16 +341,16 @@ __global__ void __launch_bounds__(BLK) km_fwd_step_wg(Geo g, const DevShot *__re
         qxz[j] = ld(R.txz, c + 3);
         qzz[j] = ld(R.tzz, c);
         const int i = colc(c) * g.nzp + zc;
-        vel_load_w(d.m, R, i, vlane && in_dom(c), in_band(g, c, z), qv[j]);
+        vel_load_b(S, M, mo, R, i, vlane && in_dom(c), in_band(g, c, z), qv[j]);
         const int cp = c - 4;
         const bool sown = own && cp >= x0 && cp < x1;
         const bool sp = in_band(g, cp, z) || (cp == d.isx && z == d.isz);
         const int ip = colc(cp) * g.nzp + zc;
-        if (sown) { qs[j].lam = d.m.lam[ip]; qs[j].mu = d.m.mu[ip]; }
+        if (sown) { qs[j].lam = M.ld(mo.lam, ip); qs[j].mu = M.ld(mo.mu, ip); }
         if (sown && sp) {
-            qs[j].xx_x = R.txx_x[ip]; qs[j].xx_z = R.txx_z[ip]; qs[j].zz_x = R.tzz_x[ip];
-            qs[j].zz_z = R.tzz_z[ip]; qs[j].xz_x = R.txz_x[ip]; qs[j].xz_z = R.txz_z[ip];
-            qs[j].ax = d.m.dampx[ip]; qs[j].az = d.m.dampz[ip];
+            qs[j].xx_x = S.ld(P.txx_x, ip); qs[j].xx_z = S.ld(P.txx_z, ip); qs[j].zz_x = S.ld(P.tzz_x, ip);
+            qs[j].zz_z = S.ld(P.tzz_z, ip); qs[j].xz_x = S.ld(P.txz_x, ip); qs[j].xz_z = S.ld(P.txz_z, ip);
+            qs[j].ax = M.ld(mo.dampx, ip); qs[j].az = M.ld(mo.dampz, ip);
         }
     };
     auto zs = [&](int c, float v) { return (zin && colin(c)) ? v : 0.0f; };
@@ -327,7 +416,7 @@ __global__ void __launch_bounds__(BLK) km_fwd_step_wg(Geo g, const DevShot *__re
                 vz_c = vs.vz;
                 if (own && c >= x0 && c < x1) {
                     const int i = c * g.nzp + z;
-                    vel_store(Wt, i, vs, sp);
+                    vel_store_b(S, Wt, i, vs, sp);
                     if (wav && z == d.igz && trc >= 0) {   // the receiver here records (k_record)
                         const size_t o = (size_t)trc * g.nt + ts;
                         d.rz[o] = vs.vz;
@@ -385,7 +474,7 @@ __global__ void __launch_bounds__(BLK) km_fwd_step_wg(Geo g, const DevShot *__re
                     ss.txz = sxz4;
                 }
                 str_math<false>(g, ss, v, sp, 1, none);
-                str_store(Wt, cp * g.nzp + z, ss, sp);
+                str_store_b(S, Wt, P, cp * g.nzp + z, ss, sp);
             }
         }
     }
@@ -435,9 +524,11 @@ __global__ void __launch_bounds__(BLK) km_adj_step_wg(Geo g, const DevShot *__re
     const int x0 = NOP + seg * g.ws_xs, x1 = min(x0 + g.ws_xs, g.nxp - NOP);
     const int Z0 = NOP + tile * WS_ROWS;
     const DevShot d = shots[shot];   // by value: the field pointers stay in registers
-    const Fields &R = par ? d.bk2 : d.bk;    // the adjoint state before the step
-    const Fields &Wt = par ? d.bk : d.bk2;   // the state after it
-    const Medium &m = d.m;
+    const Buf S(d.fw.vx), M(d.m.vp);   // the shot's state block (fw.vx leads it) and medium window
+    const FOff R = foff(par ? d.bk2 : d.bk, d.fw.vx);    // the adjoint state before the step
+    const FOff Wt = foff(par ? d.bk : d.bk2, d.fw.vx);   // the state after it
+    const SOff P = soff(d.bk, d.fw.vx);                  // the stress split parts (one buffer)
+    const MOff mo = moff(d.m);
     const int t = threadIdx.x;
     const int z = Z0 - 8 + t;
     const int zc = min(max(z, 0), g.nzp - 1);
@@ -471,28 +562,28 @@ __global__ void __launch_bounds__(BLK) km_adj_step_wg(Geo g, const DevShot *__re
             qax[j] = d.ainj[ia + g.nxp + colc(c)];
         }
         const int i4 = colc(c + 4) * g.nzp + zc;
-        qr[j].lam = m.lam[i4];
-        qr[j].mu = m.mu[i4];
-        qr[j].xx = R.txx[i4];
-        qr[j].zz = R.tzz[i4];
-        qxz[j] = R.txz[colc(c + 3) * g.nzp + zc];
+        qr[j].lam = M.ld(mo.lam, i4);
+        qr[j].mu = M.ld(mo.mu, i4);
+        qr[j].xx = S.ld(R.txx, i4);
+        qr[j].zz = S.ld(R.tzz, i4);
+        qxz[j] = S.ld(R.txz, colc(c + 3) * g.nzp + zc);
         const int i = colc(c) * g.nzp + zc;
         const bool va = vlane && in_dom(c), vsp = in_band(g, c, z) || z == d.igz;
         float *u = qv[j];
-        if (va) u[6] = m.bu[i];
-        if (va && !vsp) { u[0] = R.vx[i]; u[1] = R.vz[i]; }
+        if (va) u[6] = M.ld(mo.bu, i);
+        if (va && !vsp) { u[0] = S.ld(R.vx, i); u[1] = S.ld(R.vz, i); }
         if (va && vsp) {
-            u[0] = R.vx_x[i]; u[1] = R.vx_z[i]; u[2] = R.vz_x[i]; u[3] = R.vz_z[i];
-            u[4] = m.dampx[i]; u[5] = m.dampz[i];
+            u[0] = S.ld(R.vx_x, i); u[1] = S.ld(R.vx_z, i); u[2] = S.ld(R.vz_x, i); u[3] = S.ld(R.vz_z, i);
+            u[4] = M.ld(mo.dampx, i); u[5] = M.ld(mo.dampz, i);
         }
         const int cp = c - 4;
         const bool sown = own && cp >= x0 && cp < x1, ssp = in_band(g, cp, z);
         const int ip = colc(cp) * g.nzp + zc;
         float *q = qs[j];
-        if (sown && !ssp) { q[0] = R.txx[ip]; q[1] = R.tzz[ip]; q[2] = R.txz[ip]; }
+        if (sown && !ssp) { q[0] = S.ld(R.txx, ip); q[1] = S.ld(R.tzz, ip); q[2] = S.ld(R.txz, ip); }
         if (sown && ssp) {
-            q[0] = R.txx_x[ip]; q[1] = R.tzz_z[ip]; q[2] = R.txz_x[ip]; q[3] = R.txz_z[ip];
-            q[4] = m.dampx[ip]; q[5] = m.dampz[ip];
+            q[0] = S.ld(P.txx_x, ip); q[1] = S.ld(P.tzz_z, ip); q[2] = S.ld(P.txz_x, ip); q[3] = S.ld(P.txz_z, ip);
+            q[4] = M.ld(mo.dampx, ip); q[5] = M.ld(mo.dampz, ip);
         }
     };
     auto P1 = [&](const Raw &r, int c) { return ok(c) ? adj_p1(r.lam, r.mu, r.xx, r.zz) : 0.0f; };
@@ -516,11 +607,11 @@ __global__ void __launch_bounds__(BLK) km_adj_step_wg(Geo g, const DevShot *__re
     for (int k = 0; k < 7; ++k) {   // p1 of cs-3 .. cs+3, p2 of cs-4 .. cs+2
         const int x1c = cs - 3 + k, x2c = cs - 4 + k;
         const int i1 = colc(x1c) * g.nzp + zc, i2 = colc(x2c) * g.nzp + zc;
-        Raw r{m.lam[i1], m.mu[i1], R.txx[i1], R.tzz[i1]};
+        Raw r{M.ld(mo.lam, i1), M.ld(mo.mu, i1), S.ld(R.txx, i1), S.ld(R.tzz, i1)};
         if (!ok(x1c)) r = Raw{0.0f, 0.0f, 0.0f, 0.0f};
         w1[k] = P1(r, x1c);
         if (k >= 3) hist[k - 3] = r;   // columns cs .. cs+3
-        w2[k] = P2(m.mu[i2], R.txz[i2], x2c);
+        w2[k] = P2(M.ld(mo.mu, i2), S.ld(R.txz, i2), x2c);
     }
 #pragma unroll
     for (int j = 0; j < PD; ++j)
@@ -569,7 +660,7 @@ __global__ void __launch_bounds__(BLK) km_adj_step_wg(Geo g, const DevShot *__re
                 vel_math(g, vs, sp, dir, sx, sxz_z, sz, sxz_x);
                 vx_c = vs.vx;
                 vz_c = vs.vz;
-                if (own && c >= x0 && c < x1) vel_store(Wt, c * g.nzp + z, vs, sp);
+                if (own && c >= x0 && c < x1) vel_store_b(S, Wt, c * g.nzp + z, vs, sp);
             }
             // the windows move on
             const float dxz4 = hxz[0], dzz4 = hzz[0];
@@ -600,7 +691,7 @@ __global__ void __launch_bounds__(BLK) km_adj_step_wg(Geo g, const DevShot *__re
                 v.xz = dxz4;             // Dz+ vx (vx at column c - 4)
                 const bool sp = in_band(g, cp, z);
                 adj_str_math(g, ss, v, sp, dir);
-                adj_str_store(Wt, cp * g.nzp + z, ss, sp);
+                adj_str_store_b(S, Wt, P, cp * g.nzp + z, ss, sp);
             }
         }
     }

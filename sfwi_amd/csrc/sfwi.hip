@@ -102,6 +102,8 @@ struct sfwi_ctx {
     bool fold = true;       // the prepared batch's sources and receivers lie in the update domain and no two
                             // receivers share a cell: the workgroup marches add the source, record the traces
                             // and inject the adjoint source themselves
+    bool wg_ok = true;      // a shot's state block lies below 4 GiB, which the workgroup marches address with
+                            // 32-bit byte offsets
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
@@ -781,7 +783,7 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
     // grid: 3 x 256 = 12 x 64, 5.5 % faster; 401 x 301: 2 x 256 > 7 x 64, 9 %
     // slower), the half steps elsewhere.  Every choice gives the same bits.
     const int fstep = c->p.fuse_step != 0 ? c->p.fuse_step
-                      : (!percell && g.ws_ntiles * BLK <= g.cm_nbands * 64 ? 3 : 0);
+                      : (!percell && c->wg_ok && g.ws_ntiles * BLK <= g.cm_nbands * 64 ? 3 : 0);
     const bool fused = !percell && !born && fstep > 0;
     // fuse_step 2, 3: the workgroup form (z halo shared through LDS, 240-row tiles)
     const bool fwg = fused && fstep >= 2;
@@ -1340,6 +1342,12 @@ int sfwi_ctx_create(const sfwi_params *p, sfwi_ctx **out) {
     }
     const long nxp = p->nx + 2L * p->pml, nzp = p->nz + 2L * p->pml;
     if (nxp * nzp >= (1L << 31) / 16) return fail(nullptr, SFWI_EINVAL, "grid too large for 32-bit cell indices");
+    // a shot's state block (every model-size field of a batch slot)
+    const size_t sblock = (size_t)(43 + 2 * p->grad_rho + (p->recon == SFWI_RECON_SNAPSHOTS ? 9 : 0)) *
+                          (size_t)(nxp * nzp) * sizeof(float);
+    if (p->fuse_step >= 2 && sblock > 0xFFFFFFFFull)
+        return fail(nullptr, SFWI_EINVAL, "fuse_step %d: the workgroup marches need a shot's state block below "
+                    "4 GiB (%zu MiB on this grid)", p->fuse_step, sblock >> 20);
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return fail(nullptr, SFWI_EHIP, "no HIP device");
 
@@ -1378,6 +1386,7 @@ int sfwi_ctx_create(const sfwi_params *p, sfwi_ctx **out) {
     g.dtdx = p->dt / p->dx; g.dtdz = p->dt / p->dz;
     g.idx = 1.0f / p->dx; g.idz = 1.0f / p->dz;
     c->ncell = (size_t)nxp * nzp;
+    c->wg_ok = sblock <= 0xFFFFFFFFull;
     c->nmodel = (size_t)p->allnx * p->allnz;
     c->nrec = (size_t)p->ntr * p->nt;
     c->nint = (size_t)p->nx * p->nz;
@@ -1950,6 +1959,7 @@ int sfwi_autotune(sfwi_ctx *c, int nshots, int steps, int *march_cols, int *marc
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) r = fail(c, SFWI_EHIP, "hipEventCreate");
     for (const Cand &cd : cands) {
         if (r) break;
+        if (cd.fuse >= 2 && !c->wg_ok) continue;   // the workgroup marches' 4 GiB state-block bound
         c->p.march_cols = cd.cols;
         c->p.march_depth = cd.depth;
         c->p.march_order = cd.order;
