@@ -1116,25 +1116,50 @@ __global__ void __launch_bounds__(64) k_snap_scale(Geo g, const DevShot *__restr
 }
 
 // Store snapshot k: FP32, or v / scale in 16 bits.  buf: the forward-state
-// buffer holding the new velocity (1: DevShot::fw2).
+// buffer holding the new velocity (1: DevShot::fw2).  Each thread packs
+// SNAP_PER cells BLK apart (every load and store instruction of a wave stays
+// contiguous) and issues all their loads before its first store: one cell per
+// thread left each wave two loads in flight and launched four times the
+// waves, for 3.2 TB/s.
+constexpr int SNAP_PER = 4;
 __global__ void __launch_bounds__(BLK) k_snap_pack(Geo g, const DevShot *__restrict__ shots, int k, int buf) {
     const DevShot &d = shots[blockIdx.y];
     const Fields &F = buf ? d.fw2 : d.fw;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= g.snap_n) return;
-    const int i = snap_cell(g, t);
+    const int t0 = blockIdx.x * (BLK * SNAP_PER) + threadIdx.x;
+    float vx[SNAP_PER], vz[SNAP_PER];
+#pragma unroll
+    for (int j = 0; j < SNAP_PER; ++j) {
+        const int t = t0 + j * BLK;
+        if (t < g.snap_n) {
+            const int i = snap_cell(g, t);
+            vx[j] = F.vx[i];
+            vz[j] = F.vz[i];
+        }
+    }
     // snapshot k starts at a 64-bit offset; indices within it stay below 2^31
     const size_t o = (size_t)k * 2 * g.snap_n;
     if (g.store == 0) {
         const fptr s = d.snap32 + o;
-        s[t] = F.vx[i];
-        s[g.snap_n + t] = F.vz[i];
+#pragma unroll
+        for (int j = 0; j < SNAP_PER; ++j) {
+            const int t = t0 + j * BLK;
+            if (t < g.snap_n) {
+                s[t] = vx[j];
+                s[g.snap_n + t] = vz[j];
+            }
+        }
     } else {
         const gptr<unsigned short> s = d.snap16 + o;
         const float sx = __uint_as_float(d.snap_max[2 * (size_t)k]), sz = __uint_as_float(d.snap_max[2 * (size_t)k + 1]);
         const float ix = sx > 0.0f ? 1.0f / sx : 0.0f, iz = sz > 0.0f ? 1.0f / sz : 0.0f;
-        s[t] = pack16(g.store, F.vx[i] * ix);
-        s[g.snap_n + t] = pack16(g.store, F.vz[i] * iz);
+#pragma unroll
+        for (int j = 0; j < SNAP_PER; ++j) {
+            const int t = t0 + j * BLK;
+            if (t < g.snap_n) {
+                s[t] = pack16(g.store, vx[j] * ix);
+                s[g.snap_n + t] = pack16(g.store, vz[j] * iz);
+            }
+        }
     }
 }
 

@@ -242,6 +242,21 @@ struct MOff {
 __device__ inline MOff moff(const Medium &m) {
     return MOff{boff(m.lam, m.vp), boff(m.mu, m.vp), boff(m.bu, m.vp), boff(m.dampx, m.vp), boff(m.dampz, m.vp)};
 }
+// The value a ring slot holds on the lanes that skip its conditional load: no
+// such lane reads the slot, and leaving it unspecified lets each load write
+// its ring register directly (keeping the slot's previous value there costs a
+// register copy per operand and column).
+__device__ inline float unset() {
+    float f;
+    asm volatile("" : "=v"(f));   // a register as it stands: no instruction
+    return f;
+}
+template <class T> __device__ inline void unset_all(T &v) {
+    static_assert(sizeof(T) % sizeof(float) == 0, "a cell of floats");
+    float *f = reinterpret_cast<float *>(&v);
+#pragma unroll
+    for (unsigned k = 0; k < sizeof(T) / sizeof(float); ++k) f[k] = unset();
+}
 // vel_load_w, vel_store, str_store and adj_str_store through the two bases.
 __device__ inline void vel_load_b(const Buf &S, const Buf &M, const MOff &mo, const FOff &F, int i, bool act,
                                   bool sp, VelCell &s) {
@@ -341,11 +356,13 @@ __global__ void __launch_bounds__(BLK) km_fwd_step_wg(Geo g, const DevShot *__re
         qxz[j] = ld(R.txz, c + 3);
         qzz[j] = ld(R.tzz, c);
         const int i = colc(c) * g.nzp + zc;
+        unset_all(qv[j]);   // lanes that load no variant never read the slot
         vel_load_b(S, M, mo, R, i, vlane && in_dom(c), in_band(g, c, z), qv[j]);
         const int cp = c - 4;
         const bool sown = own && cp >= x0 && cp < x1;
         const bool sp = in_band(g, cp, z) || (cp == d.isx && z == d.isz);
         const int ip = colc(cp) * g.nzp + zc;
+        unset_all(qs[j]);
         if (sown) { qs[j].lam = M.ld(mo.lam, ip); qs[j].mu = M.ld(mo.mu, ip); }
         if (sown && sp) {
             qs[j].xx_x = S.ld(P.txx_x, ip); qs[j].xx_z = S.ld(P.txx_z, ip); qs[j].zz_x = S.ld(P.tzz_x, ip);
@@ -570,6 +587,8 @@ __global__ void __launch_bounds__(BLK) km_adj_step_wg(Geo g, const DevShot *__re
         const int i = colc(c) * g.nzp + zc;
         const bool va = vlane && in_dom(c), vsp = in_band(g, c, z) || z == d.igz;
         float *u = qv[j];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) u[k] = unset();   // lanes that load no variant never read the slot
         if (va) u[6] = M.ld(mo.bu, i);
         if (va && !vsp) { u[0] = S.ld(R.vx, i); u[1] = S.ld(R.vz, i); }
         if (va && vsp) {
@@ -580,6 +599,8 @@ __global__ void __launch_bounds__(BLK) km_adj_step_wg(Geo g, const DevShot *__re
         const bool sown = own && cp >= x0 && cp < x1, ssp = in_band(g, cp, z);
         const int ip = colc(cp) * g.nzp + zc;
         float *q = qs[j];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) q[k] = unset();
         if (sown && !ssp) { q[0] = S.ld(R.txx, ip); q[1] = S.ld(R.tzz, ip); q[2] = S.ld(R.txz, ip); }
         if (sown && ssp) {
             q[0] = S.ld(P.txx_x, ip); q[1] = S.ld(P.tzz_z, ip); q[2] = S.ld(P.txz_x, ip); q[3] = S.ld(P.txz_z, ip);

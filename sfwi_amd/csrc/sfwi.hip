@@ -730,10 +730,12 @@ void merge_range(sfwi_ctx *c, Kind kind, int off, int n) {
 // (at depth 4 they fit 2 waves per SIMD and run 6-11 % slower than at depth 2;
 // the imaging reconstruction kernels gain 1 % at depth 2, which the next
 // kernel loses again, and the others are 1-4 % faster at depth 4).  FPD: the
-// wave-level fused step's; WPD: the workgroup fused steps' (depth 1 keeps them
-// at 4 waves per SIMD, 115-127 VGPRs; at depth 2 the adjoint one needs 165 and
-// runs 3 % slower, the forward one the same).
-template <bool E, int PD, int HPD, int FPD = 2, int WPD = FPD>
+// wave-level fused step's; WPD: the workgroup forward step's, APD: the
+// workgroup adjoint step's.  At depth 1 the two keep 4 waves per SIMD (119-125
+// VGPRs); at depth 2 they need 150-151 and run at 3, where the adjoint, which
+// loads more per column, gains 1 % (361-362 us against 366 on the 1200 x 600
+// wave) and the forward loses 3-5 % (377-386 against 365).
+template <bool E, int PD, int HPD, int FPD = 2, int WPD = FPD, int APD = WPD>
 int run_batch(sfwi_ctx *c, Kind kind, int n) {
     const Geo &g = c->g;
     const int nt = g.nt;
@@ -772,7 +774,7 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
     const bool snap = grad && g.snap_k > 0;   // velocity snapshots instead of strips
     const bool save = grad && !c->p.rbc && !snap;
     const bool hs = save && c->ring < nt;
-    const dim3 snapg((unsigned)((g.snap_n + BLK - 1) / BLK), (unsigned)n);
+    const dim3 snapg((unsigned)((g.snap_n + BLK * SNAP_PER - 1) / (BLK * SNAP_PER)), (unsigned)n);   // k_snap_pack
     const int ch = c->chunk, klast = (nt - 1) / ch;
     // Fused step march (velocity and stress of a step in one launch, the state
     // double-buffered) for FAST column-march forward solves of the background
@@ -957,7 +959,7 @@ int run_batch(sfwi_ctx *c, Kind kind, int n) {
                 tend(c, t);
             }
             int t = tbeg(c, SFWI_K_ADJ_STEP, it);
-            hipLaunchKernelGGL((km_adj_step_wg<WPD>), wa_grid, blk, 0, st, ga, D, n, apar, 1, it, ainj);
+            hipLaunchKernelGGL((km_adj_step_wg<APD>), wa_grid, blk, 0, st, ga, D, n, apar, 1, it, ainj);
             tend(c, t);
         }
         for (int it = nt - 1; it >= 0 && !afused; --it) {
@@ -1061,7 +1063,7 @@ int run_direct(sfwi_ctx *c, Kind kind, int n) {
     case 1: return run_batch<false, 1, 1, 1>(c, kind, n);
     case 2: return run_batch<false, 2, 2, 2>(c, kind, n);
     case 4: return run_batch<false, 4, 4, 4>(c, kind, n);
-    default: return run_batch<false, 4, 2, 2, 1>(c, kind, n);   // automatic
+    default: return run_batch<false, 4, 2, 2, 1, 2>(c, kind, n);   // automatic
     }
 }
 
