@@ -109,8 +109,7 @@ typedef struct sfwi_params {
                            /* tiles (too few to fill the GPU) and 8 columns above  */
     int march_depth;       /* columns of operands prefetched ahead (1, 2, 4; 0: 4, and */
                            /* 2 for the two Born forward kernels, which hold twice */
-                           /* the fields; the fused steps' workgroup marches: 1    */
-                           /* forward, 2 adjoint)                                  */
+                           /* the fields; the fused steps' workgroup marches: 1)   */
     int misfit;            /* SFWI_MISFIT_*                                          */
     int march_order;       /* column-march block order: 0 default (= 3); 1 one shot  */
                            /* after another, split over the XCDs; 2 the shots of a  */

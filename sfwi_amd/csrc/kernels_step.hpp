@@ -532,6 +532,11 @@ template <int PD>
 __global__ void __launch_bounds__(BLK) km_adj_step_wg(Geo g, const DevShot *__restrict__ shots, int nshots,
                                                       int par, int dir, int it, int inj) {
     __shared__ float zt[2][4][WS_PAD];
+    // old txx, tzz, txz of each row over the last 9 columns, kept from their
+    // load as stencil operands (txx, tzz of column c + 4, txz of c + 3) for the
+    // stress update of their column eight and seven columns later; a thread
+    // reads only its own row, so no barrier guards the ring
+    __shared__ float ring[3][9][BLK];
     const int b = blockIdx.x;
     const int L = (b & 7) * g.ws_per_xcd + (b >> 3);
     const int shot = L / g.ws_wg;
@@ -601,8 +606,7 @@ __global__ void __launch_bounds__(BLK) km_adj_step_wg(Geo g, const DevShot *__re
         float *q = qs[j];
 #pragma unroll
         for (int k = 0; k < 6; ++k) q[k] = unset();
-        if (sown && !ssp) { q[0] = S.ld(R.txx, ip); q[1] = S.ld(R.tzz, ip); q[2] = S.ld(R.txz, ip); }
-        if (sown && ssp) {
+        if (sown && ssp) {   // (the totals of an unsplit cell come from the ring)
             q[0] = S.ld(P.txx_x, ip); q[1] = S.ld(P.tzz_z, ip); q[2] = S.ld(P.txz_x, ip); q[3] = S.ld(P.txz_z, ip);
             q[4] = M.ld(mo.dampx, ip); q[5] = M.ld(mo.dampz, ip);
         }
@@ -618,8 +622,8 @@ __global__ void __launch_bounds__(BLK) km_adj_step_wg(Geo g, const DevShot *__re
     const int cs = x0 - 4, ce = x1 + 4;
     // w1: p1 of c-3 .. c+4; w2: p2 of c-4 .. c+3; hist[k]: lam, mu, txx, tzz of
     // column c+k (k = 0..3); vxn / vzn: new adjoint velocity of c-8 .. c.  The
-    // stress update of column c - 4 fetches that column's old stresses with
-    // its other operands (a 12 B re-read per cell; keeping them eight columns
+    // stress update of column c - 4 takes that column's old stresses from the
+    // LDS ring (reloading them cost 12 B per cell; keeping them eight columns
     // in registers costs a wave per SIMD).
     float w1[8], w2[8], vxn[9] = {}, vzn[9] = {};
     float hxz[4] = {}, hzz[4] = {};   // Dz+ vx, Dz- vz of columns c - 4 .. c - 1
@@ -643,6 +647,12 @@ __global__ void __launch_bounds__(BLK) km_adj_step_wg(Geo g, const DevShot *__re
             const int c = c0 + j;
             if (c >= ce) break;   // uniform over the workgroup
             Raw r4 = qr[j];                       // column c + 4
+            {   // the loaded values, before the masking below
+                const int sw = (c + 4) % 9, sz3 = (c + 3) % 9;
+                ring[0][sw][t] = r4.xx;
+                ring[1][sw][t] = r4.zz;
+                ring[2][sz3][t] = qxz[j];
+            }
             if (!ok(c + 4)) r4 = Raw{0.0f, 0.0f, 0.0f, 0.0f};
             w1[7] = P1(r4, c + 4);
             w2[7] = P2(hist[3].mu, qxz[j], c + 3);   // mu, txz of column c + 3
@@ -711,6 +721,12 @@ __global__ void __launch_bounds__(BLK) km_adj_step_wg(Geo g, const DevShot *__re
                 v.zz = dzz4;             // Dz- vz (vz at column c - 4)
                 v.xz = dxz4;             // Dz+ vx (vx at column c - 4)
                 const bool sp = in_band(g, cp, z);
+                if (!sp) {
+                    const int sr = (c + 5) % 9;   // column c - 4
+                    ss.txx = ring[0][sr][t];
+                    ss.tzz = ring[1][sr][t];
+                    ss.txz = ring[2][sr][t];
+                }
                 adj_str_math(g, ss, v, sp, dir);
                 adj_str_store_b(S, Wt, P, cp * g.nzp + z, ss, sp);
             }

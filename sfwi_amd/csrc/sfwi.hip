@@ -731,10 +731,11 @@ void merge_range(sfwi_ctx *c, Kind kind, int off, int n) {
 // the imaging reconstruction kernels gain 1 % at depth 2, which the next
 // kernel loses again, and the others are 1-4 % faster at depth 4).  FPD: the
 // wave-level fused step's; WPD: the workgroup forward step's, APD: the
-// workgroup adjoint step's.  At depth 1 the two keep 4 waves per SIMD (119-125
-// VGPRs); at depth 2 they need 150-151 and run at 3, where the adjoint, which
-// loads more per column, gains 1 % (361-362 us against 366 on the 1200 x 600
-// wave) and the forward loses 3-5 % (377-386 against 365).
+// workgroup adjoint step's.  At depth 1 the two keep 4 waves per SIMD (123-125
+// VGPRs); at depth 2 the forward needs 151 and runs at 3 waves, 3-5 % slower
+// (377-386 us against 365 on the 1200 x 600 wave), and the adjoint, which
+// keeps the old stresses of its stress column in an LDS ring, 169 (2 waves,
+// 398 us against 357; without the ring it needed 150 and gained 1 %).
 template <bool E, int PD, int HPD, int FPD = 2, int WPD = FPD, int APD = WPD>
 int run_batch(sfwi_ctx *c, Kind kind, int n) {
     const Geo &g = c->g;
@@ -1063,7 +1064,7 @@ int run_direct(sfwi_ctx *c, Kind kind, int n) {
     case 1: return run_batch<false, 1, 1, 1>(c, kind, n);
     case 2: return run_batch<false, 2, 2, 2>(c, kind, n);
     case 4: return run_batch<false, 4, 4, 4>(c, kind, n);
-    default: return run_batch<false, 4, 2, 2, 1, 2>(c, kind, n);   // automatic
+    default: return run_batch<false, 4, 2, 2, 1, 1>(c, kind, n);   // automatic
     }
 }
 
