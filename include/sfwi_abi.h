@@ -143,7 +143,9 @@ typedef struct sfwi_params {
                            /* way; < 0 as two half-step launches; 0 (default,      */
                            /* which sfwi_autotune may change): 3 where the         */
                            /* 240-row tiles take no more lanes than 56-row waves,  */
-                           /* else half steps (every choice gives the same bits)   */
+                           /* else half steps (every choice gives the same bits);  */
+                           /* 2 and 3 need a shot's state block (43-52 model-size  */
+                           /* fields) below 4 GiB, else SFWI_EINVAL at create      */
     int step_cols;         /* columns per segment of both fused steps (0 = forward
                               16, adjoint 32)                                        */
 } sfwi_params;
