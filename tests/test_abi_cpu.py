@@ -51,6 +51,21 @@ def test_create_rejects_bad_parameters(bad):
     assert L.sfwi_last_error(None)
 
 
+@pytest.mark.parametrize("fuse,recon", [(2, 0), (3, 1)])
+def test_create_rejects_workgroup_marches_over_4gib(fuse, recon):
+    """The workgroup fused steps address a shot's state block with 32-bit
+    byte offsets: a forced fuse_step 2 or 3 on a grid whose block exceeds
+    4 GiB (5000 x 5000 cells with a 40-cell PML: 25.8 M cells, 43 or 52
+    fields) is refused before the device is touched."""
+    L = _lib.load()
+    h = ctypes.c_void_p()
+    p = _params(nx=5000, nz=5000, allnx=5000, allnz=5000, pml=40, fuse_step=fuse, recon=recon, mode=0)
+    rc = L.sfwi_ctx_create(ctypes.byref(p), ctypes.byref(h))
+    assert rc == _lib.SFWI_EINVAL
+    assert not h.value
+    assert b"4 GiB" in L.sfwi_last_error(None)
+
+
 def _model(nx=30, nz=20):
     vp = np.full((nx, nz), 2500.0, np.float32)
     return vp, vp / 1.8, np.full_like(vp, 2000.0)
