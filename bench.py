@@ -54,8 +54,9 @@ _SNAP16 = {
     "k_vel(forward)": 35344157, "k_str(forward)": 46374932, "k_adj_str": 34583001,
     "k_adj_vel": 48443475, "k_snap_image": 30838385, "k_snap_store": 16727600,
 }
-_SNAP16_FUSED = {   # forward 16-column segments, adjoint 32 (old stresses kept in LDS), prefetch depth 1
-    "k_step(forward)": 79207902, "k_step(adjoint)": 73351373, "k_snap_image": 30850441, "k_snap_store": 9305933,
+_SNAP16_FUSED = {   # forward 16-column segments, adjoint 32 (old stresses kept in LDS), prefetch depth 1,
+    # non-temporal state stores
+    "k_step(forward)": 79759420, "k_step(adjoint)": 73468830, "k_snap_image": 30848653, "k_snap_store": 9305933,
 }
 # measured with FP16 snapshots; BF16 snapshots move the same bytes with the same kernels (within 0.1 %)
 MEASURED_HBM_BYTES_PER_SHOT = {
